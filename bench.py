@@ -1,0 +1,265 @@
+#!/usr/bin/env python
+"""Benchmark: EasyFEA's phase-field fracture load history on one MI355X.
+
+Workload: the single-edge-notched unit square, TRI3 mesh with 200 x 200 cells
+(80,000 elements, 40,501 nodes, 81,002 displacement + 40,501 damage DOFs),
+E = 210000 MPa, nu = 0.3, plane strain, Gc = 2.7 N/mm, l0 = 0.01 mm, Miehe
+split, AT2, History irreversibility, staggered scheme with one pass per
+increment (tolConv = 1, convOption = 1), 100 tensile increments of 1e-4 mm on
+the top edge (bottom edge uy = 0, top edge ux = 0).
+
+A step is one load increment: damage assembly and solve, displacement
+assembly and solve, energies, history update and the top-edge reaction.
+``value`` is the solved DOF/s of the mixed path (FP32 matrix in the Krylov
+products, FP64 vectors and residual replacement), whole load history timed;
+the plain FP64 device path runs the same history beside it, and the mixed
+results are compared with it.  The CPU baseline is the numpy/scipy oracle
+(sparse direct solves) on the first increments of the same history.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "solved DOF/s at ≤1e-8 rel err vs FP64 (mixed vs plain); p50/p99 solve latency"
+E, NU, GC, L0 = 210000.0, 0.3, 2.7, 0.01
+DEP = 1e-4
+
+
+def make_sim(mesh, solver):
+    from easyfea_amd import Models, Simulations
+
+    mat = Models.Elastic.Isotropic(2, E, NU, planeStress=False, thickness=1.0)
+    pfm = Models.PhaseField(mat, "Miehe", "AT2", GC, L0)
+    simu = Simulations.PhaseField(mesh, pfm)
+    simu.solver = solver
+    return simu
+
+
+def run_history(mesh, solver, steps, keep_states=()):
+    import torch
+
+    simu = make_sim(mesh, solver)
+    top = mesh.Nodes_Conditions(lambda x, y, z: y == 1.0)
+    bot = mesh.Nodes_Conditions(lambda x, y, z: y == 0.0)
+    dofsY = simu.Bc_dofs_nodes(top, ["y"])
+    lat, forces, its_u, its_d, states = [], [], [], [], {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for it in range(1, steps + 1):
+        ts = time.perf_counter()
+        simu.Bc_Init()
+        simu.add_dirichlet(top, [0, it * DEP], ["x", "y"])
+        simu.add_dirichlet(bot, [0], ["y"])
+        u, d, _ = simu.Solve(1.0, 500, 1)
+        simu.Save_Iter()
+        forces.append(float(np.sum(simu.Calc_Reaction(dofsY, "elastic"))))
+        lat.append(time.perf_counter() - ts)
+        its_u.append(simu.last_solve["elastic"].iterations)
+        its_d.append(simu.last_solve["damage"].iterations)
+        if it in keep_states:
+            states[it] = (u.copy(), d.copy())
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    return dict(simu=simu, lat=np.array(lat), total=total, forces=np.array(forces), u=u, d=d,
+                its_u=np.array(its_u), its_d=np.array(its_d), states=states)
+
+
+def cpu_baseline(mesh, steps):
+    """The oracle (numpy einsum assembly, bincount scatter, scipy SuperLU)."""
+    from oracle import efea_oracle as O
+
+    ref = O.PhaseFieldOracle(mesh.connect, mesh.coord, "TRI3", E, NU, False, 1.0, "Miehe", "AT2", GC, L0)
+    top = mesh.Nodes_Conditions(lambda x, y, z: y == 1.0)
+    bot = mesh.Nodes_Conditions(lambda x, y, z: y == 0.0)
+    dofsY = O.dofs_nodes(top, ["y"], ["x", "y"])
+    states, forces = {}, []
+    t0 = time.perf_counter()
+    for it in range(1, steps + 1):
+        ref.Bc_Init()
+        ref.add_dirichlet(top, [0, it * DEP], ["x", "y"])
+        ref.add_dirichlet(bot, [0], ["y"])
+        u, d, _ = ref.Solve(1.0, 500, 1)
+        ref.Save_Iter()
+        forces.append(float(np.sum(ref.Calc_Reaction(dofsY))))
+        states[it] = (u.copy(), d.copy())
+    return time.perf_counter() - t0, states, np.array(forces)
+
+
+def spmv_roofline(A, reps):
+    """Average duration of the solver's SpMV kernel (FP32 values, the mixed
+    path's Krylov product) over ``reps`` launches, HIP events on the stream
+    the kernel runs on."""
+    import torch
+
+    from easyfea_amd import _lib
+    from easyfea_amd.device import ptr, stream_ptr
+
+    x = torch.rand(A.Ndof, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    vals = A.make_fp32()
+    args = (32, A.Nn, A.dof_n, A.group, ptr(A.node_ptr), ptr(A.node_cols), ptr(vals), None, ptr(x), ptr(y),
+            ptr(A.spmv_scratch), stream_ptr())
+    for _ in range(10):
+        _lib.call("efea_spmv", *args)
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        _lib.call("efea_spmv", *args)
+    e1.record(s)
+    e1.synchronize()
+    dt = e0.elapsed_time(e1) / 1e3 / reps  # includes the tiny state-init kernel per call
+    hA = A.host
+    nnz, nnz_node, Nn, Ndof = A.nnz, hA.node_cols.size, A.Nn, A.Ndof
+    algo_bytes = nnz * 4 + nnz_node * 4 + (Nn + 1) * 4 + Ndof * 8 + Ndof * 8
+    csr_bytes = nnz * (4 + 4) + (Ndof + 1) * 4 + 2 * Ndof * 4
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "spmv_pf200_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    peak = 8.0e12
+    return {
+        "kernel": "k_pcg_spmv<float,2,16>",
+        "bound": "hbm",
+        "achieved": algo_bytes / dt / 1e9,
+        "peak": peak / 1e9,
+        "unit": "GB/s",
+        "frac": algo_bytes / dt / peak,
+        "traffic": traffic,
+        "bytes_per_launch": algo_bytes,
+        "csr_formula_bytes": csr_bytes,
+        "avg_launch_us": dt * 1e6,
+        "nnz": nnz,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cells", type=int, default=200)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--spmv-reps", type=int, default=500)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the phase-field workload runs on a single MI355X "
+              "(multi-GPU runs are not part of this build); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import easyfea_amd as ea
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    mesh = ea.tri3_notched_square(args.cells)
+    ndof_u, ndof_d = mesh.Nn * 2, mesh.Nn
+
+    # warmup: a throwaway history (device maps, code objects, pinned buffers)
+    if args.warmup > 0:
+        run_history(mesh, "hip_mixed", args.warmup)
+        if not args.no_plain:
+            run_history(mesh, "hip_fp64", min(args.warmup, 1))
+    print(f"# warmup done ({args.warmup} increments)", file=sys.stderr, flush=True)
+
+    keep = tuple(range(1, min(args.cpu_steps, args.steps) + 1)) + (args.steps,)
+    mixed = run_history(mesh, "hip_mixed", args.steps, keep)
+    print(f"# mixed history: {mixed['total']:.3f} s", file=sys.stderr, flush=True)
+    plain = None
+    if not args.no_plain:
+        plain = run_history(mesh, "hip_fp64", args.steps, keep)
+        print(f"# plain FP64 history: {plain['total']:.3f} s", file=sys.stderr, flush=True)
+
+    dofs_per_step = ndof_u + ndof_d  # one staggered pass: one displacement + one damage solve
+    value = args.steps * dofs_per_step / mixed["total"]
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    accuracy = {}
+    if plain is not None:
+        accuracy["u_rel_err_vs_fp64_final"] = rel(mixed["u"], plain["u"])
+        accuracy["d_rel_err_vs_fp64_final"] = rel(mixed["d"], plain["d"])
+        accuracy["u_rel_err_vs_fp64_max"] = max(rel(mixed["states"][k][0], plain["states"][k][0]) for k in keep)
+        fr = np.abs(mixed["forces"] - plain["forces"]) / np.maximum(np.abs(plain["forces"]), 1e-300)
+        accuracy["force_curve_max_rel_dev_vs_fp64"] = float(fr.max())
+
+    roof = spmv_roofline(mixed["simu"]._Ku, args.spmv_reps)
+
+    cpu = None
+    if not args.no_cpu and args.cpu_steps > 0:
+        n = min(args.cpu_steps, args.steps)
+        t_cpu, st_cpu, f_cpu = cpu_baseline(mesh, n)
+        cores = len(os.sched_getaffinity(0))
+        cpu = {
+            "value": n * dofs_per_step / t_cpu,
+            "unit": "DOF/s",
+            "cores": int(os.environ.get("OMP_NUM_THREADS", cores)) if os.environ.get("OMP_NUM_THREADS") else cores,
+            "kind": "port",
+            "sample": f"first {n} of the {args.steps} increments of the same history on the host "
+                      f"(numpy einsum assembly + bincount scatter + scipy SuperLU spsolve, single-threaded "
+                      f"factorisation), {t_cpu:.1f} s",
+        }
+        accuracy["u_rel_err_vs_cpu_direct"] = max(rel(mixed["states"][k][0], st_cpu[k][0]) for k in range(1, n + 1))
+        accuracy["d_rel_err_vs_cpu_direct"] = max(rel(mixed["states"][k][1], st_cpu[k][1]) for k in range(1, n + 1))
+        accuracy["force_rel_dev_vs_cpu_direct"] = float(
+            np.max(np.abs(mixed["forces"][:n] - f_cpu) / np.abs(f_cpu)))
+
+    lat_ms = mixed["lat"] * 1e3
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["total"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (f32 matrix copy in Krylov products)",
+        "data": "synthetic: structured notched-square TRI3 mesh generated in-process",
+        "config": {
+            "workload": f"phase-field SENT square TRI3 {args.cells}x{args.cells} cells, Miehe/AT2/History, "
+                        f"{args.steps} x 1e-4 mm increments, staggered one pass",
+            "elements": int(mesh.Ne),
+            "ndof_u": int(ndof_u),
+            "ndof_d": int(ndof_d),
+            "solver": "block-Jacobi PCG, FP32-matrix Krylov + FP64 residual replacement",
+        },
+        "p50_step_ms": float(np.percentile(lat_ms, 50)),
+        "p99_step_ms": float(np.percentile(lat_ms, 99)),
+        "pcg_iters_u_mean": float(mixed["its_u"].mean()),
+        "pcg_iters_d_mean": float(mixed["its_d"].mean()),
+        "final_force_N": float(mixed["forces"][-1]),
+        "accuracy": accuracy,
+        "roofline": {k: roof[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
+        "roofline_detail": roof,
+        "cpu_baseline": cpu,
+    }
+    if plain is not None:
+        out["plain_fp64"] = {
+            "value": args.steps * dofs_per_step / plain["total"],
+            "ms_per_step": plain["total"] / args.steps * 1e3,
+            "p50_step_ms": float(np.percentile(plain["lat"] * 1e3, 50)),
+            "p99_step_ms": float(np.percentile(plain["lat"] * 1e3, 99)),
+            "pcg_iters_u_mean": float(plain["its_u"].mean()),
+            "mixed_speedup": plain["total"] / mixed["total"],
+        }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

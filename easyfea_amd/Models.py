@@ -1,0 +1,176 @@
+"""Material models of the solve path.
+
+``Elastic.Isotropic`` mirrors ``EasyFEA/Models/Elastic/_laws.py:333-505``
+(``E`` scalar or per element, Lame constants, Kelvin-Mandel ``C``);
+``PhaseField`` mirrors ``EasyFEA/Models/_phasefield.py:38-317`` for the
+Amor and Miehe splits, AT1/AT2 regularisations and the ``History``
+irreversibility.  Parameter changes notify observing simulations, which
+re-assemble lazily (``Utilities/_observers.py``, ``_simu.py:972-983``).
+"""
+
+from __future__ import annotations
+
+from enum import Enum
+from typing import Union
+
+import numpy as np
+
+
+class _Observable:
+    def __init__(self):
+        self._observers = []
+
+    def _Add_observer(self, obs) -> None:
+        if obs not in self._observers:
+            self._observers.append(obs)
+
+    def Need_Update(self, value: bool = True) -> None:
+        if value:
+            for o in self._observers:
+                o.Need_Update()
+
+
+class Isotropic(_Observable):
+    """Isotropic linear elastic material (Kelvin-Mandel notation)."""
+
+    def __init__(self, dim: int, E=210000.0, v=0.3, planeStress=True, thickness=1.0):
+        super().__init__()
+        assert dim in (2, 3), "dim must be 2 or 3"
+        self.dim = dim
+        self.E = E
+        self.v = v
+        self.planeStress = planeStress
+        self.thickness = thickness
+
+    def __setattr__(self, name, value):
+        if name == "E":
+            arr = np.asarray(value, dtype=float)
+            assert np.all(arr > 0), "E must be > 0"
+            value = float(arr) if arr.ndim == 0 else arr
+        elif name == "v":
+            assert -1 < value < 0.5, "v must be in ]-1, 0.5["
+            value = float(value)
+        elif name == "thickness":
+            assert value > 0, "thickness must be > 0"
+            value = float(value)
+        elif name == "planeStress":
+            value = bool(value)
+        super().__setattr__(name, value)
+        if name in ("E", "v", "thickness", "planeStress") and "_observers" in self.__dict__:
+            self.Need_Update()
+
+    @property
+    def isHeterogeneous(self) -> bool:
+        return isinstance(self.E, np.ndarray)
+
+    @property
+    def coef(self) -> float:
+        return np.sqrt(2)
+
+    def get_lambda(self):
+        E, v = self.E, self.v
+        lmbda = E * v / ((1 + v) * (1 - 2 * v))
+        if self.dim == 2 and self.planeStress:
+            lmbda = E * v / (1 - v**2)
+        return lmbda
+
+    def get_mu(self):
+        return self.E / (2 * (1 + self.v))
+
+    def get_bulk(self):
+        return self.get_lambda() + 2 * self.get_mu() / self.dim
+
+    @property
+    def C(self) -> np.ndarray:
+        """Kelvin-Mandel stiffness, ``(ns, ns)`` or ``(Ne, ns, ns)``."""
+        lam, mu = np.asarray(self.get_lambda()), np.asarray(self.get_mu())
+        ns = 3 if self.dim == 2 else 6
+        C = np.zeros(lam.shape + (ns, ns))
+        d = self.dim
+        for i in range(d):
+            for j in range(d):
+                C[..., i, j] = lam
+            C[..., i, i] = lam + 2 * mu
+        for k in range(d, ns):
+            C[..., k, k] = 2 * mu
+        return C
+
+    def _E_array(self) -> np.ndarray:
+        return np.atleast_1d(np.asarray(self.E, dtype=np.float64))
+
+
+class Elastic:
+    """Namespace mirroring ``EasyFEA.Models.Elastic``."""
+
+    Isotropic = Isotropic
+
+
+class PhaseField(_Observable):
+    """Phase-field fracture model."""
+
+    class ReguType(str, Enum):
+        AT1 = "AT1"
+        AT2 = "AT2"
+
+        def __str__(self):
+            return self.name
+
+    class SplitType(str, Enum):
+        Amor = "Amor"
+        Miehe = "Miehe"
+
+        def __str__(self):
+            return self.name
+
+    class SolverType(str, Enum):
+        History = "History"
+
+        def __str__(self):
+            return self.name
+
+    def __init__(self, material: Isotropic, split, regularization, Gc: Union[float, np.ndarray], l0: float,
+                 solver="History"):
+        super().__init__()
+        assert isinstance(material, Isotropic), "Must be an Isotropic elastic material"
+        self.material = material
+        self.split = PhaseField.SplitType(split)
+        self.regularization = PhaseField.ReguType(regularization)
+        self.Gc = Gc
+        self.l0 = l0
+        self.solver = PhaseField.SolverType(solver)
+        material._Add_observer(self)
+
+    def __setattr__(self, name, value):
+        if name == "Gc":
+            arr = np.asarray(value, dtype=float)
+            assert np.all(arr > 0), "Gc must be > 0"
+            value = float(arr) if arr.ndim == 0 else arr
+        elif name == "l0":
+            assert value > 0, "l0 must be > 0"
+            value = float(value)
+        super().__setattr__(name, value)
+        if name in ("Gc", "l0", "split", "regularization") and "_observers" in self.__dict__:
+            self.Need_Update()
+
+    @property
+    def dim(self) -> int:
+        return self.material.dim
+
+    @property
+    def thickness(self) -> float:
+        return self.material.thickness
+
+    @property
+    def isHeterogeneous(self) -> bool:
+        return isinstance(self.Gc, np.ndarray)
+
+    @property
+    def k(self):
+        return 3 / 4 * self.Gc * self.l0 if self.regularization == PhaseField.ReguType.AT1 else self.Gc * self.l0
+
+    @property
+    def c_w(self) -> float:
+        return 8 / 3 if self.regularization == PhaseField.ReguType.AT1 else 2.0
+
+    def _Gc_array(self) -> np.ndarray:
+        return np.atleast_1d(np.asarray(self.Gc, dtype=np.float64))

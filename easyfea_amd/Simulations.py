@@ -1,0 +1,689 @@
+"""Simulations: the reference's ``Simulations.Elastic`` / ``Simulations.PhaseField``
+API over device-resident systems.
+
+Mirrors (EasyFEA v3.5.1):
+* ``_Simu`` boundary conditions ``Bc_Init`` / ``add_dirichlet`` / ``add_neumann`` /
+  ``add_lineLoad`` / ``add_surfLoad`` / ``Bc_dofs_nodes`` / ``Bc_dofs_known_unknown``
+  (``Simulations/_simu.py:2190-2630``), ``Get_K_C_M_F`` returning scipy CSR copies
+  (``:161-175``), ``Solve`` (``:1504-1515``), ``Calc_Reaction`` (``:212-260``),
+  ``Calc_Energy`` (``:177-210``), ``Save_Iter`` / ``Set_Iter`` (``:288-471``);
+* ``Elastic`` (``Simulations/_elastic.py:52-176``);
+* ``PhaseField`` with its staggered ``Solve`` (``Simulations/_phasefield.py:86-657``).
+
+Assembly, scatter, solve and the phase-field updates run in ``libefea_hip.so``;
+boundary-condition bookkeeping (which DOFs, which values) stays on the host
+as in the reference.
+"""
+
+from __future__ import annotations
+
+import time
+from typing import Callable, Optional
+
+import numpy as np
+import scipy.sparse as sp
+import torch
+
+from . import _lib
+from ._utils import ElemType, MatrixType
+from .device import DeviceMesh, Workspace, ptr, solve, stream_ptr, to_dev
+from .mesh import Mesh
+from .Models import Isotropic
+from .Models import PhaseField as PhaseFieldModel
+from .quadrature import N_pg, dN_pg, gauss_points
+from .Solvers import REFERENCE_ONLY, SolverType
+
+
+def _print_error(msg: str) -> None:
+    print(f"\033[91m{msg}\033[0m")
+
+
+def _frame_coords(coord_e: np.ndarray, dim: int) -> np.ndarray:
+    """Boundary-element coordinates in the element's own orthonormal frame
+    (``_group_elem.py:404-479``), ``(Ne, nPe, dim)``."""
+    if dim == 1:
+        t = coord_e[:, 1] - coord_e[:, 0]
+        t = t / np.linalg.norm(t, axis=1, keepdims=True)
+        return np.einsum("end,ed->en", coord_e, t)[..., None]
+    i = coord_e[:, 1] - coord_e[:, 0]
+    i = i / np.linalg.norm(i, axis=1, keepdims=True)
+    k = np.cross(i, coord_e[:, -1] - coord_e[:, 0])
+    k = k / np.linalg.norm(k, axis=1, keepdims=True)
+    j = np.cross(k, i)
+    return np.stack([np.einsum("end,ed->en", coord_e, i), np.einsum("end,ed->en", coord_e, j)], -1)
+
+
+class _Bc:
+    __slots__ = ("problemType", "nodes", "dofs", "values", "unknowns", "description")
+
+    def __init__(self, problemType, nodes, dofs, values, unknowns, description):
+        self.problemType = problemType
+        self.nodes = np.asarray(nodes, dtype=np.int64)
+        self.dofs = np.asarray(dofs, dtype=np.int64)
+        self.values = np.asarray(values, dtype=np.float64)
+        self.unknowns = list(unknowns)
+        self.description = description
+
+
+class _Simu:
+    """Shared machinery of the device-backed simulations."""
+
+    def __init__(self, mesh: Mesh, model, folder: str = "", verbosity: bool = False):
+        self.mesh = mesh
+        self.model = model
+        self.folder = folder
+        self._verbosity = verbosity
+        self.dev = DeviceMesh(mesh)
+        self.__solver = SolverType.hip_mixed
+        self._ws = Workspace()
+        self._results: list = []
+        self.Niter = 0
+        self.tol_residual = 1e-12
+        self.tol_anorm = 1e-9
+        self.replace_ratio = 1e-3
+        self.maxit = 200000
+        self.check_every = 32
+        self.last_solve = {}
+        self.Bc_Init()
+        model._Add_observer(self)
+
+    # -------------------------------------------------------------- solver
+    @property
+    def solver(self) -> SolverType:
+        return self.__solver
+
+    @solver.setter
+    def solver(self, value):
+        try:
+            self.__solver = SolverType(value)
+        except ValueError:
+            extra = " (a host backend of the reference)" if str(value) in REFERENCE_ONLY else ""
+            _print_error(f"The solver {value}{extra} cannot be used. The solver must be in {list(SolverType)}")
+
+    def _mode(self) -> str:
+        return "fp64" if self.__solver == SolverType.hip_fp64 else "mixed"
+
+    # ------------------------------------------------------------ problems
+    def Get_problemTypes(self) -> list:
+        raise NotImplementedError
+
+    @property
+    def problemType(self) -> str:
+        return self.Get_problemTypes()[0]
+
+    def Get_unknowns(self, problemType=None) -> list:
+        raise NotImplementedError
+
+    def Get_dof_n(self, problemType=None) -> int:
+        return len(self.Get_unknowns(problemType))
+
+    def Get_dofs(self, problemType=None) -> np.ndarray:
+        return np.arange(self.mesh.Nn * self.Get_dof_n(problemType))
+
+    # ---------------------------------------------------------------- BCs
+    def Bc_Init(self) -> None:
+        self._Bc_Dirichlet: list = []
+        self._Bc_Neumann: list = []
+
+    @property
+    def Bc_Dirichlet(self) -> list:
+        return list(self._Bc_Dirichlet)
+
+    @property
+    def Bc_Neuman(self) -> list:
+        return list(self._Bc_Neumann)
+
+    def _check_problemType(self, problemType):
+        assert problemType in self.Get_problemTypes(), f"problemType must be in {self.Get_problemTypes()}"
+
+    def Bc_dofs_nodes(self, nodes, unknowns, problemType=None) -> np.ndarray:
+        """``BoundaryCondition.Get_dofs_nodes`` (``FEM/_boundary_conditions.py:160-201``)."""
+        nodes = np.asarray(nodes, dtype=np.int64).ravel()
+        if nodes.size == 0:
+            return np.array([], dtype=np.int64)
+        problemType = problemType or self.problemType
+        avail = self.Get_unknowns(problemType)
+        dofs = np.zeros((nodes.size, len(unknowns)), dtype=np.int64)
+        for d, u in enumerate(unknowns):
+            assert u in avail, f"{u} is not in {avail}"
+            dofs[:, d] = nodes * len(avail) + avail.index(u)
+        return dofs.ravel()
+
+    def _eval_nodes(self, coord_n, value):
+        if callable(value):
+            return np.asarray(value(coord_n[:, 0], coord_n[:, 1], coord_n[:, 2]), dtype=float) * np.ones(len(coord_n))
+        return np.broadcast_to(np.asarray(value, dtype=float), (len(coord_n),)).copy()
+
+    def add_dirichlet(self, nodes, values: list, unknowns: list, problemType=None, description="") -> None:
+        if len(nodes) == 0 or len(values) == 0:
+            return
+        assert len(values) == len(unknowns), f"len(values)={len(values)} must equal len(unknowns)={len(unknowns)}."
+        problemType = problemType or self.problemType
+        self._check_problemType(problemType)
+        nodes = np.asarray(nodes, dtype=np.int64)
+        coord_n = self.mesh.coord[nodes]
+        vals = np.zeros((nodes.size, len(unknowns)))
+        for d in range(len(unknowns)):
+            vals[:, d] = self._eval_nodes(coord_n, values[d])
+        dofs = self.Bc_dofs_nodes(nodes, unknowns, problemType)
+        self._Bc_Dirichlet.append(_Bc(problemType, nodes, dofs, vals.ravel(), unknowns, f"Dirichlet {description}"))
+
+    def add_neumann(self, nodes, values: list, unknowns: list, problemType=None, description="") -> None:
+        """Point loads shared equally by the nodes (``_simu.py:2471-2517, 2743-2768``)."""
+        if len(nodes) == 0 or len(values) == 0:
+            return
+        assert len(values) == len(unknowns)
+        problemType = problemType or self.problemType
+        nodes = np.asarray(nodes, dtype=np.int64)
+        coord_n = self.mesh.coord[nodes]
+        vals = np.zeros((nodes.size, len(unknowns)))
+        for d in range(len(unknowns)):
+            vals[:, d] = self._eval_nodes(coord_n, values[d]) / len(nodes)
+        dofs = self.Bc_dofs_nodes(nodes, unknowns, problemType)
+        self._Bc_Neumann.append(_Bc(problemType, nodes, dofs, vals.ravel(), unknowns, f"Neumann {description}"))
+
+    def _integrate_boundary(self, bdim: int, nodes, values, unknowns, problemType):
+        """Load integral over the elements of dimension ``bdim`` whose nodes all
+        belong to ``nodes`` (``_simu.py:2770-2842``), integrated on the device."""
+        nodes = np.asarray(nodes, dtype=np.int64)
+        all_vals, all_dofs, used = [], [], []
+        for g in self.mesh.Get_list_groupElem(bdim):
+            elements = g.Get_Elements_Nodes(nodes, exclusively=True)
+            if elements.size == 0:
+                continue
+            connect = g.connect[elements]
+            used.append(connect.ravel())
+            et = g.elemType
+            mt = MatrixType.mass
+            coord_e = self.mesh.coord[connect]
+            inDim = 2 if self.mesh.dim == 2 else 3
+            if bdim == inDim:
+                Xloc = coord_e[..., :bdim]
+            else:
+                Xloc = _frame_coords(coord_e, bdim)
+            Npg = N_pg(et, mt)
+            nPg = Npg.shape[0]
+            coord_gp = np.einsum("pn,end->epd", Npg, coord_e)
+            tab = np.concatenate([gauss_points(et, mt)[1], Npg.ravel(), dN_pg(et, mt).ravel()])
+            d_X = to_dev(np.ascontiguousarray(Xloc, dtype=np.float64))
+            d_tab = to_dev(tab)
+            vals = np.zeros((connect.shape[0] * connect.shape[1], len(unknowns)))
+            dofs = np.zeros_like(vals, dtype=np.int64)
+            for u, unk in enumerate(unknowns):
+                val = values[u]
+                if callable(val):
+                    f = np.asarray(val(coord_gp[..., 0], coord_gp[..., 1], coord_gp[..., 2]), dtype=float)
+                    f = np.broadcast_to(f, (connect.shape[0], nPg))
+                elif np.ndim(val) == 0:
+                    f = np.full((connect.shape[0], nPg), float(val))
+                else:  # nodal values
+                    ev = np.zeros(self.mesh.Nn)
+                    ev[nodes] = val
+                    f = np.einsum("pn,en->ep", Npg, ev[connect])
+                d_f = to_dev(np.ascontiguousarray(f, dtype=np.float64))
+                out = torch.empty(connect.size, dtype=torch.float64, device="cuda")
+                _lib.call("efea_boundary_load", _lib.ELEM_CODE[et.value], connect.shape[0], ptr(d_X), ptr(d_tab),
+                          nPg, ptr(d_f), ptr(out), stream_ptr())
+                vals[:, u] = out.cpu().numpy()
+                dofs[:, u] = self.Bc_dofs_nodes(connect.ravel(), [unk], problemType)
+            all_vals.append(vals.ravel())
+            all_dofs.append(dofs.ravel())
+        if not all_vals:
+            return np.array([]), np.array([], dtype=np.int64), np.array([], dtype=np.int64)
+        return np.concatenate(all_vals), np.concatenate(all_dofs), np.concatenate(used)
+
+    def add_lineLoad(self, nodes, values: list, unknowns: list, problemType=None, description="") -> None:
+        if len(nodes) == 0 or len(values) == 0:
+            return
+        assert len(values) == len(unknowns)
+        problemType = problemType or self.problemType
+        vals, dofs, used = self._integrate_boundary(1, nodes, values, unknowns, problemType)
+        self._Bc_Neumann.append(_Bc(problemType, used, dofs, vals, unknowns, f"Neumann {description}"))
+
+    def add_surfLoad(self, nodes, values: list, unknowns: list, problemType=None, description="") -> None:
+        """2D: line load times thickness; 3D: surface load (``_simu.py:2572-2630``)."""
+        if len(nodes) == 0 or len(values) == 0:
+            return
+        assert len(values) == len(unknowns)
+        problemType = problemType or self.problemType
+        if self.mesh.dim == 2:
+            vals, dofs, used = self._integrate_boundary(1, nodes, values, unknowns, problemType)
+            vals = vals * self.model.thickness
+        else:
+            vals, dofs, used = self._integrate_boundary(2, nodes, values, unknowns, problemType)
+        self._Bc_Neumann.append(_Bc(problemType, used, dofs, vals, unknowns, f"Neumann {description}"))
+
+    def _bc_arrays(self, store, problemType):
+        bcs = [bc for bc in store if bc.problemType == problemType]
+        if not bcs:
+            return np.array([], dtype=np.int64), np.array([])
+        return np.concatenate([bc.dofs for bc in bcs]), np.concatenate([bc.values for bc in bcs])
+
+    def Bc_dofs_Dirichlet(self, problemType=None) -> np.ndarray:
+        return self._bc_arrays(self._Bc_Dirichlet, problemType or self.problemType)[0]
+
+    def Bc_values_Dirichlet(self, problemType=None) -> np.ndarray:
+        return self._bc_arrays(self._Bc_Dirichlet, problemType or self.problemType)[1]
+
+    def Bc_dofs_Neumann(self, problemType=None) -> np.ndarray:
+        return self._bc_arrays(self._Bc_Neumann, problemType or self.problemType)[0]
+
+    def Bc_values_Neumann(self, problemType=None) -> np.ndarray:
+        return self._bc_arrays(self._Bc_Neumann, problemType or self.problemType)[1]
+
+    def Bc_dofs_known_unknown(self, problemType) -> tuple:
+        """Known = unique sorted Dirichlet DOFs (``_simu.py:2295-2316``)."""
+        nDof = self.mesh.Nn * self.Get_dof_n(problemType)
+        mask = np.ones(nDof, dtype=bool)
+        mask[self.Bc_dofs_Dirichlet(problemType)] = False
+        return np.nonzero(~mask)[0], np.nonzero(mask)[0]
+
+    def _neumann_vector(self, problemType) -> np.ndarray:
+        dofs, vals = self._bc_arrays(self._Bc_Neumann, problemType)
+        Ndof = self.mesh.Nn * self.Get_dof_n(problemType)
+        # duplicates add, as the (Ndof, 1) CSR of _Solver_Apply_Neumann (_simu.py:1750-1756)
+        return np.bincount(dofs, weights=vals, minlength=Ndof) if dofs.size else np.zeros(Ndof)
+
+    def _dirichlet_state(self, problemType):
+        """(known mask uint8 on device, x values at known DOFs on host)."""
+        Ndof = self.mesh.Nn * self.Get_dof_n(problemType)
+        dofs, vals = self._bc_arrays(self._Bc_Dirichlet, problemType)
+        xk = np.bincount(dofs, weights=vals, minlength=Ndof) if dofs.size else np.zeros(Ndof)
+        known = np.zeros(Ndof, dtype=np.uint8)
+        known[dofs] = 1
+        orphan = self.mesh.orphanNodes
+        if orphan.size:
+            od = self.Bc_dofs_nodes(orphan, self.Get_unknowns(problemType), problemType)
+            known[od] = 1  # orphan rows hold only the unit diagonal (_simu.py:1949-1958): x = 0
+            xk[od] = np.where(np.isin(od, dofs), xk[od], 0.0)
+        return known, xk
+
+    def _solve_system(self, A, b_dev: torch.Tensor, x_dev: torch.Tensor, problemType) -> None:
+        known_np, xk = self._dirichlet_state(problemType)
+        known = to_dev(known_np)
+        kmask = torch.from_numpy(known_np.astype(bool)).to("cuda")
+        x_dev[kmask] = to_dev(xk)[kmask]
+        t = time.perf_counter()
+        info = solve(A, b_dev, x_dev, known, self._mode(), self._ws, self.tol_residual, self.tol_anorm,
+                     self.replace_ratio, self.maxit, self.check_every)
+        self.last_solve[problemType] = info
+        if self._verbosity:
+            print(f"{problemType}: {info.iterations} it, rel res {info.rel_residual:.2e} "
+                  f"({time.perf_counter() - t:.3f} s)")
+
+    # ------------------------------------------------------------ results
+    def Save_Iter(self, iter: Optional[dict] = None) -> None:
+        it = {} if iter is None else dict(iter)
+        self._results.append(it)
+        self.Niter += 1
+
+    @property
+    def results(self) -> list:
+        return self._results
+
+    def Set_Iter(self, iter: int = -1) -> Optional[dict]:
+        if not self._results:
+            return None
+        return self._results[iter]
+
+    def Need_Update(self, value: bool = True) -> None:
+        raise NotImplementedError
+
+    def Calc_Energy(self, A: sp.csr_matrix, x: np.ndarray, dofs=None) -> float:
+        if dofs is None:
+            dofs = self.Get_dofs()
+        return 0.5 * x[dofs] @ (A[dofs] @ x)
+
+
+class Elastic(_Simu):
+    """Linear elastic simulation (``Simulations/_elastic.py``)."""
+
+    def __init__(self, mesh: Mesh, model: Isotropic, folder: str = "", verbosity=False):
+        assert isinstance(model, Isotropic), "model must be a elastic model"
+        self._needUpdate = True
+        super().__init__(mesh, model, folder, verbosity)
+        assert mesh.dim == model.dim, "mesh and material dimensions differ"
+        self.rho = 1.0
+        self._K = None
+        self._u = torch.zeros(mesh.Nn * model.dim, dtype=torch.float64, device="cuda")
+
+    def Get_problemTypes(self) -> list:
+        return ["elastic"]
+
+    def Get_unknowns(self, problemType=None) -> list:
+        return ["x", "y"] if self.model.dim == 2 else ["x", "y", "z"]
+
+    @property
+    def material(self) -> Isotropic:
+        return self.model
+
+    @property
+    def needUpdate(self) -> bool:
+        return self._needUpdate
+
+    def Need_Update(self, value: bool = True) -> None:
+        self._needUpdate = value
+
+    def Assembly(self):
+        """Element stiffness on the device and deterministic scatter."""
+        dev = self.dev
+        A = dev.csr_map(self.model.dim)
+        Ke = A.new_entries()
+        tab, nPg = dev.tables(MatrixType.rigi)
+        E = to_dev(self.model._E_array())
+        _lib.call("efea_elastic_ke", dev.code, 64, dev.Ne, ptr(dev.connect), ptr(dev.coord), ptr(tab), nPg, ptr(E),
+                  1 if self.model.isHeterogeneous else 0, self.model.v, int(self.model.planeStress),
+                  self.model.thickness, ptr(Ke), stream_ptr())
+        A.scatter(Ke)
+        self._K = A
+        self._needUpdate = False
+        return A
+
+    def _device_K(self):
+        if self._needUpdate or self._K is None:
+            self.Assembly()
+        return self._K
+
+    def Get_K_C_M_F(self, problemType=None):
+        """Fresh scipy copies ``(K, C, M, F)``; ``C = M = 0`` (static),
+        ``F`` the assembled load (none for this model)."""
+        K = self._device_K().to_scipy()
+        Ndof = K.shape[0]
+        z = sp.csr_matrix((Ndof, Ndof))
+        return K, z.copy(), z.copy(), sp.csr_matrix((Ndof, 1))
+
+    @property
+    def displacement(self) -> np.ndarray:
+        return self._u.cpu().numpy()
+
+    def Get_x0(self, problemType=None) -> np.ndarray:
+        return self.displacement
+
+    def Solve(self) -> np.ndarray:
+        A = self._device_K()
+        b = to_dev(self._neumann_vector("elastic"))
+        x = self._u.clone()
+        self._solve_system(A, b, x, "elastic")
+        self._u = x
+        return self.displacement
+
+    def Calc_Reaction(self, dofs=None, problemType=None) -> np.ndarray:
+        A = self._device_K()
+        y = A.spmv(self._u)
+        dofs = self.Get_dofs() if dofs is None else np.asarray(dofs)
+        return y.cpu().numpy()[dofs]
+
+    def _Calc_Psi_Elas(self) -> float:
+        A = self._device_K()
+        y = A.spmv(self._u)
+        return 0.5 * float(torch.dot(self._u, y))
+
+    def Save_Iter(self, iter: Optional[dict] = None) -> None:
+        it = {} if iter is None else dict(iter)
+        it["displacement"] = self.displacement
+        super().Save_Iter(it)
+
+    def Set_Iter(self, iter: int = -1) -> Optional[dict]:
+        r = super().Set_Iter(iter)
+        if r is not None:
+            self._u = to_dev(r["displacement"])
+        return r
+
+    def Result(self, option: str) -> np.ndarray:
+        u = self.displacement.reshape(-1, self.model.dim)
+        comp = {"ux": 0, "uy": 1, "uz": 2}
+        if option in comp:
+            return u[:, comp[option]].copy()
+        if option == "displacement":
+            return self.displacement
+        raise ValueError(f"unknown result {option}")
+
+
+class PhaseField(_Simu):
+    """Staggered phase-field fracture simulation (``Simulations/_phasefield.py``)."""
+
+    class ProblemTypes:
+        damage = "damage"
+        elastic = "elastic"
+
+    def __init__(self, mesh: Mesh, model: PhaseFieldModel, folder: str = "", verbosity=False):
+        assert isinstance(model, PhaseFieldModel), "model must be a phase field model"
+        assert mesh.dim == 2 and mesh.elemType in (ElemType.TRI3, ElemType.QUAD4), \
+            "the device phase-field path supports 2D TRI3 / QUAD4 meshes"
+        self._updU = False
+        self._updD = False
+        super().__init__(mesh, model, folder, verbosity)
+        model.material._Add_observer(self)
+        dim = model.dim
+        Nn, Ne = mesh.Nn, mesh.Ne
+        self._u = torch.zeros(Nn * dim, dtype=torch.float64, device="cuda")
+        self._d = torch.zeros(Nn, dtype=torch.float64, device="cuda")
+        nPgm = gauss_points(mesh.elemType, MatrixType.mass)[1].size
+        self._psiP = torch.zeros(Ne * nPgm, dtype=torch.float64, device="cuda")
+        self._H = None  # history (old psi+), None until the first Save_Iter
+        self._Ku = self._Kd = None
+        self._Fd = torch.zeros(Nn, dtype=torch.float64, device="cuda")
+        self.Niter_staggered = 0
+        self.convIter = 0.0
+
+    def Get_problemTypes(self) -> list:
+        return ["damage", "elastic"]
+
+    @property
+    def problemType(self) -> str:
+        return "elastic"
+
+    def Get_unknowns(self, problemType=None) -> list:
+        if problemType == "damage":
+            return ["d"]
+        return ["x", "y"] if self.model.dim == 2 else ["x", "y", "z"]
+
+    def Get_dof_n(self, problemType=None) -> int:
+        return 1 if problemType == "damage" else self.model.dim
+
+    @property
+    def phaseFieldModel(self) -> PhaseFieldModel:
+        return self.model
+
+    @property
+    def needUpdate(self) -> bool:
+        return not self._updD or not self._updU
+
+    def Need_Update(self, value: bool = True) -> None:
+        self._updD = not value
+        self._updU = not value
+
+    @property
+    def displacement(self) -> np.ndarray:
+        return self._u.cpu().numpy()
+
+    @property
+    def damage(self) -> np.ndarray:
+        return self._d.cpu().numpy()
+
+    def _material_args(self):
+        mat = self.model.material
+        return to_dev(mat._E_array()), (1 if mat.isHeterogeneous else 0), mat.v, int(mat.planeStress)
+
+    # --------------------------------------------------------- assembly
+    def _assemble_damage(self) -> None:
+        dev, pfm = self.dev, self.model
+        Kd = dev.csr_map(1)
+        Ke = Kd.new_entries()
+        Fe = torch.empty(dev.Ne * dev.nPe, dtype=torch.float64, device="cuda")
+        tm, nm = dev.tables(MatrixType.mass)
+        tr, nr = dev.tables(MatrixType.rigi)
+        E, Es, nu, ps = self._material_args()
+        Gc = to_dev(pfm._Gc_array())
+        _lib.call("efea_pf_damage_ke", dev.code, dev.Ne, ptr(dev.connect), ptr(dev.coord), ptr(tm), nm, ptr(tr), nr,
+                  ptr(self._u), ptr(self._H), ptr(E), Es, nu, ps, ptr(Gc), 1 if pfm.isHeterogeneous else 0,
+                  pfm.l0, 0 if pfm.split == "Amor" else 1, 0 if pfm.regularization == "AT1" else 1,
+                  pfm.thickness, ptr(self._psiP), ptr(Ke), ptr(Fe), stream_ptr())
+        Kd.scatter(Ke)
+        dev.vec_map(1).assemble(Fe, self._Fd)
+        self._Kd = Kd
+        self._updD = True
+
+    def _assemble_elastic(self) -> None:
+        dev, pfm = self.dev, self.model
+        Ku = dev.csr_map(pfm.dim)
+        Ke = Ku.new_entries()
+        tr, nr = dev.tables(MatrixType.rigi)
+        E, Es, nu, ps = self._material_args()
+        _lib.call("efea_pf_elastic_ke", dev.code, dev.Ne, ptr(dev.connect), ptr(dev.coord), ptr(tr), nr,
+                  ptr(self._u), ptr(self._d), ptr(E), Es, nu, ps, 0 if pfm.split == "Amor" else 1, pfm.thickness,
+                  ptr(Ke), stream_ptr())
+        Ku.scatter(Ke)
+        self._Ku = Ku
+        self._updU = True
+
+    def _device_K(self, problemType):
+        """``Get_K_C_M_F`` caching (``_phasefield.py:248-269``)."""
+        if problemType == "elastic":
+            if not self._updU:
+                self._assemble_elastic()
+            return self._Ku
+        if not self._updD:
+            self._assemble_damage()
+        return self._Kd
+
+    def Get_K_C_M_F(self, problemType=None):
+        problemType = problemType or "elastic"
+        A = self._device_K(problemType)
+        K = A.to_scipy()
+        z = sp.csr_matrix(K.shape)
+        if problemType == "elastic":
+            return K, z, z.copy(), sp.csr_matrix((K.shape[0], 1))
+        return K, z, z.copy(), self.dev.vec_map(1).to_scipy(self._Fd)
+
+    # --------------------------------------------------------- energies
+    def _quad(self, A, x) -> float:
+        y = A.spmv(x)
+        return 0.5 * float(torch.dot(x, y))
+
+    def _Calc_Psi_Crack(self) -> float:
+        Kd = self._device_K("damage")
+        if float(torch.linalg.vector_norm(self._d)) == 0:
+            return 0.0
+        return self._quad(Kd, self._d)
+
+    def _Calc_Psi_Elas(self) -> float:
+        Ku = self._device_K("elastic")
+        if float(torch.linalg.vector_norm(self._u)) == 0:
+            return 0.0
+        return self._quad(Ku, self._u)
+
+    def _Calc_Psi_Ext(self, fu: torch.Tensor) -> float:
+        if float(torch.linalg.vector_norm(self._u)) == 0 or float(torch.linalg.vector_norm(fu)) == 0:
+            return 0.0
+        return float(torch.dot(self._u, fu))
+
+    # ------------------------------------------------------------ solves
+    def _solve_damage(self) -> None:
+        Kd = self._device_K("damage")
+        b = self._Fd + to_dev(self._neumann_vector("damage"))
+        x = self._d.clone()
+        self._solve_system(Kd, b, x, "damage")
+        self._d = x
+
+    def _solve_elastic(self) -> None:
+        Ku = self._device_K("elastic")
+        b = to_dev(self._neumann_vector("elastic"))
+        x = self._u.clone()
+        self._solve_system(Ku, b, x, "elastic")
+        self._u = x
+
+    def Solve(self, tolConv=1.0, maxIter=500, convOption=2):
+        """Staggered scheme (``_phasefield.py:303-432``): returns ``(u, d, converged)``."""
+        assert 0 < tolConv <= 1, "tolConv must be between 0 and 1."
+        assert maxIter > 1, "Must be > 1."
+        Niter, converged, convIter = 0, False, 0.0
+        t0 = time.perf_counter()
+        if convOption == 2:
+            fu = to_dev(self._neumann_vector("elastic"))
+        while not converged and Niter < maxIter:
+            Niter += 1
+            d_n, u_n = self._d, self._u
+            if convOption == 1:
+                E_n = self._Calc_Psi_Crack()
+            elif convOption == 2:
+                E_n = self._Calc_Psi_Crack() + self._Calc_Psi_Elas() - self._Calc_Psi_Ext(fu)
+            self._solve_damage()
+            self._updU = False
+            self._solve_elastic()
+            self._updD = False
+            d_np1, u_np1 = self._d, self._u
+            if convOption == 0:
+                convIter = float(torch.max(torch.abs(d_np1 - d_n)))
+            elif convOption in (1, 2):
+                E_np1 = self._Calc_Psi_Crack()
+                if convOption == 2:
+                    E_np1 += self._Calc_Psi_Elas() - self._Calc_Psi_Ext(fu)
+                convIter = abs(E_n - E_np1) if E_np1 == 0 else abs((E_n - E_np1) / E_np1)
+            elif convOption == 3:
+                diffU = torch.abs(u_np1 - u_n)
+                nz = u_np1 != 0
+                diffU[nz] = diffU[nz] / torch.abs(u_np1[nz])
+                diffD = torch.abs(d_np1 - d_n)
+                nz = d_np1 != 0
+                diffD[nz] = diffD[nz] / torch.abs(d_np1[nz])
+                convU, convD = float(diffU.sum()), float(diffD.sum())
+                convIter = max(convD, convU)
+            if tolConv == 1 or float(d_np1.max()) == 0:
+                converged = True
+            elif convOption == 3:
+                converged = (convD <= tolConv) and (convU <= tolConv * 0.999)
+            else:
+                converged = convIter <= tolConv
+        self.Niter_staggered = Niter
+        self.convIter = convIter
+        self.timeIter = time.perf_counter() - t0
+        return self.displacement, self.damage, converged
+
+    def Save_Iter(self, iter: Optional[dict] = None) -> None:
+        """Advances the history field (``_phasefield.py:613-630``)."""
+        it = {} if iter is None else dict(iter)
+        it["Niter"] = self.Niter_staggered
+        it["convIter"] = self.convIter
+        it["timeIter"] = getattr(self, "timeIter", 0.0)
+        self._H = self._psiP.clone()
+        it["displacement"] = self._u.clone()
+        it["damage"] = self._d.clone()
+        super().Save_Iter(it)
+
+    def Set_Iter(self, iter: int = -1, resetAll=False) -> Optional[dict]:
+        r = super().Set_Iter(iter)
+        if r is None:
+            return None
+        self._u = r["displacement"].clone()
+        self._d = r["damage"].clone()
+        self._updD = self._updU = False
+        if resetAll:
+            self._H = None
+            self._assemble_damage()
+            self._H = self._psiP.clone()
+        return r
+
+    def Calc_Reaction(self, dofs=None, problemType="elastic") -> np.ndarray:
+        A = self._device_K(problemType)
+        x = self._u if problemType == "elastic" else self._d
+        y = A.spmv(x)
+        dofs = self.Get_dofs(problemType) if dofs is None else np.asarray(dofs)
+        return y.cpu().numpy()[dofs]
+
+    def Detect_Damage(self, nodes, threshold: float = 1.0) -> bool:
+        nodes = np.asarray(nodes, dtype=np.int64)
+        if nodes.size == 0:
+            return False
+        return bool(torch.any(self._d[to_dev(nodes)] >= threshold))
+
+    def Result(self, option: str) -> np.ndarray:
+        if option == "damage":
+            return self.damage
+        u = self.displacement.reshape(-1, self.model.dim)
+        comp = {"ux": 0, "uy": 1}
+        if option in comp:
+            return u[:, comp[option]].copy()
+        if option == "displacement":
+            return self.displacement
+        raise ValueError(f"unknown result {option}")

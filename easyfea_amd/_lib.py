@@ -1,0 +1,91 @@
+"""ctypes binding of ``libefea_hip.so`` (declarations: ``include/efea.h``).
+
+The product path has no CPU fallback: if the library is missing, or no GPU
+is visible, every entry point raises.
+"""
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libefea_hip.so")
+
+vp = C.c_void_p
+i32, i64, f64, sz = C.c_int, C.c_int64, C.c_double, C.c_size_t
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "efea_version": (i32, []),
+    "efea_last_error": (C.c_char_p, []),
+    "efea_elastic_ke": (i32, [i32, i32, i64, vp, vp, vp, i32, vp, i32, f64, i32, f64, vp, vp]),
+    "efea_reaction_diffusion_ke": (
+        i32,
+        [i32, i32, i64, vp, vp, vp, i32, vp, i32, vp, f64, vp, i32, vp, f64, vp, vp, vp],
+    ),
+    "efea_boundary_load": (i32, [i32, i64, vp, vp, i32, vp, vp, vp]),
+    "efea_csr_scatter": (i32, [i32, i64, vp, vp, vp, vp, vp]),
+    "efea_cast_f64_f32": (i32, [i64, vp, vp, vp]),
+    "efea_spmv_scratch_bytes": (sz, [i32, i32]),
+    "efea_spmv": (i32, [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "efea_bj_setup": (i32, [i32, i32, i32, vp, vp, vp, vp, vp, vp]),
+    "efea_pcg_workspace_bytes": (sz, [i32, i32, i32]),
+    "efea_default_group": (i32, [i32, i32]),
+    "efea_solve_pcg": (
+        i32,
+        [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, f64, f64, f64, i32, i32, vp, vp],
+    ),
+    "efea_dot": (i32, [i64, vp, vp, vp, vp, vp]),
+    "efea_gather_sum": (i32, [i64, vp, vp, vp, vp]),
+    "efea_pf_damage_ke": (
+        i32,
+        [i32, i64, vp, vp, vp, i32, vp, i32, vp, vp, vp, i32, f64, i32, vp, i32, f64, i32, i32, f64, vp, vp,
+         vp, vp],
+    ),
+    "efea_pf_elastic_ke": (i32, [i32, i64, vp, vp, vp, i32, vp, vp, vp, i32, f64, i32, i32, f64, vp, vp]),
+    "efea_pf_split_points": (i32, [i64, i32, vp, vp, i32, f64, i32, vp, vp, vp, vp, vp]),
+}
+
+ELEM_CODE = {"TRI3": 0, "QUAD4": 1, "HEXA8": 2, "TETRA4": 3, "TETRA10": 4, "SEG2": 5}
+
+ERR_NOCONV = -3
+
+_lib = None
+
+
+class EfeaError(RuntimeError):
+    def __init__(self, name, code, msg):
+        super().__init__(f"{name} failed ({code}): {msg}")
+        self.code = code
+
+
+def load(path: str = LIB_PATH):
+    """Loads the shared library (no GPU needed) and declares every symbol."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(
+            f"{path} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(hipcc --offload-arch=gfx950); there is no CPU fallback"
+        )
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def call(name: str, *args) -> int:
+    """Calls ``name``; raises ``EfeaError`` on a negative status other than
+    non-convergence, which is returned for the caller to handle."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if isinstance(rc, int) and rc < 0 and rc != ERR_NOCONV:
+        raise EfeaError(name, rc, lib.efea_last_error().decode())
+    return rc
+
+
+def last_error() -> str:
+    return load().efea_last_error().decode()

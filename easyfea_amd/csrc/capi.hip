@@ -1,0 +1,228 @@
+// C ABI of the efea HIP library: plain pointers, sizes and a hipStream_t,
+// int status returns (0 ok, negative error), a thread-local message getter.
+// Declarations and their reference counterparts: include/efea.h.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "efea_common.h"
+
+namespace efea {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int check_launch(const char* what) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("%s: %s", what, hipGetErrorString(e));
+    return EFEA_ERR_HIP;
+  }
+  return EFEA_OK;
+}
+
+// implemented in elements.hip / sparse.hip / phasefield.hip
+int elastic_ke(int, int, int64_t, const int*, const double*, const double*, int, const double*, int, double, int,
+               double, void*, hipStream_t);
+int reaction_diffusion(int, int, int64_t, const int*, const double*, const double*, int, const double*, int,
+                       const double*, double, const double*, int, const double*, double, void*, void*,
+                       hipStream_t);
+int boundary_load(int, int64_t, const double*, const double*, int, const double*, double*, hipStream_t);
+int scatter(int, int64_t, const int*, const int*, const void*, void*, hipStream_t);
+int cast_f64_f32(int64_t, const double*, float*, hipStream_t);
+int spmv(int, int, int, int, const int*, const int*, const void*, const unsigned char*, const double*, double*,
+         void*, hipStream_t);
+size_t spmv_scratch_bytes(int, int);
+int bj_setup(int, int, int, const int*, const int*, const double*, const unsigned char*, void*, hipStream_t);
+size_t pcg_workspace_bytes(int, int, int);
+int solve_pcg(int, int, int, int, const int*, const int*, const double*, const float*, const unsigned char*,
+              const void*, const double*, double*, void*, double, double, double, int, int, hipStream_t, double*);
+int dot(int64_t, const double*, const double*, double*, double*, unsigned*, hipStream_t);
+int gather_sum(int64_t, const int*, const double*, double*, hipStream_t);
+int default_group(int, int);
+int pf_damage(int, int64_t, const int*, const double*, const double*, int, const double*, int, const double*,
+              const double*, const double*, int, double, int, const double*, int, double, int, int, double,
+              double*, double*, double*, hipStream_t);
+int pf_elastic(int, int64_t, const int*, const double*, const double*, int, const double*, const double*,
+               const double*, int, double, int, int, double, double*, hipStream_t);
+int pf_split_points(int64_t, int, const double*, const double*, int, double, int, double*, double*, double*,
+                    double*, hipStream_t);
+
+}  // namespace efea
+
+using namespace efea;
+
+#define STREAM(s) ((hipStream_t)(s))
+
+extern "C" {
+
+int efea_version(void) { return EFEA_ABI_VERSION; }
+
+const char* efea_last_error(void) { return g_err; }
+
+int efea_elastic_ke(int elem_type, int precision, int64_t Ne, const int32_t* connect, const double* coord,
+                    const double* tables, int nPg, const double* E, int E_stride, double nu, int plane_stress,
+                    double thickness, void* Ke, void* stream) {
+  if (Ne < 0 || (Ne > 0 && (!connect || !coord || !tables || !E || !Ke))) {
+    set_error("efea_elastic_ke: null pointer or negative size");
+    return EFEA_ERR_ARG;
+  }
+  return elastic_ke(elem_type, precision, Ne, connect, coord, tables, nPg, E, E_stride, nu, plane_stress,
+                    thickness, Ke, STREAM(stream));
+}
+
+int efea_reaction_diffusion_ke(int elem_type, int precision, int64_t Ne, const int32_t* connect,
+                               const double* coord, const double* tables_mass, int nPg_mass,
+                               const double* tables_rigi, int nPg_rigi, const double* r, double r_const,
+                               const double* k, int k_stride, const double* f, double scale, void* Ke, void* Fe,
+                               void* stream) {
+  if (Ne < 0 || (Ne > 0 && (!connect || !coord || !tables_mass || !tables_rigi || !Ke || (Fe && !f)))) {
+    set_error("efea_reaction_diffusion_ke: null pointer or negative size");
+    return EFEA_ERR_ARG;
+  }
+  return reaction_diffusion(elem_type, precision, Ne, connect, coord, tables_mass, nPg_mass, tables_rigi,
+                            nPg_rigi, r, r_const, k, k_stride, f, scale, Ke, Fe, STREAM(stream));
+}
+
+int efea_boundary_load(int elem_type, int64_t Ne, const double* X_local, const double* tables, int nPg,
+                       const double* f, double* out, void* stream) {
+  if (Ne < 0 || (Ne > 0 && (!X_local || !tables || !f || !out))) {
+    set_error("efea_boundary_load: null pointer or negative size");
+    return EFEA_ERR_ARG;
+  }
+  return boundary_load(elem_type, Ne, X_local, tables, nPg, f, out, STREAM(stream));
+}
+
+int efea_csr_scatter(int precision, int64_t nslot, const int32_t* seg_ptr, const int32_t* perm,
+                     const void* entries, void* out_values, void* stream) {
+  if (nslot < 0 || (nslot > 0 && (!seg_ptr || !perm || !entries || !out_values))) {
+    set_error("efea_csr_scatter: null pointer or negative size");
+    return EFEA_ERR_ARG;
+  }
+  return scatter(precision, nslot, seg_ptr, perm, entries, out_values, STREAM(stream));
+}
+
+int efea_cast_f64_f32(int64_t n, const double* src, float* dst, void* stream) {
+  if (n < 0 || (n > 0 && (!src || !dst))) {
+    set_error("efea_cast_f64_f32: null pointer or negative size");
+    return EFEA_ERR_ARG;
+  }
+  return cast_f64_f32(n, src, dst, STREAM(stream));
+}
+
+size_t efea_spmv_scratch_bytes(int Nnode, int group) { return spmv_scratch_bytes(Nnode, group); }
+
+int efea_spmv(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr, const int32_t* node_cols,
+              const void* values, const uint8_t* known, const double* x, double* y, void* scratch, void* stream) {
+  if (Nnode < 0 || (Nnode > 0 && (!node_ptr || !node_cols || !values || !x || !y || !scratch))) {
+    set_error("efea_spmv: null pointer or negative size");
+    return EFEA_ERR_ARG;
+  }
+  return spmv(precision, Nnode, dof_n, group, node_ptr, node_cols, values, known, x, y, scratch, STREAM(stream));
+}
+
+int efea_bj_setup(int precision, int Nnode, int dof_n, const int32_t* node_ptr, const int32_t* node_cols,
+                  const double* values, const uint8_t* known, void* Minv, void* stream) {
+  if (Nnode < 0 || (Nnode > 0 && (!node_ptr || !node_cols || !values || !Minv))) {
+    set_error("efea_bj_setup: null pointer or negative size");
+    return EFEA_ERR_ARG;
+  }
+  return bj_setup(precision, Nnode, dof_n, node_ptr, node_cols, values, known, Minv, STREAM(stream));
+}
+
+size_t efea_pcg_workspace_bytes(int Nnode, int dof_n, int group) {
+  return pcg_workspace_bytes(Nnode, dof_n, group);
+}
+
+int efea_default_group(int dof_n, int max_neighbours) { return default_group(dof_n, max_neighbours); }
+
+int efea_solve_pcg(int mode, int Nnode, int dof_n, int group, const int32_t* node_ptr, const int32_t* node_cols,
+                   const double* values64, const float* values32, const uint8_t* known, const void* Minv,
+                   const double* b, double* x, void* workspace, size_t workspace_bytes, double tol_residual,
+                   double tol_anorm, double replace_ratio, int maxit, int check_every, void* stream,
+                   double* info) {
+  if (Nnode <= 0 || !node_ptr || !node_cols || !values64 || !Minv || !b || !x || !workspace || !info) {
+    set_error("efea_solve_pcg: null pointer or empty system");
+    return EFEA_ERR_ARG;
+  }
+  if (workspace_bytes < pcg_workspace_bytes(Nnode, dof_n, group)) {
+    set_error("efea_solve_pcg: workspace of %zu bytes, %zu needed", workspace_bytes,
+              pcg_workspace_bytes(Nnode, dof_n, group));
+    return EFEA_ERR_SIZE;
+  }
+  return solve_pcg(mode, Nnode, dof_n, group, node_ptr, node_cols, values64, values32, known, Minv, b, x,
+                   workspace, tol_residual, tol_anorm, replace_ratio, maxit, check_every, STREAM(stream), info);
+}
+
+int efea_dot(int64_t n, const double* a, const double* b, double* scratch, double* out, void* stream) {
+  if (n < 0 || !a || !scratch || !out) {
+    set_error("efea_dot: null pointer or negative size");
+    return EFEA_ERR_ARG;
+  }
+  // scratch: 1024 partials + a counter word
+  return dot(n, a, b, scratch, out, (unsigned*)(scratch + 1024), STREAM(stream));
+}
+
+int efea_gather_sum(int64_t n, const int32_t* idx, const double* y, double* out, void* stream) {
+  if (n < 0 || (n > 0 && (!idx || !y)) || !out) {
+    set_error("efea_gather_sum: null pointer or negative size");
+    return EFEA_ERR_ARG;
+  }
+  return gather_sum(n, idx, y, out, STREAM(stream));
+}
+
+int efea_pf_damage_ke(int elem_type, int64_t Ne, const int32_t* connect, const double* coord,
+                      const double* tables_mass, int nPg_mass, const double* tables_rigi, int nPg_rigi,
+                      const double* u, const double* H_old, const double* E, int E_stride, double nu,
+                      int plane_stress, const double* Gc, int Gc_stride, double l0, int split, int regu,
+                      double thickness, double* psiP, double* Kd, double* Fd, void* stream) {
+  if (Ne < 0 || (Ne > 0 && (!connect || !coord || !tables_mass || !tables_rigi || !u || !E || !Gc || !psiP ||
+                            !Kd || !Fd))) {
+    set_error("efea_pf_damage_ke: null pointer or negative size");
+    return EFEA_ERR_ARG;
+  }
+  if (split < 0 || split > 1 || regu < 0 || regu > 1) {
+    set_error("efea_pf_damage_ke: split %d / regularisation %d not supported", split, regu);
+    return EFEA_ERR_ARG;
+  }
+  return pf_damage(elem_type, Ne, connect, coord, tables_mass, nPg_mass, tables_rigi, nPg_rigi, u, H_old, E,
+                   E_stride, nu, plane_stress, Gc, Gc_stride, l0, split, regu, thickness, psiP, Kd, Fd,
+                   STREAM(stream));
+}
+
+int efea_pf_elastic_ke(int elem_type, int64_t Ne, const int32_t* connect, const double* coord,
+                       const double* tables_rigi, int nPg_rigi, const double* u, const double* d,
+                       const double* E, int E_stride, double nu, int plane_stress, int split, double thickness,
+                       double* Ke, void* stream) {
+  if (Ne < 0 || (Ne > 0 && (!connect || !coord || !tables_rigi || !u || !d || !E || !Ke))) {
+    set_error("efea_pf_elastic_ke: null pointer or negative size");
+    return EFEA_ERR_ARG;
+  }
+  if (split < 0 || split > 1) {
+    set_error("efea_pf_elastic_ke: split %d not supported", split);
+    return EFEA_ERR_ARG;
+  }
+  return pf_elastic(elem_type, Ne, connect, coord, tables_rigi, nPg_rigi, u, d, E, E_stride, nu, plane_stress,
+                    split, thickness, Ke, STREAM(stream));
+}
+
+int efea_pf_split_points(int64_t n, int split, const double* eps, const double* E, int E_stride, double nu,
+                         int plane_stress, double* cP, double* cM, double* psiP, double* psiM, void* stream) {
+  if (n < 0 || (n > 0 && (!eps || !E || !cP || !cM || !psiP || !psiM))) {
+    set_error("efea_pf_split_points: null pointer or negative size");
+    return EFEA_ERR_ARG;
+  }
+  if (split < 0 || split > 1) {
+    set_error("efea_pf_split_points: split %d not supported", split);
+    return EFEA_ERR_ARG;
+  }
+  return pf_split_points(n, split, eps, E, E_stride, nu, plane_stress, cP, cM, psiP, psiM, STREAM(stream));
+}
+
+}  // extern "C"

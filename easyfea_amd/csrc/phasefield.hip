@@ -1,0 +1,411 @@
+// Phase-field element kernels (2D: TRI3, QUAD4), FP64 throughout.
+//
+//  * k_pf_damage: at the mass points, strain from u, energy split psi+,
+//    history psi+ <- old where psi+ < old (Simulations/_phasefield.py:494-538),
+//    then the damage operators
+//      Kd_e = sum_m r wJ N^T N + k sum_r wJ dN^T dN ,   Fd_e = sum_m f wJ N
+//    (_phasefield.py:540-573; Models/_phasefield.py:236-293), x thickness.
+//  * k_pf_elastic: at the rigi points, strain from the current u, split
+//    c+ / c-, degradation g = (1 - N.d)^2 + 1e-12, c = g c+ + c-, and
+//    K_e = sum_p wJ B^T c B x thickness (_phasefield.py:444-482).
+// The split follows Models/_phasefield.py: Amor :451-483, Miehe :503-537 with
+// the 2D closed-form eigen decomposition :751-802 and the projector build
+// :1043-1128; Rp/Rm :485-501.
+#include "efea_common.h"
+
+namespace efea {
+
+struct PfParams {
+  int split;  // 0 Amor, 1 Miehe
+  int regu;   // 0 AT1, 1 AT2
+  int planeStress;
+  double nu;
+  double l0;
+  double thickness;
+  double k_res;
+};
+
+__device__ inline void lame2d(double E, double nu, int planeStress, double& lam, double& mu) {
+  lam = E * nu / ((1 + nu) * (1 - 2 * nu));
+  if (planeStress) lam = E * nu / (1 - nu * nu);
+  mu = E / (2 * (1 + nu));
+}
+
+__device__ inline double sgn(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : (x == 0 ? 0.0 : x)); }
+__device__ inline double heav(double x) { return x < 0 ? 0.0 : (x > 0 ? 1.0 : (x == 0 ? 0.5 : x)); }
+
+// Kelvin-Mandel split of the 2D strain e = [exx, eyy, sqrt(2) exy].
+__device__ inline void split2d(int split, const double e[3], double lam, double mu, double cP[3][3],
+                               double cM[3][3]) {
+  const double tr_e = e[0] + e[1];
+  const double Rp = (1 + sgn(tr_e)) / 2, Rm = (1 + sgn(-tr_e)) / 2;
+  const double IxI[3][3] = {{1, 1, 0}, {1, 1, 0}, {0, 0, 0}};
+  if (split == 0) {  // Amor
+    const double bulk = lam + 2 * mu / 2;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double Iij = (i == j) ? 1.0 : 0.0;
+        cP[i][j] = bulk * (Rp * IxI[i][j]) + 2 * mu * (Iij - 1.0 / 2 * IxI[i][j]);
+        cM[i][j] = bulk * (Rm * IxI[i][j]);
+      }
+    return;
+  }
+  // Miehe: eigenvalues / projectors of the strain tensor
+  const double r2 = sqrt(2.0);
+  const double m00 = e[0], m11 = e[1], m01 = e[2] / r2;
+  const double det = m00 * m11 - m01 * m01;
+  const double tr = m00 + m11;
+  const double dl = tr * tr - 4 * det;
+  const double sq = sqrt(dl);
+  const double v0 = (tr - sq) / 2, v1 = (tr + sq) / 2;
+  double A00 = 1, A01 = 0, A11 = 0;  // M1
+  if (v0 != v1) {
+    const double d = v0 - v1;
+    A00 = (m00 - v1) / d;
+    A01 = (m01 - 0.0) / d;
+    A11 = (m11 - v1) / d;
+  }
+  const double B00 = 1 - A00, B01 = 0 - A01, B11 = 1 - A11;  // M2 = I - M1
+  const double m1[3] = {A00, A11, A01 * r2};
+  const double m2[3] = {B00, B11, B01 * r2};
+  const double vp0 = (v0 + fabs(v0)) / 2, vp1 = (v1 + fabs(v1)) / 2;
+  const double dv0 = heav(v0), dv1 = heav(v1);
+  double vd = v0 - v1;
+  if (vd == 0) vd = 1;
+  const double betaP = (vp0 - vp1) / vd;
+  const double g0 = dv0 - betaP, g1 = dv1 - betaP;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double Iij = (i == j) ? 1.0 : 0.0;
+      const double P = ((betaP * Iij) + (g0 * (m1[i] * m1[j]))) + (g1 * (m2[i] * m2[j]));
+      const double M = Iij - P;
+      cP[i][j] = lam * (Rp * IxI[i][j]) + 2 * mu * P;
+      cM[i][j] = lam * (Rm * IxI[i][j]) + 2 * mu * M;
+    }
+}
+
+template <int NPE>
+__device__ inline void strain2d(const double (&dNx)[2][NPE], const double* ue, double e[3]) {
+  const double c = 1 / sqrt(2.0);
+  e[0] = e[1] = e[2] = 0;
+#pragma unroll
+  for (int n = 0; n < NPE; ++n) {
+    e[0] += dNx[0][n] * ue[2 * n];
+    e[1] += dNx[1][n] * ue[2 * n + 1];
+    e[2] += (dNx[1][n] * c) * ue[2 * n] + (dNx[0][n] * c) * ue[2 * n + 1];
+  }
+}
+
+template <int ET, int NPGM, int NPGR>
+__global__ void __launch_bounds__(256)
+k_pf_damage(int64_t Ne, const int* __restrict__ connect, const double* __restrict__ coord,
+            const double* __restrict__ tab_m, const double* __restrict__ tab_r, const double* __restrict__ u,
+            const double* __restrict__ H_old, const double* __restrict__ E, int E_stride,
+            const double* __restrict__ Gc, int Gc_stride, PfParams P, double* __restrict__ psiP,
+            double* __restrict__ Kd, double* __restrict__ Fd) {
+  constexpr int NPE = Elem<ET>::NPE, DIM = 2;
+  constexpr int NPAIR = NPE * NPE;
+  constexpr int EPB = 256 / NPAIR;
+  constexpr int NPGT = NPGM + NPGR;
+  using TM = Tab<NPGM, NPE, DIM>;
+  using TR = Tab<NPGR, NPE, DIM>;
+  __shared__ double s_dNx[EPB][NPGR][DIM][NPE];
+  __shared__ double s_wJr[EPB][NPGR];
+  __shared__ double s_wJm[EPB][NPGM];
+  __shared__ double s_r[EPB][NPGM];
+  __shared__ double s_f[EPB][NPGM];
+  __shared__ double s_K[EPB * NPAIR];
+
+  const int64_t e0 = (int64_t)blockIdx.x * EPB;
+  const int t = threadIdx.x;
+  if (t < EPB * NPGT) {
+    const int el = t / NPGT, q = t % NPGT;
+    const int64_t e = e0 + el;
+    if (e < Ne) {
+      double X[NPE][3];
+      double ue[2 * NPE];
+#pragma unroll
+      for (int n = 0; n < NPE; ++n) {
+        const int node = connect[e * NPE + n];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) X[n][k] = coord[(int64_t)node * 3 + k];
+        ue[2 * n] = u[(int64_t)node * 2];
+        ue[2 * n + 1] = u[(int64_t)node * 2 + 1];
+      }
+      double dNx[DIM][NPE];
+      if (q < NPGM) {
+        const double detJ = point_geometry<double, NPE, DIM>(tab_m + TM::DN + q * DIM * NPE, X, dNx);
+        s_wJm[el][q] = detJ * tab_m[TM::W + q];
+        double eps[3];
+        strain2d<NPE>(dNx, ue, eps);
+        double lam, mu;
+        lame2d(E[E_stride ? e : 0], P.nu, P.planeStress, lam, mu);
+        double cP[3][3], cM[3][3];
+        split2d(P.split, eps, lam, mu, cP, cM);
+        double psi = 0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          double sig = 0;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) sig += cP[i][j] * eps[j];
+          psi += 1.0 / 2 * eps[i] * sig;
+        }
+        if (H_old) {
+          const double old = H_old[e * NPGM + q];
+          if (psi - old < 0) psi = old;
+        }
+        psiP[e * NPGM + q] = psi;
+        const double gc = Gc[Gc_stride ? e : 0];
+        double r, f;
+        if (P.regu == 1) {
+          r = 2 * psi + gc / P.l0;
+          f = 2 * psi;
+        } else {
+          r = 2 * psi;
+          f = 2 * psi - (3 * gc) / (8 * P.l0);
+          f = (f + fabs(f)) / 2;
+        }
+        s_r[el][q] = r;
+        s_f[el][q] = f;
+      } else {
+        const int p = q - NPGM;
+        const double detJ = point_geometry<double, NPE, DIM>(tab_r + TR::DN + p * DIM * NPE, X, dNx);
+        s_wJr[el][p] = detJ * tab_r[TR::W + p];
+#pragma unroll
+        for (int i = 0; i < DIM; ++i)
+#pragma unroll
+          for (int n = 0; n < NPE; ++n) s_dNx[el][p][i][n] = dNx[i][n];
+      }
+    }
+  }
+  __syncthreads();
+
+  if (t < EPB * NPAIR) {
+    const int el = t / NPAIR, ab = t % NPAIR, a = ab / NPE, b = ab % NPE;
+    const int64_t e = e0 + el;
+    if (e < Ne) {
+      double R = 0;
+#pragma unroll
+      for (int p = 0; p < NPGM; ++p)
+        R += s_r[el][p] * s_wJm[el][p] * tab_m[TM::N + p * NPE + a] * tab_m[TM::N + p * NPE + b];
+      double D = 0;
+#pragma unroll
+      for (int p = 0; p < NPGR; ++p) {
+        double dot = 0;
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) dot += s_dNx[el][p][i][a] * s_dNx[el][p][i][b];
+        D += s_wJr[el][p] * dot;
+      }
+      const double gc = Gc[Gc_stride ? e : 0];
+      const double kd = P.regu == 1 ? gc * P.l0 : 3.0 / 4 * gc * P.l0;
+      s_K[el * NPAIR + ab] = (R + kd * D) * P.thickness;
+      if (b == 0) {
+        double F = 0;
+#pragma unroll
+        for (int p = 0; p < NPGM; ++p) F += s_f[el][p] * s_wJm[el][p] * tab_m[TM::N + p * NPE + a];
+        Fd[e * NPE + a] = F * P.thickness;
+      }
+    }
+  }
+  __syncthreads();
+  const int64_t nval = (int64_t)min((int64_t)EPB, Ne - e0) * NPAIR;
+  double* out = Kd + e0 * NPAIR;
+  for (int64_t k = t; k < nval; k += 256) out[k] = s_K[k];
+}
+
+template <int ET, int NPG>
+__global__ void __launch_bounds__(256)
+k_pf_elastic(int64_t Ne, const int* __restrict__ connect, const double* __restrict__ coord,
+             const double* __restrict__ tab, const double* __restrict__ u, const double* __restrict__ d,
+             const double* __restrict__ E, int E_stride, PfParams P, double* __restrict__ Ke) {
+  constexpr int NPE = Elem<ET>::NPE, DIM = 2, ND = 2 * NPE;
+  constexpr int NPAIR = NPE * NPE;
+  constexpr int EPB = 256 / NPAIR;
+  using TB = Tab<NPG, NPE, DIM>;
+  __shared__ double s_dNx[EPB][NPG][DIM][NPE];
+  __shared__ double s_wJ[EPB][NPG];
+  __shared__ double s_c[EPB][NPG][3][3];
+  __shared__ double s_K[EPB * ND * ND];
+
+  const int64_t e0 = (int64_t)blockIdx.x * EPB;
+  const int t = threadIdx.x;
+  if (t < EPB * NPG) {
+    const int el = t / NPG, p = t % NPG;
+    const int64_t e = e0 + el;
+    if (e < Ne) {
+      double X[NPE][3];
+      double ue[2 * NPE];
+      double de[NPE];
+#pragma unroll
+      for (int n = 0; n < NPE; ++n) {
+        const int node = connect[e * NPE + n];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) X[n][k] = coord[(int64_t)node * 3 + k];
+        ue[2 * n] = u[(int64_t)node * 2];
+        ue[2 * n + 1] = u[(int64_t)node * 2 + 1];
+        de[n] = d[node];
+      }
+      double dNx[DIM][NPE];
+      const double detJ = point_geometry<double, NPE, DIM>(tab + TB::DN + p * DIM * NPE, X, dNx);
+      s_wJ[el][p] = detJ * tab[TB::W + p];
+#pragma unroll
+      for (int i = 0; i < DIM; ++i)
+#pragma unroll
+        for (int n = 0; n < NPE; ++n) s_dNx[el][p][i][n] = dNx[i][n];
+      double eps[3];
+      strain2d<NPE>(dNx, ue, eps);
+      double lam, mu;
+      lame2d(E[E_stride ? e : 0], P.nu, P.planeStress, lam, mu);
+      double cP[3][3], cM[3][3];
+      split2d(P.split, eps, lam, mu, cP, cM);
+      double dp = 0;
+#pragma unroll
+      for (int n = 0; n < NPE; ++n) dp += tab[TB::N + p * NPE + n] * de[n];
+      const double g = (1 - dp) * (1 - dp) + P.k_res;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) s_c[el][p][i][j] = g * cP[i][j] + cM[i][j];
+    }
+  }
+  __syncthreads();
+
+  if (t < EPB * NPAIR) {
+    const int el = t / NPAIR, ab = t % NPAIR, a = ab / NPE, b = ab % NPE;
+    const int64_t e = e0 + el;
+    if (e < Ne) {
+      const double c = 1 / sqrt(2.0);
+      double acc[2][2] = {{0, 0}, {0, 0}};
+#pragma unroll
+      for (int p = 0; p < NPG; ++p) {
+        const double ax = s_dNx[el][p][0][a], ay = s_dNx[el][p][1][a];
+        const double bx = s_dNx[el][p][0][b], by = s_dNx[el][p][1][b];
+        // B columns: node a, component x -> [ax, 0, c ay]; component y -> [0, ay, c ax]
+        const double Ba[2][3] = {{ax, 0, ay * c}, {0, ay, ax * c}};
+        const double Bb[2][3] = {{bx, 0, by * c}, {0, by, bx * c}};
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            double s = 0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              double cb = 0;
+#pragma unroll
+              for (int l = 0; l < 3; ++l) cb += s_c[el][p][k][l] * Bb[j][l];
+              s += Ba[i][k] * cb;
+            }
+            acc[i][j] += s_wJ[el][p] * s;
+          }
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) s_K[el * ND * ND + (2 * a + i) * ND + 2 * b + j] = acc[i][j] * P.thickness;
+    }
+  }
+  __syncthreads();
+  const int64_t nval = (int64_t)min((int64_t)EPB, Ne - e0) * ND * ND;
+  double* out = Ke + e0 * ND * ND;
+  for (int64_t k = t; k < nval; k += 256) out[k] = s_K[k];
+}
+
+// Gauss-point split evaluation for testing and post-processing:
+// eps (n, 3) -> cP, cM (n, 3, 3), psiP, psiM (n).
+__global__ void k_pf_split_points(int64_t n, int split, const double* __restrict__ eps,
+                                  const double* __restrict__ E, int E_stride, double nu, int planeStress,
+                                  double* __restrict__ cPo, double* __restrict__ cMo, double* __restrict__ psiP,
+                                  double* __restrict__ psiM) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double e[3] = {eps[3 * i], eps[3 * i + 1], eps[3 * i + 2]};
+  double lam, mu;
+  lame2d(E[E_stride ? i : 0], nu, planeStress, lam, mu);
+  double cP[3][3], cM[3][3];
+  split2d(split, e, lam, mu, cP, cM);
+  double pp = 0, pm = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double sp = 0, sm = 0;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      sp += cP[a][b] * e[b];
+      sm += cM[a][b] * e[b];
+      cPo[9 * i + 3 * a + b] = cP[a][b];
+      cMo[9 * i + 3 * a + b] = cM[a][b];
+    }
+    pp += 1.0 / 2 * e[a] * sp;
+    pm += 1.0 / 2 * e[a] * sm;
+  }
+  psiP[i] = pp;
+  psiM[i] = pm;
+}
+
+static PfParams make_params(int split, int regu, int ps, double nu, double l0, double th) {
+  PfParams P;
+  P.split = split;
+  P.regu = regu;
+  P.planeStress = ps;
+  P.nu = nu;
+  P.l0 = l0;
+  P.thickness = th;
+  P.k_res = 1e-12;
+  return P;
+}
+
+int pf_damage(int elem, int64_t Ne, const int* connect, const double* coord, const double* tab_m, int nPgm,
+              const double* tab_r, int nPgr, const double* u, const double* H_old, const double* E,
+              int E_stride, double nu, int ps, const double* Gc, int Gc_stride, double l0, int split, int regu,
+              double thickness, double* psiP, double* Kd, double* Fd, hipStream_t s) {
+  const PfParams P = make_params(split, regu, ps, nu, l0, thickness);
+  if (Ne == 0) return EFEA_OK;
+  if (elem == ET_TRI3 && nPgm == 3 && nPgr == 1) {
+    const unsigned nb = (unsigned)((Ne + 27) / 28);
+    hipLaunchKernelGGL((k_pf_damage<ET_TRI3, 3, 1>), dim3(nb), dim3(256), 0, s, Ne, connect, coord, tab_m, tab_r,
+                       u, H_old, E, E_stride, Gc, Gc_stride, P, psiP, Kd, Fd);
+    return check_launch("pf_damage");
+  }
+  if (elem == ET_QUAD4 && nPgm == 4 && nPgr == 4) {
+    const unsigned nb = (unsigned)((Ne + 15) / 16);
+    hipLaunchKernelGGL((k_pf_damage<ET_QUAD4, 4, 4>), dim3(nb), dim3(256), 0, s, Ne, connect, coord, tab_m,
+                       tab_r, u, H_old, E, E_stride, Gc, Gc_stride, P, psiP, Kd, Fd);
+    return check_launch("pf_damage");
+  }
+  set_error("pf_damage: unsupported element %d / nPg %d,%d", elem, nPgm, nPgr);
+  return EFEA_ERR_ARG;
+}
+
+int pf_elastic(int elem, int64_t Ne, const int* connect, const double* coord, const double* tab, int nPg,
+               const double* u, const double* d, const double* E, int E_stride, double nu, int ps, int split,
+               double thickness, double* Ke, hipStream_t s) {
+  const PfParams P = make_params(split, 1, ps, nu, 1.0, thickness);
+  if (Ne == 0) return EFEA_OK;
+  if (elem == ET_TRI3 && nPg == 1) {
+    const unsigned nb = (unsigned)((Ne + 27) / 28);
+    hipLaunchKernelGGL((k_pf_elastic<ET_TRI3, 1>), dim3(nb), dim3(256), 0, s, Ne, connect, coord, tab, u, d, E,
+                       E_stride, P, Ke);
+    return check_launch("pf_elastic");
+  }
+  if (elem == ET_QUAD4 && nPg == 4) {
+    const unsigned nb = (unsigned)((Ne + 15) / 16);
+    hipLaunchKernelGGL((k_pf_elastic<ET_QUAD4, 4>), dim3(nb), dim3(256), 0, s, Ne, connect, coord, tab, u, d, E,
+                       E_stride, P, Ke);
+    return check_launch("pf_elastic");
+  }
+  set_error("pf_elastic: unsupported element %d / nPg %d", elem, nPg);
+  return EFEA_ERR_ARG;
+}
+
+int pf_split_points(int64_t n, int split, const double* eps, const double* E, int E_stride, double nu, int ps,
+                    double* cP, double* cM, double* psiP, double* psiM, hipStream_t s) {
+  if (n == 0) return EFEA_OK;
+  hipLaunchKernelGGL(k_pf_split_points, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, n, split, eps, E,
+                     E_stride, nu, ps, cP, cM, psiP, psiM);
+  return check_launch("pf_split_points");
+}
+
+}  // namespace efea

@@ -1,0 +1,149 @@
+/* efea: MI355X (gfx950) HIP library for EasyFEA's finite-element solve path.
+ *
+ * C ABI: plain pointers and sizes, no C++ or torch types.  Every pointer
+ * argument except those documented as host pointers is a device pointer;
+ * `stream` is a hipStream_t (NULL = default stream).  Every int function
+ * returns 0 on success or a negative code (EFEA_ERR_*); efea_last_error()
+ * then returns a thread-local message.  Calls are asynchronous on `stream`
+ * unless stated otherwise, and reentrant per stream.
+ *
+ * Conventions shared with the reference (EasyFEA v3.5.1):
+ *   dof = node * dof_n + component      (FEM/_boundary_conditions.py:160-201)
+ *   element DOF order [n0x, n0y, n1x, ...]   (FEM/_group_elem.py:358-380)
+ *   element matrices (Ne, nd, nd) row-major, the order of
+ *   K_e.ravel() in Simulations/_simu.py:1037
+ *   Kelvin-Mandel strains [xx, yy, sqrt2 xy] / [xx, yy, zz, sqrt2 yz, sqrt2 xz, sqrt2 xy]
+ *   (FEM/_group_elem.py:1241-1312, Models/_utils.py:111-152)
+ * Reference-element tables: one FP64 block per integration rule,
+ *   [w (nPg)] [N (nPg*nPe)] [dN (nPg*dim*nPe)]  (FEM/_gauss.py:364-516, FEM/Elems/*.py)
+ * Element codes: 0 TRI3, 1 QUAD4, 2 HEXA8, 3 TETRA4, 4 TETRA10, 5 SEG2.
+ */
+#ifndef EFEA_H
+#define EFEA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EFEA_ABI_VERSION 1
+
+#define EFEA_ERR_ARG (-1)    /* invalid argument / unsupported combination */
+#define EFEA_ERR_HIP (-2)    /* HIP runtime error */
+#define EFEA_ERR_NOCONV (-3) /* iterative solve did not converge (info filled) */
+#define EFEA_ERR_SIZE (-4)   /* workspace too small */
+
+int efea_version(void);
+const char* efea_last_error(void);
+
+/* Isotropic linear-elastic element stiffness K_e (Ne, nd, nd), precision 64
+ * or 32, Young's modulus E[e*E_stride] (E_stride 0: one value), Poisson nu,
+ * plane stress flag (2D), thickness multiplies 2D matrices.
+ * Replaces Operators.Bilinear.LinearizedElasticity (FEM/Operators/Bilinear.py:62-79)
+ * called from Simulations.Elastic.Construct_local_matrix_system
+ * (Simulations/_elastic.py:123-152) with Isotropic._Behavior (Models/Elastic/_laws.py:407-482). */
+int efea_elastic_ke(int elem_type, int precision, int64_t Ne, const int32_t* connect, const double* coord,
+                    const double* tables, int nPg, const double* E, int E_stride, double nu, int plane_stress,
+                    double thickness, void* Ke, void* stream);
+
+/* Scalar reaction-diffusion-source element operators:
+ * K_e = sum_mass r wJ N^T N + k sum_rigi wJ dN^T dN, F_e = sum_mass f wJ N, times scale.
+ * r: (Ne, nPg_mass) or NULL (then r_const); k: k[e*k_stride] or NULL (no diffusion);
+ * Fe/f may be NULL.  Replaces Operators.Bilinear.UV (Bilinear.py:42-59),
+ * Operators.Bilinear.GradU_A_GradV with A = I (Bilinear.py:229-249) and
+ * Operators.Linear.V (Operators/Linear.py:18-35). */
+int efea_reaction_diffusion_ke(int elem_type, int precision, int64_t Ne, const int32_t* connect,
+                               const double* coord, const double* tables_mass, int nPg_mass,
+                               const double* tables_rigi, int nPg_rigi, const double* r, double r_const,
+                               const double* k, int k_stride, const double* f, double scale, void* Ke, void* Fe,
+                               void* stream);
+
+/* Boundary load integral out[e, a] = sum_p wJ f[e, p] N_a (SEG2 nPg 2, QUAD4/TRI3 faces)
+ * on coordinates given in each element's own frame X_local (Ne, nPe, dim).
+ * Replaces the integration of _Simu.__Bc_Integration_Dim (Simulations/_simu.py:2770-2842). */
+int efea_boundary_load(int elem_type, int64_t Ne, const double* X_local, const double* tables, int nPg,
+                       const double* f, double* out, void* stream);
+
+/* Deterministic CSR scatter: out[s] = sum_{k=seg_ptr[s]}^{seg_ptr[s+1]-1} entries[perm[k]],
+ * accumulated sequentially from zero in increasing element-entry order, with no
+ * atomics (bit-identical to np.bincount given identical entries).
+ * Replaces _Simu.__Assemble_csr's bincount (Simulations/_simu.py:1051-1055)
+ * over the map of _Simu.__Get_csr_map (_simu.py:1062-1102). */
+int efea_csr_scatter(int precision, int64_t nslot, const int32_t* seg_ptr, const int32_t* perm,
+                     const void* entries, void* out_values, void* stream);
+
+int efea_cast_f64_f32(int64_t n, const double* src, float* dst, void* stream);
+
+/* y = K x on the canonical CSR value array indexed through node adjacency
+ * (node_ptr, node_cols): rows of `known` DOFs (uint8 per DOF, may be NULL)
+ * return 0.  precision selects FP64 or FP32 values; x, y are FP64.  Runs the
+ * solver's own SpMV kernel; scratch: efea_spmv_scratch_bytes() device bytes.
+ * Replaces the scipy products K[dofs] @ u of _Simu.Calc_Reaction (_simu.py:212-260)
+ * and Calc_Energy (_simu.py:177-210). */
+size_t efea_spmv_scratch_bytes(int Nnode, int group);
+int efea_spmv(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr, const int32_t* node_cols,
+              const void* values, const uint8_t* known, const double* x, double* y, void* scratch, void* stream);
+
+/* Block-Jacobi setup: inverse of each node's dof_n x dof_n diagonal block of the
+ * Dirichlet-reduced matrix, stored in `precision`. */
+int efea_bj_setup(int precision, int Nnode, int dof_n, const int32_t* node_ptr, const int32_t* node_cols,
+                  const double* values, const uint8_t* known, void* Minv, void* stream);
+
+size_t efea_pcg_workspace_bytes(int Nnode, int dof_n, int group);
+int efea_default_group(int dof_n, int max_neighbours);
+
+/* Dirichlet-reduced solve A_ii x_i = b_i - A_ic x_c by block-Jacobi PCG on the
+ * full-length vectors (known DOFs masked, x holds the Dirichlet values there and
+ * the initial guess elsewhere; x is overwritten with the solution).
+ * mode 64: FP64 matrix/preconditioner.  mode 32: FP32 matrix values and
+ * preconditioner in the Krylov products, FP64 vectors, and the residual replaced
+ * by the FP64 r = b - K64 x whenever it falls below replace_ratio times its
+ * running maximum.  Stops when ||r|| <= tol_residual ||r0|| or when the delayed
+ * A-norm error estimate <= tol_anorm (relative).  Synchronises `stream`.
+ * info (host, 4 doubles): iterations, final true relative residual, residual
+ * replacements, status (1 converged, 2 breakdown, 0 iteration limit).
+ * Replaces __Solver_1 (Simulations/Solvers.py:502-553) with _Solve_Axb's
+ * backends (Solvers.py:225-394; spsolve / pypardiso / cg). */
+int efea_solve_pcg(int mode, int Nnode, int dof_n, int group, const int32_t* node_ptr, const int32_t* node_cols,
+                   const double* values64, const float* values32, const uint8_t* known, const void* Minv,
+                   const double* b, double* x, void* workspace, size_t workspace_bytes, double tol_residual,
+                   double tol_anorm, double replace_ratio, int maxit, int check_every, void* stream,
+                   double* info);
+
+/* Deterministic dot product (b may be NULL: sum of a); scratch holds 1024
+ * doubles plus one zero-initialised counter word. */
+int efea_dot(int64_t n, const double* a, const double* b, double* scratch, double* out, void* stream);
+
+/* out = sum_k y[idx[k]] in index order (reaction resultants). */
+int efea_gather_sum(int64_t n, const int32_t* idx, const double* y, double* out, void* stream);
+
+/* Phase field, damage problem: psi+ at the mass points with the History
+ * irreversibility (psi+ <- H_old where smaller; H_old may be NULL), then
+ * Kd_e (Ne, nPe, nPe), Fd_e (Ne, nPe).  split 0 Amor / 1 Miehe, regu 0 AT1 / 1 AT2.
+ * Replaces PhaseField.__Calc_psiPlus_e_pg and __Construct_Damage_Matrix
+ * (Simulations/_phasefield.py:494-573) with Models/_phasefield.py:236-571, 751-1128. */
+int efea_pf_damage_ke(int elem_type, int64_t Ne, const int32_t* connect, const double* coord,
+                      const double* tables_mass, int nPg_mass, const double* tables_rigi, int nPg_rigi,
+                      const double* u, const double* H_old, const double* E, int E_stride, double nu,
+                      int plane_stress, const double* Gc, int Gc_stride, double l0, int split, int regu,
+                      double thickness, double* psiP, double* Kd, double* Fd, void* stream);
+
+/* Phase field, displacement problem: K_e with c = g(d) c+ + c- at the rigi points.
+ * Replaces PhaseField.__Construct_Elastic_Matrix (Simulations/_phasefield.py:444-482). */
+int efea_pf_elastic_ke(int elem_type, int64_t Ne, const int32_t* connect, const double* coord,
+                       const double* tables_rigi, int nPg_rigi, const double* u, const double* d,
+                       const double* E, int E_stride, double nu, int plane_stress, int split, double thickness,
+                       double* Ke, void* stream);
+
+/* Pointwise 2D split: eps (n, 3) -> c+, c- (n, 3, 3), psi+, psi- (n).
+ * Replaces PhaseField.Calc_C / Calc_psi_e_pg (Models/_phasefield.py:335-431). */
+int efea_pf_split_points(int64_t n, int split, const double* eps, const double* E, int E_stride, double nu,
+                         int plane_stress, double* cP, double* cM, double* psiP, double* psiM, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EFEA_H */

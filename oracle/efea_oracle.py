@@ -1,0 +1,954 @@
+"""CPU oracle: an FP64 numpy/scipy restatement of EasyFEA's solve path.
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and
+the CPU-baseline leg of ``bench.py`` may import this module, and only as the
+checker or the timed CPU baseline.  The product package never imports it.
+
+Every function restates the reference's algorithm from its source text
+(citations are to files of the EasyFEA v3.5.1 tree); inputs are plain arrays
+(``connect``, ``coord``) so that both this oracle and the device path consume
+the same mesh.  Pinning (see ``tests/test_oracle_*.py``): closed forms,
+invariants and the reference's own known-answer tests.  Parity unpinned:
+the 3D Miehe spectral split at exactly repeated eigenvalues (no 3D
+phase-field workload exercises it), and the staggered-loop caching order,
+which no closed form fixes (pinned only by the cross-path elastic-limit
+invariant).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as sla
+
+SQRT2 = np.sqrt(2.0)
+
+# ---------------------------------------------------------------------------
+# a1-a2: quadrature and shape functions
+# ---------------------------------------------------------------------------
+
+
+def gauss(elemType: str, matrixType: str):
+    """Points/weights, ``EasyFEA/FEM/_gauss.py:364-516`` (SEG2 :373-382,
+    TRI3 :423-430 with ``_Triangle`` :55-70, QUAD4 :449-451 with
+    ``_Quadrangle`` :111-121, TETRA4/10 :467-476 with ``_Tetrahedron``
+    :142-163, HEXA8 :478-480 with ``_Hexahedron`` :201-213)."""
+    rigi = matrixType == "rigi"
+    if elemType == "SEG2":
+        x, w = np.polynomial.legendre.leggauss(1 if rigi else 2)
+        return x.reshape(-1, 1), w
+    if elemType == "TRI3":
+        if rigi:
+            return np.array([[1 / 3, 1 / 3]]), np.array([1 / 2])
+        ksis = [1 / 6, 2 / 3, 1 / 6]
+        etas = [1 / 6, 1 / 6, 2 / 3]
+        return np.array([ksis, etas]).T, np.array([1 / 6] * 3)
+    if elemType == "QUAD4":
+        a = 1 / np.sqrt(3)
+        return np.array([[-a, a, a, -a], [-a, -a, a, a]]).T, np.ones(4)
+    if elemType in ("TETRA4", "TETRA10"):
+        if elemType == "TETRA4" and rigi:
+            return np.array([[1 / 4, 1 / 4, 1 / 4]]), np.array([1 / 6])
+        a = (5 - np.sqrt(5)) / 20
+        b = (5 + 3 * np.sqrt(5)) / 20
+        return np.array([[a, a, a, b], [a, a, b, a], [a, b, a, a]]).T, np.full(4, 1 / 24)
+    if elemType == "HEXA8":
+        a = 1 / np.sqrt(3)
+        x = [-a, -a, -a, -a, a, a, a, a]
+        y = [-a, -a, a, a, -a, -a, a, a]
+        z = [-a, a, -a, a, -a, a, -a, a]
+        return np.array([x, y, z]).T, np.ones(8)
+    raise NotImplementedError(elemType)
+
+
+def shape_functions(elemType: str, pts: np.ndarray):
+    """``(N (nPg, nPe), dN (nPg, dim, nPe))`` from the element files
+    ``EasyFEA/FEM/Elems/_seg.py:46-61``, ``_tri.py:53-69``, ``_quad.py:56-75``,
+    ``_tetra.py:178-222`` (TETRA10), ``_hexa.py:86-147``."""
+    Ns, dNs = [], []
+    for p in pts:
+        if elemType == "SEG2":
+            (r,) = p
+            N = [-(r - 1) / 2, (r + 1) / 2]
+            dN = [[-1 / 2, 1 / 2]]
+        elif elemType == "TRI3":
+            r, s = p
+            N = [-r - s + 1, r, s]
+            dN = [[-1, 1, 0], [-1, 0, 1]]
+        elif elemType == "QUAD4":
+            r, s = p
+            N = [(r - 1) * (s - 1) / 4, -(r + 1) * (s - 1) / 4, (r + 1) * (s + 1) / 4, -(r - 1) * (s + 1) / 4]
+            dN = [
+                [s / 4 - 1 / 4, 1 / 4 - s / 4, s / 4 + 1 / 4, -s / 4 - 1 / 4],
+                [r / 4 - 1 / 4, -r / 4 - 1 / 4, r / 4 + 1 / 4, 1 / 4 - r / 4],
+            ]
+        elif elemType == "TETRA4":
+            r, s, t = p
+            N = [-r - s - t + 1, r, s, t]
+            dN = [[-1, 1, 0, 0], [-1, 0, 1, 0], [-1, 0, 0, 1]]
+        elif elemType == "TETRA10":
+            r, s, t = p
+            N = [
+                (r + s + t - 1) * (2 * r + 2 * s + 2 * t - 1), r * (2 * r - 1), s * (2 * s - 1),
+                t * (2 * t - 1), -4 * r * (r + s + t - 1), 4 * r * s, -4 * s * (r + s + t - 1),
+                -4 * t * (r + s + t - 1), 4 * s * t, 4 * r * t,
+            ]
+            c = 4 * r + 4 * s + 4 * t - 3
+            dN = np.array([
+                [c, c, c], [4 * r - 1, 0, 0], [0, 4 * s - 1, 0], [0, 0, 4 * t - 1],
+                [-8 * r - 4 * s - 4 * t + 4, -4 * r, -4 * r], [4 * s, 4 * r, 0],
+                [-4 * s, -4 * r - 8 * s - 4 * t + 4, -4 * s], [-4 * t, -4 * t, -4 * r - 4 * s - 8 * t + 4],
+                [0, 4 * t, 4 * s], [4 * t, 0, 4 * r],
+            ]).T
+        elif elemType == "HEXA8":
+            r, s, t = p
+            N = [
+                -(r - 1) * (s - 1) * (t - 1) / 8, (r + 1) * (s - 1) * (t - 1) / 8,
+                -(r + 1) * (s + 1) * (t - 1) / 8, (r - 1) * (s + 1) * (t - 1) / 8,
+                (r - 1) * (s - 1) * (t + 1) / 8, -(r + 1) * (s - 1) * (t + 1) / 8,
+                (r + 1) * (s + 1) * (t + 1) / 8, -(r - 1) * (s + 1) * (t + 1) / 8,
+            ]
+            dN = np.array([
+                [-(s - 1) * (t - 1) / 8, -(r - 1) * (t - 1) / 8, -(r - 1) * (s - 1) / 8],
+                [(s - 1) * (t - 1) / 8, (r + 1) * (t - 1) / 8, (r + 1) * (s - 1) / 8],
+                [-(s + 1) * (t - 1) / 8, -(r + 1) * (t - 1) / 8, -(r + 1) * (s + 1) / 8],
+                [(s + 1) * (t - 1) / 8, (r - 1) * (t - 1) / 8, (r - 1) * (s + 1) / 8],
+                [(s - 1) * (t + 1) / 8, (r - 1) * (t + 1) / 8, (r - 1) * (s - 1) / 8],
+                [-(s - 1) * (t + 1) / 8, -(r + 1) * (t + 1) / 8, -(r + 1) * (s - 1) / 8],
+                [(s + 1) * (t + 1) / 8, (r + 1) * (t + 1) / 8, (r + 1) * (s + 1) / 8],
+                [-(s + 1) * (t + 1) / 8, -(r - 1) * (t + 1) / 8, -(r - 1) * (s + 1) / 8],
+            ]).T
+        else:
+            raise NotImplementedError(elemType)
+        Ns.append(np.asarray(N, dtype=float))
+        dNs.append(np.asarray(dN, dtype=float))
+    return np.array(Ns), np.array(dNs)
+
+
+ELEM_DIM = {"SEG2": 1, "TRI3": 2, "QUAD4": 2, "TETRA4": 3, "TETRA10": 3, "HEXA8": 3}
+
+# ---------------------------------------------------------------------------
+# a3-a4: Jacobian, dN_x, B (Kelvin-Mandel)
+# ---------------------------------------------------------------------------
+
+
+def _det(F):
+    """``EasyFEA/FEM/_linalg.py:533-573``."""
+    d = F.shape[-1]
+    if d == 1:
+        return F[..., 0, 0]
+    if d == 2:
+        return F[..., 0, 0] * F[..., 1, 1] - F[..., 1, 0] * F[..., 0, 1]
+    a11, a12, a13 = F[..., 0, 0], F[..., 0, 1], F[..., 0, 2]
+    a21, a22, a23 = F[..., 1, 0], F[..., 1, 1], F[..., 1, 2]
+    a31, a32, a33 = F[..., 2, 0], F[..., 2, 1], F[..., 2, 2]
+    return (a11 * (a22 * a33 - a32 * a23) - a12 * (a21 * a33 - a31 * a23)
+            + a13 * (a21 * a32 - a31 * a22))
+
+
+def _inv(F):
+    """Adjugate inverse, ``EasyFEA/FEM/_linalg.py:576-656``."""
+    d = F.shape[-1]
+    det = _det(F)
+    if d == 1:
+        return 1 / F
+    adj = np.zeros_like(F)
+    if d == 2:
+        adj[..., 0, 0] = F[..., 1, 1]
+        adj[..., 0, 1] = -F[..., 0, 1]
+        adj[..., 1, 0] = -F[..., 1, 0]
+        adj[..., 1, 1] = F[..., 0, 0]
+    else:
+        T = np.swapaxes(F, -1, -2)
+        a = [[T[..., i, j] for j in range(3)] for i in range(3)]
+        adj[..., 0, 0] = a[1][1] * a[2][2] - a[2][1] * a[1][2]
+        adj[..., 0, 1] = -(a[1][0] * a[2][2] - a[2][0] * a[1][2])
+        adj[..., 0, 2] = a[1][0] * a[2][1] - a[2][0] * a[1][1]
+        adj[..., 1, 0] = -(a[0][1] * a[2][2] - a[2][1] * a[0][2])
+        adj[..., 1, 1] = a[0][0] * a[2][2] - a[2][0] * a[0][2]
+        adj[..., 1, 2] = -(a[0][0] * a[2][1] - a[2][0] * a[0][1])
+        adj[..., 2, 0] = a[0][1] * a[1][2] - a[1][1] * a[0][2]
+        adj[..., 2, 1] = -(a[0][0] * a[1][2] - a[1][0] * a[0][2])
+        adj[..., 2, 2] = a[0][0] * a[1][1] - a[1][0] * a[0][1]
+    return np.einsum("...,...ij->...ij", 1 / det, adj)
+
+
+def _local_frame_coords(coord_e: np.ndarray, dim: int) -> np.ndarray:
+    """Coordinates of lower-dimensional elements in their own plane/line.
+
+    ``_group_elem.py:841-865`` rebases a boundary element into an orthonormal
+    frame before forming the Jacobian; for straight SEG2 and planar QUAD4
+    faces this preserves lengths/areas, which is what the load integral uses.
+    """
+    if dim == 1:
+        t = coord_e[:, 1] - coord_e[:, 0]
+        t = t / np.linalg.norm(t, axis=1, keepdims=True)
+        return np.einsum("end,ed->en", coord_e, t)[..., None]
+    # dim == 2 in 3D: i along edge 0-1, k normal, j = k x i
+    i = coord_e[:, 1] - coord_e[:, 0]
+    i /= np.linalg.norm(i, axis=1, keepdims=True)
+    k = np.cross(i, coord_e[:, -1] - coord_e[:, 0])
+    k /= np.linalg.norm(k, axis=1, keepdims=True)
+    j = np.cross(k, i)
+    return np.stack([np.einsum("end,ed->en", coord_e, i), np.einsum("end,ed->en", coord_e, j)], -1)
+
+
+class Geometry:
+    """Per-group geometric blocks for one integration rule.
+
+    ``F`` (``Get_F_e_pg`` :832-869), ``detJ`` (``Get_jacobian_e_pg``
+    :871-888), ``wJ`` (:890-900), ``invF`` (:902-915), ``dN_x``
+    (``Get_dN_e_pg`` :1083-1105), ``B`` (``Get_B_e_pg`` :1241-1312) of
+    ``EasyFEA/FEM/_group_elem.py``.
+    """
+
+    def __init__(self, elemType: str, connect: np.ndarray, coord: np.ndarray, matrixType: str, inDim: int):
+        self.elemType = elemType
+        self.dim = dim = ELEM_DIM[elemType]
+        pts, w = gauss(elemType, matrixType)
+        self.N_pg, self.dN_pg = shape_functions(elemType, pts)
+        self.weights = w
+        coord_e = coord[connect][..., :3]
+        if dim == inDim:
+            X = coord_e[..., :dim]
+        else:
+            X = _local_frame_coords(coord_e, dim)
+        self.F = np.einsum("pdn,enk->epdk", self.dN_pg, X)
+        self.detJ = np.abs(_det(self.F))
+        self.wJ = self.detJ * w
+        self.invF = _inv(self.F)
+        self.dN_x = np.einsum("epij,pjn->epin", self.invF, self.dN_pg)
+        self.Ne, self.nPg = self.wJ.shape
+        self.nPe = connect.shape[1]
+
+    def B(self) -> np.ndarray:
+        dN = self.dN_x
+        Ne, nPg, dim, nPe = dN.shape
+        c = 1 / np.sqrt(2)
+        if dim == 2:
+            B = np.zeros((Ne, nPg, 3, nPe * 2))
+            B[:, :, 0, 0::2] = dN[:, :, 0]
+            B[:, :, 1, 1::2] = dN[:, :, 1]
+            B[:, :, 2, 0::2] = dN[:, :, 1] * c
+            B[:, :, 2, 1::2] = dN[:, :, 0] * c
+        else:
+            B = np.zeros((Ne, nPg, 6, nPe * 3))
+            dx, dy, dz = dN[:, :, 0], dN[:, :, 1], dN[:, :, 2]
+            B[:, :, 0, 0::3] = dx
+            B[:, :, 1, 1::3] = dy
+            B[:, :, 2, 2::3] = dz
+            B[:, :, 3, 1::3] = dz * c
+            B[:, :, 3, 2::3] = dy * c
+            B[:, :, 4, 0::3] = dz * c
+            B[:, :, 4, 2::3] = dx * c
+            B[:, :, 5, 0::3] = dy * c
+            B[:, :, 5, 1::3] = dx * c
+        return B
+
+
+# ---------------------------------------------------------------------------
+# a5: isotropic Hooke tensor in Kelvin-Mandel notation
+# ---------------------------------------------------------------------------
+
+
+def lame(E, v, dim: int, planeStress: bool):
+    """``get_lambda``/``get_mu``/``get_bulk``, ``Models/Elastic/_laws.py:376-405``."""
+    E = np.asarray(E, dtype=float)
+    lmbda = E * v / ((1 + v) * (1 - 2 * v))
+    if dim == 2 and planeStress:
+        lmbda = E * v / (1 - v**2)
+    mu = E / (2 * (1 + v))
+    bulk = lmbda + 2 * mu / dim
+    return lmbda, mu, bulk
+
+
+def C_isotropic(E, v, dim: int, planeStress: bool) -> np.ndarray:
+    """``Isotropic._Behavior`` (``_laws.py:407-482``) followed by
+    ``KelvinMandel_Matrix`` (``Models/_utils.py:111-152``).  ``E`` scalar or
+    ``(Ne,)`` -> ``(3,3)``/``(6,6)`` or ``(Ne, 3, 3)``/``(Ne, 6, 6)``."""
+    lmbda, mu, _ = lame(E, v, dim, planeStress)
+    shape = np.shape(lmbda)
+    if dim == 2:
+        C = np.zeros(shape + (3, 3))
+        C[..., 0, 0] = C[..., 1, 1] = lmbda + 2 * mu
+        C[..., 0, 1] = C[..., 1, 0] = lmbda
+        C[..., 2, 2] = mu
+        T = np.array([[1, 1, SQRT2], [1, 1, SQRT2], [SQRT2, SQRT2, 2]])
+    else:
+        C = np.zeros(shape + (6, 6))
+        for i in range(3):
+            for j in range(3):
+                C[..., i, j] = lmbda
+            C[..., i, i] = lmbda + 2 * mu
+            C[..., 3 + i, 3 + i] = mu
+        T = np.ones((6, 6))
+        T[:3, 3:] = T[3:, :3] = SQRT2
+        T[3:, 3:] = 2
+    return C * T
+
+
+# ---------------------------------------------------------------------------
+# a6-a9: element operators
+# ---------------------------------------------------------------------------
+
+
+def _bcast(coef, Ne, nPg):
+    """``FeArray.broadcast`` (``FEM/_linalg.py:426-476``) for scalar,
+    ``(Ne,)`` and ``(Ne, nPg)`` coefficients."""
+    coef = np.asarray(coef, dtype=float)
+    if coef.ndim == 0:
+        return np.full((Ne, nPg), float(coef))
+    if coef.shape == (Ne, nPg):
+        return coef
+    if coef.shape == (Ne,):
+        return np.broadcast_to(coef[:, None], (Ne, nPg))
+    if coef.shape == (nPg,):
+        return np.broadcast_to(coef[None, :], (Ne, nPg))
+    raise ValueError(coef.shape)
+
+
+def K_elastic(geo: Geometry, C: np.ndarray) -> np.ndarray:
+    """``Bilinear.LinearizedElasticity`` (``FEM/Operators/Bilinear.py:62-79``):
+    ``einsum('epij,epjk->eik', (wJ*B^T) @ C, B)``.  ``C`` may be
+    ``(ns,ns)``, ``(Ne,ns,ns)`` or ``(Ne,nPg,ns,ns)``."""
+    B = geo.B()
+    left = geo.wJ[..., None, None] * np.swapaxes(B, -1, -2)
+    C = np.asarray(C, dtype=float)
+    if C.ndim == 3:
+        C = C[:, None]
+    return np.einsum("epij,epjk->eik", left @ C, B, optimize=True)
+
+
+def UV(geo: Geometry, coef=1.0, dof_n: int = 1) -> np.ndarray:
+    """``Bilinear.UV`` (:42-59) with ``Get_ReactionPart_e_pg``
+    (``_group_elem.py:1337-1360``): ``sum_p coef*wJ*N^T N``."""
+    N = geo.N_pg
+    nPg, nPe = N.shape
+    if dof_n > 1:
+        Nr = np.zeros((nPg, dof_n, nPe * dof_n))
+        for r in range(dof_n):
+            Nr[:, r, r::dof_n] = N
+    else:
+        Nr = N[:, None, :]
+    part = geo.wJ[..., None, None] * np.einsum("pki,pkj->pij", Nr, Nr)[None]
+    c = _bcast(coef, geo.Ne, geo.nPg)
+    return (c[..., None, None] * part).sum(axis=1)
+
+
+def GradU_A_GradV(geo: Geometry, A: np.ndarray, coef=1.0) -> np.ndarray:
+    """``Bilinear.GradU_A_GradV`` (:229-249) with ``Get_DiffusePart_e_pg``
+    (``_group_elem.py:1362-1380``)."""
+    dN = geo.dN_x
+    part = geo.wJ[..., None, None] * np.swapaxes(dN, -1, -2)
+    c = _bcast(coef, geo.Ne, geo.nPg)
+    return np.einsum("epij,epjk->eik", c[..., None, None] * part, np.asarray(A) @ dN, optimize=True)
+
+
+def V(geo: Geometry, f=1.0) -> np.ndarray:
+    """``Linear.V`` (``FEM/Operators/Linear.py:18-35``) with
+    ``Get_SourcePart_e_pg`` (``_group_elem.py:1382-1407``), ``dof_n = 1``."""
+    part = geo.wJ[..., None] * geo.N_pg[None]
+    c = _bcast(f, geo.Ne, geo.nPg)
+    return (c[..., None] * part).sum(axis=1)
+
+
+# ---------------------------------------------------------------------------
+# a10-a12: DOF maps and the CSR map / bincount scatter
+# ---------------------------------------------------------------------------
+
+
+def assembly_e(connect: np.ndarray, dof_n: int) -> np.ndarray:
+    """``_GroupElem._Get_assembly_e`` (``_group_elem.py:358-380``)."""
+    Ne, nPe = connect.shape
+    out = np.zeros((Ne, nPe * dof_n), dtype=np.int64)
+    for d in range(dof_n):
+        out[:, np.arange(d, nPe * dof_n, dof_n)] = np.asarray(connect) * dof_n + d
+    return out
+
+
+def rows_cols_e(connect: np.ndarray, dof_n: int):
+    """``Get_rows_e``/``Get_columns_e`` (``_group_elem.py:382-402``)."""
+    a = assembly_e(connect, dof_n)
+    Ne, nd = a.shape
+    rows = np.repeat(a, nd).reshape(Ne, nd * nd)
+    cols = np.repeat(a, nd, axis=0).reshape(Ne, nd * nd)
+    return rows, cols
+
+
+def csr_map(connect: np.ndarray, dof_n: int, Ndof: int, isMatrix=True):
+    """``_Simu.__Get_csr_map`` (``Simulations/_simu.py:1062-1102``): scipy's
+    canonical pattern and ``inv[k]`` = slot of element entry ``k``."""
+    shape = (Ndof, Ndof) if isMatrix else (Ndof, 1)
+    if isMatrix:
+        rows, cols = rows_cols_e(connect, dof_n)
+        rows, cols = rows.ravel(), cols.ravel()
+    else:
+        rows = assembly_e(connect, dof_n).ravel()
+        cols = np.zeros_like(rows)
+    m = sp.csr_matrix((np.ones(rows.size), (rows, cols)), shape=shape)
+    m.sort_indices()
+    ncol = Ndof if isMatrix else 1
+    canon = np.repeat(np.arange(shape[0]), np.diff(m.indptr)) * ncol + m.indices
+    inv = np.searchsorted(canon, rows.astype(np.int64) * ncol + cols).astype(np.int32)
+    return inv, m.indices, m.indptr, m.nnz
+
+
+def assemble_csr(values_e: np.ndarray, cmap, Ndof: int, isMatrix=True) -> sp.csr_matrix:
+    """``_Simu.__Assemble_csr`` (``_simu.py:989-1060``): one ``bincount`` in
+    element-entry order, canonical CSR out."""
+    inv, indices, indptr, nnz = cmap
+    shape = (Ndof, Ndof) if isMatrix else (Ndof, 1)
+    data = np.bincount(inv, weights=np.asarray(values_e).ravel(), minlength=nnz)
+    m = sp.csr_matrix((data, indices, indptr), shape=shape)
+    m.has_canonical_format = True
+    return m
+
+
+# ---------------------------------------------------------------------------
+# a13-a16: boundary conditions and the r1 solve
+# ---------------------------------------------------------------------------
+
+
+def elements_using_nodes(connect: np.ndarray, nodes: np.ndarray) -> np.ndarray:
+    """Elements whose every node lies in ``nodes``
+    (``Get_Elements_Nodes(exclusively=True)``, ``_group_elem.py:1500-1537``)."""
+    mark = np.zeros(max(connect.max(), np.max(nodes)) + 1, dtype=bool)
+    mark[nodes] = True
+    return np.nonzero(mark[connect].all(axis=1))[0]
+
+
+def dofs_nodes(nodes, unknowns, available) -> np.ndarray:
+    """``BoundaryCondition.Get_dofs_nodes`` (``FEM/_boundary_conditions.py:160-201``)."""
+    nodes = np.asarray(nodes, dtype=int).ravel()
+    dim = len(available)
+    out = np.zeros((nodes.size, len(unknowns)), dtype=int)
+    for d, u in enumerate(unknowns):
+        out[:, d] = nodes * dim + available.index(u)
+    return out.ravel()
+
+
+def integrate_load(elemType, connect_b, coord, inDim, nodes, values, unknowns, available):
+    """``_Simu.__Bc_Integration_Dim`` (``_simu.py:2770-2842``) for constant or
+    callable values: ``sum_p wJ * f * N`` per element node, mass points."""
+    elements = elements_using_nodes(connect_b, nodes)
+    if elements.size == 0:
+        return np.array([]), np.array([], dtype=int)
+    connect = connect_b[elements]
+    geo = Geometry(elemType, connect, coord, "mass", inDim)
+    coord_e_p = np.einsum("pn,end->epd", geo.N_pg, coord[connect])
+    vals = np.zeros((connect.shape[0] * connect.shape[1], len(unknowns)))
+    dofs = np.zeros_like(vals, dtype=int)
+    for u, unk in enumerate(unknowns):
+        val = values[u]
+        if callable(val):
+            ev = val(coord_e_p[..., 0], coord_e_p[..., 1], coord_e_p[..., 2]) * np.ones(geo.wJ.shape)
+        else:
+            ev = np.full(geo.wJ.shape, float(val))
+        v_e_p = np.einsum("ep,ep,pn->epn", geo.wJ, ev, geo.N_pg)
+        vals[:, u] = v_e_p.sum(axis=1).ravel()
+        dofs[:, u] = dofs_nodes(connect.ravel(), [unk], available)
+    return vals.ravel(), dofs.ravel()
+
+
+def known_unknown(dirichlet_dofs: np.ndarray, Ndof: int):
+    """``Bc_dofs_known_unknown`` (``_simu.py:2295-2316``)."""
+    mask = np.ones(Ndof, dtype=bool)
+    mask[np.asarray(dirichlet_dofs, dtype=int)] = False
+    return np.nonzero(~mask)[0], np.nonzero(mask)[0]
+
+
+def solve_r1(K: sp.csr_matrix, b: np.ndarray, dir_dofs, dir_vals, Ndof, orphan_dofs=()):
+    """``__Solver_1`` (``Simulations/Solvers.py:502-553``) with
+    ``__Solver_Get_Dirichlet_A_x`` (``_simu.py:1916-1984``): reduce to the
+    unknown DOFs and solve with ``scipy.sparse.linalg.spsolve``
+    (``Solvers.py:361-362``)."""
+    A = K
+    if len(orphan_dofs):
+        diag = np.zeros(Ndof)
+        diag[np.asarray(orphan_dofs)] = 1.0
+        A = A + sp.diags(diag, format="csr")
+    dir_dofs = np.asarray(dir_dofs, dtype=int)
+    x = sp.csr_matrix((np.asarray(dir_vals, float), (dir_dofs, np.zeros_like(dir_dofs))), shape=(Ndof, 1))
+    known, unknown = known_unknown(dir_dofs, Ndof)
+    Ai = A[unknown, :].tocsc()
+    Aii = Ai[:, unknown]
+    Aic = Ai[:, known]
+    bs = sp.csr_matrix(np.asarray(b, float).reshape(-1, 1))
+    bi = bs[unknown, 0]
+    xc = x[known, 0]
+    bi = bi - Aic @ xc
+    xi = sla.spsolve(Aii.tocsr(), bi)
+    xo = x.toarray().reshape(Ndof)
+    xo[unknown] = xi
+    return xo
+
+
+# ---------------------------------------------------------------------------
+# a19-a23: phase-field model
+# ---------------------------------------------------------------------------
+
+
+def IxI(dim):
+    """``PhaseField.__Build_IxI`` (``Models/_phasefield.py:218-230``)."""
+    I = np.array([1.0, 1.0, 0.0]) if dim == 2 else np.array([1.0, 1, 1, 0, 0, 0])
+    return np.outer(I, I)
+
+
+def Rp_Rm(eps: np.ndarray, dim: int):
+    """``__Rp_Rm`` (:485-501)."""
+    tr = eps[..., 0] + eps[..., 1]
+    if dim == 3:
+        tr = tr + eps[..., 2]
+    return (1 + np.sign(tr)) / 2, (1 + np.sign(-tr)) / 2
+
+
+def split_amor(eps, E, v, dim, planeStress):
+    """``__Split_Amor`` (:451-483).  ``E`` scalar or broadcastable to the
+    leading ``(Ne, nPg)`` axes of ``eps``."""
+    _, mu, bulk = lame(E, v, dim, planeStress)
+    Rp, Rm = Rp_Rm(eps, dim)
+    II = IxI(dim)
+    n = II.shape[0]
+    mu = np.asarray(mu)[..., None, None] if np.ndim(mu) else mu
+    bulk = np.asarray(bulk)[..., None, None] if np.ndim(bulk) else bulk
+    cP = bulk * (Rp[..., None, None] * II) + 2 * mu * (np.eye(n) - 1 / dim * II)
+    cM = bulk * (Rm[..., None, None] * II)
+    return cP, cM
+
+
+def vec_to_mat(vec):
+    """``Project_vector_to_matrix`` (``Models/_utils.py:155-188``)."""
+    n = vec.shape[-1]
+    dim = 2 if n == 3 else 3
+    M = np.zeros(vec.shape[:-1] + (dim, dim))
+    for d in range(dim):
+        M[..., d, d] = vec[..., d]
+    if dim == 2:
+        M[..., 0, 1] = M[..., 1, 0] = vec[..., 2] / SQRT2
+    else:
+        M[..., 1, 2] = M[..., 2, 1] = vec[..., 3] / SQRT2
+        M[..., 0, 2] = M[..., 2, 0] = vec[..., 4] / SQRT2
+        M[..., 0, 1] = M[..., 1, 0] = vec[..., 5] / SQRT2
+    return M
+
+
+def mat_to_vec(M):
+    """``Project_matrix_to_vector`` (``Models/_utils.py:191-222``)."""
+    dim = M.shape[-1]
+    if dim == 2:
+        return np.stack([M[..., 0, 0], M[..., 1, 1], M[..., 0, 1] * SQRT2], -1)
+    return np.stack([M[..., 0, 0], M[..., 1, 1], M[..., 2, 2], M[..., 1, 2] * SQRT2,
+                     M[..., 0, 2] * SQRT2, M[..., 0, 1] * SQRT2], -1)
+
+
+def eigen_2d(vec):
+    """2D branch of ``_Eigen_values_vectors_projectors`` (:751-802)."""
+    M = vec_to_mat(vec)
+    det = M[..., 0, 0] * M[..., 1, 1] - M[..., 1, 0] * M[..., 0, 1]
+    tr = M[..., 0, 0] + M[..., 1, 1]
+    delta = tr**2 - 4 * det
+    with np.errstate(invalid="ignore"):
+        s = np.sqrt(delta)
+    eigs = np.stack([(tr - s) / 2, (tr + s) / 2], -1)
+    v1mv2 = eigs[..., 0] - eigs[..., 1]
+    neq = eigs[..., 0] != eigs[..., 1]
+    M1 = np.zeros_like(M)
+    M1[..., 0, 0] = 1
+    if neq.any():
+        d = np.where(v1mv2 == 0, 1.0, v1mv2)
+        I = np.eye(2)
+        m1 = (M - eigs[..., 1, None, None] * I) / d[..., None, None]
+        M1[neq] = m1[neq]
+    M2 = np.eye(2) - M1
+    return eigs, [mat_to_vec(M1), mat_to_vec(M2)], [M1, M2]
+
+
+def eigen_3d(vec):
+    """3D branch of ``_Eigen_values_vectors_projectors`` (:804-936): He's
+    closed form with the reference's element-level case masks (a case-2/3
+    hit at any Gauss point applies that case to the element's points)."""
+    M = vec_to_mat(vec)
+    Ne, nPg = M.shape[:2]
+    I = np.broadcast_to(np.eye(3), M.shape)
+    I1 = np.trace(M, axis1=-2, axis2=-1)
+    I2 = 0.5 * (I1**2 - np.trace(M @ M, axis1=-2, axis2=-1))
+    I3 = _det(M)
+    g = I1**2 - 3 * I2
+    sg = np.sqrt(g)
+    gn0 = g != 0
+    arg = 0.5 * (2 * I1**3 - 9 * I1 * I2 + 27 * I3)
+    np.divide(arg, g ** (3 / 2), out=arg, where=gn0)
+    theta = np.arccos(arg) / 3
+    v1, v2, v3 = I1 / 3, I1 / 3, I1 / 3
+    v1, v2, v3 = v1.copy(), v2.copy(), v3.copy()
+    M1 = np.zeros_like(M)
+    M1[..., 0, 0] = 1
+    M3 = np.zeros_like(M)
+    M3[..., 2, 2] = 1
+    Irg = (1 / 3) * ((I1 - sg)[..., None, None] * I)
+    case2 = np.unique(np.nonzero(gn0 & (theta == np.pi / 3))[0])
+    if case2.size:
+        v1[case2] += -2 / 3 * sg[case2]
+        v2[case2] += 1 / 3 * sg[case2]
+        v3[case2] += 1 / 3 * sg[case2]
+        M1[case2] = ((g ** (-1 / 2))[..., None, None] * (Irg - M))[case2]
+        M3[case2] = 0.5 * (I - M1)[case2]
+    case3 = np.unique(np.nonzero(gn0 & (theta == 0))[0])
+    if case3.size:
+        v1[case3] += -1 / 3 * sg[case3]
+        v2[case3] += -1 / 3 * sg[case3]
+        v3[case3] += 2 / 3 * sg[case3]
+        M3[case3] = ((g ** (-1 / 2))[..., None, None] * (M - Irg))[case3]
+        M1[case3] = 0.5 * (I - M3)[case3]
+    case1 = np.setdiff1d(np.unique(np.nonzero(gn0 & (theta != 0) & (theta != np.pi / 3))[0]),
+                         np.union1d(case2, case3))
+    if case1.size:
+        v1[case1] += 2 / 3 * (sg * np.cos(2 * np.pi / 3 + theta))[case1]
+        v2[case1] += 2 / 3 * (sg * np.cos(2 * np.pi / 3 - theta))[case1]
+        v3[case1] += 2 / 3 * (sg * np.cos(theta))[case1]
+        a, b, c, m = v1[case1][..., None, None], v2[case1][..., None, None], v3[case1][..., None, None], M[case1]
+        E3 = np.eye(3)
+        M1[case1] = (m - b * E3) @ (m - c * E3) / ((a - b) * (a - c))
+        M3[case1] = (m - a * E3) @ (m - b * E3) / ((c - a) * (c - b))
+    eigs = np.stack([v1, v2, v3], -1)
+    M1 = M1 / np.linalg.norm(M1, axis=(-2, -1))[..., None, None]
+    M3 = M3 / np.linalg.norm(M3, axis=(-2, -1))[..., None, None]
+    M2 = I - (M1 + M3)
+    return eigs, [mat_to_vec(M1), mat_to_vec(M2), mat_to_vec(M3)], [M1, M2, M3]
+
+
+def spectral_projectors(vec, dim):
+    """``__Spectral_Decomposition`` (:1043-1243): ``projP``, ``projM``."""
+    if dim == 2:
+        val, (m1, m2), _ = eigen_2d(vec)
+        valp = (val + np.abs(val)) / 2
+        dvalp = np.heaviside(val, 0.5)
+        v = val[..., 0] - val[..., 1]
+        v = np.where(v == 0, 1.0, v)
+        BetaP = (valp[..., 0] - valp[..., 1]) / v
+        gammap = dvalp - BetaP[..., None]
+        projP = (BetaP[..., None, None] * np.eye(3)
+                 + gammap[..., 0, None, None] * np.einsum("...i,...j->...ij", m1, m1)
+                 + gammap[..., 1, None, None] * np.einsum("...i,...j->...ij", m2, m2))
+        return projP, np.eye(3) - projP
+    val, ms, Ms = eigen_3d(vec)
+    valp = (val + np.abs(val)) / 2
+    dvalp = np.heaviside(val, 0.5)
+    d12 = np.where(val[..., 0] - val[..., 1] == 0, 1.0, val[..., 0] - val[..., 1])
+    d13 = np.where(val[..., 0] - val[..., 2] == 0, 1.0, val[..., 0] - val[..., 2])
+    d23 = np.where(val[..., 1] - val[..., 2] == 0, 1.0, val[..., 1] - val[..., 2])
+    th = np.stack([(valp[..., 0] - valp[..., 1]) / (2 * d12), (valp[..., 0] - valp[..., 2]) / (2 * d13),
+                   (valp[..., 1] - valp[..., 2]) / (2 * d23)], -1)
+    P = sum(dvalp[..., a, None, None] * np.einsum("...i,...j->...ij", ms[a], ms[a]) for a in range(3))
+    rI = np.array([0, 1, 2, 1, 0, 0])
+    rJ = np.array([0, 1, 2, 2, 2, 1])
+    scale = np.ones((6, 6))
+    scale[3:, :3] = scale[:3, 3:] = SQRT2
+    scale[3:, 3:] = 2.0
+    for t, (A, Bm) in enumerate([(Ms[0], Ms[1]), (Ms[0], Ms[2]), (Ms[1], Ms[2])]):
+        Aik = A[..., rI[:, None], rI[None, :]]
+        Ail = A[..., rI[:, None], rJ[None, :]]
+        Ajl = A[..., rJ[:, None], rJ[None, :]]
+        Bik = Bm[..., rI[:, None], rI[None, :]]
+        Bil = Bm[..., rI[:, None], rJ[None, :]]
+        Bjl = Bm[..., rJ[:, None], rJ[None, :]]
+        G = (Aik * Bjl + Ail * np.swapaxes(Bil, -2, -1) + Bik * Ajl + Bil * np.swapaxes(Ail, -2, -1)) * scale
+        P = P + th[..., t, None, None] * G
+    return P, np.eye(6) - P
+
+
+def split_miehe(eps, E, v, dim, planeStress):
+    """Miehe branch of ``__Split_Strain`` (:503-537)."""
+    projP, projM = spectral_projectors(eps, dim)
+    lmbda, mu, _ = lame(E, v, dim, planeStress)
+    Rp, Rm = Rp_Rm(eps, dim)
+    II = IxI(dim)
+    lm = np.asarray(lmbda)[..., None, None] if np.ndim(lmbda) else lmbda
+    mu = np.asarray(mu)[..., None, None] if np.ndim(mu) else mu
+    cP = lm * (Rp[..., None, None] * II) + 2 * mu * projP
+    cM = lm * (Rm[..., None, None] * II) + 2 * mu * projM
+    return cP, cM
+
+
+def calc_C(split, eps, E, v, dim, planeStress):
+    """``PhaseField.Calc_C`` (:396-431) for the Amor and Miehe splits."""
+    if split == "Amor":
+        return split_amor(eps, E, v, dim, planeStress)
+    if split == "Miehe":
+        return split_miehe(eps, E, v, dim, planeStress)
+    raise NotImplementedError(split)
+
+
+def calc_psi(split, eps, E, v, dim, planeStress):
+    """``Calc_psi_e_pg`` (:335-358) via ``Calc_Sigma_e_pg`` (:360-394)."""
+    cP, cM = calc_C(split, eps, E, v, dim, planeStress)
+    sP = (cP @ eps[..., None])[..., 0]
+    sM = (cM @ eps[..., None])[..., 0]
+    return np.sum(0.5 * eps * sP, -1), np.sum(0.5 * eps * sM, -1)
+
+
+def pf_k(Gc, l0, regu):
+    """``PhaseField.k`` (:236-251)."""
+    return 3 / 4 * Gc * l0 if regu == "AT1" else Gc * l0
+
+
+def pf_r(psiP, Gc, l0, regu):
+    """``Get_r_e_pg`` (:253-271); ``Gc`` scalar or ``(Ne,)``."""
+    Gc = np.asarray(Gc, float)
+    if Gc.ndim:
+        Gc = Gc[:, None]
+    return 2 * psiP if regu == "AT1" else 2 * psiP + Gc / l0
+
+
+def pf_f(psiP, Gc, l0, regu):
+    """``Get_f_e_pg`` (:273-293)."""
+    Gc = np.asarray(Gc, float)
+    if Gc.ndim:
+        Gc = Gc[:, None]
+    if regu == "AT1":
+        f = 2 * psiP - (3 * Gc) / (8 * l0)
+        return (f + np.abs(f)) / 2
+    return 2 * psiP
+
+
+def pf_g(d, connect, N_pg, k_res=1e-12):
+    """``Get_g_e_pg`` (:295-317)."""
+    d_pg = np.einsum("pn,en->ep", N_pg, d[connect])
+    return (1 - d_pg) ** 2 + k_res
+
+
+# ---------------------------------------------------------------------------
+# Simulations: the driver-level semantics
+# ---------------------------------------------------------------------------
+
+
+class _SimuBase:
+    """Boundary-condition bookkeeping shared by the oracle simulations
+    (``_simu.py:2190-2469, 2572-2630``)."""
+
+    def __init__(self, connect, coord, elemType, dim, boundary):
+        self.connect = np.asarray(connect, dtype=np.int64)
+        self.coord = np.asarray(coord, dtype=float)
+        self.elemType = elemType
+        self.dim = dim
+        self.Nn = self.coord.shape[0]
+        self.boundary = boundary  # {elemType: connect} of lower-dimensional groups
+        used = np.zeros(self.Nn, dtype=bool)
+        used[self.connect.ravel()] = True
+        self.orphan = np.nonzero(~used)[0]
+        self.Bc_Init()
+
+    def Bc_Init(self):
+        self.dir = {}
+        self.neu = {}
+
+    def _add(self, store, pt, dofs, vals):
+        d, v = store.get(pt, (np.array([], int), np.array([])))
+        store[pt] = (np.concatenate([d, dofs]), np.concatenate([v, vals]))
+
+    def add_dirichlet(self, nodes, values, unknowns, problemType="elastic"):
+        nodes = np.asarray(nodes)
+        avail = self.unknowns(problemType)
+        vals = np.zeros((nodes.size, len(unknowns)))
+        c = self.coord[nodes]
+        for d, val in enumerate(values):
+            vals[:, d] = val(c[:, 0], c[:, 1], c[:, 2]) if callable(val) else val
+        self._add(self.dir, problemType, dofs_nodes(nodes, unknowns, avail), vals.ravel())
+
+    def add_surfLoad(self, nodes, values, unknowns, problemType="elastic", thickness=1.0):
+        avail = self.unknowns(problemType)
+        inDim = 2 if self.dim == 2 else 3
+        bdim = self.dim - 1
+        for et, cb in self.boundary.items():
+            if ELEM_DIM[et] != bdim:
+                continue
+            vals, dofs = integrate_load(et, cb, self.coord, inDim, nodes, values, unknowns, avail)
+            if self.dim == 2:
+                vals = vals * thickness
+            self._add(self.neu, problemType, dofs, vals)
+
+    def unknowns(self, problemType):
+        return ["x", "y"] if self.dim == 2 else ["x", "y", "z"]
+
+    def neumann_vector(self, problemType, Ndof):
+        dofs, vals = self.neu.get(problemType, (np.array([], int), np.array([])))
+        b = sp.csr_matrix((vals, (dofs, np.zeros(len(dofs)))), shape=(Ndof, 1))
+        return b.toarray().ravel()
+
+
+class ElasticOracle(_SimuBase):
+    """``Simulations.Elastic`` static solve (``Simulations/_elastic.py:123-152``
+    and the r1 pipeline of section 3.1)."""
+
+    def __init__(self, connect, coord, elemType, E, v, planeStress=True, thickness=1.0, boundary=None):
+        dim = ELEM_DIM[elemType]
+        super().__init__(connect, coord, elemType, dim, boundary or {})
+        self.E, self.v, self.planeStress, self.thickness = E, v, planeStress, thickness
+        self.Ndof = self.Nn * dim
+        self._cmap = None
+        self.u = np.zeros(self.Ndof)
+
+    def element_K(self):
+        geo = Geometry(self.elemType, self.connect, self.coord, "rigi", self.dim)
+        C = C_isotropic(self.E, self.v, self.dim, self.planeStress)
+        Ke = K_elastic(geo, C)
+        if self.dim == 2:
+            Ke = Ke * self.thickness
+        return Ke
+
+    def Get_K(self):
+        if self._cmap is None:
+            self._cmap = csr_map(self.connect, self.dim, self.Ndof)
+        return assemble_csr(self.element_K(), self._cmap, self.Ndof)
+
+    def add_surfLoad(self, nodes, values, unknowns, problemType="elastic"):
+        super().add_surfLoad(nodes, values, unknowns, problemType, self.thickness)
+
+    def Solve(self):
+        K = self.Get_K()
+        b = self.neumann_vector("elastic", self.Ndof)
+        dofs, vals = self.dir.get("elastic", (np.array([], int), np.array([])))
+        orphan = dofs_nodes(self.orphan, self.unknowns("elastic"), self.unknowns("elastic"))
+        self.u = solve_r1(K, b, dofs, vals, self.Ndof, orphan)
+        return self.u
+
+
+class PhaseFieldOracle(_SimuBase):
+    """``Simulations.PhaseField`` staggered solve with the ``History``
+    irreversibility (``Simulations/_phasefield.py:248-657``), Amor or Miehe
+    split, AT1/AT2.  Reproduces the assembly caching order of the reference:
+    ``Ku`` is built from the pre-solve displacement and the new damage, the
+    history ``H`` advances only in ``Save_Iter``, and ``Calc_Reaction`` uses
+    the cached ``Ku``."""
+
+    def __init__(self, connect, coord, elemType, E, v, planeStress, thickness, split, regu, Gc, l0,
+                 boundary=None):
+        dim = ELEM_DIM[elemType]
+        super().__init__(connect, coord, elemType, dim, boundary or {})
+        self.E, self.v, self.planeStress, self.thickness = E, v, planeStress, thickness
+        self.split, self.regu, self.Gc, self.l0 = split, regu, Gc, l0
+        self.u = np.zeros(self.Nn * dim)
+        self.d = np.zeros(self.Nn)
+        self.geo_r = Geometry(elemType, self.connect, self.coord, "rigi", dim)
+        self.geo_m = Geometry(elemType, self.connect, self.coord, "mass", dim)
+        self.Bm = self.geo_m.B()
+        self.Br = self.geo_r.B()
+        self.cmap_u = csr_map(self.connect, dim, self.Nn * dim)
+        self.cmap_d = csr_map(self.connect, 1, self.Nn)
+        self.cmap_f = csr_map(self.connect, 1, self.Nn, isMatrix=False)
+        self.old_psiP = np.zeros((self.connect.shape[0], self.geo_m.nPg))
+        self.psiP = None
+        self.updU = False
+        self.updD = False
+        self.Ku = self.Kd = self.Fd = None
+        self.Niter = 0
+
+    def unknowns(self, problemType):
+        return ["d"] if problemType == "damage" else super().unknowns(problemType)
+
+    def _E_pg(self):
+        E = np.asarray(self.E, float)
+        return E[:, None] if E.ndim else E
+
+    def _eps(self, B):
+        ue = self.u[assembly_e(self.connect, self.dim)]
+        return np.einsum("epij,ej->epi", B, ue)
+
+    def calc_psiP(self):
+        """``__Calc_psiPlus_e_pg`` (:494-538)."""
+        eps = self._eps(self.Bm)
+        psiP, _ = calc_psi(self.split, eps, self._E_pg(), self.v, self.dim, self.planeStress)
+        inc = psiP - self.old_psiP
+        psiP = np.where(inc < 0, self.old_psiP, psiP)
+        self.psiP = psiP
+        return psiP
+
+    def assemble_damage(self):
+        """``__Construct_Damage_Matrix`` (:540-573)."""
+        psiP = self.calc_psiP()
+        r = pf_r(psiP, self.Gc, self.l0, self.regu)
+        Ke = UV(self.geo_m, r) + GradU_A_GradV(self.geo_r, np.eye(self.dim), pf_k(self.Gc, self.l0, self.regu))
+        Fe = V(self.geo_m, pf_f(psiP, self.Gc, self.l0, self.regu))
+        if self.dim == 2:
+            Ke = Ke * self.thickness
+            Fe = Fe * self.thickness
+        self.Kd = assemble_csr(Ke, self.cmap_d, self.Nn)
+        self.Fd = assemble_csr(Fe, self.cmap_f, self.Nn, isMatrix=False).toarray().ravel()
+        self.updD = True
+
+    def element_Ku(self):
+        """``__Construct_Elastic_Matrix`` (:444-482)."""
+        eps = self._eps(self.Br)
+        cP, cM = calc_C(self.split, eps, self._E_pg(), self.v, self.dim, self.planeStress)
+        g = pf_g(self.d, self.connect, self.geo_r.N_pg)
+        c = g[..., None, None] * cP + cM
+        Ke = K_elastic(self.geo_r, c)
+        if self.dim == 2:
+            Ke = Ke * self.thickness
+        return Ke
+
+    def assemble_elastic(self):
+        self.Ku = assemble_csr(self.element_Ku(), self.cmap_u, self.Nn * self.dim)
+        self.updU = True
+
+    def Get_K(self, problemType):
+        """``Get_K_C_M_F`` (:248-269) caching."""
+        if problemType == "elastic":
+            if not self.updU:
+                self.assemble_elastic()
+            return self.Ku
+        if not self.updD:
+            self.assemble_damage()
+        return self.Kd
+
+    def psi_crack(self):
+        """``_Calc_Psi_Crack`` (:798-816)."""
+        Kd = self.Get_K("damage")
+        if np.linalg.norm(self.d) == 0:
+            return 0.0
+        return 0.5 * self.d @ (Kd @ self.d)
+
+    def _solve(self, problemType):
+        if problemType == "damage":
+            K = self.Get_K("damage")
+            b = self.Fd.copy()
+            dofs, vals = self.dir.get("damage", (np.array([], int), np.array([])))
+            self.d = solve_r1(K, b, dofs, vals, self.Nn)
+            return self.d
+        K = self.Get_K("elastic")
+        b = self.neumann_vector("elastic", self.Nn * self.dim)
+        dofs, vals = self.dir.get("elastic", (np.array([], int), np.array([])))
+        self.u = solve_r1(K, b, dofs, vals, self.Nn * self.dim)
+        return self.u
+
+    def Solve(self, tolConv=1.0, maxIter=500, convOption=1):
+        """``PhaseField.Solve`` (:303-432) for convOption 0 and 1."""
+        assert convOption in (0, 1)
+        Niter = 0
+        converged = False
+        while not converged and Niter < maxIter:
+            Niter += 1
+            d_n = self.d.copy()
+            if convOption == 1:
+                E_n = self.psi_crack()
+            d_np1 = self._solve("damage")
+            self.updU = False
+            u_np1 = self._solve("elastic")
+            self.updD = False
+            if convOption == 0:
+                conv = np.max(np.abs(d_np1 - d_n))
+            else:
+                E_np1 = self.psi_crack()
+                conv = abs(E_n - E_np1) if E_np1 == 0 else abs((E_n - E_np1) / E_np1)
+            converged = True if (tolConv == 1 or d_np1.max() == 0) else conv <= tolConv
+        self.Niter = Niter
+        return u_np1, d_np1, converged
+
+    def Save_Iter(self):
+        """History update of ``Save_Iter`` (:613-630)."""
+        self.old_psiP = self.psiP
+
+    def Calc_Reaction(self, dofs):
+        """``_Simu.Calc_Reaction`` (``_simu.py:212-260``), elliptic."""
+        K = self.Get_K("elastic")
+        return K[dofs] @ self.u

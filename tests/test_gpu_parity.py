@@ -1,0 +1,259 @@
+"""Device path vs the CPU oracle (GPU tests).
+
+Tolerances: FP64 element matrices / CSR values within 1e-12 relative
+(north-star bar for K in FP64 mode); scatter bit-exact with np.bincount on
+identical entries; sparsity pattern bit-exact; solutions within 1e-8
+relative of the oracle's sparse direct solve in both solver modes; FP32
+element matrices within 1e-5.
+"""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+import easyfea_amd as ea  # noqa: E402
+from easyfea_amd import Models, Simulations, _lib  # noqa: E402
+from easyfea_amd.device import DeviceMesh, ptr, stream_ptr, to_dev  # noqa: E402
+from easyfea_amd.quadrature import gauss_points  # noqa: E402
+from easyfea_amd._utils import MatrixType  # noqa: E402
+from oracle import efea_oracle as O  # noqa: E402
+
+
+def _rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _distorted(mesh, amp, seed=0):
+    """Perturb interior node coordinates so that Jacobians vary."""
+    rng = np.random.default_rng(seed)
+    c = mesh.coord.copy()
+    span = c.max(0) - c.min(0)
+    interior = np.ones(mesh.Nn, dtype=bool)
+    for k in range(mesh.dim):
+        interior &= (c[:, k] > c[:, k].min()) & (c[:, k] < c[:, k].max())
+    h = span[: mesh.dim].min() / max(10, round(mesh.Ne ** (1 / mesh.dim)))
+    c[interior, : mesh.dim] += amp * h * (rng.random((interior.sum(), mesh.dim)) - 0.5)
+    return ea.Mesh({et: g.connect for et, g in mesh.dict_groupElem.items()}, c)
+
+
+def _elastic_ke(mesh, E, nu, ps, th, prec):
+    dev = DeviceMesh(mesh)
+    tab, nPg = dev.tables(MatrixType.rigi)
+    Earr = to_dev(np.atleast_1d(np.asarray(E, float)))
+    nd = dev.nPe * dev.dim
+    Ke = torch.empty(dev.Ne * nd * nd, dtype=torch.float64 if prec == 64 else torch.float32, device="cuda")
+    _lib.call("efea_elastic_ke", dev.code, prec, dev.Ne, ptr(dev.connect), ptr(dev.coord), ptr(tab), nPg, ptr(Earr),
+              0 if np.ndim(E) == 0 else 1, nu, int(ps), th, ptr(Ke), stream_ptr())
+    return Ke.double().cpu().numpy().reshape(dev.Ne, nd, nd)
+
+
+@pytest.mark.parametrize("maker,et", [
+    (lambda: ea.quad4_rectangle(3.0, 2.0, 7, 5), "QUAD4"),
+    (lambda: ea.tri3_rectangle(3.0, 2.0, 7, 5), "TRI3"),
+    (lambda: ea.hexa8_box(1.0, 2.0, 1.5, 4, 3, 5), "HEXA8"),
+])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_elastic_element_matrices(maker, et, prec):
+    mesh = _distorted(maker(), 0.4)
+    dim = mesh.dim
+    rng = np.random.default_rng(1)
+    E = 210000.0 * np.exp(0.2 * rng.standard_normal(mesh.Ne))
+    Ke = _elastic_ke(mesh, E, 0.3, True, 13.0, prec)
+    geo = O.Geometry(et, mesh.connect, mesh.coord, "rigi", dim)
+    ref = O.K_elastic(geo, O.C_isotropic(E, 0.3, dim, True))
+    if dim == 2:
+        ref = ref * 13.0
+    tol = 1e-12 if prec == 64 else 1e-5
+    err = np.max(np.linalg.norm(Ke - ref, axis=(1, 2)) / np.linalg.norm(ref, axis=(1, 2)))
+    assert err < tol, err
+
+
+def test_pattern_and_scatter_bit_exact():
+    mesh = _distorted(ea.quad4_rectangle(120.0, 13.0, 36, 9), 0.3)
+    dev = DeviceMesh(mesh)
+    A = dev.csr_map(2)
+    inv, indices, indptr, nnz = O.csr_map(mesh.connect, 2, mesh.Nn * 2)
+    assert np.array_equal(A.host.indptr, indptr) and np.array_equal(A.host.indices, indices)
+    geo = O.Geometry("QUAD4", mesh.connect, mesh.coord, "rigi", 2)
+    Ke_ref = O.K_elastic(geo, O.C_isotropic(210000.0, 0.3, 2, True)) * 13.0
+    A.scatter(to_dev(Ke_ref.ravel()))
+    got = A.values.cpu().numpy()
+    want = np.bincount(inv, weights=Ke_ref.ravel(), minlength=nnz)
+    assert np.array_equal(got, want)  # same summation order as np.bincount
+    # run twice: deterministic
+    A.scatter(to_dev(Ke_ref.ravel()))
+    assert np.array_equal(A.values.cpu().numpy(), got)
+
+
+def _cantilever(nx, ny, solver):
+    L, h = 120.0, 13.0
+    mesh = ea.quad4_rectangle(L, h, nx, ny)
+    mat = Models.Elastic.Isotropic(2, 210000.0, 0.3, planeStress=True, thickness=h)
+    simu = Simulations.Elastic(mesh, mat)
+    simu.solver = solver
+    n0 = mesh.Nodes_Conditions(lambda x, y, z: x == 0)
+    nL = mesh.Nodes_Conditions(lambda x, y, z: x == L)
+    simu.add_dirichlet(n0, [0, 0], ["x", "y"])
+    simu.add_surfLoad(nL, [-800 / h**2], ["y"])
+    ref = O.ElasticOracle(mesh.connect, mesh.coord, "QUAD4", 210000.0, 0.3, True, h,
+                          {"SEG2": mesh.dict_groupElem[ea.ElemType.SEG2].connect})
+    ref.add_dirichlet(n0, [0, 0], ["x", "y"])
+    ref.add_surfLoad(nL, [-800 / h**2], ["y"])
+    return simu, ref
+
+
+@pytest.mark.parametrize("solver", ["hip_fp64", "hip_mixed"])
+def test_cantilever_full_size(solver):
+    """360 x 39 QUAD4 cantilever (28,880 DOF): u within 1e-8 of the direct solve."""
+    simu, ref = _cantilever(360, 39, solver)
+    u = simu.Solve()
+    uref = ref.Solve()
+    assert _rel(u, uref) < 1e-8
+    K = simu.Get_K_C_M_F()[0]
+    Kref = ref.Get_K()
+    assert np.array_equal(K.indptr, Kref.indptr) and np.array_equal(K.indices, Kref.indices)
+    assert np.max(np.abs(K.data - Kref.data)) <= 1e-12 * np.max(np.abs(Kref.data))
+    # total applied load
+    assert abs(simu._neumann_vector("elastic").sum() + 800.0) < 1e-9
+
+
+@pytest.mark.parametrize("solver", ["hip_fp64", "hip_mixed"])
+def test_hexa8_heterogeneous(solver):
+    n = 8
+    mesh = ea.hexa8_cube(n)
+    rng = np.random.default_rng(0)
+    s = np.sqrt(np.log(1 + 0.2**2))
+    E = np.exp(np.log(210000.0) - s**2 / 2 + s * rng.standard_normal(mesh.Ne))
+    mat = Models.Elastic.Isotropic(3, E, 0.3)
+    simu = Simulations.Elastic(mesh, mat)
+    simu.solver = solver
+    bot = mesh.Nodes_Conditions(lambda x, y, z: z == 0)
+    top = mesh.Nodes_Conditions(lambda x, y, z: z == 1)
+    simu.add_dirichlet(bot, [0, 0, 0], ["x", "y", "z"])
+    simu.add_surfLoad(top, [-10.0], ["z"])
+    u = simu.Solve()
+    ref = O.ElasticOracle(mesh.connect, mesh.coord, "HEXA8", E, 0.3, False, 1.0,
+                          {"QUAD4": mesh.dict_groupElem[ea.ElemType.QUAD4].connect})
+    ref.add_dirichlet(bot, [0, 0, 0], ["x", "y", "z"])
+    ref.add_surfLoad(top, [-10.0], ["z"])
+    uref = ref.Solve()
+    assert _rel(u, uref) < 1e-8
+    assert abs(simu._neumann_vector("elastic").sum() + 10.0) < 1e-9
+
+
+def _strain_samples(rng, n):
+    e = rng.standard_normal((n, 3)) * 1e-3
+    # special cases: zero, pure tension, pure compression, equal eigenvalues, pure shear
+    special = np.array([[0, 0, 0], [1e-3, 1e-3, 0], [-1e-3, -1e-3, 0], [2e-3, 2e-3, 0], [0, 0, 1e-3],
+                        [1e-3, -1e-3, 0], [1e-3, 0, 0], [0, -1e-3, 0]], dtype=float)
+    return np.concatenate([special, e])
+
+
+@pytest.mark.parametrize("split", ["Amor", "Miehe"])
+@pytest.mark.parametrize("ps", [False, True])
+def test_split_pointwise(split, ps):
+    rng = np.random.default_rng(3)
+    eps = _strain_samples(rng, 500)
+    n = eps.shape[0]
+    E = to_dev(np.array([210000.0]))
+    d_eps = to_dev(eps)
+    cP = torch.empty(n * 9, dtype=torch.float64, device="cuda")
+    cM = torch.empty_like(cP)
+    pP = torch.empty(n, dtype=torch.float64, device="cuda")
+    pM = torch.empty_like(pP)
+    _lib.call("efea_pf_split_points", n, 0 if split == "Amor" else 1, ptr(d_eps), ptr(E), 0, 0.3, int(ps),
+              ptr(cP), ptr(cM), ptr(pP), ptr(pM), stream_ptr())
+    rcP, rcM = O.calc_C(split, eps[None], 210000.0, 0.3, 2, ps)
+    rpP, rpM = O.calc_psi(split, eps[None], 210000.0, 0.3, 2, ps)
+    gcP = cP.cpu().numpy().reshape(n, 3, 3)
+    gcM = cM.cpu().numpy().reshape(n, 3, 3)
+    scale = np.linalg.norm(O.C_isotropic(210000.0, 0.3, 2, ps))
+    assert np.max(np.abs(gcP - rcP[0])) / scale < 1e-12
+    assert np.max(np.abs(gcM - rcM[0])) / scale < 1e-12
+    psi_scale = np.max(np.abs(rpP[0]) + np.abs(rpM[0]))
+    assert np.max(np.abs(pP.cpu().numpy() - rpP[0])) / psi_scale < 1e-12
+    assert np.max(np.abs(pM.cpu().numpy() - rpM[0])) / psi_scale < 1e-12
+
+
+@pytest.mark.parametrize("split", ["Amor", "Miehe"])
+@pytest.mark.parametrize("et", ["TRI3", "QUAD4"])
+def test_phasefield_element_operators(split, et):
+    base = ea.tri3_notched_square(12) if et == "TRI3" else ea.quad4_rectangle(1.0, 1.0, 12, 12)
+    mesh = _distorted(base, 0.3) if et == "QUAD4" else base
+    rng = np.random.default_rng(5)
+    Nn, Ne = mesh.Nn, mesh.Ne
+    u = 1e-3 * rng.standard_normal(Nn * 2)
+    d = np.clip(rng.random(Nn), 0, 1)
+    Gc = 2.7 * np.exp(0.15 * rng.standard_normal(Ne))
+    l0, th = 0.02, 2.0
+    pfo = O.PhaseFieldOracle(mesh.connect, mesh.coord, et, 210000.0, 0.3, False, th, split, "AT2", Gc, l0)
+    pfo.u, pfo.d = u.copy(), d.copy()
+    nPgm = pfo.geo_m.nPg
+    H = np.abs(rng.standard_normal((Ne, nPgm))) * 10.0
+    pfo.old_psiP = H.copy()
+    mat = Models.Elastic.Isotropic(2, 210000.0, 0.3, planeStress=False, thickness=th)
+    pfm = Models.PhaseField(mat, split, "AT2", Gc, l0)
+    simu = Simulations.PhaseField(mesh, pfm)
+    simu._u, simu._d = to_dev(u), to_dev(d)
+    simu._H = to_dev(H.ravel())
+    # damage operators
+    pfo.assemble_damage()
+    simu._assemble_damage()
+    Kd = simu._Kd.to_scipy()
+    assert np.array_equal(Kd.indptr, pfo.Kd.indptr) and np.array_equal(Kd.indices, pfo.Kd.indices)
+    assert np.max(np.abs(Kd.data - pfo.Kd.data)) <= 1e-12 * np.max(np.abs(pfo.Kd.data))
+    assert _rel(simu._Fd.cpu().numpy(), pfo.Fd) < 1e-12
+    assert _rel(simu._psiP.cpu().numpy().reshape(Ne, nPgm), pfo.psiP) < 1e-12
+    # displacement operator
+    pfo.assemble_elastic()
+    simu._assemble_elastic()
+    Ku = simu._Ku.to_scipy()
+    assert np.max(np.abs(Ku.data - pfo.Ku.data)) <= 1e-12 * np.max(np.abs(pfo.Ku.data))
+
+
+@pytest.mark.parametrize("solver", ["hip_fp64", "hip_mixed"])
+def test_staggered_steps_match_oracle(solver):
+    """Five load steps of the notched-square tension test (Miehe, AT2)."""
+    n = 40
+    mesh = ea.tri3_notched_square(n)
+    mat = Models.Elastic.Isotropic(2, 210000.0, 0.3, planeStress=False, thickness=1.0)
+    pfm = Models.PhaseField(mat, "Miehe", "AT2", 2.7, 0.05)
+    simu = Simulations.PhaseField(mesh, pfm)
+    simu.solver = solver
+    ref = O.PhaseFieldOracle(mesh.connect, mesh.coord, "TRI3", 210000.0, 0.3, False, 1.0, "Miehe", "AT2", 2.7, 0.05)
+    top = mesh.Nodes_Conditions(lambda x, y, z: y == 1.0)
+    bot = mesh.Nodes_Conditions(lambda x, y, z: y == 0.0)
+    dofsY = simu.Bc_dofs_nodes(top, ["y"])
+    f_dev, f_ref, errs = [], [], []
+    for it in range(1, 6):
+        dep = it * 2e-3
+        for s in (simu, ref):
+            s.Bc_Init()
+            s.add_dirichlet(top, [0, dep], ["x", "y"])
+            s.add_dirichlet(bot, [0], ["y"])
+        u, d, conv = simu.Solve(1.0, 500, 1)
+        ur, dr, _ = ref.Solve(1.0, 500, 1)
+        simu.Save_Iter()
+        ref.Save_Iter()
+        f_dev.append(np.sum(simu.Calc_Reaction(dofsY, "elastic")))
+        f_ref.append(np.sum(ref.Calc_Reaction(dofsY)))
+        assert conv
+        errs.append((it, _rel(u, ur), _rel(d, dr)))
+    print("step errors (it, u, d):", errs)
+    for it, eu, ed in errs:
+        assert eu < 1e-8 and ed < 1e-8, errs
+    f_dev, f_ref = np.array(f_dev), np.array(f_ref)
+    assert np.max(np.abs(f_dev - f_ref) / np.abs(f_ref)) < 1e-6
+
+
+def test_solver_deterministic():
+    simu, _ = _cantilever(60, 6, "hip_mixed")
+    u1 = simu.Solve()
+    simu._u.zero_()
+    u2 = simu.Solve()
+    assert np.array_equal(u1, u2)
