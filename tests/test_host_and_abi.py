@@ -1,0 +1,191 @@
+"""Host logic and the C ABI (CPU only: no compute call reaches a GPU)."""
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import easyfea_amd as ea
+from easyfea_amd import _lib
+from easyfea_amd.assembly_map import build_csr_map, build_vec_map
+from oracle import efea_oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+# ---------------------------------------------------------------- C ABI
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "efea.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(efea_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    lib = _lib.load()
+    declared = _declared_symbols()
+    assert len(declared) >= 15
+    for name in declared:
+        assert hasattr(lib, name), f"{name} missing from libefea_hip.so"
+        assert name in _lib.SIGNATURES, f"{name} has no ctypes signature"
+    assert lib.efea_version() == 1
+
+
+def test_abi_argument_errors_are_reported_without_gpu():
+    lib = _lib.load()
+    rc = lib.efea_elastic_ke(1, 64, -1, None, None, None, 4, None, 0, 0.3, 1, 1.0, None, None)
+    assert rc == -1
+    assert "negative size" in _lib.last_error()
+    rc = lib.efea_solve_pcg(64, 0, 2, 16, None, None, None, None, None, None, None, None, None, 0, 1e-12, 1e-9,
+                            1e-3, 10, 1, None, None)
+    assert rc == -1
+    rc = lib.efea_pf_split_points(4, 7, None, None, 0, 0.3, 0, None, None, None, None, None)
+    assert rc == -1 and "null pointer" in _lib.last_error()
+
+
+def test_workspace_sizes_and_groups():
+    lib = _lib.load()
+    assert lib.efea_default_group(2, 9) == 16 and lib.efea_default_group(1, 7) == 8
+    assert lib.efea_default_group(3, 27) == 32
+    small = lib.efea_pcg_workspace_bytes(100, 2, 16)
+    big = lib.efea_pcg_workspace_bytes(10000, 2, 16)
+    assert big > small >= 4 * 100 * 2 * 8
+
+
+# ---------------------------------------------------------------- meshes
+def test_notched_square_topology():
+    n = 20
+    m = ea.tri3_notched_square(n)
+    assert m.Ne == 2 * n * n and m.Nn == (n + 1) ** 2 + n // 2
+    # every element positively oriented, total area 1
+    X = m.coord[m.connect][..., :2]
+    area = 0.5 * ((X[:, 1, 0] - X[:, 0, 0]) * (X[:, 2, 1] - X[:, 0, 1]) - (X[:, 2, 0] - X[:, 0, 0]) * (X[:, 1, 1] - X[:, 0, 1]))
+    assert np.all(area > 0) and abs(area.sum() - 1.0) < 1e-12
+    # lower and upper crack faces share no node left of the tip
+    y = m.coord[:, 1]
+    crack = np.nonzero((np.abs(y - 0.5) < 1e-12) & (m.coord[:, 0] < 0.5 - 1e-12))[0]
+    assert crack.size == n  # originals + copies
+    below = m.connect[(X[..., 1].mean(1) < 0.5)]
+    above = m.connect[(X[..., 1].mean(1) > 0.5)]
+    assert not (set(below.ravel()) & set(above.ravel()) & set(crack.tolist()))
+    # the tip node is shared
+    tip = np.nonzero((np.abs(y - 0.5) < 1e-12) & (np.abs(m.coord[:, 0] - 0.5) < 1e-12))[0]
+    assert tip.size == 1 and tip[0] in below and tip[0] in above
+
+
+def test_rectangle_and_box_boundaries():
+    m = ea.quad4_rectangle(120.0, 13.0, 360, 39)
+    assert m.Nn == 14440 and m.Ne == 14040
+    nL = m.Nodes_Conditions(lambda x, y, z: x == 120.0)
+    assert nL.size == 40
+    seg = m.dict_groupElem[ea.ElemType.SEG2]
+    assert seg.Get_Elements_Nodes(nL).size == 39
+    b = ea.hexa8_cube(4)
+    assert b.Nn == 125 and b.Ne == 64
+    top = b.Nodes_Conditions(lambda x, y, z: z == 1.0)
+    assert b.dict_groupElem[ea.ElemType.QUAD4].Get_Elements_Nodes(top).size == 16
+    g = O.Geometry("HEXA8", b.connect, b.coord, "rigi", 3)
+    assert np.all(g.detJ > 0)
+
+
+# ---------------------------------------------------------------- maps
+@pytest.mark.parametrize("maker,dof_n", [
+    (lambda: ea.quad4_rectangle(120.0, 13.0, 36, 9), 2),
+    (lambda: ea.tri3_notched_square(16), 2),
+    (lambda: ea.tri3_notched_square(16), 1),
+    (lambda: ea.hexa8_box(1.0, 2.0, 1.5, 4, 3, 5), 3),
+])
+def test_scatter_map_matches_scipy_canonical(maker, dof_n):
+    m = maker()
+    cm = build_csr_map(m.connect, m.Nn, dof_n)
+    inv, indices, indptr, nnz = O.csr_map(m.connect, dof_n, m.Nn * dof_n)
+    assert np.array_equal(cm.indptr, indptr) and np.array_equal(cm.indices, indices)
+    assert cm.indptr.dtype == np.int32 and cm.indices.dtype == np.int32
+    slot = np.empty(cm.perm.size, dtype=np.int64)
+    slot[cm.perm] = np.repeat(np.arange(cm.nnz), np.diff(cm.seg_ptr))
+    assert np.array_equal(slot, inv)
+    # within a slot, entries in increasing element-entry order (bincount order)
+    starts = cm.seg_ptr[:-1]
+    ok = np.ones(cm.perm.size, dtype=bool)
+    ok[starts[starts < cm.perm.size]] = False
+    assert np.all(np.diff(cm.perm)[ok[1:]] > 0)
+    # gather-form scatter of random entries equals bincount exactly
+    rng = np.random.default_rng(0)
+    vals = rng.standard_normal(cm.perm.size)
+    out = np.array([vals[cm.perm[a:b]].sum() if b > a else 0.0 for a, b in zip(cm.seg_ptr[:-1], cm.seg_ptr[1:])])
+    seq = np.zeros(cm.nnz)
+    for s_ in range(cm.nnz):
+        acc = 0.0
+        for k in cm.perm[cm.seg_ptr[s_]:cm.seg_ptr[s_ + 1]]:
+            acc += vals[k]
+        seq[s_] = acc
+    assert np.array_equal(seq, np.bincount(inv, weights=vals, minlength=nnz))
+    assert np.allclose(out, seq, rtol=1e-14, atol=1e-14)
+
+
+def test_node_adjacency_matches_pattern():
+    m = ea.tri3_notched_square(8)
+    cm = build_csr_map(m.connect, m.Nn, 2)
+    # row n*2+a holds, per neighbour m, columns 2m, 2m+1
+    for n in range(0, m.Nn, 7):
+        nb = cm.node_cols[cm.node_ptr[n]:cm.node_ptr[n + 1]]
+        for a in range(2):
+            r = 2 * n + a
+            cols = cm.indices[cm.indptr[r]:cm.indptr[r + 1]]
+            assert np.array_equal(cols, (nb[:, None] * 2 + np.arange(2)).ravel())
+
+
+def test_vector_map():
+    m = ea.quad4_rectangle(2.0, 1.0, 5, 3)
+    vm = build_vec_map(m.connect, m.Nn, 1)
+    rows = m.connect.ravel()
+    vals = np.arange(rows.size, dtype=float)
+    out = np.array([vals[vm.perm[a:b]].sum() for a, b in zip(vm.seg_ptr[:-1], vm.seg_ptr[1:])])
+    assert np.array_equal(out, np.bincount(rows, weights=vals, minlength=m.Nn))
+
+
+# ---------------------------------------------------------------- models
+def test_models_parameters_and_observers():
+    mat = ea.Models.Elastic.Isotropic(2, 210000.0, 0.3, planeStress=False, thickness=2.0)
+    lam, mu, K = O.lame(210000.0, 0.3, 2, False)
+    assert abs(mat.get_lambda() - lam) < 1e-9 and abs(mat.get_mu() - mu) < 1e-9
+    assert abs(mat.get_bulk() - K) < 1e-9
+    pfm = ea.Models.PhaseField(mat, "Miehe", "AT2", 2.7, 0.01)
+    assert abs(pfm.k - 2.7 * 0.01) < 1e-18 and pfm.c_w == 2.0
+    flags = []
+
+    class Obs:
+        def Need_Update(self, value=True):
+            flags.append(value)
+
+    o = Obs()
+    pfm._Add_observer(o)
+    mat._Add_observer(o)
+    mat.E = 200000.0
+    pfm.Gc = 3.0
+    assert len(flags) >= 2
+    with pytest.raises(AssertionError):
+        mat.v = 0.5
+    with pytest.raises(AssertionError):
+        pfm.l0 = -1.0
+    with pytest.raises(ValueError):
+        ea.Models.PhaseField(mat, "Bourdin", "AT2", 1.0, 1.0)
+
+
+def test_solver_type_names():
+    assert [s.value for s in ea.SolverType] == ["hip_fp64", "hip_mixed"]
+    from easyfea_amd.Solvers import REFERENCE_ONLY
+
+    assert "scipy" in REFERENCE_ONLY and "pypardiso" in REFERENCE_ONLY
+
+
+def test_product_path_refuses_to_run_without_gpu():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    mesh = ea.quad4_rectangle(1.0, 1.0, 2, 2)
+    mat = ea.Models.Elastic.Isotropic(2)
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        ea.Simulations.Elastic(mesh, mat)
