@@ -43,7 +43,7 @@ int bj_setup(int, int, int, const int*, const int*, const double*, const unsigne
 size_t pcg_workspace_bytes(int, int, int);
 int solve_pcg(int, int, int, int, const int*, const int*, const double*, const float*, const unsigned char*,
               const void*, const double*, double*, void*, double, double, double, int, int, hipStream_t, double*);
-int dot(int64_t, const double*, const double*, double*, double*, unsigned*, hipStream_t);
+int dot(int64_t, const double*, const double*, double*, double*, hipStream_t);
 int gather_sum(int64_t, const int*, const double*, double*, hipStream_t);
 int default_group(int, int);
 int pf_damage(int, int64_t, const int*, const double*, const double*, int, const double*, int, const double*,
@@ -165,8 +165,7 @@ int efea_dot(int64_t n, const double* a, const double* b, double* scratch, doubl
     set_error("efea_dot: null pointer or negative size");
     return EFEA_ERR_ARG;
   }
-  // scratch: 1024 partials + a counter word
-  return dot(n, a, b, scratch, out, (unsigned*)(scratch + 1024), STREAM(stream));
+  return dot(n, a, b, scratch, out, STREAM(stream));
 }
 
 int efea_gather_sum(int64_t n, const int32_t* idx, const double* y, double* out, void* stream) {

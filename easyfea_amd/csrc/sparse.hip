@@ -115,130 +115,86 @@ __device__ inline void spmv_node(const SpmvArgs& A, int n, int g, const TV* __re
   }
 }
 
-// Last-block-done: returns true in the block that finished last.  Each
-// block's partial is published with a device-scope fence before the
-// counter increment; the last block reads all partials with volatile loads.
-__device__ inline bool last_block(unsigned* counter) {
-  __shared__ bool amLast;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned prev = atomicAdd(counter, 1u);
-    amLast = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  return amLast;
+// ---------------------------------------------------------------------------
+// Fence-free global reductions.
+//
+// CDNA4 splits the chip into XCDs whose L2 caches are not mutually coherent:
+// a device-scope fence is an L2 write-back, so a "last block finishes the
+// reduction" pattern (one fence per block) serialises on the write-backs.
+// Here every kernel runs a fixed grid of at most MAXB blocks (grid-stride
+// loops), writes one partial per block with a plain store, and the *next*
+// kernel reduces those partials, redundantly in every block and in a fixed
+// order (deterministic).  Visibility comes from the kernel boundary: one
+// release per dispatch instead of one per block.
+constexpr int BS = 256;
+constexpr int MAXB = 1024;
+
+static inline unsigned grid_for(int64_t threads) {
+  int64_t nb = (threads + BS - 1) / BS;
+  if (nb > MAXB) nb = MAXB;
+  if (nb < 1) nb = 1;
+  return (unsigned)nb;
 }
 
-template <int BS>
-__device__ inline double reduce_partials(const double* partial, int n, double* sh) {
-  const volatile double* vp = partial;
+// Sum of n partials (stride `stride`, offset `off`) in every thread of the block.
+__device__ inline double reduce_all(const double* __restrict__ part, int n, int stride, int off, double* sh) {
   double s = 0;
-  for (int i = threadIdx.x; i < n; i += BS) s += vp[i];
-  return block_sum<double, BS>(s, sh);
+  for (int i = threadIdx.x; i < n; i += BS) s += part[stride * i + off];
+  s = block_sum<double, BS>(s, sh);
+  __shared__ double bc;
+  if (threadIdx.x == 0) bc = s;
+  __syncthreads();
+  const double r = bc;
+  __syncthreads();
+  return r;
 }
 
-// PCG device state (one per solve workspace).  Scalars stay FP64 in every
-// mode; the stopping test uses the Strakos-Tichy delayed estimate of the
-// A-norm error: after k+D iterations, ||x* - x_k||_A^2 ~ sum_{j=k}^{k+D-1}
-// alpha_j (r_j . z_j), compared with the running total (the squared A-norm
-// of the correction solved so far).
+// PCG device state.  Scalars stay FP64 in every mode.  The stopping test is
+// the Strakos-Tichy delayed estimate of the A-norm error: after k+RING
+// iterations ||x* - x_k||_A^2 ~ sum_{j=k}^{k+RING-1} alpha_j (r_j . z_j),
+// compared with max(sum_j alpha_j r_j . z_j, x0.A x0).
+// Fields written by block 0 of a kernel are never read by other blocks of
+// that same kernel (rz_prev / rrmax_prev are the per-iteration snapshots).
 constexpr int RING = 32;
 struct PcgState {
-  double rz;       // r.z of the current iterate
-  double pq;       // p.Ap
-  double alpha, beta;
-  double rr;       // r.r (recursive residual)
-  double rrmax;    // max r.r since the last residual replacement
-  double bb;       // initial residual norm^2
-  double tolr2;    // stop when rr <= tolr2 * bb (true residual)
-  double tolA2;    // stop when window <= tolA2 * total
-  double delta2;   // replacement trigger (0: never)
-  double Sall;     // sum of alpha_j rz_j
-  double Sw;       // sum over the ring window
-  double xAx0;     // x0.A x0 over unknown rows (scale of a warm start)
+  double rz, rz_prev, alpha, beta, pq;
+  double rr, rrmax, rrmax_prev, bb;
+  double tolr2, tolA2, delta2;
+  double Sall, Sw, xAx0;
   double ring[RING];
-  int head;
-  int done;        // 0 running, 1 converged, 2 breakdown
-  int iters;
-  int replace;     // residual replacement pending
-  int nrepl;
-  unsigned counter[4];
+  int head, done, iters, replace, nrepl, pad;
 };
 
-constexpr int BS = 256;
-
-// Residual r = b - A x (FP64 values), masked; r.r into *out_rr.
-template <int DN, int G>
-__global__ void __launch_bounds__(BS)
-k_residual(SpmvArgs A, const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ r,
-           double* __restrict__ partial, double* __restrict__ out_rr, unsigned* counter) {
-  __shared__ double sh[BS / 64];
-  const int gid = blockIdx.x * BS + threadIdx.x;
-  const int n = gid / G, g = gid % G;
-  double d = 0;
-  if (n < A.Nnode) spmv_node<double, double, DN, G, 1>(A, n, g, x, b, r, d);
-  const double bsum = block_sum<double, BS>(d, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = bsum;
-  if (last_block(counter)) {
-    const double tot = reduce_partials<BS>(partial, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      *out_rr = tot;
-      *counter = 0;
-    }
+__global__ void k_state_reset(PcgState* st, double tolr2, double tolA2, double delta2) {
+  if (threadIdx.x == 0) {
+    st->rz = st->rz_prev = st->alpha = st->beta = st->pq = 0;
+    st->rr = st->rrmax = st->rrmax_prev = st->bb = 0;
+    st->tolr2 = tolr2;
+    st->tolA2 = tolA2;
+    st->delta2 = delta2;
+    st->Sall = st->Sw = st->xAx0 = 0;
+    for (int i = 0; i < RING; ++i) st->ring[i] = 0;
+    st->head = st->done = st->iters = st->replace = st->nrepl = st->pad = 0;
   }
 }
 
-// x.A x over unknown rows (y = A x as scratch) -> out.
-template <int DN, int G>
-__global__ void __launch_bounds__(BS)
-k_xax(SpmvArgs A, const double* __restrict__ x, double* __restrict__ y, double* __restrict__ partial,
-      double* __restrict__ out, unsigned* counter) {
-  __shared__ double sh[BS / 64];
-  const int gid = blockIdx.x * BS + threadIdx.x;
-  const int n = gid / G, g = gid % G;
-  double d = 0;
-  if (n < A.Nnode) spmv_node<double, double, DN, G, 0>(A, n, g, x, nullptr, y, d);
-  const double bsum = block_sum<double, BS>(d, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = bsum;
-  if (last_block(counter)) {
-    const double tot = reduce_partials<BS>(partial, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      *out = tot;
-      *counter = 0;
-    }
-  }
-}
-
-// q = A p (masked) ; pq partial ; last block: alpha = rz / pq.
+// K1: q = A p (masked rows), per-block partial of p.q.  Also the standalone SpMV.
 template <typename TM, int DN, int G>
 __global__ void __launch_bounds__(BS)
-k_pcg_spmv(SpmvArgs A, const double* __restrict__ p, double* __restrict__ q, double* __restrict__ partial,
-           PcgState* st) {
+k_pcg_spmv(SpmvArgs A, const double* __restrict__ p, double* __restrict__ q, double* __restrict__ part,
+           const PcgState* st) {
   if (st->done) return;
   __shared__ double sh[BS / 64];
-  const int gid = blockIdx.x * BS + threadIdx.x;
-  const int n = gid / G, g = gid % G;
   double d = 0;
-  if (n < A.Nnode) spmv_node<TM, double, DN, G, 0>(A, n, g, p, nullptr, q, d);
-  const double bsum = block_sum<double, BS>(d, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = bsum;
-  if (last_block(&st->counter[0])) {
-    const double pq = reduce_partials<BS>(partial, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      st->pq = pq;
-      if (st->rz == 0) {  // r = 0 exactly: converged
-        st->done = 1;
-        st->alpha = 0;
-      } else if (!(pq > 0) || !(st->rz == st->rz)) {
-        st->done = 2;
-        st->alpha = 0;
-      } else {
-        st->alpha = st->rz / pq;
-      }
-      st->counter[0] = 0;
-    }
+  const int64_t lanes = (int64_t)A.Nnode * G;
+  for (int64_t gid = (int64_t)blockIdx.x * BS + threadIdx.x; gid < lanes; gid += (int64_t)gridDim.x * BS) {
+    const int n = (int)(gid / G), g = (int)(gid % G);
+    double dd;
+    spmv_node<TM, double, DN, G, 0>(A, n, g, p, nullptr, q, dd);
+    d += dd;
   }
+  d = block_sum<double, BS>(d, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = d;
 }
 
 template <typename TP, int DN>
@@ -254,41 +210,32 @@ __device__ inline void apply_bj(const TP* __restrict__ Minv, const unsigned char
   }
 }
 
-// Convergence bookkeeping shared by the update and replacement kernels.
-__device__ inline void pcg_close_iteration(PcgState* st, double rz_new, double rr) {
-  const double term = st->alpha * st->rz;  // alpha_j (r_j . z_j)
-  st->Sall += term;
-  st->ring[st->head] = term;
-  st->head = (st->head + 1) % RING;
-  // window summed afresh: a running add/subtract would cancel catastrophically
-  double Sw = 0;
-  for (int i = 0; i < RING; ++i) Sw += st->ring[i];
-  st->Sw = Sw;
-  st->iters += 1;
-  st->beta = rz_new / st->rz;
-  st->rz = rz_new;
-  st->rr = rr;
-  if (rr > st->rrmax) st->rrmax = rr;
-  const double scale = st->Sall > st->xAx0 ? st->Sall : st->xAx0;
-  if (st->iters >= RING && st->Sw <= st->tolA2 * scale) st->done = 1;
-  if (rr <= st->tolr2 * st->bb) st->done = 1;
-  if (!(rr == rr) || !(rz_new == rz_new)) st->done = 2;
-}
-
-// x += alpha p ; r -= alpha q ; z = Minv r ; partials of r.z and r.r ;
-// last block: beta and the stopping test, or flags a residual replacement.
+// K2: alpha = rz / pq ; x += alpha p ; r -= alpha q ; z = Minv r ; partials of r.z, r.r.
 template <typename TP, int DN>
 __global__ void __launch_bounds__(BS)
 k_pcg_update(int Nnode, const unsigned char* __restrict__ known, const TP* __restrict__ Minv,
              double* __restrict__ x, double* __restrict__ r, double* __restrict__ z, const double* __restrict__ p,
-             const double* __restrict__ q, double* __restrict__ partial, PcgState* st) {
+             const double* __restrict__ q, const double* __restrict__ part1, int nb1, double* __restrict__ part2,
+             PcgState* st) {
   if (st->done) return;
   __shared__ double sh[BS / 64];
   __shared__ double sh2[BS / 64];
-  const int n = blockIdx.x * BS + threadIdx.x;
-  double rz = 0, rr = 0;
-  if (n < Nnode) {
-    const double al = st->alpha;
+  const double pq = reduce_all(part1, nb1, 1, 0, sh);
+  const double rz = st->rz;
+  if (!(pq > 0) || !(rz > 0)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->done = (rz == 0) ? 1 : 2;  // r = 0 exactly: converged
+    return;
+  }
+  const double al = rz / pq;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->alpha = al;
+    st->pq = pq;
+    st->rz_prev = rz;
+    st->rrmax_prev = st->rrmax;
+    st->replace = 0;
+  }
+  double s_rz = 0, s_rr = 0;
+  for (int n = blockIdx.x * BS + threadIdx.x; n < Nnode; n += gridDim.x * BS) {
     double rv[DN], zv[DN];
 #pragma unroll
     for (int a = 0; a < DN; ++a) {
@@ -301,51 +248,76 @@ k_pcg_update(int Nnode, const unsigned char* __restrict__ known, const TP* __res
 #pragma unroll
     for (int a = 0; a < DN; ++a) {
       z[(int64_t)n * DN + a] = zv[a];
-      rz += rv[a] * zv[a];
-      rr += rv[a] * rv[a];
+      s_rz += rv[a] * zv[a];
+      s_rr += rv[a] * rv[a];
     }
   }
-  const double b1 = block_sum<double, BS>(rz, sh);
-  const double b2 = block_sum<double, BS>(rr, sh2);
+  s_rz = block_sum<double, BS>(s_rz, sh);
+  s_rr = block_sum<double, BS>(s_rr, sh2);
   if (threadIdx.x == 0) {
-    partial[2 * blockIdx.x] = b1;
-    partial[2 * blockIdx.x + 1] = b2;
-  }
-  if (last_block(&st->counter[1])) {
-    const volatile double* vp = partial;
-    double s1 = 0, s2 = 0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += BS) {
-      s1 += vp[2 * i];
-      s2 += vp[2 * i + 1];
-    }
-    s1 = block_sum<double, BS>(s1, sh);
-    s2 = block_sum<double, BS>(s2, sh2);
-    if (threadIdx.x == 0) {
-      if (st->delta2 > 0 && s2 < st->delta2 * st->rrmax) {
-        st->replace = 1;  // the replacement kernel closes this iteration
-      } else {
-        pcg_close_iteration(st, s1, s2);
-      }
-      st->counter[1] = 0;
-    }
+    part2[2 * blockIdx.x] = s_rz;
+    part2[2 * blockIdx.x + 1] = s_rr;
   }
 }
 
-// Reliable update: r = b - A64 x, z = Minv r, then close the iteration with
-// the replaced residual (Sleijpen & van der Vorst; Clark et al. 2010).
+// Iteration bookkeeping, block 0 only.
+__device__ inline void pcg_close_iteration(PcgState* st, double rz_new, double rr, bool replaced) {
+  const double term = st->alpha * st->rz_prev;  // alpha_j (r_j . z_j)
+  st->Sall += term;
+  st->ring[st->head] = term;
+  st->head = (st->head + 1) % RING;
+  double Sw = 0;  // summed afresh: a running add/subtract would cancel catastrophically
+  for (int i = 0; i < RING; ++i) Sw += st->ring[i];
+  st->Sw = Sw;
+  st->iters += 1;
+  st->beta = rz_new / st->rz_prev;
+  st->rz = rz_new;
+  st->rr = rr;
+  st->rrmax = replaced ? rr : (rr > st->rrmax_prev ? rr : st->rrmax_prev);
+  const double scale = st->Sall > st->xAx0 ? st->Sall : st->xAx0;
+  if (st->iters >= RING && Sw <= st->tolA2 * scale) st->done = 1;
+  if (rr <= st->tolr2 * st->bb) st->done = 1;
+  if (!(rr == rr) || !(rz_new == rz_new)) st->done = 2;
+  if (replaced && rr > 1e12 * st->bb) st->done = 3;  // true residual exploded: diverged
+}
+
+// K3: beta, p = z + beta p, or flag a residual replacement (mixed mode).
+template <int DN>
+__global__ void __launch_bounds__(BS)
+k_pcg_close(int64_t ndof, const double* __restrict__ z, double* __restrict__ p, const double* __restrict__ part2,
+            int nb2, PcgState* st) {
+  if (st->done) return;
+  __shared__ double sh[BS / 64];
+  const double rz_new = reduce_all(part2, nb2, 2, 0, sh);
+  const double rr = reduce_all(part2, nb2, 2, 1, sh);
+  const bool replace = st->delta2 > 0 && rr < st->delta2 * st->rrmax_prev;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (replace)
+      st->replace = 1;
+    else
+      pcg_close_iteration(st, rz_new, rr, false);
+  }
+  if (replace) return;
+  const double beta = rz_new / st->rz_prev;
+  for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < ndof; i += (int64_t)gridDim.x * BS)
+    p[i] = z[i] + beta * p[i];
+}
+
+// K4 (mixed): reliable update r = b - A64 x, z = Minv r, partials.
 template <typename TP, int DN, int G>
 __global__ void __launch_bounds__(BS)
 k_pcg_replace(SpmvArgs A64, const TP* __restrict__ Minv, const double* __restrict__ x,
               const double* __restrict__ b, double* __restrict__ r, double* __restrict__ z,
-              double* __restrict__ partial, PcgState* st) {
+              double* __restrict__ part3, const PcgState* st) {
   if (st->done || !st->replace) return;
   __shared__ double sh[BS / 64];
   __shared__ double sh2[BS / 64];
-  const int gid = blockIdx.x * BS + threadIdx.x;
-  const int n = gid / G, g = gid % G;
-  double rz = 0, rr = 0, d;
-  if (n < A64.Nnode) {
-    spmv_node<double, double, DN, G, 1>(A64, n, g, x, b, r, d);
+  double s_rz = 0, s_rr = 0;
+  const int64_t lanes = (int64_t)A64.Nnode * G;
+  for (int64_t gid = (int64_t)blockIdx.x * BS + threadIdx.x; gid < lanes; gid += (int64_t)gridDim.x * BS) {
+    const int n = (int)(gid / G), g = (int)(gid % G);
+    double dd;
+    spmv_node<double, double, DN, G, 1>(A64, n, g, x, b, r, dd);
     if (g == 0) {
       double rv[DN], zv[DN];
 #pragma unroll
@@ -354,55 +326,87 @@ k_pcg_replace(SpmvArgs A64, const TP* __restrict__ Minv, const double* __restric
 #pragma unroll
       for (int a = 0; a < DN; ++a) {
         z[(int64_t)n * DN + a] = zv[a];
-        rz += rv[a] * zv[a];
-        rr += rv[a] * rv[a];
+        s_rz += rv[a] * zv[a];
+        s_rr += rv[a] * rv[a];
       }
     }
   }
-  const double b1 = block_sum<double, BS>(rz, sh);
-  const double b2 = block_sum<double, BS>(rr, sh2);
+  s_rz = block_sum<double, BS>(s_rz, sh);
+  s_rr = block_sum<double, BS>(s_rr, sh2);
   if (threadIdx.x == 0) {
-    partial[2 * blockIdx.x] = b1;
-    partial[2 * blockIdx.x + 1] = b2;
-  }
-  if (last_block(&st->counter[2])) {
-    const volatile double* vp = partial;
-    double s1 = 0, s2 = 0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += BS) {
-      s1 += vp[2 * i];
-      s2 += vp[2 * i + 1];
-    }
-    s1 = block_sum<double, BS>(s1, sh);
-    s2 = block_sum<double, BS>(s2, sh2);
-    if (threadIdx.x == 0) {
-      st->replace = 0;
-      st->nrepl += 1;
-      st->rrmax = 0;
-      pcg_close_iteration(st, s1, s2);
-      st->rrmax = s2;
-      st->counter[2] = 0;
-    }
+    part3[2 * blockIdx.x] = s_rz;
+    part3[2 * blockIdx.x + 1] = s_rr;
   }
 }
 
+// K5 (mixed): close the iteration with the replaced residual.
+template <int DN>
 __global__ void __launch_bounds__(BS)
-k_pcg_pupdate(int64_t ndof, const double* __restrict__ z, double* __restrict__ p, const PcgState* st) {
-  if (st->done) return;
-  const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (i < ndof) p[i] = z[i] + st->beta * p[i];
+k_pcg_replace_close(int64_t ndof, const double* __restrict__ z, double* __restrict__ p,
+                    const double* __restrict__ part3, int nb3, PcgState* st) {
+  if (st->done || !st->replace) return;
+  __shared__ double sh[BS / 64];
+  const double rz_new = reduce_all(part3, nb3, 2, 0, sh);
+  const double rr = reduce_all(part3, nb3, 2, 1, sh);
+  const double beta = rz_new / st->rz_prev;
+  for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < ndof; i += (int64_t)gridDim.x * BS)
+    p[i] = z[i] + beta * p[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->nrepl += 1;
+    pcg_close_iteration(st, rz_new, rr, true);
+  }
 }
 
-// Initial state from r (already b - A x): z = Minv r, p = z, rz, rr.
+// Residual r = b - A x (FP64), masked; per-block partials of r.r.
+template <int DN, int G>
+__global__ void __launch_bounds__(BS)
+k_residual(SpmvArgs A, const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ r,
+           double* __restrict__ part) {
+  __shared__ double sh[BS / 64];
+  double d = 0;
+  const int64_t lanes = (int64_t)A.Nnode * G;
+  for (int64_t gid = (int64_t)blockIdx.x * BS + threadIdx.x; gid < lanes; gid += (int64_t)gridDim.x * BS) {
+    const int n = (int)(gid / G), g = (int)(gid % G);
+    double dd;
+    spmv_node<double, double, DN, G, 1>(A, n, g, x, b, r, dd);
+    d += dd;
+  }
+  d = block_sum<double, BS>(d, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = d;
+}
+
+// x.A x over unknown rows (y = A x as scratch), per-block partials.
+template <int DN, int G>
+__global__ void __launch_bounds__(BS)
+k_xax(SpmvArgs A, const double* __restrict__ x, double* __restrict__ y, double* __restrict__ part) {
+  __shared__ double sh[BS / 64];
+  double d = 0;
+  const int64_t lanes = (int64_t)A.Nnode * G;
+  for (int64_t gid = (int64_t)blockIdx.x * BS + threadIdx.x; gid < lanes; gid += (int64_t)gridDim.x * BS) {
+    const int n = (int)(gid / G), g = (int)(gid % G);
+    double dd;
+    spmv_node<double, double, DN, G, 0>(A, n, g, x, nullptr, y, dd);
+    d += dd;
+  }
+  d = block_sum<double, BS>(d, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = d;
+}
+
+// Initial z = Minv r, p = z, partials of r.z and r.r; block 0 also folds the
+// x0.A x0 partials into the state.
 template <typename TP, int DN>
 __global__ void __launch_bounds__(BS)
 k_pcg_init(int Nnode, const unsigned char* __restrict__ known, const TP* __restrict__ Minv,
            const double* __restrict__ r, double* __restrict__ z, double* __restrict__ p,
-           double* __restrict__ partial, PcgState* st) {
+           const double* __restrict__ partI, int nbI, double* __restrict__ part2, PcgState* st) {
   __shared__ double sh[BS / 64];
   __shared__ double sh2[BS / 64];
-  const int n = blockIdx.x * BS + threadIdx.x;
-  double rz = 0, rr = 0;
-  if (n < Nnode) {
+  if (blockIdx.x == 0) {
+    const double xax = reduce_all(partI, nbI, 1, 0, sh);
+    if (threadIdx.x == 0) st->xAx0 = xax;
+  }
+  double s_rz = 0, s_rr = 0;
+  for (int n = blockIdx.x * BS + threadIdx.x; n < Nnode; n += gridDim.x * BS) {
     double rv[DN], zv[DN];
 #pragma unroll
     for (int a = 0; a < DN; ++a) rv[a] = r[(int64_t)n * DN + a];
@@ -411,51 +415,36 @@ k_pcg_init(int Nnode, const unsigned char* __restrict__ known, const TP* __restr
     for (int a = 0; a < DN; ++a) {
       z[(int64_t)n * DN + a] = zv[a];
       p[(int64_t)n * DN + a] = zv[a];
-      rz += rv[a] * zv[a];
-      rr += rv[a] * rv[a];
+      s_rz += rv[a] * zv[a];
+      s_rr += rv[a] * rv[a];
     }
   }
-  const double b1 = block_sum<double, BS>(rz, sh);
-  const double b2 = block_sum<double, BS>(rr, sh2);
+  s_rz = block_sum<double, BS>(s_rz, sh);
+  s_rr = block_sum<double, BS>(s_rr, sh2);
   if (threadIdx.x == 0) {
-    partial[2 * blockIdx.x] = b1;
-    partial[2 * blockIdx.x + 1] = b2;
-  }
-  if (last_block(&st->counter[1])) {
-    const volatile double* vp = partial;
-    double s1 = 0, s2 = 0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += BS) {
-      s1 += vp[2 * i];
-      s2 += vp[2 * i + 1];
-    }
-    s1 = block_sum<double, BS>(s1, sh);
-    s2 = block_sum<double, BS>(s2, sh2);
-    if (threadIdx.x == 0) {
-      st->rz = s1;
-      st->rr = s2;
-      st->rrmax = s2;
-      st->bb = s2;
-      st->done = (s2 == 0) ? 1 : 0;
-      st->counter[1] = 0;
-    }
+    part2[2 * blockIdx.x] = s_rz;
+    part2[2 * blockIdx.x + 1] = s_rr;
   }
 }
 
-__global__ void k_state_reset(PcgState* st, double tolr2, double tolA2, double delta2) {
+__global__ void __launch_bounds__(BS) k_pcg_init_scalars(const double* __restrict__ part2, int nb2, PcgState* st) {
+  __shared__ double sh[BS / 64];
+  const double rz = reduce_all(part2, nb2, 2, 0, sh);
+  const double rr = reduce_all(part2, nb2, 2, 1, sh);
   if (threadIdx.x == 0) {
-    st->rz = st->pq = st->alpha = st->beta = st->rr = st->rrmax = st->bb = 0;
-    st->tolr2 = tolr2;
-    st->tolA2 = tolA2;
-    st->delta2 = delta2;
-    st->Sall = st->Sw = st->xAx0 = 0;
-    for (int i = 0; i < RING; ++i) st->ring[i] = 0;
-    st->head = 0;
-    st->done = 0;
-    st->iters = 0;
-    st->replace = 0;
-    st->nrepl = 0;
-    st->counter[0] = st->counter[1] = st->counter[2] = st->counter[3] = 0;
+    st->rz = rz;
+    st->rr = rr;
+    st->rrmax = rr;
+    st->bb = rr;
+    st->done = (rr == 0) ? 1 : 0;
   }
+}
+
+// One-block finalize of a partial array into *out.
+__global__ void __launch_bounds__(BS) k_sum_partials(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  __shared__ double sh[BS / 64];
+  const double s = reduce_all(part, nb, 1, 0, sh);
+  if (threadIdx.x == 0) *out = s;
 }
 
 // Block-Jacobi: invert each node's diagonal dof_n x dof_n block, with the
@@ -523,22 +512,15 @@ __global__ void k_bj_setup(int Nnode, const int* __restrict__ node_ptr, const in
     for (int b = 0; b < DN; ++b) Minv[(int64_t)n * DN * DN + a * DN + b] = (T)I[a][b];
 }
 
-// Sum of y[idx[k]] in fixed order (reactions) and generic deterministic dot.
-__global__ void k_dot(int64_t n, const double* __restrict__ a, const double* __restrict__ b,
-                      double* __restrict__ partial, double* __restrict__ out, unsigned* counter) {
+// Deterministic dot product (b may be null: sum of a): partials + finalize.
+__global__ void __launch_bounds__(BS) k_dot_part(int64_t n, const double* __restrict__ a, const double* __restrict__ b,
+                                                 double* __restrict__ part) {
   __shared__ double sh[BS / 64];
   double s = 0;
   for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
     s += a[i] * (b ? b[i] : 1.0);
-  const double bs = block_sum<double, BS>(s, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = bs;
-  if (last_block(counter)) {
-    const double tot = reduce_partials<BS>(partial, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      *out = tot;
-      *counter = 0;
-    }
-  }
+  s = block_sum<double, BS>(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
 __global__ void k_gather_sum(int64_t n, const int* __restrict__ idx, const double* __restrict__ y,
@@ -550,34 +532,31 @@ __global__ void k_gather_sum(int64_t n, const int* __restrict__ idx, const doubl
   if (threadIdx.x == 0) *out = s;
 }
 
-
 // ---------------------------------------------------------------------------
 // Host-side drivers
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
-static int64_t n_partials(int Nnode, int G) {
-  const int64_t nb_spmv = ((int64_t)Nnode * G + BS - 1) / BS;
-  const int64_t nb_node = ((int64_t)Nnode + BS - 1) / BS;
-  return 2 * (nb_spmv > nb_node ? nb_spmv : nb_node) + 64;
-}
-
-size_t pcg_workspace_bytes(int Nnode, int dof_n, int group) {
-  const size_t vb = align256(8 * (size_t)Nnode * dof_n);
-  return 4 * vb + align256(8 * (size_t)n_partials(Nnode, group)) + align256(sizeof(PcgState)) + 256;
-}
 
 struct PcgWork {
   double* r;
   double* z;
   double* p;
   double* q;
-  double* partial;
+  double* part1;  // MAXB
+  double* part2;  // 2 MAXB
+  double* part3;  // 2 MAXB
+  double* partI;  // MAXB
   PcgState* st;
   double* small;
 };
 
-static PcgWork carve(void* ws, int Nnode, int dof_n, int group) {
+size_t pcg_workspace_bytes(int Nnode, int dof_n, int group) {
+  (void)group;
+  const size_t vb = align256(8 * (size_t)Nnode * dof_n);
+  return 4 * vb + align256(8 * MAXB) * 2 + align256(16 * MAXB) * 2 + align256(sizeof(PcgState)) + 256;
+}
+
+static PcgWork carve(void* ws, int Nnode, int dof_n) {
   char* c = (char*)ws;
   const size_t vb = align256(8 * (size_t)Nnode * dof_n);
   PcgWork w;
@@ -585,7 +564,10 @@ static PcgWork carve(void* ws, int Nnode, int dof_n, int group) {
   w.z = (double*)c; c += vb;
   w.p = (double*)c; c += vb;
   w.q = (double*)c; c += vb;
-  w.partial = (double*)c; c += align256(8 * (size_t)n_partials(Nnode, group));
+  w.part1 = (double*)c; c += align256(8 * MAXB);
+  w.partI = (double*)c; c += align256(8 * MAXB);
+  w.part2 = (double*)c; c += align256(16 * MAXB);
+  w.part3 = (double*)c; c += align256(16 * MAXB);
   w.st = (PcgState*)c; c += align256(sizeof(PcgState));
   w.small = (double*)c;
   return w;
@@ -605,19 +587,18 @@ static int pcg_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, cons
                    hipStream_t s, double* info) {
   const int Nnode = A64.Nnode;
   const int64_t ndof = (int64_t)Nnode * DN;
-  PcgWork w = carve(ws, Nnode, DN, G);
-  const unsigned nb_node = (unsigned)((Nnode + BS - 1) / BS);
-  const unsigned nb_dof = (unsigned)((ndof + BS - 1) / BS);
-  const unsigned nb_spmv = (unsigned)(((int64_t)Nnode * G + BS - 1) / BS);
+  PcgWork w = carve(ws, Nnode, DN);
+  const unsigned nbS = grid_for((int64_t)Nnode * G);  // node-group kernels
+  const unsigned nbN = grid_for(Nnode);               // node kernels
+  const unsigned nbD = grid_for(ndof);                // dof kernels
   const bool mixed = (void*)AK.vals != (void*)A64.vals;
   hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, w.st, tol_r * tol_r, tol_A * tol_A,
                      mixed ? delta * delta : 0.0);
-  hipLaunchKernelGGL((k_xax<DN, G>), dim3(nb_spmv), dim3(BS), 0, s, A64, (const double*)x, w.q, w.partial,
-                     &w.st->xAx0, &w.st->counter[3]);
-  hipLaunchKernelGGL((k_residual<DN, G>), dim3(nb_spmv), dim3(BS), 0, s, A64, (const double*)x, b, w.r,
-                     w.partial, w.small, &w.st->counter[3]);
-  hipLaunchKernelGGL((k_pcg_init<TP, DN>), dim3(nb_node), dim3(BS), 0, s, Nnode, A64.known, Minv,
-                     (const double*)w.r, w.z, w.p, w.partial, w.st);
+  hipLaunchKernelGGL((k_xax<DN, G>), dim3(nbS), dim3(BS), 0, s, A64, (const double*)x, w.q, w.partI);
+  hipLaunchKernelGGL((k_residual<DN, G>), dim3(nbS), dim3(BS), 0, s, A64, (const double*)x, b, w.r, w.part1);
+  hipLaunchKernelGGL((k_pcg_init<TP, DN>), dim3(nbN), dim3(BS), 0, s, Nnode, A64.known, Minv, (const double*)w.r,
+                     w.z, w.p, (const double*)w.partI, (int)nbS, w.part2, w.st);
+  hipLaunchKernelGGL(k_pcg_init_scalars, dim3(1), dim3(BS), 0, s, (const double*)w.part2, (int)nbN, w.st);
   if (int e = check_launch("pcg_init")) return e;
   PcgState* hs = host_state();
   if (!hs) {
@@ -627,15 +608,18 @@ static int pcg_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, cons
   int launched = 0;
   while (true) {
     for (int k = 0; k < check_every; ++k) {
-      hipLaunchKernelGGL((k_pcg_spmv<TM, DN, G>), dim3(nb_spmv), dim3(BS), 0, s, AK, (const double*)w.p, w.q,
-                         w.partial, w.st);
-      hipLaunchKernelGGL((k_pcg_update<TP, DN>), dim3(nb_node), dim3(BS), 0, s, Nnode, A64.known, Minv, x, w.r,
-                         w.z, (const double*)w.p, (const double*)w.q, w.partial, w.st);
-      if (mixed)
-        hipLaunchKernelGGL((k_pcg_replace<TP, DN, G>), dim3(nb_spmv), dim3(BS), 0, s, A64, Minv,
-                           (const double*)x, b, w.r, w.z, w.partial, w.st);
-      hipLaunchKernelGGL(k_pcg_pupdate, dim3(nb_dof), dim3(BS), 0, s, ndof, (const double*)w.z, w.p,
+      hipLaunchKernelGGL((k_pcg_spmv<TM, DN, G>), dim3(nbS), dim3(BS), 0, s, AK, (const double*)w.p, w.q, w.part1,
                          (const PcgState*)w.st);
+      hipLaunchKernelGGL((k_pcg_update<TP, DN>), dim3(nbN), dim3(BS), 0, s, Nnode, A64.known, Minv, x, w.r, w.z,
+                         (const double*)w.p, (const double*)w.q, (const double*)w.part1, (int)nbS, w.part2, w.st);
+      hipLaunchKernelGGL((k_pcg_close<DN>), dim3(nbD), dim3(BS), 0, s, ndof, (const double*)w.z, w.p,
+                         (const double*)w.part2, (int)nbN, w.st);
+      if (mixed) {
+        hipLaunchKernelGGL((k_pcg_replace<TP, DN, G>), dim3(nbS), dim3(BS), 0, s, A64, Minv, (const double*)x, b,
+                           w.r, w.z, w.part3, (const PcgState*)w.st);
+        hipLaunchKernelGGL((k_pcg_replace_close<DN>), dim3(nbD), dim3(BS), 0, s, ndof, (const double*)w.z, w.p,
+                           (const double*)w.part3, (int)nbS, w.st);
+      }
     }
     launched += check_every;
     if (int e = check_launch("pcg_iter")) return e;
@@ -648,9 +632,8 @@ static int pcg_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, cons
   }
   const int done = hs->done, iters = hs->iters, nrepl = hs->nrepl;
   const double bb = hs->bb;
-  // final true residual
-  hipLaunchKernelGGL((k_residual<DN, G>), dim3(nb_spmv), dim3(BS), 0, s, A64, (const double*)x, b, w.r,
-                     w.partial, w.small, &w.st->counter[3]);
+  hipLaunchKernelGGL((k_residual<DN, G>), dim3(nbS), dim3(BS), 0, s, A64, (const double*)x, b, w.r, w.part1);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(BS), 0, s, (const double*)w.part1, (int)nbS, w.small);
   double rr_true = 0;
   if (hipMemcpyAsync(&rr_true, w.small, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess) {
@@ -661,8 +644,9 @@ static int pcg_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, cons
   info[1] = bb > 0 ? sqrt(rr_true / bb) : 0.0;
   info[2] = nrepl;
   info[3] = done;
-  if (done == 2) {
-    set_error("pcg: breakdown (p.Ap <= 0 or NaN) after %d iterations", iters);
+  if (done == 2 || done == 3) {
+    set_error("pcg: %s after %d iterations", done == 2 ? "breakdown (p.Ap <= 0, r.z <= 0 or NaN)" : "divergence",
+              iters);
     return EFEA_ERR_NOCONV;
   }
   if (done != 1) {
@@ -698,34 +682,30 @@ int solve_pcg(int mode, int Nnode, int dof_n, int group, const int* node_ptr, co
   return EFEA_ERR_ARG;
 }
 
-// Standalone y = A x through the same kernel the PCG iteration launches
-// (k_pcg_spmv: product, masked rows, fused p.Ap reduction), so a timing of
-// this entry point measures the solver's SpMV.  scratch: a PcgState plus
-// partials, see spmv_scratch_bytes().
+// Standalone y = A x through the PCG's own SpMV kernel (same template
+// instance), so that timing it measures the solver's SpMV.
 size_t spmv_scratch_bytes(int Nnode, int group) {
-  return align256(sizeof(PcgState)) + align256(8 * (size_t)n_partials(Nnode, group));
+  (void)Nnode;
+  (void)group;
+  return align256(sizeof(PcgState)) + align256(8 * MAXB);
 }
 
 __global__ void k_spmv_state_init(PcgState* st) {
-  if (threadIdx.x == 0) {
-    st->done = 0;
-    st->rz = 1.0;
-    st->counter[0] = 0;
-  }
+  if (threadIdx.x == 0) st->done = 0;
 }
 
 int spmv(int prec, int Nnode, int dof_n, int group, const int* node_ptr, const int* node_cols, const void* vals,
          const unsigned char* known, const double* x, double* y, void* scratch, hipStream_t s) {
   SpmvArgs A{Nnode, node_ptr, node_cols, vals, known};
-  const unsigned nb = (unsigned)(((int64_t)Nnode * group + BS - 1) / BS);
-  if (nb == 0) return EFEA_OK;
+  if (Nnode == 0) return EFEA_OK;
+  const unsigned nb = grid_for((int64_t)Nnode * group);
   PcgState* st = (PcgState*)scratch;
-  double* partial = (double*)((char*)scratch + align256(sizeof(PcgState)));
+  double* part = (double*)((char*)scratch + align256(sizeof(PcgState)));
   hipLaunchKernelGGL(k_spmv_state_init, dim3(1), dim3(64), 0, s, st);
-#define SP(TM, D, GV)                                                                                      \
-  if (dof_n == D && group == GV) {                                                                        \
-    hipLaunchKernelGGL((k_pcg_spmv<TM, D, GV>), dim3(nb), dim3(BS), 0, s, A, x, y, partial, st);            \
-    return check_launch("spmv");                                                                          \
+#define SP(TM, D, GV)                                                                                       \
+  if (dof_n == D && group == GV) {                                                                         \
+    hipLaunchKernelGGL((k_pcg_spmv<TM, D, GV>), dim3(nb), dim3(BS), 0, s, A, x, y, part, (const PcgState*)st); \
+    return check_launch("spmv");                                                                           \
   }
   if (prec == 64) {
     SP(double, 1, 8) SP(double, 1, 16) SP(double, 2, 8) SP(double, 2, 16) SP(double, 3, 16) SP(double, 3, 32)
@@ -756,12 +736,11 @@ int bj_setup(int prec, int Nnode, int dof_n, const int* node_ptr, const int* nod
   return check_launch("bj_setup");
 }
 
-int dot(int64_t n, const double* a, const double* b, double* partial, double* out, unsigned* counter,
-        hipStream_t s) {
-  int64_t nb = (n + BS - 1) / BS;
-  if (nb > 1024) nb = 1024;
-  if (nb < 1) nb = 1;
-  hipLaunchKernelGGL(k_dot, dim3((unsigned)nb), dim3(BS), 0, s, n, a, b, partial, out, counter);
+// scratch: MAXB doubles of partials.
+int dot(int64_t n, const double* a, const double* b, double* scratch, double* out, hipStream_t s) {
+  const unsigned nb = grid_for(n);
+  hipLaunchKernelGGL(k_dot_part, dim3(nb), dim3(BS), 0, s, n, a, b, scratch);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(BS), 0, s, (const double*)scratch, (int)nb, out);
   return check_launch("dot");
 }
 

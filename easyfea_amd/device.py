@@ -175,6 +175,7 @@ class SolveInfo:
     replacements: int
     status: int
     mode: str
+    fallback: bool = False
 
 
 class Workspace:
@@ -190,15 +191,8 @@ class Workspace:
         return self.buf
 
 
-def solve(A: DeviceCsr, b: torch.Tensor, x: torch.Tensor, known: Optional[torch.Tensor], mode: str,
-          ws: Workspace, tol_residual: float = 1e-12, tol_anorm: float = 1e-9, replace_ratio: float = 1e-3,
-          maxit: int = 100000, check_every: int = 32, raise_on_fail: bool = True) -> SolveInfo:
-    """Solves the Dirichlet-reduced system on the device (see ``efea_solve_pcg``).
-
-    ``x`` carries the Dirichlet values at the known DOFs and the initial guess
-    elsewhere; it is overwritten with the solution."""
-    assert mode in ("fp64", "mixed")
-    m = 64 if mode == "fp64" else 32
+def _solve_once(A: DeviceCsr, b, x, known, m: int, ws: Workspace, tol_residual, tol_anorm, replace_ratio, maxit,
+                check_every):
     Minv = torch.empty(A.Nn * A.dof_n * A.dof_n, dtype=torch.float64 if m == 64 else torch.float32,
                        device="cuda")
     _lib.call("efea_bj_setup", m, A.Nn, A.dof_n, ptr(A.node_ptr), ptr(A.node_cols), ptr(A.values), ptr(known),
@@ -210,7 +204,26 @@ def solve(A: DeviceCsr, b: torch.Tensor, x: torch.Tensor, known: Optional[torch.
     rc = _lib.call("efea_solve_pcg", m, A.Nn, A.dof_n, A.group, ptr(A.node_ptr), ptr(A.node_cols), ptr(A.values),
                    ptr(v32), ptr(known), ptr(Minv), ptr(b), ptr(x), ptr(buf), nbytes, tol_residual, tol_anorm,
                    replace_ratio, maxit, check_every, stream_ptr(), C.cast(info, C.c_void_p))
-    si = SolveInfo(int(info[0]), float(info[1]), int(info[2]), int(info[3]), mode)
+    return rc, SolveInfo(int(info[0]), float(info[1]), int(info[2]), int(info[3]), "fp64" if m == 64 else "mixed")
+
+
+def solve(A: DeviceCsr, b: torch.Tensor, x: torch.Tensor, known: Optional[torch.Tensor], mode: str,
+          ws: Workspace, tol_residual: float = 1e-12, tol_anorm: float = 1e-9, replace_ratio: float = 1e-3,
+          maxit: int = 100000, check_every: int = 32, raise_on_fail: bool = True) -> SolveInfo:
+    """Solves the Dirichlet-reduced system on the device (see ``efea_solve_pcg``).
+
+    ``x`` carries the Dirichlet values at the known DOFs and the initial guess
+    elsewhere; it is overwritten with the solution.  A mixed-precision solve
+    that breaks down, diverges or stalls is redone in FP64 from the same
+    initial guess (``SolveInfo.fallback``)."""
+    assert mode in ("fp64", "mixed")
+    m = 64 if mode == "fp64" else 32
+    x0 = x.clone() if m == 32 else None
+    rc, si = _solve_once(A, b, x, known, m, ws, tol_residual, tol_anorm, replace_ratio, maxit, check_every)
+    if rc != 0 and m == 32:
+        x.copy_(x0)
+        rc, si = _solve_once(A, b, x, known, 64, ws, tol_residual, tol_anorm, replace_ratio, maxit, check_every)
+        si.fallback = True
     if rc != 0 and raise_on_fail:
         raise _lib.EfeaError("efea_solve_pcg", rc, _lib.last_error())
     return si

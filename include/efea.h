@@ -112,8 +112,7 @@ int efea_solve_pcg(int mode, int Nnode, int dof_n, int group, const int32_t* nod
                    double tol_anorm, double replace_ratio, int maxit, int check_every, void* stream,
                    double* info);
 
-/* Deterministic dot product (b may be NULL: sum of a); scratch holds 1024
- * doubles plus one zero-initialised counter word. */
+/* Deterministic dot product (b may be NULL: sum of a); scratch holds 1024 doubles. */
 int efea_dot(int64_t n, const double* a, const double* b, double* scratch, double* out, void* stream);
 
 /* out = sum_k y[idx[k]] in index order (reaction resultants). */
