@@ -51,6 +51,8 @@ def run_history(mesh, solver, steps, keep_states=()):
     bot = mesh.Nodes_Conditions(lambda x, y, z: y == 0.0)
     dofsY = simu.Bc_dofs_nodes(top, ["y"])
     lat, forces, its_u, its_d, states = [], [], [], [], {}
+    fallbacks = 0
+    verbose = bool(os.environ.get("EFEA_BENCH_VERBOSE"))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for it in range(1, steps + 1):
@@ -64,12 +66,16 @@ def run_history(mesh, solver, steps, keep_states=()):
         lat.append(time.perf_counter() - ts)
         its_u.append(simu.last_solve["elastic"].iterations)
         its_d.append(simu.last_solve["damage"].iterations)
+        fallbacks += int(simu.last_solve["elastic"].fallback) + int(simu.last_solve["damage"].fallback)
+        if verbose:
+            print(f"# {solver} step {it}: {lat[-1] * 1e3:.1f} ms, u {simu.last_solve['elastic']}, "
+                  f"d {simu.last_solve['damage']}, f {forces[-1]:.4f}", file=sys.stderr, flush=True)
         if it in keep_states:
             states[it] = (u.copy(), d.copy())
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
     return dict(simu=simu, lat=np.array(lat), total=total, forces=np.array(forces), u=u, d=d,
-                its_u=np.array(its_u), its_d=np.array(its_d), states=states)
+                its_u=np.array(its_u), its_d=np.array(its_d), states=states, fallbacks=fallbacks)
 
 
 def cpu_baseline(mesh, steps):
@@ -94,9 +100,9 @@ def cpu_baseline(mesh, steps):
 
 
 def spmv_roofline(A, reps):
-    """Average duration of the solver's SpMV kernel (FP32 values, the mixed
-    path's Krylov product) over ``reps`` launches, HIP events on the stream
-    the kernel runs on."""
+    """Average duration of the PCG's dominant kernel (step 1: p update fused
+    into the SpMV, FP32 values as in the mixed path) over ``reps`` launches,
+    HIP events on the stream the kernel runs on."""
     import torch
 
     from easyfea_amd import _lib
@@ -105,21 +111,20 @@ def spmv_roofline(A, reps):
     x = torch.rand(A.Ndof, dtype=torch.float64, device="cuda")
     y = torch.empty_like(x)
     vals = A.make_fp32()
-    args = (32, A.Nn, A.dof_n, A.group, ptr(A.node_ptr), ptr(A.node_cols), ptr(vals), None, ptr(x), ptr(y),
-            ptr(A.spmv_scratch), stream_ptr())
-    for _ in range(10):
-        _lib.call("efea_spmv", *args)
+    head = (32, A.Nn, A.dof_n, A.group, ptr(A.node_ptr), ptr(A.node_cols), ptr(vals), None, ptr(x), ptr(y),
+            ptr(A.spmv_scratch))
+    _lib.call("efea_spmv_repeat", *head, 20, stream_ptr())
     s = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(s)
-    for _ in range(reps):
-        _lib.call("efea_spmv", *args)
+    _lib.call("efea_spmv_repeat", *head, reps, stream_ptr())
     e1.record(s)
     e1.synchronize()
-    dt = e0.elapsed_time(e1) / 1e3 / reps  # includes the tiny state-init kernel per call
+    dt = e0.elapsed_time(e1) / 1e3 / reps  # one state-init kernel, then reps step-1 launches
     hA = A.host
     nnz, nnz_node, Nn, Ndof = A.nnz, hA.node_cols.size, A.Nn, A.Ndof
-    algo_bytes = nnz * 4 + nnz_node * 4 + (Nn + 1) * 4 + Ndof * 8 + Ndof * 8
+    # values + node columns + node offsets + mask + x read + p copy write + y write
+    algo_bytes = nnz * 4 + nnz_node * 4 + (Nn + 1) * 4 + Ndof * 1 + Ndof * 8 * 3
     csr_bytes = nnz * (4 + 4) + (Ndof + 1) * 4 + 2 * Ndof * 4
     traffic = None
     tf = os.path.join(ROOT, "profiles", "spmv_pf200_traffic.json")
@@ -128,7 +133,7 @@ def spmv_roofline(A, reps):
             traffic = json.load(f).get("bytes_per_launch")
     peak = 8.0e12
     return {
-        "kernel": "k_pcg_spmv<float,2,16>",
+        "kernel": "k_pcg_step1<float,2,16>",
         "bound": "hbm",
         "achieved": algo_bytes / dt / 1e9,
         "peak": peak / 1e9,
@@ -243,6 +248,7 @@ def main():
         "p99_step_ms": float(np.percentile(lat_ms, 99)),
         "pcg_iters_u_mean": float(mixed["its_u"].mean()),
         "pcg_iters_d_mean": float(mixed["its_d"].mean()),
+        "fp64_fallbacks": int(mixed["fallbacks"]),
         "final_force_N": float(mixed["forces"][-1]),
         "accuracy": accuracy,
         "roofline": {k: roof[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},

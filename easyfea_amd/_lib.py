@@ -27,6 +27,7 @@ SIGNATURES = {
     "efea_cast_f64_f32": (i32, [i64, vp, vp, vp]),
     "efea_spmv_scratch_bytes": (sz, [i32, i32]),
     "efea_spmv": (i32, [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "efea_spmv_repeat": (i32, [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, vp]),
     "efea_bj_setup": (i32, [i32, i32, i32, vp, vp, vp, vp, vp, vp]),
     "efea_pcg_workspace_bytes": (sz, [i32, i32, i32]),
     "efea_default_group": (i32, [i32, i32]),

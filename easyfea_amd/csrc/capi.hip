@@ -37,7 +37,7 @@ int boundary_load(int, int64_t, const double*, const double*, int, const double*
 int scatter(int, int64_t, const int*, const int*, const void*, void*, hipStream_t);
 int cast_f64_f32(int64_t, const double*, float*, hipStream_t);
 int spmv(int, int, int, int, const int*, const int*, const void*, const unsigned char*, const double*, double*,
-         void*, hipStream_t);
+         void*, int, hipStream_t);
 size_t spmv_scratch_bytes(int, int);
 int bj_setup(int, int, int, const int*, const int*, const double*, const unsigned char*, void*, hipStream_t);
 size_t pcg_workspace_bytes(int, int, int);
@@ -116,7 +116,7 @@ int efea_cast_f64_f32(int64_t n, const double* src, float* dst, void* stream) {
   return cast_f64_f32(n, src, dst, STREAM(stream));
 }
 
-size_t efea_spmv_scratch_bytes(int Nnode, int group) { return spmv_scratch_bytes(Nnode, group); }
+size_t efea_spmv_scratch_bytes(int Nnode, int dof_n) { return spmv_scratch_bytes(Nnode, dof_n); }
 
 int efea_spmv(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr, const int32_t* node_cols,
               const void* values, const uint8_t* known, const double* x, double* y, void* scratch, void* stream) {
@@ -124,7 +124,18 @@ int efea_spmv(int precision, int Nnode, int dof_n, int group, const int32_t* nod
     set_error("efea_spmv: null pointer or negative size");
     return EFEA_ERR_ARG;
   }
-  return spmv(precision, Nnode, dof_n, group, node_ptr, node_cols, values, known, x, y, scratch, STREAM(stream));
+  return spmv(precision, Nnode, dof_n, group, node_ptr, node_cols, values, known, x, y, scratch, 1, STREAM(stream));
+}
+
+int efea_spmv_repeat(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr,
+                     const int32_t* node_cols, const void* values, const uint8_t* known, const double* x, double* y,
+                     void* scratch, int reps, void* stream) {
+  if (Nnode < 0 || reps < 1 || (Nnode > 0 && (!node_ptr || !node_cols || !values || !x || !y || !scratch))) {
+    set_error("efea_spmv_repeat: null pointer, negative size or reps < 1");
+    return EFEA_ERR_ARG;
+  }
+  return spmv(precision, Nnode, dof_n, group, node_ptr, node_cols, values, known, x, y, scratch, reps,
+              STREAM(stream));
 }
 
 int efea_bj_setup(int precision, int Nnode, int dof_n, const int32_t* node_ptr, const int32_t* node_cols,

@@ -153,8 +153,9 @@ __device__ inline double reduce_all(const double* __restrict__ part, int n, int 
 // the Strakos-Tichy delayed estimate of the A-norm error: after k+RING
 // iterations ||x* - x_k||_A^2 ~ sum_{j=k}^{k+RING-1} alpha_j (r_j . z_j),
 // compared with max(sum_j alpha_j r_j . z_j, x0.A x0).
-// Fields written by block 0 of a kernel are never read by other blocks of
-// that same kernel (rz_prev / rrmax_prev are the per-iteration snapshots).
+// A field written by block 0 of a kernel is never read by the other blocks
+// of that kernel: rz_prev, rrmax_prev and par_next are per-iteration
+// snapshots that break those races.
 constexpr int RING = 32;
 struct PcgState {
   double rz, rz_prev, alpha, beta, pq;
@@ -162,8 +163,14 @@ struct PcgState {
   double tolr2, tolA2, delta2;
   double Sall, Sw, xAx0;
   double ring[RING];
-  int head, done, iters, replace, nrepl, pad;
+  int head, done, iters, nrepl;
+  int fresh;     // no iteration to close yet: the first step uses beta = 0
+  int justrep;   // the r.z / r.r partials come from a residual replacement
+  int par;       // which of the two direction buffers holds p
+  int par_next;  // the buffer the current step writes
 };
+// done: 0 running, 1 converged, 2 breakdown, 3 diverged, 4 paused for a
+// residual replacement (mixed mode; the host runs k_pcg_replace).
 
 __global__ void k_state_reset(PcgState* st, double tolr2, double tolA2, double delta2) {
   if (threadIdx.x == 0) {
@@ -174,27 +181,12 @@ __global__ void k_state_reset(PcgState* st, double tolr2, double tolA2, double d
     st->delta2 = delta2;
     st->Sall = st->Sw = st->xAx0 = 0;
     for (int i = 0; i < RING; ++i) st->ring[i] = 0;
-    st->head = st->done = st->iters = st->replace = st->nrepl = st->pad = 0;
+    st->head = st->done = st->iters = st->nrepl = 0;
+    st->fresh = 1;
+    st->justrep = 0;
+    st->par = 0;
+    st->par_next = 1;
   }
-}
-
-// K1: q = A p (masked rows), per-block partial of p.q.  Also the standalone SpMV.
-template <typename TM, int DN, int G>
-__global__ void __launch_bounds__(BS)
-k_pcg_spmv(SpmvArgs A, const double* __restrict__ p, double* __restrict__ q, double* __restrict__ part,
-           const PcgState* st) {
-  if (st->done) return;
-  __shared__ double sh[BS / 64];
-  double d = 0;
-  const int64_t lanes = (int64_t)A.Nnode * G;
-  for (int64_t gid = (int64_t)blockIdx.x * BS + threadIdx.x; gid < lanes; gid += (int64_t)gridDim.x * BS) {
-    const int n = (int)(gid / G), g = (int)(gid % G);
-    double dd;
-    spmv_node<TM, double, DN, G, 0>(A, n, g, p, nullptr, q, dd);
-    d += dd;
-  }
-  d = block_sum<double, BS>(d, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = d;
 }
 
 template <typename TP, int DN>
@@ -210,13 +202,108 @@ __device__ inline void apply_bj(const TP* __restrict__ Minv, const unsigned char
   }
 }
 
-// K2: alpha = rz / pq ; x += alpha p ; r -= alpha q ; z = Minv r ; partials of r.z, r.r.
+// Iteration bookkeeping, block 0 only.
+__device__ inline void pcg_close_iteration(PcgState* st, double rz_new, double rr, bool replaced) {
+  const double term = st->alpha * st->rz_prev;  // alpha_j (r_j . z_j)
+  st->Sall += term;
+  st->ring[st->head] = term;
+  st->head = (st->head + 1) % RING;
+  double Sw = 0;  // summed afresh: a running add/subtract would cancel catastrophically
+  for (int i = 0; i < RING; ++i) Sw += st->ring[i];
+  st->Sw = Sw;
+  st->iters += 1;
+  st->beta = rz_new / st->rz_prev;
+  st->rz = rz_new;
+  st->rr = rr;
+  st->rrmax = replaced ? rr : (rr > st->rrmax_prev ? rr : st->rrmax_prev);
+  const double scale = st->Sall > st->xAx0 ? st->Sall : st->xAx0;
+  if (st->iters >= RING && Sw <= st->tolA2 * scale) st->done = 1;
+  if (rr <= st->tolr2 * st->bb) st->done = 1;
+  if (!(rr == rr) || !(rz_new == rz_new)) st->done = 2;
+  if (replaced && rr > 1e12 * st->bb) st->done = 3;  // true residual exploded: diverged
+}
+
+// Step 1 of an iteration: close the previous iteration (beta from the r.z
+// partials of step 2 or of a replacement), form p = z + beta p_old on the
+// fly for every column it reads, store p for the rows it owns, q = A p
+// (masked rows) and per-block partials of p.q.  In mixed mode it pauses
+// the solve (done = 4) when the recursive residual has fallen below
+// delta * its running maximum.
+template <typename TM, int DN, int G>
+__global__ void __launch_bounds__(BS)
+k_pcg_step1(SpmvArgs A, const double* __restrict__ z, const double* __restrict__ P0, const double* __restrict__ P1,
+            double* __restrict__ q, const double* __restrict__ part2, int nb2, double* __restrict__ part1,
+            PcgState* st) {
+  if (st->done) return;
+  __shared__ double sh[BS / 64];
+  const int par = st->par;
+  const double* __restrict__ pold = par ? P1 : P0;
+  double* __restrict__ pnew = (double*)(par ? P0 : P1);
+  double beta = 0;
+  if (!st->fresh) {
+    const double rz_new = reduce_all(part2, nb2, 2, 0, sh);
+    const double rr = reduce_all(part2, nb2, 2, 1, sh);
+    const bool jr = st->justrep;
+    if (!jr && st->delta2 > 0 && rr < st->delta2 * st->rrmax_prev) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 4;
+      return;
+    }
+    beta = rz_new / st->rz_prev;
+    if (blockIdx.x == 0 && threadIdx.x == 0) pcg_close_iteration(st, rz_new, rr, jr);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->par_next = 1 - par;
+  const TM* __restrict__ vals = (const TM*)A.vals;
+  double d = 0;
+  const int64_t lanes = (int64_t)A.Nnode * G;
+  for (int64_t gid = (int64_t)blockIdx.x * BS + threadIdx.x; gid < lanes; gid += (int64_t)gridDim.x * BS) {
+    const int n = (int)(gid / G), g = (int)(gid % G);
+    const int p0 = A.node_ptr[n], cnt = A.node_ptr[n + 1] - p0;
+    const int64_t base = (int64_t)DN * DN * p0;
+    double sum[DN];
+#pragma unroll
+    for (int a = 0; a < DN; ++a) sum[a] = 0;
+    for (int pos = g; pos < cnt; pos += G) {
+      const int c = A.node_cols[p0 + pos];
+      double xc[DN];
+#pragma unroll
+      for (int bb = 0; bb < DN; ++bb) xc[bb] = z[(int64_t)c * DN + bb] + beta * pold[(int64_t)c * DN + bb];
+#pragma unroll
+      for (int a = 0; a < DN; ++a) {
+        const TM* v = vals + base + (int64_t)a * DN * cnt + (int64_t)pos * DN;
+#pragma unroll
+        for (int bb = 0; bb < DN; ++bb) sum[a] += (double)v[bb] * xc[bb];
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) sum[a] += __shfl_xor(sum[a], o, G);
+    }
+    if (g == 0) {
+#pragma unroll
+      for (int a = 0; a < DN; ++a) {
+        const int64_t r = (int64_t)n * DN + a;
+        const double pn = z[r] + beta * pold[r];
+        pnew[r] = pn;
+        double qv = sum[a];
+        if (A.known && A.known[r]) qv = 0;
+        q[r] = qv;
+        d += pn * qv;
+      }
+    }
+  }
+  d = block_sum<double, BS>(d, sh);
+  if (threadIdx.x == 0) part1[blockIdx.x] = d;
+}
+
+// Step 2: alpha = rz / pq ; x += alpha p ; r -= alpha q ; z = Minv r ;
+// partials of r.z and r.r.
 template <typename TP, int DN>
 __global__ void __launch_bounds__(BS)
-k_pcg_update(int Nnode, const unsigned char* __restrict__ known, const TP* __restrict__ Minv,
-             double* __restrict__ x, double* __restrict__ r, double* __restrict__ z, const double* __restrict__ p,
-             const double* __restrict__ q, const double* __restrict__ part1, int nb1, double* __restrict__ part2,
-             PcgState* st) {
+k_pcg_step2(int Nnode, const unsigned char* __restrict__ known, const TP* __restrict__ Minv,
+            double* __restrict__ x, double* __restrict__ r, double* __restrict__ z, const double* __restrict__ P0,
+            const double* __restrict__ P1, const double* __restrict__ q, const double* __restrict__ part1, int nb1,
+            double* __restrict__ part2, PcgState* st) {
   if (st->done) return;
   __shared__ double sh[BS / 64];
   __shared__ double sh2[BS / 64];
@@ -227,12 +314,16 @@ k_pcg_update(int Nnode, const unsigned char* __restrict__ known, const TP* __res
     return;
   }
   const double al = rz / pq;
+  const int pn = st->par_next;
+  const double* __restrict__ p = pn ? P1 : P0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->alpha = al;
     st->pq = pq;
     st->rz_prev = rz;
     st->rrmax_prev = st->rrmax;
-    st->replace = 0;
+    st->fresh = 0;
+    st->justrep = 0;
+    st->par = pn;
   }
   double s_rz = 0, s_rr = 0;
   for (int n = blockIdx.x * BS + threadIdx.x; n < Nnode; n += gridDim.x * BS) {
@@ -260,56 +351,13 @@ k_pcg_update(int Nnode, const unsigned char* __restrict__ known, const TP* __res
   }
 }
 
-// Iteration bookkeeping, block 0 only.
-__device__ inline void pcg_close_iteration(PcgState* st, double rz_new, double rr, bool replaced) {
-  const double term = st->alpha * st->rz_prev;  // alpha_j (r_j . z_j)
-  st->Sall += term;
-  st->ring[st->head] = term;
-  st->head = (st->head + 1) % RING;
-  double Sw = 0;  // summed afresh: a running add/subtract would cancel catastrophically
-  for (int i = 0; i < RING; ++i) Sw += st->ring[i];
-  st->Sw = Sw;
-  st->iters += 1;
-  st->beta = rz_new / st->rz_prev;
-  st->rz = rz_new;
-  st->rr = rr;
-  st->rrmax = replaced ? rr : (rr > st->rrmax_prev ? rr : st->rrmax_prev);
-  const double scale = st->Sall > st->xAx0 ? st->Sall : st->xAx0;
-  if (st->iters >= RING && Sw <= st->tolA2 * scale) st->done = 1;
-  if (rr <= st->tolr2 * st->bb) st->done = 1;
-  if (!(rr == rr) || !(rz_new == rz_new)) st->done = 2;
-  if (replaced && rr > 1e12 * st->bb) st->done = 3;  // true residual exploded: diverged
-}
-
-// K3: beta, p = z + beta p, or flag a residual replacement (mixed mode).
-template <int DN>
-__global__ void __launch_bounds__(BS)
-k_pcg_close(int64_t ndof, const double* __restrict__ z, double* __restrict__ p, const double* __restrict__ part2,
-            int nb2, PcgState* st) {
-  if (st->done) return;
-  __shared__ double sh[BS / 64];
-  const double rz_new = reduce_all(part2, nb2, 2, 0, sh);
-  const double rr = reduce_all(part2, nb2, 2, 1, sh);
-  const bool replace = st->delta2 > 0 && rr < st->delta2 * st->rrmax_prev;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    if (replace)
-      st->replace = 1;
-    else
-      pcg_close_iteration(st, rz_new, rr, false);
-  }
-  if (replace) return;
-  const double beta = rz_new / st->rz_prev;
-  for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < ndof; i += (int64_t)gridDim.x * BS)
-    p[i] = z[i] + beta * p[i];
-}
-
-// K4 (mixed): reliable update r = b - A64 x, z = Minv r, partials.
+// Reliable update (mixed mode, run by the host after a pause):
+// r = b - A64 x, z = Minv r, r.z / r.r partials in place of step 2's.
 template <typename TP, int DN, int G>
 __global__ void __launch_bounds__(BS)
 k_pcg_replace(SpmvArgs A64, const TP* __restrict__ Minv, const double* __restrict__ x,
               const double* __restrict__ b, double* __restrict__ r, double* __restrict__ z,
-              double* __restrict__ part3, const PcgState* st) {
-  if (st->done || !st->replace) return;
+              double* __restrict__ part2, PcgState* st) {
   __shared__ double sh[BS / 64];
   __shared__ double sh2[BS / 64];
   double s_rz = 0, s_rr = 0;
@@ -334,26 +382,13 @@ k_pcg_replace(SpmvArgs A64, const TP* __restrict__ Minv, const double* __restric
   s_rz = block_sum<double, BS>(s_rz, sh);
   s_rr = block_sum<double, BS>(s_rr, sh2);
   if (threadIdx.x == 0) {
-    part3[2 * blockIdx.x] = s_rz;
-    part3[2 * blockIdx.x + 1] = s_rr;
+    part2[2 * blockIdx.x] = s_rz;
+    part2[2 * blockIdx.x + 1] = s_rr;
   }
-}
-
-// K5 (mixed): close the iteration with the replaced residual.
-template <int DN>
-__global__ void __launch_bounds__(BS)
-k_pcg_replace_close(int64_t ndof, const double* __restrict__ z, double* __restrict__ p,
-                    const double* __restrict__ part3, int nb3, PcgState* st) {
-  if (st->done || !st->replace) return;
-  __shared__ double sh[BS / 64];
-  const double rz_new = reduce_all(part3, nb3, 2, 0, sh);
-  const double rr = reduce_all(part3, nb3, 2, 1, sh);
-  const double beta = rz_new / st->rz_prev;
-  for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < ndof; i += (int64_t)gridDim.x * BS)
-    p[i] = z[i] + beta * p[i];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->done = 0;
+    st->justrep = 1;
     st->nrepl += 1;
-    pcg_close_iteration(st, rz_new, rr, true);
   }
 }
 
@@ -392,8 +427,8 @@ k_xax(SpmvArgs A, const double* __restrict__ x, double* __restrict__ y, double* 
   if (threadIdx.x == 0) part[blockIdx.x] = d;
 }
 
-// Initial z = Minv r, p = z, partials of r.z and r.r; block 0 also folds the
-// x0.A x0 partials into the state.
+// Initial z = Minv r, p = z (buffer P0), partials of r.z and r.r; block 0
+// also folds the x0.A x0 partials into the state.
 template <typename TP, int DN>
 __global__ void __launch_bounds__(BS)
 k_pcg_init(int Nnode, const unsigned char* __restrict__ known, const TP* __restrict__ Minv,
@@ -540,11 +575,11 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 struct PcgWork {
   double* r;
   double* z;
-  double* p;
+  double* P0;
+  double* P1;
   double* q;
   double* part1;  // MAXB
   double* part2;  // 2 MAXB
-  double* part3;  // 2 MAXB
   double* partI;  // MAXB
   PcgState* st;
   double* small;
@@ -553,7 +588,7 @@ struct PcgWork {
 size_t pcg_workspace_bytes(int Nnode, int dof_n, int group) {
   (void)group;
   const size_t vb = align256(8 * (size_t)Nnode * dof_n);
-  return 4 * vb + align256(8 * MAXB) * 2 + align256(16 * MAXB) * 2 + align256(sizeof(PcgState)) + 256;
+  return 5 * vb + align256(8 * MAXB) * 2 + align256(16 * MAXB) + align256(sizeof(PcgState)) + 256;
 }
 
 static PcgWork carve(void* ws, int Nnode, int dof_n) {
@@ -562,12 +597,12 @@ static PcgWork carve(void* ws, int Nnode, int dof_n) {
   PcgWork w;
   w.r = (double*)c; c += vb;
   w.z = (double*)c; c += vb;
-  w.p = (double*)c; c += vb;
+  w.P0 = (double*)c; c += vb;
+  w.P1 = (double*)c; c += vb;
   w.q = (double*)c; c += vb;
   w.part1 = (double*)c; c += align256(8 * MAXB);
   w.partI = (double*)c; c += align256(8 * MAXB);
   w.part2 = (double*)c; c += align256(16 * MAXB);
-  w.part3 = (double*)c; c += align256(16 * MAXB);
   w.st = (PcgState*)c; c += align256(sizeof(PcgState));
   w.small = (double*)c;
   return w;
@@ -586,18 +621,16 @@ static int pcg_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, cons
                    void* ws, double tol_r, double tol_A, double delta, int maxit, int check_every,
                    hipStream_t s, double* info) {
   const int Nnode = A64.Nnode;
-  const int64_t ndof = (int64_t)Nnode * DN;
   PcgWork w = carve(ws, Nnode, DN);
   const unsigned nbS = grid_for((int64_t)Nnode * G);  // node-group kernels
   const unsigned nbN = grid_for(Nnode);               // node kernels
-  const unsigned nbD = grid_for(ndof);                // dof kernels
   const bool mixed = (void*)AK.vals != (void*)A64.vals;
   hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, w.st, tol_r * tol_r, tol_A * tol_A,
                      mixed ? delta * delta : 0.0);
   hipLaunchKernelGGL((k_xax<DN, G>), dim3(nbS), dim3(BS), 0, s, A64, (const double*)x, w.q, w.partI);
   hipLaunchKernelGGL((k_residual<DN, G>), dim3(nbS), dim3(BS), 0, s, A64, (const double*)x, b, w.r, w.part1);
   hipLaunchKernelGGL((k_pcg_init<TP, DN>), dim3(nbN), dim3(BS), 0, s, Nnode, A64.known, Minv, (const double*)w.r,
-                     w.z, w.p, (const double*)w.partI, (int)nbS, w.part2, w.st);
+                     w.z, w.P0, (const double*)w.partI, (int)nbS, w.part2, w.st);
   hipLaunchKernelGGL(k_pcg_init_scalars, dim3(1), dim3(BS), 0, s, (const double*)w.part2, (int)nbN, w.st);
   if (int e = check_launch("pcg_init")) return e;
   PcgState* hs = host_state();
@@ -608,18 +641,12 @@ static int pcg_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, cons
   int launched = 0;
   while (true) {
     for (int k = 0; k < check_every; ++k) {
-      hipLaunchKernelGGL((k_pcg_spmv<TM, DN, G>), dim3(nbS), dim3(BS), 0, s, AK, (const double*)w.p, w.q, w.part1,
-                         (const PcgState*)w.st);
-      hipLaunchKernelGGL((k_pcg_update<TP, DN>), dim3(nbN), dim3(BS), 0, s, Nnode, A64.known, Minv, x, w.r, w.z,
-                         (const double*)w.p, (const double*)w.q, (const double*)w.part1, (int)nbS, w.part2, w.st);
-      hipLaunchKernelGGL((k_pcg_close<DN>), dim3(nbD), dim3(BS), 0, s, ndof, (const double*)w.z, w.p,
-                         (const double*)w.part2, (int)nbN, w.st);
-      if (mixed) {
-        hipLaunchKernelGGL((k_pcg_replace<TP, DN, G>), dim3(nbS), dim3(BS), 0, s, A64, Minv, (const double*)x, b,
-                           w.r, w.z, w.part3, (const PcgState*)w.st);
-        hipLaunchKernelGGL((k_pcg_replace_close<DN>), dim3(nbD), dim3(BS), 0, s, ndof, (const double*)w.z, w.p,
-                           (const double*)w.part3, (int)nbS, w.st);
-      }
+      hipLaunchKernelGGL((k_pcg_step1<TM, DN, G>), dim3(nbS), dim3(BS), 0, s, AK, (const double*)w.z,
+                         (const double*)w.P0, (const double*)w.P1, w.q, (const double*)w.part2, (int)nbN, w.part1,
+                         w.st);
+      hipLaunchKernelGGL((k_pcg_step2<TP, DN>), dim3(nbN), dim3(BS), 0, s, Nnode, A64.known, Minv, x, w.r, w.z,
+                         (const double*)w.P0, (const double*)w.P1, (const double*)w.q, (const double*)w.part1,
+                         (int)nbS, w.part2, w.st);
     }
     launched += check_every;
     if (int e = check_launch("pcg_iter")) return e;
@@ -628,9 +655,16 @@ static int pcg_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, cons
       set_error("pcg: state readback failed");
       return EFEA_ERR_HIP;
     }
+    if (hs->done == 4) {  // mixed mode: reliable update, then continue
+      hipLaunchKernelGGL((k_pcg_replace<TP, DN, G>), dim3(nbS), dim3(BS), 0, s, A64, Minv, (const double*)x, b,
+                         w.r, w.z, w.part2, w.st);
+      if (int e = check_launch("pcg_replace")) return e;
+      if (launched >= maxit) break;
+      continue;
+    }
     if (hs->done || launched >= maxit) break;
   }
-  const int done = hs->done, iters = hs->iters, nrepl = hs->nrepl;
+  const int done = hs->done == 4 ? 0 : hs->done, iters = hs->iters, nrepl = hs->nrepl;
   const double bb = hs->bb;
   hipLaunchKernelGGL((k_residual<DN, G>), dim3(nbS), dim3(BS), 0, s, A64, (const double*)x, b, w.r, w.part1);
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(BS), 0, s, (const double*)w.part1, (int)nbS, w.small);
@@ -682,30 +716,39 @@ int solve_pcg(int mode, int Nnode, int dof_n, int group, const int* node_ptr, co
   return EFEA_ERR_ARG;
 }
 
-// Standalone y = A x through the PCG's own SpMV kernel (same template
-// instance), so that timing it measures the solver's SpMV.
-size_t spmv_scratch_bytes(int Nnode, int group) {
-  (void)Nnode;
-  (void)group;
-  return align256(sizeof(PcgState)) + align256(8 * MAXB);
+// Standalone y = A x (masked rows), FP64 vectors, FP64 or FP32 values.  It
+// runs the solver's own step-1 kernel with beta = 0 (p = x), so a timing of
+// this entry point measures the PCG's dominant kernel.  scratch:
+// spmv_scratch_bytes(Nnode, dof_n) bytes (state, partials, p copy).
+size_t spmv_scratch_bytes(int Nnode, int dof_n) {
+  return align256(sizeof(PcgState)) + align256(8 * MAXB) + align256(8 * (size_t)Nnode * dof_n);
 }
 
 __global__ void k_spmv_state_init(PcgState* st) {
-  if (threadIdx.x == 0) st->done = 0;
+  if (threadIdx.x == 0) {
+    st->done = 0;
+    st->fresh = 1;
+    st->par = 0;
+  }
 }
 
 int spmv(int prec, int Nnode, int dof_n, int group, const int* node_ptr, const int* node_cols, const void* vals,
-         const unsigned char* known, const double* x, double* y, void* scratch, hipStream_t s) {
+         const unsigned char* known, const double* x, double* y, void* scratch, int reps, hipStream_t s) {
   SpmvArgs A{Nnode, node_ptr, node_cols, vals, known};
+  if (reps < 1) reps = 1;
   if (Nnode == 0) return EFEA_OK;
   const unsigned nb = grid_for((int64_t)Nnode * group);
-  PcgState* st = (PcgState*)scratch;
-  double* part = (double*)((char*)scratch + align256(sizeof(PcgState)));
+  char* c = (char*)scratch;
+  PcgState* st = (PcgState*)c;
+  double* part = (double*)(c + align256(sizeof(PcgState)));
+  double* pcopy = (double*)(c + align256(sizeof(PcgState)) + align256(8 * MAXB));
   hipLaunchKernelGGL(k_spmv_state_init, dim3(1), dim3(64), 0, s, st);
-#define SP(TM, D, GV)                                                                                       \
-  if (dof_n == D && group == GV) {                                                                         \
-    hipLaunchKernelGGL((k_pcg_spmv<TM, D, GV>), dim3(nb), dim3(BS), 0, s, A, x, y, part, (const PcgState*)st); \
-    return check_launch("spmv");                                                                           \
+#define SP(TM, D, GV)                                                                                      \
+  if (dof_n == D && group == GV) {                                                                        \
+    for (int k = 0; k < reps; ++k)                                                                        \
+      hipLaunchKernelGGL((k_pcg_step1<TM, D, GV>), dim3(nb), dim3(BS), 0, s, A, x, x, (const double*)pcopy, y, \
+                         (const double*)part, 1, part, st);                                               \
+    return check_launch("spmv");                                                                          \
   }
   if (prec == 64) {
     SP(double, 1, 8) SP(double, 1, 16) SP(double, 2, 8) SP(double, 2, 16) SP(double, 3, 16) SP(double, 3, 32)

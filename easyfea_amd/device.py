@@ -128,7 +128,7 @@ class DeviceCsr:
         self.group = _lib.load().efea_default_group(cmap.dof_n, cmap.max_neighbours)
         self.values = torch.empty(self.nnz, dtype=torch.float64, device="cuda")
         self.values32: Optional[torch.Tensor] = None
-        self.spmv_scratch = torch.empty(_lib.load().efea_spmv_scratch_bytes(self.Nn, self.group), dtype=torch.uint8,
+        self.spmv_scratch = torch.empty(_lib.load().efea_spmv_scratch_bytes(self.Nn, self.dof_n), dtype=torch.uint8,
                                         device="cuda")
         nd = cmap.nPe * cmap.dof_n
         self.n_entries = Ne * nd * nd

@@ -79,12 +79,17 @@ int efea_cast_f64_f32(int64_t n, const double* src, float* dst, void* stream);
 /* y = K x on the canonical CSR value array indexed through node adjacency
  * (node_ptr, node_cols): rows of `known` DOFs (uint8 per DOF, may be NULL)
  * return 0.  precision selects FP64 or FP32 values; x, y are FP64.  Runs the
- * solver's own SpMV kernel; scratch: efea_spmv_scratch_bytes() device bytes.
+ * solver's own step-1 kernel (beta = 0); scratch: efea_spmv_scratch_bytes()
+ * device bytes, no initialisation needed.
  * Replaces the scipy products K[dofs] @ u of _Simu.Calc_Reaction (_simu.py:212-260)
  * and Calc_Energy (_simu.py:177-210). */
-size_t efea_spmv_scratch_bytes(int Nnode, int group);
+size_t efea_spmv_scratch_bytes(int Nnode, int dof_n);
 int efea_spmv(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr, const int32_t* node_cols,
               const void* values, const uint8_t* known, const double* x, double* y, void* scratch, void* stream);
+/* The same product launched `reps` times back to back (kernel timing). */
+int efea_spmv_repeat(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr,
+                     const int32_t* node_cols, const void* values, const uint8_t* known, const double* x, double* y,
+                     void* scratch, int reps, void* stream);
 
 /* Block-Jacobi setup: inverse of each node's dof_n x dof_n diagonal block of the
  * Dirichlet-reduced matrix, stored in `precision`. */
