@@ -300,9 +300,15 @@ class _Simu:
 
     def _solve_system(self, A, b_dev: torch.Tensor, x_dev: torch.Tensor, problemType) -> None:
         known_np, xk = self._dirichlet_state(problemType)
-        known = to_dev(known_np)
-        kmask = torch.from_numpy(known_np.astype(bool)).to("cuda")
-        x_dev[kmask] = to_dev(xk)[kmask]
+        key = (problemType, hash(known_np.tobytes()))
+        cache = self.__dict__.setdefault("_mask_cache", {})
+        if key not in cache:
+            idx = np.nonzero(known_np)[0]
+            cache.clear()
+            cache[key] = (to_dev(known_np), to_dev(idx), idx)
+        known, idx_dev, idx = cache[key]
+        if idx.size:
+            x_dev.index_copy_(0, idx_dev, to_dev(xk[idx]))
         t = time.perf_counter()
         info = solve(A, b_dev, x_dev, known, self._mode(), self._ws, self.tol_residual, self.tol_anorm,
                      self.replace_ratio, self.maxit, self.check_every)
@@ -492,6 +498,8 @@ class PhaseField(_Simu):
     def Need_Update(self, value: bool = True) -> None:
         self._updD = not value
         self._updU = not value
+        if value:
+            self.__dict__.pop("_dev_params", None)
 
     @property
     def displacement(self) -> np.ndarray:
@@ -503,7 +511,16 @@ class PhaseField(_Simu):
 
     def _material_args(self):
         mat = self.model.material
-        return to_dev(mat._E_array()), (1 if mat.isHeterogeneous else 0), mat.v, int(mat.planeStress)
+        c = self.__dict__.setdefault("_dev_params", {})
+        if "E" not in c:
+            c["E"] = to_dev(mat._E_array())
+        return c["E"], (1 if mat.isHeterogeneous else 0), mat.v, int(mat.planeStress)
+
+    def _Gc_dev(self):
+        c = self.__dict__.setdefault("_dev_params", {})
+        if "Gc" not in c:
+            c["Gc"] = to_dev(self.model._Gc_array())
+        return c["Gc"]
 
     # --------------------------------------------------------- assembly
     def _assemble_damage(self) -> None:
@@ -514,7 +531,7 @@ class PhaseField(_Simu):
         tm, nm = dev.tables(MatrixType.mass)
         tr, nr = dev.tables(MatrixType.rigi)
         E, Es, nu, ps = self._material_args()
-        Gc = to_dev(pfm._Gc_array())
+        Gc = self._Gc_dev()
         _lib.call("efea_pf_damage_ke", dev.code, dev.Ne, ptr(dev.connect), ptr(dev.coord), ptr(tm), nm, ptr(tr), nr,
                   ptr(self._u), ptr(self._H), ptr(E), Es, nu, ps, ptr(Gc), 1 if pfm.isHeterogeneous else 0,
                   pfm.l0, 0 if pfm.split == "Amor" else 1, 0 if pfm.regularization == "AT1" else 1,
@@ -668,8 +685,9 @@ class PhaseField(_Simu):
         A = self._device_K(problemType)
         x = self._u if problemType == "elastic" else self._d
         y = A.spmv(x)
-        dofs = self.Get_dofs(problemType) if dofs is None else np.asarray(dofs)
-        return y.cpu().numpy()[dofs]
+        if dofs is None:
+            return y.cpu().numpy()
+        return y[to_dev(np.asarray(dofs, dtype=np.int64))].cpu().numpy()
 
     def Detect_Damage(self, nodes, threshold: float = 1.0) -> bool:
         nodes = np.asarray(nodes, dtype=np.int64)

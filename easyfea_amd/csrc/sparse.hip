@@ -224,11 +224,13 @@ __device__ inline void pcg_close_iteration(PcgState* st, double rz_new, double r
 }
 
 // Step 1 of an iteration: close the previous iteration (beta from the r.z
-// partials of step 2 or of a replacement), form p = z + beta p_old on the
-// fly for every column it reads, store p for the rows it owns, q = A p
-// (masked rows) and per-block partials of p.q.  In mixed mode it pauses
-// the solve (done = 4) when the recursive residual has fallen below
-// delta * its running maximum.
+// partials of step 2 or of a replacement), form p = z + beta p_old for every
+// column it reads, store p for the rows it owns, q = A p (masked rows) and
+// per-block partials of p.q.  In mixed mode it pauses the solve (done = 4)
+// when the recursive residual has fallen below delta * its running maximum.
+// Latency: the partial loads and the first node group's products (A z and
+// A p_old, kept apart because beta is not known yet) are issued before the
+// block reduction that yields beta.
 template <typename TM, int DN, int G>
 __global__ void __launch_bounds__(BS)
 k_pcg_step1(SpmvArgs A, const double* __restrict__ z, const double* __restrict__ P0, const double* __restrict__ P1,
@@ -236,13 +238,59 @@ k_pcg_step1(SpmvArgs A, const double* __restrict__ z, const double* __restrict__
             PcgState* st) {
   if (st->done) return;
   __shared__ double sh[BS / 64];
+  __shared__ double sh2[BS / 64];
+  __shared__ double bc[2];
   const int par = st->par;
   const double* __restrict__ pold = par ? P1 : P0;
   double* __restrict__ pnew = (double*)(par ? P0 : P1);
+  const bool fresh = st->fresh;
+  const TM* __restrict__ vals = (const TM*)A.vals;
+  double prz = 0, prr = 0;
+  if (!fresh)
+    for (int i = threadIdx.x; i < nb2; i += BS) {
+      prz += part2[2 * i];
+      prr += part2[2 * i + 1];
+    }
+  const int64_t lanes = (int64_t)A.Nnode * G;
+  int64_t gid = (int64_t)blockIdx.x * BS + threadIdx.x;
+  const bool have = gid < lanes;
+  const int n0 = (int)(gid / G), g0 = (int)(gid % G);
+  double sz[DN], sp[DN];
+#pragma unroll
+  for (int a = 0; a < DN; ++a) sz[a] = sp[a] = 0;
+  if (have) {
+    const int p0 = A.node_ptr[n0], cnt = A.node_ptr[n0 + 1] - p0;
+    const int64_t base = (int64_t)DN * DN * p0;
+    for (int pos = g0; pos < cnt; pos += G) {
+      const int c = A.node_cols[p0 + pos];
+      double zc[DN], pc[DN];
+#pragma unroll
+      for (int bb = 0; bb < DN; ++bb) {
+        zc[bb] = z[(int64_t)c * DN + bb];
+        pc[bb] = pold[(int64_t)c * DN + bb];
+      }
+#pragma unroll
+      for (int a = 0; a < DN; ++a) {
+        const TM* v = vals + base + (int64_t)a * DN * cnt + (int64_t)pos * DN;
+#pragma unroll
+        for (int bb = 0; bb < DN; ++bb) {
+          const double vv = (double)v[bb];
+          sz[a] += vv * zc[bb];
+          sp[a] += vv * pc[bb];
+        }
+      }
+    }
+  }
   double beta = 0;
-  if (!st->fresh) {
-    const double rz_new = reduce_all(part2, nb2, 2, 0, sh);
-    const double rr = reduce_all(part2, nb2, 2, 1, sh);
+  if (!fresh) {
+    prz = block_sum<double, BS>(prz, sh);
+    prr = block_sum<double, BS>(prr, sh2);
+    if (threadIdx.x == 0) {
+      bc[0] = prz;
+      bc[1] = prr;
+    }
+    __syncthreads();
+    const double rz_new = bc[0], rr = bc[1];
     const bool jr = st->justrep;
     if (!jr && st->delta2 > 0 && rr < st->delta2 * st->rrmax_prev) {
       if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 4;
@@ -252,10 +300,30 @@ k_pcg_step1(SpmvArgs A, const double* __restrict__ z, const double* __restrict__
     if (blockIdx.x == 0 && threadIdx.x == 0) pcg_close_iteration(st, rz_new, rr, jr);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) st->par_next = 1 - par;
-  const TM* __restrict__ vals = (const TM*)A.vals;
   double d = 0;
-  const int64_t lanes = (int64_t)A.Nnode * G;
-  for (int64_t gid = (int64_t)blockIdx.x * BS + threadIdx.x; gid < lanes; gid += (int64_t)gridDim.x * BS) {
+  if (have) {
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) {
+        sz[a] += __shfl_xor(sz[a], o, G);
+        sp[a] += __shfl_xor(sp[a], o, G);
+      }
+    }
+    if (g0 == 0) {
+#pragma unroll
+      for (int a = 0; a < DN; ++a) {
+        const int64_t r = (int64_t)n0 * DN + a;
+        const double pn = z[r] + beta * pold[r];
+        pnew[r] = pn;
+        double qv = sz[a] + beta * sp[a];
+        if (A.known && A.known[r]) qv = 0;
+        q[r] = qv;
+        d += pn * qv;
+      }
+    }
+  }
+  for (gid += (int64_t)gridDim.x * BS; gid < lanes; gid += (int64_t)gridDim.x * BS) {
     const int n = (int)(gid / G), g = (int)(gid % G);
     const int p0 = A.node_ptr[n], cnt = A.node_ptr[n + 1] - p0;
     const int64_t base = (int64_t)DN * DN * p0;
@@ -297,7 +365,8 @@ k_pcg_step1(SpmvArgs A, const double* __restrict__ z, const double* __restrict__
 }
 
 // Step 2: alpha = rz / pq ; x += alpha p ; r -= alpha q ; z = Minv r ;
-// partials of r.z and r.r.
+// partials of r.z and r.r.  The first node's operands are loaded before the
+// reduction that yields alpha.
 template <typename TP, int DN>
 __global__ void __launch_bounds__(BS)
 k_pcg_step2(int Nnode, const unsigned char* __restrict__ known, const TP* __restrict__ Minv,
@@ -307,15 +376,35 @@ k_pcg_step2(int Nnode, const unsigned char* __restrict__ known, const TP* __rest
   if (st->done) return;
   __shared__ double sh[BS / 64];
   __shared__ double sh2[BS / 64];
-  const double pq = reduce_all(part1, nb1, 1, 0, sh);
+  __shared__ double bc;
+  double ppq = 0;
+  for (int i = threadIdx.x; i < nb1; i += BS) ppq += part1[i];
+  const int pn = st->par_next;
+  const double* __restrict__ p = pn ? P1 : P0;
+  int n = blockIdx.x * BS + threadIdx.x;
+  double pv[DN], qv[DN], rv[DN], xv[DN], Mv[DN * DN];
+  if (n < Nnode) {
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      const int64_t i = (int64_t)n * DN + a;
+      pv[a] = p[i];
+      qv[a] = q[i];
+      rv[a] = r[i];
+      xv[a] = x[i];
+    }
+#pragma unroll
+    for (int k = 0; k < DN * DN; ++k) Mv[k] = (double)Minv[(int64_t)n * DN * DN + k];
+  }
+  ppq = block_sum<double, BS>(ppq, sh);
+  if (threadIdx.x == 0) bc = ppq;
+  __syncthreads();
+  const double pq = bc;
   const double rz = st->rz;
   if (!(pq > 0) || !(rz > 0)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) st->done = (rz == 0) ? 1 : 2;  // r = 0 exactly: converged
     return;
   }
   const double al = rz / pq;
-  const int pn = st->par_next;
-  const double* __restrict__ p = pn ? P1 : P0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->alpha = al;
     st->pq = pq;
@@ -326,21 +415,41 @@ k_pcg_step2(int Nnode, const unsigned char* __restrict__ known, const TP* __rest
     st->par = pn;
   }
   double s_rz = 0, s_rr = 0;
-  for (int n = blockIdx.x * BS + threadIdx.x; n < Nnode; n += gridDim.x * BS) {
-    double rv[DN], zv[DN];
+  if (n < Nnode) {
+    double r2[DN];
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      const int64_t i = (int64_t)n * DN + a;
+      x[i] = xv[a] + al * pv[a];
+      r2[a] = rv[a] - al * qv[a];
+      r[i] = r2[a];
+    }
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      double zz = 0;
+#pragma unroll
+      for (int b = 0; b < DN; ++b) zz += Mv[a * DN + b] * r2[b];
+      if (known && known[(int64_t)n * DN + a]) zz = 0;
+      z[(int64_t)n * DN + a] = zz;
+      s_rz += r2[a] * zz;
+      s_rr += r2[a] * r2[a];
+    }
+  }
+  for (n += gridDim.x * BS; n < Nnode; n += gridDim.x * BS) {
+    double r2[DN], zv[DN];
 #pragma unroll
     for (int a = 0; a < DN; ++a) {
       const int64_t i = (int64_t)n * DN + a;
       x[i] += al * p[i];
-      rv[a] = r[i] - al * q[i];
-      r[i] = rv[a];
+      r2[a] = r[i] - al * q[i];
+      r[i] = r2[a];
     }
-    apply_bj<TP, DN>(Minv, known, n, rv, zv);
+    apply_bj<TP, DN>(Minv, known, n, r2, zv);
 #pragma unroll
     for (int a = 0; a < DN; ++a) {
       z[(int64_t)n * DN + a] = zv[a];
-      s_rz += rv[a] * zv[a];
-      s_rr += rv[a] * rv[a];
+      s_rz += r2[a] * zv[a];
+      s_rr += r2[a] * r2[a];
     }
   }
   s_rz = block_sum<double, BS>(s_rz, sh);
@@ -352,8 +461,10 @@ k_pcg_step2(int Nnode, const unsigned char* __restrict__ known, const TP* __rest
 }
 
 // Reliable update (mixed mode, run by the host after a pause):
-// r = b - A64 x, z = Minv r, r.z / r.r partials in place of step 2's.
-template <typename TP, int DN, int G>
+// r = b - A64 x, z = Minv r, r.z / r.r partials in place of step 2's.  It
+// runs on step 2's node grid (one lane per node) so that its partials have
+// the layout step 1 reduces.
+template <typename TP, int DN>
 __global__ void __launch_bounds__(BS)
 k_pcg_replace(SpmvArgs A64, const TP* __restrict__ Minv, const double* __restrict__ x,
               const double* __restrict__ b, double* __restrict__ r, double* __restrict__ z,
@@ -361,22 +472,18 @@ k_pcg_replace(SpmvArgs A64, const TP* __restrict__ Minv, const double* __restric
   __shared__ double sh[BS / 64];
   __shared__ double sh2[BS / 64];
   double s_rz = 0, s_rr = 0;
-  const int64_t lanes = (int64_t)A64.Nnode * G;
-  for (int64_t gid = (int64_t)blockIdx.x * BS + threadIdx.x; gid < lanes; gid += (int64_t)gridDim.x * BS) {
-    const int n = (int)(gid / G), g = (int)(gid % G);
+  for (int n = blockIdx.x * BS + threadIdx.x; n < A64.Nnode; n += gridDim.x * BS) {
     double dd;
-    spmv_node<double, double, DN, G, 1>(A64, n, g, x, b, r, dd);
-    if (g == 0) {
-      double rv[DN], zv[DN];
+    spmv_node<double, double, DN, 1, 1>(A64, n, 0, x, b, r, dd);
+    double rv[DN], zv[DN];
 #pragma unroll
-      for (int a = 0; a < DN; ++a) rv[a] = r[(int64_t)n * DN + a];
-      apply_bj<TP, DN>(Minv, A64.known, n, rv, zv);
+    for (int a = 0; a < DN; ++a) rv[a] = r[(int64_t)n * DN + a];
+    apply_bj<TP, DN>(Minv, A64.known, n, rv, zv);
 #pragma unroll
-      for (int a = 0; a < DN; ++a) {
-        z[(int64_t)n * DN + a] = zv[a];
-        s_rz += rv[a] * zv[a];
-        s_rr += rv[a] * rv[a];
-      }
+    for (int a = 0; a < DN; ++a) {
+      z[(int64_t)n * DN + a] = zv[a];
+      s_rz += rv[a] * zv[a];
+      s_rr += rv[a] * rv[a];
     }
   }
   s_rz = block_sum<double, BS>(s_rz, sh);
@@ -656,8 +763,8 @@ static int pcg_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, cons
       return EFEA_ERR_HIP;
     }
     if (hs->done == 4) {  // mixed mode: reliable update, then continue
-      hipLaunchKernelGGL((k_pcg_replace<TP, DN, G>), dim3(nbS), dim3(BS), 0, s, A64, Minv, (const double*)x, b,
-                         w.r, w.z, w.part2, w.st);
+      hipLaunchKernelGGL((k_pcg_replace<TP, DN>), dim3(nbN), dim3(BS), 0, s, A64, Minv, (const double*)x, b, w.r,
+                         w.z, w.part2, w.st);
       if (int e = check_launch("pcg_replace")) return e;
       if (launched >= maxit) break;
       continue;
