@@ -168,6 +168,12 @@ class DeviceCsr:
         return m
 
 
+# The FP32 matrix copy pays only when the Krylov SpMV is bandwidth-bound:
+# measured step-1 kernel times are equal in FP32 and FP64 for matrices of a
+# few MB (latency floor) and 23% apart at 517 MB (64^3 HEXA8).
+MIXED_MIN_MATRIX_BYTES = 128 * 2**20
+
+
 @dataclass
 class SolveInfo:
     iterations: int
@@ -218,6 +224,10 @@ def solve(A: DeviceCsr, b: torch.Tensor, x: torch.Tensor, known: Optional[torch.
     initial guess (``SolveInfo.fallback``)."""
     assert mode in ("fp64", "mixed")
     m = 64 if mode == "fp64" else 32
+    if m == 32 and A.nnz * 8 < MIXED_MIN_MATRIX_BYTES:
+        # latency-bound system: the FP32 copy saves no time per iteration
+        # (the step kernels sit at their ~5 us floor) and costs iterations
+        m = 64
     x0 = x.clone() if m == 32 else None
     rc, si = _solve_once(A, b, x, known, m, ws, tol_residual, tol_anorm, replace_ratio, maxit, check_every)
     if rc != 0 and m == 32:

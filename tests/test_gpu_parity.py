@@ -23,6 +23,15 @@ from easyfea_amd._utils import MatrixType  # noqa: E402
 from oracle import efea_oracle as O  # noqa: E402
 
 
+@pytest.fixture(autouse=True)
+def _force_fp32_krylov(monkeypatch):
+    """Small test systems sit below the size where hip_mixed switches to FP32
+    Krylov products; force that path so that it is exercised."""
+    import easyfea_amd.device as dv
+
+    monkeypatch.setattr(dv, "MIXED_MIN_MATRIX_BYTES", 0)
+
+
 def _rel(a, b):
     a, b = np.asarray(a), np.asarray(b)
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
