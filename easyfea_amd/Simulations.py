@@ -26,7 +26,7 @@ import torch
 
 from . import _lib
 from ._utils import ElemType, MatrixType
-from .device import DeviceMesh, Workspace, ptr, solve, stream_ptr, to_dev
+from .device import DeviceMesh, SolveJob, Workspace, drive, ptr, stream_ptr, to_dev
 from .mesh import Mesh
 from .Models import Isotropic
 from .Models import PhaseField as PhaseFieldModel
@@ -298,7 +298,9 @@ class _Simu:
             xk[od] = np.where(np.isin(od, dofs), xk[od], 0.0)
         return known, xk
 
-    def _solve_system(self, A, b_dev: torch.Tensor, x_dev: torch.Tensor, problemType) -> None:
+    def _make_job(self, A, b_dev: torch.Tensor, x_dev: torch.Tensor, problemType) -> SolveJob:
+        """The solve of ``problemType``: Dirichlet values written into
+        ``x_dev`` at the known DOFs, the known mask on the device."""
         known_np, xk = self._dirichlet_state(problemType)
         key = (problemType, hash(known_np.tobytes()))
         cache = self.__dict__.setdefault("_mask_cache", {})
@@ -309,13 +311,25 @@ class _Simu:
         known, idx_dev, idx = cache[key]
         if idx.size:
             x_dev.index_copy_(0, idx_dev, to_dev(xk[idx]))
+        return SolveJob(A, b_dev, x_dev, known, self._mode(), self.tol_residual, self.tol_anorm,
+                        self.replace_ratio, self.maxit, self.check_every, owner=self)
+
+    def _solve_system_gen(self, A, b_dev: torch.Tensor, x_dev: torch.Tensor, problemType):
+        """Generator form of a solve: yields the ``SolveJob`` and receives its
+        ``SolveInfo`` (from the single-system path or a batch executor)."""
+        job = self._make_job(A, b_dev, x_dev, problemType)
         t = time.perf_counter()
-        info = solve(A, b_dev, x_dev, known, self._mode(), self._ws, self.tol_residual, self.tol_anorm,
-                     self.replace_ratio, self.maxit, self.check_every)
+        info = yield job
         self.last_solve[problemType] = info
+        if info.status != 1:
+            raise _lib.EfeaError("solve", _lib.ERR_NOCONV,
+                                 f"{problemType}: PCG status {info.status} after {info.iterations} iterations")
         if self._verbosity:
             print(f"{problemType}: {info.iterations} it, rel res {info.rel_residual:.2e} "
                   f"({time.perf_counter() - t:.3f} s)")
+
+    def _solve_system(self, A, b_dev: torch.Tensor, x_dev: torch.Tensor, problemType) -> None:
+        drive(self._solve_system_gen(A, b_dev, x_dev, problemType))
 
     # ------------------------------------------------------------ results
     def Save_Iter(self, iter: Optional[dict] = None) -> None:
@@ -405,12 +419,15 @@ class Elastic(_Simu):
     def Get_x0(self, problemType=None) -> np.ndarray:
         return self.displacement
 
-    def Solve(self) -> np.ndarray:
+    def _solve_gen(self):
         A = self._device_K()
         b = to_dev(self._neumann_vector("elastic"))
         x = self._u.clone()
-        self._solve_system(A, b, x, "elastic")
+        yield from self._solve_system_gen(A, b, x, "elastic")
         self._u = x
+
+    def Solve(self) -> np.ndarray:
+        drive(self._solve_gen())
         return self.displacement
 
     def Calc_Reaction(self, dofs=None, problemType=None) -> np.ndarray:
@@ -596,22 +613,24 @@ class PhaseField(_Simu):
         return float(torch.dot(self._u, fu))
 
     # ------------------------------------------------------------ solves
-    def _solve_damage(self) -> None:
+    def _solve_damage_gen(self):
         Kd = self._device_K("damage")
         b = self._Fd + to_dev(self._neumann_vector("damage"))
         x = self._d.clone()
-        self._solve_system(Kd, b, x, "damage")
+        yield from self._solve_system_gen(Kd, b, x, "damage")
         self._d = x
 
-    def _solve_elastic(self) -> None:
+    def _solve_elastic_gen(self):
         Ku = self._device_K("elastic")
         b = to_dev(self._neumann_vector("elastic"))
         x = self._u.clone()
-        self._solve_system(Ku, b, x, "elastic")
+        yield from self._solve_system_gen(Ku, b, x, "elastic")
         self._u = x
 
-    def Solve(self, tolConv=1.0, maxIter=500, convOption=2):
-        """Staggered scheme (``_phasefield.py:303-432``): returns ``(u, d, converged)``."""
+    def _staggered(self, tolConv=1.0, maxIter=500, convOption=2):
+        """The staggered loop as a generator of solve jobs; returns
+        ``converged``.  ``Solve`` runs it on the single-system path, the
+        serving loop on a batch executor."""
         assert 0 < tolConv <= 1, "tolConv must be between 0 and 1."
         assert maxIter > 1, "Must be > 1."
         Niter, converged, convIter = 0, False, 0.0
@@ -625,9 +644,9 @@ class PhaseField(_Simu):
                 E_n = self._Calc_Psi_Crack()
             elif convOption == 2:
                 E_n = self._Calc_Psi_Crack() + self._Calc_Psi_Elas() - self._Calc_Psi_Ext(fu)
-            self._solve_damage()
+            yield from self._solve_damage_gen()
             self._updU = False
-            self._solve_elastic()
+            yield from self._solve_elastic_gen()
             self._updD = False
             d_np1, u_np1 = self._d, self._u
             if convOption == 0:
@@ -655,6 +674,11 @@ class PhaseField(_Simu):
         self.Niter_staggered = Niter
         self.convIter = convIter
         self.timeIter = time.perf_counter() - t0
+        return converged
+
+    def Solve(self, tolConv=1.0, maxIter=500, convOption=2):
+        """Staggered scheme (``_phasefield.py:303-432``): returns ``(u, d, converged)``."""
+        converged = drive(self._staggered(tolConv, maxIter, convOption))
         return self.displacement, self.damage, converged
 
     def Save_Iter(self, iter: Optional[dict] = None) -> None:

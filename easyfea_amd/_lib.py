@@ -35,6 +35,12 @@ SIGNATURES = {
         i32,
         [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, f64, f64, f64, i32, i32, vp, vp],
     ),
+    "efea_batch_create": (i32, [i32, i32, i32, i32, C.POINTER(vp)]),
+    "efea_batch_destroy": (i32, [vp]),
+    "efea_batch_add": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, f64, f64, f64, i32, vp]),
+    "efea_batch_run": (i32, [vp, i32, vp]),
+    "efea_batch_pop": (i32, [vp, C.POINTER(i32), vp]),
+    "efea_batch_active": (i32, [vp]),
     "efea_dot": (i32, [i64, vp, vp, vp, vp, vp]),
     "efea_gather_sum": (i32, [i64, vp, vp, vp, vp]),
     "efea_pf_damage_ke": (

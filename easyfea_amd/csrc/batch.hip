@@ -1,0 +1,361 @@
+// Continuous-batching PCG: many independent systems advance together, one
+// launch per PCG step for all of them.
+//
+// Each system keeps the grid it would get alone (grid_for(Nnode*G) blocks for
+// the node-group kernel, grid_for(Nnode) for the node kernel); a launch
+// concatenates those grids and a block->system map sends every block to its
+// system's descriptor.  The per-block code is the single-system code
+// (pcg_device.h), so every system's reductions run in the same order as in
+// a solo solve and the iterates are bitwise the ones efea_solve_pcg computes.
+// Systems join (efea_batch_add) and leave (efea_batch_pop) between chunks of
+// iterations; a system that has converged idles (its blocks return at once)
+// until the next poll retires it.
+#include "pcg_device.h"
+
+#include <vector>
+
+namespace efea {
+
+int pcg_finish_info(const void* state, double rr_true, double* info);
+
+struct BatchSys {
+  SpmvArgs A64;         // FP64 values: residuals and replacements
+  const void* valsK;    // values of the Krylov products (FP32 copy in mode 32)
+  const void* Minv;
+  const double* b;
+  double* x;
+  double* r;
+  double* z;
+  double* P0;
+  double* P1;
+  double* q;
+  double* part1;
+  double* part2;
+  PcgState* st;
+  int blkS0, nbS;  // block range in the node-group grid
+  int blkN0, nbN;  // block range in the node grid
+  int replace;     // set by the host at a poll: run a residual replacement
+  int pad_;
+};
+
+template <typename TM, int DN, int G>
+__global__ void __launch_bounds__(BS) kb_step1(const BatchSys* __restrict__ sys, const int* __restrict__ map) {
+  const BatchSys& S = sys[map[blockIdx.x]];
+  SpmvArgs AK = S.A64;
+  AK.vals = S.valsK;
+  pcg_step1_body<TM, DN, G>(AK, S.z, S.P0, S.P1, S.q, S.part2, S.nbN, S.part1, S.st, (int)blockIdx.x - S.blkS0,
+                            S.nbS);
+}
+
+template <typename TP, int DN>
+__global__ void __launch_bounds__(BS) kb_step2(const BatchSys* __restrict__ sys, const int* __restrict__ map) {
+  const BatchSys& S = sys[map[blockIdx.x]];
+  pcg_step2_body<TP, DN>(S.A64.Nnode, S.A64.known, (const TP*)S.Minv, S.x, S.r, S.z, S.P0, S.P1, S.q, S.part1,
+                         S.nbS, S.part2, S.st, (int)blockIdx.x - S.blkN0, S.nbN);
+}
+
+template <typename TP, int DN>
+__global__ void __launch_bounds__(BS) kb_replace(const BatchSys* __restrict__ sys, const int* __restrict__ map) {
+  const BatchSys& S = sys[map[blockIdx.x]];
+  if (!S.replace) return;
+  pcg_replace_body<TP, DN>(S.A64, (const TP*)S.Minv, S.x, S.b, S.r, S.z, S.part2, S.st,
+                           (int)blockIdx.x - S.blkN0, S.nbN);
+}
+
+// One block per system: copies its state into a contiguous array for a
+// single device-to-host read.
+__global__ void kb_gather_state(const BatchSys* __restrict__ sys, PcgState* __restrict__ out) {
+  const int* src = (const int*)sys[blockIdx.x].st;
+  int* dst = (int*)(out + blockIdx.x);
+  for (int k = threadIdx.x; k < (int)(sizeof(PcgState) / 4); k += blockDim.x) dst[k] = src[k];
+}
+
+struct BatchEntry {
+  int id;
+  BatchSys d;
+  PcgWork w;
+  int launched, maxit;
+  bool finished;
+  double info[4];
+};
+
+struct Batch {
+  int mode, dof_n, group, cap;
+  int next_id = 0;
+  std::vector<BatchEntry> active;
+  std::vector<BatchEntry> done;
+  BatchSys* h_sys[2] = {nullptr, nullptr};
+  int flip = 0;
+  BatchSys* d_sys = nullptr;
+  int* h_map = nullptr;  // [cap*MAXB node-group blocks | cap*MAXB node blocks]
+  int* d_map = nullptr;
+  PcgState* h_st = nullptr;
+  PcgState* d_st = nullptr;
+  double* h_rr = nullptr;
+  int totS = 0, totN = 0;
+  bool dirty = true;
+};
+
+static void batch_free(Batch* b) {
+  if (!b) return;
+  for (int i = 0; i < 2; ++i)
+    if (b->h_sys[i]) (void)hipHostFree(b->h_sys[i]);
+  if (b->h_map) (void)hipHostFree(b->h_map);
+  if (b->h_st) (void)hipHostFree(b->h_st);
+  if (b->h_rr) (void)hipHostFree(b->h_rr);
+  if (b->d_sys) (void)hipFree(b->d_sys);
+  if (b->d_map) (void)hipFree(b->d_map);
+  if (b->d_st) (void)hipFree(b->d_st);
+  delete b;
+}
+
+static bool supported(int dof_n, int group) {
+  return (dof_n == 1 && (group == 8 || group == 16)) || (dof_n == 2 && (group == 8 || group == 16)) ||
+         (dof_n == 3 && (group == 16 || group == 32));
+}
+
+int batch_create(int mode, int dof_n, int group, int capacity, void** out) {
+  if (mode != 64 && mode != 32) {
+    set_error("batch_create: mode %d (64 or 32)", mode);
+    return EFEA_ERR_ARG;
+  }
+  if (!supported(dof_n, group)) {
+    set_error("batch_create: unsupported dof_n %d / group %d", dof_n, group);
+    return EFEA_ERR_ARG;
+  }
+  if (capacity < 1 || capacity > 1024) {
+    set_error("batch_create: capacity %d outside [1, 1024]", capacity);
+    return EFEA_ERR_ARG;
+  }
+  Batch* b = new Batch();
+  b->mode = mode;
+  b->dof_n = dof_n;
+  b->group = group;
+  b->cap = capacity;
+  const size_t nmap = 2 * (size_t)capacity * MAXB;
+  bool ok = hipHostMalloc((void**)&b->h_sys[0], capacity * sizeof(BatchSys), hipHostMallocDefault) == hipSuccess &&
+            hipHostMalloc((void**)&b->h_sys[1], capacity * sizeof(BatchSys), hipHostMallocDefault) == hipSuccess &&
+            hipHostMalloc((void**)&b->h_map, nmap * sizeof(int), hipHostMallocDefault) == hipSuccess &&
+            hipHostMalloc((void**)&b->h_st, capacity * sizeof(PcgState), hipHostMallocDefault) == hipSuccess &&
+            hipHostMalloc((void**)&b->h_rr, capacity * sizeof(double), hipHostMallocDefault) == hipSuccess &&
+            hipMalloc((void**)&b->d_sys, capacity * sizeof(BatchSys)) == hipSuccess &&
+            hipMalloc((void**)&b->d_map, nmap * sizeof(int)) == hipSuccess &&
+            hipMalloc((void**)&b->d_st, capacity * sizeof(PcgState)) == hipSuccess;
+  if (!ok) {
+    batch_free(b);
+    set_error("batch_create: allocation failed (capacity %d)", capacity);
+    return EFEA_ERR_HIP;
+  }
+  *out = b;
+  return EFEA_OK;
+}
+
+int batch_destroy(void* h) {
+  batch_free((Batch*)h);
+  return EFEA_OK;
+}
+
+template <typename TP, int DN, int G>
+static int start_dispatch(const SpmvArgs& A64, const void* Minv, const double* b, const double* x,
+                          const PcgWork& w, double tol_r, double tol_A, double delta2, hipStream_t s) {
+  return pcg_start<TP, DN, G>(A64, (const TP*)Minv, b, x, w, tol_r, tol_A, delta2, s);
+}
+
+int batch_add(void* h, int Nnode, const int* node_ptr, const int* node_cols, const double* vals64,
+              const float* vals32, const unsigned char* known, const void* Minv, const double* b, double* x,
+              void* ws, size_t ws_bytes, double tol_r, double tol_A, double delta, int maxit, hipStream_t s) {
+  Batch* B = (Batch*)h;
+  if (ws_bytes < pcg_workspace_size(Nnode, B->dof_n)) {
+    set_error("batch_add: workspace of %zu bytes, %zu needed", ws_bytes, pcg_workspace_size(Nnode, B->dof_n));
+    return EFEA_ERR_SIZE;
+  }
+  if ((int)B->active.size() >= B->cap) {
+    set_error("batch_add: batch full (%d systems)", B->cap);
+    return EFEA_ERR_SIZE;
+  }
+  if (B->mode == 32 && !vals32) {
+    set_error("batch_add: mode 32 needs the FP32 value copy");
+    return EFEA_ERR_ARG;
+  }
+  BatchEntry e{};
+  e.id = B->next_id++;
+  e.w = carve(ws, Nnode, B->dof_n);
+  e.launched = 0;
+  e.maxit = maxit;
+  e.finished = false;
+  BatchSys& d = e.d;
+  d.A64 = SpmvArgs{Nnode, node_ptr, node_cols, vals64, known};
+  d.valsK = B->mode == 32 ? (const void*)vals32 : (const void*)vals64;
+  d.Minv = Minv;
+  d.b = b;
+  d.x = x;
+  d.r = e.w.r;
+  d.z = e.w.z;
+  d.P0 = e.w.P0;
+  d.P1 = e.w.P1;
+  d.q = e.w.q;
+  d.part1 = e.w.part1;
+  d.part2 = e.w.part2;
+  d.st = e.w.st;
+  d.nbS = (int)grid_for((int64_t)Nnode * B->group);
+  d.nbN = (int)grid_for(Nnode);
+  d.replace = 0;
+  const double delta2 = B->mode == 32 ? delta * delta : 0.0;
+  int rc = EFEA_ERR_ARG;
+#define ST(D, GV)                                                                                          \
+  if (B->dof_n == D && B->group == GV)                                                                    \
+    rc = B->mode == 64 ? start_dispatch<double, D, GV>(d.A64, Minv, b, x, e.w, tol_r, tol_A, delta2, s)   \
+                       : start_dispatch<float, D, GV>(d.A64, Minv, b, x, e.w, tol_r, tol_A, delta2, s);
+  ST(1, 8) ST(1, 16) ST(2, 8) ST(2, 16) ST(3, 16) ST(3, 32)
+#undef ST
+  if (rc) return rc;
+  B->active.push_back(e);
+  B->dirty = true;
+  return e.id;
+}
+
+static int upload_sys(Batch* B, hipStream_t s) {
+  // two pinned buffers used in turn: the previous upload from this buffer
+  // completed before the stream synchronisation of the last poll
+  BatchSys* hs = B->h_sys[B->flip];
+  B->flip ^= 1;
+  for (size_t i = 0; i < B->active.size(); ++i) hs[i] = B->active[i].d;
+  if (hipMemcpyAsync(B->d_sys, hs, B->active.size() * sizeof(BatchSys), hipMemcpyHostToDevice, s) != hipSuccess) {
+    set_error("batch: descriptor upload failed");
+    return EFEA_ERR_HIP;
+  }
+  return EFEA_OK;
+}
+
+static int rebuild(Batch* B, hipStream_t s) {
+  int* mapS = B->h_map;
+  int* mapN = B->h_map + (size_t)B->cap * MAXB;
+  int tS = 0, tN = 0;
+  for (size_t i = 0; i < B->active.size(); ++i) {
+    BatchSys& d = B->active[i].d;
+    d.blkS0 = tS;
+    d.blkN0 = tN;
+    for (int k = 0; k < d.nbS; ++k) mapS[tS + k] = (int)i;
+    for (int k = 0; k < d.nbN; ++k) mapN[tN + k] = (int)i;
+    tS += d.nbS;
+    tN += d.nbN;
+  }
+  B->totS = tS;
+  B->totN = tN;
+  if (!B->active.empty() &&
+      (hipMemcpyAsync(B->d_map, mapS, tS * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
+       hipMemcpyAsync(B->d_map + (size_t)B->cap * MAXB, mapN, tN * sizeof(int), hipMemcpyHostToDevice, s) !=
+           hipSuccess)) {
+    set_error("batch: map upload failed");
+    return EFEA_ERR_HIP;
+  }
+  B->dirty = false;
+  return upload_sys(B, s);
+}
+
+template <typename TM, typename TP, int DN, int G>
+static int batch_run_t(Batch* B, int iters, hipStream_t s) {
+  const int n = (int)B->active.size();
+  const int* mapS = B->d_map;
+  const int* mapN = B->d_map + (size_t)B->cap * MAXB;
+  for (int k = 0; k < iters; ++k) {
+    hipLaunchKernelGGL((kb_step1<TM, DN, G>), dim3(B->totS), dim3(BS), 0, s, (const BatchSys*)B->d_sys, mapS);
+    hipLaunchKernelGGL((kb_step2<TP, DN>), dim3(B->totN), dim3(BS), 0, s, (const BatchSys*)B->d_sys, mapN);
+  }
+  hipLaunchKernelGGL(kb_gather_state, dim3(n), dim3(64), 0, s, (const BatchSys*)B->d_sys, B->d_st);
+  if (int e = check_launch("batch_iter")) return e;
+  if (hipMemcpyAsync(B->h_st, B->d_st, n * sizeof(PcgState), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    set_error("batch: state readback failed");
+    return EFEA_ERR_HIP;
+  }
+  bool any_replace = false;
+  int nfin = 0;
+  for (int i = 0; i < n; ++i) {
+    BatchEntry& e = B->active[i];
+    const PcgState& st = B->h_st[i];
+    e.launched += iters;
+    e.d.replace = st.done == 4;
+    any_replace |= st.done == 4;
+    e.finished = (st.done != 0 && st.done != 4) || e.launched >= e.maxit;
+    nfin += e.finished;
+  }
+  if (any_replace) {
+    if (int e = upload_sys(B, s)) return e;
+    hipLaunchKernelGGL((kb_replace<TP, DN>), dim3(B->totN), dim3(BS), 0, s, (const BatchSys*)B->d_sys, mapN);
+    if (int e = check_launch("batch_replace")) return e;
+    for (int i = 0; i < n; ++i) B->active[i].d.replace = 0;
+  }
+  if (nfin == 0) return 0;
+  int j = 0;
+  for (int i = 0; i < n; ++i) {
+    BatchEntry& e = B->active[i];
+    if (!e.finished) continue;
+    pcg_true_residual<DN, G>(e.d.A64, e.d.b, e.d.x, e.w, s);
+    if (hipMemcpyAsync(B->h_rr + j, e.w.small, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) {
+      set_error("batch: residual readback failed");
+      return EFEA_ERR_HIP;
+    }
+    ++j;
+  }
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    set_error("batch: residual readback failed");
+    return EFEA_ERR_HIP;
+  }
+  std::vector<BatchEntry> keep;
+  keep.reserve(n);
+  j = 0;
+  for (int i = 0; i < n; ++i) {
+    BatchEntry& e = B->active[i];
+    if (!e.finished) {
+      keep.push_back(e);
+      continue;
+    }
+    (void)pcg_finish_info(&B->h_st[i], B->h_rr[j++], e.info);
+    B->done.push_back(e);
+  }
+  B->active.swap(keep);
+  B->dirty = true;
+  return nfin;
+}
+
+// Runs `iters` iterations of every active system, then polls: systems that
+// converged, failed or reached maxit move to the finished list.  Returns the
+// number of systems that finished in this call.
+int batch_run(void* h, int iters, hipStream_t s) {
+  Batch* B = (Batch*)h;
+  if (B->active.empty()) return 0;
+  if (iters < 1) iters = 1;
+  if (B->dirty) {
+    if (int e = rebuild(B, s)) return e;
+  }
+#define RN(D, GV)                                                                             \
+  if (B->dof_n == D && B->group == GV)                                                       \
+    return B->mode == 64 ? batch_run_t<double, double, D, GV>(B, iters, s)                   \
+                         : batch_run_t<float, float, D, GV>(B, iters, s);
+  RN(1, 8) RN(1, 16) RN(2, 8) RN(2, 16) RN(3, 16) RN(3, 32)
+#undef RN
+  set_error("batch_run: unsupported configuration");
+  return EFEA_ERR_ARG;
+}
+
+// Pops one finished system: its id and info[0..3] (iterations, relative
+// residual, replacements, status: 1 converged, 0 stopped at maxit, 2
+// breakdown, 3 diverged).  Returns 0, or 1 with *id = -1 when nothing has
+// finished.
+int batch_pop(void* h, int* id, double* info) {
+  Batch* B = (Batch*)h;
+  if (B->done.empty()) {
+    *id = -1;
+    return 1;
+  }
+  const BatchEntry e = B->done.front();
+  B->done.erase(B->done.begin());
+  *id = e.id;
+  for (int k = 0; k < 4; ++k) info[k] = e.info[k];
+  return EFEA_OK;
+}
+
+int batch_active(void* h) { return (int)((Batch*)h)->active.size(); }
+
+}  // namespace efea

@@ -53,6 +53,13 @@ int pf_elastic(int, int64_t, const int*, const double*, const double*, int, cons
                const double*, int, double, int, int, double, double*, hipStream_t);
 int pf_split_points(int64_t, int, const double*, const double*, int, double, int, double*, double*, double*,
                     double*, hipStream_t);
+int batch_create(int, int, int, int, void**);
+int batch_destroy(void*);
+int batch_add(void*, int, const int*, const int*, const double*, const float*, const unsigned char*, const void*,
+              const double*, double*, void*, size_t, double, double, double, int, hipStream_t);
+int batch_run(void*, int, hipStream_t);
+int batch_pop(void*, int*, double*);
+int batch_active(void*);
 
 }  // namespace efea
 
@@ -169,6 +176,60 @@ int efea_solve_pcg(int mode, int Nnode, int dof_n, int group, const int32_t* nod
   }
   return solve_pcg(mode, Nnode, dof_n, group, node_ptr, node_cols, values64, values32, known, Minv, b, x,
                    workspace, tol_residual, tol_anorm, replace_ratio, maxit, check_every, STREAM(stream), info);
+}
+
+int efea_batch_create(int mode, int dof_n, int group, int capacity, void** handle) {
+  if (!handle) {
+    set_error("efea_batch_create: null handle pointer");
+    return EFEA_ERR_ARG;
+  }
+  *handle = nullptr;
+  return batch_create(mode, dof_n, group, capacity, handle);
+}
+
+int efea_batch_destroy(void* handle) {
+  if (!handle) return EFEA_OK;
+  return batch_destroy(handle);
+}
+
+int efea_batch_add(void* handle, int Nnode, const int32_t* node_ptr, const int32_t* node_cols,
+                   const double* values64, const float* values32, const uint8_t* known, const void* Minv,
+                   const double* b, double* x, void* workspace, size_t workspace_bytes, double tol_residual,
+                   double tol_anorm, double replace_ratio, int maxit, void* stream) {
+  if (!handle || Nnode <= 0 || !node_ptr || !node_cols || !values64 || !Minv || !b || !x || !workspace) {
+    set_error("efea_batch_add: null pointer or empty system");
+    return EFEA_ERR_ARG;
+  }
+  if (maxit < 1) {
+    set_error("efea_batch_add: maxit %d", maxit);
+    return EFEA_ERR_ARG;
+  }
+  return batch_add(handle, Nnode, node_ptr, node_cols, values64, values32, known, Minv, b, x, workspace,
+                   workspace_bytes, tol_residual, tol_anorm, replace_ratio, maxit, STREAM(stream));
+}
+
+int efea_batch_run(void* handle, int iterations, void* stream) {
+  if (!handle) {
+    set_error("efea_batch_run: null handle");
+    return EFEA_ERR_ARG;
+  }
+  return batch_run(handle, iterations, STREAM(stream));
+}
+
+int efea_batch_pop(void* handle, int* id, double* info) {
+  if (!handle || !id || !info) {
+    set_error("efea_batch_pop: null pointer");
+    return EFEA_ERR_ARG;
+  }
+  return batch_pop(handle, id, info);
+}
+
+int efea_batch_active(void* handle) {
+  if (!handle) {
+    set_error("efea_batch_active: null handle");
+    return EFEA_ERR_ARG;
+  }
+  return batch_active(handle);
 }
 
 int efea_dot(int64_t n, const double* a, const double* b, double* scratch, double* out, void* stream) {

@@ -1,0 +1,488 @@
+// Device-side pieces of the PCG shared by the single-system kernels
+// (sparse.hip) and the continuous-batching kernels (batch.hip): the
+// node-block SpMV, fence-free reductions, the solver state and the bodies of
+// the two per-iteration kernels.  Each body takes the block's index and the
+// block count of its own system, so one launch can carry several systems,
+// each on the grid it would get alone (identical reduction order, identical
+// results).
+//
+// Matrix layout.  The global matrix is EasyFEA's scipy-canonical CSR
+// (sorted, duplicate-free columns, EasyFEA/Simulations/_simu.py:1062-1102).
+// Because every node couples to every DOF of each neighbouring node, row
+// (n, a) of that CSR holds, for each neighbour m of n in increasing order,
+// the dof_n columns m*dof_n + b.  The kernels therefore index the unchanged
+// CSR value array through node-level adjacency (node_ptr, node_cols):
+//   value(n, a, pos, b) = dof_n^2 * node_ptr[n] + a * dof_n * cnt(n) + pos * dof_n + b
+// which moves one int32 per dof_n^2 values instead of one per value.
+#pragma once
+#include "efea_common.h"
+
+namespace efea {
+
+struct SpmvArgs {
+  int Nnode;
+  const int* node_ptr;
+  const int* node_cols;
+  const void* vals;
+  const unsigned char* known;
+};
+
+// Node-block SpMV: G lanes cooperate on one node's dof_n rows.
+//   MODE 0: y = A x            (rows of known DOFs -> 0 when known != null)
+//   MODE 1: y = b - A x        (same masking)
+// dot_xy receives x.y (MODE 0) or y.y (MODE 1) over the node's rows.
+template <typename TM, typename TV, int DN, int G, int MODE>
+__device__ inline void spmv_node(const SpmvArgs& A, int n, int g, const TV* __restrict__ x,
+                                 const TV* __restrict__ b, TV* __restrict__ y, double& dot_xy) {
+  const TM* __restrict__ vals = (const TM*)A.vals;
+  const int p0 = A.node_ptr[n], cnt = A.node_ptr[n + 1] - p0;
+  const int64_t base = (int64_t)DN * DN * p0;
+  TV sum[DN];
+#pragma unroll
+  for (int a = 0; a < DN; ++a) sum[a] = 0;
+  for (int pos = g; pos < cnt; pos += G) {
+    const int c = A.node_cols[p0 + pos];
+    TV xc[DN];
+#pragma unroll
+    for (int bb = 0; bb < DN; ++bb) xc[bb] = x[(int64_t)c * DN + bb];
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      const TM* v = vals + base + (int64_t)a * DN * cnt + (int64_t)pos * DN;
+#pragma unroll
+      for (int bb = 0; bb < DN; ++bb) sum[a] += (TV)v[bb] * xc[bb];
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < DN; ++a) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) sum[a] += __shfl_xor(sum[a], o, G);
+  }
+  dot_xy = 0;
+  if (g == 0) {
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      const int64_t r = (int64_t)n * DN + a;
+      TV val = MODE == 1 ? b[r] - sum[a] : sum[a];
+      if (A.known && A.known[r]) val = 0;
+      y[r] = val;
+      dot_xy += (double)(MODE == 1 ? val : x[r]) * (double)val;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fence-free global reductions.
+//
+// CDNA4 splits the chip into XCDs whose L2 caches are not mutually coherent:
+// a device-scope fence is an L2 write-back, so a "last block finishes the
+// reduction" pattern (one fence per block) serialises on the write-backs.
+// Here every kernel runs a fixed grid of at most MAXB blocks per system
+// (grid-stride loops), writes one partial per block with a plain store, and
+// the *next* kernel reduces those partials, redundantly in every block and
+// in a fixed order (deterministic).  Visibility comes from the kernel
+// boundary: one release per dispatch instead of one per block.
+constexpr int BS = 256;
+constexpr int MAXB = 1024;
+
+static inline unsigned grid_for(int64_t threads) {
+  int64_t nb = (threads + BS - 1) / BS;
+  if (nb > MAXB) nb = MAXB;
+  if (nb < 1) nb = 1;
+  return (unsigned)nb;
+}
+
+// Sum of n partials (stride `stride`, offset `off`) in every thread of the block.
+__device__ inline double reduce_all(const double* __restrict__ part, int n, int stride, int off, double* sh) {
+  double s = 0;
+  for (int i = threadIdx.x; i < n; i += BS) s += part[stride * i + off];
+  s = block_sum<double, BS>(s, sh);
+  __shared__ double bc;
+  if (threadIdx.x == 0) bc = s;
+  __syncthreads();
+  const double r = bc;
+  __syncthreads();
+  return r;
+}
+
+// PCG device state.  Scalars stay FP64 in every mode.  The stopping test is
+// the Strakos-Tichy delayed estimate of the A-norm error: after k+RING
+// iterations ||x* - x_k||_A^2 ~ sum_{j=k}^{k+RING-1} alpha_j (r_j . z_j),
+// compared with max(sum_j alpha_j r_j . z_j, x0.A x0).
+// A field written by block 0 of a kernel is never read by the other blocks
+// of that kernel: rz_prev, rrmax_prev and par_next are per-iteration
+// snapshots that break those races.
+constexpr int RING = 32;
+struct PcgState {
+  double rz, rz_prev, alpha, beta, pq;
+  double rr, rrmax, rrmax_prev, bb;
+  double tolr2, tolA2, delta2;
+  double Sall, Sw, xAx0;
+  double ring[RING];
+  int head, done, iters, nrepl;
+  int fresh;     // no iteration to close yet: the first step uses beta = 0
+  int justrep;   // the r.z / r.r partials come from a residual replacement
+  int par;       // which of the two direction buffers holds p
+  int par_next;  // the buffer the current step writes
+};
+// done: 0 running, 1 converged, 2 breakdown, 3 diverged, 4 paused for a
+// residual replacement (mixed mode; the host runs the replacement kernel).
+
+template <typename TP, int DN>
+__device__ inline void apply_bj(const TP* __restrict__ Minv, const unsigned char* __restrict__ known, int n,
+                                const double (&rv)[DN], double (&zv)[DN]) {
+#pragma unroll
+  for (int a = 0; a < DN; ++a) {
+    double s = 0;
+#pragma unroll
+    for (int b = 0; b < DN; ++b) s += (double)Minv[(int64_t)n * DN * DN + a * DN + b] * rv[b];
+    if (known && known[(int64_t)n * DN + a]) s = 0;
+    zv[a] = s;
+  }
+}
+
+// Iteration bookkeeping, block 0 only.
+__device__ inline void pcg_close_iteration(PcgState* st, double rz_new, double rr, bool replaced) {
+  const double term = st->alpha * st->rz_prev;  // alpha_j (r_j . z_j)
+  st->Sall += term;
+  st->ring[st->head] = term;
+  st->head = (st->head + 1) % RING;
+  double Sw = 0;  // summed afresh: a running add/subtract would cancel catastrophically
+  for (int i = 0; i < RING; ++i) Sw += st->ring[i];
+  st->Sw = Sw;
+  st->iters += 1;
+  st->beta = rz_new / st->rz_prev;
+  st->rz = rz_new;
+  st->rr = rr;
+  st->rrmax = replaced ? rr : (rr > st->rrmax_prev ? rr : st->rrmax_prev);
+  const double scale = st->Sall > st->xAx0 ? st->Sall : st->xAx0;
+  if (st->iters >= RING && Sw <= st->tolA2 * scale) st->done = 1;
+  if (rr <= st->tolr2 * st->bb) st->done = 1;
+  if (!(rr == rr) || !(rz_new == rz_new)) st->done = 2;
+  if (replaced && rr > 1e12 * st->bb) st->done = 3;  // true residual exploded: diverged
+}
+
+// Step 1 of an iteration: close the previous iteration (beta from the r.z
+// partials of step 2 or of a replacement), form p = z + beta p_old for every
+// column it reads, store p for the rows it owns, q = A p (masked rows) and
+// per-block partials of p.q.  In mixed mode it pauses the solve (done = 4)
+// when the recursive residual has fallen below delta * its running maximum.
+// Latency: the partial loads and the first node group's products (A z and
+// A p_old, kept apart because beta is not known yet) are issued before the
+// block reduction that yields beta.
+template <typename TM, int DN, int G>
+__device__ inline void pcg_step1_body(const SpmvArgs& A, const double* __restrict__ z,
+                                      const double* __restrict__ P0, const double* __restrict__ P1,
+                                      double* __restrict__ q, const double* __restrict__ part2, int nb2,
+                                      double* __restrict__ part1, PcgState* st, int blk, int nblk) {
+  if (st->done) return;
+  __shared__ double sh[BS / 64];
+  __shared__ double sh2[BS / 64];
+  __shared__ double bc[2];
+  const int par = st->par;
+  const double* __restrict__ pold = par ? P1 : P0;
+  double* __restrict__ pnew = (double*)(par ? P0 : P1);
+  const bool fresh = st->fresh;
+  const TM* __restrict__ vals = (const TM*)A.vals;
+  double prz = 0, prr = 0;
+  if (!fresh)
+    for (int i = threadIdx.x; i < nb2; i += BS) {
+      prz += part2[2 * i];
+      prr += part2[2 * i + 1];
+    }
+  const int64_t lanes = (int64_t)A.Nnode * G;
+  int64_t gid = (int64_t)blk * BS + threadIdx.x;
+  const bool have = gid < lanes;
+  const int n0 = (int)(gid / G), g0 = (int)(gid % G);
+  double sz[DN], sp[DN];
+#pragma unroll
+  for (int a = 0; a < DN; ++a) sz[a] = sp[a] = 0;
+  if (have) {
+    const int p0 = A.node_ptr[n0], cnt = A.node_ptr[n0 + 1] - p0;
+    const int64_t base = (int64_t)DN * DN * p0;
+    for (int pos = g0; pos < cnt; pos += G) {
+      const int c = A.node_cols[p0 + pos];
+      double zc[DN], pc[DN];
+#pragma unroll
+      for (int bb = 0; bb < DN; ++bb) {
+        zc[bb] = z[(int64_t)c * DN + bb];
+        pc[bb] = pold[(int64_t)c * DN + bb];
+      }
+#pragma unroll
+      for (int a = 0; a < DN; ++a) {
+        const TM* v = vals + base + (int64_t)a * DN * cnt + (int64_t)pos * DN;
+#pragma unroll
+        for (int bb = 0; bb < DN; ++bb) {
+          const double vv = (double)v[bb];
+          sz[a] += vv * zc[bb];
+          sp[a] += vv * pc[bb];
+        }
+      }
+    }
+  }
+  double beta = 0;
+  if (!fresh) {
+    prz = block_sum<double, BS>(prz, sh);
+    prr = block_sum<double, BS>(prr, sh2);
+    if (threadIdx.x == 0) {
+      bc[0] = prz;
+      bc[1] = prr;
+    }
+    __syncthreads();
+    const double rz_new = bc[0], rr = bc[1];
+    const bool jr = st->justrep;
+    if (!jr && st->delta2 > 0 && rr < st->delta2 * st->rrmax_prev) {
+      if (blk == 0 && threadIdx.x == 0) st->done = 4;
+      return;
+    }
+    beta = rz_new / st->rz_prev;
+    if (blk == 0 && threadIdx.x == 0) pcg_close_iteration(st, rz_new, rr, jr);
+  }
+  if (blk == 0 && threadIdx.x == 0) st->par_next = 1 - par;
+  double d = 0;
+  if (have) {
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) {
+        sz[a] += __shfl_xor(sz[a], o, G);
+        sp[a] += __shfl_xor(sp[a], o, G);
+      }
+    }
+    if (g0 == 0) {
+#pragma unroll
+      for (int a = 0; a < DN; ++a) {
+        const int64_t r = (int64_t)n0 * DN + a;
+        const double pn = z[r] + beta * pold[r];
+        pnew[r] = pn;
+        double qv = sz[a] + beta * sp[a];
+        if (A.known && A.known[r]) qv = 0;
+        q[r] = qv;
+        d += pn * qv;
+      }
+    }
+  }
+  for (gid += (int64_t)nblk * BS; gid < lanes; gid += (int64_t)nblk * BS) {
+    const int n = (int)(gid / G), g = (int)(gid % G);
+    const int p0 = A.node_ptr[n], cnt = A.node_ptr[n + 1] - p0;
+    const int64_t base = (int64_t)DN * DN * p0;
+    double sum[DN];
+#pragma unroll
+    for (int a = 0; a < DN; ++a) sum[a] = 0;
+    for (int pos = g; pos < cnt; pos += G) {
+      const int c = A.node_cols[p0 + pos];
+      double xc[DN];
+#pragma unroll
+      for (int bb = 0; bb < DN; ++bb) xc[bb] = z[(int64_t)c * DN + bb] + beta * pold[(int64_t)c * DN + bb];
+#pragma unroll
+      for (int a = 0; a < DN; ++a) {
+        const TM* v = vals + base + (int64_t)a * DN * cnt + (int64_t)pos * DN;
+#pragma unroll
+        for (int bb = 0; bb < DN; ++bb) sum[a] += (double)v[bb] * xc[bb];
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) sum[a] += __shfl_xor(sum[a], o, G);
+    }
+    if (g == 0) {
+#pragma unroll
+      for (int a = 0; a < DN; ++a) {
+        const int64_t r = (int64_t)n * DN + a;
+        const double pn = z[r] + beta * pold[r];
+        pnew[r] = pn;
+        double qv = sum[a];
+        if (A.known && A.known[r]) qv = 0;
+        q[r] = qv;
+        d += pn * qv;
+      }
+    }
+  }
+  d = block_sum<double, BS>(d, sh);
+  if (threadIdx.x == 0) part1[blk] = d;
+}
+
+// Step 2: alpha = rz / pq ; x += alpha p ; r -= alpha q ; z = Minv r ;
+// partials of r.z and r.r.  The first node's operands are loaded before the
+// reduction that yields alpha.
+template <typename TP, int DN>
+__device__ inline void pcg_step2_body(int Nnode, const unsigned char* __restrict__ known,
+                                      const TP* __restrict__ Minv, double* __restrict__ x, double* __restrict__ r,
+                                      double* __restrict__ z, const double* __restrict__ P0,
+                                      const double* __restrict__ P1, const double* __restrict__ q,
+                                      const double* __restrict__ part1, int nb1, double* __restrict__ part2,
+                                      PcgState* st, int blk, int nblk) {
+  if (st->done) return;
+  __shared__ double sh[BS / 64];
+  __shared__ double sh2[BS / 64];
+  __shared__ double bc;
+  double ppq = 0;
+  for (int i = threadIdx.x; i < nb1; i += BS) ppq += part1[i];
+  const int pn = st->par_next;
+  const double* __restrict__ p = pn ? P1 : P0;
+  int n = blk * BS + threadIdx.x;
+  double pv[DN], qv[DN], rv[DN], xv[DN], Mv[DN * DN];
+  if (n < Nnode) {
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      const int64_t i = (int64_t)n * DN + a;
+      pv[a] = p[i];
+      qv[a] = q[i];
+      rv[a] = r[i];
+      xv[a] = x[i];
+    }
+#pragma unroll
+    for (int k = 0; k < DN * DN; ++k) Mv[k] = (double)Minv[(int64_t)n * DN * DN + k];
+  }
+  ppq = block_sum<double, BS>(ppq, sh);
+  if (threadIdx.x == 0) bc = ppq;
+  __syncthreads();
+  const double pq = bc;
+  const double rz = st->rz;
+  if (!(pq > 0) || !(rz > 0)) {
+    if (blk == 0 && threadIdx.x == 0) st->done = (rz == 0) ? 1 : 2;  // r = 0 exactly: converged
+    return;
+  }
+  const double al = rz / pq;
+  if (blk == 0 && threadIdx.x == 0) {
+    st->alpha = al;
+    st->pq = pq;
+    st->rz_prev = rz;
+    st->rrmax_prev = st->rrmax;
+    st->fresh = 0;
+    st->justrep = 0;
+    st->par = pn;
+  }
+  double s_rz = 0, s_rr = 0;
+  if (n < Nnode) {
+    double r2[DN];
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      const int64_t i = (int64_t)n * DN + a;
+      x[i] = xv[a] + al * pv[a];
+      r2[a] = rv[a] - al * qv[a];
+      r[i] = r2[a];
+    }
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      double zz = 0;
+#pragma unroll
+      for (int b = 0; b < DN; ++b) zz += Mv[a * DN + b] * r2[b];
+      if (known && known[(int64_t)n * DN + a]) zz = 0;
+      z[(int64_t)n * DN + a] = zz;
+      s_rz += r2[a] * zz;
+      s_rr += r2[a] * r2[a];
+    }
+  }
+  for (n += nblk * BS; n < Nnode; n += nblk * BS) {
+    double r2[DN], zv[DN];
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      const int64_t i = (int64_t)n * DN + a;
+      x[i] += al * p[i];
+      r2[a] = r[i] - al * q[i];
+      r[i] = r2[a];
+    }
+    apply_bj<TP, DN>(Minv, known, n, r2, zv);
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      z[(int64_t)n * DN + a] = zv[a];
+      s_rz += r2[a] * zv[a];
+      s_rr += r2[a] * r2[a];
+    }
+  }
+  s_rz = block_sum<double, BS>(s_rz, sh);
+  s_rr = block_sum<double, BS>(s_rr, sh2);
+  if (threadIdx.x == 0) {
+    part2[2 * blk] = s_rz;
+    part2[2 * blk + 1] = s_rr;
+  }
+}
+
+// Reliable update (mixed mode, run by the host after a pause):
+// r = b - A64 x, z = Minv r, r.z / r.r partials in place of step 2's.  It
+// runs on step 2's node grid (one lane per node) so that its partials have
+// the layout step 1 reduces.
+template <typename TP, int DN>
+__device__ inline void pcg_replace_body(const SpmvArgs& A64, const TP* __restrict__ Minv,
+                                        const double* __restrict__ x, const double* __restrict__ b,
+                                        double* __restrict__ r, double* __restrict__ z, double* __restrict__ part2,
+                                        PcgState* st, int blk, int nblk) {
+  __shared__ double sh[BS / 64];
+  __shared__ double sh2[BS / 64];
+  double s_rz = 0, s_rr = 0;
+  for (int n = blk * BS + threadIdx.x; n < A64.Nnode; n += nblk * BS) {
+    double dd;
+    spmv_node<double, double, DN, 1, 1>(A64, n, 0, x, b, r, dd);
+    double rv[DN], zv[DN];
+#pragma unroll
+    for (int a = 0; a < DN; ++a) rv[a] = r[(int64_t)n * DN + a];
+    apply_bj<TP, DN>(Minv, A64.known, n, rv, zv);
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      z[(int64_t)n * DN + a] = zv[a];
+      s_rz += rv[a] * zv[a];
+      s_rr += rv[a] * rv[a];
+    }
+  }
+  s_rz = block_sum<double, BS>(s_rz, sh);
+  s_rr = block_sum<double, BS>(s_rr, sh2);
+  if (threadIdx.x == 0) {
+    part2[2 * blk] = s_rz;
+    part2[2 * blk + 1] = s_rr;
+  }
+  if (blk == 0 && threadIdx.x == 0) {
+    st->done = 0;
+    st->justrep = 1;
+    st->nrepl += 1;
+  }
+}
+
+// Workspace of one solve: five FP64 vectors, the partial arrays and the state.
+static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct PcgWork {
+  double* r;
+  double* z;
+  double* P0;
+  double* P1;
+  double* q;
+  double* part1;  // MAXB
+  double* partI;  // MAXB
+  double* part2;  // 2 MAXB
+  PcgState* st;
+  double* small;
+};
+
+static inline size_t pcg_workspace_size(int Nnode, int dof_n) {
+  const size_t vb = align256(8 * (size_t)Nnode * dof_n);
+  return 5 * vb + align256(8 * MAXB) * 2 + align256(16 * MAXB) + align256(sizeof(PcgState)) + 256;
+}
+
+static inline PcgWork carve(void* ws, int Nnode, int dof_n) {
+  char* c = (char*)ws;
+  const size_t vb = align256(8 * (size_t)Nnode * dof_n);
+  PcgWork w;
+  w.r = (double*)c; c += vb;
+  w.z = (double*)c; c += vb;
+  w.P0 = (double*)c; c += vb;
+  w.P1 = (double*)c; c += vb;
+  w.q = (double*)c; c += vb;
+  w.part1 = (double*)c; c += align256(8 * MAXB);
+  w.partI = (double*)c; c += align256(8 * MAXB);
+  w.part2 = (double*)c; c += align256(16 * MAXB);
+  w.st = (PcgState*)c; c += align256(sizeof(PcgState));
+  w.small = (double*)c;
+  return w;
+}
+
+// Host-side helpers shared by the single-system driver and the batch engine:
+// initialisation of one solve (state, x0.A x0, r0, z0, p0, scalars) and its
+// final true residual.  Defined in sparse.hip.
+template <typename TP, int DN, int G>
+int pcg_start(const SpmvArgs& A64, const TP* Minv, const double* b, const double* x, const PcgWork& w,
+              double tol_r, double tol_A, double delta2, hipStream_t s);
+template <int DN, int G>
+void pcg_true_residual(const SpmvArgs& A64, const double* b, const double* x, const PcgWork& w, hipStream_t s);
+
+}  // namespace efea

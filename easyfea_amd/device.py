@@ -49,8 +49,32 @@ def table_block(elemType: ElemType, matrixType: MatrixType) -> np.ndarray:
     return np.concatenate([w, N_pg(elemType, matrixType).ravel(), dN_pg(elemType, matrixType).ravel()])
 
 
+class _MeshData:
+    """Device data that depends on the mesh only (connectivity, coordinates,
+    integration tables, sparsity patterns and scatter maps), shared by every
+    simulation on the same ``Mesh`` object."""
+
+    def __init__(self, mesh):
+        g = mesh.groupElem
+        self.connect_np = g.connect
+        self.connect = to_dev(g.connect.astype(np.int32))
+        self.coord = to_dev(mesh.coord.astype(np.float64))
+        self.tables = {}
+        self.patterns = {}
+        self.vec_maps = {}
+
+    @staticmethod
+    def of(mesh) -> "_MeshData":
+        data = mesh.__dict__.get("_efea_device")
+        if data is None:
+            data = _MeshData(mesh)
+            mesh.__dict__["_efea_device"] = data
+        return data
+
+
 class DeviceMesh:
-    """Connectivity, coordinates and integration tables of the main group."""
+    """Connectivity, coordinates and integration tables of the main group,
+    plus the global matrices of one simulation."""
 
     def __init__(self, mesh):
         require_gpu()
@@ -62,29 +86,38 @@ class DeviceMesh:
         self.nPe = g.nPe
         self.dim = g.dim
         self.Nn = mesh.Nn
-        self.connect_np = g.connect
-        self.connect = to_dev(g.connect.astype(np.int32))
-        self.coord = to_dev(mesh.coord.astype(np.float64))
-        self._tables = {}
+        self._shared = _MeshData.of(mesh)
+        self.connect_np = self._shared.connect_np
+        self.connect = self._shared.connect
+        self.coord = self._shared.coord
         self._maps = {}
 
     def tables(self, matrixType: MatrixType):
         mt = MatrixType(matrixType)
-        if mt not in self._tables:
-            self._tables[mt] = (to_dev(table_block(self.elemType, mt)), gauss_points(self.elemType, mt)[1].size)
-        return self._tables[mt]
+        t = self._shared.tables
+        if mt not in t:
+            t[mt] = (to_dev(table_block(self.elemType, mt)), gauss_points(self.elemType, mt)[1].size)
+        return t[mt]
+
+    def pattern(self, dof_n: int) -> "CsrPattern":
+        p = self._shared.patterns
+        if dof_n not in p:
+            p[dof_n] = CsrPattern(build_csr_map(self.connect_np, self.Nn, dof_n), self.Ne)
+        return p[dof_n]
 
     def csr_map(self, dof_n: int) -> "DeviceCsr":
+        """This simulation's matrix with ``dof_n`` DOFs per node (values of
+        its own over the mesh's shared pattern)."""
         key = ("K", dof_n)
         if key not in self._maps:
-            self._maps[key] = DeviceCsr(build_csr_map(self.connect_np, self.Nn, dof_n), self.Ne)
+            self._maps[key] = DeviceCsr(self.pattern(dof_n))
         return self._maps[key]
 
     def vec_map(self, dof_n: int) -> "DeviceVecMap":
-        key = ("F", dof_n)
-        if key not in self._maps:
-            self._maps[key] = DeviceVecMap(build_vec_map(self.connect_np, self.Nn, dof_n))
-        return self._maps[key]
+        v = self._shared.vec_maps
+        if dof_n not in v:
+            v[dof_n] = DeviceVecMap(build_vec_map(self.connect_np, self.Nn, dof_n))
+        return v[dof_n]
 
 
 class DeviceVecMap:
@@ -111,9 +144,9 @@ class DeviceVecMap:
         return m
 
 
-class DeviceCsr:
-    """A global matrix on the device: canonical CSR values (FP64, optional
-    FP32 copy), the node-adjacency indices the kernels use and the scatter map."""
+class CsrPattern:
+    """Sparsity pattern and scatter map of a global matrix on the device: the
+    node adjacency the kernels use, the scatter segments and permutation."""
 
     def __init__(self, cmap: CsrMap, Ne: int):
         self.host = cmap
@@ -126,12 +159,37 @@ class DeviceCsr:
         self.seg = to_dev(cmap.seg_ptr)
         self.perm = to_dev(cmap.perm)
         self.group = _lib.load().efea_default_group(cmap.dof_n, cmap.max_neighbours)
-        self.values = torch.empty(self.nnz, dtype=torch.float64, device="cuda")
-        self.values32: Optional[torch.Tensor] = None
-        self.spmv_scratch = torch.empty(_lib.load().efea_spmv_scratch_bytes(self.Nn, self.dof_n), dtype=torch.uint8,
-                                        device="cuda")
         nd = cmap.nPe * cmap.dof_n
         self.n_entries = Ne * nd * nd
+
+
+class DeviceCsr:
+    """A global matrix on the device: canonical CSR values (FP64, optional
+    FP32 copy) over a shared ``CsrPattern``."""
+
+    def __init__(self, pattern: CsrPattern):
+        self.pattern = pattern
+        self.host = pattern.host
+        self.Nn = pattern.Nn
+        self.dof_n = pattern.dof_n
+        self.nnz = pattern.nnz
+        self.Ne = pattern.Ne
+        self.node_ptr = pattern.node_ptr
+        self.node_cols = pattern.node_cols
+        self.seg = pattern.seg
+        self.perm = pattern.perm
+        self.group = pattern.group
+        self.n_entries = pattern.n_entries
+        self.values = torch.empty(self.nnz, dtype=torch.float64, device="cuda")
+        self.values32: Optional[torch.Tensor] = None
+        self._spmv_scratch: Optional[torch.Tensor] = None
+
+    @property
+    def spmv_scratch(self) -> torch.Tensor:
+        if self._spmv_scratch is None:
+            self._spmv_scratch = torch.empty(_lib.load().efea_spmv_scratch_bytes(self.Nn, self.dof_n),
+                                             dtype=torch.uint8, device="cuda")
+        return self._spmv_scratch
 
     @property
     def Ndof(self) -> int:
@@ -174,6 +232,19 @@ class DeviceCsr:
 MIXED_MIN_MATRIX_BYTES = 128 * 2**20
 
 
+def effective_precision(A: "DeviceCsr", mode: str, adaptive: bool = True) -> int:
+    """64 or 32: the value precision of the Krylov products for ``mode``
+    ("fp64" | "mixed").  With ``adaptive``, a mixed solve of a matrix below
+    ``MIXED_MIN_MATRIX_BYTES`` runs in FP64: the step kernels sit at their
+    ~5 us floor there, so the FP32 copy saves no time and costs iterations."""
+    assert mode in ("fp64", "mixed"), mode
+    if mode == "fp64":
+        return 64
+    if adaptive and A.nnz * 8 < MIXED_MIN_MATRIX_BYTES:
+        return 64
+    return 32
+
+
 @dataclass
 class SolveInfo:
     iterations: int
@@ -197,43 +268,188 @@ class Workspace:
         return self.buf
 
 
-def _solve_once(A: DeviceCsr, b, x, known, m: int, ws: Workspace, tol_residual, tol_anorm, replace_ratio, maxit,
-                check_every):
-    Minv = torch.empty(A.Nn * A.dof_n * A.dof_n, dtype=torch.float64 if m == 64 else torch.float32,
-                       device="cuda")
-    _lib.call("efea_bj_setup", m, A.Nn, A.dof_n, ptr(A.node_ptr), ptr(A.node_cols), ptr(A.values), ptr(known),
-              ptr(Minv), stream_ptr())
-    v32 = A.make_fp32() if m == 32 else None
-    nbytes = _lib.load().efea_pcg_workspace_bytes(A.Nn, A.dof_n, A.group)
-    buf = ws.get(nbytes)
-    info = (C.c_double * 4)()
-    rc = _lib.call("efea_solve_pcg", m, A.Nn, A.dof_n, A.group, ptr(A.node_ptr), ptr(A.node_cols), ptr(A.values),
-                   ptr(v32), ptr(known), ptr(Minv), ptr(b), ptr(x), ptr(buf), nbytes, tol_residual, tol_anorm,
-                   replace_ratio, maxit, check_every, stream_ptr(), C.cast(info, C.c_void_p))
-    return rc, SolveInfo(int(info[0]), float(info[1]), int(info[2]), int(info[3]), "fp64" if m == 64 else "mixed")
+class SolveJob:
+    """One Dirichlet-reduced solve: ``x`` carries the Dirichlet values at the
+    known DOFs and the initial guess elsewhere and receives the solution."""
+
+    def __init__(self, A: "DeviceCsr", b: torch.Tensor, x: torch.Tensor, known: Optional[torch.Tensor],
+                 mode: str = "mixed", tol_residual: float = 1e-12, tol_anorm: float = 1e-9,
+                 replace_ratio: float = 1e-3, maxit: int = 100000, check_every: int = 32, adaptive: bool = True,
+                 owner=None):
+        self.A, self.b, self.x, self.known = A, b, x, known
+        self.mode = mode
+        self.tol_residual, self.tol_anorm, self.replace_ratio = tol_residual, tol_anorm, replace_ratio
+        self.maxit, self.check_every = maxit, check_every
+        self.precision = effective_precision(A, mode, adaptive)
+        self.owner = owner
+        self.fallback = False
+        self.x0 = x.clone() if self.precision == 32 else None
+        self.Minv: Optional[torch.Tensor] = None
+
+    def setup(self, precision: int) -> None:
+        """Block-Jacobi inverse (and the FP32 value copy) for ``precision``."""
+        A = self.A
+        self.precision = precision
+        self.Minv = torch.empty(A.Nn * A.dof_n * A.dof_n,
+                                dtype=torch.float64 if precision == 64 else torch.float32, device="cuda")
+        _lib.call("efea_bj_setup", precision, A.Nn, A.dof_n, ptr(A.node_ptr), ptr(A.node_cols), ptr(A.values),
+                  ptr(self.known), ptr(self.Minv), stream_ptr())
+        if precision == 32:
+            A.make_fp32()
+
+    def restart_fp64(self) -> None:
+        """After a failed mixed solve: the initial guess back, FP64 setup."""
+        self.x.copy_(self.x0)
+        self.fallback = True
+        self.setup(64)
+
+    def info(self, raw) -> SolveInfo:
+        return SolveInfo(int(raw[0]), float(raw[1]), int(raw[2]), int(raw[3]),
+                         "fp64" if self.precision == 64 else "mixed", self.fallback)
+
+    def _solve_once(self, ws: Workspace):
+        A = self.A
+        nbytes = _lib.load().efea_pcg_workspace_bytes(A.Nn, A.dof_n, A.group)
+        buf = ws.get(nbytes)
+        raw = (C.c_double * 4)()
+        v32 = A.values32 if self.precision == 32 else None
+        rc = _lib.call("efea_solve_pcg", self.precision, A.Nn, A.dof_n, A.group, ptr(A.node_ptr),
+                       ptr(A.node_cols), ptr(A.values), ptr(v32), ptr(self.known), ptr(self.Minv), ptr(self.b),
+                       ptr(self.x), ptr(buf), nbytes, self.tol_residual, self.tol_anorm, self.replace_ratio,
+                       self.maxit, self.check_every, stream_ptr(), C.cast(raw, C.c_void_p))
+        return rc, self.info(raw)
+
+    def run(self, ws: Optional[Workspace] = None, raise_on_fail: bool = True) -> SolveInfo:
+        """Solves now with ``efea_solve_pcg``; a mixed solve that breaks down,
+        diverges or stalls is redone in FP64 from the same initial guess."""
+        ws = ws or Workspace()
+        self.setup(self.precision)
+        rc, si = self._solve_once(ws)
+        if rc != 0 and self.precision == 32:
+            self.restart_fp64()
+            rc, si = self._solve_once(ws)
+        if rc != 0 and raise_on_fail:
+            raise _lib.EfeaError("efea_solve_pcg", rc, _lib.last_error())
+        return si
 
 
 def solve(A: DeviceCsr, b: torch.Tensor, x: torch.Tensor, known: Optional[torch.Tensor], mode: str,
           ws: Workspace, tol_residual: float = 1e-12, tol_anorm: float = 1e-9, replace_ratio: float = 1e-3,
           maxit: int = 100000, check_every: int = 32, raise_on_fail: bool = True) -> SolveInfo:
-    """Solves the Dirichlet-reduced system on the device (see ``efea_solve_pcg``).
+    """Solves the Dirichlet-reduced system on the device (``efea_solve_pcg``).
 
     ``x`` carries the Dirichlet values at the known DOFs and the initial guess
     elsewhere; it is overwritten with the solution.  A mixed-precision solve
     that breaks down, diverges or stalls is redone in FP64 from the same
     initial guess (``SolveInfo.fallback``)."""
-    assert mode in ("fp64", "mixed")
-    m = 64 if mode == "fp64" else 32
-    if m == 32 and A.nnz * 8 < MIXED_MIN_MATRIX_BYTES:
-        # latency-bound system: the FP32 copy saves no time per iteration
-        # (the step kernels sit at their ~5 us floor) and costs iterations
-        m = 64
-    x0 = x.clone() if m == 32 else None
-    rc, si = _solve_once(A, b, x, known, m, ws, tol_residual, tol_anorm, replace_ratio, maxit, check_every)
-    if rc != 0 and m == 32:
-        x.copy_(x0)
-        rc, si = _solve_once(A, b, x, known, 64, ws, tol_residual, tol_anorm, replace_ratio, maxit, check_every)
-        si.fallback = True
-    if rc != 0 and raise_on_fail:
-        raise _lib.EfeaError("efea_solve_pcg", rc, _lib.last_error())
-    return si
+    job = SolveJob(A, b, x, known, mode, tol_residual, tol_anorm, replace_ratio, maxit, check_every)
+    return job.run(ws, raise_on_fail)
+
+
+def drive(gen):
+    """Runs a generator of ``SolveJob`` s to completion, solving each job
+    at once (the single-system path); returns the generator's value."""
+    try:
+        job = next(gen)
+        while True:
+            job = gen.send(job.run())
+    except StopIteration as stop:
+        return stop.value
+
+
+class BatchSolver:
+    """Continuous batching over ``efea_batch_*``: the jobs of one (precision,
+    dof_n, group) class advance together, one launch per PCG step."""
+
+    def __init__(self, precision: int, dof_n: int, group: int, capacity: int = 256):
+        self.precision, self.dof_n, self.group, self.capacity = precision, dof_n, group, capacity
+        h = C.c_void_p()
+        _lib.call("efea_batch_create", precision, dof_n, group, capacity, C.byref(h))
+        self._h = h
+        self._jobs = {}
+
+    def __len__(self) -> int:
+        return len(self._jobs)
+
+    def full(self) -> bool:
+        return len(self._jobs) >= self.capacity
+
+    def add(self, job: SolveJob) -> int:
+        A = job.A
+        assert job.precision == self.precision and A.dof_n == self.dof_n and A.group == self.group
+        nbytes = _lib.load().efea_pcg_workspace_bytes(A.Nn, A.dof_n, A.group)
+        job.ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        v32 = A.values32 if self.precision == 32 else None
+        jid = _lib.call("efea_batch_add", self._h, A.Nn, ptr(A.node_ptr), ptr(A.node_cols), ptr(A.values), ptr(v32),
+                        ptr(job.known), ptr(job.Minv), ptr(job.b), ptr(job.x), ptr(job.ws), nbytes,
+                        job.tol_residual, job.tol_anorm, job.replace_ratio, job.maxit, stream_ptr())
+        self._jobs[jid] = job
+        return jid
+
+    def run(self, iterations: int) -> list:
+        """``iterations`` PCG iterations of every job, then the finished
+        ``(job, SolveInfo)`` pairs."""
+        if not self._jobs:
+            return []
+        n = _lib.call("efea_batch_run", self._h, iterations, stream_ptr())
+        out = []
+        jid = C.c_int()
+        raw = (C.c_double * 4)()
+        for _ in range(n):
+            _lib.call("efea_batch_pop", self._h, C.byref(jid), C.cast(raw, C.c_void_p))
+            job = self._jobs.pop(jid.value)
+            job.ws = None
+            out.append((job, job.info(raw)))
+        return out
+
+    def close(self) -> None:
+        if self._h is not None and _lib._lib is not None:
+            _lib.call("efea_batch_destroy", self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BatchExecutor:
+    """Routes solve jobs to one ``BatchSolver`` per (precision, dof_n, group)
+    class and runs them ``check_every`` iterations at a time.  A mixed job
+    that fails is restarted in FP64 from its initial guess, as the single
+    path does."""
+
+    def __init__(self, capacity: int = 256, check_every: int = 32):
+        self.capacity = capacity
+        self.check_every = check_every
+        self.batches = {}
+
+    def submit(self, job: SolveJob) -> None:
+        job.setup(job.precision)
+        key = (job.precision, job.A.dof_n, job.A.group)
+        b = self.batches.get(key)
+        if b is None:
+            b = self.batches[key] = BatchSolver(*key, capacity=self.capacity)
+        b.add(job)
+
+    def active(self) -> int:
+        return sum(len(b) for b in self.batches.values())
+
+    def step(self) -> list:
+        """One chunk of every non-empty batch; returns the completed
+        ``(job, SolveInfo)`` pairs (fallback restarts are resubmitted)."""
+        done = []
+        for b in list(self.batches.values()):
+            for job, si in b.run(self.check_every):
+                if si.status != 1 and job.precision == 32:
+                    job.restart_fp64()
+                    self.submit(job)
+                    continue
+                done.append((job, si))
+        return done
+
+    def close(self) -> None:
+        for b in self.batches.values():
+            b.close()
+        self.batches = {}

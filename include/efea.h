@@ -15,7 +15,7 @@
  *   Kelvin-Mandel strains [xx, yy, sqrt2 xy] / [xx, yy, zz, sqrt2 yz, sqrt2 xz, sqrt2 xy]
  *   (FEM/_group_elem.py:1241-1312, Models/_utils.py:111-152)
  * Reference-element tables: one FP64 block per integration rule,
- *   [w (nPg)] [N (nPg*nPe)] [dN (nPg*dim*nPe)]  (FEM/_gauss.py:364-516, FEM/Elems/*.py)
+ *   [w (nPg)] [N (nPg*nPe)] [dN (nPg*dim*nPe)]  (FEM/_gauss.py:364-516 and the FEM/Elems modules)
  * Element codes: 0 TRI3, 1 QUAD4, 2 HEXA8, 3 TETRA4, 4 TETRA10, 5 SEG2.
  */
 #ifndef EFEA_H
@@ -116,6 +116,33 @@ int efea_solve_pcg(int mode, int Nnode, int dof_n, int group, const int32_t* nod
                    const double* b, double* x, void* workspace, size_t workspace_bytes, double tol_residual,
                    double tol_anorm, double replace_ratio, int maxit, int check_every, void* stream,
                    double* info);
+
+/* Continuous batching of Dirichlet-reduced solves (the serving loop).
+ * A batch holds up to `capacity` systems of one (mode, dof_n, group) class and
+ * advances all of them with one launch per PCG step; systems join
+ * (efea_batch_add) and leave (efea_batch_pop) between calls of efea_batch_run.
+ * Every system runs on the grid it would get alone, so its iterates are
+ * bitwise those of efea_solve_pcg with check_every = the `iterations` passed to
+ * efea_batch_run.  All calls on one batch use one stream; the device buffers of
+ * a system (matrix, Minv, b, x, workspace of efea_pcg_workspace_bytes bytes)
+ * stay alive until it is popped.  The handle is a host pointer.
+ * Replaces the per-simulation solves of __Solver_1 (Simulations/Solvers.py:502-553)
+ * for many simulations at once; the reference has no batched path. */
+int efea_batch_create(int mode, int dof_n, int group, int capacity, void** handle);
+int efea_batch_destroy(void* handle);
+/* Starts one solve (asynchronous); returns its id (>= 0) or an error code. */
+int efea_batch_add(void* handle, int Nnode, const int32_t* node_ptr, const int32_t* node_cols,
+                   const double* values64, const float* values32, const uint8_t* known, const void* Minv,
+                   const double* b, double* x, void* workspace, size_t workspace_bytes, double tol_residual,
+                   double tol_anorm, double replace_ratio, int maxit, void* stream);
+/* Runs `iterations` PCG iterations of every active system, then polls their
+ * states (synchronises `stream`); returns how many systems finished. */
+int efea_batch_run(void* handle, int iterations, void* stream);
+/* Pops one finished system: *id and info (host, 4 doubles, as efea_solve_pcg).
+ * Returns 0, or 1 with *id = -1 when none is waiting. */
+int efea_batch_pop(void* handle, int* id, double* info);
+/* Number of systems still iterating. */
+int efea_batch_active(void* handle);
 
 /* Deterministic dot product (b may be NULL: sum of a); scratch holds 1024 doubles. */
 int efea_dot(int64_t n, const double* a, const double* b, double* scratch, double* out, void* stream);

@@ -42,6 +42,17 @@ def test_abi_argument_errors_are_reported_without_gpu():
     assert rc == -1
     rc = lib.efea_pf_split_points(4, 7, None, None, 0, 0.3, 0, None, None, None, None, None)
     assert rc == -1 and "null pointer" in _lib.last_error()
+    h = C.c_void_p()
+    assert lib.efea_batch_create(16, 2, 16, 8, C.byref(h)) == -1 and "mode" in _lib.last_error()
+    assert lib.efea_batch_create(64, 2, 12, 8, C.byref(h)) == -1 and "group" in _lib.last_error()
+    assert lib.efea_batch_create(64, 2, 16, 0, C.byref(h)) == -1 and "capacity" in _lib.last_error()
+    assert h.value is None
+    assert lib.efea_batch_run(None, 32, None) == -1
+    assert lib.efea_batch_add(None, 10, None, None, None, None, None, None, None, None, None, 0, 1e-12, 1e-9,
+                              1e-3, 10, None) == -1
+    jid = C.c_int(5)
+    assert lib.efea_batch_pop(None, C.byref(jid), None) == -1
+    assert lib.efea_batch_destroy(None) == 0
 
 
 def test_workspace_sizes_and_groups():
@@ -189,3 +200,96 @@ def test_product_path_refuses_to_run_without_gpu():
     mat = ea.Models.Elastic.Isotropic(2)
     with pytest.raises(RuntimeError, match="no CPU fallback"):
         ea.Simulations.Elastic(mesh, mat)
+
+
+# ---------------------------------------------------------------- serving loop
+class _FakeJob:
+    def __init__(self, chunks):
+        self.chunks = chunks
+        self.owner = None
+
+
+class _FakeExecutor:
+    """Completes each job after its number of chunks (no GPU)."""
+
+    def __init__(self):
+        self.jobs = []
+        self.peak = 0
+
+    def submit(self, job):
+        self.jobs.append(job)
+        self.peak = max(self.peak, len(self.jobs))
+
+    def active(self):
+        return len(self.jobs)
+
+    def step(self):
+        done, keep = [], []
+        for j in self.jobs:
+            j.chunks -= 1
+            (done if j.chunks <= 0 else keep).append(j)
+        self.jobs = keep
+        return [(j, "info") for j in done]
+
+
+def _work(nsolves, chunks, log, rid):
+    infos = []
+    for _ in range(nsolves):
+        infos.append((yield _FakeJob(chunks)))
+    log.append(rid)
+    return infos
+
+
+def test_serving_admission_by_budget_and_continuous_batching():
+    from easyfea_amd.serving import Request, Server, percentiles
+
+    log = []
+    t = [0.0]
+    reqs = [Request(i, 0.0, (lambda i=i: _work(2, 1 + i % 3, log, i)), est_bytes=40) for i in range(6)]
+    ex = _FakeExecutor()
+    srv = Server(hbm_budget=100, executor=ex, clock=lambda: t[0])
+    srv.run(reqs)
+    assert all(r.error is None and r.result == ["info", "info"] for r in reqs)
+    assert ex.peak == 2            # two requests of 40 bytes fit a budget of 100
+    assert srv.max_resident <= 100
+    assert sorted(log) == list(range(6))
+    assert all(r.solves == 2 for r in reqs)
+    # a request larger than the budget still runs when nothing else is resident
+    big = [Request(0, 0.0, (lambda: _work(1, 1, log, 9)), est_bytes=1000)]
+    Server(hbm_budget=100, executor=_FakeExecutor(), clock=lambda: t[0]).run(big)
+    assert big[0].result == ["info"]
+    p = percentiles([1.0, 2.0, 3.0, None])
+    assert p["p50"] == 2.0
+
+
+def test_serving_failed_request_does_not_stop_others():
+    from easyfea_amd.serving import Request, Server
+
+    def bad():
+        yield _FakeJob(1)
+        raise RuntimeError("solve failed")
+
+    log = []
+    reqs = [Request(0, 0.0, bad, 1), Request(1, 0.0, (lambda: _work(1, 2, log, 1)), 1)]
+    Server(hbm_budget=10, executor=_FakeExecutor(), clock=lambda: 0.0).run(reqs)
+    assert isinstance(reqs[0].error, RuntimeError) and reqs[1].error is None and log == [1]
+
+
+def test_workload_fields_are_seeded_and_lognormal():
+    from easyfea_amd import workloads
+
+    z1 = workloads.gaussian_field((64, 64), 6.4, np.random.default_rng(3))
+    z2 = workloads.gaussian_field((64, 64), 6.4, np.random.default_rng(3))
+    assert np.array_equal(z1, z2)
+    assert abs(z1.mean()) < 1e-12 and abs(z1.std() - 1) < 1e-12
+    # neighbouring cells correlated, distant ones much less
+    c1 = np.corrcoef(z1[:, :-1].ravel(), z1[:, 1:].ravel())[0, 1]
+    assert c1 > 0.8
+    E = workloads.lognormal(210000.0, 0.2, np.random.default_rng(0).standard_normal(200000))
+    assert abs(E.mean() / 210000.0 - 1) < 3e-3 and abs(E.std() / E.mean() - 0.2) < 3e-3
+    specs = workloads.serving_elastic_specs(n=20, qps=50.0)
+    assert all(32 <= s.side <= 256 and s.E.shape == (s.side * s.side,) for s in specs)
+    arr = np.array([s.arrival for s in specs])
+    assert arr[0] == 0 and np.all(np.diff(arr) >= 0)
+    pf = workloads.serving_phasefield_specs(n=8, steps=5)
+    assert all(100 <= s.cells <= 200 for s in pf)
