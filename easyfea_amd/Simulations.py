@@ -83,6 +83,7 @@ class _Simu:
         self.replace_ratio = 1e-3
         self.maxit = 200000
         self.check_every = 32
+        self.preconditioner = "auto"  # "bj" | "two-level" | "auto"
         self.last_solve = {}
         self.Bc_Init()
         model._Add_observer(self)
@@ -312,7 +313,7 @@ class _Simu:
         if idx.size:
             x_dev.index_copy_(0, idx_dev, to_dev(xk[idx]))
         return SolveJob(A, b_dev, x_dev, known, self._mode(), self.tol_residual, self.tol_anorm,
-                        self.replace_ratio, self.maxit, self.check_every, owner=self)
+                        self.replace_ratio, self.maxit, self.check_every, owner=self, precond=self.preconditioner)
 
     def _solve_system_gen(self, A, b_dev: torch.Tensor, x_dev: torch.Tensor, problemType):
         """Generator form of a solve: yields the ``SolveJob`` and receives its

@@ -9,8 +9,10 @@
 // a solo solve and the iterates are bitwise the ones efea_solve_pcg computes.
 // Systems join (efea_batch_add) and leave (efea_batch_pop) between chunks of
 // iterations; a system that has converged idles (its blocks return at once)
-// until the next poll retires it.
-#include "pcg_device.h"
+// until the next poll retires it.  A batch runs either the one-level (block
+// Jacobi) or the two-level iteration for all its systems; the two-level one
+// adds a grid of one block per aggregate for step 2 and the prolongation.
+#include "coarse_device.h"
 
 #include <vector>
 
@@ -32,8 +34,11 @@ struct BatchSys {
   double* part1;
   double* part2;
   PcgState* st;
+  CoarseArgs C;    // two-level batches
   int blkS0, nbS;  // block range in the node-group grid
   int blkN0, nbN;  // block range in the node grid
+  int blkA0;       // first block in the aggregate grid (C.nagg blocks)
+  int nb2;         // blocks whose r.z / r.r partials step 1 reduces
   int replace;     // set by the host at a poll: run a residual replacement
   int pad_;
 };
@@ -43,7 +48,7 @@ __global__ void __launch_bounds__(BS) kb_step1(const BatchSys* __restrict__ sys,
   const BatchSys& S = sys[map[blockIdx.x]];
   SpmvArgs AK = S.A64;
   AK.vals = S.valsK;
-  pcg_step1_body<TM, DN, G>(AK, S.z, S.P0, S.P1, S.q, S.part2, S.nbN, S.part1, S.st, (int)blockIdx.x - S.blkS0,
+  pcg_step1_body<TM, DN, G>(AK, S.z, S.P0, S.P1, S.q, S.part2, S.nb2, S.part1, S.st, (int)blockIdx.x - S.blkS0,
                             S.nbS);
 }
 
@@ -60,6 +65,30 @@ __global__ void __launch_bounds__(BS) kb_replace(const BatchSys* __restrict__ sy
   if (!S.replace) return;
   pcg_replace_body<TP, DN>(S.A64, (const TP*)S.Minv, S.x, S.b, S.r, S.z, S.part2, S.st,
                            (int)blockIdx.x - S.blkN0, S.nbN);
+}
+
+template <typename TP, int DN>
+__global__ void __launch_bounds__(BS) kb2_step2(const BatchSys* __restrict__ sys, const int* __restrict__ map) {
+  const BatchSys& S = sys[map[blockIdx.x]];
+  pcg2_step2_body<TP, DN>(S.C, S.A64.known, (const TP*)S.Minv, S.x, S.r, S.z, S.P0, S.P1, S.q, S.part1, S.nbS,
+                          S.part2, S.st, (int)blockIdx.x - S.blkA0);
+}
+
+// ONLY_REPLACED: after a replacement, only the replaced systems get the
+// coarse correction of their new residual.
+template <int DN, bool ONLY_REPLACED>
+__global__ void __launch_bounds__(BS) kb2_prolong(const BatchSys* __restrict__ sys, const int* __restrict__ map) {
+  const BatchSys& S = sys[map[blockIdx.x]];
+  if (ONLY_REPLACED && !S.replace) return;
+  pcg2_prolong_body<DN>(S.C, S.A64.known, S.z, S.part2, S.st, (int)blockIdx.x - S.blkA0);
+}
+
+template <typename TP, int DN>
+__global__ void __launch_bounds__(BS) kb2_replace(const BatchSys* __restrict__ sys, const int* __restrict__ map) {
+  const BatchSys& S = sys[map[blockIdx.x]];
+  if (!S.replace) return;
+  pcg2_replace_body<TP, DN>(S.C, S.A64, (const TP*)S.Minv, S.x, S.b, S.r, S.z, S.part2, S.st,
+                            (int)blockIdx.x - S.blkA0);
 }
 
 // One block per system: copies its state into a contiguous array for a
@@ -79,20 +108,23 @@ struct BatchEntry {
   double info[4];
 };
 
+constexpr int MAXA = 4096;  // aggregates per system in a two-level batch
+
 struct Batch {
   int mode, dof_n, group, cap;
+  int two_level;
   int next_id = 0;
   std::vector<BatchEntry> active;
   std::vector<BatchEntry> done;
   BatchSys* h_sys[2] = {nullptr, nullptr};
   int flip = 0;
   BatchSys* d_sys = nullptr;
-  int* h_map = nullptr;  // [cap*MAXB node-group blocks | cap*MAXB node blocks]
+  int* h_map = nullptr;  // [cap*MAXB node-group | cap*MAXB node | cap*MAXA aggregate blocks]
   int* d_map = nullptr;
   PcgState* h_st = nullptr;
   PcgState* d_st = nullptr;
   double* h_rr = nullptr;
-  int totS = 0, totN = 0;
+  int totS = 0, totN = 0, totA = 0;
   bool dirty = true;
 };
 
@@ -114,7 +146,9 @@ static bool supported(int dof_n, int group) {
          (dof_n == 3 && (group == 16 || group == 32));
 }
 
-int batch_create(int mode, int dof_n, int group, int capacity, void** out) {
+static size_t map_size(int cap) { return 2 * (size_t)cap * MAXB + (size_t)cap * MAXA; }
+
+int batch_create(int mode, int dof_n, int group, int two_level, int capacity, void** out) {
   if (mode != 64 && mode != 32) {
     set_error("batch_create: mode %d (64 or 32)", mode);
     return EFEA_ERR_ARG;
@@ -132,7 +166,8 @@ int batch_create(int mode, int dof_n, int group, int capacity, void** out) {
   b->dof_n = dof_n;
   b->group = group;
   b->cap = capacity;
-  const size_t nmap = 2 * (size_t)capacity * MAXB;
+  b->two_level = two_level ? 1 : 0;
+  const size_t nmap = map_size(capacity);
   bool ok = hipHostMalloc((void**)&b->h_sys[0], capacity * sizeof(BatchSys), hipHostMallocDefault) == hipSuccess &&
             hipHostMalloc((void**)&b->h_sys[1], capacity * sizeof(BatchSys), hipHostMallocDefault) == hipSuccess &&
             hipHostMalloc((void**)&b->h_map, nmap * sizeof(int), hipHostMallocDefault) == hipSuccess &&
@@ -156,17 +191,29 @@ int batch_destroy(void* h) {
 }
 
 template <typename TP, int DN, int G>
-static int start_dispatch(const SpmvArgs& A64, const void* Minv, const double* b, const double* x,
-                          const PcgWork& w, double tol_r, double tol_A, double delta2, hipStream_t s) {
-  return pcg_start<TP, DN, G>(A64, (const TP*)Minv, b, x, w, tol_r, tol_A, delta2, s);
+static int start_dispatch(const SpmvArgs& A64, const void* Minv, const CoarseArgs* C, const double* b,
+                          const double* x, const Pcg2Work& v, double tol_r, double tol_A, double delta2,
+                          hipStream_t s) {
+  if (C) return pcg2_start<TP, DN, G>(A64, (const TP*)Minv, *C, b, x, v, tol_r, tol_A, delta2, s);
+  return pcg_start<TP, DN, G>(A64, (const TP*)Minv, b, x, v.w, tol_r, tol_A, delta2, s);
 }
 
 int batch_add(void* h, int Nnode, const int* node_ptr, const int* node_cols, const double* vals64,
-              const float* vals32, const unsigned char* known, const void* Minv, const double* b, double* x,
+              const float* vals32, const unsigned char* known, const void* Minv, int nagg, const int* agg_ptr,
+              const int* agg_nodes, const double* rel, const double* Ainv, int ld, const double* b, double* x,
               void* ws, size_t ws_bytes, double tol_r, double tol_A, double delta, int maxit, hipStream_t s) {
   Batch* B = (Batch*)h;
-  if (ws_bytes < pcg_workspace_size(Nnode, B->dof_n)) {
-    set_error("batch_add: workspace of %zu bytes, %zu needed", ws_bytes, pcg_workspace_size(Nnode, B->dof_n));
+  if (B->two_level) {
+    if (nagg < 1 || nagg > MAXA || !agg_ptr || !agg_nodes || !rel || !Ainv || ld < 1) {
+      set_error("batch_add: two-level batch needs 1..%d aggregates and the coarse arrays", MAXA);
+      return EFEA_ERR_ARG;
+    }
+  } else {
+    nagg = 0;
+  }
+  const size_t need = B->two_level ? pcg2_workspace_size(Nnode, B->dof_n, nagg) : pcg_workspace_size(Nnode, B->dof_n);
+  if (ws_bytes < need) {
+    set_error("batch_add: workspace of %zu bytes, %zu needed", ws_bytes, need);
     return EFEA_ERR_SIZE;
   }
   if ((int)B->active.size() >= B->cap) {
@@ -179,7 +226,8 @@ int batch_add(void* h, int Nnode, const int* node_ptr, const int* node_cols, con
   }
   BatchEntry e{};
   e.id = B->next_id++;
-  e.w = carve(ws, Nnode, B->dof_n);
+  const Pcg2Work v = B->two_level ? carve2(ws, Nnode, B->dof_n, nagg) : Pcg2Work{carve(ws, Nnode, B->dof_n), nullptr};
+  e.w = v.w;
   e.launched = 0;
   e.maxit = maxit;
   e.finished = false;
@@ -199,13 +247,16 @@ int batch_add(void* h, int Nnode, const int* node_ptr, const int* node_cols, con
   d.st = e.w.st;
   d.nbS = (int)grid_for((int64_t)Nnode * B->group);
   d.nbN = (int)grid_for(Nnode);
+  d.C = CoarseArgs{nagg, agg_ptr, agg_nodes, rel, Ainv, ld, v.rc};
+  d.nb2 = B->two_level ? nagg : d.nbN;
   d.replace = 0;
   const double delta2 = B->mode == 32 ? delta * delta : 0.0;
   int rc = EFEA_ERR_ARG;
+  const CoarseArgs* Cp = B->two_level ? &d.C : nullptr;
 #define ST(D, GV)                                                                                          \
   if (B->dof_n == D && B->group == GV)                                                                    \
-    rc = B->mode == 64 ? start_dispatch<double, D, GV>(d.A64, Minv, b, x, e.w, tol_r, tol_A, delta2, s)   \
-                       : start_dispatch<float, D, GV>(d.A64, Minv, b, x, e.w, tol_r, tol_A, delta2, s);
+    rc = B->mode == 64 ? start_dispatch<double, D, GV>(d.A64, Minv, Cp, b, x, v, tol_r, tol_A, delta2, s) \
+                       : start_dispatch<float, D, GV>(d.A64, Minv, Cp, b, x, v, tol_r, tol_A, delta2, s);
   ST(1, 8) ST(1, 16) ST(2, 8) ST(2, 16) ST(3, 16) ST(3, 32)
 #undef ST
   if (rc) return rc;
@@ -230,22 +281,29 @@ static int upload_sys(Batch* B, hipStream_t s) {
 static int rebuild(Batch* B, hipStream_t s) {
   int* mapS = B->h_map;
   int* mapN = B->h_map + (size_t)B->cap * MAXB;
-  int tS = 0, tN = 0;
+  int* mapA = B->h_map + 2 * (size_t)B->cap * MAXB;
+  int tS = 0, tN = 0, tA = 0;
   for (size_t i = 0; i < B->active.size(); ++i) {
     BatchSys& d = B->active[i].d;
     d.blkS0 = tS;
     d.blkN0 = tN;
+    d.blkA0 = tA;
     for (int k = 0; k < d.nbS; ++k) mapS[tS + k] = (int)i;
     for (int k = 0; k < d.nbN; ++k) mapN[tN + k] = (int)i;
+    for (int k = 0; k < d.C.nagg; ++k) mapA[tA + k] = (int)i;
     tS += d.nbS;
     tN += d.nbN;
+    tA += d.C.nagg;
   }
   B->totS = tS;
   B->totN = tN;
+  B->totA = tA;
+  const size_t offN = (size_t)B->cap * MAXB, offA = 2 * (size_t)B->cap * MAXB;
   if (!B->active.empty() &&
       (hipMemcpyAsync(B->d_map, mapS, tS * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
-       hipMemcpyAsync(B->d_map + (size_t)B->cap * MAXB, mapN, tN * sizeof(int), hipMemcpyHostToDevice, s) !=
-           hipSuccess)) {
+       hipMemcpyAsync(B->d_map + offN, mapN, tN * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
+       (tA > 0 &&
+        hipMemcpyAsync(B->d_map + offA, mapA, tA * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess))) {
     set_error("batch: map upload failed");
     return EFEA_ERR_HIP;
   }
@@ -258,11 +316,18 @@ static int batch_run_t(Batch* B, int iters, hipStream_t s) {
   const int n = (int)B->active.size();
   const int* mapS = B->d_map;
   const int* mapN = B->d_map + (size_t)B->cap * MAXB;
+  const int* mapA = B->d_map + 2 * (size_t)B->cap * MAXB;
+  const BatchSys* dsys = B->d_sys;
   for (int k = 0; k < iters; ++k) {
-    hipLaunchKernelGGL((kb_step1<TM, DN, G>), dim3(B->totS), dim3(BS), 0, s, (const BatchSys*)B->d_sys, mapS);
-    hipLaunchKernelGGL((kb_step2<TP, DN>), dim3(B->totN), dim3(BS), 0, s, (const BatchSys*)B->d_sys, mapN);
+    hipLaunchKernelGGL((kb_step1<TM, DN, G>), dim3(B->totS), dim3(BS), 0, s, dsys, mapS);
+    if (B->two_level) {
+      hipLaunchKernelGGL((kb2_step2<TP, DN>), dim3(B->totA), dim3(BS), 0, s, dsys, mapA);
+      hipLaunchKernelGGL((kb2_prolong<DN, false>), dim3(B->totA), dim3(BS), 0, s, dsys, mapA);
+    } else {
+      hipLaunchKernelGGL((kb_step2<TP, DN>), dim3(B->totN), dim3(BS), 0, s, dsys, mapN);
+    }
   }
-  hipLaunchKernelGGL(kb_gather_state, dim3(n), dim3(64), 0, s, (const BatchSys*)B->d_sys, B->d_st);
+  hipLaunchKernelGGL(kb_gather_state, dim3(n), dim3(64), 0, s, dsys, B->d_st);
   if (int e = check_launch("batch_iter")) return e;
   if (hipMemcpyAsync(B->h_st, B->d_st, n * sizeof(PcgState), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess) {
@@ -282,7 +347,12 @@ static int batch_run_t(Batch* B, int iters, hipStream_t s) {
   }
   if (any_replace) {
     if (int e = upload_sys(B, s)) return e;
-    hipLaunchKernelGGL((kb_replace<TP, DN>), dim3(B->totN), dim3(BS), 0, s, (const BatchSys*)B->d_sys, mapN);
+    if (B->two_level) {
+      hipLaunchKernelGGL((kb2_replace<TP, DN>), dim3(B->totA), dim3(BS), 0, s, dsys, mapA);
+      hipLaunchKernelGGL((kb2_prolong<DN, true>), dim3(B->totA), dim3(BS), 0, s, dsys, mapA);
+    } else {
+      hipLaunchKernelGGL((kb_replace<TP, DN>), dim3(B->totN), dim3(BS), 0, s, dsys, mapN);
+    }
     if (int e = check_launch("batch_replace")) return e;
     for (int i = 0; i < n; ++i) B->active[i].d.replace = 0;
   }

@@ -53,13 +53,21 @@ int pf_elastic(int, int64_t, const int*, const double*, const double*, int, cons
                const double*, int, double, int, int, double, double*, hipStream_t);
 int pf_split_points(int64_t, int, const double*, const double*, int, double, int, double*, double*, double*,
                     double*, hipStream_t);
-int batch_create(int, int, int, int, void**);
+int batch_create(int, int, int, int, int, void**);
 int batch_destroy(void*);
 int batch_add(void*, int, const int*, const int*, const double*, const float*, const unsigned char*, const void*,
-              const double*, double*, void*, size_t, double, double, double, int, hipStream_t);
+              int, const int*, const int*, const double*, const double*, int, const double*, double*, void*, size_t,
+              double, double, double, int, hipStream_t);
 int batch_run(void*, int, hipStream_t);
 int batch_pop(void*, int*, double*);
 int batch_active(void*);
+int coarse_ld(int, int);
+int coarse_setup(int, int, const int*, const int*, const int*, const int*, const int*, const int*, const int*,
+                 const double*, const unsigned char*, const double*, int, double*, double*, hipStream_t);
+size_t pcg2_workspace_bytes(int, int, int);
+int solve_pcg2(int, int, int, int, const int*, const int*, const double*, const float*, const unsigned char*,
+               const void*, int, const int*, const int*, const double*, const double*, int, const double*, double*,
+               void*, double, double, double, int, int, hipStream_t, double*);
 
 }  // namespace efea
 
@@ -178,13 +186,13 @@ int efea_solve_pcg(int mode, int Nnode, int dof_n, int group, const int32_t* nod
                    workspace, tol_residual, tol_anorm, replace_ratio, maxit, check_every, STREAM(stream), info);
 }
 
-int efea_batch_create(int mode, int dof_n, int group, int capacity, void** handle) {
+int efea_batch_create(int mode, int dof_n, int group, int two_level, int capacity, void** handle) {
   if (!handle) {
     set_error("efea_batch_create: null handle pointer");
     return EFEA_ERR_ARG;
   }
   *handle = nullptr;
-  return batch_create(mode, dof_n, group, capacity, handle);
+  return batch_create(mode, dof_n, group, two_level, capacity, handle);
 }
 
 int efea_batch_destroy(void* handle) {
@@ -193,7 +201,8 @@ int efea_batch_destroy(void* handle) {
 }
 
 int efea_batch_add(void* handle, int Nnode, const int32_t* node_ptr, const int32_t* node_cols,
-                   const double* values64, const float* values32, const uint8_t* known, const void* Minv,
+                   const double* values64, const float* values32, const uint8_t* known, const void* Minv, int nagg,
+                   const int32_t* agg_ptr, const int32_t* agg_nodes, const double* rel, const double* Ainv, int ld,
                    const double* b, double* x, void* workspace, size_t workspace_bytes, double tol_residual,
                    double tol_anorm, double replace_ratio, int maxit, void* stream) {
   if (!handle || Nnode <= 0 || !node_ptr || !node_cols || !values64 || !Minv || !b || !x || !workspace) {
@@ -204,8 +213,9 @@ int efea_batch_add(void* handle, int Nnode, const int32_t* node_ptr, const int32
     set_error("efea_batch_add: maxit %d", maxit);
     return EFEA_ERR_ARG;
   }
-  return batch_add(handle, Nnode, node_ptr, node_cols, values64, values32, known, Minv, b, x, workspace,
-                   workspace_bytes, tol_residual, tol_anorm, replace_ratio, maxit, STREAM(stream));
+  return batch_add(handle, Nnode, node_ptr, node_cols, values64, values32, known, Minv, nagg, agg_ptr, agg_nodes,
+                   rel, Ainv, ld, b, x, workspace, workspace_bytes, tol_residual, tol_anorm, replace_ratio, maxit,
+                   STREAM(stream));
 }
 
 int efea_batch_run(void* handle, int iterations, void* stream) {
@@ -230,6 +240,55 @@ int efea_batch_active(void* handle) {
     return EFEA_ERR_ARG;
   }
   return batch_active(handle);
+}
+
+int efea_coarse_ld(int nagg, int dof_n) {
+  if (nagg < 1 || dof_n < 1 || dof_n > 3) {
+    set_error("efea_coarse_ld: nagg %d / dof_n %d", nagg, dof_n);
+    return EFEA_ERR_ARG;
+  }
+  return coarse_ld(nagg, dof_n);
+}
+
+int efea_coarse_setup(int dof_n, int nblocks, const int32_t* pair_ptr, const int32_t* pair_node,
+                      const int32_t* pair_pos, const int32_t* blk_I, const int32_t* blk_J, const int32_t* node_ptr,
+                      const int32_t* node_cols, const double* values64, const uint8_t* known, const double* rel,
+                      int ld, double* Ainv, double* pivot_scratch, void* stream) {
+  if (nblocks < 0 || ld <= 0 || !Ainv || !pivot_scratch ||
+      (nblocks > 0 && (!pair_ptr || !pair_node || !pair_pos || !blk_I || !blk_J || !node_ptr || !node_cols ||
+                       !values64 || !rel))) {
+    set_error("efea_coarse_setup: null pointer or negative size");
+    return EFEA_ERR_ARG;
+  }
+  return coarse_setup(dof_n, nblocks, pair_ptr, pair_node, pair_pos, blk_I, blk_J, node_ptr, node_cols, values64,
+                      known, rel, ld, Ainv, pivot_scratch, STREAM(stream));
+}
+
+size_t efea_pcg2_workspace_bytes(int Nnode, int dof_n, int nagg) { return pcg2_workspace_bytes(Nnode, dof_n, nagg); }
+
+int efea_solve_pcg2(int mode, int Nnode, int dof_n, int group, const int32_t* node_ptr, const int32_t* node_cols,
+                    const double* values64, const float* values32, const uint8_t* known, const void* Minv, int nagg,
+                    const int32_t* agg_ptr, const int32_t* agg_nodes, const double* rel, const double* Ainv, int ld,
+                    const double* b, double* x, void* workspace, size_t workspace_bytes, double tol_residual,
+                    double tol_anorm, double replace_ratio, int maxit, int check_every, void* stream,
+                    double* info) {
+  if (Nnode <= 0 || nagg <= 0 || !node_ptr || !node_cols || !values64 || !Minv || !agg_ptr || !agg_nodes || !rel ||
+      !Ainv || !b || !x || !workspace || !info) {
+    set_error("efea_solve_pcg2: null pointer or empty system");
+    return EFEA_ERR_ARG;
+  }
+  if (ld < coarse_ld(nagg, dof_n)) {
+    set_error("efea_solve_pcg2: leading dimension %d < %d", ld, coarse_ld(nagg, dof_n));
+    return EFEA_ERR_ARG;
+  }
+  if (workspace_bytes < pcg2_workspace_bytes(Nnode, dof_n, nagg)) {
+    set_error("efea_solve_pcg2: workspace of %zu bytes, %zu needed", workspace_bytes,
+              pcg2_workspace_bytes(Nnode, dof_n, nagg));
+    return EFEA_ERR_SIZE;
+  }
+  return solve_pcg2(mode, Nnode, dof_n, group, node_ptr, node_cols, values64, values32, known, Minv, nagg, agg_ptr,
+                    agg_nodes, rel, Ainv, ld, b, x, workspace, tol_residual, tol_anorm, replace_ratio, maxit,
+                    check_every, STREAM(stream), info);
 }
 
 int efea_dot(int64_t n, const double* a, const double* b, double* scratch, double* out, void* stream) {

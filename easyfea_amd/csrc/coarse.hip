@@ -1,0 +1,492 @@
+// Two-level preconditioner: block Jacobi plus an additive coarse correction
+// over node aggregates, M^-1 r = B^-1 r + P Ac^-1 P^T r.
+//
+// P holds per aggregate the rigid-body modes of its nodes (2D: ux, uy and the
+// rotation about the aggregate centroid; 3D: three translations and three
+// rotations; scalar problems: the constant), zero at known DOFs.  Per matrix:
+//   * k_coarse_assemble: Ac = P^T A P, one wavefront per nonzero
+//     aggregate-pair block (I, J), summing over a host-sorted list of node
+//     pairs (deterministic, no atomics);
+//   * blocked Gauss-Jordan inversion of the SPD matrix Ac in place (pivot
+//     block in LDS, then row panel, trailing update and column panel), then
+//     symmetrisation.
+// Per PCG iteration, on a grid of one block per aggregate:
+//   * step 2 (alpha, x, r, z = B^-1 r, partials of r.z and r.r) also forms
+//     the aggregate's restriction rc_I = P_I^T r;
+//   * the prolongation kernel computes its rows zc_I = (Ac^-1)_I rc, adds
+//     P_I zc_I to z and rc_I . zc_I to the block's r.z partial.
+// Step 1 (SpMV and p update) is the one-level kernel unchanged.
+#include "coarse_device.h"
+
+namespace efea {
+
+int pcg_finish_info(const void* state, double rr_true, double* info);
+
+constexpr int GB = 32;  // Gauss-Jordan block
+
+__global__ void k_fill_f64(int64_t n, double* __restrict__ a, double v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Coarse matrix.  pair_ptr[cb]..pair_ptr[cb+1] lists the (node, neighbour
+// position) pairs whose nodes lie in aggregates (blk_I[cb], blk_J[cb]).
+template <int DN>
+__global__ void __launch_bounds__(256)
+k_coarse_assemble(int ncb, const int* __restrict__ pair_ptr, const int* __restrict__ pair_node,
+                  const int* __restrict__ pair_pos, const int* __restrict__ blk_I, const int* __restrict__ blk_J,
+                  const int* __restrict__ node_ptr, const int* __restrict__ node_cols,
+                  const double* __restrict__ vals, const unsigned char* __restrict__ known,
+                  const double* __restrict__ rel, double* __restrict__ Ac, int ld) {
+  constexpr int NM = Modes<DN>::NM;
+  const int cb = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (cb >= ncb) return;  // whole wavefront exits together
+  double acc[NM][NM];
+#pragma unroll
+  for (int m = 0; m < NM; ++m)
+#pragma unroll
+    for (int l = 0; l < NM; ++l) acc[m][l] = 0;
+  for (int k = pair_ptr[cb] + lane; k < pair_ptr[cb + 1]; k += 64) {
+    const int i = pair_node[k], pos = pair_pos[k];
+    const int p0 = node_ptr[i], cnt = node_ptr[i + 1] - p0;
+    const int j = node_cols[p0 + pos];
+    const int64_t base = (int64_t)DN * DN * p0;
+    const double* ri = rel + 3 * (int64_t)i;
+    const double* rj = rel + 3 * (int64_t)j;
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      if (known && known[(int64_t)i * DN + a]) continue;
+      double T[NM];
+#pragma unroll
+      for (int l = 0; l < NM; ++l) T[l] = 0;
+#pragma unroll
+      for (int b = 0; b < DN; ++b) {
+        if (known && known[(int64_t)j * DN + b]) continue;
+        const double v = vals[base + (int64_t)a * DN * cnt + (int64_t)pos * DN + b];
+#pragma unroll
+        for (int l = 0; l < NM; ++l) T[l] += v * pmode<DN>(rj, b, l);
+      }
+#pragma unroll
+      for (int m = 0; m < NM; ++m) {
+        const double pm = pmode<DN>(ri, a, m);
+#pragma unroll
+        for (int l = 0; l < NM; ++l) acc[m][l] += pm * T[l];
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < NM; ++m)
+#pragma unroll
+    for (int l = 0; l < NM; ++l) {
+      double v = acc[m][l];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      acc[m][l] = v;
+    }
+  if (lane == 0) {
+    const int I = blk_I[cb], J = blk_J[cb];
+#pragma unroll
+    for (int m = 0; m < NM; ++m)
+#pragma unroll
+      for (int l = 0; l < NM; ++l) Ac[(int64_t)(I * NM + m) * ld + J * NM + l] = acc[m][l];
+  }
+}
+
+// Unit diagonal where a mode vanishes on the unknown DOFs (and in the padding).
+__global__ void k_coarse_guard(int ld, double* __restrict__ Ac) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < ld && Ac[(int64_t)k * ld + k] == 0.0) Ac[(int64_t)k * ld + k] = 1.0;
+}
+
+// ---------------------------------------------------------------------------
+// Blocked Gauss-Jordan inversion, pivot block k (rows/cols k*GB .. k*GB+GB-1).
+
+// Inverts the pivot block in LDS, writes it to Pb and in place.
+__global__ void __launch_bounds__(256) k_gj_pivot(double* __restrict__ A, int ld, int k, double* __restrict__ Pb) {
+  __shared__ double S[GB][GB + 1];
+  __shared__ double piv;
+  const int k0 = k * GB;
+  for (int e = threadIdx.x; e < GB * GB; e += 256) S[e / GB][e % GB] = A[(int64_t)(k0 + e / GB) * ld + k0 + e % GB];
+  __syncthreads();
+  for (int s = 0; s < GB; ++s) {
+    if (threadIdx.x == 0) piv = 1.0 / S[s][s];
+    __syncthreads();
+    const double p = piv;
+    double nv[GB * GB / 256];
+    for (int q = 0; q < GB * GB / 256; ++q) {
+      const int e = threadIdx.x + q * 256, i = e / GB, j = e % GB;
+      double v;
+      if (i == s && j == s) v = p;
+      else if (i == s) v = S[s][j] * p;
+      else if (j == s) v = -S[i][s] * p;
+      else v = S[i][j] - S[i][s] * S[s][j] * p;
+      nv[q] = v;
+    }
+    __syncthreads();
+    for (int q = 0; q < GB * GB / 256; ++q) {
+      const int e = threadIdx.x + q * 256;
+      S[e / GB][e % GB] = nv[q];
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < GB * GB; e += 256) {
+    const int i = e / GB, j = e % GB;
+    Pb[e] = S[i][j];
+    A[(int64_t)(k0 + i) * ld + k0 + j] = S[i][j];
+  }
+}
+
+// Row panel: A[k, j] <- P A[k, j] for every block column j != k.
+__global__ void __launch_bounds__(256) k_gj_row(double* __restrict__ A, int ld, int k, const double* __restrict__ Pb) {
+  __shared__ double Ps[GB][GB + 1];
+  __shared__ double T[GB][GB + 1];
+  const int nbk = ld / GB;
+  int jb = blockIdx.x;
+  if (jb >= k) ++jb;
+  if (jb >= nbk) return;
+  const int k0 = k * GB, j0 = jb * GB;
+  for (int e = threadIdx.x; e < GB * GB; e += 256) {
+    Ps[e / GB][e % GB] = Pb[e];
+    T[e / GB][e % GB] = A[(int64_t)(k0 + e / GB) * ld + j0 + e % GB];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < GB * GB; e += 256) {
+    const int i = e / GB, j = e % GB;
+    double s = 0;
+#pragma unroll 8
+    for (int t = 0; t < GB; ++t) s += Ps[i][t] * T[t][j];
+    A[(int64_t)(k0 + i) * ld + j0 + j] = s;
+  }
+}
+
+// Trailing update: A[i, j] -= A[i, k] A[k, j] for block rows i != k, block
+// columns j != k (A[k, j] already scaled by the row panel).
+__global__ void __launch_bounds__(256) k_gj_update(double* __restrict__ A, int ld, int k) {
+  __shared__ double L[GB][GB + 1];
+  __shared__ double R[GB][GB + 1];
+  const int nbk = ld / GB;
+  int ib = blockIdx.y, jb = blockIdx.x;
+  if (ib >= k) ++ib;
+  if (jb >= k) ++jb;
+  if (ib >= nbk || jb >= nbk) return;
+  const int k0 = k * GB, i0 = ib * GB, j0 = jb * GB;
+  for (int e = threadIdx.x; e < GB * GB; e += 256) {
+    L[e / GB][e % GB] = A[(int64_t)(i0 + e / GB) * ld + k0 + e % GB];
+    R[e / GB][e % GB] = A[(int64_t)(k0 + e / GB) * ld + j0 + e % GB];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < GB * GB; e += 256) {
+    const int i = e / GB, j = e % GB;
+    double s = 0;
+#pragma unroll 8
+    for (int t = 0; t < GB; ++t) s += L[i][t] * R[t][j];
+    A[(int64_t)(i0 + i) * ld + j0 + j] -= s;
+  }
+}
+
+// Column panel: A[i, k] <- -A[i, k] P for every block row i != k.
+__global__ void __launch_bounds__(256) k_gj_col(double* __restrict__ A, int ld, int k, const double* __restrict__ Pb) {
+  __shared__ double Ps[GB][GB + 1];
+  __shared__ double T[GB][GB + 1];
+  const int nbk = ld / GB;
+  int ib = blockIdx.x;
+  if (ib >= k) ++ib;
+  if (ib >= nbk) return;
+  const int k0 = k * GB, i0 = ib * GB;
+  for (int e = threadIdx.x; e < GB * GB; e += 256) {
+    Ps[e / GB][e % GB] = Pb[e];
+    T[e / GB][e % GB] = A[(int64_t)(i0 + e / GB) * ld + k0 + e % GB];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < GB * GB; e += 256) {
+    const int i = e / GB, j = e % GB;
+    double s = 0;
+#pragma unroll 8
+    for (int t = 0; t < GB; ++t) s += T[i][t] * Ps[t][j];
+    A[(int64_t)(i0 + i) * ld + k0 + j] = -s;
+  }
+}
+
+// X <- (X + X^T) / 2 (upper triangle pairs).
+__global__ void k_symmetrize(double* __restrict__ A, int ld) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)ld * ld) return;
+  const int i = (int)(e / ld), j = (int)(e % ld);
+  if (j <= i) return;
+  const double v = 0.5 * (A[(int64_t)i * ld + j] + A[(int64_t)j * ld + i]);
+  A[(int64_t)i * ld + j] = v;
+  A[(int64_t)j * ld + i] = v;
+}
+
+template <int DN>
+static int coarse_setup_t(int ncb, const int* pair_ptr, const int* pair_node, const int* pair_pos,
+                          const int* blk_I, const int* blk_J, const int* node_ptr, const int* node_cols,
+                          const double* vals, const unsigned char* known, const double* rel, int ld, double* Ac,
+                          double* Pb, hipStream_t s) {
+  const int64_t n2 = (int64_t)ld * ld;
+  hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, n2, Ac, 0.0);
+  if (ncb > 0)
+    hipLaunchKernelGGL((k_coarse_assemble<DN>), dim3((unsigned)((ncb + 3) / 4)), dim3(256), 0, s, ncb, pair_ptr,
+                       pair_node, pair_pos, blk_I, blk_J, node_ptr, node_cols, vals, known, rel, Ac, ld);
+  hipLaunchKernelGGL(k_coarse_guard, dim3((unsigned)((ld + 255) / 256)), dim3(256), 0, s, ld, Ac);
+  const int nbk = ld / GB;
+  for (int k = 0; k < nbk; ++k) {
+    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(256), 0, s, Ac, ld, k, Pb);
+    if (nbk > 1) {
+      hipLaunchKernelGGL(k_gj_row, dim3(nbk - 1), dim3(256), 0, s, Ac, ld, k, (const double*)Pb);
+      hipLaunchKernelGGL(k_gj_update, dim3(nbk - 1, nbk - 1), dim3(256), 0, s, Ac, ld, k);
+      hipLaunchKernelGGL(k_gj_col, dim3(nbk - 1), dim3(256), 0, s, Ac, ld, k, (const double*)Pb);
+    }
+  }
+  hipLaunchKernelGGL(k_symmetrize, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, Ac, ld);
+  return check_launch("coarse_setup");
+}
+
+int coarse_setup(int dof_n, int ncb, const int* pair_ptr, const int* pair_node, const int* pair_pos,
+                 const int* blk_I, const int* blk_J, const int* node_ptr, const int* node_cols, const double* vals,
+                 const unsigned char* known, const double* rel, int ld, double* Ac, double* Pb, hipStream_t s) {
+  if (ld % GB != 0) {
+    set_error("coarse_setup: leading dimension %d not a multiple of %d", ld, GB);
+    return EFEA_ERR_ARG;
+  }
+  if (dof_n == 1)
+    return coarse_setup_t<1>(ncb, pair_ptr, pair_node, pair_pos, blk_I, blk_J, node_ptr, node_cols, vals, known,
+                             rel, ld, Ac, Pb, s);
+  if (dof_n == 2)
+    return coarse_setup_t<2>(ncb, pair_ptr, pair_node, pair_pos, blk_I, blk_J, node_ptr, node_cols, vals, known,
+                             rel, ld, Ac, Pb, s);
+  if (dof_n == 3)
+    return coarse_setup_t<3>(ncb, pair_ptr, pair_node, pair_pos, blk_I, blk_J, node_ptr, node_cols, vals, known,
+                             rel, ld, Ac, Pb, s);
+  set_error("coarse_setup: dof_n %d", dof_n);
+  return EFEA_ERR_ARG;
+}
+
+int coarse_ld(int nagg, int dof_n) {
+  const int nc = nagg * (dof_n == 1 ? 1 : (dof_n == 2 ? 3 : 6));
+  return (nc + GB - 1) / GB * GB;
+}
+
+// ---------------------------------------------------------------------------
+// Per-iteration kernels on the aggregate grid (block = aggregate).
+
+template <typename TP, int DN>
+__global__ void __launch_bounds__(BS)
+k_pcg2_step2(CoarseArgs C, const unsigned char* __restrict__ known, const TP* __restrict__ Minv,
+             double* __restrict__ x, double* __restrict__ r, double* __restrict__ z, const double* __restrict__ P0,
+             const double* __restrict__ P1, const double* __restrict__ q, const double* __restrict__ part1, int nb1,
+             double* __restrict__ part2, PcgState* st) {
+  pcg2_step2_body<TP, DN>(C, known, Minv, x, r, z, P0, P1, q, part1, nb1, part2, st, blockIdx.x);
+}
+
+template <int DN>
+__global__ void __launch_bounds__(BS)
+k_pcg2_prolong(CoarseArgs C, const unsigned char* __restrict__ known, double* __restrict__ z,
+               double* __restrict__ part2, PcgState* st) {
+  pcg2_prolong_body<DN>(C, known, z, part2, st, blockIdx.x);
+}
+
+template <typename TP, int DN>
+__global__ void __launch_bounds__(BS)
+k_pcg2_init(CoarseArgs C, const unsigned char* __restrict__ known, const TP* __restrict__ Minv,
+            const double* __restrict__ r, double* __restrict__ z, double* __restrict__ P0,
+            const double* __restrict__ partI, int nbI, double* __restrict__ part2, PcgState* st) {
+  pcg2_init_body<TP, DN>(C, known, Minv, r, z, P0, partI, nbI, part2, st, blockIdx.x);
+}
+
+template <typename TP, int DN>
+__global__ void __launch_bounds__(BS)
+k_pcg2_replace(CoarseArgs C, SpmvArgs A64, const TP* __restrict__ Minv, const double* __restrict__ x,
+               const double* __restrict__ b, double* __restrict__ r, double* __restrict__ z,
+               double* __restrict__ part2, PcgState* st) {
+  pcg2_replace_body<TP, DN>(C, A64, Minv, x, b, r, z, part2, st, blockIdx.x);
+}
+
+// Step 1 on a two-level solve: the one-level kernel (partials of step 2 now
+// come from nagg blocks).
+template <typename TM, int DN, int G>
+__global__ void __launch_bounds__(BS)
+k_pcg2_step1(SpmvArgs A, const double* __restrict__ z, const double* __restrict__ P0, const double* __restrict__ P1,
+             double* __restrict__ q, const double* __restrict__ part2, int nb2, double* __restrict__ part1,
+             PcgState* st) {
+  pcg_step1_body<TM, DN, G>(A, z, P0, P1, q, part2, nb2, part1, st, blockIdx.x, gridDim.x);
+}
+
+__global__ void k_state_reset2(PcgState* st, double tolr2, double tolA2, double delta2) {
+  if (threadIdx.x == 0) {
+    st->rz = st->rz_prev = st->alpha = st->beta = st->pq = 0;
+    st->rr = st->rrmax = st->rrmax_prev = st->bb = 0;
+    st->tolr2 = tolr2;
+    st->tolA2 = tolA2;
+    st->delta2 = delta2;
+    st->Sall = st->Sw = st->xAx0 = 0;
+    for (int i = 0; i < RING; ++i) st->ring[i] = 0;
+    st->head = st->done = st->iters = st->nrepl = 0;
+    st->fresh = 1;
+    st->justrep = 0;
+    st->par = 0;
+    st->par_next = 1;
+  }
+}
+
+__global__ void __launch_bounds__(BS) k_pcg2_init_scalars(const double* __restrict__ part2, int nb2, PcgState* st) {
+  __shared__ double sh[BS / 64];
+  const double rz = reduce_all(part2, nb2, 2, 0, sh);
+  const double rr = reduce_all(part2, nb2, 2, 1, sh);
+  if (threadIdx.x == 0) {
+    st->rz = rz;
+    st->rr = rr;
+    st->rrmax = rr;
+    st->bb = rr;
+    st->done = (rr == 0) ? 1 : 0;
+  }
+}
+
+// x0.A x0 and r0 = b - A x0 on the node-group grid.
+template <int DN, int G>
+__global__ void __launch_bounds__(BS)
+k_xax_residual(SpmvArgs A, const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ r,
+               double* __restrict__ y, double* __restrict__ partI) {
+  __shared__ double sh[BS / 64];
+  double d = 0;
+  const int64_t lanes = (int64_t)A.Nnode * G;
+  for (int64_t gid = (int64_t)blockIdx.x * BS + threadIdx.x; gid < lanes; gid += (int64_t)gridDim.x * BS) {
+    const int n = (int)(gid / G), g = (int)(gid % G);
+    double dd, d2;
+    spmv_node<double, double, DN, G, 0>(A, n, g, x, nullptr, y, dd);
+    spmv_node<double, double, DN, G, 1>(A, n, g, x, b, r, d2);
+    d += dd;
+  }
+  d = block_sum<double, BS>(d, sh);
+  if (threadIdx.x == 0) partI[blockIdx.x] = d;
+}
+
+size_t pcg2_workspace_bytes(int Nnode, int dof_n, int nagg) { return pcg2_workspace_size(Nnode, dof_n, nagg); }
+
+static PcgState* host_state2() {
+  static thread_local PcgState* h = nullptr;
+  if (!h) {
+    if (hipHostMalloc((void**)&h, sizeof(PcgState), hipHostMallocDefault) != hipSuccess) h = nullptr;
+  }
+  return h;
+}
+
+template <typename TP, int DN, int G>
+int pcg2_start(const SpmvArgs& A64, const TP* Minv, const CoarseArgs& C, const double* b, const double* x,
+               const Pcg2Work& v, double tol_r, double tol_A, double delta2, hipStream_t s) {
+  const PcgWork& w = v.w;
+  CoarseArgs Cw = C;
+  Cw.rc = v.rc;
+  const unsigned nbS = grid_for((int64_t)A64.Nnode * G);
+  const unsigned nbA = (unsigned)C.nagg;
+  hipLaunchKernelGGL(k_state_reset2, dim3(1), dim3(64), 0, s, w.st, tol_r * tol_r, tol_A * tol_A, delta2);
+  hipLaunchKernelGGL((k_xax_residual<DN, G>), dim3(nbS), dim3(BS), 0, s, A64, x, b, w.r, w.q, w.partI);
+  hipLaunchKernelGGL((k_pcg2_init<TP, DN>), dim3(nbA), dim3(BS), 0, s, Cw, A64.known, Minv, (const double*)w.r, w.z,
+                     w.P0, (const double*)w.partI, (int)nbS, w.part2, w.st);
+  hipLaunchKernelGGL((k_pcg2_prolong<DN>), dim3(nbA), dim3(BS), 0, s, Cw, A64.known, w.z, w.part2, w.st);
+  hipLaunchKernelGGL(k_pcg2_init_scalars, dim3(1), dim3(BS), 0, s, (const double*)w.part2, (int)nbA, w.st);
+  return check_launch("pcg2_init");
+}
+
+#define EFEA_PCG2_START(TP, DN, G)                                                                        \
+  template int pcg2_start<TP, DN, G>(const SpmvArgs&, const TP*, const CoarseArgs&, const double*,         \
+                                     const double*, const Pcg2Work&, double, double, double, hipStream_t);
+EFEA_PCG2_START(double, 1, 8)
+EFEA_PCG2_START(double, 1, 16)
+EFEA_PCG2_START(double, 2, 8)
+EFEA_PCG2_START(double, 2, 16)
+EFEA_PCG2_START(double, 3, 16)
+EFEA_PCG2_START(double, 3, 32)
+EFEA_PCG2_START(float, 1, 8)
+EFEA_PCG2_START(float, 1, 16)
+EFEA_PCG2_START(float, 2, 8)
+EFEA_PCG2_START(float, 2, 16)
+EFEA_PCG2_START(float, 3, 16)
+EFEA_PCG2_START(float, 3, 32)
+#undef EFEA_PCG2_START
+
+template <typename TM, typename TP, int DN, int G>
+static int pcg2_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, const CoarseArgs& C, const double* b,
+                    double* x, void* ws, double tol_r, double tol_A, double delta, int maxit, int check_every,
+                    hipStream_t s, double* info) {
+  const int Nnode = A64.Nnode;
+  const Pcg2Work v = carve2(ws, Nnode, DN, C.nagg);
+  const PcgWork& w = v.w;
+  CoarseArgs Cw = C;
+  Cw.rc = v.rc;
+  const unsigned nbS = grid_for((int64_t)Nnode * G);
+  const unsigned nbA = (unsigned)C.nagg;
+  const bool mixed = (void*)AK.vals != (void*)A64.vals;
+  if (int e = pcg2_start<TP, DN, G>(A64, Minv, C, b, x, v, tol_r, tol_A, mixed ? delta * delta : 0.0, s)) return e;
+  PcgState* hs = host_state2();
+  if (!hs) {
+    set_error("pcg2: pinned host allocation failed");
+    return EFEA_ERR_HIP;
+  }
+  int launched = 0;
+  while (true) {
+    for (int k = 0; k < check_every; ++k) {
+      hipLaunchKernelGGL((k_pcg2_step1<TM, DN, G>), dim3(nbS), dim3(BS), 0, s, AK, (const double*)w.z,
+                         (const double*)w.P0, (const double*)w.P1, w.q, (const double*)w.part2, (int)nbA, w.part1,
+                         w.st);
+      hipLaunchKernelGGL((k_pcg2_step2<TP, DN>), dim3(nbA), dim3(BS), 0, s, Cw, A64.known, Minv, x, w.r, w.z,
+                         (const double*)w.P0, (const double*)w.P1, (const double*)w.q, (const double*)w.part1,
+                         (int)nbS, w.part2, w.st);
+      hipLaunchKernelGGL((k_pcg2_prolong<DN>), dim3(nbA), dim3(BS), 0, s, Cw, A64.known, w.z, w.part2, w.st);
+    }
+    launched += check_every;
+    if (int e = check_launch("pcg2_iter")) return e;
+    if (hipMemcpyAsync(hs, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      set_error("pcg2: state readback failed");
+      return EFEA_ERR_HIP;
+    }
+    if (hs->done == 4) {
+      hipLaunchKernelGGL((k_pcg2_replace<TP, DN>), dim3(nbA), dim3(BS), 0, s, Cw, A64, Minv, (const double*)x, b,
+                         w.r, w.z, w.part2, w.st);
+      hipLaunchKernelGGL((k_pcg2_prolong<DN>), dim3(nbA), dim3(BS), 0, s, Cw, A64.known, w.z, w.part2, w.st);
+      if (int e = check_launch("pcg2_replace")) return e;
+      if (launched >= maxit) break;
+      continue;
+    }
+    if (hs->done || launched >= maxit) break;
+  }
+  const PcgState fin = *hs;
+  pcg_true_residual<DN, G>(A64, b, x, w, s);
+  double rr_true = 0;
+  if (hipMemcpyAsync(&rr_true, w.small, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    set_error("pcg2: residual readback failed");
+    return EFEA_ERR_HIP;
+  }
+  return pcg_finish_info(&fin, rr_true, info);
+}
+
+int solve_pcg2(int mode, int Nnode, int dof_n, int group, const int* node_ptr, const int* node_cols,
+               const double* vals64, const float* vals32, const unsigned char* known, const void* Minv, int nagg,
+               const int* agg_ptr, const int* agg_nodes, const double* rel, const double* Ainv, int ld,
+               const double* b, double* x, void* ws, double tol_r, double tol_A, double delta, int maxit,
+               int check_every, hipStream_t s, double* info) {
+  SpmvArgs A64{Nnode, node_ptr, node_cols, vals64, known};
+  SpmvArgs A32{Nnode, node_ptr, node_cols, vals32, known};
+  CoarseArgs C{nagg, agg_ptr, agg_nodes, rel, Ainv, ld, nullptr};
+  if (check_every < 1) check_every = 1;
+#define RUN2(D, GV)                                                                                            \
+  if (dof_n == D && group == GV) {                                                                            \
+    if (mode == 64)                                                                                           \
+      return pcg2_run<double, double, D, GV>(A64, A64, (const double*)Minv, C, b, x, ws, tol_r, tol_A, delta, \
+                                             maxit, check_every, s, info);                                    \
+    return pcg2_run<float, float, D, GV>(A64, A32, (const float*)Minv, C, b, x, ws, tol_r, tol_A, delta,      \
+                                         maxit, check_every, s, info);                                        \
+  }
+  if (mode == 64 || (mode == 32 && vals32)) {
+    RUN2(1, 8) RUN2(1, 16) RUN2(2, 8) RUN2(2, 16) RUN2(3, 16) RUN2(3, 32)
+  }
+#undef RUN2
+  set_error("solve_pcg2: unsupported mode %d / dof_n %d / group %d", mode, dof_n, group);
+  return EFEA_ERR_ARG;
+}
+
+}  // namespace efea

@@ -1,0 +1,297 @@
+// Device-side pieces of the two-level preconditioner shared by the
+// single-system kernels (coarse.hip) and the batch engine (batch.hip): the
+// rigid-body modes, the per-aggregate bodies of step 2 (with restriction),
+// the prolongation, the initial and the replacement residual passes, and the
+// workspace layout.  Each body takes the aggregate index `blk` of its block.
+#pragma once
+#include "pcg_device.h"
+
+namespace efea {
+
+// Rigid-body mode m of DOF component a at a node with coordinates r relative
+// to its aggregate centroid.
+template <int DN>
+__device__ inline double pmode(const double* __restrict__ r, int a, int m) {
+  if constexpr (DN == 1) {
+    return 1.0;
+  } else if constexpr (DN == 2) {
+    if (m < 2) return a == m ? 1.0 : 0.0;
+    return a == 0 ? -r[1] : r[0];
+  } else {
+    if (m < 3) return a == m ? 1.0 : 0.0;
+    // rotations about x, y, z: w x r
+    const int w = m - 3;
+    if (a == w) return 0.0;
+    if (w == 0) return a == 1 ? -r[2] : r[1];
+    if (w == 1) return a == 0 ? r[2] : -r[0];
+    return a == 0 ? -r[1] : r[0];
+  }
+}
+
+template <int DN> struct Modes { static constexpr int NM = DN == 1 ? 1 : (DN == 2 ? 3 : 6); };
+
+struct CoarseArgs {
+  int nagg;
+  const int* agg_ptr;
+  const int* agg_nodes;
+  const double* rel;
+  const double* Ainv;  // ld x ld
+  int ld;
+  double* rc;  // nagg * NM
+};
+
+// Step 2 with restriction: alpha = rz / pq ; x += alpha p ; r -= alpha q ;
+// z = B^-1 r ; partials of r.z (block Jacobi part) and r.r ; rc_I = P_I^T r.
+template <typename TP, int DN>
+__device__ inline void pcg2_step2_body(CoarseArgs C, const unsigned char* __restrict__ known, const TP* __restrict__ Minv,
+             double* __restrict__ x, double* __restrict__ r, double* __restrict__ z, const double* __restrict__ P0,
+             const double* __restrict__ P1, const double* __restrict__ q, const double* __restrict__ part1, int nb1,
+             double* __restrict__ part2, PcgState* st, int blk) {
+  constexpr int NM = Modes<DN>::NM;
+  if (st->done) return;
+  __shared__ double sh[BS / 64];
+  __shared__ double sh2[BS / 64];
+  __shared__ double bc;
+  double ppq = 0;
+  for (int i = threadIdx.x; i < nb1; i += BS) ppq += part1[i];
+  ppq = block_sum<double, BS>(ppq, sh);
+  if (threadIdx.x == 0) bc = ppq;
+  __syncthreads();
+  const double pq = bc;
+  const double rz = st->rz;
+  if (!(pq > 0) || !(rz > 0)) {
+    if (blk == 0 && threadIdx.x == 0) st->done = (rz == 0) ? 1 : 2;
+    return;
+  }
+  const int pn = st->par_next;
+  const double* __restrict__ p = pn ? P1 : P0;
+  const double al = rz / pq;
+  if (blk == 0 && threadIdx.x == 0) {
+    st->alpha = al;
+    st->pq = pq;
+    st->rz_prev = rz;
+    st->rrmax_prev = st->rrmax;
+    st->fresh = 0;
+    st->justrep = 0;
+    st->par = pn;
+  }
+  double s_rz = 0, s_rr = 0, rcv[NM];
+#pragma unroll
+  for (int m = 0; m < NM; ++m) rcv[m] = 0;
+  const int k0 = C.agg_ptr[blk], k1 = C.agg_ptr[blk + 1];
+  for (int k = k0 + threadIdx.x; k < k1; k += BS) {
+    const int n = C.agg_nodes[k];
+    double r2[DN], zv[DN];
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      const int64_t i = (int64_t)n * DN + a;
+      x[i] += al * p[i];
+      r2[a] = r[i] - al * q[i];
+      r[i] = r2[a];
+    }
+    apply_bj<TP, DN>(Minv, known, n, r2, zv);
+    const double* rn = C.rel + 3 * (int64_t)n;
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      z[(int64_t)n * DN + a] = zv[a];
+      s_rz += r2[a] * zv[a];
+      s_rr += r2[a] * r2[a];
+#pragma unroll
+      for (int m = 0; m < NM; ++m) rcv[m] += pmode<DN>(rn, a, m) * r2[a];  // r = 0 at known DOFs
+    }
+  }
+  s_rz = block_sum<double, BS>(s_rz, sh);
+  s_rr = block_sum<double, BS>(s_rr, sh2);
+  if (threadIdx.x == 0) {
+    part2[2 * blk] = s_rz;
+    part2[2 * blk + 1] = s_rr;
+  }
+#pragma unroll
+  for (int m = 0; m < NM; ++m) {
+    const double v = block_sum<double, BS>(rcv[m], sh);
+    if (threadIdx.x == 0) C.rc[blk * NM + m] = v;
+  }
+}
+
+// z += P_I zc_I with zc_I = (Ac^-1)_I rc ; r.z partial += rc_I . zc_I.
+template <int DN>
+__device__ inline void pcg2_prolong_body(CoarseArgs C, const unsigned char* __restrict__ known, double* __restrict__ z,
+               double* __restrict__ part2, PcgState* st, int blk) {
+  constexpr int NM = Modes<DN>::NM;
+  if (st->done) return;
+  __shared__ double sh[BS / 64];
+  __shared__ double zcs[NM];
+  const int nc = C.nagg * NM;
+  for (int m = 0; m < NM; ++m) {
+    const double* row = C.Ainv + (int64_t)(blk * NM + m) * C.ld;
+    double s = 0;
+    for (int c = threadIdx.x; c < nc; c += BS) s += row[c] * C.rc[c];
+    s = block_sum<double, BS>(s, sh);
+    if (threadIdx.x == 0) zcs[m] = s;
+  }
+  __syncthreads();
+  double zc[NM];
+#pragma unroll
+  for (int m = 0; m < NM; ++m) zc[m] = zcs[m];
+  const int k0 = C.agg_ptr[blk], k1 = C.agg_ptr[blk + 1];
+  for (int k = k0 + threadIdx.x; k < k1; k += BS) {
+    const int n = C.agg_nodes[k];
+    const double* rn = C.rel + 3 * (int64_t)n;
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      const int64_t i = (int64_t)n * DN + a;
+      if (known && known[i]) continue;
+      double v = 0;
+#pragma unroll
+      for (int m = 0; m < NM; ++m) v += pmode<DN>(rn, a, m) * zc[m];
+      z[i] += v;
+    }
+  }
+  if (threadIdx.x == 0) {
+    double d = 0;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) d += C.rc[blk * NM + m] * zc[m];
+    part2[2 * blk] += d;
+  }
+}
+
+// Initial z = B^-1 r, P0 = 0, partials of r.z and r.r and the restriction;
+// block 0 also folds the x0.A x0 partials into the state.
+template <typename TP, int DN>
+__device__ inline void pcg2_init_body(CoarseArgs C, const unsigned char* __restrict__ known, const TP* __restrict__ Minv,
+            const double* __restrict__ r, double* __restrict__ z, double* __restrict__ P0,
+            const double* __restrict__ partI, int nbI, double* __restrict__ part2, PcgState* st, int blk) {
+  constexpr int NM = Modes<DN>::NM;
+  __shared__ double sh[BS / 64];
+  __shared__ double sh2[BS / 64];
+  if (blk == 0) {
+    const double xax = reduce_all(partI, nbI, 1, 0, sh);
+    if (threadIdx.x == 0) st->xAx0 = xax;
+  }
+  double s_rz = 0, s_rr = 0, rcv[NM];
+#pragma unroll
+  for (int m = 0; m < NM; ++m) rcv[m] = 0;
+  const int k0 = C.agg_ptr[blk], k1 = C.agg_ptr[blk + 1];
+  for (int k = k0 + threadIdx.x; k < k1; k += BS) {
+    const int n = C.agg_nodes[k];
+    double rv[DN], zv[DN];
+#pragma unroll
+    for (int a = 0; a < DN; ++a) rv[a] = r[(int64_t)n * DN + a];
+    apply_bj<TP, DN>(Minv, known, n, rv, zv);
+    const double* rn = C.rel + 3 * (int64_t)n;
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      z[(int64_t)n * DN + a] = zv[a];
+      P0[(int64_t)n * DN + a] = 0.0;
+      s_rz += rv[a] * zv[a];
+      s_rr += rv[a] * rv[a];
+#pragma unroll
+      for (int m = 0; m < NM; ++m) rcv[m] += pmode<DN>(rn, a, m) * rv[a];
+    }
+  }
+  s_rz = block_sum<double, BS>(s_rz, sh);
+  s_rr = block_sum<double, BS>(s_rr, sh2);
+  if (threadIdx.x == 0) {
+    part2[2 * blk] = s_rz;
+    part2[2 * blk + 1] = s_rr;
+  }
+#pragma unroll
+  for (int m = 0; m < NM; ++m) {
+    const double v = block_sum<double, BS>(rcv[m], sh);
+    if (threadIdx.x == 0) C.rc[blk * NM + m] = v;
+  }
+}
+
+// Reliable update: r = b - A64 x, z = B^-1 r, partials, restriction; the
+// prolongation kernel follows.
+template <typename TP, int DN>
+__device__ inline void pcg2_replace_body(CoarseArgs C, SpmvArgs A64, const TP* __restrict__ Minv, const double* __restrict__ x,
+               const double* __restrict__ b, double* __restrict__ r, double* __restrict__ z,
+               double* __restrict__ part2, PcgState* st, int blk) {
+  constexpr int NM = Modes<DN>::NM;
+  __shared__ double sh[BS / 64];
+  __shared__ double sh2[BS / 64];
+  double s_rz = 0, s_rr = 0, rcv[NM];
+#pragma unroll
+  for (int m = 0; m < NM; ++m) rcv[m] = 0;
+  const int k0 = C.agg_ptr[blk], k1 = C.agg_ptr[blk + 1];
+  for (int k = k0 + threadIdx.x; k < k1; k += BS) {
+    const int n = C.agg_nodes[k];
+    double dd;
+    spmv_node<double, double, DN, 1, 1>(A64, n, 0, x, b, r, dd);
+    double rv[DN], zv[DN];
+#pragma unroll
+    for (int a = 0; a < DN; ++a) rv[a] = r[(int64_t)n * DN + a];
+    apply_bj<TP, DN>(Minv, A64.known, n, rv, zv);
+    const double* rn = C.rel + 3 * (int64_t)n;
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      z[(int64_t)n * DN + a] = zv[a];
+      s_rz += rv[a] * zv[a];
+      s_rr += rv[a] * rv[a];
+#pragma unroll
+      for (int m = 0; m < NM; ++m) rcv[m] += pmode<DN>(rn, a, m) * rv[a];
+    }
+  }
+  s_rz = block_sum<double, BS>(s_rz, sh);
+  s_rr = block_sum<double, BS>(s_rr, sh2);
+  if (threadIdx.x == 0) {
+    part2[2 * blk] = s_rz;
+    part2[2 * blk + 1] = s_rr;
+  }
+#pragma unroll
+  for (int m = 0; m < NM; ++m) {
+    const double v = block_sum<double, BS>(rcv[m], sh);
+    if (threadIdx.x == 0) C.rc[blk * NM + m] = v;
+  }
+  if (blk == 0 && threadIdx.x == 0) {
+    st->done = 0;
+    st->justrep = 1;
+    st->nrepl += 1;
+  }
+}
+
+// Workspace of a two-level solve: the one-level workspace (partials sized
+// for max(MAXB, nagg) blocks) plus the restriction vector.
+static inline size_t pcg2_workspace_size(int Nnode, int dof_n, int nagg) {
+  const int nb = nagg > MAXB ? nagg : MAXB;
+  const int nm = dof_n == 1 ? 1 : (dof_n == 2 ? 3 : 6);
+  const size_t vb = align256(8 * (size_t)Nnode * dof_n);
+  return 5 * vb + align256(8 * (size_t)MAXB) * 2 + align256(16 * (size_t)nb) + align256(sizeof(PcgState)) + 256 +
+         align256(8 * (size_t)nagg * nm);
+}
+
+// The one-level work arrays (rc in `small`'s place is separate) of a
+// two-level workspace.
+struct Pcg2Work {
+  PcgWork w;
+  double* rc;
+};
+
+static inline Pcg2Work carve2(void* ws, int Nnode, int dof_n, int nagg) {
+  const int nb = nagg > MAXB ? nagg : MAXB;
+  char* c = (char*)ws;
+  const size_t vb = align256(8 * (size_t)Nnode * dof_n);
+  Pcg2Work v;
+  PcgWork& w = v.w;
+  w.r = (double*)c; c += vb;
+  w.z = (double*)c; c += vb;
+  w.P0 = (double*)c; c += vb;
+  w.P1 = (double*)c; c += vb;
+  w.q = (double*)c; c += vb;
+  w.part1 = (double*)c; c += align256(8 * (size_t)MAXB);
+  w.partI = (double*)c; c += align256(8 * (size_t)MAXB);
+  w.part2 = (double*)c; c += align256(16 * (size_t)nb);
+  w.st = (PcgState*)c; c += align256(sizeof(PcgState));
+  w.small = (double*)c; c += 256;
+  v.rc = (double*)c;
+  return v;
+}
+
+// Initialisation of one two-level solve (state, x0.A x0, r0, z0 = M^-1 r0
+// with P0 = 0, scalars).  Defined in coarse.hip.
+template <typename TP, int DN, int G>
+int pcg2_start(const SpmvArgs& A64, const TP* Minv, const CoarseArgs& C, const double* b, const double* x,
+               const Pcg2Work& w, double tol_r, double tol_A, double delta2, hipStream_t s);
+
+}  // namespace efea

@@ -18,6 +18,7 @@ import torch
 from . import _lib
 from ._utils import ElemType, MatrixType
 from .assembly_map import CsrMap, VecMap, build_csr_map, build_vec_map
+from .coarse import aggregate, coarse_pairs, default_target, n_modes
 from .quadrature import N_pg, dN_pg, gauss_points
 
 
@@ -102,7 +103,8 @@ class DeviceMesh:
     def pattern(self, dof_n: int) -> "CsrPattern":
         p = self._shared.patterns
         if dof_n not in p:
-            p[dof_n] = CsrPattern(build_csr_map(self.connect_np, self.Nn, dof_n), self.Ne)
+            p[dof_n] = CsrPattern(build_csr_map(self.connect_np, self.Nn, dof_n), self.Ne, self.mesh.coord,
+                                  self.dim)
         return p[dof_n]
 
     def csr_map(self, dof_n: int) -> "DeviceCsr":
@@ -146,10 +148,14 @@ class DeviceVecMap:
 
 class CsrPattern:
     """Sparsity pattern and scatter map of a global matrix on the device: the
-    node adjacency the kernels use, the scatter segments and permutation."""
+    node adjacency the kernels use, the scatter segments and permutation, and
+    the coarse spaces built on it."""
 
-    def __init__(self, cmap: CsrMap, Ne: int):
+    def __init__(self, cmap: CsrMap, Ne: int, coord: Optional[np.ndarray] = None, dim: Optional[int] = None):
         self.host = cmap
+        self.coord = coord
+        self.dim = dim
+        self._coarse = {}
         self.Nn = cmap.Nn
         self.dof_n = cmap.dof_n
         self.nnz = cmap.nnz
@@ -161,6 +167,46 @@ class CsrPattern:
         self.group = _lib.load().efea_default_group(cmap.dof_n, cmap.max_neighbours)
         nd = cmap.nPe * cmap.dof_n
         self.n_entries = Ne * nd * nd
+
+    def coarse(self, target: Optional[int] = None) -> "CoarseSpace":
+        target = default_target(self.dim) if target is None else int(target)
+        if target not in self._coarse:
+            assert self.coord is not None, "the coarse space needs node coordinates"
+            self._coarse[target] = CoarseSpace(self, target)
+        return self._coarse[target]
+
+
+class CoarseSpace:
+    """Aggregates, rigid-body modes and the pair lists of ``Ac = P^T A P`` on
+    the device (mesh data: built once per pattern and aggregate size)."""
+
+    def __init__(self, pattern: "CsrPattern", target: int):
+        h = pattern.host
+        self.target = target
+        self.dof_n = pattern.dof_n
+        self.ag = aggregate(pattern.coord, h.node_ptr, h.node_cols, target)
+        cp = coarse_pairs(self.ag, h.node_ptr, h.node_cols)
+        self.nagg = self.ag.nagg
+        self.nm = n_modes(pattern.dim, pattern.dof_n)
+        self.ld = _lib.call("efea_coarse_ld", self.nagg, self.dof_n)
+        self.agg_ptr = to_dev(self.ag.agg_ptr)
+        self.agg_nodes = to_dev(self.ag.agg_nodes)
+        self.rel = to_dev(self.ag.rel)
+        self.ncb = int(cp.blk_I.size)
+        self.pair_ptr = to_dev(cp.pair_ptr)
+        self.pair_node = to_dev(cp.pair_node)
+        self.pair_pos = to_dev(cp.pair_pos)
+        self.blk_I = to_dev(cp.blk_I)
+        self.blk_J = to_dev(cp.blk_J)
+        self.pivot = torch.empty(1024, dtype=torch.float64, device="cuda")
+
+    def setup(self, A: "DeviceCsr", known: Optional[torch.Tensor]) -> torch.Tensor:
+        """``Ac^-1`` (ld x ld, FP64) of ``A``'s current FP64 values."""
+        Ainv = torch.empty(self.ld * self.ld, dtype=torch.float64, device="cuda")
+        _lib.call("efea_coarse_setup", self.dof_n, self.ncb, ptr(self.pair_ptr), ptr(self.pair_node),
+                  ptr(self.pair_pos), ptr(self.blk_I), ptr(self.blk_J), ptr(A.node_ptr), ptr(A.node_cols),
+                  ptr(A.values), ptr(known), ptr(self.rel), self.ld, ptr(Ainv), ptr(self.pivot), stream_ptr())
+        return Ainv
 
 
 class DeviceCsr:
@@ -180,9 +226,13 @@ class DeviceCsr:
         self.perm = pattern.perm
         self.group = pattern.group
         self.n_entries = pattern.n_entries
+        self.dim = pattern.dim
         self.values = torch.empty(self.nnz, dtype=torch.float64, device="cuda")
         self.values32: Optional[torch.Tensor] = None
         self._spmv_scratch: Optional[torch.Tensor] = None
+
+    def coarse(self, target: Optional[int] = None) -> CoarseSpace:
+        return self.pattern.coarse(target)
 
     @property
     def spmv_scratch(self) -> torch.Tensor:
@@ -245,6 +295,19 @@ def effective_precision(A: "DeviceCsr", mode: str, adaptive: bool = True) -> int
     return 32
 
 
+def choose_preconditioner(A: "DeviceCsr", precond: str = "auto") -> str:
+    """"bj" (nodal block Jacobi) or "two-level" (block Jacobi + rigid-body
+    coarse correction).  auto: two-level for vector problems with at least
+    four aggregates of the default size, block Jacobi otherwise (scalar
+    damage problems converge in tens of iterations with it)."""
+    assert precond in ("auto", "bj", "two-level"), precond
+    if precond != "auto":
+        return precond
+    if A.dof_n >= 2 and A.dim is not None and A.Nn >= 4 * default_target(A.dim):
+        return "two-level"
+    return "bj"
+
+
 @dataclass
 class SolveInfo:
     iterations: int
@@ -253,6 +316,7 @@ class SolveInfo:
     status: int
     mode: str
     fallback: bool = False
+    precond: str = "bj"
 
 
 class Workspace:
@@ -275,8 +339,12 @@ class SolveJob:
     def __init__(self, A: "DeviceCsr", b: torch.Tensor, x: torch.Tensor, known: Optional[torch.Tensor],
                  mode: str = "mixed", tol_residual: float = 1e-12, tol_anorm: float = 1e-9,
                  replace_ratio: float = 1e-3, maxit: int = 100000, check_every: int = 32, adaptive: bool = True,
-                 owner=None):
+                 owner=None, precond: str = "auto", coarse_target: Optional[int] = None):
         self.A, self.b, self.x, self.known = A, b, x, known
+        self.precond = choose_preconditioner(A, precond)
+        self.coarse_target = coarse_target
+        self.cs: Optional[CoarseSpace] = None
+        self.Ainv: Optional[torch.Tensor] = None
         self.mode = mode
         self.tol_residual, self.tol_anorm, self.replace_ratio = tol_residual, tol_anorm, replace_ratio
         self.maxit, self.check_every = maxit, check_every
@@ -296,6 +364,9 @@ class SolveJob:
                   ptr(self.known), ptr(self.Minv), stream_ptr())
         if precision == 32:
             A.make_fp32()
+        if self.precond == "two-level" and self.Ainv is None:
+            self.cs = A.coarse(self.coarse_target)
+            self.Ainv = self.cs.setup(A, self.known)
 
     def restart_fp64(self) -> None:
         """After a failed mixed solve: the initial guess back, FP64 setup."""
@@ -305,14 +376,24 @@ class SolveJob:
 
     def info(self, raw) -> SolveInfo:
         return SolveInfo(int(raw[0]), float(raw[1]), int(raw[2]), int(raw[3]),
-                         "fp64" if self.precision == 64 else "mixed", self.fallback)
+                         "fp64" if self.precision == 64 else "mixed", self.fallback, self.precond)
 
     def _solve_once(self, ws: Workspace):
         A = self.A
-        nbytes = _lib.load().efea_pcg_workspace_bytes(A.Nn, A.dof_n, A.group)
-        buf = ws.get(nbytes)
         raw = (C.c_double * 4)()
         v32 = A.values32 if self.precision == 32 else None
+        if self.precond == "two-level":
+            cs = self.cs
+            nbytes = _lib.load().efea_pcg2_workspace_bytes(A.Nn, A.dof_n, cs.nagg)
+            buf = ws.get(nbytes)
+            rc = _lib.call("efea_solve_pcg2", self.precision, A.Nn, A.dof_n, A.group, ptr(A.node_ptr),
+                           ptr(A.node_cols), ptr(A.values), ptr(v32), ptr(self.known), ptr(self.Minv), cs.nagg,
+                           ptr(cs.agg_ptr), ptr(cs.agg_nodes), ptr(cs.rel), ptr(self.Ainv), cs.ld, ptr(self.b),
+                           ptr(self.x), ptr(buf), nbytes, self.tol_residual, self.tol_anorm, self.replace_ratio,
+                           self.maxit, self.check_every, stream_ptr(), C.cast(raw, C.c_void_p))
+            return rc, self.info(raw)
+        nbytes = _lib.load().efea_pcg_workspace_bytes(A.Nn, A.dof_n, A.group)
+        buf = ws.get(nbytes)
         rc = _lib.call("efea_solve_pcg", self.precision, A.Nn, A.dof_n, A.group, ptr(A.node_ptr),
                        ptr(A.node_cols), ptr(A.values), ptr(v32), ptr(self.known), ptr(self.Minv), ptr(self.b),
                        ptr(self.x), ptr(buf), nbytes, self.tol_residual, self.tol_anorm, self.replace_ratio,
@@ -361,10 +442,11 @@ class BatchSolver:
     """Continuous batching over ``efea_batch_*``: the jobs of one (precision,
     dof_n, group) class advance together, one launch per PCG step."""
 
-    def __init__(self, precision: int, dof_n: int, group: int, capacity: int = 256):
+    def __init__(self, precision: int, dof_n: int, group: int, precond: str = "bj", capacity: int = 256):
         self.precision, self.dof_n, self.group, self.capacity = precision, dof_n, group, capacity
+        self.precond = precond
         h = C.c_void_p()
-        _lib.call("efea_batch_create", precision, dof_n, group, capacity, C.byref(h))
+        _lib.call("efea_batch_create", precision, dof_n, group, int(precond == "two-level"), capacity, C.byref(h))
         self._h = h
         self._jobs = {}
 
@@ -377,11 +459,18 @@ class BatchSolver:
     def add(self, job: SolveJob) -> int:
         A = job.A
         assert job.precision == self.precision and A.dof_n == self.dof_n and A.group == self.group
-        nbytes = _lib.load().efea_pcg_workspace_bytes(A.Nn, A.dof_n, A.group)
+        assert job.precond == self.precond
+        cs = job.cs
+        if cs is not None:
+            nbytes = _lib.load().efea_pcg2_workspace_bytes(A.Nn, A.dof_n, cs.nagg)
+            coarse = (cs.nagg, ptr(cs.agg_ptr), ptr(cs.agg_nodes), ptr(cs.rel), ptr(job.Ainv), cs.ld)
+        else:
+            nbytes = _lib.load().efea_pcg_workspace_bytes(A.Nn, A.dof_n, A.group)
+            coarse = (0, None, None, None, None, 0)
         job.ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
         v32 = A.values32 if self.precision == 32 else None
         jid = _lib.call("efea_batch_add", self._h, A.Nn, ptr(A.node_ptr), ptr(A.node_cols), ptr(A.values), ptr(v32),
-                        ptr(job.known), ptr(job.Minv), ptr(job.b), ptr(job.x), ptr(job.ws), nbytes,
+                        ptr(job.known), ptr(job.Minv), *coarse, ptr(job.b), ptr(job.x), ptr(job.ws), nbytes,
                         job.tol_residual, job.tol_anorm, job.replace_ratio, job.maxit, stream_ptr())
         self._jobs[jid] = job
         return jid
@@ -415,8 +504,8 @@ class BatchSolver:
 
 
 class BatchExecutor:
-    """Routes solve jobs to one ``BatchSolver`` per (precision, dof_n, group)
-    class and runs them ``check_every`` iterations at a time.  A mixed job
+    """Routes solve jobs to one ``BatchSolver`` per (precision, dof_n, group,
+    preconditioner) class and runs them ``check_every`` iterations at a time.  A mixed job
     that fails is restarted in FP64 from its initial guess, as the single
     path does."""
 
@@ -427,7 +516,7 @@ class BatchExecutor:
 
     def submit(self, job: SolveJob) -> None:
         job.setup(job.precision)
-        key = (job.precision, job.A.dof_n, job.A.group)
+        key = (job.precision, job.A.dof_n, job.A.group, job.precond)
         b = self.batches.get(key)
         if b is None:
             b = self.batches[key] = BatchSolver(*key, capacity=self.capacity)

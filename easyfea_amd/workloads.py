@@ -150,11 +150,12 @@ def square_quad4(side: int) -> Mesh:
     return quad4_rectangle(1.0, 1.0, side, side)
 
 
-def elastic_request(spec: ElasticRequestSpec, mesh: Mesh, mode: str):
+def elastic_request(spec: ElasticRequestSpec, mesh: Mesh, mode: str, precond: str = "auto"):
     """The request's work as a generator of solve jobs; returns (u, info)."""
     mat = Models.Elastic.Isotropic(2, E=spec.E, v=0.3, planeStress=True, thickness=1.0)
     simu = Simulations.Elastic(mesh, mat)
     simu.solver = "hip_fp64" if mode == "fp64" else "hip_mixed"
+    simu.preconditioner = precond
     left, right = _square_sides(mesh)
     simu.add_dirichlet(left, [0, 0], ["x", "y"])
     simu.add_surfLoad(right, [1.0], ["x"])
@@ -184,14 +185,15 @@ class PhaseFieldRequestSpec:
 
 def serving_phasefield_specs(n: int = 64, steps: int = 50, ne_lo: int = 20000, ne_hi: int = 80000,
                              seed: int = 0) -> list:
-    """Realisations: Ne ~ U[ne_lo, ne_hi] -> cells = round(sqrt(Ne/2)); Gc a
+    """Realisations: Ne ~ U[ne_lo, ne_hi] -> cells = sqrt(Ne/2) rounded to an
+    even count (the notch ends at the centre node); Gc a
     correlated lognormal field (mean 2.7, CoV 0.15, correlation length
     0.05 mm); all arrive at t = 0 and are admitted by memory budget."""
     rng = np.random.default_rng(seed)
     ne = rng.uniform(ne_lo, ne_hi, n)
     out = []
     for rid in range(n):
-        cells = int(round(math.sqrt(ne[rid] / 2.0)))
+        cells = 2 * int(round(math.sqrt(ne[rid] / 2.0) / 2.0))
         z = gaussian_field((cells, cells), 0.05 * cells, np.random.default_rng(10_000 + rid))
         out.append(PhaseFieldRequestSpec(rid, cells, z.T.ravel(), 0.0, steps))
     return out

@@ -117,22 +117,54 @@ int efea_solve_pcg(int mode, int Nnode, int dof_n, int group, const int32_t* nod
                    double tol_anorm, double replace_ratio, int maxit, int check_every, void* stream,
                    double* info);
 
+/* Two-level preconditioner: M^-1 = B^-1 + P Ac^-1 P^T with B the nodal block
+ * Jacobi and P the rigid-body modes (2D: 3, 3D: 6, scalar: 1 per aggregate) of
+ * node aggregates, zero at known DOFs; rel (Nnode, 3) holds node coordinates
+ * relative to their aggregate's centroid.
+ * efea_coarse_setup assembles Ac = P^T A P from (node, neighbour position)
+ * pairs grouped by aggregate block (blk_I[k], blk_J[k]) = pairs
+ * pair_ptr[k]..pair_ptr[k+1]-1, and inverts it in place (blocked Gauss-Jordan,
+ * FP64) into Ainv (ld x ld, ld = efea_coarse_ld; pivot_scratch: 1024 doubles).
+ * efea_solve_pcg2 is efea_solve_pcg with that preconditioner; agg_ptr /
+ * agg_nodes list the nodes of each aggregate.  EasyFEA solves the reduced
+ * system directly (Solvers.py:225-394); the preconditioner changes the
+ * iteration count, not the solution. */
+int efea_coarse_ld(int nagg, int dof_n);
+int efea_coarse_setup(int dof_n, int nblocks, const int32_t* pair_ptr, const int32_t* pair_node,
+                      const int32_t* pair_pos, const int32_t* blk_I, const int32_t* blk_J, const int32_t* node_ptr,
+                      const int32_t* node_cols, const double* values64, const uint8_t* known, const double* rel,
+                      int ld, double* Ainv, double* pivot_scratch, void* stream);
+size_t efea_pcg2_workspace_bytes(int Nnode, int dof_n, int nagg);
+int efea_solve_pcg2(int mode, int Nnode, int dof_n, int group, const int32_t* node_ptr, const int32_t* node_cols,
+                    const double* values64, const float* values32, const uint8_t* known, const void* Minv, int nagg,
+                    const int32_t* agg_ptr, const int32_t* agg_nodes, const double* rel, const double* Ainv, int ld,
+                    const double* b, double* x, void* workspace, size_t workspace_bytes, double tol_residual,
+                    double tol_anorm, double replace_ratio, int maxit, int check_every, void* stream,
+                    double* info);
+
 /* Continuous batching of Dirichlet-reduced solves (the serving loop).
  * A batch holds up to `capacity` systems of one (mode, dof_n, group) class and
  * advances all of them with one launch per PCG step; systems join
  * (efea_batch_add) and leave (efea_batch_pop) between calls of efea_batch_run.
  * Every system runs on the grid it would get alone, so its iterates are
- * bitwise those of efea_solve_pcg with check_every = the `iterations` passed to
+ * bitwise those of efea_solve_pcg (efea_solve_pcg2 in a two-level batch) with
+ * check_every = the `iterations` passed to
  * efea_batch_run.  All calls on one batch use one stream; the device buffers of
- * a system (matrix, Minv, b, x, workspace of efea_pcg_workspace_bytes bytes)
- * stay alive until it is popped.  The handle is a host pointer.
+ * a system (matrix, Minv, b, x, workspace of efea_pcg_workspace_bytes bytes,
+ * or efea_pcg2_workspace_bytes in a two-level batch, coarse arrays) stay alive
+ * until it is popped.  two_level = 1: every system of the batch uses the
+ * two-level preconditioner (efea_solve_pcg2's iterates; the coarse arguments
+ * of efea_batch_add are used, at most 4096 aggregates per system), 0: block
+ * Jacobi (efea_solve_pcg's; coarse arguments ignored).  The handle is a host
+ * pointer.
  * Replaces the per-simulation solves of __Solver_1 (Simulations/Solvers.py:502-553)
  * for many simulations at once; the reference has no batched path. */
-int efea_batch_create(int mode, int dof_n, int group, int capacity, void** handle);
+int efea_batch_create(int mode, int dof_n, int group, int two_level, int capacity, void** handle);
 int efea_batch_destroy(void* handle);
 /* Starts one solve (asynchronous); returns its id (>= 0) or an error code. */
 int efea_batch_add(void* handle, int Nnode, const int32_t* node_ptr, const int32_t* node_cols,
-                   const double* values64, const float* values32, const uint8_t* known, const void* Minv,
+                   const double* values64, const float* values32, const uint8_t* known, const void* Minv, int nagg,
+                   const int32_t* agg_ptr, const int32_t* agg_nodes, const double* rel, const double* Ainv, int ld,
                    const double* b, double* x, void* workspace, size_t workspace_bytes, double tol_residual,
                    double tol_anorm, double replace_ratio, int maxit, void* stream);
 /* Runs `iterations` PCG iterations of every active system, then polls their

@@ -36,21 +36,23 @@ def _elastic_spec(rid, side):
     return workloads.ElasticRequestSpec(rid, side, workloads.lognormal(210000.0, 0.2, z.T.ravel()), 0.0)
 
 
-def _alone(spec, mode):
+def _alone(spec, mode, precond="auto"):
     mesh = workloads.square_quad4(spec.side)
-    u, info = drive(workloads.elastic_request(spec, mesh, mode))
+    u, info = drive(workloads.elastic_request(spec, mesh, mode, precond))
     return u.clone(), info
 
 
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
-def test_batch_bitwise_equals_single_with_staggered_joins(mode):
-    specs = [_elastic_spec(i, s) for i, s in enumerate([14, 23, 31, 9])]
-    ref = [_alone(s, mode) for s in specs]
+@pytest.mark.parametrize("precond,sides", [("bj", [14, 23, 31, 9]), ("two-level", [36, 45, 20, 52])])
+def test_batch_bitwise_equals_single_with_staggered_joins(mode, precond, sides):
+    specs = [_elastic_spec(i, s) for i, s in enumerate(sides)]
+    ref = [_alone(s, mode, precond) for s in specs]
+    assert all(info.precond == precond for _, info in ref)
     ex = BatchExecutor(capacity=8, check_every=32)
     gens, results = {}, {}
 
     def start(spec):
-        g = workloads.elastic_request(spec, workloads.square_quad4(spec.side), mode)
+        g = workloads.elastic_request(spec, workloads.square_quad4(spec.side), mode, precond)
         job = next(g)
         job.owner = spec.rid
         gens[spec.rid] = g
@@ -73,7 +75,7 @@ def test_batch_bitwise_equals_single_with_staggered_joins(mode):
         u, info = results[spec.rid]
         print(f"side {spec.side} {mode}: {info.iterations} it (alone {info_ref.iterations}), "
               f"replacements {info.replacements}")
-        assert info.status == 1
+        assert info.status == 1 and info.precond == precond
         assert info.iterations == info_ref.iterations
         assert info.replacements == info_ref.replacements
         assert info.rel_residual == info_ref.rel_residual

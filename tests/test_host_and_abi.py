@@ -43,13 +43,17 @@ def test_abi_argument_errors_are_reported_without_gpu():
     rc = lib.efea_pf_split_points(4, 7, None, None, 0, 0.3, 0, None, None, None, None, None)
     assert rc == -1 and "null pointer" in _lib.last_error()
     h = C.c_void_p()
-    assert lib.efea_batch_create(16, 2, 16, 8, C.byref(h)) == -1 and "mode" in _lib.last_error()
-    assert lib.efea_batch_create(64, 2, 12, 8, C.byref(h)) == -1 and "group" in _lib.last_error()
-    assert lib.efea_batch_create(64, 2, 16, 0, C.byref(h)) == -1 and "capacity" in _lib.last_error()
+    assert lib.efea_batch_create(16, 2, 16, 0, 8, C.byref(h)) == -1 and "mode" in _lib.last_error()
+    assert lib.efea_batch_create(64, 2, 12, 0, 8, C.byref(h)) == -1 and "group" in _lib.last_error()
+    assert lib.efea_batch_create(64, 2, 16, 1, 0, C.byref(h)) == -1 and "capacity" in _lib.last_error()
     assert h.value is None
     assert lib.efea_batch_run(None, 32, None) == -1
-    assert lib.efea_batch_add(None, 10, None, None, None, None, None, None, None, None, None, 0, 1e-12, 1e-9,
-                              1e-3, 10, None) == -1
+    assert lib.efea_batch_add(None, 10, None, None, None, None, None, None, 0, None, None, None, None, 0, None,
+                              None, None, 0, 1e-12, 1e-9, 1e-3, 10, None) == -1
+    assert lib.efea_coarse_ld(0, 2) == -1
+    assert lib.efea_coarse_ld(100, 2) == 320 and lib.efea_coarse_ld(100, 3) == 608 and lib.efea_coarse_ld(5, 1) == 32
+    assert lib.efea_solve_pcg2(64, 10, 2, 16, None, None, None, None, None, None, 0, None, None, None, None, 0,
+                               None, None, None, 0, 1e-12, 1e-9, 1e-3, 10, 32, None, None) == -1
     jid = C.c_int(5)
     assert lib.efea_batch_pop(None, C.byref(jid), None) == -1
     assert lib.efea_batch_destroy(None) == 0
@@ -292,4 +296,4 @@ def test_workload_fields_are_seeded_and_lognormal():
     arr = np.array([s.arrival for s in specs])
     assert arr[0] == 0 and np.all(np.diff(arr) >= 0)
     pf = workloads.serving_phasefield_specs(n=8, steps=5)
-    assert all(100 <= s.cells <= 200 for s in pf)
+    assert all(100 <= s.cells <= 200 and s.cells % 2 == 0 for s in pf)
