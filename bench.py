@@ -154,6 +154,114 @@ def spmv_roofline(A, reps, precision):
     }
 
 
+def _cpu_cantilever():
+    """The oracle's cantilever: assembly + SuperLU direct solve on the host."""
+    from easyfea_amd import workloads
+    from easyfea_amd._utils import ElemType
+    from oracle import efea_oracle as O
+
+    mesh = workloads.cantilever().mesh
+    t = time.perf_counter()
+    ref = O.ElasticOracle(mesh.connect, mesh.coord, "QUAD4", 210000.0, 0.3, True, 13.0,
+                          {"SEG2": mesh.dict_groupElem[ElemType.SEG2].connect})
+    ref.add_dirichlet(mesh.Nodes_Conditions(lambda x, y, z: x == 0), [0, 0], ["x", "y"])
+    ref.add_surfLoad(mesh.Nodes_Conditions(lambda x, y, z: x == 120.0), [-800.0 / 13.0 ** 2], ["y"])
+    u = ref.Solve()
+    return u, time.perf_counter() - t
+
+
+def _cpu_cube(n):
+    from easyfea_amd import mesh as M
+    from easyfea_amd._utils import ElemType
+    from oracle import efea_oracle as O
+
+    m = M.hexa8_cube(n)
+    rng = np.random.default_rng(0)
+    s = np.sqrt(np.log(1.04))
+    E = np.exp(rng.normal(np.log(210000.0) - 0.5 * s * s, s, m.Ne))
+    t = time.perf_counter()
+    ref = O.ElasticOracle(m.connect, m.coord, "HEXA8", E, 0.3, False, 1.0,
+                          {"QUAD4": m.dict_groupElem[ElemType.QUAD4].connect})
+    ref.add_dirichlet(m.Nodes_Conditions(lambda x, y, z: z == 0), [0, 0, 0], ["x", "y", "z"])
+    ref.add_surfLoad(m.Nodes_Conditions(lambda x, y, z: z == 1.0), [-10.0], ["z"])
+    u = ref.Solve()
+    return u, time.perf_counter() - t
+
+
+def bench_single(args) -> dict:
+    """One elastic solve of the cantilever (360 x 39 QUAD4) or of the 64^3
+    HEXA8 cube: every (precision, preconditioner) pair, DOF/s of the best
+    repeat, p50/p99 over the repeats, error against the FP64 reference."""
+    from easyfea_amd import solve_bench
+
+    res = solve_bench.run(args.workload, 64, 5)
+    cores = len(os.sched_getaffinity(0))
+    cpu = None
+    if args.workload == "cantilever":
+        u_ref, t_cpu = _cpu_cantilever()
+        res["reference"] = "oracle sparse direct solve (scipy SuperLU)"
+        for r in res["runs"]:
+            r["u_rel_err_vs_reference"] = float(np.linalg.norm(r["_u"] - u_ref) / np.linalg.norm(u_ref))
+        if not args.no_cpu:
+            cpu = {"value": u_ref.size / t_cpu, "unit": "DOF/s", "cores": cores, "kind": "port",
+                   "sample": f"one full cantilever solve on the host (numpy assembly, SuperLU), {t_cpu:.2f} s"}
+    elif not args.no_cpu:
+        u16, t_cpu = _cpu_cube(16)
+        cpu = {"value": u16.size / t_cpu, "unit": "DOF/s", "cores": cores, "kind": "port",
+               "sample": f"oracle direct solve of the 16^3 cube ({u16.size} DOF; the 64^3 one does not fit "
+                         f"SuperLU) on the host, {t_cpu:.2f} s"}
+    for r in res["runs"]:
+        r.pop("_u")
+    runs = {(r["precond"], r["solver"]): r for r in res["runs"]}
+    head = runs[("two-level", "hip_mixed")]
+    plain = runs[("two-level", "hip_fp64")]
+    cfg = ("cantilever QUAD4 360x39, plane stress, t=13, -800 N at x=120" if args.workload == "cantilever"
+           else "HEXA8 64^3 unit cube, lognormal E (CoV 0.2, seed 0), clamped z=0, -10 MPa on z=1")
+    return {
+        "metric": METRIC, "value": head["value"], "unit": "DOF/s", "n_gpus": 1, "steps": 1,
+        "warmup": 1, "ms_per_step": 1e3 * head["ndof"] / head["value"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 vectors (f32 matrix copy when mixed)",
+        "data": "synthetic mesh generated in-process",
+        "config": {"workload": cfg, "ndof": head["ndof"], "solver": "two-level PCG, hip_mixed"},
+        "p50_solve_ms": head["p50_ms"], "p99_solve_ms": head["p99_ms"],
+        "u_rel_err_vs_reference": head["u_rel_err_vs_reference"], "reference": res["reference"],
+        "plain_fp64": {"value": plain["value"], "mixed_speedup": head["value"] / plain["value"]},
+        "runs": res["runs"], "cpu_baseline": cpu,
+    }
+
+
+def bench_serving(args) -> dict:
+    """The serving streams (easyfea_amd.serving_bench) with the oracle's
+    host solves of a bounded sample as the CPU baseline."""
+    from easyfea_amd import serving_bench
+
+    wl = args.workload[6:]
+    argv = ["--workload", wl, "--compare-fp64"]
+    out = serving_bench.main(argv)
+    cpu = None
+    if wl == "elastic" and not args.no_cpu:
+        from easyfea_amd._utils import ElemType
+        from oracle import efea_oracle as O
+
+        specs, meshes = out["_specs"][:8], out["_meshes"]
+        t0, ndof = time.perf_counter(), 0
+        for spec in specs:
+            m = meshes.get(spec.side)
+            ref = O.ElasticOracle(m.connect, m.coord, "QUAD4", spec.E, 0.3, True, 1.0,
+                                  {"SEG2": m.dict_groupElem[ElemType.SEG2].connect})
+            ref.add_dirichlet(m.Nodes_Conditions(lambda x, y, z: x == 0.0), [0, 0], ["x", "y"])
+            ref.add_surfLoad(m.Nodes_Conditions(lambda x, y, z: x == 1.0), [1.0], ["x"])
+            ref.Solve()
+            ndof += 2 * m.Nn
+        t = time.perf_counter() - t0
+        cpu = {"value": ndof / t, "unit": "DOF/s", "cores": len(os.sched_getaffinity(0)), "kind": "port",
+               "sample": f"first {len(specs)} requests of the stream solved one after another on the host, {t:.1f} s"}
+    line = {k: v for k, v in out.items() if not k.startswith("_")}
+    line.update({"metric": METRIC, "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                 "data": "synthetic meshes and random fields generated in-process", "cpu_baseline": cpu})
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +272,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=500)
+    ap.add_argument("--workload", default="sent",
+                    choices=["sent", "cantilever", "cube", "serve-elastic", "serve-phasefield"],
+                    help="sent: the phase-field history above (the headline); the others print the "
+                         "measurement line of the cantilever / 64^3 cube single solves or of the serving "
+                         "streams (easyfea_amd.solve_bench / easyfea_amd.serving_bench)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the phase-field workload runs on a single MI355X "
@@ -176,6 +289,12 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if args.workload in ("cantilever", "cube"):
+        print(json.dumps(bench_single(args)))
+        return
+    if args.workload.startswith("serve-"):
+        print(json.dumps(bench_serving(args)))
+        return
     mesh = ea.tri3_notched_square(args.cells)
     ndof_u, ndof_d = mesh.Nn * 2, mesh.Nn
 

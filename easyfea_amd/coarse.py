@@ -100,9 +100,10 @@ def aggregate(coord: np.ndarray, node_ptr: np.ndarray, node_cols: np.ndarray, ta
 
 
 def default_target(dim: int) -> int:
-    """Nodes per aggregate: ~16 x 16 in 2D, ~8 x 8 x 8 in 3D (coarse sizes of a
-    few hundred to a few thousand unknowns on the measured meshes)."""
-    return 256 if dim == 2 else 512
+    """Largest part of the bisection: parts hold target/2 .. target nodes
+    (2D: ~12 x 12, 3D: ~9 x 9 x 9), which keeps the dense coarse inverse at a
+    few hundred (2D) to a few thousand (3D) unknowns on the measured meshes."""
+    return 256 if dim == 2 else 1024
 
 
 def prolongator(ag: Aggregation, dim: int, dof_n: int, known: np.ndarray) -> sp.csr_matrix:

@@ -279,10 +279,14 @@ int reaction_diffusion(int elem, int prec, int64_t Ne, const int* connect, const
     if (elem == ET_TRI3 && nPgm == 3 && nPgr == 1) RD(double, ET_TRI3, 3, 1);
     if (elem == ET_QUAD4 && nPgm == 4 && nPgr == 4) RD(double, ET_QUAD4, 4, 4);
     if (elem == ET_HEXA8 && nPgm == 8 && nPgr == 8) RD(double, ET_HEXA8, 8, 8);
+    if (elem == ET_TETRA4 && nPgm == 4 && nPgr == 1) RD(double, ET_TETRA4, 4, 1);
+    if (elem == ET_TETRA10 && nPgm == 4 && nPgr == 4) RD(double, ET_TETRA10, 4, 4);
   } else if (prec == 32) {
     if (elem == ET_TRI3 && nPgm == 3 && nPgr == 1) RD(float, ET_TRI3, 3, 1);
     if (elem == ET_QUAD4 && nPgm == 4 && nPgr == 4) RD(float, ET_QUAD4, 4, 4);
     if (elem == ET_HEXA8 && nPgm == 8 && nPgr == 8) RD(float, ET_HEXA8, 8, 8);
+    if (elem == ET_TETRA4 && nPgm == 4 && nPgr == 1) RD(float, ET_TETRA4, 4, 1);
+    if (elem == ET_TETRA10 && nPgm == 4 && nPgr == 4) RD(float, ET_TETRA10, 4, 4);
   }
 #undef RD
   set_error("reaction_diffusion: unsupported element %d / nPg %d,%d / precision %d", elem, nPgm, nPgr, prec);

@@ -281,3 +281,41 @@ def hexa8_box(
 
 def hexa8_cube(n: int, L: float = 1.0) -> Mesh:
     return hexa8_box(L, L, L, n, n, n)
+
+
+def tetra4_box(Lx: float, Ly: float, Lz: float, nx: int, ny: int, nz: int) -> Mesh:
+    """TETRA4 mesh of ``[0,Lx] x [0,Ly] x [0,Lz]``: every cell of the
+    ``nx x ny x nz`` grid split into the six tetrahedra along its main
+    diagonal (the same diagonal direction in every cell, so faces conform),
+    nodes numbered as in ``hexa8_box``, positively oriented."""
+    grid = hexa8_box(Lx, Ly, Lz, nx, ny, nz)
+    h = grid.dict_groupElem[ElemType.HEXA8].connect
+    # cell corners v[a + 2b + 4c] at offsets (a, b, c)
+    v = h[:, [0, 1, 3, 2, 4, 5, 7, 6]]
+    tets = []
+    for order in ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)):
+        path, idx = [0], 0
+        for ax in order:
+            idx += 1 << ax
+            path.append(idx)
+        tets.append(v[:, path])
+    connect = np.concatenate(tets)
+    X = grid.coord[connect]
+    vol = np.einsum("ei,ei->e", np.cross(X[:, 1] - X[:, 0], X[:, 2] - X[:, 0]), X[:, 3] - X[:, 0])
+    flip = vol < 0
+    connect[flip] = connect[flip][:, [0, 2, 1, 3]]
+    return Mesh({ElemType.TETRA4: connect}, grid.coord)
+
+
+def tetra10_from_tetra4(mesh: Mesh) -> Mesh:
+    """Quadratic tetrahedra on the same vertices: mid-edge nodes appended
+    (one per edge), element order [v0, v1, v2, v3, m01, m12, m02, m03, m23, m13]
+    (``EasyFEA/FEM/Elems/_tetra.py``)."""
+    c4 = mesh.dict_groupElem[ElemType.TETRA4].connect
+    pairs = [(0, 1), (1, 2), (0, 2), (0, 3), (2, 3), (1, 3)]
+    e = np.stack([np.sort(c4[:, list(p)], axis=1) for p in pairs], 1).reshape(-1, 2)
+    uniq, inv = np.unique(e, axis=0, return_inverse=True)
+    mid = 0.5 * (mesh.coord[uniq[:, 0]] + mesh.coord[uniq[:, 1]])
+    coord = np.concatenate([mesh.coord, mid])
+    connect = np.concatenate([c4, mesh.Nn + inv.reshape(-1, 6)], 1)
+    return Mesh({ElemType.TETRA10: connect}, coord)

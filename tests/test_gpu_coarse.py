@@ -26,7 +26,8 @@ def _force_fp32_krylov(monkeypatch):
 
 
 def _rel(a, b):
-    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b))
+    a, b = np.asarray(a), np.asarray(b)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
 @pytest.mark.parametrize("maker,dof_n,target", [

@@ -65,6 +65,8 @@ def _elastic_ke(mesh, E, nu, ps, th, prec):
     (lambda: ea.quad4_rectangle(3.0, 2.0, 7, 5), "QUAD4"),
     (lambda: ea.tri3_rectangle(3.0, 2.0, 7, 5), "TRI3"),
     (lambda: ea.hexa8_box(1.0, 2.0, 1.5, 4, 3, 5), "HEXA8"),
+    (lambda: ea.mesh.tetra4_box(1.0, 2.0, 1.5, 3, 2, 4), "TETRA4"),
+    (lambda: ea.mesh.tetra10_from_tetra4(ea.mesh.tetra4_box(1.0, 2.0, 1.5, 3, 2, 4)), "TETRA10"),
 ])
 @pytest.mark.parametrize("prec", [64, 32])
 def test_elastic_element_matrices(maker, et, prec):
@@ -80,6 +82,39 @@ def test_elastic_element_matrices(maker, et, prec):
     tol = 1e-12 if prec == 64 else 1e-5
     err = np.max(np.linalg.norm(Ke - ref, axis=(1, 2)) / np.linalg.norm(ref, axis=(1, 2)))
     assert err < tol, err
+
+
+@pytest.mark.parametrize("maker,et", [
+    (lambda: ea.quad4_rectangle(3.0, 2.0, 7, 5), "QUAD4"),
+    (lambda: ea.tri3_rectangle(3.0, 2.0, 7, 5), "TRI3"),
+    (lambda: ea.hexa8_box(1.0, 2.0, 1.5, 4, 3, 5), "HEXA8"),
+    (lambda: ea.mesh.tetra4_box(1.0, 2.0, 1.5, 3, 2, 4), "TETRA4"),
+    (lambda: ea.mesh.tetra10_from_tetra4(ea.mesh.tetra4_box(1.0, 2.0, 1.5, 3, 2, 4)), "TETRA10"),
+])
+def test_reaction_diffusion_source_operators(maker, et):
+    """K_e = UV(r) + GradU_A_GradV(I, k), F_e = V(f) (Bilinear.py:42-59, 229-249;
+    Linear.py:18-35) with r, f per mass point and k per element."""
+    mesh = _distorted(maker(), 0.3)
+    dev = DeviceMesh(mesh)
+    tm, nm = dev.tables(MatrixType.mass)
+    tr, nr = dev.tables(MatrixType.rigi)
+    rng = np.random.default_rng(2)
+    r = rng.random((mesh.Ne, nm)) + 0.5
+    f = rng.standard_normal((mesh.Ne, nm))
+    k = rng.random(mesh.Ne) + 0.1
+    nPe = dev.nPe
+    Ke = torch.empty(mesh.Ne * nPe * nPe, dtype=torch.float64, device="cuda")
+    Fe = torch.empty(mesh.Ne * nPe, dtype=torch.float64, device="cuda")
+    r_d, k_d, f_d = to_dev(r), to_dev(k), to_dev(f)  # kept alive until the kernel has run
+    _lib.call("efea_reaction_diffusion_ke", dev.code, 64, dev.Ne, ptr(dev.connect), ptr(dev.coord), ptr(tm), nm,
+              ptr(tr), nr, ptr(r_d), 0.0, ptr(k_d), 1, ptr(f_d), 2.0, ptr(Ke), ptr(Fe), stream_ptr())
+    gm = O.Geometry(et, mesh.connect, mesh.coord, "mass", mesh.dim)
+    gr = O.Geometry(et, mesh.connect, mesh.coord, "rigi", mesh.dim)
+    Kref = 2.0 * (O.UV(gm, r) + O.GradU_A_GradV(gr, np.eye(mesh.dim), k))
+    Fref = 2.0 * O.V(gm, f)
+    Kd = Ke.cpu().numpy().reshape(mesh.Ne, nPe, nPe)
+    assert np.abs(Kd - Kref).max() <= 1e-12 * np.abs(Kref).max()
+    assert np.abs(Fe.cpu().numpy().reshape(mesh.Ne, nPe) - Fref).max() <= 1e-12 * np.abs(Fref).max()
 
 
 def test_pattern_and_scatter_bit_exact():

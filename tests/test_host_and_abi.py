@@ -297,3 +297,64 @@ def test_workload_fields_are_seeded_and_lognormal():
     assert arr[0] == 0 and np.all(np.diff(arr) >= 0)
     pf = workloads.serving_phasefield_specs(n=8, steps=5)
     assert all(100 <= s.cells <= 200 and s.cells % 2 == 0 for s in pf)
+
+
+def test_tetra_meshes():
+    from easyfea_amd.mesh import tetra4_box, tetra10_from_tetra4
+
+    m = tetra4_box(1.0, 2.0, 1.5, 3, 2, 4)
+    assert m.Ne == 6 * 3 * 2 * 4 and m.Nn == 4 * 3 * 5
+    X = m.coord[m.connect]
+    vol = np.einsum("ei,ei->e", np.cross(X[:, 1] - X[:, 0], X[:, 2] - X[:, 0]), X[:, 3] - X[:, 0]) / 6
+    assert np.all(vol > 0) and abs(vol.sum() - 3.0) < 1e-12
+    # conforming: every interior triangle face is shared by exactly two tetrahedra
+    faces = np.sort(np.concatenate([m.connect[:, [0, 1, 2]], m.connect[:, [0, 1, 3]], m.connect[:, [0, 2, 3]],
+                                    m.connect[:, [1, 2, 3]]]), axis=1)
+    _, cnt = np.unique(faces, axis=0, return_counts=True)
+    assert cnt.max() == 2
+    m10 = tetra10_from_tetra4(m)
+    c = m10.connect
+    for k, (a, b) in enumerate([(0, 1), (1, 2), (0, 2), (0, 3), (2, 3), (1, 3)]):
+        assert np.allclose(m10.coord[c[:, 4 + k]], 0.5 * (m10.coord[c[:, a]] + m10.coord[c[:, b]]))
+    g = O.Geometry("TETRA10", c, m10.coord, "rigi", 3)
+    assert abs(g.wJ.sum() - 3.0) < 1e-12
+
+
+def test_aggregation_covers_nodes_and_splits_crack_lips():
+    from easyfea_amd.coarse import aggregate, coarse_pairs, prolongator
+
+    m = ea.tri3_notched_square(40)
+    cm = build_csr_map(m.connect, m.Nn, 2)
+    ag = aggregate(m.coord, cm.node_ptr, cm.node_cols, 64)
+    assert np.array_equal(np.sort(ag.agg_nodes), np.arange(m.Nn))
+    sizes = np.diff(ag.agg_ptr)
+    assert sizes.min() >= 16 and sizes.max() <= 64 + 64 // 4 * 4
+    # every aggregate is connected in the node graph
+    import scipy.sparse as sps
+    from scipy.sparse.csgraph import connected_components
+
+    rows = np.repeat(np.arange(m.Nn), np.diff(cm.node_ptr))
+    G = sps.csr_matrix((np.ones(rows.size), (rows, cm.node_cols)), shape=(m.Nn, m.Nn))
+    for a in range(ag.nagg):
+        nodes = ag.agg_nodes[ag.agg_ptr[a]:ag.agg_ptr[a + 1]]
+        assert connected_components(G[nodes][:, nodes], directed=False)[0] == 1
+    # the two copies of a crack-lip node never share an aggregate
+    y, x = m.coord[:, 1], m.coord[:, 0]
+    lips = np.nonzero((np.abs(y - 0.5) < 1e-12) & (x < 0.5 - 1e-12))[0]
+    pos = {tuple(np.round(m.coord[i, :2], 12)): [] for i in lips}
+    for i in lips:
+        pos[tuple(np.round(m.coord[i, :2], 12))].append(ag.agg[i])
+    assert all(len(v) == 2 and v[0] != v[1] for v in pos.values())
+    # rigid-body modes: K P = 0 for an unconstrained elastic operator
+    geo = O.Geometry("TRI3", m.connect, m.coord, "rigi", 2)
+    K = O.assemble_csr(O.K_elastic(geo, O.C_isotropic(1.0, 0.3, 2, True)), O.csr_map(m.connect, 2, 2 * m.Nn),
+                       2 * m.Nn)
+    P = prolongator(ag, 2, 2, np.zeros(2 * m.Nn, bool))
+    # the x-translation mode summed over all aggregates is a global rigid mode
+    t = np.zeros(ag.nagg * 3)
+    t[0::3] = 1.0
+    assert np.abs(K @ (P @ t)).max() < 1e-12
+    cp = coarse_pairs(ag, cm.node_ptr, cm.node_cols)
+    assert cp.pair_ptr[-1] == cm.node_cols.size
+    keys = cp.blk_I.astype(np.int64) * ag.nagg + cp.blk_J
+    assert np.all(np.diff(keys) > 0)
