@@ -25,6 +25,7 @@ SIGNATURES = {
     "efea_boundary_load": (i32, [i32, i64, vp, vp, i32, vp, vp, vp]),
     "efea_csr_scatter": (i32, [i32, i64, vp, vp, vp, vp, vp]),
     "efea_cast_f64_f32": (i32, [i64, vp, vp, vp]),
+    "efea_split_f32": (i32, [i32, i32, vp, vp, vp, vp, vp]),
     "efea_spmv_scratch_bytes": (sz, [i32, i32]),
     "efea_spmv": (i32, [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
     "efea_spmv_repeat": (i32, [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, vp]),
@@ -33,21 +34,21 @@ SIGNATURES = {
     "efea_default_group": (i32, [i32, i32]),
     "efea_solve_pcg": (
         i32,
-        [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, f64, f64, f64, i32, i32, vp, vp],
+        [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, f64, f64, f64, i32, i32, vp, vp],
     ),
     "efea_coarse_ld": (i32, [i32, i32]),
     "efea_coarse_setup": (i32, [i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp]),
     "efea_pcg2_workspace_bytes": (sz, [i32, i32, i32]),
     "efea_solve_pcg2": (
         i32,
-        [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, i32, vp, vp, vp, sz, f64, f64, f64, i32,
-         i32, vp, vp],
+        [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, i32, vp, vp, vp, sz, f64, f64, f64,
+         i32, i32, vp, vp],
     ),
     "efea_batch_create": (i32, [i32, i32, i32, i32, i32, C.POINTER(vp)]),
     "efea_batch_destroy": (i32, [vp]),
     "efea_batch_add": (
         i32,
-        [vp, i32, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, i32, vp, vp, vp, sz, f64, f64, f64, i32, vp],
+        [vp, i32, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, i32, vp, vp, vp, sz, f64, f64, f64, i32, vp],
     ),
     "efea_batch_run": (i32, [vp, i32, vp]),
     "efea_batch_pop": (i32, [vp, C.POINTER(i32), vp]),

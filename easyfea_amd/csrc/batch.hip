@@ -23,6 +23,7 @@ int pcg_finish_info(const void* state, double rr_true, double* info);
 struct BatchSys {
   SpmvArgs A64;         // FP64 values: residuals and replacements
   const void* valsK;    // values of the Krylov products (FP32 copy in mode 32)
+  const void* corrK;    // FP32 row-sum correction (mode 32) or null
   const void* Minv;
   const double* b;
   double* x;
@@ -48,6 +49,7 @@ __global__ void __launch_bounds__(BS) kb_step1(const BatchSys* __restrict__ sys,
   const BatchSys& S = sys[map[blockIdx.x]];
   SpmvArgs AK = S.A64;
   AK.vals = S.valsK;
+  AK.corr = S.corrK;
   pcg_step1_body<TM, DN, G>(AK, S.z, S.P0, S.P1, S.q, S.part2, S.nb2, S.part1, S.st, (int)blockIdx.x - S.blkS0,
                             S.nbS);
 }
@@ -199,7 +201,8 @@ static int start_dispatch(const SpmvArgs& A64, const void* Minv, const CoarseArg
 }
 
 int batch_add(void* h, int Nnode, const int* node_ptr, const int* node_cols, const double* vals64,
-              const float* vals32, const unsigned char* known, const void* Minv, int nagg, const int* agg_ptr,
+              const float* vals32, const float* corr32, const unsigned char* known, const void* Minv, int nagg,
+              const int* agg_ptr,
               const int* agg_nodes, const double* rel, const double* Ainv, int ld, const double* b, double* x,
               void* ws, size_t ws_bytes, double tol_r, double tol_A, double delta, int maxit, hipStream_t s) {
   Batch* B = (Batch*)h;
@@ -234,6 +237,7 @@ int batch_add(void* h, int Nnode, const int* node_ptr, const int* node_cols, con
   BatchSys& d = e.d;
   d.A64 = SpmvArgs{Nnode, node_ptr, node_cols, vals64, known};
   d.valsK = B->mode == 32 ? (const void*)vals32 : (const void*)vals64;
+  d.corrK = B->mode == 32 ? (const void*)corr32 : nullptr;
   d.Minv = Minv;
   d.b = b;
   d.x = x;

@@ -41,8 +41,10 @@ int spmv(int, int, int, int, const int*, const int*, const void*, const unsigned
 size_t spmv_scratch_bytes(int, int);
 int bj_setup(int, int, int, const int*, const int*, const double*, const unsigned char*, void*, hipStream_t);
 size_t pcg_workspace_bytes(int, int, int);
-int solve_pcg(int, int, int, int, const int*, const int*, const double*, const float*, const unsigned char*,
-              const void*, const double*, double*, void*, double, double, double, int, int, hipStream_t, double*);
+int solve_pcg(int, int, int, int, const int*, const int*, const double*, const float*, const float*,
+              const unsigned char*, const void*, const double*, double*, void*, double, double, double, int, int,
+              hipStream_t, double*);
+int split_f32(int, int, const int*, const double*, float*, float*, hipStream_t);
 int dot(int64_t, const double*, const double*, double*, double*, hipStream_t);
 int gather_sum(int64_t, const int*, const double*, double*, hipStream_t);
 int default_group(int, int);
@@ -55,9 +57,9 @@ int pf_split_points(int64_t, int, const double*, const double*, int, double, int
                     double*, hipStream_t);
 int batch_create(int, int, int, int, int, void**);
 int batch_destroy(void*);
-int batch_add(void*, int, const int*, const int*, const double*, const float*, const unsigned char*, const void*,
-              int, const int*, const int*, const double*, const double*, int, const double*, double*, void*, size_t,
-              double, double, double, int, hipStream_t);
+int batch_add(void*, int, const int*, const int*, const double*, const float*, const float*, const unsigned char*,
+              const void*, int, const int*, const int*, const double*, const double*, int, const double*, double*,
+              void*, size_t, double, double, double, int, hipStream_t);
 int batch_run(void*, int, hipStream_t);
 int batch_pop(void*, int*, double*);
 int batch_active(void*);
@@ -65,9 +67,9 @@ int coarse_ld(int, int);
 int coarse_setup(int, int, const int*, const int*, const int*, const int*, const int*, const int*, const int*,
                  const double*, const unsigned char*, const double*, int, double*, double*, hipStream_t);
 size_t pcg2_workspace_bytes(int, int, int);
-int solve_pcg2(int, int, int, int, const int*, const int*, const double*, const float*, const unsigned char*,
-               const void*, int, const int*, const int*, const double*, const double*, int, const double*, double*,
-               void*, double, double, double, int, int, hipStream_t, double*);
+int solve_pcg2(int, int, int, int, const int*, const int*, const double*, const float*, const float*,
+               const unsigned char*, const void*, int, const int*, const int*, const double*, const double*, int,
+               const double*, double*, void*, double, double, double, int, int, hipStream_t, double*);
 
 }  // namespace efea
 
@@ -123,6 +125,15 @@ int efea_csr_scatter(int precision, int64_t nslot, const int32_t* seg_ptr, const
   return scatter(precision, nslot, seg_ptr, perm, entries, out_values, STREAM(stream));
 }
 
+int efea_split_f32(int Nnode, int dof_n, const int32_t* node_ptr, const double* values64, float* values32,
+                   float* corr32, void* stream) {
+  if (Nnode < 0 || (Nnode > 0 && (!node_ptr || !values64 || !values32 || !corr32))) {
+    set_error("efea_split_f32: null pointer or negative size");
+    return EFEA_ERR_ARG;
+  }
+  return split_f32(Nnode, dof_n, node_ptr, values64, values32, corr32, STREAM(stream));
+}
+
 int efea_cast_f64_f32(int64_t n, const double* src, float* dst, void* stream) {
   if (n < 0 || (n > 0 && (!src || !dst))) {
     set_error("efea_cast_f64_f32: null pointer or negative size");
@@ -169,7 +180,8 @@ size_t efea_pcg_workspace_bytes(int Nnode, int dof_n, int group) {
 int efea_default_group(int dof_n, int max_neighbours) { return default_group(dof_n, max_neighbours); }
 
 int efea_solve_pcg(int mode, int Nnode, int dof_n, int group, const int32_t* node_ptr, const int32_t* node_cols,
-                   const double* values64, const float* values32, const uint8_t* known, const void* Minv,
+                   const double* values64, const float* values32, const float* corr32, const uint8_t* known,
+                   const void* Minv,
                    const double* b, double* x, void* workspace, size_t workspace_bytes, double tol_residual,
                    double tol_anorm, double replace_ratio, int maxit, int check_every, void* stream,
                    double* info) {
@@ -182,7 +194,7 @@ int efea_solve_pcg(int mode, int Nnode, int dof_n, int group, const int32_t* nod
               pcg_workspace_bytes(Nnode, dof_n, group));
     return EFEA_ERR_SIZE;
   }
-  return solve_pcg(mode, Nnode, dof_n, group, node_ptr, node_cols, values64, values32, known, Minv, b, x,
+  return solve_pcg(mode, Nnode, dof_n, group, node_ptr, node_cols, values64, values32, corr32, known, Minv, b, x,
                    workspace, tol_residual, tol_anorm, replace_ratio, maxit, check_every, STREAM(stream), info);
 }
 
@@ -201,7 +213,8 @@ int efea_batch_destroy(void* handle) {
 }
 
 int efea_batch_add(void* handle, int Nnode, const int32_t* node_ptr, const int32_t* node_cols,
-                   const double* values64, const float* values32, const uint8_t* known, const void* Minv, int nagg,
+                   const double* values64, const float* values32, const float* corr32, const uint8_t* known,
+                   const void* Minv, int nagg,
                    const int32_t* agg_ptr, const int32_t* agg_nodes, const double* rel, const double* Ainv, int ld,
                    const double* b, double* x, void* workspace, size_t workspace_bytes, double tol_residual,
                    double tol_anorm, double replace_ratio, int maxit, void* stream) {
@@ -213,7 +226,8 @@ int efea_batch_add(void* handle, int Nnode, const int32_t* node_ptr, const int32
     set_error("efea_batch_add: maxit %d", maxit);
     return EFEA_ERR_ARG;
   }
-  return batch_add(handle, Nnode, node_ptr, node_cols, values64, values32, known, Minv, nagg, agg_ptr, agg_nodes,
+  return batch_add(handle, Nnode, node_ptr, node_cols, values64, values32, corr32, known, Minv, nagg, agg_ptr,
+                   agg_nodes,
                    rel, Ainv, ld, b, x, workspace, workspace_bytes, tol_residual, tol_anorm, replace_ratio, maxit,
                    STREAM(stream));
 }
@@ -267,7 +281,8 @@ int efea_coarse_setup(int dof_n, int nblocks, const int32_t* pair_ptr, const int
 size_t efea_pcg2_workspace_bytes(int Nnode, int dof_n, int nagg) { return pcg2_workspace_bytes(Nnode, dof_n, nagg); }
 
 int efea_solve_pcg2(int mode, int Nnode, int dof_n, int group, const int32_t* node_ptr, const int32_t* node_cols,
-                    const double* values64, const float* values32, const uint8_t* known, const void* Minv, int nagg,
+                    const double* values64, const float* values32, const float* corr32, const uint8_t* known,
+                    const void* Minv, int nagg,
                     const int32_t* agg_ptr, const int32_t* agg_nodes, const double* rel, const double* Ainv, int ld,
                     const double* b, double* x, void* workspace, size_t workspace_bytes, double tol_residual,
                     double tol_anorm, double replace_ratio, int maxit, int check_every, void* stream,
@@ -286,7 +301,8 @@ int efea_solve_pcg2(int mode, int Nnode, int dof_n, int group, const int32_t* no
               pcg2_workspace_bytes(Nnode, dof_n, nagg));
     return EFEA_ERR_SIZE;
   }
-  return solve_pcg2(mode, Nnode, dof_n, group, node_ptr, node_cols, values64, values32, known, Minv, nagg, agg_ptr,
+  return solve_pcg2(mode, Nnode, dof_n, group, node_ptr, node_cols, values64, values32, corr32, known, Minv, nagg,
+                    agg_ptr,
                     agg_nodes, rel, Ainv, ld, b, x, workspace, tol_residual, tol_anorm, replace_ratio, maxit,
                     check_every, STREAM(stream), info);
 }

@@ -101,71 +101,72 @@ __global__ void k_coarse_guard(int ld, double* __restrict__ Ac) {
 }
 
 // ---------------------------------------------------------------------------
-// Blocked Gauss-Jordan inversion, pivot block k (rows/cols k*GB .. k*GB+GB-1).
+// Blocked Gauss-Jordan inversion of the SPD matrix in place (no pivoting),
+// GB x GB blocks.  Block step k with P = inv(A_kk):
+//   A_ij -= A_ik P A_kj   (i, j != k)      k_gj_update
+//   A_kj <- P A_kj, A_ik <- -A_ik P         k_gj_panels (+ the next pivot)
+//   A_kk <- P                               the pivot inversion itself
+// The trailing update reads the panels before they are scaled (it forms
+// P A_kj itself), so the panels and the inversion of the next pivot block
+// (final once the update of step k is done) share one launch: two launches
+// per block step.
 
-// Inverts the pivot block in LDS, writes it to Pb and in place.
-__global__ void __launch_bounds__(256) k_gj_pivot(double* __restrict__ A, int ld, int k, double* __restrict__ Pb) {
-  __shared__ double S[GB][GB + 1];
-  __shared__ double piv;
+// Wavefront broadcast of lane `lane`'s value (lane is a compile-time constant
+// after unrolling).
+__device__ inline double readlane_f64(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Inverts pivot block k with the first wavefront of the block: lane l < GB
+// holds row l in registers; each elimination step broadcasts the pivot row
+// with readlane (no LDS, no barrier).  Writes P to Pb and in place.
+__device__ inline void gj_pivot_wave(double* __restrict__ A, int ld, int k, double* __restrict__ Pb) {
+  const int l = threadIdx.x;
+  if (l >= 64) return;
   const int k0 = k * GB;
-  for (int e = threadIdx.x; e < GB * GB; e += 256) S[e / GB][e % GB] = A[(int64_t)(k0 + e / GB) * ld + k0 + e % GB];
-  __syncthreads();
+  const bool own = l < GB;
+  double row[GB];
+#pragma unroll
+  for (int j = 0; j < GB; ++j) row[j] = own ? A[(int64_t)(k0 + l) * ld + k0 + j] : 0.0;
+#pragma unroll
   for (int s = 0; s < GB; ++s) {
-    if (threadIdx.x == 0) piv = 1.0 / S[s][s];
-    __syncthreads();
-    const double p = piv;
-    double nv[GB * GB / 256];
-    for (int q = 0; q < GB * GB / 256; ++q) {
-      const int e = threadIdx.x + q * 256, i = e / GB, j = e % GB;
-      double v;
-      if (i == s && j == s) v = p;
-      else if (i == s) v = S[s][j] * p;
-      else if (j == s) v = -S[i][s] * p;
-      else v = S[i][j] - S[i][s] * S[s][j] * p;
-      nv[q] = v;
+    double rs[GB];
+#pragma unroll
+    for (int j = 0; j < GB; ++j) rs[j] = readlane_f64(row[j], s);
+    const double p = 1.0 / rs[s];
+    const double ap = row[s] * p;
+    const bool piv = l == s;
+#pragma unroll
+    for (int j = 0; j < GB; ++j) {
+      const double upd = (j == s) ? -ap : row[j] - ap * rs[j];
+      const double prow = (j == s) ? p : rs[j] * p;
+      row[j] = piv ? prow : upd;
     }
-    __syncthreads();
-    for (int q = 0; q < GB * GB / 256; ++q) {
-      const int e = threadIdx.x + q * 256;
-      S[e / GB][e % GB] = nv[q];
-    }
-    __syncthreads();
   }
-  for (int e = threadIdx.x; e < GB * GB; e += 256) {
-    const int i = e / GB, j = e % GB;
-    Pb[e] = S[i][j];
-    A[(int64_t)(k0 + i) * ld + k0 + j] = S[i][j];
+  if (own) {
+#pragma unroll
+    for (int j = 0; j < GB; ++j) {
+      Pb[l * GB + j] = row[j];
+      A[(int64_t)(k0 + l) * ld + k0 + j] = row[j];
+    }
   }
 }
 
-// Row panel: A[k, j] <- P A[k, j] for every block column j != k.
-__global__ void __launch_bounds__(256) k_gj_row(double* __restrict__ A, int ld, int k, const double* __restrict__ Pb) {
+__global__ void __launch_bounds__(64) k_gj_pivot(double* __restrict__ A, int ld, int k, double* __restrict__ Pb) {
+  gj_pivot_wave(A, ld, k, Pb);
+}
+
+// Trailing update of tile (i, j), i, j != k: A_ij -= A_ik (P A_kj), panels
+// still unscaled.
+__global__ void __launch_bounds__(256) k_gj_update(double* __restrict__ A, int ld, int k,
+                                                   const double* __restrict__ Pb) {
   __shared__ double Ps[GB][GB + 1];
-  __shared__ double T[GB][GB + 1];
-  const int nbk = ld / GB;
-  int jb = blockIdx.x;
-  if (jb >= k) ++jb;
-  if (jb >= nbk) return;
-  const int k0 = k * GB, j0 = jb * GB;
-  for (int e = threadIdx.x; e < GB * GB; e += 256) {
-    Ps[e / GB][e % GB] = Pb[e];
-    T[e / GB][e % GB] = A[(int64_t)(k0 + e / GB) * ld + j0 + e % GB];
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < GB * GB; e += 256) {
-    const int i = e / GB, j = e % GB;
-    double s = 0;
-#pragma unroll 8
-    for (int t = 0; t < GB; ++t) s += Ps[i][t] * T[t][j];
-    A[(int64_t)(k0 + i) * ld + j0 + j] = s;
-  }
-}
-
-// Trailing update: A[i, j] -= A[i, k] A[k, j] for block rows i != k, block
-// columns j != k (A[k, j] already scaled by the row panel).
-__global__ void __launch_bounds__(256) k_gj_update(double* __restrict__ A, int ld, int k) {
   __shared__ double L[GB][GB + 1];
   __shared__ double R[GB][GB + 1];
+  __shared__ double T[GB][GB + 1];
   const int nbk = ld / GB;
   int ib = blockIdx.y, jb = blockIdx.x;
   if (ib >= k) ++ib;
@@ -173,39 +174,63 @@ __global__ void __launch_bounds__(256) k_gj_update(double* __restrict__ A, int l
   if (ib >= nbk || jb >= nbk) return;
   const int k0 = k * GB, i0 = ib * GB, j0 = jb * GB;
   for (int e = threadIdx.x; e < GB * GB; e += 256) {
-    L[e / GB][e % GB] = A[(int64_t)(i0 + e / GB) * ld + k0 + e % GB];
-    R[e / GB][e % GB] = A[(int64_t)(k0 + e / GB) * ld + j0 + e % GB];
+    const int r = e / GB, c = e % GB;
+    Ps[r][c] = Pb[e];
+    L[r][c] = A[(int64_t)(i0 + r) * ld + k0 + c];
+    R[r][c] = A[(int64_t)(k0 + r) * ld + j0 + c];
   }
   __syncthreads();
   for (int e = threadIdx.x; e < GB * GB; e += 256) {
-    const int i = e / GB, j = e % GB;
+    const int r = e / GB, c = e % GB;
     double s = 0;
 #pragma unroll 8
-    for (int t = 0; t < GB; ++t) s += L[i][t] * R[t][j];
-    A[(int64_t)(i0 + i) * ld + j0 + j] -= s;
+    for (int t = 0; t < GB; ++t) s += Ps[r][t] * R[t][c];
+    T[r][c] = s;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < GB * GB; e += 256) {
+    const int r = e / GB, c = e % GB;
+    double s = 0;
+#pragma unroll 8
+    for (int t = 0; t < GB; ++t) s += L[r][t] * T[t][c];
+    A[(int64_t)(i0 + r) * ld + j0 + c] -= s;
   }
 }
 
-// Column panel: A[i, k] <- -A[i, k] P for every block row i != k.
-__global__ void __launch_bounds__(256) k_gj_col(double* __restrict__ A, int ld, int k, const double* __restrict__ Pb) {
+// Panels of step k (blocks 0 .. nbk-2: row tiles, nbk-1 .. 2nbk-3: column
+// tiles) and, in the last block, the inversion of pivot block k+1.
+__global__ void __launch_bounds__(256) k_gj_panels(double* __restrict__ A, int ld, int k,
+                                                   const double* __restrict__ Pb, double* __restrict__ Pnext) {
   __shared__ double Ps[GB][GB + 1];
   __shared__ double T[GB][GB + 1];
   const int nbk = ld / GB;
-  int ib = blockIdx.x;
-  if (ib >= k) ++ib;
-  if (ib >= nbk) return;
-  const int k0 = k * GB, i0 = ib * GB;
+  const int b = blockIdx.x;
+  if (b == 2 * (nbk - 1)) {
+    if (k + 1 < nbk) gj_pivot_wave(A, ld, k + 1, Pnext);
+    return;
+  }
+  const bool rowp = b < nbk - 1;
+  int t = rowp ? b : b - (nbk - 1);
+  if (t >= k) ++t;
+  const int k0 = k * GB, t0 = t * GB;
   for (int e = threadIdx.x; e < GB * GB; e += 256) {
-    Ps[e / GB][e % GB] = Pb[e];
-    T[e / GB][e % GB] = A[(int64_t)(i0 + e / GB) * ld + k0 + e % GB];
+    const int r = e / GB, c = e % GB;
+    Ps[r][c] = Pb[e];
+    T[r][c] = rowp ? A[(int64_t)(k0 + r) * ld + t0 + c] : A[(int64_t)(t0 + r) * ld + k0 + c];
   }
   __syncthreads();
   for (int e = threadIdx.x; e < GB * GB; e += 256) {
-    const int i = e / GB, j = e % GB;
+    const int r = e / GB, c = e % GB;
     double s = 0;
+    if (rowp) {
 #pragma unroll 8
-    for (int t = 0; t < GB; ++t) s += T[i][t] * Ps[t][j];
-    A[(int64_t)(i0 + i) * ld + k0 + j] = -s;
+      for (int q = 0; q < GB; ++q) s += Ps[r][q] * T[q][c];
+      A[(int64_t)(k0 + r) * ld + t0 + c] = s;
+    } else {
+#pragma unroll 8
+      for (int q = 0; q < GB; ++q) s += T[r][q] * Ps[q][c];
+      A[(int64_t)(t0 + r) * ld + k0 + c] = -s;
+    }
   }
 }
 
@@ -232,12 +257,14 @@ static int coarse_setup_t(int ncb, const int* pair_ptr, const int* pair_node, co
                        pair_node, pair_pos, blk_I, blk_J, node_ptr, node_cols, vals, known, rel, Ac, ld);
   hipLaunchKernelGGL(k_coarse_guard, dim3((unsigned)((ld + 255) / 256)), dim3(256), 0, s, ld, Ac);
   const int nbk = ld / GB;
+  // Pb holds two GB x GB pivot inverses used in turn
+  hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(64), 0, s, Ac, ld, 0, Pb);
   for (int k = 0; k < nbk; ++k) {
-    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(256), 0, s, Ac, ld, k, Pb);
+    double* Pk = Pb + (k & 1) * GB * GB;
+    double* Pn = Pb + ((k + 1) & 1) * GB * GB;
     if (nbk > 1) {
-      hipLaunchKernelGGL(k_gj_row, dim3(nbk - 1), dim3(256), 0, s, Ac, ld, k, (const double*)Pb);
-      hipLaunchKernelGGL(k_gj_update, dim3(nbk - 1, nbk - 1), dim3(256), 0, s, Ac, ld, k);
-      hipLaunchKernelGGL(k_gj_col, dim3(nbk - 1), dim3(256), 0, s, Ac, ld, k, (const double*)Pb);
+      hipLaunchKernelGGL(k_gj_update, dim3(nbk - 1, nbk - 1), dim3(256), 0, s, Ac, ld, k, (const double*)Pk);
+      hipLaunchKernelGGL(k_gj_panels, dim3(2 * (nbk - 1) + 1), dim3(256), 0, s, Ac, ld, k, (const double*)Pk, Pn);
     }
   }
   hipLaunchKernelGGL(k_symmetrize, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, Ac, ld);
@@ -465,12 +492,13 @@ static int pcg2_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, con
 }
 
 int solve_pcg2(int mode, int Nnode, int dof_n, int group, const int* node_ptr, const int* node_cols,
-               const double* vals64, const float* vals32, const unsigned char* known, const void* Minv, int nagg,
+               const double* vals64, const float* vals32, const float* corr32, const unsigned char* known,
+               const void* Minv, int nagg,
                const int* agg_ptr, const int* agg_nodes, const double* rel, const double* Ainv, int ld,
                const double* b, double* x, void* ws, double tol_r, double tol_A, double delta, int maxit,
                int check_every, hipStream_t s, double* info) {
   SpmvArgs A64{Nnode, node_ptr, node_cols, vals64, known};
-  SpmvArgs A32{Nnode, node_ptr, node_cols, vals32, known};
+  SpmvArgs A32{Nnode, node_ptr, node_cols, vals32, known, corr32};
   CoarseArgs C{nagg, agg_ptr, agg_nodes, rel, Ainv, ld, nullptr};
   if (check_every < 1) check_every = 1;
 #define RUN2(D, GV)                                                                                            \

@@ -50,7 +50,6 @@ __device__ inline void pcg2_step2_body(CoarseArgs C, const unsigned char* __rest
   constexpr int NM = Modes<DN>::NM;
   if (st->done) return;
   __shared__ double sh[BS / 64];
-  __shared__ double sh2[BS / 64];
   __shared__ double bc;
   double ppq = 0;
   for (int i = threadIdx.x; i < nb1; i += BS) ppq += part1[i];
@@ -100,16 +99,18 @@ __device__ inline void pcg2_step2_body(CoarseArgs C, const unsigned char* __rest
       for (int m = 0; m < NM; ++m) rcv[m] += pmode<DN>(rn, a, m) * r2[a];  // r = 0 at known DOFs
     }
   }
-  s_rz = block_sum<double, BS>(s_rz, sh);
-  s_rr = block_sum<double, BS>(s_rr, sh2);
-  if (threadIdx.x == 0) {
-    part2[2 * blk] = s_rz;
-    part2[2 * blk + 1] = s_rr;
-  }
+  __shared__ double shn[NM + 2][BS / 64];
+  double red[NM + 2];
+  red[0] = s_rz;
+  red[1] = s_rr;
 #pragma unroll
-  for (int m = 0; m < NM; ++m) {
-    const double v = block_sum<double, BS>(rcv[m], sh);
-    if (threadIdx.x == 0) C.rc[blk * NM + m] = v;
+  for (int m = 0; m < NM; ++m) red[2 + m] = rcv[m];
+  block_sum_all<NM + 2, BS>(red, shn);
+  if (threadIdx.x == 0) {
+    part2[2 * blk] = red[0];
+    part2[2 * blk + 1] = red[1];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) C.rc[blk * NM + m] = red[2 + m];
   }
 }
 
@@ -119,15 +120,15 @@ __device__ inline void pcg2_prolong_body(CoarseArgs C, const unsigned char* __re
                double* __restrict__ part2, PcgState* st, int blk) {
   constexpr int NM = Modes<DN>::NM;
   if (st->done) return;
-  __shared__ double sh[BS / 64];
   __shared__ double zcs[NM];
   const int nc = C.nagg * NM;
-  for (int m = 0; m < NM; ++m) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int m = w; m < NM; m += BS / 64) {  // one wavefront per coarse row
     const double* row = C.Ainv + (int64_t)(blk * NM + m) * C.ld;
     double s = 0;
-    for (int c = threadIdx.x; c < nc; c += BS) s += row[c] * C.rc[c];
-    s = block_sum<double, BS>(s, sh);
-    if (threadIdx.x == 0) zcs[m] = s;
+    for (int c = lane; c < nc; c += 64) s += row[c] * C.rc[c];
+    s = warp_sum(s);
+    if (lane == 0) zcs[m] = s;
   }
   __syncthreads();
   double zc[NM];
@@ -163,7 +164,6 @@ __device__ inline void pcg2_init_body(CoarseArgs C, const unsigned char* __restr
             const double* __restrict__ partI, int nbI, double* __restrict__ part2, PcgState* st, int blk) {
   constexpr int NM = Modes<DN>::NM;
   __shared__ double sh[BS / 64];
-  __shared__ double sh2[BS / 64];
   if (blk == 0) {
     const double xax = reduce_all(partI, nbI, 1, 0, sh);
     if (threadIdx.x == 0) st->xAx0 = xax;
@@ -189,16 +189,18 @@ __device__ inline void pcg2_init_body(CoarseArgs C, const unsigned char* __restr
       for (int m = 0; m < NM; ++m) rcv[m] += pmode<DN>(rn, a, m) * rv[a];
     }
   }
-  s_rz = block_sum<double, BS>(s_rz, sh);
-  s_rr = block_sum<double, BS>(s_rr, sh2);
-  if (threadIdx.x == 0) {
-    part2[2 * blk] = s_rz;
-    part2[2 * blk + 1] = s_rr;
-  }
+  __shared__ double shn[NM + 2][BS / 64];
+  double red[NM + 2];
+  red[0] = s_rz;
+  red[1] = s_rr;
 #pragma unroll
-  for (int m = 0; m < NM; ++m) {
-    const double v = block_sum<double, BS>(rcv[m], sh);
-    if (threadIdx.x == 0) C.rc[blk * NM + m] = v;
+  for (int m = 0; m < NM; ++m) red[2 + m] = rcv[m];
+  block_sum_all<NM + 2, BS>(red, shn);
+  if (threadIdx.x == 0) {
+    part2[2 * blk] = red[0];
+    part2[2 * blk + 1] = red[1];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) C.rc[blk * NM + m] = red[2 + m];
   }
 }
 
@@ -210,7 +212,6 @@ __device__ inline void pcg2_replace_body(CoarseArgs C, SpmvArgs A64, const TP* _
                double* __restrict__ part2, PcgState* st, int blk) {
   constexpr int NM = Modes<DN>::NM;
   __shared__ double sh[BS / 64];
-  __shared__ double sh2[BS / 64];
   double s_rz = 0, s_rr = 0, rcv[NM];
 #pragma unroll
   for (int m = 0; m < NM; ++m) rcv[m] = 0;
@@ -233,16 +234,18 @@ __device__ inline void pcg2_replace_body(CoarseArgs C, SpmvArgs A64, const TP* _
       for (int m = 0; m < NM; ++m) rcv[m] += pmode<DN>(rn, a, m) * rv[a];
     }
   }
-  s_rz = block_sum<double, BS>(s_rz, sh);
-  s_rr = block_sum<double, BS>(s_rr, sh2);
-  if (threadIdx.x == 0) {
-    part2[2 * blk] = s_rz;
-    part2[2 * blk + 1] = s_rr;
-  }
+  __shared__ double shn[NM + 2][BS / 64];
+  double red[NM + 2];
+  red[0] = s_rz;
+  red[1] = s_rr;
 #pragma unroll
-  for (int m = 0; m < NM; ++m) {
-    const double v = block_sum<double, BS>(rcv[m], sh);
-    if (threadIdx.x == 0) C.rc[blk * NM + m] = v;
+  for (int m = 0; m < NM; ++m) red[2 + m] = rcv[m];
+  block_sum_all<NM + 2, BS>(red, shn);
+  if (threadIdx.x == 0) {
+    part2[2 * blk] = red[0];
+    part2[2 * blk + 1] = red[1];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) C.rc[blk * NM + m] = red[2 + m];
   }
   if (blk == 0 && threadIdx.x == 0) {
     st->done = 0;

@@ -121,4 +121,26 @@ __device__ inline T block_sum(T v, T* sh) {
   return s;
 }
 
+// Deterministic block reduction of N values at once (each value summed in the
+// order of block_sum), result valid in every thread; one barrier pair.
+template <int N, int BS>
+__device__ inline void block_sum_all(double (&v)[N], double (*sh)[BS / 64]) {
+#pragma unroll
+  for (int q = 0; q < N; ++q) v[q] = warp_sum(v[q]);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) sh[q][w] = v[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    double s = 0;
+#pragma unroll
+    for (int i = 0; i < BS / 64; ++i) s += sh[q][i];
+    v[q] = s;
+  }
+  __syncthreads();
+}
+
 }  // namespace efea

@@ -25,6 +25,10 @@ struct SpmvArgs {
   const int* node_cols;
   const void* vals;
   const unsigned char* known;
+  // FP32 values only: per node, the dof_n x dof_n block row sum of
+  // (K64 - K32), added to the node's own rows so that the rounded operator
+  // keeps K's row sums (rigid translations in the null space).  May be null.
+  const void* corr = nullptr;
 };
 
 // Node-block SpMV: G lanes cooperate on one node's dof_n rows.
@@ -157,7 +161,8 @@ __device__ inline void pcg_close_iteration(PcgState* st, double rz_new, double r
   const double scale = st->Sall > st->xAx0 ? st->Sall : st->xAx0;
   if (st->iters >= RING && Sw <= st->tolA2 * scale) st->done = 1;
   if (rr <= st->tolr2 * st->bb) st->done = 1;
-  if (!(rr == rr) || !(rz_new == rz_new)) st->done = 2;
+  // non-finite values (NaN, or an overflow that would pass the tests above)
+  if (!isfinite(rr) || !isfinite(rz_new) || !isfinite(Sw) || !isfinite(st->Sall)) st->done = 2;
   if (replaced && rr > 1e12 * st->bb) st->done = 3;  // true residual exploded: diverged
 }
 
@@ -249,15 +254,22 @@ __device__ inline void pcg_step1_body(const SpmvArgs& A, const double* __restric
       }
     }
     if (g0 == 0) {
+      double pn[DN];
+#pragma unroll
+      for (int a = 0; a < DN; ++a) pn[a] = z[(int64_t)n0 * DN + a] + beta * pold[(int64_t)n0 * DN + a];
 #pragma unroll
       for (int a = 0; a < DN; ++a) {
         const int64_t r = (int64_t)n0 * DN + a;
-        const double pn = z[r] + beta * pold[r];
-        pnew[r] = pn;
+        pnew[r] = pn[a];
         double qv = sz[a] + beta * sp[a];
+        if (A.corr) {
+          const TM* cr = (const TM*)A.corr + (int64_t)n0 * DN * DN + a * DN;
+#pragma unroll
+          for (int b = 0; b < DN; ++b) qv += (double)cr[b] * pn[b];
+        }
         if (A.known && A.known[r]) qv = 0;
         q[r] = qv;
-        d += pn * qv;
+        d += pn[a] * qv;
       }
     }
   }
@@ -286,15 +298,22 @@ __device__ inline void pcg_step1_body(const SpmvArgs& A, const double* __restric
       for (int o = G / 2; o > 0; o >>= 1) sum[a] += __shfl_xor(sum[a], o, G);
     }
     if (g == 0) {
+      double pn[DN];
+#pragma unroll
+      for (int a = 0; a < DN; ++a) pn[a] = z[(int64_t)n * DN + a] + beta * pold[(int64_t)n * DN + a];
 #pragma unroll
       for (int a = 0; a < DN; ++a) {
         const int64_t r = (int64_t)n * DN + a;
-        const double pn = z[r] + beta * pold[r];
-        pnew[r] = pn;
+        pnew[r] = pn[a];
         double qv = sum[a];
+        if (A.corr) {
+          const TM* cr = (const TM*)A.corr + (int64_t)n * DN * DN + a * DN;
+#pragma unroll
+          for (int b = 0; b < DN; ++b) qv += (double)cr[b] * pn[b];
+        }
         if (A.known && A.known[r]) qv = 0;
         q[r] = qv;
-        d += pn * qv;
+        d += pn[a] * qv;
       }
     }
   }

@@ -42,6 +42,50 @@ __global__ void k_cast64to32(int64_t n, const double* __restrict__ a, float* __r
   if (i < n) b[i] = (float)a[i];
 }
 
+// FP32 copy of the values and, per node, the dof_n x dof_n block row sums
+// of the rounding differences (K64 - K32), accumulated in FP64 and stored in
+// FP32 (see SpmvArgs::corr).  One thread per node.
+template <int DN>
+__global__ void k_split_f32(int Nnode, const int* __restrict__ node_ptr, const double* __restrict__ v64,
+                            float* __restrict__ v32, float* __restrict__ corr) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= Nnode) return;
+  const int p0 = node_ptr[n], cnt = node_ptr[n + 1] - p0;
+  const int64_t base = (int64_t)DN * DN * p0;
+#pragma unroll
+  for (int a = 0; a < DN; ++a) {
+    double c[DN];
+#pragma unroll
+    for (int b = 0; b < DN; ++b) c[b] = 0;
+    for (int pos = 0; pos < cnt; ++pos) {
+#pragma unroll
+      for (int b = 0; b < DN; ++b) {
+        const int64_t k = base + (int64_t)a * DN * cnt + (int64_t)pos * DN + b;
+        const double v = v64[k];
+        const float f = (float)v;
+        v32[k] = f;
+        c[b] += v - (double)f;
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < DN; ++b) corr[(int64_t)n * DN * DN + a * DN + b] = (float)c[b];
+  }
+}
+
+int split_f32(int Nnode, int dof_n, const int* node_ptr, const double* v64, float* v32, float* corr,
+              hipStream_t s) {
+  if (Nnode == 0) return EFEA_OK;
+  const unsigned nb = (unsigned)((Nnode + 255) / 256);
+  if (dof_n == 1) hipLaunchKernelGGL(k_split_f32<1>, dim3(nb), dim3(256), 0, s, Nnode, node_ptr, v64, v32, corr);
+  else if (dof_n == 2) hipLaunchKernelGGL(k_split_f32<2>, dim3(nb), dim3(256), 0, s, Nnode, node_ptr, v64, v32, corr);
+  else if (dof_n == 3) hipLaunchKernelGGL(k_split_f32<3>, dim3(nb), dim3(256), 0, s, Nnode, node_ptr, v64, v32, corr);
+  else {
+    set_error("split_f32: dof_n %d", dof_n);
+    return EFEA_ERR_ARG;
+  }
+  return check_launch("split_f32");
+}
+
 int cast_f64_f32(int64_t n, const double* a, float* b, hipStream_t s) {
   if (n == 0) return EFEA_OK;
   hipLaunchKernelGGL(k_cast64to32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a, b);
@@ -341,9 +385,12 @@ template void pcg_true_residual<3, 32>(const SpmvArgs&, const double*, const dou
 // Reads back the state and the true residual and fills info[0..3]
 // (iterations, relative residual, replacements, status).
 static int pcg_finish(const PcgState& hs, double rr_true, double* info) {
-  const int done = hs.done == 4 ? 0 : hs.done, iters = hs.iters;
+  int done = hs.done == 4 ? 0 : hs.done;
+  const int iters = hs.iters;
   info[0] = iters;
   info[1] = hs.bb > 0 ? sqrt(rr_true / hs.bb) : 0.0;
+  // a "converged" iterate whose true residual is not small is a divergence
+  if (done == 1 && !(info[1] <= 1e-4)) done = 3;
   info[2] = hs.nrepl;
   info[3] = done;
   if (done == 2 || done == 3) {
@@ -418,11 +465,11 @@ static int pcg_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, cons
 // preconditioner for the Krylov products, FP64 vectors, reliable FP64
 // residual replacement.
 int solve_pcg(int mode, int Nnode, int dof_n, int group, const int* node_ptr, const int* node_cols,
-              const double* vals64, const float* vals32, const unsigned char* known, const void* Minv,
-              const double* b, double* x, void* ws, double tol_r, double tol_A, double delta, int maxit,
-              int check_every, hipStream_t s, double* info) {
+              const double* vals64, const float* vals32, const float* corr32, const unsigned char* known,
+              const void* Minv, const double* b, double* x, void* ws, double tol_r, double tol_A, double delta,
+              int maxit, int check_every, hipStream_t s, double* info) {
   SpmvArgs A64{Nnode, node_ptr, node_cols, vals64, known};
-  SpmvArgs A32{Nnode, node_ptr, node_cols, vals32, known};
+  SpmvArgs A32{Nnode, node_ptr, node_cols, vals32, known, corr32};
   if (check_every < 1) check_every = 1;
 #define RUN(D, GV)                                                                                          \
   if (dof_n == D && group == GV) {                                                                         \

@@ -198,7 +198,7 @@ class CoarseSpace:
         self.pair_pos = to_dev(cp.pair_pos)
         self.blk_I = to_dev(cp.blk_I)
         self.blk_J = to_dev(cp.blk_J)
-        self.pivot = torch.empty(1024, dtype=torch.float64, device="cuda")
+        self.pivot = torch.empty(2048, dtype=torch.float64, device="cuda")
 
     def setup(self, A: "DeviceCsr", known: Optional[torch.Tensor]) -> torch.Tensor:
         """``Ac^-1`` (ld x ld, FP64) of ``A``'s current FP64 values."""
@@ -229,6 +229,7 @@ class DeviceCsr:
         self.dim = pattern.dim
         self.values = torch.empty(self.nnz, dtype=torch.float64, device="cuda")
         self.values32: Optional[torch.Tensor] = None
+        self.corr32: Optional[torch.Tensor] = None
         self._spmv_scratch: Optional[torch.Tensor] = None
 
     def coarse(self, target: Optional[int] = None) -> CoarseSpace:
@@ -253,11 +254,15 @@ class DeviceCsr:
         _lib.call("efea_csr_scatter", 64, self.nnz, ptr(self.seg), ptr(self.perm), ptr(Ke), ptr(self.values),
                   stream_ptr())
         self.values32 = None
+        self.corr32 = None
 
     def make_fp32(self) -> torch.Tensor:
+        """FP32 copy of the values and its row-sum correction (``efea_split_f32``)."""
         if self.values32 is None:
             self.values32 = torch.empty(self.nnz, dtype=torch.float32, device="cuda")
-            _lib.call("efea_cast_f64_f32", self.nnz, ptr(self.values), ptr(self.values32), stream_ptr())
+            self.corr32 = torch.empty(self.Nn * self.dof_n * self.dof_n, dtype=torch.float32, device="cuda")
+            _lib.call("efea_split_f32", self.Nn, self.dof_n, ptr(self.node_ptr), ptr(self.values),
+                      ptr(self.values32), ptr(self.corr32), stream_ptr())
         return self.values32
 
     def spmv(self, x: torch.Tensor, y: Optional[torch.Tensor] = None, known: Optional[torch.Tensor] = None,
@@ -382,12 +387,14 @@ class SolveJob:
         A = self.A
         raw = (C.c_double * 4)()
         v32 = A.values32 if self.precision == 32 else None
+        c32 = A.corr32 if self.precision == 32 else None
         if self.precond == "two-level":
             cs = self.cs
             nbytes = _lib.load().efea_pcg2_workspace_bytes(A.Nn, A.dof_n, cs.nagg)
             buf = ws.get(nbytes)
             rc = _lib.call("efea_solve_pcg2", self.precision, A.Nn, A.dof_n, A.group, ptr(A.node_ptr),
-                           ptr(A.node_cols), ptr(A.values), ptr(v32), ptr(self.known), ptr(self.Minv), cs.nagg,
+                           ptr(A.node_cols), ptr(A.values), ptr(v32), ptr(c32), ptr(self.known), ptr(self.Minv),
+                           cs.nagg,
                            ptr(cs.agg_ptr), ptr(cs.agg_nodes), ptr(cs.rel), ptr(self.Ainv), cs.ld, ptr(self.b),
                            ptr(self.x), ptr(buf), nbytes, self.tol_residual, self.tol_anorm, self.replace_ratio,
                            self.maxit, self.check_every, stream_ptr(), C.cast(raw, C.c_void_p))
@@ -395,7 +402,8 @@ class SolveJob:
         nbytes = _lib.load().efea_pcg_workspace_bytes(A.Nn, A.dof_n, A.group)
         buf = ws.get(nbytes)
         rc = _lib.call("efea_solve_pcg", self.precision, A.Nn, A.dof_n, A.group, ptr(A.node_ptr),
-                       ptr(A.node_cols), ptr(A.values), ptr(v32), ptr(self.known), ptr(self.Minv), ptr(self.b),
+                       ptr(A.node_cols), ptr(A.values), ptr(v32), ptr(c32), ptr(self.known), ptr(self.Minv),
+                       ptr(self.b),
                        ptr(self.x), ptr(buf), nbytes, self.tol_residual, self.tol_anorm, self.replace_ratio,
                        self.maxit, self.check_every, stream_ptr(), C.cast(raw, C.c_void_p))
         return rc, self.info(raw)
@@ -469,8 +477,9 @@ class BatchSolver:
             coarse = (0, None, None, None, None, 0)
         job.ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
         v32 = A.values32 if self.precision == 32 else None
+        c32 = A.corr32 if self.precision == 32 else None
         jid = _lib.call("efea_batch_add", self._h, A.Nn, ptr(A.node_ptr), ptr(A.node_cols), ptr(A.values), ptr(v32),
-                        ptr(job.known), ptr(job.Minv), *coarse, ptr(job.b), ptr(job.x), ptr(job.ws), nbytes,
+                        ptr(c32), ptr(job.known), ptr(job.Minv), *coarse, ptr(job.b), ptr(job.x), ptr(job.ws), nbytes,
                         job.tol_residual, job.tol_anorm, job.replace_ratio, job.maxit, stream_ptr())
         self._jobs[jid] = job
         return jid

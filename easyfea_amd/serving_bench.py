@@ -40,14 +40,18 @@ def _elastic_requests(specs, meshes, mode):
 
 
 def _prepare_meshes(meshes, keys, dof_n):
-    """Builds each mesh and its device pattern once (set-up)."""
+    """Builds each mesh, its device patterns and, where the solves will use
+    the two-level preconditioner, its aggregates once (set-up)."""
+    from .coarse import default_target
     from .device import DeviceMesh
 
     t = time.perf_counter()
     for k in sorted(set(keys)):
         m = meshes.get(k)
         for dn in dof_n:
-            DeviceMesh(m).pattern(dn)
+            p = DeviceMesh(m).pattern(dn)
+            if dn >= 2 and m.Nn >= 4 * default_target(m.dim):
+                p.coarse()
     return time.perf_counter() - t
 
 

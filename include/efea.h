@@ -76,6 +76,14 @@ int efea_csr_scatter(int precision, int64_t nslot, const int32_t* seg_ptr, const
 
 int efea_cast_f64_f32(int64_t n, const double* src, float* dst, void* stream);
 
+/* FP32 copy of the CSR values plus, per node, the dof_n x dof_n block row sums
+ * of the rounding differences (K64 - K32) in FP32 (corr32: Nnode * dof_n^2).
+ * Adding corr32 to the diagonal blocks keeps the rounded operator's row sums
+ * equal to K's, so rigid translations stay in its null space; without it the
+ * FP32 operator of a cracked body stalls the mixed solve near 1e-6 error. */
+int efea_split_f32(int Nnode, int dof_n, const int32_t* node_ptr, const double* values64, float* values32,
+                   float* corr32, void* stream);
+
 /* y = K x on the canonical CSR value array indexed through node adjacency
  * (node_ptr, node_cols): rows of `known` DOFs (uint8 per DOF, may be NULL)
  * return 0.  precision selects FP64 or FP32 values; x, y are FP64.  Runs the
@@ -102,17 +110,20 @@ int efea_default_group(int dof_n, int max_neighbours);
 /* Dirichlet-reduced solve A_ii x_i = b_i - A_ic x_c by block-Jacobi PCG on the
  * full-length vectors (known DOFs masked, x holds the Dirichlet values there and
  * the initial guess elsewhere; x is overwritten with the solution).
- * mode 64: FP64 matrix/preconditioner.  mode 32: FP32 matrix values and
- * preconditioner in the Krylov products, FP64 vectors, and the residual replaced
+ * mode 64: FP64 matrix/preconditioner.  mode 32: FP32 matrix values (with the
+ * efea_split_f32 correction, corr32, which may be NULL) and preconditioner in
+ * the Krylov products, FP64 vectors, and the residual replaced
  * by the FP64 r = b - K64 x whenever it falls below replace_ratio times its
  * running maximum.  Stops when ||r|| <= tol_residual ||r0|| or when the delayed
  * A-norm error estimate <= tol_anorm (relative).  Synchronises `stream`.
  * info (host, 4 doubles): iterations, final true relative residual, residual
- * replacements, status (1 converged, 2 breakdown, 0 iteration limit).
+ * replacements, status (1 converged, 2 breakdown or non-finite values,
+ * 3 diverged, 0 iteration limit).
  * Replaces __Solver_1 (Simulations/Solvers.py:502-553) with _Solve_Axb's
  * backends (Solvers.py:225-394; spsolve / pypardiso / cg). */
 int efea_solve_pcg(int mode, int Nnode, int dof_n, int group, const int32_t* node_ptr, const int32_t* node_cols,
-                   const double* values64, const float* values32, const uint8_t* known, const void* Minv,
+                   const double* values64, const float* values32, const float* corr32, const uint8_t* known,
+                   const void* Minv,
                    const double* b, double* x, void* workspace, size_t workspace_bytes, double tol_residual,
                    double tol_anorm, double replace_ratio, int maxit, int check_every, void* stream,
                    double* info);
@@ -124,7 +135,7 @@ int efea_solve_pcg(int mode, int Nnode, int dof_n, int group, const int32_t* nod
  * efea_coarse_setup assembles Ac = P^T A P from (node, neighbour position)
  * pairs grouped by aggregate block (blk_I[k], blk_J[k]) = pairs
  * pair_ptr[k]..pair_ptr[k+1]-1, and inverts it in place (blocked Gauss-Jordan,
- * FP64) into Ainv (ld x ld, ld = efea_coarse_ld; pivot_scratch: 1024 doubles).
+ * FP64) into Ainv (ld x ld, ld = efea_coarse_ld; pivot_scratch: 2048 doubles).
  * efea_solve_pcg2 is efea_solve_pcg with that preconditioner; agg_ptr /
  * agg_nodes list the nodes of each aggregate.  EasyFEA solves the reduced
  * system directly (Solvers.py:225-394); the preconditioner changes the
@@ -136,7 +147,8 @@ int efea_coarse_setup(int dof_n, int nblocks, const int32_t* pair_ptr, const int
                       int ld, double* Ainv, double* pivot_scratch, void* stream);
 size_t efea_pcg2_workspace_bytes(int Nnode, int dof_n, int nagg);
 int efea_solve_pcg2(int mode, int Nnode, int dof_n, int group, const int32_t* node_ptr, const int32_t* node_cols,
-                    const double* values64, const float* values32, const uint8_t* known, const void* Minv, int nagg,
+                    const double* values64, const float* values32, const float* corr32, const uint8_t* known,
+                    const void* Minv, int nagg,
                     const int32_t* agg_ptr, const int32_t* agg_nodes, const double* rel, const double* Ainv, int ld,
                     const double* b, double* x, void* workspace, size_t workspace_bytes, double tol_residual,
                     double tol_anorm, double replace_ratio, int maxit, int check_every, void* stream,
@@ -163,7 +175,8 @@ int efea_batch_create(int mode, int dof_n, int group, int two_level, int capacit
 int efea_batch_destroy(void* handle);
 /* Starts one solve (asynchronous); returns its id (>= 0) or an error code. */
 int efea_batch_add(void* handle, int Nnode, const int32_t* node_ptr, const int32_t* node_cols,
-                   const double* values64, const float* values32, const uint8_t* known, const void* Minv, int nagg,
+                   const double* values64, const float* values32, const float* corr32, const uint8_t* known,
+                   const void* Minv, int nagg,
                    const int32_t* agg_ptr, const int32_t* agg_nodes, const double* rel, const double* Ainv, int ld,
                    const double* b, double* x, void* workspace, size_t workspace_bytes, double tol_residual,
                    double tol_anorm, double replace_ratio, int maxit, void* stream);

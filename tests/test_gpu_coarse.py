@@ -142,3 +142,52 @@ def test_two_level_staggered_matches_oracle(solver):
         f, fr = np.sum(simu.Calc_Reaction(dofsY, "elastic")), np.sum(ref.Calc_Reaction(dofsY))
         assert abs(f - fr) <= 1e-6 * abs(fr)
         assert simu.last_solve["elastic"].precond == "two-level"
+
+
+def test_fp32_split_keeps_block_row_sums():
+    """efea_split_f32: FP32 values are the rounded FP64 ones and K32 + corr32
+    has K64's block row sums to FP32 accuracy of the (tiny) correction."""
+    mesh = ea.tri3_notched_square(30)
+    dev = DeviceMesh(mesh)
+    A = dev.csr_map(2)
+    rng = np.random.default_rng(3)
+    Ne, nPe = mesh.connect.shape
+    G = rng.standard_normal((Ne, 6, 6))
+    A.scatter(to_dev((np.einsum("eij,ekj->eik", G, G) * 1e5).ravel()))
+    A.make_fp32()
+    v64 = A.values.cpu().numpy()
+    v32 = A.values32.cpu().numpy()
+    assert np.array_equal(v32, v64.astype(np.float32))
+    corr = A.corr32.cpu().numpy().reshape(mesh.Nn, 2, 2)
+    K64 = A.to_scipy().tocoo()
+    d = (K64.data - K64.data.astype(np.float32).astype(np.float64))
+    ref = np.zeros((mesh.Nn, 2, 2))
+    np.add.at(ref, (K64.row // 2, K64.row % 2, K64.col % 2), d)
+    assert np.abs(corr - ref).max() <= 1e-6 * np.abs(ref).max() + 1e-30
+
+
+def test_mixed_two_level_on_cracked_body_reaches_fp64_accuracy():
+    """The notched square's displacement system (194 x 194 cells, first
+    increment): FP32 values without the row-sum correction stall near 5e-7
+    error and then diverge; with it the mixed solve matches FP64 to 1e-8."""
+    from easyfea_amd.device import SolveJob
+
+    mesh = ea.tri3_notched_square(194)
+    mat = Models.Elastic.Isotropic(2, 210000.0, 0.3, planeStress=False, thickness=1.0)
+    out = {}
+    for solver in ("hip_fp64", "hip_mixed"):
+        simu = Simulations.PhaseField(mesh, Models.PhaseField(mat, "Amor", "AT2", 2.7, 0.01))
+        simu.solver = solver
+        simu.preconditioner = "two-level"
+        top = mesh.Nodes_Conditions(lambda x, y, z: y == 1.0)
+        bot = mesh.Nodes_Conditions(lambda x, y, z: y == 0.0)
+        simu.add_dirichlet(bot, [0], ["y"])
+        simu.add_dirichlet(top, [0, 1e-4], ["x", "y"])
+        simu.Solve(1.0, 500, 1)
+        out[solver] = (simu.displacement, simu.last_solve["elastic"])
+    u64, i64 = out["hip_fp64"]
+    u32, i32 = out["hip_mixed"]
+    print(f"fp64 {i64.iterations} it, mixed {i32.iterations} it ({i32.mode}), rel res {i32.rel_residual:.1e}")
+    assert i32.mode == "mixed" and not i32.fallback and i32.status == 1
+    assert i32.iterations <= 1.3 * i64.iterations
+    assert _rel(u32, u64) < 1e-8
