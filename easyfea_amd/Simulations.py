@@ -80,10 +80,11 @@ class _Simu:
         self.Niter = 0
         self.tol_residual = 1e-12
         self.tol_anorm = 1e-9
-        self.replace_ratio = 1e-3
+        self.replace_ratio = 0.0  # mixed mode: FP64 residual replacement at every poll
         self.maxit = 200000
         self.check_every = 32
         self.preconditioner = "auto"  # "bj" | "two-level" | "auto"
+        self.max_blocks = 0  # grid cap of the solver kernels (0: the library's default)
         self.last_solve = {}
         self.Bc_Init()
         model._Add_observer(self)
@@ -313,7 +314,8 @@ class _Simu:
         if idx.size:
             x_dev.index_copy_(0, idx_dev, to_dev(xk[idx]))
         return SolveJob(A, b_dev, x_dev, known, self._mode(), self.tol_residual, self.tol_anorm,
-                        self.replace_ratio, self.maxit, self.check_every, owner=self, precond=self.preconditioner)
+                        self.replace_ratio, self.maxit, self.check_every, owner=self, precond=self.preconditioner,
+                        max_blocks=self.max_blocks)
 
     def _solve_system_gen(self, A, b_dev: torch.Tensor, x_dev: torch.Tensor, problemType):
         """Generator form of a solve: yields the ``SolveJob`` and receives its

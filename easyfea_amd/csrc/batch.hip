@@ -1,8 +1,8 @@
 // Continuous-batching PCG: many independent systems advance together, one
 // launch per PCG step for all of them.
 //
-// Each system keeps the grid it would get alone (grid_for(Nnode*G) blocks for
-// the node-group kernel, grid_for(Nnode) for the node kernel); a launch
+// Each system keeps the grid it would get alone (grid_for(Nnode*G, cap) blocks
+// for the node-group kernel, grid_for(Nnode, cap) for the node kernel); a launch
 // concatenates those grids and a block->system map sends every block to its
 // system's descriptor.  The per-block code is the single-system code
 // (pcg_device.h), so every system's reductions run in the same order as in
@@ -61,12 +61,20 @@ __global__ void __launch_bounds__(BS) kb_step2(const BatchSys* __restrict__ sys,
                          S.nbS, S.part2, S.st, (int)blockIdx.x - S.blkN0, S.nbN);
 }
 
-template <typename TP, int DN>
-__global__ void __launch_bounds__(BS) kb_replace(const BatchSys* __restrict__ sys, const int* __restrict__ map) {
+template <int DN, int G>
+__global__ void __launch_bounds__(BS) kb_replace_residual(const BatchSys* __restrict__ sys,
+                                                          const int* __restrict__ map) {
   const BatchSys& S = sys[map[blockIdx.x]];
   if (!S.replace) return;
-  pcg_replace_body<TP, DN>(S.A64, (const TP*)S.Minv, S.x, S.b, S.r, S.z, S.part2, S.st,
-                           (int)blockIdx.x - S.blkN0, S.nbN);
+  residual_body<DN, G>(S.A64, S.x, S.b, S.r, (int)blockIdx.x - S.blkS0, S.nbS);
+}
+
+template <typename TP, int DN>
+__global__ void __launch_bounds__(BS) kb_replace_z(const BatchSys* __restrict__ sys, const int* __restrict__ map) {
+  const BatchSys& S = sys[map[blockIdx.x]];
+  if (!S.replace) return;
+  rebuild_z_body<TP, DN>(S.A64.Nnode, S.A64.known, (const TP*)S.Minv, S.r, S.z, S.part2, S.st,
+                         (int)blockIdx.x - S.blkN0, S.nbN);
 }
 
 template <typename TP, int DN>
@@ -86,11 +94,11 @@ __global__ void __launch_bounds__(BS) kb2_prolong(const BatchSys* __restrict__ s
 }
 
 template <typename TP, int DN>
-__global__ void __launch_bounds__(BS) kb2_replace(const BatchSys* __restrict__ sys, const int* __restrict__ map) {
+__global__ void __launch_bounds__(BS) kb2_replace_z(const BatchSys* __restrict__ sys, const int* __restrict__ map) {
   const BatchSys& S = sys[map[blockIdx.x]];
   if (!S.replace) return;
-  pcg2_replace_body<TP, DN>(S.C, S.A64, (const TP*)S.Minv, S.x, S.b, S.r, S.z, S.part2, S.st,
-                            (int)blockIdx.x - S.blkA0);
+  pcg2_rebuild_z_body<TP, DN>(S.C, S.A64.known, (const TP*)S.Minv, S.r, S.z, S.part2, S.st,
+                              (int)blockIdx.x - S.blkA0);
 }
 
 // One block per system: copies its state into a contiguous array for a
@@ -107,6 +115,7 @@ struct BatchEntry {
   PcgWork w;
   int launched, maxit;
   bool finished;
+  bool periodic;  // mixed mode, replace_ratio = 0: residual replaced at every poll
   double info[4];
 };
 
@@ -204,7 +213,8 @@ int batch_add(void* h, int Nnode, const int* node_ptr, const int* node_cols, con
               const float* vals32, const float* corr32, const unsigned char* known, const void* Minv, int nagg,
               const int* agg_ptr,
               const int* agg_nodes, const double* rel, const double* Ainv, int ld, const double* b, double* x,
-              void* ws, size_t ws_bytes, double tol_r, double tol_A, double delta, int maxit, hipStream_t s) {
+              void* ws, size_t ws_bytes, double tol_r, double tol_A, double delta, int maxit, int max_blocks,
+              hipStream_t s) {
   Batch* B = (Batch*)h;
   if (B->two_level) {
     if (nagg < 1 || nagg > MAXA || !agg_ptr || !agg_nodes || !rel || !Ainv || ld < 1) {
@@ -234,8 +244,9 @@ int batch_add(void* h, int Nnode, const int* node_ptr, const int* node_cols, con
   e.launched = 0;
   e.maxit = maxit;
   e.finished = false;
+  e.periodic = delta <= 0;
   BatchSys& d = e.d;
-  d.A64 = SpmvArgs{Nnode, node_ptr, node_cols, vals64, known};
+  d.A64 = SpmvArgs{Nnode, node_ptr, node_cols, vals64, known, nullptr, max_blocks};
   d.valsK = B->mode == 32 ? (const void*)vals32 : (const void*)vals64;
   d.corrK = B->mode == 32 ? (const void*)corr32 : nullptr;
   d.Minv = Minv;
@@ -249,8 +260,8 @@ int batch_add(void* h, int Nnode, const int* node_ptr, const int* node_cols, con
   d.part1 = e.w.part1;
   d.part2 = e.w.part2;
   d.st = e.w.st;
-  d.nbS = (int)grid_for((int64_t)Nnode * B->group);
-  d.nbN = (int)grid_for(Nnode);
+  d.nbS = (int)grid_for((int64_t)Nnode * B->group, max_blocks);
+  d.nbN = (int)grid_for(Nnode, max_blocks);
   d.C = CoarseArgs{nagg, agg_ptr, agg_nodes, rel, Ainv, ld, v.rc};
   d.nb2 = B->two_level ? nagg : d.nbN;
   d.replace = 0;
@@ -344,18 +355,20 @@ static int batch_run_t(Batch* B, int iters, hipStream_t s) {
     BatchEntry& e = B->active[i];
     const PcgState& st = B->h_st[i];
     e.launched += iters;
-    e.d.replace = st.done == 4;
-    any_replace |= st.done == 4;
     e.finished = (st.done != 0 && st.done != 4) || e.launched >= e.maxit;
+    // mixed mode: replacement after a pause, or at every poll (periodic)
+    e.d.replace = st.done == 4 || (B->mode == 32 && e.periodic && st.done == 0 && !e.finished);
+    any_replace |= e.d.replace != 0;
     nfin += e.finished;
   }
   if (any_replace) {
     if (int e = upload_sys(B, s)) return e;
+    hipLaunchKernelGGL((kb_replace_residual<DN, G>), dim3(B->totS), dim3(BS), 0, s, dsys, mapS);
     if (B->two_level) {
-      hipLaunchKernelGGL((kb2_replace<TP, DN>), dim3(B->totA), dim3(BS), 0, s, dsys, mapA);
+      hipLaunchKernelGGL((kb2_replace_z<TP, DN>), dim3(B->totA), dim3(BS), 0, s, dsys, mapA);
       hipLaunchKernelGGL((kb2_prolong<DN, true>), dim3(B->totA), dim3(BS), 0, s, dsys, mapA);
     } else {
-      hipLaunchKernelGGL((kb_replace<TP, DN>), dim3(B->totN), dim3(BS), 0, s, dsys, mapN);
+      hipLaunchKernelGGL((kb_replace_z<TP, DN>), dim3(B->totN), dim3(BS), 0, s, dsys, mapN);
     }
     if (int e = check_launch("batch_replace")) return e;
     for (int i = 0; i < n; ++i) B->active[i].d.replace = 0;

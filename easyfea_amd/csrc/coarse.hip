@@ -21,8 +21,13 @@
 namespace efea {
 
 int pcg_finish_info(const void* state, double rr_true, double* info);
+template <int DN, int G>
+void launch_replace_residual(const SpmvArgs& A64, const double* x, const double* b, double* r, hipStream_t s);
 
 constexpr int GB = 32;  // Gauss-Jordan block
+// Up to this many blocks per dimension the two-launch step (update forming
+// P A_kj per tile) wins: the launches, not the flops, bound it.
+constexpr int GJ_FUSED_MAX = 32;
 
 __global__ void k_fill_f64(int64_t n, double* __restrict__ a, double v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -159,8 +164,10 @@ __global__ void __launch_bounds__(64) k_gj_pivot(double* __restrict__ A, int ld,
   gj_pivot_wave(A, ld, k, Pb);
 }
 
-// Trailing update of tile (i, j), i, j != k: A_ij -= A_ik (P A_kj), panels
-// still unscaled.
+// Trailing update of tile (i, j), i, j != k: A_ij -= A_ik (P A_kj) with the
+// row panel unscaled (PRESCALED false: the tile forms P A_kj itself) or
+// A_ij -= A_ik A_kj with the row panel already scaled (PRESCALED true).
+template <bool PRESCALED>
 __global__ void __launch_bounds__(256) k_gj_update(double* __restrict__ A, int ld, int k,
                                                    const double* __restrict__ Pb) {
   __shared__ double Ps[GB][GB + 1];
@@ -175,7 +182,7 @@ __global__ void __launch_bounds__(256) k_gj_update(double* __restrict__ A, int l
   const int k0 = k * GB, i0 = ib * GB, j0 = jb * GB;
   for (int e = threadIdx.x; e < GB * GB; e += 256) {
     const int r = e / GB, c = e % GB;
-    Ps[r][c] = Pb[e];
+    if (!PRESCALED) Ps[r][c] = Pb[e];
     L[r][c] = A[(int64_t)(i0 + r) * ld + k0 + c];
     R[r][c] = A[(int64_t)(k0 + r) * ld + j0 + c];
   }
@@ -183,8 +190,12 @@ __global__ void __launch_bounds__(256) k_gj_update(double* __restrict__ A, int l
   for (int e = threadIdx.x; e < GB * GB; e += 256) {
     const int r = e / GB, c = e % GB;
     double s = 0;
+    if (PRESCALED) {
+      s = R[r][c];
+    } else {
 #pragma unroll 8
-    for (int t = 0; t < GB; ++t) s += Ps[r][t] * R[t][c];
+      for (int t = 0; t < GB; ++t) s += Ps[r][t] * R[t][c];
+    }
     T[r][c] = s;
   }
   __syncthreads();
@@ -197,20 +208,23 @@ __global__ void __launch_bounds__(256) k_gj_update(double* __restrict__ A, int l
   }
 }
 
-// Panels of step k (blocks 0 .. nbk-2: row tiles, nbk-1 .. 2nbk-3: column
-// tiles) and, in the last block, the inversion of pivot block k+1.
+// Panels of step k and the inversion of pivot block k+1.  WHICH 0: blocks
+// 0 .. nbk-2 row tiles, nbk-1 .. 2nbk-3 column tiles, then the pivot block;
+// WHICH 1: row tiles only; WHICH 2: column tiles, then the pivot block.
+template <int WHICH>
 __global__ void __launch_bounds__(256) k_gj_panels(double* __restrict__ A, int ld, int k,
                                                    const double* __restrict__ Pb, double* __restrict__ Pnext) {
   __shared__ double Ps[GB][GB + 1];
   __shared__ double T[GB][GB + 1];
   const int nbk = ld / GB;
+  const int nrow = WHICH == 2 ? 0 : nbk - 1, ncol = WHICH == 1 ? 0 : nbk - 1;
   const int b = blockIdx.x;
-  if (b == 2 * (nbk - 1)) {
-    if (k + 1 < nbk) gj_pivot_wave(A, ld, k + 1, Pnext);
+  if (b >= nrow + ncol) {
+    if (WHICH != 1 && k + 1 < nbk) gj_pivot_wave(A, ld, k + 1, Pnext);
     return;
   }
-  const bool rowp = b < nbk - 1;
-  int t = rowp ? b : b - (nbk - 1);
+  const bool rowp = b < nrow;
+  int t = rowp ? b : b - nrow;
   if (t >= k) ++t;
   const int k0 = k * GB, t0 = t * GB;
   for (int e = threadIdx.x; e < GB * GB; e += 256) {
@@ -262,9 +276,15 @@ static int coarse_setup_t(int ncb, const int* pair_ptr, const int* pair_node, co
   for (int k = 0; k < nbk; ++k) {
     double* Pk = Pb + (k & 1) * GB * GB;
     double* Pn = Pb + ((k + 1) & 1) * GB * GB;
-    if (nbk > 1) {
-      hipLaunchKernelGGL(k_gj_update, dim3(nbk - 1, nbk - 1), dim3(256), 0, s, Ac, ld, k, (const double*)Pk);
-      hipLaunchKernelGGL(k_gj_panels, dim3(2 * (nbk - 1) + 1), dim3(256), 0, s, Ac, ld, k, (const double*)Pk, Pn);
+    if (nbk > 1 && nbk <= GJ_FUSED_MAX) {
+      // two launches per step: the update forms P A_kj itself
+      hipLaunchKernelGGL(k_gj_update<false>, dim3(nbk - 1, nbk - 1), dim3(256), 0, s, Ac, ld, k, (const double*)Pk);
+      hipLaunchKernelGGL(k_gj_panels<0>, dim3(2 * (nbk - 1) + 1), dim3(256), 0, s, Ac, ld, k, (const double*)Pk, Pn);
+    } else if (nbk > 1) {
+      // large matrices: scale the row panel once (half the update flops)
+      hipLaunchKernelGGL(k_gj_panels<1>, dim3(nbk - 1), dim3(256), 0, s, Ac, ld, k, (const double*)Pk, Pn);
+      hipLaunchKernelGGL(k_gj_update<true>, dim3(nbk - 1, nbk - 1), dim3(256), 0, s, Ac, ld, k, (const double*)Pk);
+      hipLaunchKernelGGL(k_gj_panels<2>, dim3(nbk), dim3(256), 0, s, Ac, ld, k, (const double*)Pk, Pn);
     }
   }
   hipLaunchKernelGGL(k_symmetrize, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, Ac, ld);
@@ -325,10 +345,9 @@ k_pcg2_init(CoarseArgs C, const unsigned char* __restrict__ known, const TP* __r
 
 template <typename TP, int DN>
 __global__ void __launch_bounds__(BS)
-k_pcg2_replace(CoarseArgs C, SpmvArgs A64, const TP* __restrict__ Minv, const double* __restrict__ x,
-               const double* __restrict__ b, double* __restrict__ r, double* __restrict__ z,
-               double* __restrict__ part2, PcgState* st) {
-  pcg2_replace_body<TP, DN>(C, A64, Minv, x, b, r, z, part2, st, blockIdx.x);
+k_pcg2_replace_z(CoarseArgs C, const unsigned char* __restrict__ known, const TP* __restrict__ Minv,
+                 const double* __restrict__ r, double* __restrict__ z, double* __restrict__ part2, PcgState* st) {
+  pcg2_rebuild_z_body<TP, DN>(C, known, Minv, r, z, part2, st, blockIdx.x);
 }
 
 // Step 1 on a two-level solve: the one-level kernel (partials of step 2 now
@@ -406,7 +425,7 @@ int pcg2_start(const SpmvArgs& A64, const TP* Minv, const CoarseArgs& C, const d
   const PcgWork& w = v.w;
   CoarseArgs Cw = C;
   Cw.rc = v.rc;
-  const unsigned nbS = grid_for((int64_t)A64.Nnode * G);
+  const unsigned nbS = grid_for((int64_t)A64.Nnode * G, A64.cap);
   const unsigned nbA = (unsigned)C.nagg;
   hipLaunchKernelGGL(k_state_reset2, dim3(1), dim3(64), 0, s, w.st, tol_r * tol_r, tol_A * tol_A, delta2);
   hipLaunchKernelGGL((k_xax_residual<DN, G>), dim3(nbS), dim3(BS), 0, s, A64, x, b, w.r, w.q, w.partI);
@@ -443,7 +462,7 @@ static int pcg2_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, con
   const PcgWork& w = v.w;
   CoarseArgs Cw = C;
   Cw.rc = v.rc;
-  const unsigned nbS = grid_for((int64_t)Nnode * G);
+  const unsigned nbS = grid_for((int64_t)Nnode * G, A64.cap);
   const unsigned nbA = (unsigned)C.nagg;
   const bool mixed = (void*)AK.vals != (void*)A64.vals;
   if (int e = pcg2_start<TP, DN, G>(A64, Minv, C, b, x, v, tol_r, tol_A, mixed ? delta * delta : 0.0, s)) return e;
@@ -470,9 +489,10 @@ static int pcg2_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, con
       set_error("pcg2: state readback failed");
       return EFEA_ERR_HIP;
     }
-    if (hs->done == 4) {
-      hipLaunchKernelGGL((k_pcg2_replace<TP, DN>), dim3(nbA), dim3(BS), 0, s, Cw, A64, Minv, (const double*)x, b,
-                         w.r, w.z, w.part2, w.st);
+    if (hs->done == 4 || (mixed && delta <= 0 && hs->done == 0 && launched < maxit)) {
+      launch_replace_residual<DN, G>(A64, (const double*)x, b, w.r, s);
+      hipLaunchKernelGGL((k_pcg2_replace_z<TP, DN>), dim3(nbA), dim3(BS), 0, s, Cw, A64.known, Minv,
+                         (const double*)w.r, w.z, w.part2, w.st);
       hipLaunchKernelGGL((k_pcg2_prolong<DN>), dim3(nbA), dim3(BS), 0, s, Cw, A64.known, w.z, w.part2, w.st);
       if (int e = check_launch("pcg2_replace")) return e;
       if (launched >= maxit) break;
@@ -496,9 +516,9 @@ int solve_pcg2(int mode, int Nnode, int dof_n, int group, const int* node_ptr, c
                const void* Minv, int nagg,
                const int* agg_ptr, const int* agg_nodes, const double* rel, const double* Ainv, int ld,
                const double* b, double* x, void* ws, double tol_r, double tol_A, double delta, int maxit,
-               int check_every, hipStream_t s, double* info) {
-  SpmvArgs A64{Nnode, node_ptr, node_cols, vals64, known};
-  SpmvArgs A32{Nnode, node_ptr, node_cols, vals32, known, corr32};
+               int check_every, int max_blocks, hipStream_t s, double* info) {
+  SpmvArgs A64{Nnode, node_ptr, node_cols, vals64, known, nullptr, max_blocks};
+  SpmvArgs A32{Nnode, node_ptr, node_cols, vals32, known, corr32, max_blocks};
   CoarseArgs C{nagg, agg_ptr, agg_nodes, rel, Ainv, ld, nullptr};
   if (check_every < 1) check_every = 1;
 #define RUN2(D, GV)                                                                                            \

@@ -51,19 +51,39 @@ __device__ inline void pcg2_step2_body(CoarseArgs C, const unsigned char* __rest
   if (st->done) return;
   __shared__ double sh[BS / 64];
   __shared__ double bc;
+  const int pn = st->par_next;
+  const double rz = st->rz;
+  const double* __restrict__ p = pn ? P1 : P0;
+  const int k0 = C.agg_ptr[blk], k1 = C.agg_ptr[blk + 1];
+  // the first node's operands are loaded before the reduction that yields alpha
+  const int kf = k0 + threadIdx.x;
+  const bool have = kf < k1;
+  const int nf = have ? C.agg_nodes[kf] : 0;
+  double xf[DN], pf[DN], rf[DN], qf[DN], Mf[DN * DN], relf[3];
+  if (have) {
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      const int64_t i = (int64_t)nf * DN + a;
+      xf[a] = x[i];
+      pf[a] = p[i];
+      rf[a] = r[i];
+      qf[a] = q[i];
+    }
+#pragma unroll
+    for (int e = 0; e < DN * DN; ++e) Mf[e] = (double)Minv[(int64_t)nf * DN * DN + e];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) relf[d] = C.rel[3 * (int64_t)nf + d];
+  }
   double ppq = 0;
   for (int i = threadIdx.x; i < nb1; i += BS) ppq += part1[i];
   ppq = block_sum<double, BS>(ppq, sh);
   if (threadIdx.x == 0) bc = ppq;
   __syncthreads();
   const double pq = bc;
-  const double rz = st->rz;
   if (!(pq > 0) || !(rz > 0)) {
     if (blk == 0 && threadIdx.x == 0) st->done = (rz == 0) ? 1 : 2;
     return;
   }
-  const int pn = st->par_next;
-  const double* __restrict__ p = pn ? P1 : P0;
   const double al = rz / pq;
   if (blk == 0 && threadIdx.x == 0) {
     st->alpha = al;
@@ -77,8 +97,29 @@ __device__ inline void pcg2_step2_body(CoarseArgs C, const unsigned char* __rest
   double s_rz = 0, s_rr = 0, rcv[NM];
 #pragma unroll
   for (int m = 0; m < NM; ++m) rcv[m] = 0;
-  const int k0 = C.agg_ptr[blk], k1 = C.agg_ptr[blk + 1];
-  for (int k = k0 + threadIdx.x; k < k1; k += BS) {
+  if (have) {
+    double r2[DN];
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      const int64_t i = (int64_t)nf * DN + a;
+      x[i] = xf[a] + al * pf[a];
+      r2[a] = rf[a] - al * qf[a];
+      r[i] = r2[a];
+    }
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      double zz = 0;
+#pragma unroll
+      for (int b = 0; b < DN; ++b) zz += Mf[a * DN + b] * r2[b];
+      if (known && known[(int64_t)nf * DN + a]) zz = 0;
+      z[(int64_t)nf * DN + a] = zz;
+      s_rz += r2[a] * zz;
+      s_rr += r2[a] * r2[a];
+#pragma unroll
+      for (int m = 0; m < NM; ++m) rcv[m] += pmode<DN>(relf, a, m) * r2[a];  // r = 0 at known DOFs
+    }
+  }
+  for (int k = kf + BS; k < k1; k += BS) {
     const int n = C.agg_nodes[k];
     double r2[DN], zv[DN];
 #pragma unroll
@@ -96,7 +137,7 @@ __device__ inline void pcg2_step2_body(CoarseArgs C, const unsigned char* __rest
       s_rz += r2[a] * zv[a];
       s_rr += r2[a] * r2[a];
 #pragma unroll
-      for (int m = 0; m < NM; ++m) rcv[m] += pmode<DN>(rn, a, m) * r2[a];  // r = 0 at known DOFs
+      for (int m = 0; m < NM; ++m) rcv[m] += pmode<DN>(rn, a, m) * r2[a];
     }
   }
   __shared__ double shn[NM + 2][BS / 64];
@@ -122,6 +163,22 @@ __device__ inline void pcg2_prolong_body(CoarseArgs C, const unsigned char* __re
   if (st->done) return;
   __shared__ double zcs[NM];
   const int nc = C.nagg * NM;
+  const int k0 = C.agg_ptr[blk], k1 = C.agg_ptr[blk + 1];
+  // the first node's data is loaded before the coarse rows
+  const int kf = k0 + threadIdx.x;
+  const bool have = kf < k1;
+  const int nf = have ? C.agg_nodes[kf] : 0;
+  double relf[3], zf[DN];
+  bool kfree[DN];
+  if (have) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) relf[d] = C.rel[3 * (int64_t)nf + d];
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      zf[a] = z[(int64_t)nf * DN + a];
+      kfree[a] = !(known && known[(int64_t)nf * DN + a]);
+    }
+  }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int m = w; m < NM; m += BS / 64) {  // one wavefront per coarse row
     const double* row = C.Ainv + (int64_t)(blk * NM + m) * C.ld;
@@ -134,8 +191,17 @@ __device__ inline void pcg2_prolong_body(CoarseArgs C, const unsigned char* __re
   double zc[NM];
 #pragma unroll
   for (int m = 0; m < NM; ++m) zc[m] = zcs[m];
-  const int k0 = C.agg_ptr[blk], k1 = C.agg_ptr[blk + 1];
-  for (int k = k0 + threadIdx.x; k < k1; k += BS) {
+  if (have) {
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      if (!kfree[a]) continue;
+      double v = 0;
+#pragma unroll
+      for (int m = 0; m < NM; ++m) v += pmode<DN>(relf, a, m) * zc[m];
+      z[(int64_t)nf * DN + a] = zf[a] + v;
+    }
+  }
+  for (int k = kf + BS; k < k1; k += BS) {
     const int n = C.agg_nodes[k];
     const double* rn = C.rel + 3 * (int64_t)n;
 #pragma unroll
@@ -204,26 +270,25 @@ __device__ inline void pcg2_init_body(CoarseArgs C, const unsigned char* __restr
   }
 }
 
-// Reliable update: r = b - A64 x, z = B^-1 r, partials, restriction; the
+// Residual replacement, second kernel (aggregate grid, after residual_body):
+// z = B^-1 r, partials, restriction; block 0 resumes the solve.  The
 // prolongation kernel follows.
 template <typename TP, int DN>
-__device__ inline void pcg2_replace_body(CoarseArgs C, SpmvArgs A64, const TP* __restrict__ Minv, const double* __restrict__ x,
-               const double* __restrict__ b, double* __restrict__ r, double* __restrict__ z,
-               double* __restrict__ part2, PcgState* st, int blk) {
+__device__ inline void pcg2_rebuild_z_body(CoarseArgs C, const unsigned char* __restrict__ known,
+                                           const TP* __restrict__ Minv, const double* __restrict__ r,
+                                           double* __restrict__ z, double* __restrict__ part2, PcgState* st,
+                                           int blk) {
   constexpr int NM = Modes<DN>::NM;
-  __shared__ double sh[BS / 64];
   double s_rz = 0, s_rr = 0, rcv[NM];
 #pragma unroll
   for (int m = 0; m < NM; ++m) rcv[m] = 0;
   const int k0 = C.agg_ptr[blk], k1 = C.agg_ptr[blk + 1];
   for (int k = k0 + threadIdx.x; k < k1; k += BS) {
     const int n = C.agg_nodes[k];
-    double dd;
-    spmv_node<double, double, DN, 1, 1>(A64, n, 0, x, b, r, dd);
     double rv[DN], zv[DN];
 #pragma unroll
     for (int a = 0; a < DN; ++a) rv[a] = r[(int64_t)n * DN + a];
-    apply_bj<TP, DN>(Minv, A64.known, n, rv, zv);
+    apply_bj<TP, DN>(Minv, known, n, rv, zv);
     const double* rn = C.rel + 3 * (int64_t)n;
 #pragma unroll
     for (int a = 0; a < DN; ++a) {

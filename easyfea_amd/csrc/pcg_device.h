@@ -29,6 +29,9 @@ struct SpmvArgs {
   // (K64 - K32), added to the node's own rows so that the rounded operator
   // keeps K's row sums (rigid translations in the null space).  May be null.
   const void* corr = nullptr;
+  // most blocks of this system's grids (<= 0: MAXB); part of the reduction
+  // order, so two solves agree bitwise only with the same cap
+  int cap = 0;
 };
 
 // Node-block SpMV: G lanes cooperate on one node's dof_n rows.
@@ -88,9 +91,10 @@ __device__ inline void spmv_node(const SpmvArgs& A, int n, int g, const TV* __re
 constexpr int BS = 256;
 constexpr int MAXB = 1024;
 
-static inline unsigned grid_for(int64_t threads) {
+static inline unsigned grid_for(int64_t threads, int cap = 0) {
+  const int64_t mx = (cap > 0 && cap < MAXB) ? cap : MAXB;
   int64_t nb = (threads + BS - 1) / BS;
-  if (nb > MAXB) nb = MAXB;
+  if (nb > mx) nb = mx;
   if (nb < 1) nb = 1;
   return (unsigned)nb;
 }
@@ -418,37 +422,45 @@ __device__ inline void pcg_step2_body(int Nnode, const unsigned char* __restrict
   }
 }
 
-// Reliable update (mixed mode, run by the host after a pause):
-// r = b - A64 x, z = Minv r, r.z / r.r partials in place of step 2's.  It
-// runs on step 2's node grid (one lane per node) so that its partials have
-// the layout step 1 reduces.
-template <typename TP, int DN>
-__device__ inline void pcg_replace_body(const SpmvArgs& A64, const TP* __restrict__ Minv,
-                                        const double* __restrict__ x, const double* __restrict__ b,
-                                        double* __restrict__ r, double* __restrict__ z, double* __restrict__ part2,
-                                        PcgState* st, int blk, int nblk) {
-  __shared__ double sh[BS / 64];
-  __shared__ double sh2[BS / 64];
-  double s_rz = 0, s_rr = 0;
-  for (int n = blk * BS + threadIdx.x; n < A64.Nnode; n += nblk * BS) {
+// Residual replacement (mixed mode), first kernel: r = b - A64 x on the
+// node-group grid (coalesced like the SpMV), masked rows.
+template <int DN, int G>
+__device__ inline void residual_body(const SpmvArgs& A64, const double* __restrict__ x,
+                                     const double* __restrict__ b, double* __restrict__ r, int blk, int nblk) {
+  const int64_t lanes = (int64_t)A64.Nnode * G;
+  for (int64_t gid = (int64_t)blk * BS + threadIdx.x; gid < lanes; gid += (int64_t)nblk * BS) {
+    const int n = (int)(gid / G), g = (int)(gid % G);
     double dd;
-    spmv_node<double, double, DN, 1, 1>(A64, n, 0, x, b, r, dd);
+    spmv_node<double, double, DN, G, 1>(A64, n, g, x, b, r, dd);
+  }
+}
+
+// Residual replacement, second kernel (node grid, the layout step 1
+// reduces): z = Minv r, partials of r.z and r.r; block 0 resumes the solve
+// with the replaced residual.
+template <typename TP, int DN>
+__device__ inline void rebuild_z_body(int Nnode, const unsigned char* __restrict__ known,
+                                      const TP* __restrict__ Minv, const double* __restrict__ r,
+                                      double* __restrict__ z, double* __restrict__ part2, PcgState* st, int blk,
+                                      int nblk) {
+  __shared__ double shn[2][BS / 64];
+  double red[2] = {0, 0};
+  for (int n = blk * BS + threadIdx.x; n < Nnode; n += nblk * BS) {
     double rv[DN], zv[DN];
 #pragma unroll
     for (int a = 0; a < DN; ++a) rv[a] = r[(int64_t)n * DN + a];
-    apply_bj<TP, DN>(Minv, A64.known, n, rv, zv);
+    apply_bj<TP, DN>(Minv, known, n, rv, zv);
 #pragma unroll
     for (int a = 0; a < DN; ++a) {
       z[(int64_t)n * DN + a] = zv[a];
-      s_rz += rv[a] * zv[a];
-      s_rr += rv[a] * rv[a];
+      red[0] += rv[a] * zv[a];
+      red[1] += rv[a] * rv[a];
     }
   }
-  s_rz = block_sum<double, BS>(s_rz, sh);
-  s_rr = block_sum<double, BS>(s_rr, sh2);
+  block_sum_all<2, BS>(red, shn);
   if (threadIdx.x == 0) {
-    part2[2 * blk] = s_rz;
-    part2[2 * blk + 1] = s_rr;
+    part2[2 * blk] = red[0];
+    part2[2 * blk + 1] = red[1];
   }
   if (blk == 0 && threadIdx.x == 0) {
     st->done = 0;

@@ -44,38 +44,56 @@ __global__ void k_cast64to32(int64_t n, const double* __restrict__ a, float* __r
 
 // FP32 copy of the values and, per node, the dof_n x dof_n block row sums
 // of the rounding differences (K64 - K32), accumulated in FP64 and stored in
-// FP32 (see SpmvArgs::corr).  One thread per node.
+// FP32 (see SpmvArgs::corr).  16 lanes per node walk its positions (coalesced
+// like the SpMV) and reduce with shuffles.
 template <int DN>
-__global__ void k_split_f32(int Nnode, const int* __restrict__ node_ptr, const double* __restrict__ v64,
-                            float* __restrict__ v32, float* __restrict__ corr) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= Nnode) return;
-  const int p0 = node_ptr[n], cnt = node_ptr[n + 1] - p0;
-  const int64_t base = (int64_t)DN * DN * p0;
+__global__ void __launch_bounds__(256) k_split_f32(int Nnode, const int* __restrict__ node_ptr,
+                                                   const double* __restrict__ v64, float* __restrict__ v32,
+                                                   float* __restrict__ corr) {
+  constexpr int G = 16;
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = (int)(gid / G), g = (int)(gid % G);
+  const bool have = n < Nnode;
+  double c[DN][DN];
 #pragma unroll
-  for (int a = 0; a < DN; ++a) {
-    double c[DN];
+  for (int a = 0; a < DN; ++a)
 #pragma unroll
-    for (int b = 0; b < DN; ++b) c[b] = 0;
-    for (int pos = 0; pos < cnt; ++pos) {
+    for (int b = 0; b < DN; ++b) c[a][b] = 0;
+  if (have) {
+    const int p0 = node_ptr[n], cnt = node_ptr[n + 1] - p0;
+    const int64_t base = (int64_t)DN * DN * p0;
+    for (int pos = g; pos < cnt; pos += G) {
 #pragma unroll
-      for (int b = 0; b < DN; ++b) {
-        const int64_t k = base + (int64_t)a * DN * cnt + (int64_t)pos * DN + b;
-        const double v = v64[k];
-        const float f = (float)v;
-        v32[k] = f;
-        c[b] += v - (double)f;
-      }
+      for (int a = 0; a < DN; ++a)
+#pragma unroll
+        for (int b = 0; b < DN; ++b) {
+          const int64_t k = base + (int64_t)a * DN * cnt + (int64_t)pos * DN + b;
+          const double v = v64[k];
+          const float f = (float)v;
+          v32[k] = f;
+          c[a][b] += v - (double)f;
+        }
     }
+  }
 #pragma unroll
-    for (int b = 0; b < DN; ++b) corr[(int64_t)n * DN * DN + a * DN + b] = (float)c[b];
+  for (int a = 0; a < DN; ++a)
+#pragma unroll
+    for (int b = 0; b < DN; ++b) {
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) c[a][b] += __shfl_xor(c[a][b], o, G);
+    }
+  if (have && g == 0) {
+#pragma unroll
+    for (int a = 0; a < DN; ++a)
+#pragma unroll
+      for (int b = 0; b < DN; ++b) corr[(int64_t)n * DN * DN + a * DN + b] = (float)c[a][b];
   }
 }
 
 int split_f32(int Nnode, int dof_n, const int* node_ptr, const double* v64, float* v32, float* corr,
               hipStream_t s) {
   if (Nnode == 0) return EFEA_OK;
-  const unsigned nb = (unsigned)((Nnode + 255) / 256);
+  const unsigned nb = (unsigned)(((int64_t)Nnode * 16 + 255) / 256);
   if (dof_n == 1) hipLaunchKernelGGL(k_split_f32<1>, dim3(nb), dim3(256), 0, s, Nnode, node_ptr, v64, v32, corr);
   else if (dof_n == 2) hipLaunchKernelGGL(k_split_f32<2>, dim3(nb), dim3(256), 0, s, Nnode, node_ptr, v64, v32, corr);
   else if (dof_n == 3) hipLaunchKernelGGL(k_split_f32<3>, dim3(nb), dim3(256), 0, s, Nnode, node_ptr, v64, v32, corr);
@@ -135,12 +153,18 @@ k_pcg_step2(int Nnode, const unsigned char* __restrict__ known, const TP* __rest
   pcg_step2_body<TP, DN>(Nnode, known, Minv, x, r, z, P0, P1, q, part1, nb1, part2, st, blockIdx.x, gridDim.x);
 }
 
+template <int DN, int G>
+__global__ void __launch_bounds__(BS)
+k_replace_residual(SpmvArgs A64, const double* __restrict__ x, const double* __restrict__ b,
+                   double* __restrict__ r) {
+  residual_body<DN, G>(A64, x, b, r, blockIdx.x, gridDim.x);
+}
+
 template <typename TP, int DN>
 __global__ void __launch_bounds__(BS)
-k_pcg_replace(SpmvArgs A64, const TP* __restrict__ Minv, const double* __restrict__ x,
-              const double* __restrict__ b, double* __restrict__ r, double* __restrict__ z,
-              double* __restrict__ part2, PcgState* st) {
-  pcg_replace_body<TP, DN>(A64, Minv, x, b, r, z, part2, st, blockIdx.x, gridDim.x);
+k_replace_z(int Nnode, const unsigned char* __restrict__ known, const TP* __restrict__ Minv,
+            const double* __restrict__ r, double* __restrict__ z, double* __restrict__ part2, PcgState* st) {
+  rebuild_z_body<TP, DN>(Nnode, known, Minv, r, z, part2, st, blockIdx.x, gridDim.x);
 }
 
 // Residual r = b - A x (FP64), masked; per-block partials of r.r.
@@ -340,8 +364,8 @@ template <typename TP, int DN, int G>
 int pcg_start(const SpmvArgs& A64, const TP* Minv, const double* b, const double* x, const PcgWork& w,
               double tol_r, double tol_A, double delta2, hipStream_t s) {
   const int Nnode = A64.Nnode;
-  const unsigned nbS = grid_for((int64_t)Nnode * G);
-  const unsigned nbN = grid_for(Nnode);
+  const unsigned nbS = grid_for((int64_t)Nnode * G, A64.cap);
+  const unsigned nbN = grid_for(Nnode, A64.cap);
   hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, w.st, tol_r * tol_r, tol_A * tol_A, delta2);
   hipLaunchKernelGGL((k_xax<DN, G>), dim3(nbS), dim3(BS), 0, s, A64, x, w.q, w.partI);
   hipLaunchKernelGGL((k_residual<DN, G>), dim3(nbS), dim3(BS), 0, s, A64, x, b, w.r, w.part1);
@@ -354,7 +378,7 @@ int pcg_start(const SpmvArgs& A64, const TP* Minv, const double* b, const double
 // Final true residual r.r = |b - A x|^2 into w.small[0].  Asynchronous on s.
 template <int DN, int G>
 void pcg_true_residual(const SpmvArgs& A64, const double* b, const double* x, const PcgWork& w, hipStream_t s) {
-  const unsigned nbS = grid_for((int64_t)A64.Nnode * G);
+  const unsigned nbS = grid_for((int64_t)A64.Nnode * G, A64.cap);
   hipLaunchKernelGGL((k_residual<DN, G>), dim3(nbS), dim3(BS), 0, s, A64, x, b, w.r, w.part1);
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(BS), 0, s, (const double*)w.part1, (int)nbS, w.small);
 }
@@ -381,6 +405,20 @@ template void pcg_true_residual<2, 8>(const SpmvArgs&, const double*, const doub
 template void pcg_true_residual<2, 16>(const SpmvArgs&, const double*, const double*, const PcgWork&, hipStream_t);
 template void pcg_true_residual<3, 16>(const SpmvArgs&, const double*, const double*, const PcgWork&, hipStream_t);
 template void pcg_true_residual<3, 32>(const SpmvArgs&, const double*, const double*, const PcgWork&, hipStream_t);
+
+// r = b - A64 x (masked rows) on the node-group grid; used by the two-level
+// driver's residual replacement.
+template <int DN, int G>
+void launch_replace_residual(const SpmvArgs& A64, const double* x, const double* b, double* r, hipStream_t s) {
+  hipLaunchKernelGGL((k_replace_residual<DN, G>), dim3(grid_for((int64_t)A64.Nnode * G, A64.cap)), dim3(BS), 0, s, A64, x, b,
+                     r);
+}
+template void launch_replace_residual<1, 8>(const SpmvArgs&, const double*, const double*, double*, hipStream_t);
+template void launch_replace_residual<1, 16>(const SpmvArgs&, const double*, const double*, double*, hipStream_t);
+template void launch_replace_residual<2, 8>(const SpmvArgs&, const double*, const double*, double*, hipStream_t);
+template void launch_replace_residual<2, 16>(const SpmvArgs&, const double*, const double*, double*, hipStream_t);
+template void launch_replace_residual<3, 16>(const SpmvArgs&, const double*, const double*, double*, hipStream_t);
+template void launch_replace_residual<3, 32>(const SpmvArgs&, const double*, const double*, double*, hipStream_t);
 
 // Reads back the state and the true residual and fills info[0..3]
 // (iterations, relative residual, replacements, status).
@@ -415,8 +453,8 @@ static int pcg_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, cons
                    hipStream_t s, double* info) {
   const int Nnode = A64.Nnode;
   PcgWork w = carve(ws, Nnode, DN);
-  const unsigned nbS = grid_for((int64_t)Nnode * G);  // node-group kernels
-  const unsigned nbN = grid_for(Nnode);               // node kernels
+  const unsigned nbS = grid_for((int64_t)Nnode * G, A64.cap);  // node-group kernels
+  const unsigned nbN = grid_for(Nnode, A64.cap);               // node kernels
   const bool mixed = (void*)AK.vals != (void*)A64.vals;
   if (int e = pcg_start<TP, DN, G>(A64, Minv, b, x, w, tol_r, tol_A, mixed ? delta * delta : 0.0, s)) return e;
   PcgState* hs = host_state();
@@ -441,9 +479,12 @@ static int pcg_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, cons
       set_error("pcg: state readback failed");
       return EFEA_ERR_HIP;
     }
-    if (hs->done == 4) {  // mixed mode: reliable update, then continue
-      hipLaunchKernelGGL((k_pcg_replace<TP, DN>), dim3(nbN), dim3(BS), 0, s, A64, Minv, (const double*)x, b, w.r,
-                         w.z, w.part2, w.st);
+    // mixed mode: the residual is replaced by the FP64 b - K x after a pause
+    // (replace_ratio > 0) or at every poll (replace_ratio = 0: periodic)
+    if (hs->done == 4 || (mixed && delta <= 0 && hs->done == 0 && launched < maxit)) {
+      hipLaunchKernelGGL((k_replace_residual<DN, G>), dim3(nbS), dim3(BS), 0, s, A64, (const double*)x, b, w.r);
+      hipLaunchKernelGGL((k_replace_z<TP, DN>), dim3(nbN), dim3(BS), 0, s, Nnode, A64.known, Minv,
+                         (const double*)w.r, w.z, w.part2, w.st);
       if (int e = check_launch("pcg_replace")) return e;
       if (launched >= maxit) break;
       continue;
@@ -467,9 +508,9 @@ static int pcg_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, cons
 int solve_pcg(int mode, int Nnode, int dof_n, int group, const int* node_ptr, const int* node_cols,
               const double* vals64, const float* vals32, const float* corr32, const unsigned char* known,
               const void* Minv, const double* b, double* x, void* ws, double tol_r, double tol_A, double delta,
-              int maxit, int check_every, hipStream_t s, double* info) {
-  SpmvArgs A64{Nnode, node_ptr, node_cols, vals64, known};
-  SpmvArgs A32{Nnode, node_ptr, node_cols, vals32, known, corr32};
+              int maxit, int check_every, int max_blocks, hipStream_t s, double* info) {
+  SpmvArgs A64{Nnode, node_ptr, node_cols, vals64, known, nullptr, max_blocks};
+  SpmvArgs A32{Nnode, node_ptr, node_cols, vals32, known, corr32, max_blocks};
   if (check_every < 1) check_every = 1;
 #define RUN(D, GV)                                                                                          \
   if (dof_n == D && group == GV) {                                                                         \

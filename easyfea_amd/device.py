@@ -343,10 +343,11 @@ class SolveJob:
 
     def __init__(self, A: "DeviceCsr", b: torch.Tensor, x: torch.Tensor, known: Optional[torch.Tensor],
                  mode: str = "mixed", tol_residual: float = 1e-12, tol_anorm: float = 1e-9,
-                 replace_ratio: float = 1e-3, maxit: int = 100000, check_every: int = 32, adaptive: bool = True,
-                 owner=None, precond: str = "auto", coarse_target: Optional[int] = None):
+                 replace_ratio: float = 0.0, maxit: int = 100000, check_every: int = 32, adaptive: bool = True,
+                 owner=None, precond: str = "auto", coarse_target: Optional[int] = None, max_blocks: int = 0):
         self.A, self.b, self.x, self.known = A, b, x, known
         self.precond = choose_preconditioner(A, precond)
+        self.max_blocks = int(max_blocks)
         self.coarse_target = coarse_target
         self.cs: Optional[CoarseSpace] = None
         self.Ainv: Optional[torch.Tensor] = None
@@ -397,7 +398,7 @@ class SolveJob:
                            cs.nagg,
                            ptr(cs.agg_ptr), ptr(cs.agg_nodes), ptr(cs.rel), ptr(self.Ainv), cs.ld, ptr(self.b),
                            ptr(self.x), ptr(buf), nbytes, self.tol_residual, self.tol_anorm, self.replace_ratio,
-                           self.maxit, self.check_every, stream_ptr(), C.cast(raw, C.c_void_p))
+                           self.maxit, self.check_every, self.max_blocks, stream_ptr(), C.cast(raw, C.c_void_p))
             return rc, self.info(raw)
         nbytes = _lib.load().efea_pcg_workspace_bytes(A.Nn, A.dof_n, A.group)
         buf = ws.get(nbytes)
@@ -405,7 +406,7 @@ class SolveJob:
                        ptr(A.node_cols), ptr(A.values), ptr(v32), ptr(c32), ptr(self.known), ptr(self.Minv),
                        ptr(self.b),
                        ptr(self.x), ptr(buf), nbytes, self.tol_residual, self.tol_anorm, self.replace_ratio,
-                       self.maxit, self.check_every, stream_ptr(), C.cast(raw, C.c_void_p))
+                       self.maxit, self.check_every, self.max_blocks, stream_ptr(), C.cast(raw, C.c_void_p))
         return rc, self.info(raw)
 
     def run(self, ws: Optional[Workspace] = None, raise_on_fail: bool = True) -> SolveInfo:
@@ -423,7 +424,7 @@ class SolveJob:
 
 
 def solve(A: DeviceCsr, b: torch.Tensor, x: torch.Tensor, known: Optional[torch.Tensor], mode: str,
-          ws: Workspace, tol_residual: float = 1e-12, tol_anorm: float = 1e-9, replace_ratio: float = 1e-3,
+          ws: Workspace, tol_residual: float = 1e-12, tol_anorm: float = 1e-9, replace_ratio: float = 0.0,
           maxit: int = 100000, check_every: int = 32, raise_on_fail: bool = True) -> SolveInfo:
     """Solves the Dirichlet-reduced system on the device (``efea_solve_pcg``).
 
@@ -480,7 +481,7 @@ class BatchSolver:
         c32 = A.corr32 if self.precision == 32 else None
         jid = _lib.call("efea_batch_add", self._h, A.Nn, ptr(A.node_ptr), ptr(A.node_cols), ptr(A.values), ptr(v32),
                         ptr(c32), ptr(job.known), ptr(job.Minv), *coarse, ptr(job.b), ptr(job.x), ptr(job.ws), nbytes,
-                        job.tol_residual, job.tol_anorm, job.replace_ratio, job.maxit, stream_ptr())
+                        job.tol_residual, job.tol_anorm, job.replace_ratio, job.maxit, job.max_blocks, stream_ptr())
         self._jobs[jid] = job
         return jid
 
@@ -522,29 +523,40 @@ class BatchExecutor:
         self.capacity = capacity
         self.check_every = check_every
         self.batches = {}
+        self.waiting = {}  # jobs of a full batch, in submission order
 
     def submit(self, job: SolveJob) -> None:
-        job.setup(job.precision)
         key = (job.precision, job.A.dof_n, job.A.group, job.precond)
         b = self.batches.get(key)
         if b is None:
             b = self.batches[key] = BatchSolver(*key, capacity=self.capacity)
+            self.waiting[key] = []
+        if b.full() or self.waiting[key]:
+            self.waiting[key].append(job)
+            return
+        job.setup(job.precision)
         b.add(job)
 
     def active(self) -> int:
-        return sum(len(b) for b in self.batches.values())
+        return sum(len(b) for b in self.batches.values()) + sum(len(w) for w in self.waiting.values())
 
     def step(self) -> list:
         """One chunk of every non-empty batch; returns the completed
-        ``(job, SolveInfo)`` pairs (fallback restarts are resubmitted)."""
+        ``(job, SolveInfo)`` pairs (fallback restarts are resubmitted);
+        waiting jobs take the slots that freed up."""
         done = []
-        for b in list(self.batches.values()):
+        for key, b in list(self.batches.items()):
             for job, si in b.run(self.check_every):
                 if si.status != 1 and job.precision == 32:
                     job.restart_fp64()
                     self.submit(job)
                     continue
                 done.append((job, si))
+            w = self.waiting[key]
+            while w and not b.full():
+                job = w.pop(0)
+                job.setup(job.precision)
+                b.add(job)
         return done
 
     def close(self) -> None:

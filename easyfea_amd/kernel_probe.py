@@ -34,6 +34,11 @@ def _system(workload: str):
         mesh = hexa8_cube(n)
         simu = Simulations.Elastic(mesh, Models.Elastic.Isotropic(3, 210000.0, 0.3))
         return simu.Assembly()
+    if workload.startswith("square"):
+        n = int(workload[6:])
+        mesh = quad4_rectangle(1.0, 1.0, n, n)
+        simu = Simulations.Elastic(mesh, Models.Elastic.Isotropic(2, 210000.0, 0.3, True, 1.0))
+        return simu.Assembly()
     if workload == "cantilever":
         mesh = quad4_rectangle(120.0, 13.0, 360, 39)
         simu = Simulations.Elastic(mesh, Models.Elastic.Isotropic(2, 210000.0, 0.3, True, 13.0))
@@ -49,12 +54,12 @@ def algorithmic_bytes(A, value_bytes: int) -> int:
             + A.Ndof * 8 * 2)
 
 
-def time_step1(A, precision: int, reps: int) -> float:
+def time_step1(A, precision: int, reps: int, group: int = 0) -> float:
     x = torch.rand(A.Ndof, dtype=torch.float64, device="cuda")
     y = torch.empty_like(x)
     vals = A.values if precision == 64 else A.make_fp32()
-    head = (precision, A.Nn, A.dof_n, A.group, ptr(A.node_ptr), ptr(A.node_cols), ptr(vals), None, ptr(x), ptr(y),
-            ptr(A.spmv_scratch))
+    head = (precision, A.Nn, A.dof_n, group or A.group, ptr(A.node_ptr), ptr(A.node_cols), ptr(vals), None, ptr(x),
+            ptr(y), ptr(A.spmv_scratch))
     _lib.call("efea_spmv_repeat", *head, 10, stream_ptr())
     s = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -70,6 +75,7 @@ def main():
     ap.add_argument("--workload", default="pf200")
     ap.add_argument("--reps", type=int, default=200)
     ap.add_argument("--copy-mib", type=int, default=256)
+    ap.add_argument("--groups", default="", help="comma-separated lanes-per-node values to time as well")
     args = ap.parse_args()
     torch.cuda.set_device(0)
     A = _system(args.workload)
@@ -78,6 +84,11 @@ def main():
         dt = time_step1(A, prec, args.reps)
         b = algorithmic_bytes(A, vb)
         out[f"fp{prec}"] = {"avg_us": dt * 1e6, "bytes": b, "GBps": b / dt / 1e9, "frac_of_8TBps": b / dt / 8e12}
+    for g in [int(v) for v in args.groups.split(",") if v]:
+        for prec, vb in ((32, 4), (64, 8)):
+            dt = time_step1(A, prec, args.reps, g)
+            b = algorithmic_bytes(A, vb)
+            out[f"fp{prec}_g{g}"] = {"avg_us": dt * 1e6, "GBps": b / dt / 1e9}
     n = args.copy_mib * 2**20 // 8
     src = torch.rand(n, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)

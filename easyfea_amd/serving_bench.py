@@ -28,13 +28,13 @@ from .serving import Request, Server, estimate_bytes, percentiles
 GIB = 2**30
 
 
-def _elastic_requests(specs, meshes, mode):
+def _elastic_requests(specs, meshes, mode, max_blocks=0):
     reqs = []
     for spec in specs:
         m = meshes.get(spec.side)
         nnz = m.__dict__.setdefault("_nnz2", None) or 18 * m.Nn * 2
         reqs.append(Request(spec.rid, spec.arrival,
-                            (lambda s=spec, m_=m: workloads.elastic_request(s, m_, mode)),
+                            (lambda s=spec, m_=m: workloads.elastic_request(s, m_, mode, "auto", max_blocks)),
                             estimate_bytes(m.Nn, m.Ne, 4, (2,), (nnz,), mode == "mixed")))
     return reqs
 
@@ -61,14 +61,15 @@ def run_elastic(args) -> dict:
     specs = workloads.serving_elastic_specs(args.requests, args.qps, args.side_lo, args.side_hi, args.seed)
     meshes = workloads.MeshCache(workloads.square_quad4)
     setup_s = _prepare_meshes(meshes, [s.side for s in specs], (2,))
-    # warm-up: code objects, pinned buffers
-    warm = workloads.serving_elastic_specs(4, 0.0, 32, 48, 99)
-    wm = workloads.MeshCache(workloads.square_quad4)
-    Server(args.hbm_gib * GIB, capacity=args.capacity).run(_elastic_requests(warm, wm, args.mode))
+    # warm-up: the whole stream once (code objects, pinned buffers, and the
+    # device allocator's cache of every buffer size the stream uses)
+    for mode in ("fp64", "mixed"):
+        Server(args.hbm_gib * GIB, capacity=args.capacity).run(_elastic_requests(specs, meshes, mode,
+                                                                                 args.max_blocks))
     torch.cuda.synchronize()
 
     def stream(mode):
-        reqs = _elastic_requests(specs, meshes, mode)
+        reqs = _elastic_requests(specs, meshes, mode, args.max_blocks)
         srv = Server(args.hbm_gib * GIB, capacity=args.capacity, check_every=args.check_every)
         t0 = time.perf_counter()
         srv.run(reqs)
@@ -85,6 +86,7 @@ def run_elastic(args) -> dict:
         "workload": f"serving stream: {len(specs)} QUAD4 plane-stress squares, side U{{{args.side_lo}..{args.side_hi}}}, "
                     f"correlated lognormal E (CoV 0.2), Poisson arrivals at {args.qps} QPS",
         "mode": args.mode,
+        "max_blocks": args.max_blocks,
         "value": ndof / wall,
         "unit": "DOF/s",
         "wall_s": wall,
@@ -131,16 +133,22 @@ def run_phasefield(args) -> dict:
             m = meshes.get(s.cells)
             est = estimate_bytes(m.Nn, m.Ne, 3, (2, 1), (28 * m.Nn, 7 * m.Nn), mode == "mixed")
             est += s.steps * m.Nn * 3 * 8 * 2  # saved iterations (u, d per increment)
-            reqs.append(Request(s.rid, 0.0, (lambda s_=s, m_=m: workloads.phasefield_request(s_, m_, mode, clock)),
+            reqs.append(Request(s.rid, 0.0, (lambda s_=s, m_=m: workloads.phasefield_request(s_, m_, mode, clock,
+                                                                                             args.max_blocks)),
                                 est))
         return reqs
 
-    warm = workloads.serving_phasefield_specs(2, 2, 2000, 3000, seed=77)
-    Server(args.hbm_gib * GIB).run(requests(warm, args.mode))
-    torch.cuda.synchronize()
-
     def fresh(specs_):
-        return [workloads.PhaseFieldRequestSpec(s.rid, s.cells, s.Gc, 0.0, s.steps) for s in specs_]
+        return [workloads.PhaseFieldRequestSpec(s.rid, s.cells, s.Gc, 0.0, min(s.steps, 2)) if warm else
+                workloads.PhaseFieldRequestSpec(s.rid, s.cells, s.Gc, 0.0, s.steps) for s in specs_]
+
+    # warm-up: two increments of every realisation in both modes (code objects,
+    # pinned buffers, the device allocator's cache)
+    warm = True
+    for mode in ("fp64", "mixed"):
+        Server(args.hbm_gib * GIB, capacity=args.capacity).run(requests(fresh(specs), mode))
+    torch.cuda.synchronize()
+    warm = False
 
     def stream(specs_, mode):
         srv = Server(args.hbm_gib * GIB, capacity=args.capacity, check_every=args.check_every)
@@ -199,6 +207,8 @@ def main(argv=None):
     ap.add_argument("--capacity", type=int, default=256)
     ap.add_argument("--check-every", type=int, default=32)
     ap.add_argument("--hbm-gib", type=float, default=200.0)
+    ap.add_argument("--max-blocks", type=int, default=0,
+                    help="grid cap per system of the solver kernels (0: the library default)")
     ap.add_argument("--compare-fp64", action="store_true")
     ap.add_argument("--adaptive", action="store_true",
                     help="keep FP64 Krylov products for matrices under 128 MiB (single-solve policy)")

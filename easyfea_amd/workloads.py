@@ -150,12 +150,13 @@ def square_quad4(side: int) -> Mesh:
     return quad4_rectangle(1.0, 1.0, side, side)
 
 
-def elastic_request(spec: ElasticRequestSpec, mesh: Mesh, mode: str, precond: str = "auto"):
+def elastic_request(spec: ElasticRequestSpec, mesh: Mesh, mode: str, precond: str = "auto", max_blocks: int = 0):
     """The request's work as a generator of solve jobs; returns (u, info)."""
     mat = Models.Elastic.Isotropic(2, E=spec.E, v=0.3, planeStress=True, thickness=1.0)
     simu = Simulations.Elastic(mesh, mat)
     simu.solver = "hip_fp64" if mode == "fp64" else "hip_mixed"
     simu.preconditioner = precond
+    simu.max_blocks = max_blocks
     left, right = _square_sides(mesh)
     simu.add_dirichlet(left, [0, 0], ["x", "y"])
     simu.add_surfLoad(right, [1.0], ["x"])
@@ -199,12 +200,14 @@ def serving_phasefield_specs(n: int = 64, steps: int = 50, ne_lo: int = 20000, n
     return out
 
 
-def phasefield_request(spec: PhaseFieldRequestSpec, mesh: Mesh, mode: str, clock: Callable[[], float]):
+def phasefield_request(spec: PhaseFieldRequestSpec, mesh: Mesh, mode: str, clock: Callable[[], float],
+                       max_blocks: int = 0):
     """``spec.steps`` increments of the staggered loop (Amor, AT2, one pass per
     increment) as a generator of solve jobs; records per-increment latency and
     the reaction force on the top edge."""
     Gc_e = lognormal(2.7, 0.15, spec.Gc[cell_of_elements(mesh, spec.cells)])
     simu = _sent_on(mesh, Gc_e, mode)
+    simu.max_blocks = max_blocks
     dofs = sent_top_y_dofs(simu)
     for n in range(1, spec.steps + 1):
         t = clock()

@@ -112,20 +112,22 @@ int efea_default_group(int dof_n, int max_neighbours);
  * the initial guess elsewhere; x is overwritten with the solution).
  * mode 64: FP64 matrix/preconditioner.  mode 32: FP32 matrix values (with the
  * efea_split_f32 correction, corr32, which may be NULL) and preconditioner in
- * the Krylov products, FP64 vectors, and the residual replaced
- * by the FP64 r = b - K64 x whenever it falls below replace_ratio times its
- * running maximum.  Stops when ||r|| <= tol_residual ||r0|| or when the delayed
+ * the Krylov products, FP64 vectors, and the recursive residual replaced by
+ * the FP64 r = b - K64 x at every poll (replace_ratio = 0), or whenever it
+ * falls below replace_ratio times its running maximum (replace_ratio > 0).  Stops when ||r|| <= tol_residual ||r0|| or when the delayed
  * A-norm error estimate <= tol_anorm (relative).  Synchronises `stream`.
  * info (host, 4 doubles): iterations, final true relative residual, residual
  * replacements, status (1 converged, 2 breakdown or non-finite values,
- * 3 diverged, 0 iteration limit).
+ * 3 diverged, 0 iteration limit).  max_blocks caps the blocks of each kernel's
+ * grid (<= 0: 1024); it fixes the reduction order, so solves agree bitwise
+ * only with the same cap.  Small caps suit many systems batched together.
  * Replaces __Solver_1 (Simulations/Solvers.py:502-553) with _Solve_Axb's
  * backends (Solvers.py:225-394; spsolve / pypardiso / cg). */
 int efea_solve_pcg(int mode, int Nnode, int dof_n, int group, const int32_t* node_ptr, const int32_t* node_cols,
                    const double* values64, const float* values32, const float* corr32, const uint8_t* known,
                    const void* Minv,
                    const double* b, double* x, void* workspace, size_t workspace_bytes, double tol_residual,
-                   double tol_anorm, double replace_ratio, int maxit, int check_every, void* stream,
+                   double tol_anorm, double replace_ratio, int maxit, int check_every, int max_blocks, void* stream,
                    double* info);
 
 /* Two-level preconditioner: M^-1 = B^-1 + P Ac^-1 P^T with B the nodal block
@@ -151,8 +153,8 @@ int efea_solve_pcg2(int mode, int Nnode, int dof_n, int group, const int32_t* no
                     const void* Minv, int nagg,
                     const int32_t* agg_ptr, const int32_t* agg_nodes, const double* rel, const double* Ainv, int ld,
                     const double* b, double* x, void* workspace, size_t workspace_bytes, double tol_residual,
-                    double tol_anorm, double replace_ratio, int maxit, int check_every, void* stream,
-                    double* info);
+                    double tol_anorm, double replace_ratio, int maxit, int check_every, int max_blocks,
+                    void* stream, double* info);
 
 /* Continuous batching of Dirichlet-reduced solves (the serving loop).
  * A batch holds up to `capacity` systems of one (mode, dof_n, group) class and
@@ -179,7 +181,7 @@ int efea_batch_add(void* handle, int Nnode, const int32_t* node_ptr, const int32
                    const void* Minv, int nagg,
                    const int32_t* agg_ptr, const int32_t* agg_nodes, const double* rel, const double* Ainv, int ld,
                    const double* b, double* x, void* workspace, size_t workspace_bytes, double tol_residual,
-                   double tol_anorm, double replace_ratio, int maxit, void* stream);
+                   double tol_anorm, double replace_ratio, int maxit, int max_blocks, void* stream);
 /* Runs `iterations` PCG iterations of every active system, then polls their
  * states (synchronises `stream`); returns how many systems finished. */
 int efea_batch_run(void* handle, int iterations, void* stream);

@@ -30,13 +30,16 @@ def _rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-@pytest.mark.parametrize("maker,dof_n,target", [
-    (lambda: ea.tri3_notched_square(24), 2, 48),
-    (lambda: ea.quad4_rectangle(3.0, 1.0, 30, 10), 2, 40),
-    (lambda: ea.hexa8_box(1.0, 1.0, 1.0, 6, 5, 4), 3, 40),
-    (lambda: ea.tri3_notched_square(24), 1, 48),
+@pytest.mark.parametrize("maker,dof_n,target,known_frac", [
+    (lambda: ea.tri3_notched_square(24), 2, 48, 0.1),
+    (lambda: ea.quad4_rectangle(3.0, 1.0, 30, 10), 2, 40, 0.1),
+    (lambda: ea.hexa8_box(1.0, 1.0, 1.0, 6, 5, 4), 3, 40, 0.1),
+    (lambda: ea.tri3_notched_square(24), 1, 48, 0.1),
+    # > 32 Gauss-Jordan blocks (the three-launch step with a pre-scaled row
+    # panel); small aggregates, so no known DOFs that could make modes dependent
+    (lambda: ea.quad4_rectangle(4.0, 2.0, 80, 40), 2, 8, 0.0),
 ])
-def test_coarse_matrix_and_inverse(maker, dof_n, target):
+def test_coarse_matrix_and_inverse(maker, dof_n, target, known_frac):
     mesh = maker()
     dev = DeviceMesh(mesh)
     A = dev.csr_map(dof_n)
@@ -48,7 +51,7 @@ def test_coarse_matrix_and_inverse(maker, dof_n, target):
     Ke = np.einsum("eij,ekj->eik", G, G) + nd * np.eye(nd)
     A.scatter(to_dev(Ke.ravel()))
     known = np.zeros(mesh.Nn * dof_n, dtype=np.uint8)
-    known[rng.choice(known.size, known.size // 10, replace=False)] = 1
+    known[rng.choice(known.size, int(known.size * known_frac), replace=False)] = 1
     cs = A.coarse(target)
     Ainv = cs.setup(A, to_dev(known))
     K = A.to_scipy()
@@ -58,6 +61,9 @@ def test_coarse_matrix_and_inverse(maker, dof_n, target):
     d0 = np.diag(Ac) == 0
     Ac[d0, d0] = 1.0
     X = Ainv.cpu().numpy().reshape(cs.ld, cs.ld)[:nc, :nc]
+    assert np.linalg.cond(Ac) < 1e10
+    if known_frac == 0.0:
+        assert cs.ld // 32 > 32, "case meant to exercise the three-launch Gauss-Jordan step"
     ref = np.linalg.inv(Ac)
     assert np.abs(X - ref).max() <= 1e-9 * np.abs(ref).max()
     assert np.array_equal(X, X.T)

@@ -148,3 +148,27 @@ def test_server_phasefield_realisations_match_alone():
         assert np.array_equal(np.array(spec.forces), f_ref)
         assert torch.equal(simu._d, d_ref) and torch.equal(simu._u, u_ref)
         assert len(spec.step_latency) == spec.steps
+
+
+def test_executor_queues_jobs_beyond_batch_capacity():
+    """A batch of capacity 2 fed 5 solves: the extra jobs wait and join as
+    slots free up; every result stays bitwise the solo one."""
+    specs = [_elastic_spec(200 + i, s) for i, s in enumerate([12, 18, 9, 15, 21])]
+    ref = [_alone(s, "mixed", "bj") for s in specs]
+    ex = BatchExecutor(capacity=2, check_every=32)
+    gens, results = {}, {}
+    for spec in specs:
+        g = workloads.elastic_request(spec, workloads.square_quad4(spec.side), "mixed", "bj")
+        job = next(g)
+        job.owner = spec.rid
+        gens[spec.rid] = g
+        ex.submit(job)
+    assert ex.active() == 5
+    while ex.active():
+        assert max(len(b) for b in ex.batches.values()) <= 2
+        for job, info in ex.step():
+            _finish(gens, results, job, info)
+    ex.close()
+    for spec, (u_ref, info_ref) in zip(specs, ref):
+        u, info = results[spec.rid]
+        assert info.iterations == info_ref.iterations and torch.equal(u, u_ref)

@@ -38,7 +38,7 @@ def test_abi_argument_errors_are_reported_without_gpu():
     assert rc == -1
     assert "negative size" in _lib.last_error()
     rc = lib.efea_solve_pcg(64, 0, 2, 16, None, None, None, None, None, None, None, None, None, None, 0, 1e-12,
-                            1e-9, 1e-3, 10, 1, None, None)
+                            1e-9, 1e-3, 10, 1, 0, None, None)
     assert rc == -1
     rc = lib.efea_pf_split_points(4, 7, None, None, 0, 0.3, 0, None, None, None, None, None)
     assert rc == -1 and "null pointer" in _lib.last_error()
@@ -49,11 +49,11 @@ def test_abi_argument_errors_are_reported_without_gpu():
     assert h.value is None
     assert lib.efea_batch_run(None, 32, None) == -1
     assert lib.efea_batch_add(None, 10, None, None, None, None, None, None, None, 0, None, None, None, None, 0,
-                              None, None, None, 0, 1e-12, 1e-9, 1e-3, 10, None) == -1
+                              None, None, None, 0, 1e-12, 1e-9, 1e-3, 10, 0, None) == -1
     assert lib.efea_coarse_ld(0, 2) == -1
     assert lib.efea_coarse_ld(100, 2) == 320 and lib.efea_coarse_ld(100, 3) == 608 and lib.efea_coarse_ld(5, 1) == 32
     assert lib.efea_solve_pcg2(64, 10, 2, 16, None, None, None, None, None, None, None, 0, None, None, None, None,
-                               0, None, None, None, 0, 1e-12, 1e-9, 1e-3, 10, 32, None, None) == -1
+                               0, None, None, None, 0, 1e-12, 1e-9, 1e-3, 10, 32, 0, None, None) == -1
     assert lib.efea_split_f32(-1, 2, None, None, None, None, None) == -1
     jid = C.c_int(5)
     assert lib.efea_batch_pop(None, C.byref(jid), None) == -1
