@@ -33,13 +33,14 @@ E, NU, GC, L0 = 210000.0, 0.3, 2.7, 0.01
 DEP = 1e-4
 
 
-def make_sim(mesh, solver):
+def make_sim(mesh, solver, coarse_reuse=True):
     from easyfea_amd import Models, Simulations
 
     mat = Models.Elastic.Isotropic(2, E, NU, planeStress=False, thickness=1.0)
     pfm = Models.PhaseField(mat, "Miehe", "AT2", GC, L0)
     simu = Simulations.PhaseField(mesh, pfm)
     simu.solver = solver
+    simu.coarse_reuse = coarse_reuse
     # one-time setup outside the timed region: sparsity patterns and scatter maps
     simu.dev.csr_map(2)
     simu.dev.csr_map(1)
@@ -49,15 +50,15 @@ def make_sim(mesh, solver):
     return simu
 
 
-def run_history(mesh, solver, steps, keep_states=()):
+def run_history(mesh, solver, steps, keep_states=(), coarse_reuse=True):
     import torch
 
-    simu = make_sim(mesh, solver)
+    simu = make_sim(mesh, solver, coarse_reuse)
     top = mesh.Nodes_Conditions(lambda x, y, z: y == 1.0)
     bot = mesh.Nodes_Conditions(lambda x, y, z: y == 0.0)
     dofsY = simu.Bc_dofs_nodes(top, ["y"])
     lat, forces, its_u, its_d, states = [], [], [], [], {}
-    fallbacks = 0
+    fallbacks = rebuilds = 0
     verbose = bool(os.environ.get("EFEA_BENCH_VERBOSE"))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -73,6 +74,8 @@ def run_history(mesh, solver, steps, keep_states=()):
         its_u.append(simu.last_solve["elastic"].iterations)
         its_d.append(simu.last_solve["damage"].iterations)
         fallbacks += int(simu.last_solve["elastic"].fallback) + int(simu.last_solve["damage"].fallback)
+        rebuilds += int(simu.last_solve["elastic"].precond == "two-level" and
+                        not simu.last_solve["elastic"].coarse_reused)
         if verbose:
             print(f"# {solver} step {it}: {lat[-1] * 1e3:.1f} ms, u {simu.last_solve['elastic']}, "
                   f"d {simu.last_solve['damage']}, f {forces[-1]:.4f}", file=sys.stderr, flush=True)
@@ -81,7 +84,8 @@ def run_history(mesh, solver, steps, keep_states=()):
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
     return dict(simu=simu, lat=np.array(lat), total=total, forces=np.array(forces), u=u, d=d,
-                its_u=np.array(its_u), its_d=np.array(its_d), states=states, fallbacks=fallbacks)
+                its_u=np.array(its_u), its_d=np.array(its_d), states=states, fallbacks=fallbacks,
+                coarse_rebuilds=rebuilds)
 
 
 def cpu_baseline(mesh, steps):
@@ -272,6 +276,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=500)
+    ap.add_argument("--no-coarse-reuse", action="store_true",
+                    help="rebuild the coarse inverse at every displacement solve")
     ap.add_argument("--workload", default="sent",
                     choices=["sent", "cantilever", "cube", "serve-elastic", "serve-phasefield"],
                     help="sent: the phase-field history above (the headline); the others print the "
@@ -306,11 +312,12 @@ def main():
     print(f"# warmup done ({args.warmup} increments)", file=sys.stderr, flush=True)
 
     keep = tuple(range(1, min(args.cpu_steps, args.steps) + 1)) + (args.steps,)
-    mixed = run_history(mesh, "hip_mixed", args.steps, keep)
+    reuse = not args.no_coarse_reuse
+    mixed = run_history(mesh, "hip_mixed", args.steps, keep, reuse)
     print(f"# mixed history: {mixed['total']:.3f} s", file=sys.stderr, flush=True)
     plain = None
     if not args.no_plain:
-        plain = run_history(mesh, "hip_fp64", args.steps, keep)
+        plain = run_history(mesh, "hip_fp64", args.steps, keep, reuse)
         print(f"# plain FP64 history: {plain['total']:.3f} s", file=sys.stderr, flush=True)
 
     dofs_per_step = ndof_u + ndof_d  # one staggered pass: one displacement + one damage solve
@@ -371,14 +378,18 @@ def main():
             "elements": int(mesh.Ne),
             "ndof_u": int(ndof_u),
             "ndof_d": int(ndof_d),
-            "solver": "block-Jacobi PCG; mixed mode keeps FP64 Krylov products below 128 MiB of matrix "
-                      "(latency-bound), FP32 values + FP64 residual replacement above",
+            "solver": "displacement: two-level PCG (nodal block Jacobi + rigid-body aggregation coarse space"
+                      + (", coarse inverse kept over increments until the iteration count grows 20%"
+                         if reuse else "") + "); damage: block-Jacobi PCG; mixed mode keeps FP64 Krylov "
+                      "products below 128 MiB of matrix (latency-bound), FP32 values + FP64 residual "
+                      "replacement above",
         },
         "p50_step_ms": float(np.percentile(lat_ms, 50)),
         "p99_step_ms": float(np.percentile(lat_ms, 99)),
         "pcg_iters_u_mean": float(mixed["its_u"].mean()),
         "pcg_iters_d_mean": float(mixed["its_d"].mean()),
         "fp64_fallbacks": int(mixed["fallbacks"]),
+        "coarse_rebuilds": int(mixed["coarse_rebuilds"]),
         "final_force_N": float(mixed["forces"][-1]),
         "accuracy": accuracy,
         "roofline": {k: roof[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},

@@ -85,6 +85,11 @@ class _Simu:
         self.check_every = 32
         self.preconditioner = "auto"  # "bj" | "two-level" | "auto"
         self.max_blocks = 0  # grid cap of the solver kernels (0: the library's default)
+        # keep the two-level coarse inverse over successive solves of one matrix
+        # until their iteration count grows (device.COARSE_REUSE_GROWTH)
+        self.coarse_reuse = True
+        # read the solver state back first near the previous solve's iteration count
+        self.predict_polls = True
         self.last_solve = {}
         self.Bc_Init()
         model._Add_observer(self)
@@ -304,18 +309,21 @@ class _Simu:
         """The solve of ``problemType``: Dirichlet values written into
         ``x_dev`` at the known DOFs, the known mask on the device."""
         known_np, xk = self._dirichlet_state(problemType)
-        key = (problemType, hash(known_np.tobytes()))
+        # one device mask per problem type, kept while the Dirichlet DOFs stay the
+        # same (the tensor's identity then also keys the lagged coarse inverse)
+        key = hash(known_np.tobytes())
         cache = self.__dict__.setdefault("_mask_cache", {})
-        if key not in cache:
+        if cache.get(problemType, (None,))[0] != key:
             idx = np.nonzero(known_np)[0]
-            cache.clear()
-            cache[key] = (to_dev(known_np), to_dev(idx), idx)
-        known, idx_dev, idx = cache[key]
+            cache[problemType] = (key, to_dev(known_np), to_dev(idx), idx)
+        _, known, idx_dev, idx = cache[problemType]
         if idx.size:
             x_dev.index_copy_(0, idx_dev, to_dev(xk[idx]))
+        prev = self.last_solve.get(problemType)
+        expected = prev.iterations if self.predict_polls and prev is not None and prev.status == 1 else None
         return SolveJob(A, b_dev, x_dev, known, self._mode(), self.tol_residual, self.tol_anorm,
                         self.replace_ratio, self.maxit, self.check_every, owner=self, precond=self.preconditioner,
-                        max_blocks=self.max_blocks)
+                        max_blocks=self.max_blocks, expected_iterations=expected)
 
     def _solve_system_gen(self, A, b_dev: torch.Tensor, x_dev: torch.Tensor, problemType):
         """Generator form of a solve: yields the ``SolveJob`` and receives its
@@ -323,6 +331,7 @@ class _Simu:
         job = self._make_job(A, b_dev, x_dev, problemType)
         t = time.perf_counter()
         info = yield job
+        job.record(info)
         self.last_solve[problemType] = info
         if info.status != 1:
             raise _lib.EfeaError("solve", _lib.ERR_NOCONV,

@@ -34,7 +34,7 @@ SIGNATURES = {
     "efea_default_group": (i32, [i32, i32]),
     "efea_solve_pcg": (
         i32,
-        [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, f64, f64, f64, i32, i32, i32, vp, vp],
+        [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, f64, f64, f64, i32, i32, i32, i32, vp, vp],
     ),
     "efea_coarse_ld": (i32, [i32, i32]),
     "efea_coarse_setup": (i32, [i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp]),
@@ -42,7 +42,7 @@ SIGNATURES = {
     "efea_solve_pcg2": (
         i32,
         [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, i32, vp, vp, vp, sz, f64, f64, f64,
-         i32, i32, i32, vp, vp],
+         i32, i32, i32, i32, vp, vp],
     ),
     "efea_batch_create": (i32, [i32, i32, i32, i32, i32, C.POINTER(vp)]),
     "efea_batch_destroy": (i32, [vp]),

@@ -153,8 +153,10 @@ static void batch_free(Batch* b) {
 }
 
 static bool supported(int dof_n, int group) {
-  return (dof_n == 1 && (group == 8 || group == 16)) || (dof_n == 2 && (group == 8 || group == 16)) ||
-         (dof_n == 3 && (group == 16 || group == 32));
+#define SUP(D, GV) if (dof_n == D && group == GV) return true;
+  EFEA_GROUPS(SUP)
+#undef SUP
+  return false;
 }
 
 static size_t map_size(int cap) { return 2 * (size_t)cap * MAXB + (size_t)cap * MAXA; }
@@ -272,7 +274,7 @@ int batch_add(void* h, int Nnode, const int* node_ptr, const int* node_cols, con
   if (B->dof_n == D && B->group == GV)                                                                    \
     rc = B->mode == 64 ? start_dispatch<double, D, GV>(d.A64, Minv, Cp, b, x, v, tol_r, tol_A, delta2, s) \
                        : start_dispatch<float, D, GV>(d.A64, Minv, Cp, b, x, v, tol_r, tol_A, delta2, s);
-  ST(1, 8) ST(1, 16) ST(2, 8) ST(2, 16) ST(3, 16) ST(3, 32)
+  EFEA_GROUPS(ST)
 #undef ST
   if (rc) return rc;
   B->active.push_back(e);
@@ -420,7 +422,7 @@ int batch_run(void* h, int iters, hipStream_t s) {
   if (B->dof_n == D && B->group == GV)                                                       \
     return B->mode == 64 ? batch_run_t<double, double, D, GV>(B, iters, s)                   \
                          : batch_run_t<float, float, D, GV>(B, iters, s);
-  RN(1, 8) RN(1, 16) RN(2, 8) RN(2, 16) RN(3, 16) RN(3, 32)
+  EFEA_GROUPS(RN)
 #undef RN
   set_error("batch_run: unsupported configuration");
   return EFEA_ERR_ARG;

@@ -43,7 +43,7 @@ int bj_setup(int, int, int, const int*, const int*, const double*, const unsigne
 size_t pcg_workspace_bytes(int, int, int);
 int solve_pcg(int, int, int, int, const int*, const int*, const double*, const float*, const float*,
               const unsigned char*, const void*, const double*, double*, void*, double, double, double, int, int,
-              int, hipStream_t, double*);
+              int, int, hipStream_t, double*);
 int split_f32(int, int, const int*, const double*, float*, float*, hipStream_t);
 int dot(int64_t, const double*, const double*, double*, double*, hipStream_t);
 int gather_sum(int64_t, const int*, const double*, double*, hipStream_t);
@@ -69,7 +69,7 @@ int coarse_setup(int, int, const int*, const int*, const int*, const int*, const
 size_t pcg2_workspace_bytes(int, int, int);
 int solve_pcg2(int, int, int, int, const int*, const int*, const double*, const float*, const float*,
                const unsigned char*, const void*, int, const int*, const int*, const double*, const double*, int,
-               const double*, double*, void*, double, double, double, int, int, int, hipStream_t, double*);
+               const double*, double*, void*, double, double, double, int, int, int, int, hipStream_t, double*);
 
 }  // namespace efea
 
@@ -183,8 +183,8 @@ int efea_solve_pcg(int mode, int Nnode, int dof_n, int group, const int32_t* nod
                    const double* values64, const float* values32, const float* corr32, const uint8_t* known,
                    const void* Minv,
                    const double* b, double* x, void* workspace, size_t workspace_bytes, double tol_residual,
-                   double tol_anorm, double replace_ratio, int maxit, int check_every, int max_blocks, void* stream,
-                   double* info) {
+                   double tol_anorm, double replace_ratio, int maxit, int check_every, int first_check,
+                   int max_blocks, void* stream, double* info) {
   if (Nnode <= 0 || !node_ptr || !node_cols || !values64 || !Minv || !b || !x || !workspace || !info) {
     set_error("efea_solve_pcg: null pointer or empty system");
     return EFEA_ERR_ARG;
@@ -195,8 +195,8 @@ int efea_solve_pcg(int mode, int Nnode, int dof_n, int group, const int32_t* nod
     return EFEA_ERR_SIZE;
   }
   return solve_pcg(mode, Nnode, dof_n, group, node_ptr, node_cols, values64, values32, corr32, known, Minv, b, x,
-                   workspace, tol_residual, tol_anorm, replace_ratio, maxit, check_every, max_blocks, STREAM(stream),
-                   info);
+                   workspace, tol_residual, tol_anorm, replace_ratio, maxit, check_every, first_check, max_blocks,
+                   STREAM(stream), info);
 }
 
 int efea_batch_create(int mode, int dof_n, int group, int two_level, int capacity, void** handle) {
@@ -286,8 +286,8 @@ int efea_solve_pcg2(int mode, int Nnode, int dof_n, int group, const int32_t* no
                     const void* Minv, int nagg,
                     const int32_t* agg_ptr, const int32_t* agg_nodes, const double* rel, const double* Ainv, int ld,
                     const double* b, double* x, void* workspace, size_t workspace_bytes, double tol_residual,
-                    double tol_anorm, double replace_ratio, int maxit, int check_every, int max_blocks,
-                    void* stream, double* info) {
+                    double tol_anorm, double replace_ratio, int maxit, int check_every, int first_check,
+                    int max_blocks, void* stream, double* info) {
   if (Nnode <= 0 || nagg <= 0 || !node_ptr || !node_cols || !values64 || !Minv || !agg_ptr || !agg_nodes || !rel ||
       !Ainv || !b || !x || !workspace || !info) {
     set_error("efea_solve_pcg2: null pointer or empty system");
@@ -305,7 +305,7 @@ int efea_solve_pcg2(int mode, int Nnode, int dof_n, int group, const int32_t* no
   return solve_pcg2(mode, Nnode, dof_n, group, node_ptr, node_cols, values64, values32, corr32, known, Minv, nagg,
                     agg_ptr,
                     agg_nodes, rel, Ainv, ld, b, x, workspace, tol_residual, tol_anorm, replace_ratio, maxit,
-                    check_every, max_blocks, STREAM(stream), info);
+                    check_every, first_check, max_blocks, STREAM(stream), info);
 }
 
 int efea_dot(int64_t n, const double* a, const double* b, double* scratch, double* out, void* stream) {

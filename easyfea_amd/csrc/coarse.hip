@@ -401,8 +401,8 @@ k_xax_residual(SpmvArgs A, const double* __restrict__ x, const double* __restric
   for (int64_t gid = (int64_t)blockIdx.x * BS + threadIdx.x; gid < lanes; gid += (int64_t)gridDim.x * BS) {
     const int n = (int)(gid / G), g = (int)(gid % G);
     double dd, d2;
-    spmv_node<double, double, DN, G, 0>(A, n, g, x, nullptr, y, dd);
-    spmv_node<double, double, DN, G, 1>(A, n, g, x, b, r, d2);
+    spmv_node<double, DN, G, 0>(A, n, g, x, nullptr, y, dd);
+    spmv_node<double, DN, G, 1>(A, n, g, x, b, r, d2);
     d += dd;
   }
   d = block_sum<double, BS>(d, sh);
@@ -436,27 +436,18 @@ int pcg2_start(const SpmvArgs& A64, const TP* Minv, const CoarseArgs& C, const d
   return check_launch("pcg2_init");
 }
 
-#define EFEA_PCG2_START(TP, DN, G)                                                                        \
-  template int pcg2_start<TP, DN, G>(const SpmvArgs&, const TP*, const CoarseArgs&, const double*,         \
-                                     const double*, const Pcg2Work&, double, double, double, hipStream_t);
-EFEA_PCG2_START(double, 1, 8)
-EFEA_PCG2_START(double, 1, 16)
-EFEA_PCG2_START(double, 2, 8)
-EFEA_PCG2_START(double, 2, 16)
-EFEA_PCG2_START(double, 3, 16)
-EFEA_PCG2_START(double, 3, 32)
-EFEA_PCG2_START(float, 1, 8)
-EFEA_PCG2_START(float, 1, 16)
-EFEA_PCG2_START(float, 2, 8)
-EFEA_PCG2_START(float, 2, 16)
-EFEA_PCG2_START(float, 3, 16)
-EFEA_PCG2_START(float, 3, 32)
+#define EFEA_PCG2_START(DN, G)                                                                             \
+  template int pcg2_start<double, DN, G>(const SpmvArgs&, const double*, const CoarseArgs&, const double*,   \
+                                         const double*, const Pcg2Work&, double, double, double, hipStream_t); \
+  template int pcg2_start<float, DN, G>(const SpmvArgs&, const float*, const CoarseArgs&, const double*,     \
+                                        const double*, const Pcg2Work&, double, double, double, hipStream_t);
+EFEA_GROUPS(EFEA_PCG2_START)
 #undef EFEA_PCG2_START
 
 template <typename TM, typename TP, int DN, int G>
 static int pcg2_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, const CoarseArgs& C, const double* b,
                     double* x, void* ws, double tol_r, double tol_A, double delta, int maxit, int check_every,
-                    hipStream_t s, double* info) {
+                    int first_check, hipStream_t s, double* info) {
   const int Nnode = A64.Nnode;
   const Pcg2Work v = carve2(ws, Nnode, DN, C.nagg);
   const PcgWork& w = v.w;
@@ -473,7 +464,8 @@ static int pcg2_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, con
   }
   int launched = 0;
   while (true) {
-    for (int k = 0; k < check_every; ++k) {
+    const int chunk = launched == 0 && first_check > 0 ? first_check : check_every;
+    for (int k = 0; k < chunk; ++k) {
       hipLaunchKernelGGL((k_pcg2_step1<TM, DN, G>), dim3(nbS), dim3(BS), 0, s, AK, (const double*)w.z,
                          (const double*)w.P0, (const double*)w.P1, w.q, (const double*)w.part2, (int)nbA, w.part1,
                          w.st);
@@ -482,7 +474,7 @@ static int pcg2_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, con
                          (int)nbS, w.part2, w.st);
       hipLaunchKernelGGL((k_pcg2_prolong<DN>), dim3(nbA), dim3(BS), 0, s, Cw, A64.known, w.z, w.part2, w.st);
     }
-    launched += check_every;
+    launched += chunk;
     if (int e = check_launch("pcg2_iter")) return e;
     if (hipMemcpyAsync(hs, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
@@ -516,7 +508,7 @@ int solve_pcg2(int mode, int Nnode, int dof_n, int group, const int* node_ptr, c
                const void* Minv, int nagg,
                const int* agg_ptr, const int* agg_nodes, const double* rel, const double* Ainv, int ld,
                const double* b, double* x, void* ws, double tol_r, double tol_A, double delta, int maxit,
-               int check_every, int max_blocks, hipStream_t s, double* info) {
+               int check_every, int first_check, int max_blocks, hipStream_t s, double* info) {
   SpmvArgs A64{Nnode, node_ptr, node_cols, vals64, known, nullptr, max_blocks};
   SpmvArgs A32{Nnode, node_ptr, node_cols, vals32, known, corr32, max_blocks};
   CoarseArgs C{nagg, agg_ptr, agg_nodes, rel, Ainv, ld, nullptr};
@@ -525,12 +517,12 @@ int solve_pcg2(int mode, int Nnode, int dof_n, int group, const int* node_ptr, c
   if (dof_n == D && group == GV) {                                                                            \
     if (mode == 64)                                                                                           \
       return pcg2_run<double, double, D, GV>(A64, A64, (const double*)Minv, C, b, x, ws, tol_r, tol_A, delta, \
-                                             maxit, check_every, s, info);                                    \
+                                             maxit, check_every, first_check, s, info);                       \
     return pcg2_run<float, float, D, GV>(A64, A32, (const float*)Minv, C, b, x, ws, tol_r, tol_A, delta,      \
-                                         maxit, check_every, s, info);                                        \
+                                         maxit, check_every, first_check, s, info);                           \
   }
   if (mode == 64 || (mode == 32 && vals32)) {
-    RUN2(1, 8) RUN2(1, 16) RUN2(2, 8) RUN2(2, 16) RUN2(3, 16) RUN2(3, 32)
+    EFEA_GROUPS(RUN2)
   }
 #undef RUN2
   set_error("solve_pcg2: unsupported mode %d / dof_n %d / group %d", mode, dof_n, group);

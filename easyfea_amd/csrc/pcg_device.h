@@ -34,45 +34,85 @@ struct SpmvArgs {
   int cap = 0;
 };
 
+// Row-block products of one node against NV vectors.  Lane g of the node's
+// G lanes takes positions g, g+G, g+2G, ... (its summation order) and
+// accumulates acc[k][a] += K(n, a, pos, b) * X_k(col(pos), b).  The
+// positions are visited U at a time with every load of a pass issued before
+// the first use (indices past cnt are clamped to the last position and their
+// products dropped), so one pass costs one column-index -> vector latency
+// chain instead of one per position.  load(c, xv) fills xv[k][b] = X_k(c, b).
+template <int G>
+struct GatherUnroll {
+  static constexpr int U = G <= 4 ? 3 : 2;
+};
+
+template <typename TM, int DN, int G, int NV, typename Load>
+__device__ __forceinline__ void gather_rows(const TM* __restrict__ vals, const int* __restrict__ cols, int p0,
+                                            int cnt, int g, const Load& load, double (&acc)[NV][DN]) {
+  constexpr int U = GatherUnroll<G>::U;
+  const int64_t base = (int64_t)DN * DN * p0;
+  for (int pos0 = g; pos0 < cnt; pos0 += U * G) {
+    int pc[U], c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      pc[u] = min(pos0 + u * G, cnt - 1);
+      c[u] = cols[p0 + pc[u]];
+    }
+    double v[U][DN][DN];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int a = 0; a < DN; ++a)
+#pragma unroll
+        for (int b = 0; b < DN; ++b) v[u][a][b] = (double)vals[base + (int64_t)a * DN * cnt + (int64_t)pc[u] * DN + b];
+    double xv[U][NV][DN];
+#pragma unroll
+    for (int u = 0; u < U; ++u) load(c[u], xv[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (pos0 + u * G < cnt) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+#pragma unroll
+          for (int a = 0; a < DN; ++a)
+#pragma unroll
+            for (int b = 0; b < DN; ++b) acc[k][a] += v[u][a][b] * xv[u][k][b];
+      }
+    }
+  }
+}
+
 // Node-block SpMV: G lanes cooperate on one node's dof_n rows.
 //   MODE 0: y = A x            (rows of known DOFs -> 0 when known != null)
 //   MODE 1: y = b - A x        (same masking)
 // dot_xy receives x.y (MODE 0) or y.y (MODE 1) over the node's rows.
-template <typename TM, typename TV, int DN, int G, int MODE>
-__device__ inline void spmv_node(const SpmvArgs& A, int n, int g, const TV* __restrict__ x,
-                                 const TV* __restrict__ b, TV* __restrict__ y, double& dot_xy) {
-  const TM* __restrict__ vals = (const TM*)A.vals;
+template <typename TM, int DN, int G, int MODE>
+__device__ inline void spmv_node(const SpmvArgs& A, int n, int g, const double* __restrict__ x,
+                                 const double* __restrict__ b, double* __restrict__ y, double& dot_xy) {
   const int p0 = A.node_ptr[n], cnt = A.node_ptr[n + 1] - p0;
-  const int64_t base = (int64_t)DN * DN * p0;
-  TV sum[DN];
+  double sum[1][DN];
 #pragma unroll
-  for (int a = 0; a < DN; ++a) sum[a] = 0;
-  for (int pos = g; pos < cnt; pos += G) {
-    const int c = A.node_cols[p0 + pos];
-    TV xc[DN];
+  for (int a = 0; a < DN; ++a) sum[0][a] = 0;
+  gather_rows<TM, DN, G, 1>((const TM*)A.vals, A.node_cols, p0, cnt, g,
+                            [&](int c, double (&xv)[1][DN]) {
 #pragma unroll
-    for (int bb = 0; bb < DN; ++bb) xc[bb] = x[(int64_t)c * DN + bb];
-#pragma unroll
-    for (int a = 0; a < DN; ++a) {
-      const TM* v = vals + base + (int64_t)a * DN * cnt + (int64_t)pos * DN;
-#pragma unroll
-      for (int bb = 0; bb < DN; ++bb) sum[a] += (TV)v[bb] * xc[bb];
-    }
-  }
+                              for (int bb = 0; bb < DN; ++bb) xv[0][bb] = x[(int64_t)c * DN + bb];
+                            },
+                            sum);
 #pragma unroll
   for (int a = 0; a < DN; ++a) {
 #pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) sum[a] += __shfl_xor(sum[a], o, G);
+    for (int o = G / 2; o > 0; o >>= 1) sum[0][a] += __shfl_xor(sum[0][a], o, G);
   }
   dot_xy = 0;
   if (g == 0) {
 #pragma unroll
     for (int a = 0; a < DN; ++a) {
       const int64_t r = (int64_t)n * DN + a;
-      TV val = MODE == 1 ? b[r] - sum[a] : sum[a];
+      double val = MODE == 1 ? b[r] - sum[0][a] : sum[0][a];
       if (A.known && A.known[r]) val = 0;
       y[r] = val;
-      dot_xy += (double)(MODE == 1 ? val : x[r]) * (double)val;
+      dot_xy += (MODE == 1 ? val : x[r]) * val;
     }
   }
 }
@@ -90,6 +130,10 @@ __device__ inline void spmv_node(const SpmvArgs& A, int n, int g, const TV* __re
 // boundary: one release per dispatch instead of one per block.
 constexpr int BS = 256;
 constexpr int MAXB = 1024;
+
+// Supported (dof_n, lanes per node) pairs of the node-group kernels; every
+// dispatch site expands this list.
+#define EFEA_GROUPS(X) X(1, 4) X(1, 8) X(1, 16) X(2, 4) X(2, 8) X(2, 16) X(3, 8) X(3, 16) X(3, 32)
 
 static inline unsigned grid_for(int64_t threads, int cap = 0) {
   const int64_t mx = (cap > 0 && cap < MAXB) ? cap : MAXB;
@@ -207,25 +251,22 @@ __device__ inline void pcg_step1_body(const SpmvArgs& A, const double* __restric
   for (int a = 0; a < DN; ++a) sz[a] = sp[a] = 0;
   if (have) {
     const int p0 = A.node_ptr[n0], cnt = A.node_ptr[n0 + 1] - p0;
-    const int64_t base = (int64_t)DN * DN * p0;
-    for (int pos = g0; pos < cnt; pos += G) {
-      const int c = A.node_cols[p0 + pos];
-      double zc[DN], pc[DN];
+    double acc[2][DN];
 #pragma unroll
-      for (int bb = 0; bb < DN; ++bb) {
-        zc[bb] = z[(int64_t)c * DN + bb];
-        pc[bb] = pold[(int64_t)c * DN + bb];
-      }
+    for (int a = 0; a < DN; ++a) acc[0][a] = acc[1][a] = 0;
+    gather_rows<TM, DN, G, 2>(vals, A.node_cols, p0, cnt, g0,
+                              [&](int c, double (&xv)[2][DN]) {
 #pragma unroll
-      for (int a = 0; a < DN; ++a) {
-        const TM* v = vals + base + (int64_t)a * DN * cnt + (int64_t)pos * DN;
+                                for (int bb = 0; bb < DN; ++bb) {
+                                  xv[0][bb] = z[(int64_t)c * DN + bb];
+                                  xv[1][bb] = pold[(int64_t)c * DN + bb];
+                                }
+                              },
+                              acc);
 #pragma unroll
-        for (int bb = 0; bb < DN; ++bb) {
-          const double vv = (double)v[bb];
-          sz[a] += vv * zc[bb];
-          sp[a] += vv * pc[bb];
-        }
-      }
+    for (int a = 0; a < DN; ++a) {
+      sz[a] = acc[0][a];
+      sp[a] = acc[1][a];
     }
   }
   double beta = 0;
@@ -277,29 +318,36 @@ __device__ inline void pcg_step1_body(const SpmvArgs& A, const double* __restric
       }
     }
   }
-  for (gid += (int64_t)nblk * BS; gid < lanes; gid += (int64_t)nblk * BS) {
+  // later nodes of this block: p is known, one product; the next node's
+  // offsets are loaded before this node's gather
+  const int64_t stride = (int64_t)nblk * BS;
+  gid += stride;
+  int np0 = 0, np1 = 0;
+  if (gid < lanes) {
+    np0 = A.node_ptr[gid / G];
+    np1 = A.node_ptr[gid / G + 1];
+  }
+  for (; gid < lanes; gid += stride) {
     const int n = (int)(gid / G), g = (int)(gid % G);
-    const int p0 = A.node_ptr[n], cnt = A.node_ptr[n + 1] - p0;
-    const int64_t base = (int64_t)DN * DN * p0;
-    double sum[DN];
-#pragma unroll
-    for (int a = 0; a < DN; ++a) sum[a] = 0;
-    for (int pos = g; pos < cnt; pos += G) {
-      const int c = A.node_cols[p0 + pos];
-      double xc[DN];
-#pragma unroll
-      for (int bb = 0; bb < DN; ++bb) xc[bb] = z[(int64_t)c * DN + bb] + beta * pold[(int64_t)c * DN + bb];
-#pragma unroll
-      for (int a = 0; a < DN; ++a) {
-        const TM* v = vals + base + (int64_t)a * DN * cnt + (int64_t)pos * DN;
-#pragma unroll
-        for (int bb = 0; bb < DN; ++bb) sum[a] += (double)v[bb] * xc[bb];
-      }
+    const int p0 = np0, cnt = np1 - np0;
+    if (gid + stride < lanes) {
+      np0 = A.node_ptr[(gid + stride) / G];
+      np1 = A.node_ptr[(gid + stride) / G + 1];
     }
+    double sum[1][DN];
+#pragma unroll
+    for (int a = 0; a < DN; ++a) sum[0][a] = 0;
+    gather_rows<TM, DN, G, 1>(vals, A.node_cols, p0, cnt, g,
+                              [&](int c, double (&xv)[1][DN]) {
+#pragma unroll
+                                for (int bb = 0; bb < DN; ++bb)
+                                  xv[0][bb] = z[(int64_t)c * DN + bb] + beta * pold[(int64_t)c * DN + bb];
+                              },
+                              sum);
 #pragma unroll
     for (int a = 0; a < DN; ++a) {
 #pragma unroll
-      for (int o = G / 2; o > 0; o >>= 1) sum[a] += __shfl_xor(sum[a], o, G);
+      for (int o = G / 2; o > 0; o >>= 1) sum[0][a] += __shfl_xor(sum[0][a], o, G);
     }
     if (g == 0) {
       double pn[DN];
@@ -309,7 +357,7 @@ __device__ inline void pcg_step1_body(const SpmvArgs& A, const double* __restric
       for (int a = 0; a < DN; ++a) {
         const int64_t r = (int64_t)n * DN + a;
         pnew[r] = pn[a];
-        double qv = sum[a];
+        double qv = sum[0][a];
         if (A.corr) {
           const TM* cr = (const TM*)A.corr + (int64_t)n * DN * DN + a * DN;
 #pragma unroll
@@ -431,7 +479,7 @@ __device__ inline void residual_body(const SpmvArgs& A64, const double* __restri
   for (int64_t gid = (int64_t)blk * BS + threadIdx.x; gid < lanes; gid += (int64_t)nblk * BS) {
     const int n = (int)(gid / G), g = (int)(gid % G);
     double dd;
-    spmv_node<double, double, DN, G, 1>(A64, n, g, x, b, r, dd);
+    spmv_node<double, DN, G, 1>(A64, n, g, x, b, r, dd);
   }
 }
 

@@ -178,7 +178,7 @@ k_residual(SpmvArgs A, const double* __restrict__ x, const double* __restrict__ 
   for (int64_t gid = (int64_t)blockIdx.x * BS + threadIdx.x; gid < lanes; gid += (int64_t)gridDim.x * BS) {
     const int n = (int)(gid / G), g = (int)(gid % G);
     double dd;
-    spmv_node<double, double, DN, G, 1>(A, n, g, x, b, r, dd);
+    spmv_node<double, DN, G, 1>(A, n, g, x, b, r, dd);
     d += dd;
   }
   d = block_sum<double, BS>(d, sh);
@@ -195,7 +195,7 @@ k_xax(SpmvArgs A, const double* __restrict__ x, double* __restrict__ y, double* 
   for (int64_t gid = (int64_t)blockIdx.x * BS + threadIdx.x; gid < lanes; gid += (int64_t)gridDim.x * BS) {
     const int n = (int)(gid / G), g = (int)(gid % G);
     double dd;
-    spmv_node<double, double, DN, G, 0>(A, n, g, x, nullptr, y, dd);
+    spmv_node<double, DN, G, 0>(A, n, g, x, nullptr, y, dd);
     d += dd;
   }
   d = block_sum<double, BS>(d, sh);
@@ -383,28 +383,15 @@ void pcg_true_residual(const SpmvArgs& A64, const double* b, const double* x, co
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(BS), 0, s, (const double*)w.part1, (int)nbS, w.small);
 }
 
-#define EFEA_PCG_HELPERS(TP, DN, G)                                                                         \
-  template int pcg_start<TP, DN, G>(const SpmvArgs&, const TP*, const double*, const double*, const PcgWork&, \
-                                    double, double, double, hipStream_t);
-EFEA_PCG_HELPERS(double, 1, 8)
-EFEA_PCG_HELPERS(double, 1, 16)
-EFEA_PCG_HELPERS(double, 2, 8)
-EFEA_PCG_HELPERS(double, 2, 16)
-EFEA_PCG_HELPERS(double, 3, 16)
-EFEA_PCG_HELPERS(double, 3, 32)
-EFEA_PCG_HELPERS(float, 1, 8)
-EFEA_PCG_HELPERS(float, 1, 16)
-EFEA_PCG_HELPERS(float, 2, 8)
-EFEA_PCG_HELPERS(float, 2, 16)
-EFEA_PCG_HELPERS(float, 3, 16)
-EFEA_PCG_HELPERS(float, 3, 32)
-#undef EFEA_PCG_HELPERS
-template void pcg_true_residual<1, 8>(const SpmvArgs&, const double*, const double*, const PcgWork&, hipStream_t);
-template void pcg_true_residual<1, 16>(const SpmvArgs&, const double*, const double*, const PcgWork&, hipStream_t);
-template void pcg_true_residual<2, 8>(const SpmvArgs&, const double*, const double*, const PcgWork&, hipStream_t);
-template void pcg_true_residual<2, 16>(const SpmvArgs&, const double*, const double*, const PcgWork&, hipStream_t);
-template void pcg_true_residual<3, 16>(const SpmvArgs&, const double*, const double*, const PcgWork&, hipStream_t);
-template void pcg_true_residual<3, 32>(const SpmvArgs&, const double*, const double*, const PcgWork&, hipStream_t);
+#define EFEA_PCG_START(DN, G)                                                                              \
+  template int pcg_start<double, DN, G>(const SpmvArgs&, const double*, const double*, const double*,        \
+                                        const PcgWork&, double, double, double, hipStream_t);                \
+  template int pcg_start<float, DN, G>(const SpmvArgs&, const float*, const double*, const double*,          \
+                                       const PcgWork&, double, double, double, hipStream_t);                 \
+  template void pcg_true_residual<DN, G>(const SpmvArgs&, const double*, const double*, const PcgWork&,      \
+                                         hipStream_t);
+EFEA_GROUPS(EFEA_PCG_START)
+#undef EFEA_PCG_START
 
 // r = b - A64 x (masked rows) on the node-group grid; used by the two-level
 // driver's residual replacement.
@@ -413,12 +400,10 @@ void launch_replace_residual(const SpmvArgs& A64, const double* x, const double*
   hipLaunchKernelGGL((k_replace_residual<DN, G>), dim3(grid_for((int64_t)A64.Nnode * G, A64.cap)), dim3(BS), 0, s, A64, x, b,
                      r);
 }
-template void launch_replace_residual<1, 8>(const SpmvArgs&, const double*, const double*, double*, hipStream_t);
-template void launch_replace_residual<1, 16>(const SpmvArgs&, const double*, const double*, double*, hipStream_t);
-template void launch_replace_residual<2, 8>(const SpmvArgs&, const double*, const double*, double*, hipStream_t);
-template void launch_replace_residual<2, 16>(const SpmvArgs&, const double*, const double*, double*, hipStream_t);
-template void launch_replace_residual<3, 16>(const SpmvArgs&, const double*, const double*, double*, hipStream_t);
-template void launch_replace_residual<3, 32>(const SpmvArgs&, const double*, const double*, double*, hipStream_t);
+#define EFEA_REPLACE_RESIDUAL(DN, G) \
+  template void launch_replace_residual<DN, G>(const SpmvArgs&, const double*, const double*, double*, hipStream_t);
+EFEA_GROUPS(EFEA_REPLACE_RESIDUAL)
+#undef EFEA_REPLACE_RESIDUAL
 
 // Reads back the state and the true residual and fills info[0..3]
 // (iterations, relative residual, replacements, status).
@@ -450,7 +435,7 @@ int pcg_finish_info(const void* state, double rr_true, double* info) {
 template <typename TM, typename TP, int DN, int G>
 static int pcg_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, const double* b, double* x,
                    void* ws, double tol_r, double tol_A, double delta, int maxit, int check_every,
-                   hipStream_t s, double* info) {
+                   int first_check, hipStream_t s, double* info) {
   const int Nnode = A64.Nnode;
   PcgWork w = carve(ws, Nnode, DN);
   const unsigned nbS = grid_for((int64_t)Nnode * G, A64.cap);  // node-group kernels
@@ -464,7 +449,8 @@ static int pcg_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, cons
   }
   int launched = 0;
   while (true) {
-    for (int k = 0; k < check_every; ++k) {
+    const int chunk = launched == 0 && first_check > 0 ? first_check : check_every;
+    for (int k = 0; k < chunk; ++k) {
       hipLaunchKernelGGL((k_pcg_step1<TM, DN, G>), dim3(nbS), dim3(BS), 0, s, AK, (const double*)w.z,
                          (const double*)w.P0, (const double*)w.P1, w.q, (const double*)w.part2, (int)nbN, w.part1,
                          w.st);
@@ -472,7 +458,7 @@ static int pcg_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, cons
                          (const double*)w.P0, (const double*)w.P1, (const double*)w.q, (const double*)w.part1,
                          (int)nbS, w.part2, w.st);
     }
-    launched += check_every;
+    launched += chunk;
     if (int e = check_launch("pcg_iter")) return e;
     if (hipMemcpyAsync(hs, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
@@ -508,7 +494,7 @@ static int pcg_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, cons
 int solve_pcg(int mode, int Nnode, int dof_n, int group, const int* node_ptr, const int* node_cols,
               const double* vals64, const float* vals32, const float* corr32, const unsigned char* known,
               const void* Minv, const double* b, double* x, void* ws, double tol_r, double tol_A, double delta,
-              int maxit, int check_every, int max_blocks, hipStream_t s, double* info) {
+              int maxit, int check_every, int first_check, int max_blocks, hipStream_t s, double* info) {
   SpmvArgs A64{Nnode, node_ptr, node_cols, vals64, known, nullptr, max_blocks};
   SpmvArgs A32{Nnode, node_ptr, node_cols, vals32, known, corr32, max_blocks};
   if (check_every < 1) check_every = 1;
@@ -516,12 +502,12 @@ int solve_pcg(int mode, int Nnode, int dof_n, int group, const int* node_ptr, co
   if (dof_n == D && group == GV) {                                                                         \
     if (mode == 64)                                                                                        \
       return pcg_run<double, double, D, GV>(A64, A64, (const double*)Minv, b, x, ws, tol_r, tol_A, delta,  \
-                                            maxit, check_every, s, info);                                  \
+                                            maxit, check_every, first_check, s, info);                     \
     return pcg_run<float, float, D, GV>(A64, A32, (const float*)Minv, b, x, ws, tol_r, tol_A, delta, maxit, \
-                                        check_every, s, info);                                             \
+                                        check_every, first_check, s, info);                                \
   }
   if (mode == 64 || (mode == 32 && vals32)) {
-    RUN(1, 8) RUN(1, 16) RUN(2, 8) RUN(2, 16) RUN(3, 16) RUN(3, 32)
+    EFEA_GROUPS(RUN)
   }
 #undef RUN
   set_error("solve_pcg: unsupported mode %d / dof_n %d / group %d", mode, dof_n, group);
@@ -563,9 +549,13 @@ int spmv(int prec, int Nnode, int dof_n, int group, const int* node_ptr, const i
     return check_launch("spmv");                                                                          \
   }
   if (prec == 64) {
-    SP(double, 1, 8) SP(double, 1, 16) SP(double, 2, 8) SP(double, 2, 16) SP(double, 3, 16) SP(double, 3, 32)
+#define SP64(D, GV) SP(double, D, GV)
+    EFEA_GROUPS(SP64)
+#undef SP64
   } else if (prec == 32) {
-    SP(float, 1, 8) SP(float, 1, 16) SP(float, 2, 8) SP(float, 2, 16) SP(float, 3, 16) SP(float, 3, 32)
+#define SP32(D, GV) SP(float, D, GV)
+    EFEA_GROUPS(SP32)
+#undef SP32
   }
 #undef SP
   set_error("spmv: unsupported precision %d / dof_n %d / group %d", prec, dof_n, group);
@@ -604,9 +594,16 @@ int gather_sum(int64_t n, const int* idx, const double* y, double* out, hipStrea
   return check_launch("gather_sum");
 }
 
+// Lanes per node from the largest neighbour count, as measured on MI355X
+// (PCG step-1 kernel, FP64 and FP32 values, TRI3/QUAD4 meshes of 10^4..10^6
+// nodes, HEXA8/TETRA4/TETRA10 of 2*10^3..3*10^5): up to 12 neighbours of
+// 1-2 DOF blocks, 4 lanes (three positions per lane in one pass) beat 8 by
+// 10-25% and 16 by up to 2.6x on large meshes; 3-DOF blocks take 8 lanes up
+// to 16 neighbours (TETRA4) and 16 beyond (HEXA8: 27, TETRA10: 65; 32 lanes
+// lose up to 25% in FP32).  Counts between the measured families interpolate.
 int default_group(int dof_n, int max_cnt) {
-  if (dof_n == 3) return max_cnt > 16 ? 32 : 16;
-  return max_cnt > 8 ? 16 : 8;
+  if (dof_n == 3) return max_cnt > 16 ? 16 : 8;
+  return max_cnt > 24 ? 16 : max_cnt > 12 ? 8 : 4;
 }
 
 }  // namespace efea

@@ -322,6 +322,28 @@ class SolveInfo:
     mode: str
     fallback: bool = False
     precond: str = "bj"
+    coarse_reused: bool = False
+
+
+# Lagged coarse inverse.  Any symmetric positive definite Ac^-1 keeps
+# B^-1 + P Ac^-1 P^T a valid preconditioner, so a sequence of solves on one
+# slowly changing matrix (the staggered phase-field increments) can keep the
+# inverse of an earlier Ac: the solution still meets the same stopping test,
+# only the iteration count can grow.  The inverse is rebuilt when a solve
+# needs COARSE_REUSE_GROWTH times the iterations of the first solve after
+# the last rebuild, after COARSE_REUSE_MAX reuses, or when the Dirichlet mask
+# changes.
+COARSE_REUSE_GROWTH = 1.2
+COARSE_REUSE_MAX = 100
+
+
+@dataclass
+class _CoarseCache:
+    Ainv: torch.Tensor
+    known: Optional[torch.Tensor]
+    ref_iterations: Optional[int] = None
+    uses: int = 0
+    stale: bool = False
 
 
 class Workspace:
@@ -344,8 +366,10 @@ class SolveJob:
     def __init__(self, A: "DeviceCsr", b: torch.Tensor, x: torch.Tensor, known: Optional[torch.Tensor],
                  mode: str = "mixed", tol_residual: float = 1e-12, tol_anorm: float = 1e-9,
                  replace_ratio: float = 0.0, maxit: int = 100000, check_every: int = 32, adaptive: bool = True,
-                 owner=None, precond: str = "auto", coarse_target: Optional[int] = None, max_blocks: int = 0):
+                 owner=None, precond: str = "auto", coarse_target: Optional[int] = None, max_blocks: int = 0,
+                 expected_iterations: Optional[int] = None):
         self.A, self.b, self.x, self.known = A, b, x, known
+        self.expected_iterations = expected_iterations
         self.precond = choose_preconditioner(A, precond)
         self.max_blocks = int(max_blocks)
         self.coarse_target = coarse_target
@@ -359,6 +383,8 @@ class SolveJob:
         self.fallback = False
         self.x0 = x.clone() if self.precision == 32 else None
         self.Minv: Optional[torch.Tensor] = None
+        self.coarse_reused = False
+        self._cache: Optional[_CoarseCache] = None
 
     def setup(self, precision: int) -> None:
         """Block-Jacobi inverse (and the FP32 value copy) for ``precision``."""
@@ -372,7 +398,32 @@ class SolveJob:
             A.make_fp32()
         if self.precond == "two-level" and self.Ainv is None:
             self.cs = A.coarse(self.coarse_target)
-            self.Ainv = self.cs.setup(A, self.known)
+            self._coarse_inverse()
+
+    def _coarse_inverse(self) -> None:
+        """``Ac^-1`` of the current values, or the owner's lagged one."""
+        caches = None if self.owner is None or not getattr(self.owner, "coarse_reuse", False) else \
+            self.owner.__dict__.setdefault("_coarse_caches", {})
+        key = (id(self.A), self.cs.target)
+        c = caches.get(key) if caches is not None else None
+        if c is not None and not c.stale and c.known is self.known and c.Ainv.numel() == self.cs.ld ** 2:
+            c.uses += 1
+            self.Ainv, self.coarse_reused, self._cache = c.Ainv, True, c
+            return
+        self.Ainv = self.cs.setup(self.A, self.known)
+        if caches is not None:
+            self._cache = caches[key] = _CoarseCache(self.Ainv, self.known)
+
+    def record(self, info: "SolveInfo") -> None:
+        """Updates the lagged coarse inverse's rebuild test with a finished
+        solve's iteration count."""
+        c = self._cache
+        if c is None:
+            return
+        if c.ref_iterations is None:
+            c.ref_iterations = info.iterations
+        elif info.iterations > COARSE_REUSE_GROWTH * c.ref_iterations or c.uses >= COARSE_REUSE_MAX:
+            c.stale = True
 
     def restart_fp64(self) -> None:
         """After a failed mixed solve: the initial guess back, FP64 setup."""
@@ -382,11 +433,24 @@ class SolveJob:
 
     def info(self, raw) -> SolveInfo:
         return SolveInfo(int(raw[0]), float(raw[1]), int(raw[2]), int(raw[3]),
-                         "fp64" if self.precision == 64 else "mixed", self.fallback, self.precond)
+                         "fp64" if self.precision == 64 else "mixed", self.fallback, self.precond,
+                         self.coarse_reused)
+
+    def _polls(self) -> tuple:
+        """(check_every, first_check) of the host's state read-backs.  With
+        an expected iteration count (the previous solve of a sequence), the
+        first read-back comes just before it and later ones every 8
+        iterations; FP32 solves keep the regular polls, where their residual
+        replacements happen."""
+        n = self.expected_iterations
+        if self.precision == 64 and n:
+            return min(self.check_every, 8), max(1, int(0.95 * n))
+        return self.check_every, 0
 
     def _solve_once(self, ws: Workspace):
         A = self.A
         raw = (C.c_double * 4)()
+        check_every, first_check = self._polls()
         v32 = A.values32 if self.precision == 32 else None
         c32 = A.corr32 if self.precision == 32 else None
         if self.precond == "two-level":
@@ -398,7 +462,8 @@ class SolveJob:
                            cs.nagg,
                            ptr(cs.agg_ptr), ptr(cs.agg_nodes), ptr(cs.rel), ptr(self.Ainv), cs.ld, ptr(self.b),
                            ptr(self.x), ptr(buf), nbytes, self.tol_residual, self.tol_anorm, self.replace_ratio,
-                           self.maxit, self.check_every, self.max_blocks, stream_ptr(), C.cast(raw, C.c_void_p))
+                           self.maxit, check_every, first_check, self.max_blocks, stream_ptr(),
+                           C.cast(raw, C.c_void_p))
             return rc, self.info(raw)
         nbytes = _lib.load().efea_pcg_workspace_bytes(A.Nn, A.dof_n, A.group)
         buf = ws.get(nbytes)
@@ -406,7 +471,8 @@ class SolveJob:
                        ptr(A.node_cols), ptr(A.values), ptr(v32), ptr(c32), ptr(self.known), ptr(self.Minv),
                        ptr(self.b),
                        ptr(self.x), ptr(buf), nbytes, self.tol_residual, self.tol_anorm, self.replace_ratio,
-                       self.maxit, self.check_every, self.max_blocks, stream_ptr(), C.cast(raw, C.c_void_p))
+                       self.maxit, check_every, first_check, self.max_blocks, stream_ptr(),
+                       C.cast(raw, C.c_void_p))
         return rc, self.info(raw)
 
     def run(self, ws: Optional[Workspace] = None, raise_on_fail: bool = True) -> SolveInfo:

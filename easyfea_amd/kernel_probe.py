@@ -95,6 +95,16 @@ def main():
     dst.copy_(src)
     torch.cuda.synchronize()
     out["copy_bytes_each_way"] = n * 8
+    # achievable streaming rate on this device: device-to-device copies
+    # (read + write bytes over the event-timed average)
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(20):
+        dst.copy_(src)
+    e1.record(s)
+    e1.synchronize()
+    out["copy_GBps"] = 2 * n * 8 / (e0.elapsed_time(e1) / 1e3 / 20) / 1e9
     print(json.dumps(out))
 
 

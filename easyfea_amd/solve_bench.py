@@ -55,6 +55,7 @@ def run(workload: str, cells: int, repeats: int) -> dict:
             t0 = time.perf_counter()
             simu = make(solver)
             simu.preconditioner = precond
+            simu.coarse_reuse = False  # every timed solve builds its own coarse inverse
             dev = simu.dev
             A = dev.csr_map(simu.model.dim)
             if precond == "two-level":

@@ -105,6 +105,10 @@ int efea_bj_setup(int precision, int Nnode, int dof_n, const int32_t* node_ptr, 
                   const double* values, const uint8_t* known, void* Minv, void* stream);
 
 size_t efea_pcg_workspace_bytes(int Nnode, int dof_n, int group);
+/* Lanes per node (`group`) of the solver kernels for a pattern whose nodes have
+ * at most `max_neighbours` neighbours (self included): 4, 8 or 16, from timings
+ * of the PCG step-1 kernel on MI355X (`python -m easyfea_amd.autotune`).
+ * Supported (dof_n, group) pairs: (1|2, 4|8|16) and (3, 8|16|32). */
 int efea_default_group(int dof_n, int max_neighbours);
 
 /* Dirichlet-reduced solve A_ii x_i = b_i - A_ic x_c by block-Jacobi PCG on the
@@ -116,6 +120,11 @@ int efea_default_group(int dof_n, int max_neighbours);
  * the FP64 r = b - K64 x at every poll (replace_ratio = 0), or whenever it
  * falls below replace_ratio times its running maximum (replace_ratio > 0).  Stops when ||r|| <= tol_residual ||r0|| or when the delayed
  * A-norm error estimate <= tol_anorm (relative).  Synchronises `stream`.
+ * The host reads the solver state back (and, in mode 32, replaces the
+ * residual) after `first_check` iterations (<= 0: check_every), then every
+ * `check_every` iterations; kernels launched after convergence return at once.
+ * A caller that expects about n iterations (a sequence of similar solves)
+ * passes first_check ~ n and a small check_every to save read-backs.
  * info (host, 4 doubles): iterations, final true relative residual, residual
  * replacements, status (1 converged, 2 breakdown or non-finite values,
  * 3 diverged, 0 iteration limit).  max_blocks caps the blocks of each kernel's
@@ -127,8 +136,8 @@ int efea_solve_pcg(int mode, int Nnode, int dof_n, int group, const int32_t* nod
                    const double* values64, const float* values32, const float* corr32, const uint8_t* known,
                    const void* Minv,
                    const double* b, double* x, void* workspace, size_t workspace_bytes, double tol_residual,
-                   double tol_anorm, double replace_ratio, int maxit, int check_every, int max_blocks, void* stream,
-                   double* info);
+                   double tol_anorm, double replace_ratio, int maxit, int check_every, int first_check,
+                   int max_blocks, void* stream, double* info);
 
 /* Two-level preconditioner: M^-1 = B^-1 + P Ac^-1 P^T with B the nodal block
  * Jacobi and P the rigid-body modes (2D: 3, 3D: 6, scalar: 1 per aggregate) of
@@ -153,8 +162,8 @@ int efea_solve_pcg2(int mode, int Nnode, int dof_n, int group, const int32_t* no
                     const void* Minv, int nagg,
                     const int32_t* agg_ptr, const int32_t* agg_nodes, const double* rel, const double* Ainv, int ld,
                     const double* b, double* x, void* workspace, size_t workspace_bytes, double tol_residual,
-                    double tol_anorm, double replace_ratio, int maxit, int check_every, int max_blocks,
-                    void* stream, double* info);
+                    double tol_anorm, double replace_ratio, int maxit, int check_every, int first_check,
+                    int max_blocks, void* stream, double* info);
 
 /* Continuous batching of Dirichlet-reduced solves (the serving loop).
  * A batch holds up to `capacity` systems of one (mode, dof_n, group) class and

@@ -38,7 +38,7 @@ def test_abi_argument_errors_are_reported_without_gpu():
     assert rc == -1
     assert "negative size" in _lib.last_error()
     rc = lib.efea_solve_pcg(64, 0, 2, 16, None, None, None, None, None, None, None, None, None, None, 0, 1e-12,
-                            1e-9, 1e-3, 10, 1, 0, None, None)
+                            1e-9, 1e-3, 10, 1, 0, 0, None, None)
     assert rc == -1
     rc = lib.efea_pf_split_points(4, 7, None, None, 0, 0.3, 0, None, None, None, None, None)
     assert rc == -1 and "null pointer" in _lib.last_error()
@@ -53,7 +53,7 @@ def test_abi_argument_errors_are_reported_without_gpu():
     assert lib.efea_coarse_ld(0, 2) == -1
     assert lib.efea_coarse_ld(100, 2) == 320 and lib.efea_coarse_ld(100, 3) == 608 and lib.efea_coarse_ld(5, 1) == 32
     assert lib.efea_solve_pcg2(64, 10, 2, 16, None, None, None, None, None, None, None, 0, None, None, None, None,
-                               0, None, None, None, 0, 1e-12, 1e-9, 1e-3, 10, 32, 0, None, None) == -1
+                               0, None, None, None, 0, 1e-12, 1e-9, 1e-3, 10, 32, 0, 0, None, None) == -1
     assert lib.efea_split_f32(-1, 2, None, None, None, None, None) == -1
     jid = C.c_int(5)
     assert lib.efea_batch_pop(None, C.byref(jid), None) == -1
@@ -62,8 +62,12 @@ def test_abi_argument_errors_are_reported_without_gpu():
 
 def test_workspace_sizes_and_groups():
     lib = _lib.load()
-    assert lib.efea_default_group(2, 9) == 16 and lib.efea_default_group(1, 7) == 8
-    assert lib.efea_default_group(3, 27) == 32
+    # TRI3 / QUAD4 (7-9 neighbours): 4 lanes; quadratic 2D (up to 25): 16
+    assert lib.efea_default_group(2, 9) == 4 and lib.efea_default_group(1, 7) == 4
+    assert lib.efea_default_group(2, 21) == 8 and lib.efea_default_group(1, 25) == 16
+    # TETRA4 (15): 8 lanes; HEXA8 (27) and TETRA10 (65): 16
+    assert lib.efea_default_group(3, 15) == 8
+    assert lib.efea_default_group(3, 27) == 16 and lib.efea_default_group(3, 65) == 16
     small = lib.efea_pcg_workspace_bytes(100, 2, 16)
     big = lib.efea_pcg_workspace_bytes(10000, 2, 16)
     assert big > small >= 4 * 100 * 2 * 8
