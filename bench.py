@@ -109,42 +109,60 @@ def cpu_baseline(mesh, steps):
     return time.perf_counter() - t0, states, np.array(forces)
 
 
-def spmv_roofline(A, reps, precision):
-    """Average duration of the PCG's dominant kernel (step 1: p update fused
-    into the SpMV, FP32 values as in the mixed path) over ``reps`` launches,
-    HIP events on the stream the kernel runs on."""
+def step1_roofline(simu, problemType, reps):
+    """Average duration of the solve's dominant kernel - step 1: p update
+    fused into the node-block SpMV (k_pcg2_step1 in a two-level solve,
+    k_pcg_step1 in a block-Jacobi one) - over ``reps`` launches through
+    ``efea_step1_bench``: the solve's own kernel, grid, matrix values
+    (FP32 copy in a mixed solve), Dirichlet mask and partial count, in the
+    state of a running solve; HIP events on the stream the kernels run on."""
     import torch
 
     from easyfea_amd import _lib
     from easyfea_amd.device import ptr, stream_ptr
 
-    x = torch.rand(A.Ndof, dtype=torch.float64, device="cuda")
-    y = torch.empty_like(x)
+    A = simu._device_K(problemType)
+    info = simu.last_solve[problemType]
+    two_level = info.precond == "two-level"
+    precision = 64 if info.mode == "fp64" else 32
+    known = simu._mask_cache[problemType][1]
+    nagg = A.coarse().nagg if two_level else 0
+    z = torch.rand(A.Ndof, dtype=torch.float64, device="cuda")
+    q = torch.empty_like(z)
     vals = A.make_fp32() if precision == 32 else A.values
-    head = (precision, A.Nn, A.dof_n, A.group, ptr(A.node_ptr), ptr(A.node_cols), ptr(vals), None, ptr(x), ptr(y),
-            ptr(A.spmv_scratch))
-    _lib.call("efea_spmv_repeat", *head, 20, stream_ptr())
+    scratch = torch.empty(_lib.load().efea_step1_bench_scratch_bytes(A.Nn, A.dof_n, nagg), dtype=torch.uint8,
+                          device="cuda")
+    head = (int(two_level), precision, A.Nn, A.dof_n, A.group, ptr(A.node_ptr), ptr(A.node_cols), ptr(vals),
+            ptr(A.corr32 if precision == 32 else None), ptr(known), nagg, ptr(z), ptr(q), ptr(scratch))
+    _lib.call("efea_step1_bench", *head, 20, stream_ptr())
     s = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(s)
-    _lib.call("efea_spmv_repeat", *head, reps, stream_ptr())
+    _lib.call("efea_step1_bench", *head, reps, stream_ptr())
     e1.record(s)
     e1.synchronize()
-    dt = e0.elapsed_time(e1) / 1e3 / reps  # one state-init kernel, then reps step-1 launches
+    dt = e0.elapsed_time(e1) / 1e3 / reps  # one state kernel, then reps step-1 launches
     hA = A.host
     nnz, nnz_node, Nn, Ndof = A.nnz, hA.node_cols.size, A.Nn, A.Ndof
     vb = 4 if precision == 32 else 8
-    # values + node columns + node offsets + mask + z, p_old read + p, q written
+    # values (+ FP32 row-sum correction) + node columns + node offsets + mask
+    # + z, p_old read + p, q written
     algo_bytes = nnz * vb + nnz_node * 4 + (Nn + 1) * 4 + Ndof * 1 + Ndof * 8 * 4
+    if precision == 32:
+        algo_bytes += Nn * A.dof_n * A.dof_n * 4
     csr_bytes = nnz * (vb + 4) + (Ndof + 1) * 4 + 2 * Ndof * vb
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "spmv_pf200_traffic.json")
-    if os.path.exists(tf) and A.Ndof == 81002:
+    tf = os.path.join(ROOT, "profiles", "step1_pf200_traffic.json")
+    if os.path.exists(tf):
         with open(tf) as f:
-            traffic = json.load(f)["kernels"][f"fp{precision}"]["hbm_bytes_per_launch"]
+            t = json.load(f)
+        if t.get("ndof") == Ndof and t.get("precision") == precision and \
+                ("k_pcg2_step1" if two_level else "k_pcg_step1") in t.get("kernel", ""):
+            traffic = t["hbm_bytes_per_launch"]
     peak = 8.0e12
+    kname = "k_pcg2_step1" if two_level else "k_pcg_step1"
     return {
-        "kernel": f"k_pcg_step1<{'float' if precision == 32 else 'double'},{A.dof_n},{A.group}>",
+        "kernel": f"{kname}<{'float' if precision == 32 else 'double'},{A.dof_n},{A.group}>",
         "bound": "hbm",
         "achieved": algo_bytes / dt / 1e9,
         "peak": peak / 1e9,
@@ -154,6 +172,7 @@ def spmv_roofline(A, reps, precision):
         "bytes_per_launch": algo_bytes,
         "csr_formula_bytes": csr_bytes,
         "avg_launch_us": dt * 1e6,
+        "launches_timed": reps,
         "nnz": nnz,
     }
 
@@ -276,6 +295,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=500)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="end with one device-to-device copy of this many MiB (HBM counter calibration)")
     ap.add_argument("--no-coarse-reuse", action="store_true",
                     help="rebuild the coarse inverse at every displacement solve")
     ap.add_argument("--workload", default="sent",
@@ -334,10 +355,8 @@ def main():
         fr = np.abs(mixed["forces"] - plain["forces"]) / np.maximum(np.abs(plain["forces"]), 1e-300)
         accuracy["force_curve_max_rel_dev_vs_fp64"] = float(fr.max())
 
-    from easyfea_amd.device import MIXED_MIN_MATRIX_BYTES
 
-    Ku = mixed["simu"]._Ku
-    roof = spmv_roofline(Ku, args.spmv_reps, 32 if Ku.nnz * 8 >= MIXED_MIN_MATRIX_BYTES else 64)
+    roof = step1_roofline(mixed["simu"], "elastic", args.spmv_reps)
 
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
@@ -405,7 +424,13 @@ def main():
             "pcg_iters_u_mean": float(plain["its_u"].mean()),
             "mixed_speedup": plain["total"] / mixed["total"],
         }
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
+    if args.copy_mib > 0:
+        n = args.copy_mib * 2**20 // 8
+        src = torch.rand(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
 
 
 if __name__ == "__main__":

@@ -29,6 +29,8 @@ SIGNATURES = {
     "efea_spmv_scratch_bytes": (sz, [i32, i32]),
     "efea_spmv": (i32, [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
     "efea_spmv_repeat": (i32, [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, vp]),
+    "efea_step1_bench_scratch_bytes": (sz, [i32, i32, i32]),
+    "efea_step1_bench": (i32, [i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, i32, vp]),
     "efea_bj_setup": (i32, [i32, i32, i32, vp, vp, vp, vp, vp, vp]),
     "efea_pcg_workspace_bytes": (sz, [i32, i32, i32]),
     "efea_default_group": (i32, [i32, i32]),

@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include "efea_common.h"
+#include "pcg_device.h"
 
 namespace efea {
 
@@ -45,6 +46,10 @@ int solve_pcg(int, int, int, int, const int*, const int*, const double*, const f
               const unsigned char*, const void*, const double*, double*, void*, double, double, double, int, int,
               int, int, hipStream_t, double*);
 int split_f32(int, int, const int*, const double*, float*, float*, hipStream_t);
+int step1_bench(int, int, int, int, const int*, const int*, const void*, const float*, const unsigned char*,
+                const double*, double*, void*, int, hipStream_t);
+int pcg2_step1_bench(int, int, int, int, const int*, const int*, const void*, const float*, const unsigned char*, int,
+                     const double*, double*, void*, int, hipStream_t);
 int dot(int64_t, const double*, const double*, double*, double*, hipStream_t);
 int gather_sum(int64_t, const int*, const double*, double*, hipStream_t);
 int default_group(int, int);
@@ -162,6 +167,26 @@ int efea_spmv_repeat(int precision, int Nnode, int dof_n, int group, const int32
   }
   return spmv(precision, Nnode, dof_n, group, node_ptr, node_cols, values, known, x, y, scratch, reps,
               STREAM(stream));
+}
+
+size_t efea_step1_bench_scratch_bytes(int Nnode, int dof_n, int nagg) {
+  if (Nnode < 0 || dof_n < 1 || nagg < 0) return 0;
+  return step1_bench_bytes(Nnode, dof_n, nagg > MAXB ? nagg : MAXB);
+}
+
+int efea_step1_bench(int two_level, int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr,
+                     const int32_t* node_cols, const void* values, const float* corr32, const uint8_t* known,
+                     int nagg, const double* z, double* q, void* scratch, int reps, void* stream) {
+  if (Nnode <= 0 || reps < 1 || !node_ptr || !node_cols || !values || !z || !q || !scratch ||
+      (two_level && nagg <= 0)) {
+    set_error("efea_step1_bench: null pointer, empty system, reps < 1 or nagg <= 0");
+    return EFEA_ERR_ARG;
+  }
+  if (two_level)
+    return pcg2_step1_bench(precision, Nnode, dof_n, group, node_ptr, node_cols, values, corr32, known, nagg, z, q,
+                            scratch, reps, STREAM(stream));
+  return step1_bench(precision, Nnode, dof_n, group, node_ptr, node_cols, values, corr32, known, z, q, scratch,
+                     reps, STREAM(stream));
 }
 
 int efea_bj_setup(int precision, int Nnode, int dof_n, const int32_t* node_ptr, const int32_t* node_cols,

@@ -503,6 +503,39 @@ static int pcg2_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, con
   return pcg_finish_info(&fin, rr_true, info);
 }
 
+int step1_bench_init(const Step1Bench& w, int nb2, int64_t ndof, hipStream_t s);
+
+// `reps` launches of the two-level solve's step-1 kernel (k_pcg2_step1: r.z
+// partials from the nagg aggregate blocks) on its solve grid, in the state of
+// a running solve (timing harness).
+int pcg2_step1_bench(int prec, int Nnode, int dof_n, int group, const int* node_ptr, const int* node_cols,
+                     const void* vals, const float* corr32, const unsigned char* known, int nagg, const double* z,
+                     double* q, void* scratch, int reps, hipStream_t s) {
+  SpmvArgs A{Nnode, node_ptr, node_cols, vals, known, prec == 32 ? (const void*)corr32 : nullptr, 0};
+  const unsigned nbS = grid_for((int64_t)Nnode * group);
+  const Step1Bench w = step1_bench_carve(scratch, Nnode, dof_n, nagg > MAXB ? nagg : MAXB);
+  if (int e = step1_bench_init(w, nagg, (int64_t)Nnode * dof_n, s)) return e;
+#define SB(TM, D, GV)                                                                                        \
+  if (dof_n == D && group == GV) {                                                                          \
+    for (int k = 0; k < reps; ++k)                                                                          \
+      hipLaunchKernelGGL((k_pcg2_step1<TM, D, GV>), dim3(nbS), dim3(BS), 0, s, A, z, (const double*)w.P0,     \
+                         (const double*)w.P1, q, (const double*)w.part2, nagg, w.part1, w.st);               \
+    return check_launch("pcg2_step1_bench");                                                                \
+  }
+  if (prec == 64) {
+#define SB64(D, GV) SB(double, D, GV)
+    EFEA_GROUPS(SB64)
+#undef SB64
+  } else if (prec == 32) {
+#define SB32(D, GV) SB(float, D, GV)
+    EFEA_GROUPS(SB32)
+#undef SB32
+  }
+#undef SB
+  set_error("pcg2_step1_bench: unsupported precision %d / dof_n %d / group %d", prec, dof_n, group);
+  return EFEA_ERR_ARG;
+}
+
 int solve_pcg2(int mode, int Nnode, int dof_n, int group, const int* node_ptr, const int* node_cols,
                const double* vals64, const float* vals32, const float* corr32, const unsigned char* known,
                const void* Minv, int nagg,

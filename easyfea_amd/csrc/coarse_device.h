@@ -48,9 +48,13 @@ __device__ inline void pcg2_step2_body(CoarseArgs C, const unsigned char* __rest
              const double* __restrict__ P1, const double* __restrict__ q, const double* __restrict__ part1, int nb1,
              double* __restrict__ part2, PcgState* st, int blk) {
   constexpr int NM = Modes<DN>::NM;
-  if (st->done) return;
+  // the state test follows the loads and the block reduction (a launch after
+  // convergence does the loads and returns there)
   __shared__ double sh[BS / 64];
   __shared__ double bc;
+  double pp1[1];
+  partial_share<1>(part1, nb1, pp1);
+  double ppq = pp1[0];
   const int pn = st->par_next;
   const double rz = st->rz;
   const double* __restrict__ p = pn ? P1 : P0;
@@ -74,11 +78,10 @@ __device__ inline void pcg2_step2_body(CoarseArgs C, const unsigned char* __rest
 #pragma unroll
     for (int d = 0; d < 3; ++d) relf[d] = C.rel[3 * (int64_t)nf + d];
   }
-  double ppq = 0;
-  for (int i = threadIdx.x; i < nb1; i += BS) ppq += part1[i];
   ppq = block_sum<double, BS>(ppq, sh);
   if (threadIdx.x == 0) bc = ppq;
   __syncthreads();
+  if (st->done) return;
   const double pq = bc;
   if (!(pq > 0) || !(rz > 0)) {
     if (blk == 0 && threadIdx.x == 0) st->done = (rz == 0) ? 1 : 2;
@@ -160,7 +163,7 @@ template <int DN>
 __device__ inline void pcg2_prolong_body(CoarseArgs C, const unsigned char* __restrict__ known, double* __restrict__ z,
                double* __restrict__ part2, PcgState* st, int blk) {
   constexpr int NM = Modes<DN>::NM;
-  if (st->done) return;
+  // the state test follows the loads and the coarse rows
   __shared__ double zcs[NM];
   const int nc = C.nagg * NM;
   const int k0 = C.agg_ptr[blk], k1 = C.agg_ptr[blk + 1];
@@ -188,6 +191,7 @@ __device__ inline void pcg2_prolong_body(CoarseArgs C, const unsigned char* __re
     if (lane == 0) zcs[m] = s;
   }
   __syncthreads();
+  if (st->done) return;
   double zc[NM];
 #pragma unroll
   for (int m = 0; m < NM; ++m) zc[m] = zcs[m];

@@ -19,6 +19,8 @@
 
 namespace efea {
 
+static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
 struct SpmvArgs {
   int Nnode;
   const int* node_ptr;
@@ -164,6 +166,23 @@ __device__ inline double reduce_all(const double* __restrict__ part, int n, int 
 // of that kernel: rz_prev, rrmax_prev and par_next are per-iteration
 // snapshots that break those races.
 constexpr int RING = 32;
+// This thread's share (stride BS) of n partials of S values each.  The first
+// load sits outside the loop, so that it is in flight together with the
+// loads that follow it in the kernel (n <= BS leaves the loop empty).
+template <int S>
+__device__ inline void partial_share(const double* __restrict__ part, int n, double (&out)[S]) {
+#pragma unroll
+  for (int k = 0; k < S; ++k) out[k] = 0;
+  if ((int)threadIdx.x < n) {
+#pragma unroll
+    for (int k = 0; k < S; ++k) out[k] = part[S * threadIdx.x + k];
+  }
+  for (int i = threadIdx.x + BS; i < n; i += BS) {
+#pragma unroll
+    for (int k = 0; k < S; ++k) out[k] += part[S * i + k];
+  }
+}
+
 struct PcgState {
   double rz, rz_prev, alpha, beta, pq;
   double rr, rrmax, rrmax_prev, bb;
@@ -227,30 +246,36 @@ __device__ inline void pcg_step1_body(const SpmvArgs& A, const double* __restric
                                       const double* __restrict__ P0, const double* __restrict__ P1,
                                       double* __restrict__ q, const double* __restrict__ part2, int nb2,
                                       double* __restrict__ part1, PcgState* st, int blk, int nblk) {
-  if (st->done) return;
-  __shared__ double sh[BS / 64];
-  __shared__ double sh2[BS / 64];
-  __shared__ double bc[2];
-  const int par = st->par;
-  const double* __restrict__ pold = par ? P1 : P0;
-  double* __restrict__ pnew = (double*)(par ? P0 : P1);
+  const int done = st->done, par = st->par;
   const bool fresh = st->fresh;
-  const TM* __restrict__ vals = (const TM*)A.vals;
-  double prz = 0, prr = 0;
-  if (!fresh)
-    for (int i = threadIdx.x; i < nb2; i += BS) {
-      prz += part2[2 * i];
-      prr += part2[2 * i + 1];
-    }
   const int64_t lanes = (int64_t)A.Nnode * G;
   int64_t gid = (int64_t)blk * BS + threadIdx.x;
   const bool have = gid < lanes;
   const int n0 = (int)(gid / G), g0 = (int)(gid % G);
-  double sz[DN], sp[DN];
-#pragma unroll
-  for (int a = 0; a < DN; ++a) sz[a] = sp[a] = 0;
+  // Loads that do not depend on the solver state are issued with the state
+  // read, so that their latencies overlap: the node's offsets and the r.z /
+  // r.r partials (also read on a fresh start, where they are not used).
+  int p0 = 0, p1 = 0;
   if (have) {
-    const int p0 = A.node_ptr[n0], cnt = A.node_ptr[n0 + 1] - p0;
+    p0 = A.node_ptr[n0];
+    p1 = A.node_ptr[n0 + 1];
+  }
+  double pr2[2];
+  partial_share<2>(part2, nb2, pr2);
+  double prz = pr2[0], prr = pr2[1];
+  // p1 < p0 never holds; testing it keeps the offset loads ahead of the branch
+  if (done || p1 < p0) return;
+  __shared__ double sh[BS / 64];
+  __shared__ double sh2[BS / 64];
+  __shared__ double bc[2];
+  const double* __restrict__ pold = par ? P1 : P0;
+  double* __restrict__ pnew = (double*)(par ? P0 : P1);
+  const TM* __restrict__ vals = (const TM*)A.vals;
+  double sz[DN], sp[DN], zo[DN], po[DN];
+#pragma unroll
+  for (int a = 0; a < DN; ++a) sz[a] = sp[a] = zo[a] = po[a] = 0;
+  if (have) {
+    const int cnt = p1 - p0;
     double acc[2][DN];
 #pragma unroll
     for (int a = 0; a < DN; ++a) acc[0][a] = acc[1][a] = 0;
@@ -267,6 +292,14 @@ __device__ inline void pcg_step1_body(const SpmvArgs& A, const double* __restric
     for (int a = 0; a < DN; ++a) {
       sz[a] = acc[0][a];
       sp[a] = acc[1][a];
+    }
+    // the node's own z and p_old, ahead of the reduction that yields beta
+    if (g0 == 0) {
+#pragma unroll
+      for (int a = 0; a < DN; ++a) {
+        zo[a] = z[(int64_t)n0 * DN + a];
+        po[a] = pold[(int64_t)n0 * DN + a];
+      }
     }
   }
   double beta = 0;
@@ -301,7 +334,7 @@ __device__ inline void pcg_step1_body(const SpmvArgs& A, const double* __restric
     if (g0 == 0) {
       double pn[DN];
 #pragma unroll
-      for (int a = 0; a < DN; ++a) pn[a] = z[(int64_t)n0 * DN + a] + beta * pold[(int64_t)n0 * DN + a];
+      for (int a = 0; a < DN; ++a) pn[a] = zo[a] + beta * po[a];
 #pragma unroll
       for (int a = 0; a < DN; ++a) {
         const int64_t r = (int64_t)n0 * DN + a;
@@ -373,6 +406,57 @@ __device__ inline void pcg_step1_body(const SpmvArgs& A, const double* __restric
   if (threadIdx.x == 0) part1[blk] = d;
 }
 
+// Timing harness of the step-1 kernel: scratch layout and the state of a
+// running solve whose stopping tests never fire (previous iteration closed
+// from nb2 partials of r.z = r.r = 1, beta = 1, tolerances 0, no pause).
+struct Step1Bench {
+  PcgState* st;
+  double* part1;
+  double* part2;
+  double* P0;
+  double* P1;
+};
+
+inline size_t step1_bench_bytes(int Nnode, int dof_n, int nb2max) {
+  const size_t vb = align256(8 * (size_t)Nnode * dof_n);
+  return align256(sizeof(PcgState)) + align256(8 * (size_t)MAXB) + align256(16 * (size_t)nb2max) + 2 * vb;
+}
+
+inline Step1Bench step1_bench_carve(void* scratch, int Nnode, int dof_n, int nb2max) {
+  char* c = (char*)scratch;
+  Step1Bench b;
+  b.st = (PcgState*)c; c += align256(sizeof(PcgState));
+  b.part1 = (double*)c; c += align256(8 * (size_t)MAXB);
+  b.part2 = (double*)c; c += align256(16 * (size_t)nb2max);
+  b.P0 = (double*)c; c += align256(8 * (size_t)Nnode * dof_n);
+  b.P1 = (double*)c;
+  return b;
+}
+
+__device__ inline void step1_bench_state_body(PcgState* st, double* part2, int nb2, double* P0, int64_t n,
+                                              int blk, int nblk) {
+  for (int64_t i = (int64_t)blk * blockDim.x + threadIdx.x; i < n; i += (int64_t)nblk * blockDim.x) P0[i] = 0;
+  if (blk == 0) {
+    for (int i = threadIdx.x; i < nb2; i += blockDim.x) part2[2 * i] = part2[2 * i + 1] = 1.0 / nb2;
+    if (threadIdx.x == 0) {
+      st->rz = st->rz_prev = st->alpha = st->pq = st->rr = st->rrmax = st->rrmax_prev = 1;
+      st->beta = 0;
+      st->bb = st->xAx0 = 1;
+      st->tolr2 = st->tolA2 = st->delta2 = 0;
+      st->Sall = st->Sw = 0;
+      for (int i = 0; i < RING; ++i) st->ring[i] = 0;
+      st->head = 0;
+      st->done = 0;
+      st->iters = RING;
+      st->nrepl = 0;
+      st->fresh = 0;
+      st->justrep = 0;
+      st->par = 0;
+      st->par_next = 1;
+    }
+  }
+}
+
 // Step 2: alpha = rz / pq ; x += alpha p ; r -= alpha q ; z = Minv r ;
 // partials of r.z and r.r.  The first node's operands are loaded before the
 // reduction that yields alpha.
@@ -383,12 +467,14 @@ __device__ inline void pcg_step2_body(int Nnode, const unsigned char* __restrict
                                       const double* __restrict__ P1, const double* __restrict__ q,
                                       const double* __restrict__ part1, int nb1, double* __restrict__ part2,
                                       PcgState* st, int blk, int nblk) {
-  if (st->done) return;
+  // Every load is issued before the state test, which follows the block
+  // reduction (a launch after convergence does the loads and returns there).
   __shared__ double sh[BS / 64];
   __shared__ double sh2[BS / 64];
   __shared__ double bc;
-  double ppq = 0;
-  for (int i = threadIdx.x; i < nb1; i += BS) ppq += part1[i];
+  double pp1[1];
+  partial_share<1>(part1, nb1, pp1);
+  double ppq = pp1[0];
   const int pn = st->par_next;
   const double* __restrict__ p = pn ? P1 : P0;
   int n = blk * BS + threadIdx.x;
@@ -408,6 +494,7 @@ __device__ inline void pcg_step2_body(int Nnode, const unsigned char* __restrict
   ppq = block_sum<double, BS>(ppq, sh);
   if (threadIdx.x == 0) bc = ppq;
   __syncthreads();
+  if (st->done) return;
   const double pq = bc;
   const double rz = st->rz;
   if (!(pq > 0) || !(rz > 0)) {
@@ -518,7 +605,6 @@ __device__ inline void rebuild_z_body(int Nnode, const unsigned char* __restrict
 }
 
 // Workspace of one solve: five FP64 vectors, the partial arrays and the state.
-static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct PcgWork {
   double* r;

@@ -562,6 +562,46 @@ int spmv(int prec, int Nnode, int dof_n, int group, const int* node_ptr, const i
   return EFEA_ERR_ARG;
 }
 
+__global__ void __launch_bounds__(BS) k_step1_bench_state(PcgState* st, double* part2, int nb2, double* P0,
+                                                          int64_t n) {
+  step1_bench_state_body(st, part2, nb2, P0, n, blockIdx.x, gridDim.x);
+}
+
+int step1_bench_init(const Step1Bench& w, int nb2, int64_t ndof, hipStream_t s) {
+  hipLaunchKernelGGL(k_step1_bench_state, dim3(grid_for(ndof)), dim3(BS), 0, s, w.st, w.part2, nb2, w.P0, ndof);
+  return check_launch("step1_bench_init");
+}
+
+// `reps` launches of the block-Jacobi solve's step-1 kernel (k_pcg_step1) on
+// its solve grid, in the state of a running solve (timing harness).
+int step1_bench(int prec, int Nnode, int dof_n, int group, const int* node_ptr, const int* node_cols,
+                const void* vals, const float* corr32, const unsigned char* known, const double* z, double* q,
+                void* scratch, int reps, hipStream_t s) {
+  SpmvArgs A{Nnode, node_ptr, node_cols, vals, known, prec == 32 ? (const void*)corr32 : nullptr, 0};
+  const unsigned nbS = grid_for((int64_t)Nnode * group), nbN = grid_for(Nnode);
+  const Step1Bench w = step1_bench_carve(scratch, Nnode, dof_n, MAXB);
+  if (int e = step1_bench_init(w, (int)nbN, (int64_t)Nnode * dof_n, s)) return e;
+#define SB(TM, D, GV)                                                                                        \
+  if (dof_n == D && group == GV) {                                                                          \
+    for (int k = 0; k < reps; ++k)                                                                          \
+      hipLaunchKernelGGL((k_pcg_step1<TM, D, GV>), dim3(nbS), dim3(BS), 0, s, A, z, (const double*)w.P0,      \
+                         (const double*)w.P1, q, (const double*)w.part2, (int)nbN, w.part1, w.st);           \
+    return check_launch("step1_bench");                                                                     \
+  }
+  if (prec == 64) {
+#define SB64(D, GV) SB(double, D, GV)
+    EFEA_GROUPS(SB64)
+#undef SB64
+  } else if (prec == 32) {
+#define SB32(D, GV) SB(float, D, GV)
+    EFEA_GROUPS(SB32)
+#undef SB32
+  }
+#undef SB
+  set_error("step1_bench: unsupported precision %d / dof_n %d / group %d", prec, dof_n, group);
+  return EFEA_ERR_ARG;
+}
+
 int bj_setup(int prec, int Nnode, int dof_n, const int* node_ptr, const int* node_cols, const double* vals,
              const unsigned char* known, void* Minv, hipStream_t s) {
   const unsigned nb = (unsigned)((Nnode + 127) / 128);

@@ -99,6 +99,19 @@ int efea_spmv_repeat(int precision, int Nnode, int dof_n, int group, const int32
                      const int32_t* node_cols, const void* values, const uint8_t* known, const double* x, double* y,
                      void* scratch, int reps, void* stream);
 
+/* Timing harness of the solver's dominant kernel (no counterpart in the
+ * reference; for benchmarks and profiles): `reps` back-to-back launches of the
+ * step-1 kernel (p = z + beta p; q = A p, masked; partials of p.q) exactly as a
+ * solve launches it - k_pcg2_step1 with the r.z partials of `nagg` aggregate
+ * blocks when two_level, k_pcg_step1 with those of the node grid otherwise -
+ * in the state of a running solve whose stopping tests never fire.  z: input
+ * (Nnode*dof_n doubles); q: output; corr32 as in efea_solve_pcg (mode 32);
+ * scratch: efea_step1_bench_scratch_bytes(Nnode, dof_n, nagg) bytes. */
+size_t efea_step1_bench_scratch_bytes(int Nnode, int dof_n, int nagg);
+int efea_step1_bench(int two_level, int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr,
+                     const int32_t* node_cols, const void* values, const float* corr32, const uint8_t* known,
+                     int nagg, const double* z, double* q, void* scratch, int reps, void* stream);
+
 /* Block-Jacobi setup: inverse of each node's dof_n x dof_n diagonal block of the
  * Dirichlet-reduced matrix, stored in `precision`. */
 int efea_bj_setup(int precision, int Nnode, int dof_n, const int32_t* node_ptr, const int32_t* node_cols,
