@@ -190,6 +190,7 @@ def run_phasefield(args) -> dict:
         out["fp64"] = {"value": ndof / wall64, "wall_s": wall64}
         out["force_curve_max_rel_dev_vs_fp64"] = fdev
         out["mixed_speedup"] = wall64 / wall
+    out["_specs"] = specs
     return out
 
 
