@@ -53,6 +53,8 @@ SIGNATURES = {
         [vp, i32, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, i32, vp, vp, vp, sz, f64, f64, f64, i32, i32, vp],
     ),
     "efea_batch_run": (i32, [vp, i32, vp]),
+    "efea_batch_launch": (i32, [vp, i32, vp]),
+    "efea_batch_wait": (i32, [vp, vp]),
     "efea_batch_pop": (i32, [vp, C.POINTER(i32), vp]),
     "efea_batch_active": (i32, [vp]),
     "efea_dot": (i32, [i64, vp, vp, vp, vp, vp]),

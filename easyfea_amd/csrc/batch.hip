@@ -137,6 +137,10 @@ struct Batch {
   double* h_rr = nullptr;
   int totS = 0, totN = 0, totA = 0;
   bool dirty = true;
+  // a launched chunk whose states have not been read yet: the first
+  // `inflight` entries of `active` ran `inflight_iters` iterations
+  int inflight = 0, inflight_iters = 0;
+  hipStream_t inflight_stream = nullptr;
 };
 
 static void batch_free(Batch* b) {
@@ -199,7 +203,9 @@ int batch_create(int mode, int dof_n, int group, int two_level, int capacity, vo
 }
 
 int batch_destroy(void* h) {
-  batch_free((Batch*)h);
+  Batch* B = (Batch*)h;
+  if (B && B->inflight) (void)hipStreamSynchronize(B->inflight_stream);  // kernels still read its buffers
+  batch_free(B);
   return EFEA_OK;
 }
 
@@ -329,7 +335,7 @@ static int rebuild(Batch* B, hipStream_t s) {
 }
 
 template <typename TM, typename TP, int DN, int G>
-static int batch_run_t(Batch* B, int iters, hipStream_t s) {
+static int batch_launch_t(Batch* B, int iters, hipStream_t s) {
   const int n = (int)B->active.size();
   const int* mapS = B->d_map;
   const int* mapN = B->d_map + (size_t)B->cap * MAXB;
@@ -346,8 +352,28 @@ static int batch_run_t(Batch* B, int iters, hipStream_t s) {
   }
   hipLaunchKernelGGL(kb_gather_state, dim3(n), dim3(64), 0, s, dsys, B->d_st);
   if (int e = check_launch("batch_iter")) return e;
-  if (hipMemcpyAsync(B->h_st, B->d_st, n * sizeof(PcgState), hipMemcpyDeviceToHost, s) != hipSuccess ||
-      hipStreamSynchronize(s) != hipSuccess) {
+  if (hipMemcpyAsync(B->h_st, B->d_st, n * sizeof(PcgState), hipMemcpyDeviceToHost, s) != hipSuccess) {
+    set_error("batch: state readback failed");
+    return EFEA_ERR_HIP;
+  }
+  B->inflight = n;
+  B->inflight_iters = iters;
+  B->inflight_stream = s;
+  return EFEA_OK;
+}
+
+// Reads the states of the launched chunk (synchronising the stream), queues
+// the residual replacements, finishes the systems that stopped.  Systems
+// added while the chunk ran (entries past `inflight`) are left as they are.
+template <typename TM, typename TP, int DN, int G>
+static int batch_wait_t(Batch* B, hipStream_t s) {
+  const int n = B->inflight, iters = B->inflight_iters;
+  B->inflight = 0;
+  const int* mapS = B->d_map;
+  const int* mapN = B->d_map + (size_t)B->cap * MAXB;
+  const int* mapA = B->d_map + 2 * (size_t)B->cap * MAXB;
+  const BatchSys* dsys = B->d_sys;
+  if (hipStreamSynchronize(s) != hipSuccess) {
     set_error("batch: state readback failed");
     return EFEA_ERR_HIP;
   }
@@ -364,6 +390,8 @@ static int batch_run_t(Batch* B, int iters, hipStream_t s) {
     nfin += e.finished;
   }
   if (any_replace) {
+    // the kernels address systems through the maps of the chunk, which cover
+    // the first n descriptors only
     if (int e = upload_sys(B, s)) return e;
     hipLaunchKernelGGL((kb_replace_residual<DN, G>), dim3(B->totS), dim3(BS), 0, s, dsys, mapS);
     if (B->two_level) {
@@ -392,11 +420,11 @@ static int batch_run_t(Batch* B, int iters, hipStream_t s) {
     return EFEA_ERR_HIP;
   }
   std::vector<BatchEntry> keep;
-  keep.reserve(n);
+  keep.reserve(B->active.size());
   j = 0;
-  for (int i = 0; i < n; ++i) {
+  for (int i = 0; i < (int)B->active.size(); ++i) {
     BatchEntry& e = B->active[i];
-    if (!e.finished) {
+    if (i >= n || !e.finished) {
       keep.push_back(e);
       continue;
     }
@@ -408,24 +436,51 @@ static int batch_run_t(Batch* B, int iters, hipStream_t s) {
   return nfin;
 }
 
-// Runs `iters` iterations of every active system, then polls: systems that
-// converged, failed or reached maxit move to the finished list.  Returns the
-// number of systems that finished in this call.
-int batch_run(void* h, int iters, hipStream_t s) {
+// Launches `iters` iterations of every active system and the read-back of
+// their states, without waiting (at most one chunk in flight per batch).
+int batch_launch(void* h, int iters, hipStream_t s) {
   Batch* B = (Batch*)h;
+  if (B->inflight) {
+    set_error("batch_launch: a chunk is already in flight (call batch_wait first)");
+    return EFEA_ERR_ARG;
+  }
   if (B->active.empty()) return 0;
   if (iters < 1) iters = 1;
   if (B->dirty) {
     if (int e = rebuild(B, s)) return e;
   }
-#define RN(D, GV)                                                                             \
+#define LN(D, GV)                                                                             \
   if (B->dof_n == D && B->group == GV)                                                       \
-    return B->mode == 64 ? batch_run_t<double, double, D, GV>(B, iters, s)                   \
-                         : batch_run_t<float, float, D, GV>(B, iters, s);
-  EFEA_GROUPS(RN)
-#undef RN
-  set_error("batch_run: unsupported configuration");
+    return B->mode == 64 ? batch_launch_t<double, double, D, GV>(B, iters, s)                \
+                         : batch_launch_t<float, float, D, GV>(B, iters, s);
+  EFEA_GROUPS(LN)
+#undef LN
+  set_error("batch_launch: unsupported configuration");
   return EFEA_ERR_ARG;
+}
+
+// Waits for the chunk in flight; systems that converged, failed or reached
+// maxit move to the finished list.  Returns how many finished (0 when no
+// chunk was in flight).
+int batch_wait(void* h, hipStream_t s) {
+  Batch* B = (Batch*)h;
+  if (!B->inflight) return 0;
+#define WT(D, GV)                                                                             \
+  if (B->dof_n == D && B->group == GV)                                                       \
+    return B->mode == 64 ? batch_wait_t<double, double, D, GV>(B, s)                         \
+                         : batch_wait_t<float, float, D, GV>(B, s);
+  EFEA_GROUPS(WT)
+#undef WT
+  set_error("batch_wait: unsupported configuration");
+  return EFEA_ERR_ARG;
+}
+
+// Runs `iters` iterations of every active system, then polls: systems that
+// converged, failed or reached maxit move to the finished list.  Returns the
+// number of systems that finished in this call.
+int batch_run(void* h, int iters, hipStream_t s) {
+  if (int e = batch_launch(h, iters, s)) return e;
+  return batch_wait(h, s);
 }
 
 // Pops one finished system: its id and info[0..3] (iterations, relative

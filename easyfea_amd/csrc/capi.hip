@@ -66,6 +66,8 @@ int batch_add(void*, int, const int*, const int*, const double*, const float*, c
               const void*, int, const int*, const int*, const double*, const double*, int, const double*, double*,
               void*, size_t, double, double, double, int, int, hipStream_t);
 int batch_run(void*, int, hipStream_t);
+int batch_launch(void*, int, hipStream_t);
+int batch_wait(void*, hipStream_t);
 int batch_pop(void*, int*, double*);
 int batch_active(void*);
 int coarse_ld(int, int);
@@ -264,6 +266,22 @@ int efea_batch_run(void* handle, int iterations, void* stream) {
     return EFEA_ERR_ARG;
   }
   return batch_run(handle, iterations, STREAM(stream));
+}
+
+int efea_batch_launch(void* handle, int iterations, void* stream) {
+  if (!handle) {
+    set_error("efea_batch_launch: null handle");
+    return EFEA_ERR_ARG;
+  }
+  return batch_launch(handle, iterations, STREAM(stream));
+}
+
+int efea_batch_wait(void* handle, void* stream) {
+  if (!handle) {
+    set_error("efea_batch_wait: null handle");
+    return EFEA_ERR_ARG;
+  }
+  return batch_wait(handle, STREAM(stream));
 }
 
 int efea_batch_pop(void* handle, int* id, double* info) {

@@ -379,7 +379,8 @@ class SolveJob:
         self.tol_residual, self.tol_anorm, self.replace_ratio = tol_residual, tol_anorm, replace_ratio
         self.maxit, self.check_every = maxit, check_every
         self.precision = effective_precision(A, mode, adaptive)
-        self.owner = owner
+        self.owner = owner  # the simulation (lagged coarse inverse cache)
+        self.request = None  # the serving request waiting for this solve
         self.fallback = False
         self.x0 = x.clone() if self.precision == 32 else None
         self.Minv: Optional[torch.Tensor] = None
@@ -515,11 +516,15 @@ def drive(gen):
 
 class BatchSolver:
     """Continuous batching over ``efea_batch_*``: the jobs of one (precision,
-    dof_n, group) class advance together, one launch per PCG step."""
+    dof_n, group) class advance together, one launch per PCG step, on the
+    batch's own stream.  A job's inputs come from the caller's current
+    stream, which the batch stream waits for when the job joins."""
 
     def __init__(self, precision: int, dof_n: int, group: int, precond: str = "bj", capacity: int = 256):
         self.precision, self.dof_n, self.group, self.capacity = precision, dof_n, group, capacity
         self.precond = precond
+        self.stream = torch.cuda.Stream()
+        self._inflight = False
         h = C.c_void_p()
         _lib.call("efea_batch_create", precision, dof_n, group, int(precond == "two-level"), capacity, C.byref(h))
         self._h = h
@@ -545,18 +550,41 @@ class BatchSolver:
         job.ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
         v32 = A.values32 if self.precision == 32 else None
         c32 = A.corr32 if self.precision == 32 else None
+        ready = torch.cuda.Event()
+        ready.record(torch.cuda.current_stream())  # the job's set-up and inputs
+        self.stream.wait_event(ready)
         jid = _lib.call("efea_batch_add", self._h, A.Nn, ptr(A.node_ptr), ptr(A.node_cols), ptr(A.values), ptr(v32),
                         ptr(c32), ptr(job.known), ptr(job.Minv), *coarse, ptr(job.b), ptr(job.x), ptr(job.ws), nbytes,
-                        job.tol_residual, job.tol_anorm, job.replace_ratio, job.maxit, job.max_blocks, stream_ptr())
+                        job.tol_residual, job.tol_anorm, job.replace_ratio, job.maxit, job.max_blocks,
+                        self.stream.cuda_stream)
         self._jobs[jid] = job
         return jid
+
+    def launch(self, iterations: int) -> bool:
+        """Enqueues ``iterations`` PCG iterations of every job (returns at
+        once); ``wait`` collects them."""
+        if not self._jobs or self._inflight:
+            return False
+        _lib.call("efea_batch_launch", self._h, iterations, self.stream.cuda_stream)
+        self._inflight = True
+        return True
+
+    def wait(self) -> list:
+        """Waits for the launched chunk; the finished ``(job, SolveInfo)``
+        pairs (their results are complete on the device)."""
+        if not self._inflight:
+            return []
+        self._inflight = False
+        n = _lib.call("efea_batch_wait", self._h, self.stream.cuda_stream)
+        return self._pop(n)
 
     def run(self, iterations: int) -> list:
         """``iterations`` PCG iterations of every job, then the finished
         ``(job, SolveInfo)`` pairs."""
-        if not self._jobs:
-            return []
-        n = _lib.call("efea_batch_run", self._h, iterations, stream_ptr())
+        self.launch(iterations)
+        return self.wait()
+
+    def _pop(self, n: int) -> list:
         out = []
         jid = C.c_int()
         raw = (C.c_double * 4)()
@@ -606,13 +634,25 @@ class BatchExecutor:
     def active(self) -> int:
         return sum(len(b) for b in self.batches.values()) + sum(len(w) for w in self.waiting.values())
 
+    def launch(self) -> bool:
+        """Enqueues one chunk of every non-empty batch (returns at once)."""
+        launched = False
+        for b in self.batches.values():
+            launched |= b.launch(self.check_every)
+        return launched
+
     def step(self) -> list:
-        """One chunk of every non-empty batch; returns the completed
+        """One chunk of every non-empty batch, then ``collect``."""
+        self.launch()
+        return self.collect()
+
+    def collect(self) -> list:
+        """Waits for the launched chunks; returns the completed
         ``(job, SolveInfo)`` pairs (fallback restarts are resubmitted);
         waiting jobs take the slots that freed up."""
         done = []
         for key, b in list(self.batches.items()):
-            for job, si in b.run(self.check_every):
+            for job, si in b.wait():
                 if si.status != 1 and job.precision == 32:
                     job.restart_fp64()
                     self.submit(job)

@@ -45,7 +45,11 @@ class GroupElem:
 
     @property
     def nodes(self) -> np.ndarray:
-        return np.unique(self.connect)
+        c = self.connect
+        cached = self.__dict__.get("_nodes_cache")
+        if cached is None or cached[0] is not c:  # sorted unique nodes, kept per connectivity array
+            cached = self.__dict__["_nodes_cache"] = (c, np.unique(c))
+        return cached[1].copy()
 
     def Get_assembly_e(self, dof_n: int) -> np.ndarray:
         """Element DOF rows ``[n0*dof_n+0, n0*dof_n+1, ...]`` (``_group_elem.py:358-380``)."""

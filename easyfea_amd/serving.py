@@ -12,9 +12,11 @@ as ``yield job`` / ``info = ...``, e.g. ``Elastic._solve_gen`` or
   running batch;
 * advances every active solve ``check_every`` PCG iterations per chunk, all
   systems of one (precision, DOF per node, lanes per node) class in one
-  launch per PCG step (``efea_batch_run``);
-* hands each finished solve's ``SolveInfo`` back to its request, which
-  either yields its next solve or completes.
+  launch per PCG step, each class on its own stream (``efea_batch_launch``);
+* while a chunk runs, hands each solve finished in the previous chunk its
+  ``SolveInfo``: its request either yields its next solve (host work and
+  assembly kernels overlap the chunk; the solve joins at the next chunk) or
+  completes (``efea_batch_wait`` collects the chunk).
 
 Latency of a request = completion time - arrival time; the loop records
 admission and completion times and the per-request results.  The reference
@@ -91,7 +93,7 @@ class Server:
             req.finished = now_fn()
             req._gen = None
             return
-        job.owner = req
+        job.request = req
         req.solves += 1
         self.exec.submit(job)
 
@@ -104,6 +106,7 @@ class Server:
         def now() -> float:
             return self.clock() - t0
 
+        done_prev: list = []  # solves finished in the last chunk, handed back during the next one
         while pending or active:
             t = now()
             if t > time_limit:
@@ -122,13 +125,19 @@ class Server:
                     continue
                 active.append(req)
                 self._advance(req, None, now)
-            # one chunk of every batch; finished solves go back to their requests
+            # one chunk of every batch, enqueued without waiting; while it runs
+            # the host hands the previous chunk's finished solves back to their
+            # requests (their next assembly and solve set-up go onto the
+            # current stream, their solves join the batches at the next chunk)
+            launched = False
             if self.exec.active():
                 self.max_active = max(self.max_active, self.exec.active())
                 self.max_resident = max(self.max_resident, used)
                 self.chunks += 1
-                for job, info in self.exec.step():
-                    self._advance(job.owner, info, now)
+                launched = self.exec.launch()
+            for job, info in done_prev:
+                self._advance(job.request, info, now)
+            done_prev = self.exec.collect() if launched else []
             # retire completed requests
             still = []
             for req in active:

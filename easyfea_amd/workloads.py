@@ -49,9 +49,12 @@ def lognormal(mean: float, cov: float, z: np.ndarray) -> np.ndarray:
 def cell_of_elements(mesh: Mesh, n: int) -> np.ndarray:
     """Flat index ``i + n*j`` of the cell of an ``n x n`` grid on the unit
     square holding each element's centroid."""
-    c = mesh.coord[mesh.connect].mean(axis=1)
-    ij = np.clip(np.floor(c[:, :2] * n).astype(np.int64), 0, n - 1)
-    return ij[:, 0] + n * ij[:, 1]
+    cache = mesh.__dict__.setdefault("_cell_of_elements", {})
+    if n not in cache:
+        c = mesh.coord[mesh.connect].mean(axis=1)
+        ij = np.clip(np.floor(c[:, :2] * n).astype(np.int64), 0, n - 1)
+        cache[n] = ij[:, 0] + n * ij[:, 1]
+    return cache[n]
 
 
 # ------------------------------------------------------------ single solves
@@ -78,8 +81,10 @@ def sent(cells: int, split: str = "Miehe", Gc=2.7, l0: float = 0.01, solver=None
 def sent_bc(simu: Simulations.PhaseField, uy: float) -> None:
     """Bottom uy = 0; top ux = 0, uy = ``uy`` (``examples/PhaseField/Tension.py``)."""
     mesh = simu.mesh
-    c = simu.__dict__.setdefault("_sent_nodes", (mesh.Nodes_Conditions(lambda x, y, z: y == 0.0),
-                                                   mesh.Nodes_Conditions(lambda x, y, z: y == 1.0)))
+    c = simu.__dict__.get("_sent_nodes")
+    if c is None:
+        c = simu.__dict__["_sent_nodes"] = (mesh.Nodes_Conditions(lambda x, y, z: y == 0.0),
+                                            mesh.Nodes_Conditions(lambda x, y, z: y == 1.0))
     bottom, top = c
     simu.Bc_Init()
     simu.add_dirichlet(bottom, [0], ["y"])

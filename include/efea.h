@@ -205,8 +205,19 @@ int efea_batch_add(void* handle, int Nnode, const int32_t* node_ptr, const int32
                    const double* b, double* x, void* workspace, size_t workspace_bytes, double tol_residual,
                    double tol_anorm, double replace_ratio, int maxit, int max_blocks, void* stream);
 /* Runs `iterations` PCG iterations of every active system, then polls their
- * states (synchronises `stream`); returns how many systems finished. */
+ * states (synchronises `stream`); returns how many systems finished.
+ * efea_batch_run = efea_batch_launch + efea_batch_wait. */
 int efea_batch_run(void* handle, int iterations, void* stream);
+/* The two halves of efea_batch_run, so that the host can work while the GPU
+ * runs the chunk: efea_batch_launch enqueues the iterations and the state
+ * read-back and returns at once (one chunk in flight per batch; a second
+ * launch before the wait is an error); efea_batch_wait synchronises `stream`,
+ * reads the states and returns how many systems finished (0 with nothing in
+ * flight).  Systems added between the two join at the next launch; the
+ * caller orders work from other streams before efea_batch_add (an event the
+ * batch stream waits on). */
+int efea_batch_launch(void* handle, int iterations, void* stream);
+int efea_batch_wait(void* handle, void* stream);
 /* Pops one finished system: *id and info (host, 4 doubles, as efea_solve_pcg).
  * Returns 0, or 1 with *id = -1 when none is waiting. */
 int efea_batch_pop(void* handle, int* id, double* info);

@@ -216,6 +216,7 @@ class _FakeJob:
     def __init__(self, chunks):
         self.chunks = chunks
         self.owner = None
+        self.request = None
 
 
 class _FakeExecutor:
@@ -223,6 +224,7 @@ class _FakeExecutor:
 
     def __init__(self):
         self.jobs = []
+        self.inflight = []
         self.peak = 0
 
     def submit(self, job):
@@ -232,12 +234,20 @@ class _FakeExecutor:
     def active(self):
         return len(self.jobs)
 
-    def step(self):
-        done, keep = [], []
-        for j in self.jobs:
+    def launch(self):
+        self.inflight = list(self.jobs)
+        return bool(self.inflight)
+
+    def collect(self):
+        # only the jobs present at launch advanced; jobs submitted while the
+        # chunk ran join the next one
+        done = []
+        for j in self.inflight:
             j.chunks -= 1
-            (done if j.chunks <= 0 else keep).append(j)
-        self.jobs = keep
+            if j.chunks <= 0:
+                done.append(j)
+        self.jobs = [j for j in self.jobs if j not in done]
+        self.inflight = []
         return [(j, "info") for j in done]
 
 
