@@ -90,6 +90,8 @@ class _Simu:
         self.coarse_reuse = True
         # read the solver state back first near the previous solve's iteration count
         self.predict_polls = True
+        # largest aggregate of the two-level coarse space (None: 256 nodes in 2D, 1024 in 3D)
+        self.coarse_target = None
         self.last_solve = {}
         self.Bc_Init()
         model._Add_observer(self)
@@ -285,6 +287,20 @@ class _Simu:
         mask[self.Bc_dofs_Dirichlet(problemType)] = False
         return np.nonzero(~mask)[0], np.nonzero(mask)[0]
 
+    def _neumann_device(self, problemType) -> torch.Tensor:
+        """``_neumann_vector`` on the device, kept while the Neumann conditions
+        stay the same (a zero vector when there are none)."""
+        dofs, vals = self._bc_arrays(self._Bc_Neumann, problemType)
+        key = (dofs.tobytes(), vals.tobytes())
+        cache = self.__dict__.setdefault("_neumann_cache", {})
+        c = cache.get(problemType)
+        if c is None or c[0] != key:
+            Ndof = self.mesh.Nn * self.Get_dof_n(problemType)
+            v = torch.zeros(Ndof, dtype=torch.float64, device="cuda") if dofs.size == 0 else \
+                to_dev(self._neumann_vector(problemType))
+            c = cache[problemType] = (key, v)
+        return c[1]
+
     def _neumann_vector(self, problemType) -> np.ndarray:
         dofs, vals = self._bc_arrays(self._Bc_Neumann, problemType)
         Ndof = self.mesh.Nn * self.Get_dof_n(problemType)
@@ -318,12 +334,21 @@ class _Simu:
             cache[problemType] = (key, to_dev(known_np), to_dev(idx), idx)
         _, known, idx_dev, idx = cache[problemType]
         if idx.size:
-            x_dev.index_copy_(0, idx_dev, to_dev(xk[idx]))
+            # pinned staging, copied without a host wait; the buffer is rewritten only
+            # at this problem's next solve, after this solve (which follows the copy
+            # on the device) has finished
+            pins = self.__dict__.setdefault("_dirichlet_pinned", {})
+            pin = pins.get(problemType)
+            if pin is None or pin.numel() != idx.size:
+                pin = pins[problemType] = torch.empty(idx.size, dtype=torch.float64, pin_memory=True)
+            pin.numpy()[:] = xk[idx]
+            x_dev.index_copy_(0, idx_dev, pin.to("cuda", non_blocking=True))
         prev = self.last_solve.get(problemType)
         expected = prev.iterations if self.predict_polls and prev is not None and prev.status == 1 else None
         return SolveJob(A, b_dev, x_dev, known, self._mode(), self.tol_residual, self.tol_anorm,
                         self.replace_ratio, self.maxit, self.check_every, owner=self, precond=self.preconditioner,
-                        max_blocks=self.max_blocks, expected_iterations=expected)
+                        max_blocks=self.max_blocks, expected_iterations=expected,
+                        coarse_target=self.coarse_target)
 
     def _solve_system_gen(self, A, b_dev: torch.Tensor, x_dev: torch.Tensor, problemType):
         """Generator form of a solve: yields the ``SolveJob`` and receives its
@@ -433,7 +458,7 @@ class Elastic(_Simu):
 
     def _solve_gen(self):
         A = self._device_K()
-        b = to_dev(self._neumann_vector("elastic"))
+        b = self._neumann_device("elastic")
         x = self._u.clone()
         yield from self._solve_system_gen(A, b, x, "elastic")
         self._u = x
@@ -624,17 +649,33 @@ class PhaseField(_Simu):
             return 0.0
         return float(torch.dot(self._u, fu))
 
+    # device scalars of the same energies (no host wait): with zero d or u the
+    # products are exactly 0, the value the early returns above give
+    def _psi_crack_dev(self) -> torch.Tensor:
+        Kd = self._device_K("damage")
+        return 0.5 * torch.dot(self._d, Kd.spmv(self._d))
+
+    def _psi_elas_dev(self) -> torch.Tensor:
+        Ku = self._device_K("elastic")
+        return 0.5 * torch.dot(self._u, Ku.spmv(self._u))
+
+    def _energy_dev(self, convOption: int, fu: Optional[torch.Tensor]) -> torch.Tensor:
+        E = self._psi_crack_dev()
+        if convOption == 2:
+            E = E + self._psi_elas_dev() - torch.dot(self._u, fu)
+        return E
+
     # ------------------------------------------------------------ solves
     def _solve_damage_gen(self):
         Kd = self._device_K("damage")
-        b = self._Fd + to_dev(self._neumann_vector("damage"))
+        b = self._Fd + self._neumann_device("damage")
         x = self._d.clone()
         yield from self._solve_system_gen(Kd, b, x, "damage")
         self._d = x
 
     def _solve_elastic_gen(self):
         Ku = self._device_K("elastic")
-        b = to_dev(self._neumann_vector("elastic"))
+        b = self._neumann_device("elastic")
         x = self._u.clone()
         yield from self._solve_system_gen(Ku, b, x, "elastic")
         self._u = x
@@ -647,27 +688,25 @@ class PhaseField(_Simu):
         assert maxIter > 1, "Must be > 1."
         Niter, converged, convIter = 0, False, 0.0
         t0 = time.perf_counter()
-        if convOption == 2:
-            fu = to_dev(self._neumann_vector("elastic"))
+        fu = self._neumann_device("elastic") if convOption == 2 else None
+        # the convergence measure stays a device scalar unless the loop's
+        # control needs it (tolConv < 1): no host wait between the solves
         while not converged and Niter < maxIter:
             Niter += 1
             d_n, u_n = self._d, self._u
-            if convOption == 1:
-                E_n = self._Calc_Psi_Crack()
-            elif convOption == 2:
-                E_n = self._Calc_Psi_Crack() + self._Calc_Psi_Elas() - self._Calc_Psi_Ext(fu)
+            if convOption in (1, 2):
+                E_n = self._energy_dev(convOption, fu)
             yield from self._solve_damage_gen()
             self._updU = False
             yield from self._solve_elastic_gen()
             self._updD = False
             d_np1, u_np1 = self._d, self._u
             if convOption == 0:
-                convIter = float(torch.max(torch.abs(d_np1 - d_n)))
+                convIter = torch.max(torch.abs(d_np1 - d_n))
             elif convOption in (1, 2):
-                E_np1 = self._Calc_Psi_Crack()
-                if convOption == 2:
-                    E_np1 += self._Calc_Psi_Elas() - self._Calc_Psi_Ext(fu)
-                convIter = abs(E_n - E_np1) if E_np1 == 0 else abs((E_n - E_np1) / E_np1)
+                E_np1 = self._energy_dev(convOption, fu)
+                dE = torch.abs(E_n - E_np1)
+                convIter = torch.where(E_np1 == 0, dE, dE / torch.abs(E_np1))
             elif convOption == 3:
                 diffU = torch.abs(u_np1 - u_n)
                 nz = u_np1 != 0
@@ -682,11 +721,20 @@ class PhaseField(_Simu):
             elif convOption == 3:
                 converged = (convD <= tolConv) and (convU <= tolConv * 0.999)
             else:
-                converged = convIter <= tolConv
+                converged = float(convIter) <= tolConv
         self.Niter_staggered = Niter
-        self.convIter = convIter
+        self._convIter = convIter
         self.timeIter = time.perf_counter() - t0
         return converged
+
+    @property
+    def convIter(self) -> float:
+        c = self.__dict__.get("_convIter", 0.0)
+        return float(c)
+
+    @convIter.setter
+    def convIter(self, value: float) -> None:
+        self._convIter = value
 
     def Solve(self, tolConv=1.0, maxIter=500, convOption=2):
         """Staggered scheme (``_phasefield.py:303-432``): returns ``(u, d, converged)``."""

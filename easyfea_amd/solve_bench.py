@@ -41,7 +41,7 @@ def _timed_solves(simu, repeats):
     return np.array(lat)
 
 
-def run(workload: str, cells: int, repeats: int) -> dict:
+def run(workload: str, cells: int, repeats: int, coarse_target=None) -> dict:
     import torch
 
     from . import workloads
@@ -56,10 +56,11 @@ def run(workload: str, cells: int, repeats: int) -> dict:
             simu = make(solver)
             simu.preconditioner = precond
             simu.coarse_reuse = False  # every timed solve builds its own coarse inverse
+            simu.coarse_target = coarse_target
             dev = simu.dev
             A = dev.csr_map(simu.model.dim)
             if precond == "two-level":
-                A.coarse()
+                A.coarse(coarse_target)
             setup_s = time.perf_counter() - t0
             simu.Solve()  # warm-up: code objects, workspaces
             lat = _timed_solves(simu, repeats)
@@ -70,7 +71,9 @@ def run(workload: str, cells: int, repeats: int) -> dict:
                 value=ndof / lat.min(), p50_ms=float(np.percentile(lat, 50) * 1e3),
                 p99_ms=float(np.percentile(lat, 99) * 1e3), iterations=info.iterations,
                 rel_residual=info.rel_residual, replacements=info.replacements, mode=info.mode,
-                fallback=info.fallback, setup_s=setup_s, u=u, ndof=ndof)
+                fallback=info.fallback, setup_s=setup_s, u=u, ndof=ndof,
+                coarse_unknowns=A.coarse(coarse_target).nagg * (6 if simu.model.dim == 3 else 3)
+                if precond == "two-level" else None)
             del simu
             torch.cuda.empty_cache()
     ref_kind, ref_u = None, None
@@ -106,8 +109,10 @@ def main(argv=None):
     ap.add_argument("--workload", choices=["cantilever", "cube"], default="cantilever")
     ap.add_argument("--cells", type=int, default=64)
     ap.add_argument("--repeats", type=int, default=5)
+    ap.add_argument("--coarse-target", type=int, default=None,
+                    help="largest aggregate of the coarse space (default: 256 nodes in 2D, 1024 in 3D)")
     args = ap.parse_args(argv)
-    out = run(args.workload, args.cells, args.repeats)
+    out = run(args.workload, args.cells, args.repeats, args.coarse_target)
     for r in out["runs"]:
         r.pop("_u")
     print(json.dumps(out), flush=True)
