@@ -55,6 +55,10 @@ def test_abi_argument_errors_are_reported_without_gpu():
     assert lib.efea_solve_pcg2(64, 10, 2, 16, None, None, None, None, None, None, None, 0, None, None, None, None,
                                0, None, None, None, 0, 1e-12, 1e-9, 1e-3, 10, 32, 0, 0, None, None) == -1
     assert lib.efea_split_f32(-1, 2, None, None, None, None, None) == -1
+    assert lib.efea_step1_bench(1, 64, 10, 2, 4, None, None, None, None, None, 0, None, None, None, 3, None) == -1
+    assert "nagg" in _lib.last_error()
+    assert lib.efea_step1_bench_scratch_bytes(100, 2, 16) >= 2 * 100 * 2 * 8
+    assert lib.efea_batch_launch(None, 8, None) == -1 and lib.efea_batch_wait(None, None) == -1
     jid = C.c_int(5)
     assert lib.efea_batch_pop(None, C.byref(jid), None) == -1
     assert lib.efea_batch_destroy(None) == 0
