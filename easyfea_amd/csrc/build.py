@@ -47,7 +47,9 @@ def build(verbose: bool = False) -> str:
     if errors:
         raise RuntimeError("hipcc failed:\n" + "\n".join(errors))
     tmp = OUT + ".tmp"
-    link = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
+    # rocBLAS: DGEMMs of the blocked coarse-matrix inverse
+    link = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp, "-L/opt/rocm/lib", "-lrocblas",
+            "-Wl,-rpath,/opt/rocm/lib"]
     if verbose:
         print(" ".join(link), flush=True)
     subprocess.check_call(link)

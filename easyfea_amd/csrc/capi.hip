@@ -71,6 +71,7 @@ int batch_wait(void*, hipStream_t);
 int batch_pop(void*, int*, double*);
 int batch_active(void*);
 int coarse_ld(int, int);
+size_t coarse_scratch_doubles(int);
 int coarse_setup(int, int, const int*, const int*, const int*, const int*, const int*, const int*, const int*,
                  const double*, const unsigned char*, const double*, int, double*, double*, hipStream_t);
 size_t pcg2_workspace_bytes(int, int, int);
@@ -308,18 +309,20 @@ int efea_coarse_ld(int nagg, int dof_n) {
   return coarse_ld(nagg, dof_n);
 }
 
+size_t efea_coarse_scratch_bytes(int ld) { return ld > 0 ? 8 * coarse_scratch_doubles(ld) : 0; }
+
 int efea_coarse_setup(int dof_n, int nblocks, const int32_t* pair_ptr, const int32_t* pair_node,
                       const int32_t* pair_pos, const int32_t* blk_I, const int32_t* blk_J, const int32_t* node_ptr,
                       const int32_t* node_cols, const double* values64, const uint8_t* known, const double* rel,
-                      int ld, double* Ainv, double* pivot_scratch, void* stream) {
-  if (nblocks < 0 || ld <= 0 || !Ainv || !pivot_scratch ||
+                      int ld, double* Ainv, double* scratch, void* stream) {
+  if (nblocks < 0 || ld <= 0 || !Ainv || !scratch ||
       (nblocks > 0 && (!pair_ptr || !pair_node || !pair_pos || !blk_I || !blk_J || !node_ptr || !node_cols ||
                        !values64 || !rel))) {
     set_error("efea_coarse_setup: null pointer or negative size");
     return EFEA_ERR_ARG;
   }
   return coarse_setup(dof_n, nblocks, pair_ptr, pair_node, pair_pos, blk_I, blk_J, node_ptr, node_cols, values64,
-                      known, rel, ld, Ainv, pivot_scratch, STREAM(stream));
+                      known, rel, ld, Ainv, scratch, STREAM(stream));
 }
 
 size_t efea_pcg2_workspace_bytes(int Nnode, int dof_n, int nagg) { return pcg2_workspace_bytes(Nnode, dof_n, nagg); }

@@ -16,6 +16,11 @@
 //   * the prolongation kernel computes its rows zc_I = (Ac^-1)_I rc, adds
 //     P_I zc_I to z and rc_I . zc_I to the block's r.z partial.
 // Step 1 (SpMV and p update) is the one-level kernel unchanged.
+#include <rocblas/rocblas.h>
+
+#include <map>
+#include <mutex>
+
 #include "coarse_device.h"
 
 namespace efea {
@@ -28,6 +33,40 @@ constexpr int GB = 32;  // Gauss-Jordan block
 // Up to this many blocks per dimension the two-launch step (update forming
 // P A_kj per tile) wins: the launches, not the flops, bound it.
 constexpr int GJ_FUSED_MAX = 32;
+// Above this order the inverse takes NB-block steps with DGEMM panels and
+// updates (the GB-tile kernels re-stream the whole matrix once per 32 columns).
+constexpr int GJ_BLAS_MIN = 1024;
+constexpr int GJ_NB = 256;
+
+// Doubles of scratch efea_coarse_setup needs for an ld x ld coarse matrix.
+size_t coarse_scratch_doubles(int ld) {
+  size_t n = 2 * (size_t)GB * GB;
+  if (ld > GJ_BLAS_MIN) n += (size_t)GJ_NB * GJ_NB + 2 * (size_t)ld * GJ_NB;
+  return n;
+}
+
+// One rocBLAS handle per device, bound to the caller's stream at each use.
+static rocblas_handle blas_handle(hipStream_t s) {
+  static std::mutex mu;
+  static std::map<int, rocblas_handle> handles;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    set_error("coarse_setup: no device");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  rocblas_handle& h = handles[dev];
+  if (!h && rocblas_create_handle(&h) != rocblas_status_success) {
+    h = nullptr;
+    set_error("coarse_setup: rocblas_create_handle failed");
+    return nullptr;
+  }
+  if (rocblas_set_stream(h, s) != rocblas_status_success) {
+    set_error("coarse_setup: rocblas_set_stream failed");
+    return nullptr;
+  }
+  return h;
+}
 
 __global__ void k_fill_f64(int64_t n, double* __restrict__ a, double v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -168,13 +207,13 @@ __global__ void __launch_bounds__(64) k_gj_pivot(double* __restrict__ A, int ld,
 // row panel unscaled (PRESCALED false: the tile forms P A_kj itself) or
 // A_ij -= A_ik A_kj with the row panel already scaled (PRESCALED true).
 template <bool PRESCALED>
-__global__ void __launch_bounds__(256) k_gj_update(double* __restrict__ A, int ld, int k,
+__global__ void __launch_bounds__(256) k_gj_update(double* __restrict__ A, int ld, int n, int k,
                                                    const double* __restrict__ Pb) {
   __shared__ double Ps[GB][GB + 1];
   __shared__ double L[GB][GB + 1];
   __shared__ double R[GB][GB + 1];
   __shared__ double T[GB][GB + 1];
-  const int nbk = ld / GB;
+  const int nbk = n / GB;
   int ib = blockIdx.y, jb = blockIdx.x;
   if (ib >= k) ++ib;
   if (jb >= k) ++jb;
@@ -212,11 +251,11 @@ __global__ void __launch_bounds__(256) k_gj_update(double* __restrict__ A, int l
 // 0 .. nbk-2 row tiles, nbk-1 .. 2nbk-3 column tiles, then the pivot block;
 // WHICH 1: row tiles only; WHICH 2: column tiles, then the pivot block.
 template <int WHICH>
-__global__ void __launch_bounds__(256) k_gj_panels(double* __restrict__ A, int ld, int k,
+__global__ void __launch_bounds__(256) k_gj_panels(double* __restrict__ A, int ld, int n, int k,
                                                    const double* __restrict__ Pb, double* __restrict__ Pnext) {
   __shared__ double Ps[GB][GB + 1];
   __shared__ double T[GB][GB + 1];
-  const int nbk = ld / GB;
+  const int nbk = n / GB;
   const int nrow = WHICH == 2 ? 0 : nbk - 1, ncol = WHICH == 1 ? 0 : nbk - 1;
   const int b = blockIdx.x;
   if (b >= nrow + ncol) {
@@ -259,6 +298,79 @@ __global__ void k_symmetrize(double* __restrict__ A, int ld) {
   A[(int64_t)j * ld + i] = v;
 }
 
+// In-place inverse of the n x n SPD block at A (leading dimension ld, n a
+// multiple of GB) with the GB x GB kernels.  Pb: 2 GB^2 doubles (two pivot
+// inverses used in turn).
+static void gj_invert(double* A, int ld, int n, double* Pb, hipStream_t s) {
+  const int nbk = n / GB;
+  hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(64), 0, s, A, ld, 0, Pb);
+  for (int k = 0; k < nbk; ++k) {
+    double* Pk = Pb + (k & 1) * GB * GB;
+    double* Pn = Pb + ((k + 1) & 1) * GB * GB;
+    if (nbk > 1 && nbk <= GJ_FUSED_MAX) {
+      // two launches per step: the update forms P A_kj itself
+      hipLaunchKernelGGL(k_gj_update<false>, dim3(nbk - 1, nbk - 1), dim3(256), 0, s, A, ld, n, k, (const double*)Pk);
+      hipLaunchKernelGGL(k_gj_panels<0>, dim3(2 * (nbk - 1) + 1), dim3(256), 0, s, A, ld, n, k, (const double*)Pk,
+                         Pn);
+    } else if (nbk > 1) {
+      // large matrices: scale the row panel once (half the update flops)
+      hipLaunchKernelGGL(k_gj_panels<1>, dim3(nbk - 1), dim3(256), 0, s, A, ld, n, k, (const double*)Pk, Pn);
+      hipLaunchKernelGGL(k_gj_update<true>, dim3(nbk - 1, nbk - 1), dim3(256), 0, s, A, ld, n, k, (const double*)Pk);
+      hipLaunchKernelGGL(k_gj_panels<2>, dim3(nbk), dim3(256), 0, s, A, ld, n, k, (const double*)Pk, Pn);
+    }
+  }
+}
+
+// Large coarse matrices: Gauss-Jordan with NB x NB block steps whose panel
+// products and trailing update are rocBLAS DGEMMs.  The buffer is read as a
+// column-major matrix B; the matrix is symmetric, so inverting B inverts A.
+// Step k (rows/columns kb of size nb): P = inv(B_kk) (GB kernels on a copy),
+// C = B(:, kb), W = P B(kb, :), B -= C W, then B(kb, :) = W,
+// B(:, kb) = -C P, B_kk = P.  Scratch: Pb (2 GB^2), P (NB^2), C (ld x NB),
+// W (NB x ld) doubles.
+static int gj_invert_blocked(double* B, int ld, double* scratch, hipStream_t s) {
+  rocblas_handle h = blas_handle(s);
+  if (!h) return EFEA_ERR_HIP;
+  double* Pb = scratch;
+  double* P = Pb + 2 * GB * GB;
+  double* Cw = P + (size_t)GJ_NB * GJ_NB;
+  double* Ww = Cw + (size_t)ld * GJ_NB;
+  const double one = 1.0, minus = -1.0, zero = 0.0;
+  for (int k0 = 0; k0 < ld; k0 += GJ_NB) {
+    const int nb = ld - k0 < GJ_NB ? ld - k0 : GJ_NB;  // a multiple of GB
+    double* Bkk = B + (size_t)k0 * ld + k0;
+    if (hipMemcpy2DAsync(P, nb * sizeof(double), Bkk, ld * sizeof(double), nb * sizeof(double), nb,
+                         hipMemcpyDeviceToDevice, s) != hipSuccess ||
+        hipMemcpy2DAsync(Cw, ld * sizeof(double), B + (size_t)k0 * ld, ld * sizeof(double), ld * sizeof(double), nb,
+                         hipMemcpyDeviceToDevice, s) != hipSuccess) {
+      set_error("coarse_setup: panel copy failed");
+      return EFEA_ERR_HIP;
+    }
+    gj_invert(P, nb, nb, Pb, s);
+    // W = P B(kb, :)                     (nb x ld)
+    // B = B - C W                        (ld x ld)
+    // B(:, kb) = -C P                    (ld x nb)
+    if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nb, ld, nb, &one, P, nb, B + k0, ld, &zero,
+                      Ww, nb) != rocblas_status_success ||
+        rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, ld, ld, nb, &minus, Cw, ld, Ww, nb, &one, B,
+                      ld) != rocblas_status_success ||
+        rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, ld, nb, nb, &minus, Cw, ld, P, nb, &zero,
+                      B + (size_t)k0 * ld, ld) != rocblas_status_success) {
+      set_error("coarse_setup: rocblas_dgemm failed");
+      return EFEA_ERR_HIP;
+    }
+    // B(kb, :) = W, then B_kk = P
+    if (hipMemcpy2DAsync(B + k0, ld * sizeof(double), Ww, nb * sizeof(double), nb * sizeof(double), ld,
+                         hipMemcpyDeviceToDevice, s) != hipSuccess ||
+        hipMemcpy2DAsync(Bkk, ld * sizeof(double), P, nb * sizeof(double), nb * sizeof(double), nb,
+                         hipMemcpyDeviceToDevice, s) != hipSuccess) {
+      set_error("coarse_setup: panel copy failed");
+      return EFEA_ERR_HIP;
+    }
+  }
+  return EFEA_OK;
+}
+
 template <int DN>
 static int coarse_setup_t(int ncb, const int* pair_ptr, const int* pair_node, const int* pair_pos,
                           const int* blk_I, const int* blk_J, const int* node_ptr, const int* node_cols,
@@ -270,22 +382,10 @@ static int coarse_setup_t(int ncb, const int* pair_ptr, const int* pair_node, co
     hipLaunchKernelGGL((k_coarse_assemble<DN>), dim3((unsigned)((ncb + 3) / 4)), dim3(256), 0, s, ncb, pair_ptr,
                        pair_node, pair_pos, blk_I, blk_J, node_ptr, node_cols, vals, known, rel, Ac, ld);
   hipLaunchKernelGGL(k_coarse_guard, dim3((unsigned)((ld + 255) / 256)), dim3(256), 0, s, ld, Ac);
-  const int nbk = ld / GB;
-  // Pb holds two GB x GB pivot inverses used in turn
-  hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(64), 0, s, Ac, ld, 0, Pb);
-  for (int k = 0; k < nbk; ++k) {
-    double* Pk = Pb + (k & 1) * GB * GB;
-    double* Pn = Pb + ((k + 1) & 1) * GB * GB;
-    if (nbk > 1 && nbk <= GJ_FUSED_MAX) {
-      // two launches per step: the update forms P A_kj itself
-      hipLaunchKernelGGL(k_gj_update<false>, dim3(nbk - 1, nbk - 1), dim3(256), 0, s, Ac, ld, k, (const double*)Pk);
-      hipLaunchKernelGGL(k_gj_panels<0>, dim3(2 * (nbk - 1) + 1), dim3(256), 0, s, Ac, ld, k, (const double*)Pk, Pn);
-    } else if (nbk > 1) {
-      // large matrices: scale the row panel once (half the update flops)
-      hipLaunchKernelGGL(k_gj_panels<1>, dim3(nbk - 1), dim3(256), 0, s, Ac, ld, k, (const double*)Pk, Pn);
-      hipLaunchKernelGGL(k_gj_update<true>, dim3(nbk - 1, nbk - 1), dim3(256), 0, s, Ac, ld, k, (const double*)Pk);
-      hipLaunchKernelGGL(k_gj_panels<2>, dim3(nbk), dim3(256), 0, s, Ac, ld, k, (const double*)Pk, Pn);
-    }
+  if (ld > GJ_BLAS_MIN) {
+    if (int e = gj_invert_blocked(Ac, ld, Pb, s)) return e;
+  } else {
+    gj_invert(Ac, ld, ld, Pb, s);
   }
   hipLaunchKernelGGL(k_symmetrize, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, Ac, ld);
   return check_launch("coarse_setup");

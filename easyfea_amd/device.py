@@ -198,14 +198,15 @@ class CoarseSpace:
         self.pair_pos = to_dev(cp.pair_pos)
         self.blk_I = to_dev(cp.blk_I)
         self.blk_J = to_dev(cp.blk_J)
-        self.pivot = torch.empty(2048, dtype=torch.float64, device="cuda")
+        self.scratch = torch.empty(_lib.load().efea_coarse_scratch_bytes(self.ld) // 8, dtype=torch.float64,
+                                   device="cuda")
 
     def setup(self, A: "DeviceCsr", known: Optional[torch.Tensor]) -> torch.Tensor:
         """``Ac^-1`` (ld x ld, FP64) of ``A``'s current FP64 values."""
         Ainv = torch.empty(self.ld * self.ld, dtype=torch.float64, device="cuda")
         _lib.call("efea_coarse_setup", self.dof_n, self.ncb, ptr(self.pair_ptr), ptr(self.pair_node),
                   ptr(self.pair_pos), ptr(self.blk_I), ptr(self.blk_J), ptr(A.node_ptr), ptr(A.node_cols),
-                  ptr(A.values), ptr(known), ptr(self.rel), self.ld, ptr(Ainv), ptr(self.pivot), stream_ptr())
+                  ptr(A.values), ptr(known), ptr(self.rel), self.ld, ptr(Ainv), ptr(self.scratch), stream_ptr())
         return Ainv
 
 

@@ -159,16 +159,19 @@ int efea_solve_pcg(int mode, int Nnode, int dof_n, int group, const int32_t* nod
  * efea_coarse_setup assembles Ac = P^T A P from (node, neighbour position)
  * pairs grouped by aggregate block (blk_I[k], blk_J[k]) = pairs
  * pair_ptr[k]..pair_ptr[k+1]-1, and inverts it in place (blocked Gauss-Jordan,
- * FP64) into Ainv (ld x ld, ld = efea_coarse_ld; pivot_scratch: 2048 doubles).
+ * FP64; above order 1024 with 256-column steps whose products are rocBLAS
+ * DGEMMs) into Ainv (ld x ld, ld = efea_coarse_ld; scratch:
+ * efea_coarse_scratch_bytes(ld) bytes).
  * efea_solve_pcg2 is efea_solve_pcg with that preconditioner; agg_ptr /
  * agg_nodes list the nodes of each aggregate.  EasyFEA solves the reduced
  * system directly (Solvers.py:225-394); the preconditioner changes the
  * iteration count, not the solution. */
 int efea_coarse_ld(int nagg, int dof_n);
+size_t efea_coarse_scratch_bytes(int ld);
 int efea_coarse_setup(int dof_n, int nblocks, const int32_t* pair_ptr, const int32_t* pair_node,
                       const int32_t* pair_pos, const int32_t* blk_I, const int32_t* blk_J, const int32_t* node_ptr,
                       const int32_t* node_cols, const double* values64, const uint8_t* known, const double* rel,
-                      int ld, double* Ainv, double* pivot_scratch, void* stream);
+                      int ld, double* Ainv, double* scratch, void* stream);
 size_t efea_pcg2_workspace_bytes(int Nnode, int dof_n, int nagg);
 int efea_solve_pcg2(int mode, int Nnode, int dof_n, int group, const int32_t* node_ptr, const int32_t* node_cols,
                     const double* values64, const float* values32, const float* corr32, const uint8_t* known,
