@@ -164,7 +164,7 @@ __device__ inline void pcg2_prolong_body(CoarseArgs C, const unsigned char* __re
                double* __restrict__ part2, PcgState* st, int blk) {
   constexpr int NM = Modes<DN>::NM;
   // the state test follows the loads and the coarse rows
-  __shared__ double zcs[NM];
+  __shared__ double shz[NM][BS / 64];
   const int nc = C.nagg * NM;
   const int k0 = C.agg_ptr[blk], k1 = C.agg_ptr[blk + 1];
   // the first node's data is loaded before the coarse rows
@@ -182,19 +182,36 @@ __device__ inline void pcg2_prolong_body(CoarseArgs C, const unsigned char* __re
       kfree[a] = !(known && known[(int64_t)nf * DN + a]);
     }
   }
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int m = w; m < NM; m += BS / 64) {  // one wavefront per coarse row
-    const double* row = C.Ainv + (int64_t)(blk * NM + m) * C.ld;
-    double s = 0;
-    for (int c = lane; c < nc; c += 64) s += row[c] * C.rc[c];
-    s = warp_sum(s);
-    if (lane == 0) zcs[m] = s;
-  }
-  __syncthreads();
-  if (st->done) return;
+  // The aggregate's NM rows of Ac^-1 times rc: every thread takes columns
+  // c = t, t + BS, ... of all NM rows (one rc load serves NM products), four
+  // columns per pass with their loads issued together, then one block
+  // reduction of the NM sums (a wavefront per row walking the row alone paid
+  // one load latency per 64 columns).
+  const double* __restrict__ rows = C.Ainv + (int64_t)(blk * NM) * C.ld;
   double zc[NM];
 #pragma unroll
-  for (int m = 0; m < NM; ++m) zc[m] = zcs[m];
+  for (int m = 0; m < NM; ++m) zc[m] = 0;
+  int c = threadIdx.x;
+  for (; c + 3 * BS < nc; c += 4 * BS) {
+    double rv[4], av[4][NM];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) rv[u] = C.rc[c + u * BS];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int m = 0; m < NM; ++m) av[u][m] = rows[(int64_t)m * C.ld + c + u * BS];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int m = 0; m < NM; ++m) zc[m] += av[u][m] * rv[u];
+  }
+  for (; c < nc; c += BS) {
+    const double rv = C.rc[c];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) zc[m] += rows[(int64_t)m * C.ld + c] * rv;
+  }
+  block_sum_all<NM, BS>(zc, shz);
+  if (st->done) return;
   if (have) {
 #pragma unroll
     for (int a = 0; a < DN; ++a) {
