@@ -26,6 +26,14 @@ from .mesh import tri3_notched_square
 from .serving import Request, Server, estimate_bytes, percentiles
 
 GIB = 2**30
+_MESHES = {}  # one mesh cache per generator, shared by the runs of a process
+
+
+def _mesh_cache(maker):
+    c = _MESHES.get(maker)
+    if c is None:
+        c = _MESHES[maker] = workloads.MeshCache(maker)
+    return c
 
 
 def _elastic_requests(specs, meshes, mode, max_blocks=0):
@@ -59,7 +67,7 @@ def run_elastic(args) -> dict:
     import torch
 
     specs = workloads.serving_elastic_specs(args.requests, args.qps, args.side_lo, args.side_hi, args.seed)
-    meshes = workloads.MeshCache(workloads.square_quad4)
+    meshes = _mesh_cache(workloads.square_quad4)
     setup_s = _prepare_meshes(meshes, [s.side for s in specs], (2,))
     # warm-up: the whole stream once (code objects, pinned buffers, and the
     # device allocator's cache of every buffer size the stream uses)
@@ -123,7 +131,7 @@ def run_phasefield(args) -> dict:
     import torch
 
     specs = workloads.serving_phasefield_specs(args.realisations, args.steps, seed=args.seed)
-    meshes = workloads.MeshCache(tri3_notched_square)
+    meshes = _mesh_cache(tri3_notched_square)
     setup_s = _prepare_meshes(meshes, [s.cells for s in specs], (2, 1))
     clock = time.perf_counter
 
@@ -168,6 +176,7 @@ def run_phasefield(args) -> dict:
         "workload": f"{len(specs)} concurrent phase-field realisations, notched square TRI3, Ne U[20k, 80k], "
                     f"lognormal Gc field (CoV 0.15), Amor/AT2, {args.steps} x 1e-4 mm increments",
         "mode": args.mode,
+        "max_blocks": args.max_blocks,
         "value": ndof / wall,
         "unit": "DOF/s",
         "wall_s": wall,
@@ -208,12 +217,15 @@ def main(argv=None):
     ap.add_argument("--capacity", type=int, default=256)
     ap.add_argument("--check-every", type=int, default=32)
     ap.add_argument("--hbm-gib", type=float, default=200.0)
-    ap.add_argument("--max-blocks", type=int, default=0,
-                    help="grid cap per system of the solver kernels (0: the library default)")
+    ap.add_argument("--max-blocks", type=int, default=None,
+                    help="grid cap per system of the solver kernels (0: the library default, 1024); default: "
+                         "64 for the phase-field batch (throughput), 0 for the elastic stream (latency)")
     ap.add_argument("--compare-fp64", action="store_true")
     ap.add_argument("--adaptive", action="store_true",
                     help="keep FP64 Krylov products for matrices under 128 MiB (single-solve policy)")
     args = ap.parse_args(argv)
+    if args.max_blocks is None:
+        args.max_blocks = 64 if args.workload == "phasefield" else 0
     import torch
 
     from . import device
