@@ -489,14 +489,106 @@ class Elastic(_Simu):
             self._u = to_dev(r["displacement"])
         return r
 
-    def Result(self, option: str) -> np.ndarray:
-        u = self.displacement.reshape(-1, self.model.dim)
-        comp = {"ux": 0, "uy": 1, "uz": 2}
-        if option in comp:
-            return u[:, comp[option]].copy()
-        if option == "displacement":
-            return self.displacement
-        raise ValueError(f"unknown result {option}")
+    # ------------------------------------------------------------ results
+    def Results_Available(self) -> list:
+        """``Elastic.Results_Available`` (``Simulations/_elastic.py:203-229``;
+        the static solve has zero speed and acceleration, ZZ1 is not provided)."""
+        dim = self.model.dim
+        out = ["displacement", "displacement_norm", "displacement_matrix", "speed", "speed_norm", "accel",
+               "accel_norm"]
+        comps = ["xx", "yy", "xy"] if dim == 2 else ["xx", "yy", "zz", "yz", "xz", "xy"]
+        axes = ["x", "y"] if dim == 2 else ["x", "y", "z"]
+        out += ["u" + a for a in axes] + ["v" + a for a in axes] + ["a" + a for a in axes]
+        out += ["S" + c for c in comps] + ["E" + c for c in comps]
+        out += ["Svm", "Stress", "Evm", "Strain", "Wdef", "Wdef_e"]
+        return out
+
+    def _strain_stress_e(self) -> torch.Tensor:
+        """(Ne, 2 ns + 3) per element on the device: strain components, Evm,
+        stress components, Svm, W_e (``efea_elastic_strain_stress``)."""
+        dev = self.dev
+        ns = 3 if self.model.dim == 2 else 6
+        out = torch.empty((dev.Ne, 2 * ns + 3), dtype=torch.float64, device="cuda")
+        tab, nPg = dev.tables(MatrixType.rigi)
+        E = to_dev(self.model._E_array())
+        _lib.call("efea_elastic_strain_stress", dev.code, dev.Ne, ptr(dev.connect), ptr(dev.coord), ptr(tab), nPg,
+                  ptr(self._u), ptr(E), 1 if self.model.isHeterogeneous else 0, self.model.v,
+                  int(self.model.planeStress), self.model.thickness, ptr(out), stream_ptr())
+        return out
+
+    def _reshape_values(self, values, nodeValues: bool) -> np.ndarray:
+        """``Results_Reshape_values`` (``Simulations/_simu.py:3095-3153``):
+        element values go to the nodes by ``Mesh.Get_Node_Values`` (on the
+        device), node values to the elements as the mean over their nodes."""
+        Nn, Ne = self.mesh.Nn, self.dev.Ne
+        is_t = isinstance(values, torch.Tensor)
+        size = values.numel() if is_t else np.asarray(values).size
+        is1d = values.dim() == 1 if is_t else np.asarray(values).ndim == 1
+        if nodeValues:
+            if size % Nn == 0:
+                v = values.cpu().numpy() if is_t else np.asarray(values)
+                return v.ravel() if is1d else v.reshape(Nn, -1)
+            if size % Ne == 0:
+                vt = values if is_t else to_dev(np.asarray(values, dtype=np.float64))
+                vn = self.dev.node_values(vt.reshape(Ne, -1)).cpu().numpy()
+                return vn.ravel() if is1d else vn
+        else:
+            if size % Ne == 0:
+                v = values.cpu().numpy() if is_t else np.asarray(values)
+                return v.ravel() if is1d else v.reshape(Ne, -1)
+            if size % Nn == 0:
+                v = (values.cpu().numpy() if is_t else np.asarray(values)).reshape(Nn, -1)
+                ve = v[self.dev.connect_np].mean(axis=1)
+                return ve.ravel() if is1d else ve
+        raise Exception("Unexpected conditions occurred during the calculation.")
+
+    def Result(self, result: str, nodeValues: bool = True, iter: Optional[int] = None):
+        """``Elastic.Result`` (``Simulations/_elastic.py:231-321``): node (or,
+        with ``nodeValues=False``, element) values of ``result``; ``Wdef`` is
+        a float.  Strain and stress results come from the device kernel."""
+        if iter is not None:
+            self.Set_Iter(iter)
+        if result not in self.Results_Available():
+            print(f"\n The result '{result}' is not available.\n Available results: {self.Results_Available()}")
+            return None
+        dim, Nn = self.model.dim, self.mesh.Nn
+        axes = {"x": 0, "y": 1, "z": 2}
+        if result in ("ux", "uy", "uz"):
+            values = self.displacement.reshape(Nn, -1)[:, axes[result[1]]]
+        elif result == "displacement":
+            values = self.displacement
+        elif result == "displacement_norm":
+            values = np.linalg.norm(self.displacement.reshape(Nn, -1), axis=1)
+        elif result == "displacement_matrix":
+            values = np.zeros((Nn, 3))
+            values[:, :dim] = self.displacement.reshape(Nn, -1)
+        elif result[0] in "va" and result[1:] in axes or result in ("speed", "accel"):
+            # static solve: no speed or acceleration
+            values = np.zeros(Nn) if len(result) == 2 else np.zeros(Nn * dim)
+        elif result in ("speed_norm", "accel_norm"):
+            values = np.zeros(Nn)
+        elif result == "Wdef":
+            return float(self._strain_stress_e()[:, -1].sum())
+        else:
+            ns = 3 if dim == 2 else 6
+            e = self._strain_stress_e()
+            comps = ["xx", "yy", "xy"] if dim == 2 else ["xx", "yy", "zz", "yz", "xz", "xy"]
+            if result == "Wdef_e":
+                values = e[:, -1]
+            elif result == "Strain":
+                values = e[:, :ns]
+            elif result == "Stress":
+                values = e[:, ns + 1:2 * ns + 1]
+            elif result == "Evm":
+                values = e[:, ns]
+            elif result == "Svm":
+                values = e[:, 2 * ns + 1]
+            elif result[0] == "E":
+                values = e[:, comps.index(result[1:])]
+            else:
+                values = e[:, ns + 1 + comps.index(result[1:])]
+            values = values.contiguous()
+        return self._reshape_values(values, nodeValues)
 
 
 class PhaseField(_Simu):

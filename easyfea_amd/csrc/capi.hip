@@ -46,6 +46,9 @@ int solve_pcg(int, int, int, int, const int*, const int*, const double*, const f
               const unsigned char*, const void*, const double*, double*, void*, double, double, double, int, int,
               int, int, hipStream_t, double*);
 int split_f32(int, int, const int*, const double*, float*, float*, hipStream_t);
+int strain_stress(int, int64_t, const int*, const double*, const double*, int, const double*, const double*, int,
+                  double, int, double, double*, hipStream_t);
+int node_average(int, int, const int*, const int*, const double*, double*, hipStream_t);
 int step1_bench(int, int, int, int, const int*, const int*, const void*, const float*, const unsigned char*,
                 const double*, double*, void*, int, hipStream_t);
 int pcg2_step1_bench(int, int, int, int, const int*, const int*, const void*, const float*, const unsigned char*, int,
@@ -100,6 +103,26 @@ int efea_elastic_ke(int elem_type, int precision, int64_t Ne, const int32_t* con
   }
   return elastic_ke(elem_type, precision, Ne, connect, coord, tables, nPg, E, E_stride, nu, plane_stress,
                     thickness, Ke, STREAM(stream));
+}
+
+int efea_elastic_strain_stress(int elem_type, int64_t Ne, const int32_t* connect, const double* coord,
+                               const double* tables, int nPg, const double* u, const double* E, int E_stride,
+                               double nu, int plane_stress, double thickness, double* out, void* stream) {
+  if (Ne < 0 || (Ne > 0 && (!connect || !coord || !tables || !u || !E || !out))) {
+    set_error("efea_elastic_strain_stress: null pointer or negative size");
+    return EFEA_ERR_ARG;
+  }
+  return strain_stress(elem_type, Ne, connect, coord, tables, nPg, u, E, E_stride, nu, plane_stress, thickness, out,
+                       STREAM(stream));
+}
+
+int efea_node_average(int Nn, int ncol, const int32_t* node_ptr, const int32_t* node_elems, const double* values_e,
+                      double* values_n, void* stream) {
+  if (Nn < 0 || ncol < 1 || (Nn > 0 && (!node_ptr || !node_elems || !values_e || !values_n))) {
+    set_error("efea_node_average: null pointer, negative size or ncol < 1");
+    return EFEA_ERR_ARG;
+  }
+  return node_average(Nn, ncol, node_ptr, node_elems, values_e, values_n, STREAM(stream));
 }
 
 int efea_reaction_diffusion_ke(int elem_type, int precision, int64_t Ne, const int32_t* connect,

@@ -100,6 +100,29 @@ class DeviceMesh:
             t[mt] = (to_dev(table_block(self.elemType, mt)), gauss_points(self.elemType, mt)[1].size)
         return t[mt]
 
+    def node_incidence(self):
+        """(node_ptr, node_elems) int32 on the device: the elements holding
+        each node, in increasing element order (the node -> element incidence
+        of ``_GroupElem.Get_connect_n_e``)."""
+        c = self._shared.__dict__.get("incidence")
+        if c is None:
+            Ne, nPe = self.connect_np.shape
+            rows = self.connect_np.ravel().astype(np.int64)
+            cols = np.repeat(np.arange(Ne, dtype=np.int64), nPe)
+            order = np.lexsort((cols, rows))
+            ptr_ = np.zeros(self.Nn + 1, dtype=np.int32)
+            ptr_[1:] = np.cumsum(np.bincount(rows, minlength=self.Nn))
+            c = self._shared.__dict__["incidence"] = (to_dev(ptr_), to_dev(cols[order].astype(np.int32)))
+        return c
+
+    def node_values(self, values_e: torch.Tensor) -> torch.Tensor:
+        """Element values (Ne, k) -> node values (Nn, k) (``Mesh.Get_Node_Values``)."""
+        v = values_e.reshape(self.Ne, -1).contiguous()
+        out = torch.empty((self.Nn, v.shape[1]), dtype=torch.float64, device="cuda")
+        n_ptr, n_el = self.node_incidence()
+        _lib.call("efea_node_average", self.Nn, v.shape[1], ptr(n_ptr), ptr(n_el), ptr(v), ptr(out), stream_ptr())
+        return out
+
     def pattern(self, dof_n: int) -> "CsrPattern":
         p = self._shared.patterns
         if dof_n not in p:

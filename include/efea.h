@@ -48,6 +48,24 @@ int efea_elastic_ke(int elem_type, int precision, int64_t Ne, const int32_t* con
                     const double* tables, int nPg, const double* E, int E_stride, double nu, int plane_stress,
                     double thickness, void* Ke, void* stream);
 
+/* Linear-elastic post-processing per element (Simulations/_elastic.py:229-380,
+ * Models/Elastic/_laws.py:127-185, Models/_utils.py:302-434): at each stiffness
+ * integration point eps = B u_e (Kelvin-Mandel) and sig = C eps (isotropic,
+ * E per element when E_stride = 1); out (Ne, 2 ns + 3) FP64 holds per element
+ * [strain tensor components (ns, shear without the sqrt 2), Evm, stress
+ * components (ns), Svm, W_e] - components and von Mises measures averaged over
+ * the points with equal weights, W_e = 1/2 sum_p wJ sig.eps (x thickness in
+ * 2D).  ns = 3 in 2D [xx, yy, xy], 6 in 3D [xx, yy, zz, yz, xz, xy]. */
+int efea_elastic_strain_stress(int elem_type, int64_t Ne, const int32_t* connect, const double* coord,
+                               const double* tables, int nPg, const double* u, const double* E, int E_stride,
+                               double nu, int plane_stress, double thickness, double* out, void* stream);
+/* Element -> node values (Mesh.Get_Node_Values, FEM/_mesh.py:822-873): for node n,
+ * the mean of values_e (Ne, ncol) over the elements listed in
+ * node_elems[node_ptr[n] .. node_ptr[n+1]) (increasing element order); 0 for a
+ * node of no element. */
+int efea_node_average(int Nn, int ncol, const int32_t* node_ptr, const int32_t* node_elems, const double* values_e,
+                      double* values_n, void* stream);
+
 /* Scalar reaction-diffusion-source element operators:
  * K_e = sum_mass r wJ N^T N + k sum_rigi wJ dN^T dN, F_e = sum_mass f wJ N, times scale.
  * r: (Ne, nPg_mass) or NULL (then r_const); k: k[e*k_stride] or NULL (no diffusion);

@@ -718,6 +718,86 @@ def pf_g(d, connect, N_pg, k_res=1e-12):
 
 
 # ---------------------------------------------------------------------------
+# Post-processing of linear-elastic results (section 8(f), rank 1)
+# ---------------------------------------------------------------------------
+
+
+def strain_e_pg(geo: Geometry, u: np.ndarray, connect: np.ndarray) -> np.ndarray:
+    """``Calc_Epsilon_e_pg`` (``Models/Elastic/_laws.py:127-157``):
+    ``B_e_pg @ u_e`` (Kelvin-Mandel) at the rule's points, ``(Ne, nPg, ns)``."""
+    ue = u[assembly_e(connect, geo.dim)]
+    return np.einsum("epij,ej->epi", geo.B(), ue)
+
+
+def stress_e_pg(eps: np.ndarray, C: np.ndarray) -> np.ndarray:
+    """``Calc_Sigma_e_pg`` (``_laws.py:159-185``): ``C @ eps`` with ``C``
+    ``(ns, ns)`` or ``(Ne, ns, ns)``."""
+    C = np.asarray(C, dtype=float)
+    if C.ndim == 2:
+        return np.einsum("ij,epj->epi", C, eps)
+    return np.einsum("eij,epj->epi", C, eps)
+
+
+def result_in_field(field_e_pg: np.ndarray, result: str, coef=SQRT2) -> np.ndarray:
+    """``__Result_in_Strain_or_Stress_field`` (``Models/_utils.py:302-392``):
+    the shear components divided by ``coef``, then the component, the von
+    Mises measure or the whole field at every point."""
+    f = np.array(field_e_pg, dtype=float, copy=True)
+    ns = f.shape[-1]
+    if ns == 3:
+        f[:, :, 2] *= 1 / coef
+        xx, yy, xy = (f[:, :, i] for i in range(3))
+        comps = {"xx": xx, "yy": yy, "xy": xy}
+        vm = np.sqrt(xx**2 + yy**2 - xx * yy + 3 * xy**2)
+    else:
+        f[:, :, 3:] *= 1 / coef
+        xx, yy, zz, yz, xz, xy = (f[:, :, i] for i in range(6))
+        comps = {"xx": xx, "yy": yy, "zz": zz, "yz": yz, "xz": xz, "xy": xy}
+        vm = np.sqrt(0.5 * ((xx - yy) ** 2 + (yy - zz) ** 2 + (zz - xx) ** 2 + 6 * (xy**2 + yz**2 + xz**2)))
+    for k, v in comps.items():
+        if k in result:
+            return v
+    if "vm" in result:
+        return vm
+    if result in ("Strain", "Stress"):
+        return f
+    raise ValueError(result)
+
+
+def strain_or_stress_result_e(field_e_pg: np.ndarray, result: str, coef=SQRT2) -> np.ndarray:
+    """``Result_strain_or_stress_field_e`` (``Models/_utils.py:395-434``) for
+    one group: the point values averaged over the points, ``(Ne,)`` or
+    ``(Ne, ns)``."""
+    return np.asarray(result_in_field(field_e_pg, result, coef)).mean(1)
+
+
+def psi_elas_e(geo: Geometry, eps: np.ndarray, sig: np.ndarray, thickness: float = 1.0) -> np.ndarray:
+    """``Elastic._Calc_Psi_Elas(returnScalar=False)`` (``Simulations/_elastic.py:323-380``):
+    ``1/2 sum_p wJ sig . eps`` per element, times the thickness in 2D."""
+    w = 0.5 * np.einsum("ep,epi,epi->e", geo.wJ, sig, eps)
+    return w * thickness if geo.dim == 2 else w
+
+
+def node_values(connect: np.ndarray, Nn: int, values_e: np.ndarray) -> np.ndarray:
+    """``Mesh.Get_Node_Values`` (``FEM/_mesh.py:822-873``) with the incidence of
+    ``_GroupElem.Get_connect_n_e`` (``FEM/_group_elem.py:322-344``): per node,
+    the sum of the surrounding elements' values divided by their count; nodes
+    of no element stay 0."""
+    Ne, nPe = connect.shape
+    v = np.asarray(values_e, dtype=float)
+    one_d = v.ndim == 1
+    v = v.reshape(Ne, -1)
+    G = sp.csr_matrix((np.ones(nPe * Ne), (connect.ravel(), np.repeat(np.arange(Ne), nPe))), shape=(Nn, Ne))
+    nodes = np.unique(connect)
+    Gn = G[nodes]
+    count = np.asarray(Gn.sum(axis=1)).reshape(-1, 1)
+    out = np.zeros((Nn, v.shape[1]))
+    for c in range(v.shape[1]):
+        out[nodes, c] = ((Gn @ v[:, c].reshape(Ne, 1)) * 1 / count).ravel()
+    return out.ravel() if one_d else out
+
+
+# ---------------------------------------------------------------------------
 # Simulations: the driver-level semantics
 # ---------------------------------------------------------------------------
 

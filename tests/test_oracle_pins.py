@@ -470,3 +470,106 @@ def test_history_field_monotone_and_elastic_limit():
     pf2.Solve(1.0, 10, 1)
     fp = pf2.Calc_Reaction(dofsY).sum()
     assert abs(fp - fe) < 1e-9 * abs(fe)
+
+
+# ------------------------------------------------ post-processing (8(f) rank 1)
+def _distorted_mesh(et, seed=11):
+    makers = {
+        "TRI3": lambda: ea.tri3_rectangle(2.0, 1.0, 5, 4),
+        "QUAD4": lambda: ea.quad4_rectangle(2.0, 1.0, 5, 4),
+        "HEXA8": lambda: ea.hexa8_box(1.0, 1.5, 2.0, 3, 2, 3),
+        "TETRA4": lambda: ea.mesh.tetra4_box(1.0, 1.5, 2.0, 2, 2, 3),
+        "TETRA10": lambda: ea.mesh.tetra10_from_tetra4(ea.mesh.tetra4_box(1.0, 1.5, 2.0, 2, 2, 2)),
+    }
+    m = makers[et]()
+    c = m.coord.copy()
+    rng = np.random.default_rng(seed)
+    dim = m.dim
+    inner = np.ones(m.Nn, bool)
+    for k in range(dim):
+        inner &= (c[:, k] > c[:, k].min()) & (c[:, k] < c[:, k].max())
+    c[inner, :dim] += 0.05 * (rng.random((inner.sum(), dim)) - 0.5)
+    return m, c
+
+
+@pytest.mark.parametrize("et", ELEMS2D + ELEMS3D)
+def test_strain_stress_of_a_linear_field_are_the_closed_forms(et):
+    """u = A x + b is reproduced exactly by every element (isoparametric
+    patch), so eps = sym(A) at every point and sig = lam tr(eps) I + 2 mu eps;
+    the von Mises measures and the energy follow in closed form."""
+    m, c = _distorted_mesh(et)
+    dim = m.dim
+    rng = np.random.default_rng(3)
+    A = 1e-3 * rng.standard_normal((dim, dim))
+    u = (c[:, :dim] @ A.T + 1e-3).ravel()
+    E, nu = 2.0e5 * np.exp(0.1 * rng.standard_normal(m.Ne)), 0.3
+    geo = O.Geometry(et, m.connect, c, "rigi", dim)
+    eps = O.strain_e_pg(geo, u, m.connect)
+    sig = O.stress_e_pg(eps, O.C_isotropic(E, nu, dim, True))
+    e_t = 0.5 * (A + A.T)
+    lam, mu, _ = O.lame(E, nu, dim, True)
+    s_t = lam[:, None, None] * np.trace(e_t) * np.eye(dim) + 2 * mu[:, None, None] * e_t
+    names = ["xx", "yy", "xy"] if dim == 2 else ["xx", "yy", "zz", "yz", "xz", "xy"]
+    idx = {"x": 0, "y": 1, "z": 2}
+    for n in names:
+        i, j = idx[n[0]], idx[n[1]]
+        assert np.allclose(O.strain_or_stress_result_e(eps, "E" + n), e_t[i, j], rtol=0, atol=1e-15)
+        assert np.allclose(O.strain_or_stress_result_e(sig, "S" + n), s_t[:, i, j], rtol=1e-12, atol=1e-12)
+
+    def vm(t):
+        d = t - np.trace(t, axis1=-2, axis2=-1)[..., None, None] * np.eye(dim) / 3 if dim == 3 else None
+        if dim == 3:
+            return np.sqrt(1.5 * np.sum(d * d, axis=(-2, -1)))
+        xx, yy, xy = t[..., 0, 0], t[..., 1, 1], t[..., 0, 1]
+        return np.sqrt(xx**2 + yy**2 - xx * yy + 3 * xy**2)
+
+    assert np.allclose(O.strain_or_stress_result_e(sig, "Svm"), vm(s_t), rtol=1e-12)
+    assert np.allclose(O.strain_or_stress_result_e(eps, "Evm"), vm(e_t), rtol=1e-12)
+    full = O.strain_or_stress_result_e(eps, "Strain")
+    assert full.shape == (m.Ne, len(names))
+    # energy density of the constant field times the element volume
+    vol = geo.wJ.sum(1)
+    w = O.psi_elas_e(geo, eps, sig, thickness=2.0)
+    dens = 0.5 * np.einsum("eij,ij->e", s_t, e_t)
+    assert np.allclose(w, dens * vol * (2.0 if dim == 2 else 1.0), rtol=1e-12)
+
+
+@pytest.mark.parametrize("et", ["QUAD4", "HEXA8", "TETRA10"])
+def test_deformation_energy_equals_half_u_K_u(et):
+    """Cross-path invariant: sum_e 1/2 int sig.eps = 1/2 u^T K u for any u."""
+    m, c = _distorted_mesh(et)
+    dim = m.dim
+    rng = np.random.default_rng(8)
+    u = rng.standard_normal(m.Nn * dim)
+    E = 2.0e5 * np.exp(0.2 * rng.standard_normal(m.Ne))
+    C = O.C_isotropic(E, 0.3, dim, False)
+    geo = O.Geometry(et, m.connect, c, "rigi", dim)
+    eps = O.strain_e_pg(geo, u, m.connect)
+    W = O.psi_elas_e(geo, eps, O.stress_e_pg(eps, C)).sum()
+    K = O.assemble_csr(O.K_elastic(geo, C), O.csr_map(m.connect, dim, m.Nn * dim), m.Nn * dim)
+    assert abs(W - 0.5 * u @ (K @ u)) <= 1e-12 * abs(W)
+
+
+def test_von_mises_special_states():
+    """Uniaxial stress gives |s|, pure shear sqrt(3)|t| (2D and 3D), a
+    hydrostatic 3D state 0; shear enters the Kelvin-Mandel field times sqrt 2."""
+    s, t = 3.0, 2.0
+    f2 = np.array([[[s, 0, 0], [0, 0, np.sqrt(2) * t]]])
+    assert np.allclose(O.result_in_field(f2, "vm")[0], [abs(s), np.sqrt(3) * t])
+    assert np.allclose(O.result_in_field(f2, "xy")[0], [0, t])
+    f3 = np.array([[[s, 0, 0, 0, 0, 0], [0, 0, 0, 0, 0, np.sqrt(2) * t], [5, 5, 5, 0, 0, 0]]])
+    assert np.allclose(O.result_in_field(f3, "vm")[0], [abs(s), np.sqrt(3) * t, 0])
+    assert np.allclose(O.result_in_field(f3, "xy")[0], [0, t, 0])
+
+
+def test_node_values_averages_surrounding_elements():
+    """Two triangles sharing an edge, one unused node: shared nodes take the
+    mean, the others their element's value, the unused node 0."""
+    connect = np.array([[0, 1, 2], [1, 3, 2]])
+    v = np.array([2.0, 6.0])
+    out = O.node_values(connect, 5, v)
+    assert np.array_equal(out, [2.0, 4.0, 4.0, 6.0, 0.0])
+    two = O.node_values(connect, 5, np.stack([v, -v], 1))
+    assert np.array_equal(two[:, 1], -out)
+    m, _ = _distorted_mesh("HEXA8")
+    assert np.allclose(O.node_values(m.connect, m.Nn, np.full(m.Ne, 7.5)), 7.5)
