@@ -365,6 +365,32 @@ class _Simu:
             print(f"{problemType}: {info.iterations} it, rel res {info.rel_residual:.2e} "
                   f"({time.perf_counter() - t:.3f} s)")
 
+    def _reshape_values(self, values, nodeValues: bool) -> np.ndarray:
+        """``Results_Reshape_values`` (``Simulations/_simu.py:3095-3153``):
+        element values go to the nodes by ``Mesh.Get_Node_Values`` (on the
+        device), node values to the elements as the mean over their nodes."""
+        Nn, Ne = self.mesh.Nn, self.dev.Ne
+        is_t = isinstance(values, torch.Tensor)
+        size = values.numel() if is_t else np.asarray(values).size
+        is1d = values.dim() == 1 if is_t else np.asarray(values).ndim == 1
+        if nodeValues:
+            if size % Nn == 0:
+                v = values.cpu().numpy() if is_t else np.asarray(values)
+                return v.ravel() if is1d else v.reshape(Nn, -1)
+            if size % Ne == 0:
+                vt = values if is_t else to_dev(np.asarray(values, dtype=np.float64))
+                vn = self.dev.node_values(vt.reshape(Ne, -1)).cpu().numpy()
+                return vn.ravel() if is1d else vn
+        else:
+            if size % Ne == 0:
+                v = values.cpu().numpy() if is_t else np.asarray(values)
+                return v.ravel() if is1d else v.reshape(Ne, -1)
+            if size % Nn == 0:
+                v = (values.cpu().numpy() if is_t else np.asarray(values)).reshape(Nn, -1)
+                ve = v[self.dev.connect_np].mean(axis=1)
+                return ve.ravel() if is1d else ve
+        raise Exception("Unexpected conditions occurred during the calculation.")
+
     def _solve_system(self, A, b_dev: torch.Tensor, x_dev: torch.Tensor, problemType) -> None:
         drive(self._solve_system_gen(A, b_dev, x_dev, problemType))
 
@@ -515,32 +541,6 @@ class Elastic(_Simu):
                   ptr(self._u), ptr(E), 1 if self.model.isHeterogeneous else 0, self.model.v,
                   int(self.model.planeStress), self.model.thickness, ptr(out), stream_ptr())
         return out
-
-    def _reshape_values(self, values, nodeValues: bool) -> np.ndarray:
-        """``Results_Reshape_values`` (``Simulations/_simu.py:3095-3153``):
-        element values go to the nodes by ``Mesh.Get_Node_Values`` (on the
-        device), node values to the elements as the mean over their nodes."""
-        Nn, Ne = self.mesh.Nn, self.dev.Ne
-        is_t = isinstance(values, torch.Tensor)
-        size = values.numel() if is_t else np.asarray(values).size
-        is1d = values.dim() == 1 if is_t else np.asarray(values).ndim == 1
-        if nodeValues:
-            if size % Nn == 0:
-                v = values.cpu().numpy() if is_t else np.asarray(values)
-                return v.ravel() if is1d else v.reshape(Nn, -1)
-            if size % Ne == 0:
-                vt = values if is_t else to_dev(np.asarray(values, dtype=np.float64))
-                vn = self.dev.node_values(vt.reshape(Ne, -1)).cpu().numpy()
-                return vn.ravel() if is1d else vn
-        else:
-            if size % Ne == 0:
-                v = values.cpu().numpy() if is_t else np.asarray(values)
-                return v.ravel() if is1d else v.reshape(Ne, -1)
-            if size % Nn == 0:
-                v = (values.cpu().numpy() if is_t else np.asarray(values)).reshape(Nn, -1)
-                ve = v[self.dev.connect_np].mean(axis=1)
-                return ve.ravel() if is1d else ve
-        raise Exception("Unexpected conditions occurred during the calculation.")
 
     def Result(self, result: str, nodeValues: bool = True, iter: Optional[int] = None):
         """``Elastic.Result`` (``Simulations/_elastic.py:231-321``): node (or,
@@ -871,13 +871,78 @@ class PhaseField(_Simu):
             return False
         return bool(torch.any(self._d[to_dev(nodes)] >= threshold))
 
-    def Result(self, option: str) -> np.ndarray:
-        if option == "damage":
-            return self.damage
-        u = self.displacement.reshape(-1, self.model.dim)
-        comp = {"ux": 0, "uy": 1}
-        if option in comp:
-            return u[:, comp[option]].copy()
-        if option == "displacement":
-            return self.displacement
-        raise ValueError(f"unknown result {option}")
+    def Results_Available(self) -> list:
+        """``PhaseField.Results_Available`` (``Simulations/_phasefield.py:659-680``)."""
+        return ["displacement", "displacement_norm", "displacement_matrix", "ux", "uy", "Sxx", "Syy", "Sxy",
+                "Exx", "Eyy", "Exy", "Svm", "Stress", "Evm", "Strain", "damage", "psiP", "Psi_Crack", "Wdef"]
+
+    def _strain_stress_e(self) -> torch.Tensor:
+        """(Ne, 9) per element on the device: strain components, Evm, damaged
+        stress components, Svm, W_e (``efea_pf_strain_stress``)."""
+        dev, pfm = self.dev, self.model
+        out = torch.empty((dev.Ne, 9), dtype=torch.float64, device="cuda")
+        tr, nr = dev.tables(MatrixType.rigi)
+        E, Es, nu, ps = self._material_args()
+        _lib.call("efea_pf_strain_stress", dev.code, dev.Ne, ptr(dev.connect), ptr(dev.coord), ptr(tr), nr,
+                  ptr(self._u), ptr(self._d), ptr(E), Es, nu, ps, 0 if pfm.split == "Amor" else 1, pfm.thickness,
+                  ptr(out), stream_ptr())
+        return out
+
+    def _psiP_now(self) -> torch.Tensor:
+        """psi+ at the mass points from the current u with the history
+        (``__Calc_psiPlus_e_pg``, ``_phasefield.py:494-538``), (Ne, nPg)."""
+        dev, pfm = self.dev, self.model
+        tm, nm = dev.tables(MatrixType.mass)
+        tr, nr = dev.tables(MatrixType.rigi)
+        E, Es, nu, ps = self._material_args()
+        psiP = torch.empty(dev.Ne * nm, dtype=torch.float64, device="cuda")
+        Ke = torch.empty(dev.Ne * dev.nPe * dev.nPe, dtype=torch.float64, device="cuda")
+        Fe = torch.empty(dev.Ne * dev.nPe, dtype=torch.float64, device="cuda")
+        _lib.call("efea_pf_damage_ke", dev.code, dev.Ne, ptr(dev.connect), ptr(dev.coord), ptr(tm), nm, ptr(tr), nr,
+                  ptr(self._u), ptr(self._H), ptr(E), Es, nu, ps, ptr(self._Gc_dev()),
+                  1 if pfm.isHeterogeneous else 0, pfm.l0, 0 if pfm.split == "Amor" else 1,
+                  0 if pfm.regularization == "AT1" else 1, pfm.thickness, ptr(psiP), ptr(Ke), ptr(Fe), stream_ptr())
+        return psiP.reshape(dev.Ne, nm)
+
+    def Result(self, result: str, nodeValues: bool = True, iter: Optional[int] = None):
+        """``PhaseField.Result`` (``Simulations/_phasefield.py:682-760``)."""
+        if iter is not None:
+            self.Set_Iter(iter)
+        if result not in self.Results_Available():
+            print(f"\n The result '{result}' is not available.\n Available results: {self.Results_Available()}")
+            return None
+        Nn = self.mesh.Nn
+        if result == "Wdef":
+            return float(self._psi_elas_dev())
+        if result == "Psi_Crack":
+            return float(self._psi_crack_dev())
+        if result == "psiP":
+            values = self._psiP_now().mean(1).contiguous()
+        elif result == "damage":
+            values = self.damage
+        elif result in ("ux", "uy"):
+            values = self.displacement.reshape(Nn, -1)[:, 0 if result == "ux" else 1]
+        elif result == "displacement":
+            values = self.displacement
+        elif result == "displacement_norm":
+            values = np.linalg.norm(self.displacement.reshape(Nn, -1), axis=1)
+        elif result == "displacement_matrix":
+            values = np.zeros((Nn, 3))
+            values[:, :2] = self.displacement.reshape(Nn, -1)
+        else:
+            e = self._strain_stress_e()
+            comps = ["xx", "yy", "xy"]
+            if result == "Strain":
+                values = e[:, :3]
+            elif result == "Stress":
+                values = e[:, 4:7]
+            elif result == "Evm":
+                values = e[:, 3]
+            elif result == "Svm":
+                values = e[:, 7]
+            elif result[0] == "E":
+                values = e[:, comps.index(result[1:])]
+            else:
+                values = e[:, 4 + comps.index(result[1:])]
+            values = values.contiguous()
+        return self._reshape_values(values, nodeValues)

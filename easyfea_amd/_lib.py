@@ -69,6 +69,7 @@ SIGNATURES = {
     ),
     "efea_pf_elastic_ke": (i32, [i32, i64, vp, vp, vp, i32, vp, vp, vp, i32, f64, i32, i32, f64, vp, vp]),
     "efea_pf_split_points": (i32, [i64, i32, vp, vp, i32, f64, i32, vp, vp, vp, vp, vp]),
+    "efea_pf_strain_stress": (i32, [i32, i64, vp, vp, vp, i32, vp, vp, vp, i32, f64, i32, i32, f64, vp, vp]),
 }
 
 ELEM_CODE = {"TRI3": 0, "QUAD4": 1, "HEXA8": 2, "TETRA4": 3, "TETRA10": 4, "SEG2": 5}

@@ -61,6 +61,8 @@ int pf_damage(int, int64_t, const int*, const double*, const double*, int, const
               double*, double*, double*, hipStream_t);
 int pf_elastic(int, int64_t, const int*, const double*, const double*, int, const double*, const double*,
                const double*, int, double, int, int, double, double*, hipStream_t);
+int pf_strain_stress(int, int64_t, const int*, const double*, const double*, int, const double*, const double*,
+                     const double*, int, double, int, int, double, double*, hipStream_t);
 int pf_split_points(int64_t, int, const double*, const double*, int, double, int, double*, double*, double*,
                     double*, hipStream_t);
 int batch_create(int, int, int, int, int, void**);
@@ -410,6 +412,22 @@ int efea_pf_damage_ke(int elem_type, int64_t Ne, const int32_t* connect, const d
   return pf_damage(elem_type, Ne, connect, coord, tables_mass, nPg_mass, tables_rigi, nPg_rigi, u, H_old, E,
                    E_stride, nu, plane_stress, Gc, Gc_stride, l0, split, regu, thickness, psiP, Kd, Fd,
                    STREAM(stream));
+}
+
+int efea_pf_strain_stress(int elem_type, int64_t Ne, const int32_t* connect, const double* coord,
+                          const double* tables_rigi, int nPg_rigi, const double* u, const double* d, const double* E,
+                          int E_stride, double nu, int plane_stress, int split, double thickness, double* out,
+                          void* stream) {
+  if (Ne < 0 || (Ne > 0 && (!connect || !coord || !tables_rigi || !u || !d || !E || !out))) {
+    set_error("efea_pf_strain_stress: null pointer or negative size");
+    return EFEA_ERR_ARG;
+  }
+  if (split < 0 || split > 1) {
+    set_error("efea_pf_strain_stress: split %d not supported", split);
+    return EFEA_ERR_ARG;
+  }
+  return pf_strain_stress(elem_type, Ne, connect, coord, tables_rigi, nPg_rigi, u, d, E, E_stride, nu, plane_stress,
+                          split, thickness, out, STREAM(stream));
 }
 
 int efea_pf_elastic_ke(int elem_type, int64_t Ne, const int32_t* connect, const double* coord,

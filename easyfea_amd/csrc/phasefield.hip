@@ -314,6 +314,79 @@ k_pf_elastic(int64_t Ne, const int* __restrict__ connect, const double* __restri
   for (int64_t k = t; k < nval; k += 256) out[k] = s_K[k];
 }
 
+// Phase-field results per element (Simulations/_phasefield.py:682-760,
+// 838-916): at each rigi point eps = B u_e, the split c+ / c-, and the damaged
+// stress sig = g(d) (c+ eps) + c- eps with g = (1 - N.d)^2 + 1e-12.  Output row
+// as the elastic post-processing: [eps xx, yy, xy (tensor), Evm, sig xx, yy,
+// xy, Svm, 1/2 sum_p wJ sig.eps x thickness], components and von Mises
+// measures averaged over the points.  One lane per element.
+template <int ET, int NPG>
+__global__ void __launch_bounds__(256)
+k_pf_strain_stress(int64_t Ne, const int* __restrict__ connect, const double* __restrict__ coord,
+                   const double* __restrict__ tab, const double* __restrict__ u, const double* __restrict__ d,
+                   const double* __restrict__ E, int E_stride, PfParams P, double* __restrict__ out) {
+  constexpr int NPE = Elem<ET>::NPE, DIM = 2;
+  using TB = Tab<NPG, NPE, DIM>;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= Ne) return;
+  double X[NPE][3], ue[2 * NPE], de[NPE];
+#pragma unroll
+  for (int n = 0; n < NPE; ++n) {
+    const int node = connect[e * NPE + n];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) X[n][k] = coord[(int64_t)node * 3 + k];
+    ue[2 * n] = u[(int64_t)node * 2];
+    ue[2 * n + 1] = u[(int64_t)node * 2 + 1];
+    de[n] = d[node];
+  }
+  double lam, mu;
+  lame2d(E[E_stride ? e : 0], P.nu, P.planeStress, lam, mu);
+  const double isq2 = 1 / sqrt(2.0);
+  double se[3] = {0, 0, 0}, ss[3] = {0, 0, 0}, evm = 0, svm = 0, w = 0;
+#pragma unroll
+  for (int p = 0; p < NPG; ++p) {
+    double dNx[DIM][NPE];
+    const double detJ = point_geometry<double, NPE, DIM>(tab + TB::DN + p * DIM * NPE, X, dNx);
+    double eps[3];
+    strain2d<NPE>(dNx, ue, eps);
+    double cP[3][3], cM[3][3];
+    split2d(P.split, eps, lam, mu, cP, cM);
+    double dp = 0;
+#pragma unroll
+    for (int n = 0; n < NPE; ++n) dp += tab[TB::N + p * NPE + n] * de[n];
+    const double g = (1 - dp) * (1 - dp) + P.k_res;
+    double sig[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      double sp = 0, sm = 0;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        sp += cP[a][b] * eps[b];
+        sm += cM[a][b] * eps[b];
+      }
+      sig[a] = g * sp + sm;
+    }
+    w += detJ * tab[TB::W + p] * (sig[0] * eps[0] + sig[1] * eps[1] + sig[2] * eps[2]);
+    const double te[3] = {eps[0], eps[1], eps[2] * isq2}, ts[3] = {sig[0], sig[1], sig[2] * isq2};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      se[k] += te[k];
+      ss[k] += ts[k];
+    }
+    evm += sqrt(te[0] * te[0] + te[1] * te[1] - te[0] * te[1] + 3 * te[2] * te[2]);
+    svm += sqrt(ts[0] * ts[0] + ts[1] * ts[1] - ts[0] * ts[1] + 3 * ts[2] * ts[2]);
+  }
+  double* o = out + e * 9;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    o[k] = se[k] / NPG;
+    o[4 + k] = ss[k] / NPG;
+  }
+  o[3] = evm / NPG;
+  o[7] = svm / NPG;
+  o[8] = 0.5 * w * P.thickness;
+}
+
 // Gauss-point split evaluation for testing and post-processing:
 // eps (n, 3) -> cP, cM (n, 3, 3), psiP, psiM (n).
 __global__ void k_pf_split_points(int64_t n, int split, const double* __restrict__ eps,
@@ -397,6 +470,26 @@ int pf_elastic(int elem, int64_t Ne, const int* connect, const double* coord, co
     return check_launch("pf_elastic");
   }
   set_error("pf_elastic: unsupported element %d / nPg %d", elem, nPg);
+  return EFEA_ERR_ARG;
+}
+
+int pf_strain_stress(int elem, int64_t Ne, const int* connect, const double* coord, const double* tab, int nPg,
+                     const double* u, const double* d, const double* E, int E_stride, double nu, int ps, int split,
+                     double thickness, double* out, hipStream_t s) {
+  const PfParams P = make_params(split, 1, ps, nu, 1.0, thickness);
+  if (Ne == 0) return EFEA_OK;
+  const unsigned nb = (unsigned)((Ne + 255) / 256);
+  if (elem == ET_TRI3 && nPg == 1) {
+    hipLaunchKernelGGL((k_pf_strain_stress<ET_TRI3, 1>), dim3(nb), dim3(256), 0, s, Ne, connect, coord, tab, u, d,
+                       E, E_stride, P, out);
+    return check_launch("pf_strain_stress");
+  }
+  if (elem == ET_QUAD4 && nPg == 4) {
+    hipLaunchKernelGGL((k_pf_strain_stress<ET_QUAD4, 4>), dim3(nb), dim3(256), 0, s, Ne, connect, coord, tab, u, d,
+                       E, E_stride, P, out);
+    return check_launch("pf_strain_stress");
+  }
+  set_error("pf_strain_stress: unsupported element %d / nPg %d", elem, nPg);
   return EFEA_ERR_ARG;
 }
 

@@ -269,6 +269,15 @@ int efea_pf_elastic_ke(int elem_type, int64_t Ne, const int32_t* connect, const 
                        const double* E, int E_stride, double nu, int plane_stress, int split, double thickness,
                        double* Ke, void* stream);
 
+/* Phase-field results per element (PhaseField.Result strain / stress options,
+ * Simulations/_phasefield.py:682-760 with _Calc_Sigma_e_pg :868-916): the
+ * damaged stress g(d) c+ eps + c- eps at the rigi points; out (Ne, 9) as
+ * efea_elastic_strain_stress in 2D. */
+int efea_pf_strain_stress(int elem_type, int64_t Ne, const int32_t* connect, const double* coord,
+                          const double* tables_rigi, int nPg_rigi, const double* u, const double* d, const double* E,
+                          int E_stride, double nu, int plane_stress, int split, double thickness, double* out,
+                          void* stream);
+
 /* Pointwise 2D split: eps (n, 3) -> c+, c- (n, 3, 3), psi+, psi- (n).
  * Replaces PhaseField.Calc_C / Calc_psi_e_pg (Models/_phasefield.py:335-431). */
 int efea_pf_split_points(int64_t n, int split, const double* eps, const double* E, int E_stride, double nu,

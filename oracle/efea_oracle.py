@@ -778,6 +778,20 @@ def psi_elas_e(geo: Geometry, eps: np.ndarray, sig: np.ndarray, thickness: float
     return w * thickness if geo.dim == 2 else w
 
 
+def pf_stress_e_pg(geo: Geometry, u: np.ndarray, d: np.ndarray, connect: np.ndarray, split: str, E, v: float,
+                   planeStress: bool) -> np.ndarray:
+    """``PhaseField._Calc_Sigma_e_pg`` (``Simulations/_phasefield.py:868-916``):
+    ``g(d) * (c+ eps) + c- eps`` at the rule's points, with the split of
+    ``calc_C`` and ``g = (1 - N.d)^2 + 1e-12``."""
+    eps = strain_e_pg(geo, u, connect)
+    E = np.asarray(E, float)
+    cP, cM = calc_C(split, eps, E[:, None] if E.ndim else E, v, geo.dim, planeStress)
+    sP = np.einsum("epij,epj->epi", cP, eps)
+    sM = np.einsum("epij,epj->epi", cM, eps)
+    g = pf_g(d, connect, geo.N_pg)
+    return g[..., None] * sP + sM
+
+
 def node_values(connect: np.ndarray, Nn: int, values_e: np.ndarray) -> np.ndarray:
     """``Mesh.Get_Node_Values`` (``FEM/_mesh.py:822-873``) with the incidence of
     ``_GroupElem.Get_connect_n_e`` (``FEM/_group_elem.py:322-344``): per node,

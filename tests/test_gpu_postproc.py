@@ -98,3 +98,41 @@ def test_wdef_equals_half_u_K_u_on_the_device():
     K = simu.Get_K_C_M_F()[0]
     u = simu.displacement
     assert abs(simu.Result("Wdef") - 0.5 * u @ (K @ u)) <= 1e-11 * abs(simu.Result("Wdef"))
+
+
+@pytest.mark.parametrize("split", ["Amor", "Miehe"])
+@pytest.mark.parametrize("et", ["TRI3", "QUAD4"])
+def test_phasefield_results_match_the_oracle(split, et):
+    m = MAKERS[et]()
+    rng = np.random.default_rng(6)
+    c = m.coord.copy()
+    inner = (c[:, 0] > 0) & (c[:, 0] < c[:, 0].max()) & (c[:, 1] > 0) & (c[:, 1] < c[:, 1].max())
+    c[inner, :2] += 0.04 * (rng.random((inner.sum(), 2)) - 0.5)
+    mesh = ea.Mesh({t: g.connect for t, g in m.dict_groupElem.items()}, c)
+    E = 210000.0 * np.exp(0.1 * rng.standard_normal(mesh.Ne))
+    mat = Models.Elastic.Isotropic(2, E, 0.3, False, 1.0)
+    simu = Simulations.PhaseField(mesh, Models.PhaseField(mat, split, "AT2", 2.7, 0.05))
+    u = 1e-3 * rng.standard_normal(mesh.Nn * 2)
+    d = rng.random(mesh.Nn)
+    simu._u = torch.from_numpy(u).cuda()
+    simu._d = torch.from_numpy(d).cuda()
+    geo = O.Geometry(et, mesh.connect, mesh.coord, "rigi", 2)
+    eps = O.strain_e_pg(geo, u, mesh.connect)
+    sig = O.pf_stress_e_pg(geo, u, d, mesh.connect, split, E, 0.3, False)
+    for c_ in ["xx", "yy", "xy", "vm"]:
+        for kind, field in (("S", sig), ("E", eps)):
+            name = kind + c_
+            ref_e = O.strain_or_stress_result_e(field, name)
+            assert _rel(simu.Result(name, nodeValues=False), ref_e) < 1e-12, name
+            assert _rel(simu.Result(name), O.node_values(mesh.connect, mesh.Nn, ref_e)) < 1e-12, name
+    assert np.array_equal(simu.Result("damage"), d)
+    # energies: 1/2 u.Ku u and 1/2 d.Kd d of the cached matrices
+    Ku = simu.Get_K_C_M_F("elastic")[0]
+    assert abs(simu.Result("Wdef") - 0.5 * u @ (Ku @ u)) <= 1e-11 * abs(simu.Result("Wdef"))
+    Kd = simu.Get_K_C_M_F("damage")[0]
+    assert abs(simu.Result("Psi_Crack") - 0.5 * d @ (Kd @ d)) <= 1e-11 * abs(simu.Result("Psi_Crack"))
+    # psi+ from the current u (no history yet): the split's psi+ at the mass points
+    geo_m = O.Geometry(et, mesh.connect, mesh.coord, "mass", 2)
+    psiP, _ = O.calc_psi(split, np.einsum("epij,ej->epi", geo_m.B(), u[O.assembly_e(mesh.connect, 2)]),
+                         E[:, None], 0.3, 2, False)
+    assert _rel(simu.Result("psiP", nodeValues=False), psiP.mean(1)) < 1e-12

@@ -573,3 +573,22 @@ def test_node_values_averages_surrounding_elements():
     assert np.array_equal(two[:, 1], -out)
     m, _ = _distorted_mesh("HEXA8")
     assert np.allclose(O.node_values(m.connect, m.Nn, np.full(m.Ne, 7.5)), 7.5)
+
+
+@pytest.mark.parametrize("split", ["Amor", "Miehe"])
+def test_damaged_stress_limits(split):
+    """d = 0: g = 1 + 1e-12 and c+ + c- = C, so the damaged stress is the
+    elastic one; d = 1: only c- eps remains (plus 1e-12 c+ eps), which for a
+    pure dilatation (all eigenvalues positive) vanishes for both splits."""
+    m = ea.tri3_rectangle(2.0, 1.0, 5, 4)
+    rng = np.random.default_rng(4)
+    u = 1e-3 * rng.standard_normal(m.Nn * 2)
+    E = 2e5 * np.exp(0.1 * rng.standard_normal(m.Ne))
+    geo = O.Geometry("TRI3", m.connect, m.coord, "rigi", 2)
+    s0 = O.pf_stress_e_pg(geo, u, np.zeros(m.Nn), m.connect, split, E, 0.3, False)
+    se = O.stress_e_pg(O.strain_e_pg(geo, u, m.connect), O.C_isotropic(E, 0.3, 2, False))
+    assert np.max(np.abs(s0 - se)) <= 2e-12 * np.max(np.abs(se))
+    dil = (m.coord[:, :2] * 1e-3).ravel()  # u = 1e-3 x: eps = 1e-3 I
+    s1 = O.pf_stress_e_pg(geo, dil, np.ones(m.Nn), m.connect, split, E, 0.3, False)
+    s_el = O.stress_e_pg(O.strain_e_pg(geo, dil, m.connect), O.C_isotropic(E, 0.3, 2, False))
+    assert np.allclose(s1, 1e-12 * s_el, rtol=1e-9, atol=0)
