@@ -457,7 +457,9 @@ __global__ void __launch_bounds__(BS)
 k_pcg2_step1(SpmvArgs A, const double* __restrict__ z, const double* __restrict__ P0, const double* __restrict__ P1,
              double* __restrict__ q, const double* __restrict__ part2, int nb2, double* __restrict__ part1,
              PcgState* st) {
-  pcg_step1_body<TM, DN, G>(A, z, P0, P1, q, part2, nb2, part1, st, blockIdx.x, gridDim.x);
+  const bool one_pass = (int64_t)gridDim.x * BS >= (int64_t)A.Nnode * G;
+  pcg_step1_body<TM, DN, G>(A, z, P0, P1, q, part2, nb2, part1, st,
+                            one_pass ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x, gridDim.x);
 }
 
 __global__ void k_state_reset2(PcgState* st, double tolr2, double tolA2, double delta2) {

@@ -137,6 +137,19 @@ constexpr int MAXB = 1024;
 // dispatch site expands this list.
 #define EFEA_GROUPS(X) X(1, 4) X(1, 8) X(1, 16) X(2, 4) X(2, 8) X(2, 16) X(3, 8) X(3, 16) X(3, 32)
 
+// Logical block of hardware block b in a grid of nb: the dispatcher deals
+// blocks round-robin to the 8 XCDs (b, b+8, b+16, ... on XCD b % 8); this
+// bijection gives each XCD one contiguous range of logical blocks, so the
+// neighbour rows a block gathers were mostly fetched into the same XCD's L2 by
+// its neighbouring blocks.
+// Used when the grid covers its rows in one pass (grid-stride passes keep the
+// plain order, which measured faster on the 64^3 cube).
+__device__ inline int xcd_block(int b, int nb) {
+  constexpr int NX = 8;
+  const int per = nb / NX, rem = nb % NX, x = b % NX, k = b / NX;
+  return x * per + (x < rem ? x : rem) + k;
+}
+
 static inline unsigned grid_for(int64_t threads, int cap = 0) {
   const int64_t mx = (cap > 0 && cap < MAXB) ? cap : MAXB;
   int64_t nb = (threads + BS - 1) / BS;
