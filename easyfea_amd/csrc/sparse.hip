@@ -152,7 +152,10 @@ k_pcg_step2(int Nnode, const unsigned char* __restrict__ known, const TP* __rest
             double* __restrict__ x, double* __restrict__ r, double* __restrict__ z, const double* __restrict__ P0,
             const double* __restrict__ P1, const double* __restrict__ q, const double* __restrict__ part1, int nb1,
             double* __restrict__ part2, PcgState* st) {
-  pcg_step2_body<TP, DN>(Nnode, known, Minv, x, r, z, P0, P1, q, part1, nb1, part2, st, blockIdx.x, gridDim.x);
+  // the same node -> XCD ranges as step 1 (whose p and q it reads)
+  const bool one_pass = (int64_t)gridDim.x * BS >= Nnode;
+  pcg_step2_body<TP, DN>(Nnode, known, Minv, x, r, z, P0, P1, q, part1, nb1, part2, st,
+                         one_pass ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x, gridDim.x);
 }
 
 template <int DN, int G>
