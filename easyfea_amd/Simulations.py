@@ -324,16 +324,29 @@ class _Simu:
     def _make_job(self, A, b_dev: torch.Tensor, x_dev: torch.Tensor, problemType) -> SolveJob:
         """The solve of ``problemType``: Dirichlet values written into
         ``x_dev`` at the known DOFs, the known mask on the device."""
-        known_np, xk = self._dirichlet_state(problemType)
-        # one device mask per problem type, kept while the Dirichlet DOFs stay the
-        # same (the tensor's identity then also keys the lagged coarse inverse)
-        key = hash(known_np.tobytes())
+        # One device mask per problem type, kept while the Dirichlet DOFs stay the
+        # same (the tensor's identity then also keys the lagged coarse inverse):
+        # known = the Dirichlet DOFs and the orphan DOFs, sorted; per solve only
+        # the values at the known DOFs are formed (duplicates add, orphans 0
+        # unless prescribed), as _dirichlet_state does over the full vector.
+        dofs, vals = self._bc_arrays(self._Bc_Dirichlet, problemType)
+        key = dofs.tobytes()
         cache = self.__dict__.setdefault("_mask_cache", {})
+        inv_cache = self.__dict__.setdefault("_mask_inverse", {})
         if cache.get(problemType, (None,))[0] != key:
-            idx = np.nonzero(known_np)[0]
+            Ndof = self.mesh.Nn * self.Get_dof_n(problemType)
+            orphan = self.mesh.orphanNodes
+            od = self.Bc_dofs_nodes(orphan, self.Get_unknowns(problemType), problemType) if orphan.size else \
+                np.array([], dtype=np.int64)
+            idx = np.unique(np.concatenate([dofs.astype(np.int64), np.asarray(od, dtype=np.int64)]))
+            known_np = np.zeros(Ndof, dtype=np.uint8)
+            known_np[idx] = 1
             cache[problemType] = (key, to_dev(known_np), to_dev(idx), idx)
+            inv_cache[problemType] = np.searchsorted(idx, dofs)
         _, known, idx_dev, idx = cache[problemType]
         if idx.size:
+            xk_idx = np.bincount(inv_cache[problemType], weights=vals, minlength=idx.size) if dofs.size else \
+                np.zeros(idx.size)
             # pinned staging, copied without a host wait; the buffer is rewritten only
             # at this problem's next solve, after this solve (which follows the copy
             # on the device) has finished
@@ -341,7 +354,7 @@ class _Simu:
             pin = pins.get(problemType)
             if pin is None or pin.numel() != idx.size:
                 pin = pins[problemType] = torch.empty(idx.size, dtype=torch.float64, pin_memory=True)
-            pin.numpy()[:] = xk[idx]
+            pin.numpy()[:] = xk_idx
             x_dev.index_copy_(0, idx_dev, pin.to("cuda", non_blocking=True))
         prev = self.last_solve.get(problemType)
         expected = prev.iterations if self.predict_polls and prev is not None and prev.status == 1 else None
