@@ -174,3 +174,32 @@ class PhaseField(_Observable):
 
     def _Gc_array(self) -> np.ndarray:
         return np.atleast_1d(np.asarray(self.Gc, dtype=np.float64))
+
+
+class Thermal(_Observable):
+    """Thermal model (``EasyFEA/Models/_thermal.py:17-56``): conductivity
+    ``k`` and mass heat capacity ``c``, each a scalar or one value per
+    element (>= 0), and a thickness.  Its ``dim`` is 1 (one unknown)."""
+
+    def __init__(self, k: Union[float, np.ndarray], c: Union[float, np.ndarray] = 0.0, thickness: float = 1.0):
+        super().__init__()
+        self.dim = 1
+        self.k = k
+        self.c = c
+        self.thickness = thickness
+
+    def __setattr__(self, name, value):
+        if name in ("k", "c"):
+            arr = np.asarray(value, dtype=float)
+            assert np.all(arr >= 0.0), f"{name} must be >= 0"
+            value = float(arr) if arr.ndim == 0 else arr
+        elif name == "thickness":
+            assert np.ndim(value) == 0 and value >= 0.0, "thickness must be a scalar >= 0"
+            value = float(value)
+        super().__setattr__(name, value)
+        if name in ("k", "c", "thickness") and "_observers" in self.__dict__:
+            self.Need_Update()
+
+    @property
+    def isHeterogeneous(self) -> bool:
+        return isinstance(self.k, np.ndarray) or isinstance(self.c, np.ndarray)

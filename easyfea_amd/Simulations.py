@@ -7,8 +7,12 @@ Mirrors (EasyFEA v3.5.1):
   (``Simulations/_simu.py:2190-2630``), ``Get_K_C_M_F`` returning scipy CSR copies
   (``:161-175``), ``Solve`` (``:1504-1515``), ``Calc_Reaction`` (``:212-260``),
   ``Calc_Energy`` (``:177-210``), ``Save_Iter`` / ``Set_Iter`` (``:288-471``);
-* ``Elastic`` (``Simulations/_elastic.py:52-176``);
-* ``PhaseField`` with its staggered ``Solve`` (``Simulations/_phasefield.py:86-657``).
+* ``Elastic`` (``Simulations/_elastic.py:52-176``), static or dynamic;
+* ``Thermal`` (``Simulations/_thermal.py``), steady or transient;
+* ``PhaseField`` with its staggered ``Solve`` (``Simulations/_phasefield.py:86-657``);
+* the time schemes ``Solver_Set_Elliptic/Parabolic/Hyperbolic_Algorithm``
+  (``Simulations/_simu.py:1161-1290``) and one step per ``Solve``
+  (``:1517-1640, 1735-1899``).
 
 Assembly, scatter, solve and the phase-field updates run in ``libefea_hip.so``;
 boundary-condition bookkeeping (which DOFs, which values) stays on the host
@@ -26,12 +30,14 @@ import torch
 
 from . import _lib
 from ._utils import ElemType, MatrixType
-from .device import DeviceMesh, SolveJob, Workspace, drive, ptr, stream_ptr, to_dev
+from .device import DeviceCsr, DeviceMesh, SolveJob, Workspace, drive, lincomb, ptr, stream_ptr, to_dev
 from .mesh import Mesh
 from .Models import Isotropic
 from .Models import PhaseField as PhaseFieldModel
+from .Models import Thermal as ThermalModel
 from .quadrature import N_pg, dN_pg, gauss_points
-from .Solvers import REFERENCE_ONLY, SolverType
+from .Solvers import REFERENCE_ONLY, AlgoType, SolverType
+from .time_schemes import expand, history_coefs, matrix_coefs
 
 
 def _print_error(msg: str) -> None:
@@ -93,6 +99,9 @@ class _Simu:
         # largest aggregate of the two-level coarse space (None: 256 nodes in 2D, 1024 in 3D)
         self.coarse_target = None
         self.last_solve = {}
+        self._algo = AlgoType.elliptic
+        self._parabolicParams = None
+        self._hyperbolicParams = None
         self.Bc_Init()
         model._Add_observer(self)
 
@@ -111,6 +120,112 @@ class _Simu:
 
     def _mode(self) -> str:
         return "fp64" if self.__solver == SolverType.hip_fp64 else "mixed"
+
+    # -------------------------------------------------------- time schemes
+    @property
+    def algo(self) -> AlgoType:
+        return self._algo
+
+    def Solver_Set_Elliptic_Algorithm(self) -> None:
+        """``K u = F`` (``_simu.py:1161-1171``)."""
+        self._algo = AlgoType.elliptic
+
+    def Solver_Set_Parabolic_Algorithm(self, dt: float, alpha=1 / 2) -> None:
+        """``K u^{n+1} + C v^{n+1} = F^{n+1}`` with ``u^{n+1} = u^n + dt
+        v^{n+alpha}`` (``_simu.py:1173-1212``)."""
+        assert dt > 0, "Time increment must be > 0"
+        self._algo = AlgoType.parabolic
+        self._parabolicParams = (float(dt), float(alpha))
+
+    def Solver_Set_Hyperbolic_Algorithm(self, dt: float, algo=AlgoType.newmark, beta=0.25, gamma=0.5,
+                                        alpha=0.5) -> None:
+        """``K u + C v + M a = F`` with ``algo`` in ``AlgoType.Get_Hyperbolic_Types()``
+        (``_simu.py:1214-1290``); ``hht_newmark`` derives beta and gamma from alpha."""
+        algo = AlgoType(algo)
+        types = AlgoType.Get_Hyperbolic_Types()
+        assert algo in types, f"algo must be in {types}"
+        assert dt > 0, "Time increment must be > 0"
+        if algo == AlgoType.hht_newmark:
+            assert 0 <= alpha <= 1 / 3, "hht_newmark requires alpha in [0, 1/3]"
+            beta = 1 / 4 * (1 + alpha) ** 2
+            gamma = 1 / 2 + alpha
+        else:
+            assert 0 <= alpha < 1
+        self._algo = algo
+        self._hyperbolicParams = (float(dt), float(beta), float(gamma), float(alpha))
+
+    def _time_params(self) -> tuple:
+        if self._algo == AlgoType.parabolic:
+            return self._parabolicParams
+        if self._algo in AlgoType.Get_Hyperbolic_Types():
+            return self._hyperbolicParams
+        return ()
+
+    def _time_bases(self, problemType) -> dict:
+        """``{"K"|"C"|"M": DeviceCsr}``: the assembled base matrices of
+        ``problemType`` on one pattern (absent ones are zero)."""
+        raise NotImplementedError
+
+    def _time_damping(self, problemType) -> dict:
+        """``C`` as weights of the base matrices (``{"C": 1.0}`` if C is a base)."""
+        return {"C": 1.0}
+
+    def _scheme_matrix(self, problemType, bases: dict, coefs: dict) -> DeviceCsr:
+        """``A = coefK K + coefC C + coefM M`` on the device
+        (``_simu.py:1893-1899``), kept while the coefficients and the base
+        matrices' values stay the same.  A changed ``A`` drops its lagged
+        coarse inverse."""
+        if len(coefs) == 1:
+            (name, c), = coefs.items()
+            if c == 1.0:
+                return bases[name]
+        names = sorted(coefs)
+        K = bases["K"]
+        A = self.dev.matrix("A:" + problemType, K.dof_n)
+        key = tuple((n, coefs[n], id(bases[n]), bases[n].version) for n in names)
+        if getattr(A, "_scheme_key", None) != key:
+            A.combine([(coefs[n], bases[n]) for n in names])
+            A._scheme_key = key
+            caches = self.__dict__.get("_coarse_caches")
+            if caches:
+                for k in [k for k in caches if k[0] == id(A)]:
+                    del caches[k]
+        return A
+
+    def _time_step_gen(self, problemType, u: torch.Tensor, v: torch.Tensor, a: torch.Tensor):
+        """One step of the current scheme for ``problemType`` from the state
+        ``(u, v, a)``: builds ``A`` and ``b`` on the device, yields the solve,
+        then returns the new ``(u, v, a)`` (``_Solver_Solve_problemType``,
+        ``_simu.py:1529-1550``, with ``_Solver_Apply_Neumann`` :1735-1860,
+        ``_Solver_Apply_Dirichlet`` :1862-1911 and
+        ``_Solver_Update_solutions`` :1552-1640).  Not for the elliptic
+        scheme, whose solve is ``K u = F`` alone."""
+        algo = self._algo
+        assert algo != AlgoType.elliptic
+        bases = self._time_bases(problemType)
+        b = self._neumann_device(problemType)
+        params = self._time_params()
+        damping = self._time_damping(problemType)
+        acoefs = {n: c for n, c in expand(dict(zip("KCM", matrix_coefs(algo, params))), damping).items()
+                  if n in bases}
+        A = self._scheme_matrix(problemType, bases, acoefs)
+        b = b.clone()
+        w = torch.empty_like(b)
+        for name, (cu, cv, ca) in expand(history_coefs(algo, params), damping).items():
+            if name not in bases:
+                continue
+            lincomb(w, [(c, s) for c, s in ((cu, u), (cv, v), (ca, a)) if c != 0])
+            lincomb(b, [(1.0, b), (1.0, bases[name].spmv(w))])
+        explicit = algo == AlgoType.euler_explicit
+        x = (a if explicit else u).clone()
+        # the explicit scheme's unknown is a^n: constrained DOFs have zero acceleration (_simu.py:1905-1907)
+        yield from self._solve_system_gen(A, b, x, problemType, zero_dirichlet=explicit)
+        u1, v1, a1 = torch.empty_like(u), torch.empty_like(v), torch.empty_like(a)
+        dt, *rest = params
+        beta, gamma, alpha = (0.0, 0.0, rest[0]) if algo == AlgoType.parabolic else tuple(rest)
+        _lib.call("efea_time_update", _lib.SCHEME_CODE[algo.value], u.numel(), dt, beta, gamma, alpha, ptr(u),
+                  ptr(v), ptr(a), ptr(x), ptr(u1), ptr(v1), ptr(a1), stream_ptr())
+        return u1, v1, a1
 
     # ------------------------------------------------------------ problems
     def Get_problemTypes(self) -> list:
@@ -321,9 +436,11 @@ class _Simu:
             xk[od] = np.where(np.isin(od, dofs), xk[od], 0.0)
         return known, xk
 
-    def _make_job(self, A, b_dev: torch.Tensor, x_dev: torch.Tensor, problemType) -> SolveJob:
-        """The solve of ``problemType``: Dirichlet values written into
-        ``x_dev`` at the known DOFs, the known mask on the device."""
+    def _make_job(self, A, b_dev: torch.Tensor, x_dev: torch.Tensor, problemType,
+                  zero_dirichlet: bool = False) -> SolveJob:
+        """The solve of ``problemType``: Dirichlet values (zeros with
+        ``zero_dirichlet``) written into ``x_dev`` at the known DOFs, the
+        known mask on the device."""
         # One device mask per problem type, kept while the Dirichlet DOFs stay the
         # same (the tensor's identity then also keys the lagged coarse inverse):
         # known = the Dirichlet DOFs and the orphan DOFs, sorted; per solve only
@@ -344,7 +461,9 @@ class _Simu:
             cache[problemType] = (key, to_dev(known_np), to_dev(idx), idx)
             inv_cache[problemType] = np.searchsorted(idx, dofs)
         _, known, idx_dev, idx = cache[problemType]
-        if idx.size:
+        if idx.size and zero_dirichlet:
+            x_dev.index_fill_(0, idx_dev, 0.0)
+        elif idx.size:
             xk_idx = np.bincount(inv_cache[problemType], weights=vals, minlength=idx.size) if dofs.size else \
                 np.zeros(idx.size)
             # pinned staging, copied without a host wait; the buffer is rewritten only
@@ -363,10 +482,11 @@ class _Simu:
                         max_blocks=self.max_blocks, expected_iterations=expected,
                         coarse_target=self.coarse_target)
 
-    def _solve_system_gen(self, A, b_dev: torch.Tensor, x_dev: torch.Tensor, problemType):
+    def _solve_system_gen(self, A, b_dev: torch.Tensor, x_dev: torch.Tensor, problemType,
+                          zero_dirichlet: bool = False):
         """Generator form of a solve: yields the ``SolveJob`` and receives its
         ``SolveInfo`` (from the single-system path or a batch executor)."""
-        job = self._make_job(A, b_dev, x_dev, problemType)
+        job = self._make_job(A, b_dev, x_dev, problemType, zero_dirichlet)
         t = time.perf_counter()
         info = yield job
         job.record(info)
@@ -439,9 +559,31 @@ class Elastic(_Simu):
         self._needUpdate = True
         super().__init__(mesh, model, folder, verbosity)
         assert mesh.dim == model.dim, "mesh and material dimensions differ"
-        self.rho = 1.0
+        self._rho = 1.0
+        self._rayleigh = (0.0, 0.0)  # (coefM, coefK) of C = coefK K + coefM M
         self._K = None
-        self._u = torch.zeros(mesh.Nn * model.dim, dtype=torch.float64, device="cuda")
+        self._M = None
+        n = mesh.Nn * model.dim
+        self._u = torch.zeros(n, dtype=torch.float64, device="cuda")
+        self._v = torch.zeros(n, dtype=torch.float64, device="cuda")
+        self._a = torch.zeros(n, dtype=torch.float64, device="cuda")
+
+    @property
+    def rho(self):
+        """Density, a scalar or one value per element (``_simu.py:706``)."""
+        return self._rho
+
+    @rho.setter
+    def rho(self, value):
+        arr = np.asarray(value, dtype=float)
+        assert np.all(arr >= 0.0), "rho must be >= 0"
+        self._rho = float(arr) if arr.ndim == 0 else arr
+        self.Need_Update()
+
+    def Set_Rayleigh_Damping_Coefs(self, coefM=0.0, coefK=0.0) -> None:
+        """Damping ``C = coefK K + coefM M`` (``Simulations/_elastic.py:152-156``)."""
+        self._rayleigh = (float(coefM), float(coefK))
+        self.Need_Update()
 
     def Get_problemTypes(self) -> list:
         return ["elastic"]
@@ -459,6 +601,8 @@ class Elastic(_Simu):
 
     def Need_Update(self, value: bool = True) -> None:
         self._needUpdate = value
+        if value:
+            self._M = None
 
     def Assembly(self):
         """Element stiffness on the device and deterministic scatter."""
@@ -480,13 +624,43 @@ class Elastic(_Simu):
             self.Assembly()
         return self._K
 
+    def _device_M(self) -> DeviceCsr:
+        """Mass matrix ``sum rho wJ N^T N`` (x thickness in 2D) over the mass
+        rule (``Simulations/_elastic.py:135-142``), assembled on first use
+        after each update."""
+        self._device_K()  # brings the simulation up to date (clears a stale M)
+        if self._M is None:
+            dev = self.dev
+            M = dev.matrix("M", self.model.dim)
+            Me = M.new_entries()
+            tab, nPg = dev.tables(MatrixType.mass)
+            rho = self._rho
+            het = isinstance(rho, np.ndarray)
+            rho_dev = to_dev(np.ascontiguousarray(rho, dtype=np.float64)) if het else None
+            scale = self.model.thickness if self.model.dim == 2 else 1.0
+            _lib.call("efea_uv_ke", dev.code, 64, dev.Ne, ptr(dev.connect), ptr(dev.coord), ptr(tab), nPg,
+                      ptr(rho_dev), 1 if het else 0, 0.0 if het else float(rho), self.model.dim, scale, ptr(Me),
+                      stream_ptr())
+            M.scatter(Me)
+            self._M = M
+        return self._M
+
+    def _time_bases(self, problemType) -> dict:
+        return {"K": self._device_K(), "M": self._device_M()}
+
+    def _time_damping(self, problemType) -> dict:
+        cM, cK = self._rayleigh
+        return {n: c for n, c in (("K", cK), ("M", cM)) if c != 0.0}
+
     def Get_K_C_M_F(self, problemType=None):
-        """Fresh scipy copies ``(K, C, M, F)``; ``C = M = 0`` (static),
-        ``F`` the assembled load (none for this model)."""
+        """Fresh scipy copies ``(K, C, M, F)`` with the mass matrix ``M``, the
+        Rayleigh damping ``C = coefK K + coefM M`` and ``F`` the assembled
+        load (none for this model) (``_simu.py:161-175``)."""
         K = self._device_K().to_scipy()
-        Ndof = K.shape[0]
-        z = sp.csr_matrix((Ndof, Ndof))
-        return K, z.copy(), z.copy(), sp.csr_matrix((Ndof, 1))
+        M = self._device_M().to_scipy()
+        cM, cK = self._rayleigh
+        C = (cK * K + cM * M).tocsr()
+        return K, C, M, sp.csr_matrix((K.shape[0], 1))
 
     @property
     def displacement(self) -> np.ndarray:
@@ -495,12 +669,23 @@ class Elastic(_Simu):
     def Get_x0(self, problemType=None) -> np.ndarray:
         return self.displacement
 
+    @property
+    def speed(self) -> np.ndarray:
+        return self._v.cpu().numpy()
+
+    @property
+    def accel(self) -> np.ndarray:
+        return self._a.cpu().numpy()
+
     def _solve_gen(self):
-        A = self._device_K()
-        b = self._neumann_device("elastic")
-        x = self._u.clone()
-        yield from self._solve_system_gen(A, b, x, "elastic")
-        self._u = x
+        if self.algo == AlgoType.elliptic:
+            A = self._device_K()
+            b = self._neumann_device("elastic")
+            x = self._u.clone()
+            yield from self._solve_system_gen(A, b, x, "elastic")
+            self._u = x
+        else:
+            self._u, self._v, self._a = yield from self._time_step_gen("elastic", self._u, self._v, self._a)
 
     def Solve(self) -> np.ndarray:
         drive(self._solve_gen())
@@ -518,20 +703,30 @@ class Elastic(_Simu):
         return 0.5 * float(torch.dot(self._u, y))
 
     def Save_Iter(self, iter: Optional[dict] = None) -> None:
+        """Displacement, plus speed and acceleration under a hyperbolic scheme
+        (``Simulations/_elastic.py:164-176``)."""
         it = {} if iter is None else dict(iter)
         it["displacement"] = self.displacement
+        if self.algo in AlgoType.Get_Hyperbolic_Types():
+            it["speed"] = self.speed
+            it["accel"] = self.accel
         super().Save_Iter(it)
 
     def Set_Iter(self, iter: int = -1) -> Optional[dict]:
         r = super().Set_Iter(iter)
         if r is not None:
             self._u = to_dev(r["displacement"])
+            if self.algo in AlgoType.Get_Hyperbolic_Types() and "speed" in r and "accel" in r:
+                self._v, self._a = to_dev(r["speed"]), to_dev(r["accel"])
+            else:
+                self._v, self._a = torch.zeros_like(self._u), torch.zeros_like(self._u)
         return r
 
     # ------------------------------------------------------------ results
     def Results_Available(self) -> list:
         """``Elastic.Results_Available`` (``Simulations/_elastic.py:203-229``;
-        the static solve has zero speed and acceleration, ZZ1 is not provided)."""
+        ZZ1 is not provided).  Speed and acceleration are zero until a
+        hyperbolic scheme has stepped."""
         dim = self.model.dim
         out = ["displacement", "displacement_norm", "displacement_matrix", "speed", "speed_norm", "accel",
                "accel_norm"]
@@ -575,11 +770,12 @@ class Elastic(_Simu):
         elif result == "displacement_matrix":
             values = np.zeros((Nn, 3))
             values[:, :dim] = self.displacement.reshape(Nn, -1)
-        elif result[0] in "va" and result[1:] in axes or result in ("speed", "accel"):
-            # static solve: no speed or acceleration
-            values = np.zeros(Nn) if len(result) == 2 else np.zeros(Nn * dim)
+        elif result[0] in "va" and result[1:] in axes:
+            values = (self.speed if result[0] == "v" else self.accel).reshape(Nn, -1)[:, axes[result[1]]]
+        elif result in ("speed", "accel"):
+            values = self.speed if result == "speed" else self.accel
         elif result in ("speed_norm", "accel_norm"):
-            values = np.zeros(Nn)
+            values = np.linalg.norm((self.speed if result == "speed_norm" else self.accel).reshape(Nn, -1), axis=1)
         elif result == "Wdef":
             return float(self._strain_stress_e()[:, -1].sum())
         else:
@@ -601,6 +797,170 @@ class Elastic(_Simu):
             else:
                 values = e[:, ns + 1 + comps.index(result[1:])]
             values = values.contiguous()
+        return self._reshape_values(values, nodeValues)
+
+
+class Thermal(_Simu):
+    """Heat transfer (``Simulations/_thermal.py``): the temperature ``t``, one
+    DOF per node, steady (elliptic scheme) or transient (parabolic scheme).
+
+    ``K_e = sum_rigi k wJ dN^T dN`` and ``C_e = sum_mass rho c wJ N^T N``
+    (``_thermal.py:114-137``).  The reference scales both by the thickness
+    only when the simulation's dimension -- the model's, 1 for a thermal
+    model (``_simu.py:621``) -- is 2, so its element matrices carry no
+    thickness, while line and surface loads on 2D meshes do
+    (``_simu.py:2572-2630``); both are mirrored."""
+
+    def __init__(self, mesh: Mesh, model: ThermalModel, folder: str = "", verbosity=False):
+        assert isinstance(model, ThermalModel), "model must be a thermal model"
+        self._needUpdate = True
+        super().__init__(mesh, model, folder, verbosity)
+        self._rho = 1.0
+        self._K = None
+        self._C = None
+        self._u = torch.zeros(mesh.Nn, dtype=torch.float64, device="cuda")
+        self._v = torch.zeros(mesh.Nn, dtype=torch.float64, device="cuda")
+        self._a = torch.zeros(mesh.Nn, dtype=torch.float64, device="cuda")
+
+    def Get_problemTypes(self) -> list:
+        return ["thermal"]
+
+    def Get_unknowns(self, problemType=None) -> list:
+        return ["t"]
+
+    def Get_dof_n(self, problemType=None) -> int:
+        return 1
+
+    @property
+    def thermalModel(self) -> ThermalModel:
+        return self.model
+
+    @property
+    def rho(self):
+        """Density, a scalar or one value per element (``_simu.py:706``)."""
+        return self._rho
+
+    @rho.setter
+    def rho(self, value):
+        arr = np.asarray(value, dtype=float)
+        assert np.all(arr >= 0.0), "rho must be >= 0"
+        self._rho = float(arr) if arr.ndim == 0 else arr
+        self.Need_Update()
+
+    @property
+    def needUpdate(self) -> bool:
+        return self._needUpdate
+
+    def Need_Update(self, value: bool = True) -> None:
+        self._needUpdate = value
+
+    def Assembly(self):
+        """Conduction ``K`` (``efea_reaction_diffusion_ke`` without reaction)
+        and capacity ``C`` (``efea_uv_ke``) on the device, scattered."""
+        dev, m = self.dev, self.thermalModel
+        K = dev.matrix("K", 1)
+        C = dev.matrix("C", 1)
+        tab_m, nPgm = dev.tables(MatrixType.mass)
+        tab_r, nPgr = dev.tables(MatrixType.rigi)
+        k_het = isinstance(m.k, np.ndarray)
+        k_dev = to_dev(np.ascontiguousarray(np.atleast_1d(m.k), dtype=np.float64))
+        Ke = K.new_entries()
+        _lib.call("efea_reaction_diffusion_ke", dev.code, 64, dev.Ne, ptr(dev.connect), ptr(dev.coord), ptr(tab_m),
+                  nPgm, ptr(tab_r), nPgr, None, 0.0, ptr(k_dev), 1 if k_het else 0, None, 1.0, ptr(Ke), None,
+                  stream_ptr())
+        K.scatter(Ke)
+        coef = self._rho * m.c
+        c_het = isinstance(coef, np.ndarray)
+        c_dev = to_dev(np.ascontiguousarray(coef, dtype=np.float64)) if c_het else None
+        Ce = C.new_entries()
+        _lib.call("efea_uv_ke", dev.code, 64, dev.Ne, ptr(dev.connect), ptr(dev.coord), ptr(tab_m), nPgm, ptr(c_dev),
+                  1 if c_het else 0, 0.0 if c_het else float(coef), 1, 1.0, ptr(Ce), stream_ptr())
+        C.scatter(Ce)
+        self._K, self._C = K, C
+        self._needUpdate = False
+        return K
+
+    def _device_KC(self):
+        if self._needUpdate or self._K is None:
+            self.Assembly()
+        return self._K, self._C
+
+    def _time_bases(self, problemType) -> dict:
+        K, C = self._device_KC()
+        return {"K": K, "C": C}
+
+    def Get_K_C_M_F(self, problemType=None):
+        """Fresh scipy copies ``(K, C, M, F)``: conduction, capacity, ``M = 0``
+        and no assembled load (``_simu.py:161-175``)."""
+        K, C = self._device_KC()
+        K, C = K.to_scipy(), C.to_scipy()
+        return K, C, sp.csr_matrix(K.shape), sp.csr_matrix((K.shape[0], 1))
+
+    @property
+    def thermal(self) -> np.ndarray:
+        return self._u.cpu().numpy()
+
+    @property
+    def thermalDot(self) -> np.ndarray:
+        return self._v.cpu().numpy()
+
+    def Get_x0(self, problemType=None) -> np.ndarray:
+        return self.thermal
+
+    def _solve_gen(self):
+        if self.algo == AlgoType.elliptic:
+            K, _ = self._device_KC()
+            x = self._u.clone()
+            yield from self._solve_system_gen(K, self._neumann_device("thermal"), x, "thermal")
+            self._u = x
+        else:
+            self._u, self._v, self._a = yield from self._time_step_gen("thermal", self._u, self._v, self._a)
+
+    def Solve(self) -> np.ndarray:
+        drive(self._solve_gen())
+        return self.thermal
+
+    def Calc_Reaction(self, dofs=None, problemType=None) -> np.ndarray:
+        """``K t`` at ``dofs`` (all by default)."""
+        K, _ = self._device_KC()
+        y = K.spmv(self._u)
+        dofs = self.Get_dofs() if dofs is None else np.asarray(dofs)
+        return y.cpu().numpy()[dofs]
+
+    def Save_Iter(self, iter: Optional[dict] = None) -> None:
+        """Temperature, plus its rate under the parabolic scheme (``_thermal.py:139-150``)."""
+        it = {} if iter is None else dict(iter)
+        it["thermal"] = self.thermal
+        if self.algo == AlgoType.parabolic:
+            it["thermalDot"] = self.thermalDot
+        super().Save_Iter(it)
+
+    def Set_Iter(self, iter: int = -1) -> Optional[dict]:
+        r = super().Set_Iter(iter)
+        if r is not None:
+            self._u = to_dev(r["thermal"])
+            if self.algo == AlgoType.parabolic and "thermalDot" in r:
+                self._v = to_dev(r["thermalDot"])
+            else:
+                self._v = torch.zeros_like(self._u)
+        return r
+
+    def Results_Available(self) -> list:
+        return ["thermal", "thermalDot", "displacement_matrix"]
+
+    def Result(self, result: str, nodeValues: bool = True, iter: Optional[int] = None):
+        """``Thermal.Result`` (``_thermal.py:171-200``)."""
+        if iter is not None:
+            self.Set_Iter(iter)
+        if result not in self.Results_Available():
+            print(f"\n The result '{result}' is not available.\n Available results: {self.Results_Available()}")
+            return None
+        if result == "thermal":
+            values = self.thermal
+        elif result == "thermalDot":
+            values = self.thermalDot
+        else:
+            values = np.zeros((self.mesh.Nn, 3))
         return self._reshape_values(values, nodeValues)
 
 

@@ -30,3 +30,31 @@ REFERENCE_ONLY = ("pypardiso", "petsc", "scipy", "lsq_linear", "cg", "bicg", "gm
 
 class ResolType(str, Enum):
     r1 = "1"
+
+
+class AlgoType(str, Enum):
+    """Time scheme of ``K u^t + C v^t + M a^t = F^t`` (``Simulations/Solvers.py:64-190``):
+    ``elliptic`` (``K u = F``), ``parabolic`` (generalized trapezoidal,
+    ``alpha``), and the hyperbolic ``newmark``, ``midpoint``, ``hht``,
+    ``hht_newmark``, ``euler_implicit`` and ``euler_explicit``."""
+
+    elliptic = "elliptic"
+    parabolic = "parabolic"
+    newmark = "newmark"
+    midpoint = "midpoint"
+    hht = "hht"
+    hht_newmark = "hht_newmark"
+    euler_implicit = "euler_implicit"
+    euler_explicit = "euler_explicit"
+
+    def __str__(self) -> str:
+        return self.name
+
+    @staticmethod
+    def Get_Hyperbolic_Types() -> list:
+        return [AlgoType.newmark, AlgoType.midpoint, AlgoType.hht, AlgoType.hht_newmark, AlgoType.euler_implicit,
+                AlgoType.euler_explicit]
+
+    @staticmethod
+    def Get_Hyperbolic_and_Parabolic_Types() -> list:
+        return AlgoType.Get_Hyperbolic_Types() + [AlgoType.parabolic]

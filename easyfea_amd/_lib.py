@@ -70,7 +70,14 @@ SIGNATURES = {
     "efea_pf_elastic_ke": (i32, [i32, i64, vp, vp, vp, i32, vp, vp, vp, i32, f64, i32, i32, f64, vp, vp]),
     "efea_pf_split_points": (i32, [i64, i32, vp, vp, i32, f64, i32, vp, vp, vp, vp, vp]),
     "efea_pf_strain_stress": (i32, [i32, i64, vp, vp, vp, i32, vp, vp, vp, i32, f64, i32, i32, f64, vp, vp]),
+    "efea_uv_ke": (i32, [i32, i32, i64, vp, vp, vp, i32, vp, i32, f64, i32, f64, vp, vp]),
+    "efea_lincomb": (i32, [i64, f64, vp, f64, vp, f64, vp, f64, vp, vp, vp]),
+    "efea_time_update": (i32, [i32, i64, f64, f64, f64, f64, vp, vp, vp, vp, vp, vp, vp, vp]),
 }
+
+# time schemes (EFEA_SCHEME_* of include/efea.h)
+SCHEME_CODE = {"elliptic": 0, "parabolic": 1, "newmark": 2, "midpoint": 3, "hht": 4, "hht_newmark": 5,
+               "euler_implicit": 6, "euler_explicit": 7}
 
 ELEM_CODE = {"TRI3": 0, "QUAD4": 1, "HEXA8": 2, "TETRA4": 3, "TETRA10": 4, "SEG2": 5}
 

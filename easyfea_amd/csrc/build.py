@@ -7,11 +7,14 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 ROOT = os.path.dirname(PKG)
-SOURCES = ["capi.hip", "elements.hip", "sparse.hip", "batch.hip", "coarse.hip", "phasefield.hip", "postproc.hip"]
+SOURCES = ["capi.hip", "elements.hip", "sparse.hip", "batch.hip", "coarse.hip", "phasefield.hip", "postproc.hip",
+           "dynamics.hip"]
 # The phase-field split branches on exact equality of eigenvalues (as the
 # reference does); contracting a*b - c*d into an FMA changes which states
 # compare equal, so that file is compiled operation by operation.
-EXTRA = {"phasefield.hip": ["-ffp-contract=off"]}
+# The time-scheme state update mirrors the reference's numpy expressions
+# operation by operation, which also needs uncontracted arithmetic.
+EXTRA = {"phasefield.hip": ["-ffp-contract=off"], "dynamics.hip": ["-ffp-contract=off"]}
 OUT = os.path.join(PKG, "libefea_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMPILE_FLAGS = [

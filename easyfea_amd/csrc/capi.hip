@@ -65,6 +65,12 @@ int pf_strain_stress(int, int64_t, const int*, const double*, const double*, int
                      const double*, int, double, int, int, double, double*, hipStream_t);
 int pf_split_points(int64_t, int, const double*, const double*, int, double, int, double*, double*, double*,
                     double*, hipStream_t);
+int uv_ke(int, int, int64_t, const int*, const double*, const double*, int, const double*, int, double, int, double,
+          void*, hipStream_t);
+int lincomb(int64_t, double, const double*, double, const double*, double, const double*, double, const double*,
+            double*, hipStream_t);
+int time_update(int, int64_t, double, double, double, double, const double*, const double*, const double*,
+                const double*, double*, double*, double*, hipStream_t);
 int batch_create(int, int, int, int, int, void**);
 int batch_destroy(void*);
 int batch_add(void*, int, const int*, const int*, const double*, const float*, const float*, const unsigned char*,
@@ -457,6 +463,44 @@ int efea_pf_split_points(int64_t n, int split, const double* eps, const double* 
     return EFEA_ERR_ARG;
   }
   return pf_split_points(n, split, eps, E, E_stride, nu, plane_stress, cP, cM, psiP, psiM, STREAM(stream));
+}
+
+int efea_uv_ke(int elem_type, int precision, int64_t Ne, const int32_t* connect, const double* coord,
+               const double* tables_mass, int nPg_mass, const double* coef, int coef_stride, double coef_const,
+               int dof_n, double scale, void* Ke, void* stream) {
+  if (Ne < 0 || (Ne > 0 && (!connect || !coord || !tables_mass || !Ke))) {
+    set_error("efea_uv_ke: null pointer or negative size");
+    return EFEA_ERR_ARG;
+  }
+  return uv_ke(elem_type, precision, Ne, connect, coord, tables_mass, nPg_mass, coef, coef_stride, coef_const, dof_n,
+               scale, Ke, STREAM(stream));
+}
+
+int efea_lincomb(int64_t n, double c0, const double* x0, double c1, const double* x1, double c2, const double* x2,
+                 double c3, const double* x3, double* out, void* stream) {
+  if (n < 0 || (n > 0 && (!x0 || !out))) {
+    set_error("efea_lincomb: null pointer or negative size");
+    return EFEA_ERR_ARG;
+  }
+  return lincomb(n, c0, x0, c1, x1, c2, x2, c3, x3, out, STREAM(stream));
+}
+
+int efea_time_update(int scheme, int64_t n, double dt, double beta, double gamma, double alpha, const double* u_n,
+                     const double* v_n, const double* a_n, const double* x, double* u_out, double* v_out,
+                     double* a_out, void* stream) {
+  if (n < 0 || (n > 0 && (!u_n || !v_n || !a_n || !x || !u_out))) {
+    set_error("efea_time_update: null pointer or negative size");
+    return EFEA_ERR_ARG;
+  }
+  if (scheme < EFEA_SCHEME_ELLIPTIC || scheme > EFEA_SCHEME_EULER_EXPLICIT) {
+    set_error("efea_time_update: unknown scheme %d", scheme);
+    return EFEA_ERR_ARG;
+  }
+  if (scheme != EFEA_SCHEME_ELLIPTIC && !(dt > 0)) {
+    set_error("efea_time_update: dt must be > 0");
+    return EFEA_ERR_ARG;
+  }
+  return time_update(scheme, n, dt, beta, gamma, alpha, u_n, v_n, a_n, x, u_out, v_out, a_out, STREAM(stream));
 }
 
 }  // extern "C"

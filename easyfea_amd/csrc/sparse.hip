@@ -430,8 +430,13 @@ static int pcg_finish(const PcgState& hs, double rr_true, double* info) {
   const int iters = hs.iters;
   info[0] = iters;
   info[1] = hs.bb > 0 ? sqrt(rr_true / hs.bb) : 0.0;
-  // a "converged" iterate whose true residual is not small is a divergence
-  if (done == 1 && !(info[1] <= 1e-4)) done = 3;
+  // A "converged" iterate whose true residual is not small is a divergence,
+  // unless the iterations changed x by a negligible energy (sum alpha r.z
+  // within the A-norm tolerance of x0.A x0): a start at the solution (a
+  // steady state carried into a transient step, a repeated solve) has an
+  // initial residual at round-off that no iteration reduces further.
+  const bool moved = !(hs.Sall <= hs.tolA2 * hs.xAx0);
+  if (done == 1 && !(info[1] <= 1e-4) && moved) done = 3;
   info[2] = hs.nrepl;
   info[3] = done;
   if (done == 2 || done == 3) {

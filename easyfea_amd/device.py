@@ -133,7 +133,12 @@ class DeviceMesh:
     def csr_map(self, dof_n: int) -> "DeviceCsr":
         """This simulation's matrix with ``dof_n`` DOFs per node (values of
         its own over the mesh's shared pattern)."""
-        key = ("K", dof_n)
+        return self.matrix("K", dof_n)
+
+    def matrix(self, name: str, dof_n: int) -> "DeviceCsr":
+        """This simulation's matrix ``name`` (e.g. "K", "M", "C", the scheme
+        matrix "A") with ``dof_n`` DOFs per node, over the shared pattern."""
+        key = (name, dof_n)
         if key not in self._maps:
             self._maps[key] = DeviceCsr(self.pattern(dof_n))
         return self._maps[key]
@@ -255,6 +260,7 @@ class DeviceCsr:
         self.values32: Optional[torch.Tensor] = None
         self.corr32: Optional[torch.Tensor] = None
         self._spmv_scratch: Optional[torch.Tensor] = None
+        self.version = 0  # counts value changes (scatter / combine)
 
     def coarse(self, target: Optional[int] = None) -> CoarseSpace:
         return self.pattern.coarse(target)
@@ -279,6 +285,20 @@ class DeviceCsr:
                   stream_ptr())
         self.values32 = None
         self.corr32 = None
+        self.version += 1
+
+    def combine(self, terms: list) -> None:
+        """values = sum of c * X.values over ``terms`` [(c, DeviceCsr)], 1 to 4
+        matrices on this pattern (``efea_lincomb``)."""
+        assert 1 <= len(terms) <= 4 and all(X.pattern is self.pattern for _, X in terms)
+        args = []
+        for k in range(4):
+            c, X = terms[k] if k < len(terms) else (0.0, None)
+            args += [float(c), ptr(X.values) if X is not None else None]
+        _lib.call("efea_lincomb", self.nnz, *args, ptr(self.values), stream_ptr())
+        self.values32 = None
+        self.corr32 = None
+        self.version += 1
 
     def make_fp32(self) -> torch.Tensor:
         """FP32 copy of the values and its row-sum correction (``efea_split_f32``)."""
@@ -309,6 +329,20 @@ class DeviceCsr:
 # measured step-1 kernel times are equal in FP32 and FP64 for matrices of a
 # few MB (latency floor) and 23% apart at 517 MB (64^3 HEXA8).
 MIXED_MIN_MATRIX_BYTES = 128 * 2**20
+
+
+def lincomb(out: torch.Tensor, terms: list) -> torch.Tensor:
+    """out = sum of c * x over ``terms`` [(c, x)], 1 to 4 FP64 device
+    vectors of out's size (``efea_lincomb``; out may be one of the x)."""
+    assert 1 <= len(terms) <= 4
+    args = []
+    for k in range(4):
+        c, x = terms[k] if k < len(terms) else (0.0, None)
+        if x is not None:
+            assert x.dtype == torch.float64 and x.numel() == out.numel() and x.is_contiguous()
+        args += [float(c), ptr(x)]
+    _lib.call("efea_lincomb", out.numel(), *args, ptr(out), stream_ptr())
+    return out
 
 
 def effective_precision(A: "DeviceCsr", mode: str, adaptive: bool = True) -> int:

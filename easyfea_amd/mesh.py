@@ -110,6 +110,10 @@ class Mesh:
         return self.groupElem.Ne
 
     @property
+    def nPe(self) -> int:
+        return self.groupElem.nPe
+
+    @property
     def elemType(self) -> ElemType:
         return self.groupElem.elemType
 

@@ -283,6 +283,45 @@ int efea_pf_strain_stress(int elem_type, int64_t Ne, const int32_t* connect, con
 int efea_pf_split_points(int64_t n, int split, const double* eps, const double* E, int E_stride, double nu,
                          int plane_stress, double* cP, double* cM, double* psiP, double* psiM, void* stream);
 
+/* ---- time integration (parabolic and hyperbolic schemes) ---------------- */
+
+/* Time schemes, the reference's AlgoType (Simulations/Solvers.py:64-190). */
+#define EFEA_SCHEME_ELLIPTIC 0
+#define EFEA_SCHEME_PARABOLIC 1
+#define EFEA_SCHEME_NEWMARK 2
+#define EFEA_SCHEME_MIDPOINT 3
+#define EFEA_SCHEME_HHT 4
+#define EFEA_SCHEME_HHT_NEWMARK 5
+#define EFEA_SCHEME_EULER_IMPLICIT 6
+#define EFEA_SCHEME_EULER_EXPLICIT 7
+
+/* Mass / capacity element matrices over the mass rule:
+ * Ke[e, (a,i), (b,j)] = delta_ij sum_p coef wJ N_a N_b x scale, (Ne, nPe dof_n, nPe dof_n),
+ * dof_n 1 or the element dimension.  coef: per point (coef_stride = nPg), per
+ * element (coef_stride = 1) or NULL (then coef_const).  precision 64.
+ * Replaces Operators.Bilinear.UV with dof_n (FEM/Operators/Bilinear.py:42-59),
+ * the M_e of Elastic.Construct_local_matrix_system (Simulations/_elastic.py:135-148)
+ * and the C_e of Thermal.Construct_local_matrix_system (Simulations/_thermal.py:114-130). */
+int efea_uv_ke(int elem_type, int precision, int64_t Ne, const int32_t* connect, const double* coord,
+               const double* tables_mass, int nPg_mass, const double* coef, int coef_stride, double coef_const,
+               int dof_n, double scale, void* Ke, void* stream);
+
+/* out = c0 x0 + c1 x1 + c2 x2 + c3 x3 over n entries (x1..x3 may be NULL: term absent;
+ * out may be one of the x).  Forms A = coefK K + coefC C + coefM M over one pattern's
+ * values (Simulations/_simu.py:1895-1899) and the history vectors of the
+ * time-scheme right-hand sides (_Solver_Apply_Neumann, _simu.py:1735-1860). */
+int efea_lincomb(int64_t n, double c0, const double* x0, double c1, const double* x1, double c2, const double* x2,
+                 double c3, const double* x3, double* out, void* stream);
+
+/* The state after a time-step solve: (u, v, a) from (u_n, v_n, a_n) and the
+ * solve's unknown x (u^{n+1}, or a^n for EFEA_SCHEME_EULER_EXPLICIT), the
+ * reference's expressions evaluated in its operation order (bit-identical to
+ * them).  v_out / a_out may be NULL; outputs may alias inputs.
+ * Replaces _Simu._Solver_Update_solutions (Simulations/_simu.py:1552-1640). */
+int efea_time_update(int scheme, int64_t n, double dt, double beta, double gamma, double alpha, const double* u_n,
+                     const double* v_n, const double* a_n, const double* x, double* u_out, double* v_out,
+                     double* a_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

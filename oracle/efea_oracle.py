@@ -1046,3 +1046,235 @@ class PhaseFieldOracle(_SimuBase):
         """``_Simu.Calc_Reaction`` (``_simu.py:212-260``), elliptic."""
         K = self.Get_K("elastic")
         return K[dofs] @ self.u
+
+
+# ---------------------------------------------------------------------------
+# a24-a28: time schemes, dynamic elastic and thermal simulations
+# ---------------------------------------------------------------------------
+
+HYPERBOLIC = ("newmark", "midpoint", "hht", "hht_newmark", "euler_implicit", "euler_explicit")
+
+
+def hyperbolic_params(dt, algo="newmark", beta=0.25, gamma=0.5, alpha=0.5):
+    """``Solver_Set_Hyperbolic_Algorithm`` (``Simulations/_simu.py:1214-1281``):
+    ``(dt, beta, gamma, alpha)``, hht_newmark deriving beta and gamma."""
+    assert algo in HYPERBOLIC and dt > 0
+    if algo == "hht_newmark":
+        assert 0 <= alpha <= 1 / 3
+        beta = 1 / 4 * (1 + alpha) ** 2
+        gamma = 1 / 2 + alpha
+    else:
+        assert 0 <= alpha < 1
+    return (dt, beta, gamma, alpha)
+
+
+def time_coefs(algo, params):
+    """``_Solver_Get_K_C_M_coefs_for_time_scheme`` (``_simu.py:1384-1436``)."""
+    if algo == "parabolic":
+        dt, alpha = params
+        return 1, 1 / (alpha * dt), 0
+    dt, beta, gamma, alpha = params
+    return {
+        "newmark": (1, gamma / (beta * dt), 1 / (beta * dt ** 2)),
+        "hht": (1 - alpha, (1 - alpha) * gamma / (beta * dt), (1 - alpha) / (beta * dt ** 2)),
+        "midpoint": (0.5, 1 / dt, 2 / dt ** 2),
+        "hht_newmark": (1 - alpha, gamma / (beta * dt), 1 / (beta * dt ** 2)),
+        "euler_implicit": (1, 1 / dt, 1 / dt ** 2),
+        "euler_explicit": (0, 0, 1),
+    }[algo]
+
+
+def time_rhs(algo, params, K, C, M, F, u_n, v_n, a_n):
+    """``_Solver_Apply_Neumann`` (``_simu.py:1735-1860``): ``b`` of the step
+    with the history terms applied as the reference writes them (matrix sums
+    times predictors).  ``F``: the Neumann vector (dense)."""
+    b = np.array(F, dtype=float)
+    if algo == "parabolic":
+        dt, alpha = params
+        ut = u_n + (1 - alpha) * dt * v_n
+        return b + 1 / (alpha * dt) * (C @ ut)
+    dt, beta, gamma, alpha = params
+    if algo in ("newmark", "hht_newmark"):
+        ut = u_n + dt * v_n + dt ** 2 / 2 * (1 - 2 * beta) * a_n
+        vt = v_n + dt * (1 - gamma) * a_n
+        cC, cM = gamma / (beta * dt), 1 / (beta * dt ** 2)
+        b = b + (cC * C + cM * M) @ ut
+        b = b - C @ vt
+        if algo == "hht_newmark":
+            b = b - alpha * (K @ u_n)
+        return b
+    if algo == "midpoint":
+        b = b + (2 / dt ** 2 * M + 1 / dt * C - 1 / 2 * K) @ u_n
+        return b + (2 / dt * M) @ v_n
+    if algo == "hht":
+        cM, cC = 1 / (beta * dt ** 2), gamma / (beta * dt)
+        b = b - ((alpha - 1) * (cM * M + cC * C) + alpha * K) @ u_n
+        cM, cC = (alpha - 1) / (beta * dt), (alpha - 1) * (gamma / beta) + 1
+        b = b - (cM * M + cC * C) @ v_n
+        cM, cC = (alpha - 1) / (2 * beta) + 1, dt * (alpha - 1) * (gamma / (2 * beta) - 1)
+        return b - (cM * M + cC * C) @ a_n
+    if algo == "euler_implicit":
+        b = b + (1 / dt ** 2 * M + 1 / dt * C) @ u_n
+        return b + (1 / dt * M) @ v_n
+    if algo == "euler_explicit":
+        return b - K @ u_n - C @ v_n
+    raise ValueError(algo)
+
+
+def time_update(algo, params, u_n, v_n, a_n, u_np1):
+    """``_Solver_Update_solutions`` (``_simu.py:1552-1640``): ``(u, v, a)``
+    after the step (``None`` = unchanged), in the reference's operation order."""
+    if algo == "parabolic":
+        dt, alpha = params
+        vt_np1 = u_n + ((1 - alpha) * dt * v_n)
+        return u_np1, (u_np1 - vt_np1) / (alpha * dt), None
+    dt, beta, gamma, alpha = params
+    if algo in ("newmark", "hht_newmark"):
+        ut_np1 = u_n + dt * v_n + dt ** 2 / 2 * (1 - 2 * beta) * a_n
+        vt_np1 = v_n + dt * (1 - gamma) * a_n
+        a_np1 = (u_np1 - ut_np1) / (beta * dt ** 2)
+        return u_np1, vt_np1 + gamma * dt * a_np1, a_np1
+    if algo == "midpoint":
+        v_np1 = 2 / dt * (u_np1 - u_n) - v_n
+        return u_np1, v_np1, 2 / dt * (v_np1 - v_n) - a_n
+    if algo == "hht":
+        a_np1 = 1 / (beta * dt) * ((u_np1 - u_n) / dt - v_n) + (1 - 1 / (2 * beta)) * a_n
+        return u_np1, dt * ((1 - gamma) * a_n + gamma * a_np1) + v_n, a_np1
+    if algo == "euler_implicit":
+        v_np1 = (u_np1 - u_n) / dt
+        return u_np1, v_np1, (v_np1 - v_n) / dt
+    if algo == "euler_explicit":
+        a_np1 = u_np1
+        return u_n + dt * v_n, v_n + dt * a_np1, a_np1
+    raise ValueError(algo)
+
+
+def time_evaluate(algo, params, u_n, v_n, a_n, u_np1):
+    """``_Solver_Evaluate_u_v_a_for_time_scheme`` (``_simu.py:1292-1382``):
+    the states ``(u_t, v_t, a_t)`` at which the scheme enforces
+    ``K u_t + C v_t + M a_t = F`` (``a_t`` None: no mass term)."""
+    if algo == "parabolic":
+        dt, alpha = params
+        ut_np1 = u_n + (1 - alpha) * dt * v_n
+        return u_np1, (u_np1 - ut_np1) / (alpha * dt), None
+    dt, beta, gamma, alpha = params
+    if algo == "newmark":
+        ut_np1 = u_n + dt * v_n + dt ** 2 / 2 * (1 - 2 * beta) * a_n
+        vt_np1 = v_n + dt * (1 - gamma) * a_n
+        a_t = (u_np1 - ut_np1) / (beta * dt ** 2)
+        return u_np1, vt_np1 + gamma * dt * a_t, a_t
+    if algo == "hht":
+        a_np1 = 1 / (beta * dt ** 2) * (u_np1 - u_n - dt * v_n) + (1 - 1 / (2 * beta)) * a_n
+        v_np1 = dt * ((1 - gamma) * a_n + gamma * a_np1) + v_n
+        return ((1 - alpha) * u_np1 + alpha * u_n, (1 - alpha) * v_np1 + alpha * v_n,
+                (1 - alpha) * a_np1 + alpha * a_n)
+    if algo == "midpoint":
+        v_np1 = 2 / dt * (u_np1 - u_n) - v_n
+        a_np1 = 2 / dt * (v_np1 - v_n) - a_n
+        return (u_np1 + u_n) / 2, (v_np1 + v_n) / 2, (a_np1 + a_n) / 2
+    if algo == "hht_newmark":
+        a_np1 = 1 / (beta * dt ** 2) * (u_np1 - u_n - dt * v_n) + (1 - 1 / (2 * beta)) * a_n
+        v_np1 = dt * ((1 - gamma) * a_n + gamma * a_np1) + v_n
+        return (1 - alpha) * u_np1 + alpha * u_n, v_np1, a_np1
+    if algo == "euler_implicit":
+        v_np1 = (u_np1 - u_n) / dt
+        return u_np1, v_np1, (v_np1 - v_n) / dt
+    if algo == "euler_explicit":
+        return u_n, v_n, None
+    raise ValueError(algo)
+
+
+def time_step(algo, params, K, C, M, F, u_n, v_n, a_n, dir_dofs=(), dir_vals=(), orphan_dofs=()):
+    """One step of ``_Solver_Solve_problemType`` (``_simu.py:1529-1550``):
+    ``A = coefK K + coefC C + coefM M`` (:1893-1899), ``b`` from
+    ``time_rhs``, the r1 solve with the Dirichlet values (zeros for the
+    explicit scheme's unknown ``a^n``, :1905-1907), then ``time_update``.
+    Returns ``(u, v, a)`` with unchanged entries carried over."""
+    Ndof = len(u_n)
+    cK, cC, cM = time_coefs(algo, params)
+    A = (cK * K + cC * C + cM * M).tocsr()
+    b = time_rhs(algo, params, K, C, M, F, u_n, v_n, a_n)
+    vals = np.zeros(len(dir_vals)) if algo == "euler_explicit" else np.asarray(dir_vals, float)
+    x = solve_r1(A, b, np.asarray(dir_dofs, int), vals, Ndof, orphan_dofs)
+    u, v, a = time_update(algo, params, u_n, v_n, a_n, x)
+    return u, (v_n if v is None else v), (a_n if a is None else a)
+
+
+class DynamicElasticOracle(ElasticOracle):
+    """``Simulations.Elastic`` with a time scheme: mass ``M_e = UV(rho,
+    dof_n = dim)`` over the mass rule and Rayleigh ``C = coefK K + coefM M``
+    (``Simulations/_elastic.py:123-156``), element matrices times the
+    thickness in 2D."""
+
+    def __init__(self, *args, rho=1.0, rayleigh=(0.0, 0.0), **kw):
+        super().__init__(*args, **kw)
+        self.rho = rho
+        self.rayleigh = rayleigh  # (coefM, coefK)
+        self.speed = np.zeros(self.Ndof)
+        self.accel = np.zeros(self.Ndof)
+
+    def Get_M(self):
+        geo = Geometry(self.elemType, self.connect, self.coord, "mass", self.dim)
+        Me = UV(geo, self.rho, self.dim)
+        if self.dim == 2:
+            Me = Me * self.thickness
+        if self._cmap is None:
+            self._cmap = csr_map(self.connect, self.dim, self.Ndof)
+        return assemble_csr(Me, self._cmap, self.Ndof)
+
+    def Get_K_C_M(self):
+        K, M = self.Get_K(), self.Get_M()
+        cM, cK = self.rayleigh
+        return K, (cK * K + cM * M).tocsr(), M
+
+    def Step(self, algo, params):
+        K, C, M = self.Get_K_C_M()
+        F = self.neumann_vector("elastic", self.Ndof)
+        dofs, vals = self.dir.get("elastic", (np.array([], int), np.array([])))
+        orphan = dofs_nodes(self.orphan, self.unknowns("elastic"), self.unknowns("elastic"))
+        self.u, self.speed, self.accel = time_step(algo, params, K, C, M, F, self.u, self.speed, self.accel, dofs,
+                                                   vals, orphan)
+        return self.u
+
+
+class ThermalOracle(_SimuBase):
+    """``Simulations.Thermal`` (``Simulations/_thermal.py``): ``K_e =
+    GradUGradV(k)`` over the rigi rule and ``C_e = UV(rho c)`` over the mass
+    rule, neither scaled by the thickness (the simulation's dimension is the
+    model's, 1; ``_simu.py:621``, ``_thermal.py:130-134``); loads on 2D meshes
+    are (``add_surfLoad``, ``_simu.py:2572-2630``)."""
+
+    def __init__(self, connect, coord, elemType, k, c, rho=1.0, thickness=1.0, boundary=None):
+        dim = ELEM_DIM[elemType]
+        super().__init__(connect, coord, elemType, dim, boundary or {})
+        self.k, self.c, self.rho, self.thickness = k, c, rho, thickness
+        self.cmap = csr_map(self.connect, 1, self.Nn)
+        self.u = np.zeros(self.Nn)
+        self.speed = np.zeros(self.Nn)
+        self.accel = np.zeros(self.Nn)
+
+    def unknowns(self, problemType):
+        return ["t"]
+
+    def add_surfLoad(self, nodes, values, unknowns, problemType="thermal"):
+        super().add_surfLoad(nodes, values, unknowns, problemType, self.thickness)
+
+    def Get_K_C(self):
+        geo_r = Geometry(self.elemType, self.connect, self.coord, "rigi", self.dim)
+        geo_m = Geometry(self.elemType, self.connect, self.coord, "mass", self.dim)
+        Ke = GradU_A_GradV(geo_r, np.eye(self.dim), self.k)
+        Ce = UV(geo_m, np.asarray(self.rho) * np.asarray(self.c), 1)
+        return assemble_csr(Ke, self.cmap, self.Nn), assemble_csr(Ce, self.cmap, self.Nn)
+
+    def Step(self, algo="elliptic", params=None):
+        K, C = self.Get_K_C()
+        F = self.neumann_vector("thermal", self.Nn)
+        dofs, vals = self.dir.get("thermal", (np.array([], int), np.array([])))
+        orphan = dofs_nodes(self.orphan, ["t"], ["t"])
+        if algo == "elliptic":
+            self.u = solve_r1(K, F, dofs, vals, self.Nn, orphan)
+        else:
+            M = sp.csr_matrix(K.shape)
+            self.u, self.speed, self.accel = time_step(algo, params, K, C, M, F, self.u, self.speed, self.accel,
+                                                       dofs, vals, orphan)
+        return self.u

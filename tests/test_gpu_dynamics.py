@@ -1,0 +1,283 @@
+"""Time integration on the device vs the oracle (GPU tests).
+
+* ``efea_uv_ke`` (mass / capacity element matrices) within 1e-14 relative;
+* ``efea_time_update`` bit-identical to the reference's update expressions
+  for every scheme, ``efea_lincomb`` bit-identical to the numpy sum;
+* dynamic ``Simulations.Elastic`` (every hyperbolic scheme, Rayleigh
+  damping, surface load, clamp) and ``Simulations.Thermal`` (steady and
+  parabolic, per-element k and c) within 1e-8 of the oracle's direct solves
+  over several steps; scipy copies of K, C, M against the oracle's assembly;
+* Save_Iter / Set_Iter restore speed and acceleration.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+import easyfea_amd as ea  # noqa: E402
+from easyfea_amd import Models, Simulations, _lib  # noqa: E402
+from easyfea_amd._utils import MatrixType  # noqa: E402
+from easyfea_amd.device import DeviceMesh, ptr, stream_ptr, to_dev  # noqa: E402
+from oracle import efea_oracle as O  # noqa: E402
+
+ALGOS = ["newmark", "midpoint", "hht", "hht_newmark", "euler_implicit", "euler_explicit"]
+MAKERS = {
+    "TRI3": lambda: ea.tri3_rectangle(2.0, 1.0, 8, 4),
+    "QUAD4": lambda: ea.quad4_rectangle(2.0, 1.0, 8, 4),
+    "HEXA8": lambda: ea.hexa8_box(2.0, 1.0, 1.0, 5, 3, 3),
+    "TETRA4": lambda: ea.mesh.tetra4_box(2.0, 1.0, 1.0, 4, 2, 2),
+    "TETRA10": lambda: ea.mesh.tetra10_from_tetra4(ea.mesh.tetra4_box(2.0, 1.0, 1.0, 3, 2, 2)),
+}
+BOUNDARY = {2: "SEG2", 3: "QUAD4"}
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _distorted(et, amp=0.03, seed=0):
+    m = MAKERS[et]()
+    if et == "TETRA10":
+        return m
+    rng = np.random.default_rng(seed)
+    c = m.coord.copy()
+    inner = np.ones(m.Nn, bool)
+    for k in range(m.dim):
+        inner &= (c[:, k] > c[:, k].min()) & (c[:, k] < c[:, k].max())
+    c[inner, :m.dim] += amp * (rng.random((inner.sum(), m.dim)) - 0.5)
+    return ea.Mesh({t: g.connect for t, g in m.dict_groupElem.items()}, c)
+
+
+# ------------------------------------------------------------------ kernels
+@pytest.mark.parametrize("et", list(MAKERS))
+def test_uv_kernel_matches_the_oracle(et):
+    mesh = _distorted(et)
+    dev = DeviceMesh(mesh)
+    tab, nPg = dev.tables(MatrixType.mass)
+    geo = O.Geometry(et, mesh.connect, mesh.coord, "mass", mesh.dim)
+    rng = np.random.default_rng(1)
+    rho = 1 + rng.random(mesh.Ne)
+    rho_pg = 1 + rng.random((mesh.Ne, nPg))
+    for dof_n in (1, mesh.dim):
+        nd = mesh.nPe * dof_n
+        for coef, stride, const, ref_coef in ((None, 0, 2.5, 2.5), (rho, 1, 0.0, rho), (rho_pg, nPg, 0.0, rho_pg)):
+            out = torch.empty(mesh.Ne * nd * nd, dtype=torch.float64, device="cuda")
+            c_dev = None if coef is None else to_dev(np.ascontiguousarray(coef))
+            print(f"{et}: dof_n {dof_n}, coef stride {stride}", flush=True)
+            _lib.call("efea_uv_ke", dev.code, 64, dev.Ne, ptr(dev.connect), ptr(dev.coord), ptr(tab), nPg, ptr(c_dev),
+                      stride, const, dof_n, 0.5, ptr(out), stream_ptr())
+            ref = 0.5 * O.UV(geo, ref_coef, dof_n)
+            got = out.cpu().numpy().reshape(mesh.Ne, nd, nd)
+            assert _rel(got, ref) < 1e-14
+
+
+@pytest.mark.parametrize("algo", ["elliptic", "parabolic"] + ALGOS)
+def test_time_update_is_bitwise_the_reference_expressions(algo):
+    rng = np.random.default_rng(4)
+    n = 10007
+    u, v, a, x = (rng.standard_normal(n) for _ in range(4))
+    if algo == "parabolic":
+        params, (dt, beta, gamma, alpha) = (0.013, 0.6), (0.013, 0.0, 0.0, 0.6)
+    elif algo == "elliptic":
+        params, (dt, beta, gamma, alpha) = (), (0.0, 0.0, 0.0, 0.0)
+    else:
+        params = O.hyperbolic_params(0.013, algo, 0.27, 0.55, 0.3 if algo != "hht_newmark" else 0.2)
+        dt, beta, gamma, alpha = params
+    d = [to_dev(t) for t in (u, v, a, x)]
+    outs = [torch.full((n,), np.nan, dtype=torch.float64, device="cuda") for _ in range(3)]
+    _lib.call("efea_time_update", _lib.SCHEME_CODE[algo], n, dt, beta, gamma, alpha, *[ptr(t) for t in d],
+              *[ptr(t) for t in outs], stream_ptr())
+    got = [t.cpu().numpy() for t in outs]
+    if algo == "elliptic":
+        ref = (x, v, a)
+    else:
+        ru, rv, ra = O.time_update(algo, params, u, v, a, x)
+        ref = (ru, rv, a if ra is None else ra)
+    for g, r, name in zip(got, ref, "uva"):
+        assert np.array_equal(g, r), f"{algo}: {name} differs"
+
+
+def test_lincomb_is_bitwise_the_numpy_sum():
+    rng = np.random.default_rng(5)
+    xs = [rng.standard_normal(4099) for _ in range(4)]
+    cs = [0.3, -1.7, 2.5e-3, 11.0]
+    d = [to_dev(x) for x in xs]
+    out = torch.empty(4099, dtype=torch.float64, device="cuda")
+    for k in range(1, 5):
+        args = []
+        for i in range(4):
+            args += [cs[i] if i < k else 0.0, ptr(d[i]) if i < k else None]
+        _lib.call("efea_lincomb", 4099, *args, ptr(out), stream_ptr())
+        ref = cs[0] * xs[0]
+        for i in range(1, k):
+            ref = ref + cs[i] * xs[i]
+        assert np.array_equal(out.cpu().numpy(), ref)
+    # in place (out is x0)
+    _lib.call("efea_lincomb", 4099, 2.0, ptr(d[0]), 1.0, ptr(d[1]), 0.0, None, 0.0, None, ptr(d[0]), stream_ptr())
+    assert np.array_equal(d[0].cpu().numpy(), 2.0 * xs[0] + xs[1])
+
+
+# ------------------------------------------------------------------ elastic dynamics
+def _dynamic_pair(et, algo, params, rayleigh=(0.4, 0.002), rho=None):
+    mesh = _distorted(et, seed=2)
+    dim = mesh.dim
+    rng = np.random.default_rng(3)
+    E = 200.0 * np.exp(0.2 * rng.standard_normal(mesh.Ne))
+    rho = (1 + rng.random(mesh.Ne)) if rho is None else rho
+    th = 0.5 if dim == 2 else 1.0
+    mat = Models.Elastic.Isotropic(dim, E, 0.3, True, th) if dim == 2 else Models.Elastic.Isotropic(3, E, 0.3)
+    simu = Simulations.Elastic(mesh, mat)
+    simu.solver = "hip_fp64"
+    simu.tol_anorm = 1e-13
+    simu.tol_residual = 1e-15
+    simu.rho = rho
+    simu.Set_Rayleigh_Damping_Coefs(*rayleigh)
+    unk = ["x", "y", "z"][:dim]
+    clamp = mesh.Nodes_Conditions(lambda x, y, z: x == 0)
+    right = mesh.Nodes_Conditions(lambda x, y, z: x == 2.0)
+    bt = ea.ElemType(BOUNDARY[dim])
+    ref = O.DynamicElasticOracle(mesh.connect, mesh.coord, et, E, 0.3, True, th,
+                                 {bt.value: mesh.dict_groupElem[bt].connect}, rho=rho, rayleigh=rayleigh)
+    for s in (simu, ref):
+        s.add_dirichlet(clamp, [0.0] * dim, unk)
+        s.add_surfLoad(right, [-0.3], ["y"])
+    if algo == "parabolic":
+        simu.Solver_Set_Parabolic_Algorithm(*params)
+    else:
+        simu.Solver_Set_Hyperbolic_Algorithm(params[0], algo, *params[1:])
+    return simu, ref
+
+
+@pytest.mark.parametrize("algo", ALGOS)
+@pytest.mark.parametrize("et", ["QUAD4", "TRI3", "HEXA8"])
+def test_elastic_dynamics_matches_the_oracle(et, algo):
+    dt = 0.002 if algo == "euler_explicit" else 0.05
+    params = O.hyperbolic_params(dt, algo, 0.25, 0.5, 0.15 if algo == "hht_newmark" else 0.3)
+    simu, ref = _dynamic_pair(et, algo, (dt, 0.25, 0.5, params[3]))
+    assert simu._hyperbolicParams == params
+    for step in range(4):
+        simu.Solve()
+        ref.Step(algo, params)
+        print(f"{et} {algo} step {step}: {simu.last_solve['elastic'].iterations} it", flush=True)
+        assert _rel(simu.displacement, ref.u) < 1e-8
+        assert _rel(simu.speed, ref.speed) < 1e-8
+        assert _rel(simu.accel, ref.accel) < 1e-8
+    if algo == "euler_explicit":  # constrained DOFs keep zero acceleration
+        assert np.all(simu.accel[simu.Bc_dofs_Dirichlet()] == 0.0)
+
+
+def test_elastic_scipy_K_C_M_match_the_oracle():
+    simu, ref = _dynamic_pair("QUAD4", "newmark", (0.05, 0.25, 0.5, 0.0))
+    K, C, M, F = simu.Get_K_C_M_F()
+    Kr, Cr, Mr = ref.Get_K_C_M()
+    for a, b in ((K, Kr), (C, Cr), (M, Mr)):
+        assert np.array_equal(a.indptr, b.indptr) and np.array_equal(a.indices, b.indices)
+        assert np.max(np.abs(a.data - b.data)) <= 1e-13 * np.max(np.abs(b.data))
+    assert F.shape == (K.shape[0], 1) and F.nnz == 0
+
+
+def test_elastic_dynamic_save_and_set_iter():
+    simu, _ = _dynamic_pair("QUAD4", "newmark", (0.05, 0.25, 0.5, 0.0))
+    states = []
+    for _ in range(3):
+        simu.Solve()
+        simu.Save_Iter()
+        states.append((simu.displacement, simu.speed, simu.accel))
+    assert {"displacement", "speed", "accel"} <= set(simu.results[0])
+    simu.Set_Iter(1)
+    for got, want in zip((simu.displacement, simu.speed, simu.accel), states[1]):
+        assert np.array_equal(got, want)
+    assert np.array_equal(simu.Result("vy"), states[1][1].reshape(-1, 2)[:, 1])
+    assert np.allclose(simu.Result("accel_norm"), np.linalg.norm(states[1][2].reshape(-1, 2), axis=1))
+    # the scheme matrix is kept while dt and the matrices stay, rebuilt when dt changes
+    simu.Solve()
+    A = simu.dev.matrix("A:elastic", 2)
+    v0 = A.version
+    simu.Solve()
+    assert A.version == v0
+    simu.Solver_Set_Hyperbolic_Algorithm(0.02)
+    simu.Solve()
+    assert A.version == v0 + 1
+
+
+# ------------------------------------------------------------------ thermal
+def _thermal_pair(et, seed=0):
+    mesh = _distorted(et, seed=seed)
+    rng = np.random.default_rng(seed + 10)
+    k = 1 + rng.random(mesh.Ne)
+    c = 0.5 + rng.random(mesh.Ne)
+    rho = 1.3
+    model = Models.Thermal(k, c, thickness=2.0)
+    simu = Simulations.Thermal(mesh, model)
+    simu.solver = "hip_fp64"
+    simu.tol_anorm = 1e-13
+    simu.tol_residual = 1e-15
+    simu.rho = rho
+    bt = ea.ElemType(BOUNDARY[mesh.dim])
+    faces = bt in mesh.dict_groupElem  # the tetrahedral meshes carry no face groups
+    ref = O.ThermalOracle(mesh.connect, mesh.coord, et, k, c, rho, 2.0,
+                          {bt.value: mesh.dict_groupElem[bt].connect} if faces else {})
+    x0 = mesh.Nodes_Conditions(lambda x, y, z: x == 0)
+    xL = mesh.Nodes_Conditions(lambda x, y, z: x == 2.0)
+    top = mesh.Nodes_Conditions(lambda x, y, z: y == 1.0)
+    for s in (simu, ref):
+        s.add_dirichlet(x0, [lambda x, y, z: 10 * y], ["t"], "thermal")
+        if faces:
+            s.add_surfLoad(top, [5.0], ["t"], "thermal")
+        else:
+            s.add_dirichlet(xL, [5.0], ["t"], "thermal")
+    return simu, ref
+
+
+@pytest.mark.parametrize("et", ["TRI3", "QUAD4", "HEXA8", "TETRA4", "TETRA10"])
+def test_thermal_steady_and_parabolic_match_the_oracle(et):
+    simu, ref = _thermal_pair(et)
+    t = simu.Solve()
+    assert _rel(t, ref.Step()) < 1e-9
+    K, C, M, _ = simu.Get_K_C_M_F()
+    Kr, Cr = ref.Get_K_C()
+    assert np.max(np.abs(K.data - Kr.data)) <= 1e-13 * np.max(np.abs(Kr.data))
+    assert np.max(np.abs(C.data - Cr.data)) <= 1e-13 * np.max(np.abs(Cr.data)) and M.nnz == 0
+    # the transient from a zero temperature (from the steady state it would stay there)
+    simu, ref = _thermal_pair(et, seed=1)
+    for alpha in (0.5, 1.0):
+        simu.Solver_Set_Parabolic_Algorithm(0.05, alpha)
+        for step in range(3):
+            simu.Solve()
+            ref.Step("parabolic", (0.05, alpha))
+            print(f"{et} alpha {alpha} step {step}: {simu.last_solve['thermal'].iterations} it", flush=True)
+            assert _rel(simu.thermal, ref.u) < 1e-9
+            assert _rel(simu.thermalDot, ref.speed) < 1e-8
+    simu.Save_Iter()
+    assert set(simu.results[-1]) == {"thermal", "thermalDot"}
+    assert np.array_equal(simu.Result("thermalDot"), simu.thermalDot)
+    assert simu.Result("displacement_matrix").shape == (simu.mesh.Nn, 3)
+
+
+def test_thermal_model_change_triggers_reassembly():
+    simu, _ = _thermal_pair("QUAD4")
+    simu.Get_K_C_M_F()
+    assert not simu.needUpdate
+    simu.thermalModel.k = 2.0
+    assert simu.needUpdate
+    simu.Need_Update(False)
+    simu.thermalModel.c = 0.2
+    assert simu.needUpdate
+
+
+def test_a_solve_started_at_its_solution_converges():
+    """A repeated solve starts from its own solution: its initial residual is
+    at round-off and the tight residual test cannot be met; the A-norm test
+    ends it and the solve reports success with the solution unchanged."""
+    simu, _ = _thermal_pair("QUAD4", seed=3)
+    simu.Solve()
+    t0 = simu.thermal
+    simu.Solve()
+    info = simu.last_solve["thermal"]
+    print(f"repeat: {info.iterations} it, rel residual {info.rel_residual:.2e}", flush=True)
+    assert info.status == 1
+    assert _rel(simu.thermal, t0) < 1e-12

@@ -6,6 +6,7 @@ import re
 
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 import easyfea_amd as ea
 from easyfea_amd import _lib
@@ -377,3 +378,68 @@ def test_aggregation_covers_nodes_and_splits_crack_lips():
     assert cp.pair_ptr[-1] == cm.node_cols.size
     keys = cp.blk_I.astype(np.int64) * ag.nagg + cp.blk_J
     assert np.all(np.diff(keys) > 0)
+
+
+# ---------------------------------------------------------------- time schemes
+@pytest.mark.parametrize("algo", ["parabolic", "newmark", "midpoint", "hht", "hht_newmark", "euler_implicit",
+                                  "euler_explicit"])
+def test_time_scheme_tables_reproduce_the_reference_rhs(algo):
+    """The host's expanded (matrix x history vector) form of A and b equals
+    the oracle's restatement of _Solver_Apply_Neumann, Rayleigh damping folded in."""
+    from easyfea_amd.Solvers import AlgoType
+    from easyfea_amd.time_schemes import expand, history_coefs, matrix_coefs
+
+    rng = np.random.default_rng(11)
+    n = 30
+    Kd = rng.standard_normal((n, n))
+    Md = rng.standard_normal((n, n))
+    K, M = sp.csr_matrix(Kd @ Kd.T), sp.csr_matrix(Md @ Md.T + n * np.eye(n))
+    cM, cK = 0.3, 0.02
+    C = (cK * K + cM * M).tocsr()
+    F, u, v, a = rng.standard_normal((4, n))
+    params = (0.04, 0.6) if algo == "parabolic" else O.hyperbolic_params(0.04, algo, 0.27, 0.55, 0.2)
+    ref_b = O.time_rhs(algo, params, K, C, M, F, u, v, a)
+    damping = {"K": cK, "M": cM}
+    bases = {"K": K, "M": M}
+    b = F.copy()
+    for name, (ku, kv, ka) in expand(history_coefs(AlgoType(algo), params), damping).items():
+        b += bases[name] @ (ku * u + kv * v + ka * a)
+    assert np.linalg.norm(b - ref_b) <= 1e-12 * np.linalg.norm(ref_b)
+    A = sum(c * bases[nm] for nm, c in expand(dict(zip("KCM", matrix_coefs(AlgoType(algo), params))),
+                                               damping).items())
+    cK_, cC_, cM_ = O.time_coefs(algo, params)
+    ref_A = (cK_ * K + cC_ * C + cM_ * M).toarray()
+    assert np.abs(A.toarray() - ref_A).max() <= 1e-12 * np.abs(ref_A).max()
+    # thermal form: C a base matrix of its own
+    tb = expand(history_coefs(AlgoType(algo), params), {"C": 1.0})
+    assert set(tb) <= {"K", "C", "M"}
+
+
+def test_time_abi_argument_errors_without_gpu():
+    lib = _lib.load()
+    assert lib.efea_time_update(9, 10, 0.1, 0.25, 0.5, 0.0, None, None, None, None, None, None, None, None) == -1
+    assert lib.efea_time_update(2, 10, 0.1, 0.25, 0.5, 0.0, None, None, None, None, None, None, None, None) == -1
+    assert "null pointer" in _lib.last_error()
+    one = (C.c_double * 1)()
+    p = C.cast(one, C.c_void_p)
+    assert lib.efea_time_update(2, 1, 0.0, 0.25, 0.5, 0.0, p, p, p, p, p, None, None, None) == -1
+    assert "dt" in _lib.last_error()
+    assert lib.efea_lincomb(-1, 1.0, None, 0.0, None, 0.0, None, 0.0, None, None, None) == -1
+    assert lib.efea_uv_ke(1, 64, -1, None, None, None, 4, None, 0, 1.0, 2, 1.0, None, None) == -1
+
+
+def test_thermal_model_parameters_and_observers():
+    m = ea.Models.Thermal(1.0, 2.0, thickness=0.5)
+    assert m.dim == 1 and m.k == 1.0 and m.c == 2.0 and not m.isHeterogeneous
+    flags = []
+
+    class Obs:
+        def Need_Update(self, value=True):
+            flags.append(value)
+
+    m._Add_observer(Obs())
+    m.k = np.array([1.0, 2.0])
+    m.c = 0.0
+    assert flags == [True, True] and m.isHeterogeneous
+    with pytest.raises(AssertionError):
+        m.k = -1.0
