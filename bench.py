@@ -253,6 +253,80 @@ def bench_single(args) -> dict:
     }
 
 
+def _cpu_dynamics(workload, cells, steps):
+    """The oracle's time steps on the host (matrices assembled once, then per
+    step the reference's right-hand side and a SuperLU solve of the
+    Dirichlet-reduced system): (DOF, seconds per step)."""
+    from easyfea_amd import mesh as MS
+    from easyfea_amd import workloads
+    from easyfea_amd._utils import ElemType
+    from oracle import efea_oracle as O
+
+    if workload == "newmark-cube":
+        m = MS.hexa8_cube(cells)
+        E = workloads.cube_E(m.Ne, 0)
+        algo, params = "newmark", O.hyperbolic_params(5e-8)
+        ref = O.DynamicElasticOracle(m.connect, m.coord, "HEXA8", E, 0.3, False, 1.0,
+                                     {"QUAD4": m.dict_groupElem[ElemType.QUAD4].connect}, rho=workloads.RHO_STEEL)
+        ref.add_dirichlet(m.Nodes_Conditions(lambda x, y, z: z == 0), [0, 0, 0], ["x", "y", "z"])
+        ref.add_surfLoad(m.Nodes_Conditions(lambda x, y, z: z == 1.0), [-10.0], ["z"])
+        K, C, M = ref.Get_K_C_M()
+        u = np.zeros(ref.Ndof)
+        pt, unk = "elastic", ["x", "y", "z"]
+    else:
+        m = MS.quad4_rectangle(1.0, 1.0, cells, cells)
+        algo, params = "parabolic", (1e-4, 0.5)
+        ref = O.ThermalOracle(m.connect, m.coord, "QUAD4", workloads.thermal_k(cells, 0), 1.0, 1.0, 1.0,
+                              {"SEG2": m.dict_groupElem[ElemType.SEG2].connect})
+        ref.add_dirichlet(m.Nodes_Conditions(lambda x, y, z: x == 0), [0.0], ["t"], "thermal")
+        ref.add_surfLoad(m.Nodes_Conditions(lambda x, y, z: y == 1.0), [5.0], ["t"], "thermal")
+        K, C = ref.Get_K_C()
+        M = 0 * C
+        u = np.zeros(m.Nn)
+        pt, unk = "thermal", ["t"]
+    F = ref.neumann_vector(pt, u.size)
+    dofs, vals = ref.dir[pt]
+    orphan = O.dofs_nodes(ref.orphan, unk, unk)
+    v, a = np.zeros_like(u), np.zeros_like(u)
+    t = time.perf_counter()
+    for _ in range(steps):
+        u, v, a = O.time_step(algo, params, K, C, M, F, u, v, a, dofs, vals, orphan)
+    return u.size, (time.perf_counter() - t) / steps
+
+
+def bench_dynamics(args) -> dict:
+    """Time stepping (easyfea_amd.dynamics_bench): Newmark on the 64^3 cube or
+    Crank-Nicolson on the 1024^2 square, with the oracle's host steps of a
+    smaller mesh of the same problem as the CPU baseline."""
+    from easyfea_amd import dynamics_bench
+
+    cube = args.workload == "newmark-cube"
+    cells = 64 if cube else 1024
+    res = dynamics_bench.run(args.workload, cells, args.steps, args.warmup)
+    cpu = None
+    if not args.no_cpu:
+        cpu_cells = 16 if cube else 256
+        ndof, s_step = _cpu_dynamics(args.workload, cpu_cells, max(1, args.cpu_steps))
+        cpu = {"value": ndof / s_step, "unit": "DOF*steps/s", "cores": len(os.sched_getaffinity(0)),
+               "kind": "port",
+               "sample": f"{max(1, args.cpu_steps)} oracle time steps of the {cpu_cells}"
+                         f"{'^3 cube' if cube else '^2 square'} ({ndof} DOF; matrices assembled once, "
+                         f"SuperLU per step) on the host, {s_step:.2f} s per step"}
+    cfg = ("HEXA8 64^3 unit cube, lognormal E (CoV 0.2, seed 0), steel rho, clamped z=0, -10 MPa on z=1 "
+           "applied at t=0; Newmark beta 1/4 gamma 1/2, dt 5e-8 s" if cube else
+           "QUAD4 1024^2 unit square, lognormal k (mean 1, CoV 0.3), c = rho = 1, t=0 at x=0, flux 5 on y=1; "
+           "Crank-Nicolson, dt 1e-4")
+    return {
+        "metric": "time-step throughput", "value": res["value"], "unit": "DOF*steps/s", "n_gpus": 1,
+        "steps": res["steps"], "warmup": res["warmup"], "ms_per_step": res["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 vectors (f32 matrix copy when mixed)", "data": "synthetic mesh generated in-process",
+        "config": {"workload": cfg, "ndof": res["ndof"], "solver": f"{res['precond']} PCG, {res['mode']}"},
+        "p50_step_ms": res["p50_ms"], "p99_step_ms": res["p99_ms"], "iterations_mean": res["iterations_mean"],
+        "iterations_max": res["iterations_max"], "cpu_baseline": cpu,
+    }
+
+
 def bench_serving(args) -> dict:
     """The serving streams (easyfea_amd.serving_bench) with the oracle's
     host solves of a bounded sample as the CPU baseline."""
@@ -333,10 +407,12 @@ def main():
     ap.add_argument("--no-coarse-reuse", action="store_true",
                     help="rebuild the coarse inverse at every displacement solve")
     ap.add_argument("--workload", default="sent",
-                    choices=["sent", "cantilever", "cube", "serve-elastic", "serve-phasefield"],
+                    choices=["sent", "cantilever", "cube", "serve-elastic", "serve-phasefield", "newmark-cube",
+                             "thermal-square"],
                     help="sent: the phase-field history above (the headline); the others print the "
-                         "measurement line of the cantilever / 64^3 cube single solves or of the serving "
-                         "streams (easyfea_amd.solve_bench / easyfea_amd.serving_bench)")
+                         "measurement line of the cantilever / 64^3 cube single solves, of the serving "
+                         "streams or of the time-stepping workloads (easyfea_amd.solve_bench / "
+                         "easyfea_amd.serving_bench / easyfea_amd.dynamics_bench)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the phase-field workload runs on a single MI355X "
@@ -354,6 +430,9 @@ def main():
         return
     if args.workload.startswith("serve-"):
         print(json.dumps(bench_serving(args)))
+        return
+    if args.workload in ("newmark-cube", "thermal-square"):
+        print(json.dumps(bench_dynamics(args)))
         return
     mesh = ea.tri3_notched_square(args.cells)
     ndof_u, ndof_d = mesh.Nn * 2, mesh.Nn

@@ -98,6 +98,8 @@ class _Simu:
         self.predict_polls = True
         # largest aggregate of the two-level coarse space (None: 256 nodes in 2D, 1024 in 3D)
         self.coarse_target = None
+        # time steps start the solve from the state extrapolated over the step
+        self.extrapolate_guess = True
         self.last_solve = {}
         self._algo = AlgoType.elliptic
         self._parabolicParams = None
@@ -217,7 +219,15 @@ class _Simu:
             lincomb(w, [(c, s) for c, s in ((cu, u), (cv, v), (ca, a)) if c != 0])
             lincomb(b, [(1.0, b), (1.0, bases[name].spmv(w))])
         explicit = algo == AlgoType.euler_explicit
-        x = (a if explicit else u).clone()
+        if explicit or not self.extrapolate_guess:
+            x = (a if explicit else u).clone()
+        else:
+            # initial guess u + dt v + dt^2/2 a (u + dt v for the parabolic scheme):
+            # the solution within the solver tolerance does not depend on it, the
+            # iteration count does; the Dirichlet values replace it at the known DOFs
+            dt = params[0]
+            terms = [(1.0, u), (dt, v)] + ([] if algo == AlgoType.parabolic else [(dt * dt / 2, a)])
+            x = lincomb(torch.empty_like(u), terms)
         # the explicit scheme's unknown is a^n: constrained DOFs have zero acceleration (_simu.py:1905-1907)
         yield from self._solve_system_gen(A, b, x, problemType, zero_dirichlet=explicit)
         u1, v1, a1 = torch.empty_like(u), torch.empty_like(v), torch.empty_like(a)

@@ -35,7 +35,7 @@ SIGNATURES = {
     "efea_step1_bench": (i32, [i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, i32, vp]),
     "efea_bj_setup": (i32, [i32, i32, i32, vp, vp, vp, vp, vp, vp]),
     "efea_pcg_workspace_bytes": (sz, [i32, i32, i32]),
-    "efea_default_group": (i32, [i32, i32]),
+    "efea_default_group": (i32, [i32, i32, i32]),
     "efea_solve_pcg": (
         i32,
         [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, f64, f64, f64, i32, i32, i32, i32, vp, vp],

@@ -3,7 +3,7 @@
 ``python -m easyfea_amd.autotune [--reps 200] [--quick]``
 
 Every Krylov kernel that multiplies by a global matrix gives each node
-``G`` lanes (a power of two, 4..32) that stride over the node's neighbour
+``G`` lanes (a power of two, 2..32) that stride over the node's neighbour
 blocks and combine their partial rows with cross-lane shuffles.  Small
 ``G`` leaves lanes idle on nodes with few neighbours; large ``G`` costs
 more shuffle rounds and fewer nodes per wave.  This tuner times the PCG's
@@ -21,7 +21,7 @@ import json
 
 import numpy as np
 
-CANDIDATES = {1: (4, 8, 16), 2: (4, 8, 16), 3: (8, 16, 32)}
+CANDIDATES = {1: (2, 4, 8, 16), 2: (4, 8, 16), 3: (8, 16, 32)}
 
 
 def _cases(quick: bool):
@@ -58,7 +58,7 @@ def tune_pattern(pattern, reps: int) -> dict:
     cnt = np.diff(pattern.host.node_ptr)
     row = {"Nn": int(A.Nn), "dof_n": int(A.dof_n), "nnz": int(A.nnz), "max_cnt": int(cnt.max()),
            "mean_cnt": float(cnt.mean()),
-           "default": int(_lib.load().efea_default_group(A.dof_n, int(cnt.max())))}
+           "default": int(_lib.load().efea_default_group(A.dof_n, int(cnt.max()), A.Nn))}
     for prec in (64, 32):
         t = {G: time_step1(A, prec, reps, G) * 1e6 for G in CANDIDATES[A.dof_n]}
         row[f"fp{prec}_us"] = {str(k): round(v, 3) for k, v in t.items()}

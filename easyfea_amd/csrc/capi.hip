@@ -55,7 +55,7 @@ int pcg2_step1_bench(int, int, int, int, const int*, const int*, const void*, co
                      const double*, double*, void*, int, hipStream_t);
 int dot(int64_t, const double*, const double*, double*, double*, hipStream_t);
 int gather_sum(int64_t, const int*, const double*, double*, hipStream_t);
-int default_group(int, int);
+int default_group(int, int, int);
 int pf_damage(int, int64_t, const int*, const double*, const double*, int, const double*, int, const double*,
               const double*, const double*, int, double, int, const double*, int, double, int, int, double,
               double*, double*, double*, hipStream_t);
@@ -236,7 +236,9 @@ size_t efea_pcg_workspace_bytes(int Nnode, int dof_n, int group) {
   return pcg_workspace_bytes(Nnode, dof_n, group);
 }
 
-int efea_default_group(int dof_n, int max_neighbours) { return default_group(dof_n, max_neighbours); }
+int efea_default_group(int dof_n, int max_neighbours, int Nnode) {
+  return default_group(dof_n, max_neighbours, Nnode);
+}
 
 int efea_solve_pcg(int mode, int Nnode, int dof_n, int group, const int32_t* node_ptr, const int32_t* node_cols,
                    const double* values64, const float* values32, const float* corr32, const uint8_t* known,

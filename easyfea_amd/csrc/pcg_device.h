@@ -45,7 +45,7 @@ struct SpmvArgs {
 // chain instead of one per position.  load(c, xv) fills xv[k][b] = X_k(c, b).
 template <int G>
 struct GatherUnroll {
-  static constexpr int U = G <= 4 ? 3 : 2;
+  static constexpr int U = G <= 2 ? 5 : (G <= 4 ? 3 : 2);
 };
 
 template <typename TM, int DN, int G, int NV, typename Load>
@@ -135,7 +135,7 @@ constexpr int MAXB = 1024;
 
 // Supported (dof_n, lanes per node) pairs of the node-group kernels; every
 // dispatch site expands this list.
-#define EFEA_GROUPS(X) X(1, 4) X(1, 8) X(1, 16) X(2, 4) X(2, 8) X(2, 16) X(3, 8) X(3, 16) X(3, 32)
+#define EFEA_GROUPS(X) X(1, 2) X(1, 4) X(1, 8) X(1, 16) X(2, 4) X(2, 8) X(2, 16) X(3, 8) X(3, 16) X(3, 32)
 
 // Logical block of hardware block b in a grid of nb: the dispatcher deals
 // blocks round-robin to the 8 XCDs (b, b+8, b+16, ... on XCD b % 8); this

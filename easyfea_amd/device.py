@@ -192,7 +192,7 @@ class CsrPattern:
         self.node_cols = to_dev(cmap.node_cols)
         self.seg = to_dev(cmap.seg_ptr)
         self.perm = to_dev(cmap.perm)
-        self.group = _lib.load().efea_default_group(cmap.dof_n, cmap.max_neighbours)
+        self.group = _lib.load().efea_default_group(cmap.dof_n, cmap.max_neighbours, cmap.Nn)
         nd = cmap.nPe * cmap.dof_n
         self.n_entries = Ne * nd * nd
 

@@ -6,6 +6,12 @@
 * ``sent``: the single-edge-notched square in tension (AT2 phase field,
   ``examples/PhaseField/Tension.py`` boundary conditions).
 * ``cube``: HEXA8 unit cube, lognormal i.i.d. Young's modulus per element.
+* ``newmark_cube``: the cube in structural dynamics (steel density in
+  tonne/mm^3, the load applied suddenly at t = 0), Newmark average
+  acceleration with about 25 steps per period of the first shear mode.
+* ``thermal_square``: QUAD4 unit square, lognormal conductivity field,
+  t = 0 at x = 0, a heat flux of 5 through y = 1, Crank-Nicolson steps
+  (``examples/Thermal/Thermal3.py``-like conditions).
 * ``serving_elastic``: a stream of QUAD4 plane-stress squares of random side
   with a correlated lognormal Young's modulus field, Poisson arrivals.
 * ``serving_phasefield``: phase-field realisations of the notched square with
@@ -96,16 +102,55 @@ def sent_top_y_dofs(simu: Simulations.PhaseField) -> np.ndarray:
     return simu.Bc_dofs_nodes(top, ["y"])
 
 
-def cube(n: int, seed: int = 0, solver=None) -> Simulations.Elastic:
-    mesh = hexa8_cube(n)
+def cube_E(Ne: int, seed: int = 0) -> np.ndarray:
+    """I.i.d. lognormal Young's modulus per element (mean 210000, CoV 0.2)."""
     rng = np.random.default_rng(seed)
     s = math.sqrt(math.log(1.04))
-    E = np.exp(rng.normal(math.log(210000.0) - 0.5 * s * s, s, mesh.Ne))
+    return np.exp(rng.normal(math.log(210000.0) - 0.5 * s * s, s, Ne))
+
+
+def cube(n: int, seed: int = 0, solver=None) -> Simulations.Elastic:
+    mesh = hexa8_cube(n)
+    E = cube_E(mesh.Ne, seed)
     simu = Simulations.Elastic(mesh, Models.Elastic.Isotropic(3, E=E, v=0.3))
     if solver:
         simu.solver = solver
     simu.add_dirichlet(mesh.Nodes_Conditions(lambda x, y, z: z == 0), [0, 0, 0], ["x", "y", "z"])
     simu.add_surfLoad(mesh.Nodes_Conditions(lambda x, y, z: z == 1.0), [-10.0], ["z"])
+    return simu
+
+
+RHO_STEEL = 7.85e-9  # tonne / mm^3 (with MPa and mm)
+
+
+def newmark_cube(n: int, seed: int = 0, solver=None, dt: float = 5e-8) -> Simulations.Elastic:
+    """The cube of ``cube`` with steel density and the Newmark scheme
+    (beta 1/4, gamma 1/2).  The first shear period of the clamped unit cube
+    is about 4 L / sqrt(G / rho) = 1.2e-6 s, so dt = 5e-8 s takes ~25 steps
+    per period."""
+    simu = cube(n, seed, solver)
+    simu.rho = RHO_STEEL
+    simu.Solver_Set_Hyperbolic_Algorithm(dt)
+    return simu
+
+
+def thermal_k(n: int, seed: int = 0) -> np.ndarray:
+    """Conductivity per element of the n x n square: a correlated lognormal
+    field (mean 1, CoV 0.3, correlation 0.05 of the side)."""
+    return lognormal(1.0, 0.3, gaussian_field((n, n), 0.05 * n, np.random.default_rng(seed)).T.ravel())
+
+
+def thermal_square(n: int, seed: int = 0, solver=None, dt: float = 1e-4, alpha: float = 0.5) -> Simulations.Thermal:
+    """QUAD4 n x n unit square, k a correlated lognormal field (mean 1, CoV
+    0.3, correlation 0.05 of the side), c = rho = 1; t = 0 at x = 0, heat flux
+    5 in through y = 1; the parabolic scheme with ``dt`` and ``alpha``."""
+    mesh = quad4_rectangle(1.0, 1.0, n, n)
+    simu = Simulations.Thermal(mesh, Models.Thermal(thermal_k(n, seed), 1.0))
+    if solver:
+        simu.solver = solver
+    simu.add_dirichlet(mesh.Nodes_Conditions(lambda x, y, z: x == 0), [0.0], ["t"])
+    simu.add_surfLoad(mesh.Nodes_Conditions(lambda x, y, z: y == 1.0), [5.0], ["t"])
+    simu.Solver_Set_Parabolic_Algorithm(dt, alpha)
     return simu
 
 

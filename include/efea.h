@@ -136,11 +136,11 @@ int efea_bj_setup(int precision, int Nnode, int dof_n, const int32_t* node_ptr, 
                   const double* values, const uint8_t* known, void* Minv, void* stream);
 
 size_t efea_pcg_workspace_bytes(int Nnode, int dof_n, int group);
-/* Lanes per node (`group`) of the solver kernels for a pattern whose nodes have
- * at most `max_neighbours` neighbours (self included): 4, 8 or 16, from timings
- * of the PCG step-1 kernel on MI355X (`python -m easyfea_amd.autotune`).
- * Supported (dof_n, group) pairs: (1|2, 4|8|16) and (3, 8|16|32). */
-int efea_default_group(int dof_n, int max_neighbours);
+/* Lanes per node (`group`) of the solver kernels for a pattern of Nnode nodes
+ * with at most `max_neighbours` neighbours (self included): 2, 4, 8 or 16, from
+ * timings of the PCG step-1 kernel on MI355X (`python -m easyfea_amd.autotune`).
+ * Supported (dof_n, group) pairs: (1, 2|4|8|16), (2, 4|8|16) and (3, 8|16|32). */
+int efea_default_group(int dof_n, int max_neighbours, int Nnode);
 
 /* Dirichlet-reduced solve A_ii x_i = b_i - A_ic x_c by block-Jacobi PCG on the
  * full-length vectors (known DOFs masked, x holds the Dirichlet values there and

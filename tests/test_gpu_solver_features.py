@@ -141,3 +141,48 @@ def test_batch_launch_and_wait_without_systems():
         assert _lib.load().efea_batch_wait(h, None) == 0
     finally:
         _lib.load().efea_batch_destroy(h)
+
+
+GROUPS = [(1, 2), (1, 4), (1, 8), (1, 16), (2, 4), (2, 8), (2, 16), (3, 8), (3, 16), (3, 32)]
+
+
+def _simu_for(dof_n):
+    import easyfea_amd as ea
+    from easyfea_amd import Models, Simulations
+
+    if dof_n == 1:
+        mesh = ea.quad4_rectangle(1.0, 1.0, 30, 20)
+        simu = Simulations.Thermal(mesh, Models.Thermal(1.3, 1.0))
+        simu.add_dirichlet(mesh.Nodes_Conditions(lambda x, y, z: x == 0), [0.0], ["t"])
+        simu.add_dirichlet(mesh.Nodes_Conditions(lambda x, y, z: x == 1.0), [2.0], ["t"])
+        return simu, lambda: simu._device_KC()[0]
+    if dof_n == 2:
+        mesh = ea.tri3_rectangle(2.0, 1.0, 30, 15)
+        simu = Simulations.Elastic(mesh, Models.Elastic.Isotropic(2, 210000.0, 0.3))
+        simu.add_dirichlet(mesh.Nodes_Conditions(lambda x, y, z: x == 0), [0, 0], ["x", "y"])
+        simu.add_dirichlet(mesh.Nodes_Conditions(lambda x, y, z: x == 2.0), [1e-3, 0], ["x", "y"])
+        return simu, simu._device_K
+    mesh = ea.hexa8_box(2.0, 1.0, 1.0, 10, 5, 5)
+    simu = Simulations.Elastic(mesh, Models.Elastic.Isotropic(3, 210000.0, 0.3))
+    simu.add_dirichlet(mesh.Nodes_Conditions(lambda x, y, z: x == 0), [0, 0, 0], ["x", "y", "z"])
+    simu.add_dirichlet(mesh.Nodes_Conditions(lambda x, y, z: x == 2.0), [1e-3, 0, 0], ["x", "y", "z"])
+    return simu, simu._device_K
+
+
+@pytest.mark.parametrize("dof_n,group", GROUPS)
+def test_every_lanes_per_node_group_multiplies_and_solves(dof_n, group):
+    simu, getK = _simu_for(dof_n)
+    simu.solver = "hip_fp64"
+    simu.preconditioner = "bj"
+    u_default = simu.Solve().copy()
+    K = getK()
+    K.group = group
+    print(f"dof_n {dof_n}, group {group}: Nn {K.Nn}", flush=True)
+    x = torch.rand(K.Ndof, dtype=torch.float64, device="cuda")
+    y = K.spmv(x).cpu().numpy()
+    ref = K.to_scipy() @ x.cpu().numpy()
+    assert np.max(np.abs(y - ref)) <= 1e-13 * np.max(np.abs(ref))
+    simu._u.zero_()
+    u = simu.Solve()
+    assert simu.last_solve[simu.problemType].status == 1
+    assert np.linalg.norm(u - u_default) <= 1e-7 * np.linalg.norm(u_default)

@@ -67,12 +67,13 @@ def test_abi_argument_errors_are_reported_without_gpu():
 
 def test_workspace_sizes_and_groups():
     lib = _lib.load()
-    # TRI3 / QUAD4 (7-9 neighbours): 4 lanes; quadratic 2D (up to 25): 16
-    assert lib.efea_default_group(2, 9) == 4 and lib.efea_default_group(1, 7) == 4
-    assert lib.efea_default_group(2, 21) == 8 and lib.efea_default_group(1, 25) == 16
+    # TRI3 / QUAD4 (7-9 neighbours): 4 lanes, 2 for scalar patterns from 5e4 nodes; quadratic 2D (up to 25): 16
+    assert lib.efea_default_group(2, 9, 10**6) == 4 and lib.efea_default_group(1, 7, 40501) == 4
+    assert lib.efea_default_group(1, 9, 10**6) == 2 and lib.efea_default_group(1, 13, 10**6) == 8
+    assert lib.efea_default_group(2, 21, 100) == 8 and lib.efea_default_group(1, 25, 100) == 16
     # TETRA4 (15): 8 lanes; HEXA8 (27) and TETRA10 (65): 16
-    assert lib.efea_default_group(3, 15) == 8
-    assert lib.efea_default_group(3, 27) == 16 and lib.efea_default_group(3, 65) == 16
+    assert lib.efea_default_group(3, 15, 1000) == 8
+    assert lib.efea_default_group(3, 27, 1000) == 16 and lib.efea_default_group(3, 65, 1000) == 16
     small = lib.efea_pcg_workspace_bytes(100, 2, 16)
     big = lib.efea_pcg_workspace_bytes(10000, 2, 16)
     assert big > small >= 4 * 100 * 2 * 8
