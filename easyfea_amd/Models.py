@@ -116,14 +116,41 @@ class PhaseField(_Observable):
             return self.name
 
     class SplitType(str, Enum):
+        """``PhaseField.SplitType`` (``Models/_phasefield.py:55-78``)."""
+
+        Bourdin = "Bourdin"
         Amor = "Amor"
         Miehe = "Miehe"
+        He = "He"
+        Stress = "Stress"
+        Zhang = "Zhang"
+        AnisotStrain = "AnisotStrain"
+        AnisotStrain_PM = "AnisotStrain_PM"
+        AnisotStrain_MP = "AnisotStrain_MP"
+        AnisotStrain_NoCross = "AnisotStrain_NoCross"
+        AnisotStress = "AnisotStress"
+        AnisotStress_PM = "AnisotStress_PM"
+        AnisotStress_MP = "AnisotStress_MP"
+        AnisotStress_NoCross = "AnisotStress_NoCross"
 
         def __str__(self):
             return self.name
 
+    # the splits of the device kernels (EFEA_SPLIT_* of include/efea.h); the
+    # others make g c+ + c- non-symmetric, which the PCG cannot solve
+    SPLIT_CODE = {"Amor": 0, "Miehe": 1, "Bourdin": 2, "Stress": 3, "He": 4, "AnisotStrain": 5,
+                  "AnisotStrain_NoCross": 6, "AnisotStress": 7, "AnisotStress_NoCross": 8}
+
+    @staticmethod
+    def Get_splits() -> list:
+        return list(PhaseField.SplitType)
+
     class SolverType(str, Enum):
+        """Irreversibility (``Models/_phasefield.py:103-112``)."""
+
         History = "History"
+        HistoryDamage = "HistoryDamage"
+        BoundConstrain = "BoundConstrain"
 
         def __str__(self):
             return self.name

@@ -985,6 +985,8 @@ class PhaseField(_Simu):
         assert isinstance(model, PhaseFieldModel), "model must be a phase field model"
         assert mesh.dim == 2 and mesh.elemType in (ElemType.TRI3, ElemType.QUAD4), \
             "the device phase-field path supports 2D TRI3 / QUAD4 meshes"
+        assert model.solver != PhaseFieldModel.SolverType.BoundConstrain, \
+            "BoundConstrain (scipy lsq_linear on the host) has no device solver; use History or HistoryDamage"
         self._updU = False
         self._updD = False
         super().__init__(mesh, model, folder, verbosity)
@@ -996,6 +998,7 @@ class PhaseField(_Simu):
         nPgm = gauss_points(mesh.elemType, MatrixType.mass)[1].size
         self._psiP = torch.zeros(Ne * nPgm, dtype=torch.float64, device="cuda")
         self._H = None  # history (old psi+), None until the first Save_Iter
+        self._H0 = None  # zeros: the history before the first Save_Iter
         self._Ku = self._Kd = None
         self._Fd = torch.zeros(Nn, dtype=torch.float64, device="cuda")
         self.Niter_staggered = 0
@@ -1045,6 +1048,32 @@ class PhaseField(_Simu):
             c["E"] = to_dev(mat._E_array())
         return c["E"], (1 if mat.isHeterogeneous else 0), mat.v, int(mat.planeStress)
 
+    def _history_arg(self) -> Optional[torch.Tensor]:
+        """The old psi+ the damage kernel clamps against: with the History
+        solver the saved field, zeros before the first Save_Iter (the
+        reference starts from zeros, ``_phasefield.py:512-529``; a split whose
+        psi+ can be negative, e.g. AnisotStrain, sees the difference); no
+        clamp (None) for the other solvers."""
+        if self.model.solver != PhaseFieldModel.SolverType.History:
+            return None
+        if self._H is not None:
+            return self._H
+        if self._H0 is None or self._H0.numel() != self._psiP.numel():
+            self._H0 = torch.zeros_like(self._psiP)
+        return self._H0
+
+    def _split_code(self) -> int:
+        """``EFEA_SPLIT_*`` of the model's split; Zhang and the ``_PM`` /
+        ``_MP`` variants make ``g c+ + c-`` non-symmetric, which the device's
+        conjugate-gradient solve cannot take."""
+        split = str(self.model.split)
+        code = PhaseFieldModel.SPLIT_CODE.get(split)
+        if code is None:
+            raise NotImplementedError(
+                f"split {split}: its displacement stiffness g c+ + c- is not symmetric; the device solver "
+                f"(PCG) needs a symmetric matrix. Device splits: {list(PhaseFieldModel.SPLIT_CODE)}")
+        return code
+
     def _Gc_dev(self):
         c = self.__dict__.setdefault("_dev_params", {})
         if "Gc" not in c:
@@ -1062,8 +1091,8 @@ class PhaseField(_Simu):
         E, Es, nu, ps = self._material_args()
         Gc = self._Gc_dev()
         _lib.call("efea_pf_damage_ke", dev.code, dev.Ne, ptr(dev.connect), ptr(dev.coord), ptr(tm), nm, ptr(tr), nr,
-                  ptr(self._u), ptr(self._H), ptr(E), Es, nu, ps, ptr(Gc), 1 if pfm.isHeterogeneous else 0,
-                  pfm.l0, 0 if pfm.split == "Amor" else 1, 0 if pfm.regularization == "AT1" else 1,
+                  ptr(self._u), ptr(self._history_arg()), ptr(E), Es, nu, ps, ptr(Gc), 1 if pfm.isHeterogeneous else 0,
+                  pfm.l0, self._split_code(), 0 if pfm.regularization == "AT1" else 1,
                   pfm.thickness, ptr(self._psiP), ptr(Ke), ptr(Fe), stream_ptr())
         Kd.scatter(Ke)
         dev.vec_map(1).assemble(Fe, self._Fd)
@@ -1077,7 +1106,7 @@ class PhaseField(_Simu):
         tr, nr = dev.tables(MatrixType.rigi)
         E, Es, nu, ps = self._material_args()
         _lib.call("efea_pf_elastic_ke", dev.code, dev.Ne, ptr(dev.connect), ptr(dev.coord), ptr(tr), nr,
-                  ptr(self._u), ptr(self._d), ptr(E), Es, nu, ps, 0 if pfm.split == "Amor" else 1, pfm.thickness,
+                  ptr(self._u), ptr(self._d), ptr(E), Es, nu, ps, self._split_code(), pfm.thickness,
                   ptr(Ke), stream_ptr())
         Ku.scatter(Ke)
         self._Ku = Ku
@@ -1212,9 +1241,17 @@ class PhaseField(_Simu):
         self._convIter = value
 
     def Solve(self, tolConv=1.0, maxIter=500, convOption=2):
-        """Staggered scheme (``_phasefield.py:303-432``): returns ``(u, d, converged)``."""
+        """Staggered scheme (``_phasefield.py:303-432``): returns ``(u, d, converged)``.
+        With the ``HistoryDamage`` solver the returned ``d`` is the larger of
+        the damage before the call and after it, node by node; the
+        simulation's own damage field is the solved one (``:338, 412-416``)."""
+        history_damage = self.model.solver == PhaseFieldModel.SolverType.HistoryDamage
+        d_old = self.damage if history_damage else None
         converged = drive(self._staggered(tolConv, maxIter, convOption))
-        return self.displacement, self.damage, converged
+        d = self.damage
+        if history_damage:
+            d = np.maximum(d_old, d)
+        return self.displacement, d, converged
 
     def Save_Iter(self, iter: Optional[dict] = None) -> None:
         """Advances the history field (``_phasefield.py:613-630``)."""
@@ -1267,7 +1304,7 @@ class PhaseField(_Simu):
         tr, nr = dev.tables(MatrixType.rigi)
         E, Es, nu, ps = self._material_args()
         _lib.call("efea_pf_strain_stress", dev.code, dev.Ne, ptr(dev.connect), ptr(dev.coord), ptr(tr), nr,
-                  ptr(self._u), ptr(self._d), ptr(E), Es, nu, ps, 0 if pfm.split == "Amor" else 1, pfm.thickness,
+                  ptr(self._u), ptr(self._d), ptr(E), Es, nu, ps, self._split_code(), pfm.thickness,
                   ptr(out), stream_ptr())
         return out
 
@@ -1282,8 +1319,8 @@ class PhaseField(_Simu):
         Ke = torch.empty(dev.Ne * dev.nPe * dev.nPe, dtype=torch.float64, device="cuda")
         Fe = torch.empty(dev.Ne * dev.nPe, dtype=torch.float64, device="cuda")
         _lib.call("efea_pf_damage_ke", dev.code, dev.Ne, ptr(dev.connect), ptr(dev.coord), ptr(tm), nm, ptr(tr), nr,
-                  ptr(self._u), ptr(self._H), ptr(E), Es, nu, ps, ptr(self._Gc_dev()),
-                  1 if pfm.isHeterogeneous else 0, pfm.l0, 0 if pfm.split == "Amor" else 1,
+                  ptr(self._u), ptr(self._history_arg()), ptr(E), Es, nu, ps, ptr(self._Gc_dev()),
+                  1 if pfm.isHeterogeneous else 0, pfm.l0, self._split_code(),
                   0 if pfm.regularization == "AT1" else 1, pfm.thickness, ptr(psiP), ptr(Ke), ptr(Fe), stream_ptr())
         return psiP.reshape(dev.Ne, nm)
 

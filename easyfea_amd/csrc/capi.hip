@@ -413,7 +413,7 @@ int efea_pf_damage_ke(int elem_type, int64_t Ne, const int32_t* connect, const d
     set_error("efea_pf_damage_ke: null pointer or negative size");
     return EFEA_ERR_ARG;
   }
-  if (split < 0 || split > 1 || regu < 0 || regu > 1) {
+  if (split < EFEA_SPLIT_AMOR || split > EFEA_SPLIT_ANISOT_STRESS_NOCROSS || regu < 0 || regu > 1) {
     set_error("efea_pf_damage_ke: split %d / regularisation %d not supported", split, regu);
     return EFEA_ERR_ARG;
   }
@@ -430,7 +430,7 @@ int efea_pf_strain_stress(int elem_type, int64_t Ne, const int32_t* connect, con
     set_error("efea_pf_strain_stress: null pointer or negative size");
     return EFEA_ERR_ARG;
   }
-  if (split < 0 || split > 1) {
+  if (split < EFEA_SPLIT_AMOR || split > EFEA_SPLIT_ANISOT_STRESS_NOCROSS) {
     set_error("efea_pf_strain_stress: split %d not supported", split);
     return EFEA_ERR_ARG;
   }
@@ -446,7 +446,7 @@ int efea_pf_elastic_ke(int elem_type, int64_t Ne, const int32_t* connect, const 
     set_error("efea_pf_elastic_ke: null pointer or negative size");
     return EFEA_ERR_ARG;
   }
-  if (split < 0 || split > 1) {
+  if (split < EFEA_SPLIT_AMOR || split > EFEA_SPLIT_ANISOT_STRESS_NOCROSS) {
     set_error("efea_pf_elastic_ke: split %d not supported", split);
     return EFEA_ERR_ARG;
   }
@@ -460,7 +460,7 @@ int efea_pf_split_points(int64_t n, int split, const double* eps, const double* 
     set_error("efea_pf_split_points: null pointer or negative size");
     return EFEA_ERR_ARG;
   }
-  if (split < 0 || split > 1) {
+  if (split < EFEA_SPLIT_AMOR || split > EFEA_SPLIT_ANISOT_STRESS_NOCROSS) {
     set_error("efea_pf_split_points: split %d not supported", split);
     return EFEA_ERR_ARG;
   }

@@ -16,7 +16,7 @@
 namespace efea {
 
 struct PfParams {
-  int split;  // 0 Amor, 1 Miehe
+  int split;  // EFEA_SPLIT_*
   int regu;   // 0 AT1, 1 AT2
   int planeStress;
   double nu;
@@ -34,27 +34,27 @@ __device__ inline void lame2d(double E, double nu, int planeStress, double& lam,
 __device__ inline double sgn(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : (x == 0 ? 0.0 : x)); }
 __device__ inline double heav(double x) { return x < 0 ? 0.0 : (x > 0 ? 1.0 : (x == 0 ? 0.5 : x)); }
 
-// Kelvin-Mandel split of the 2D strain e = [exx, eyy, sqrt(2) exy].
-__device__ inline void split2d(int split, const double e[3], double lam, double mu, double cP[3][3],
-                               double cM[3][3]) {
-  const double tr_e = e[0] + e[1];
-  const double Rp = (1 + sgn(tr_e)) / 2, Rm = (1 + sgn(-tr_e)) / 2;
-  const double IxI[3][3] = {{1, 1, 0}, {1, 1, 0}, {0, 0, 0}};
-  if (split == 0) {  // Amor
-    const double bulk = lam + 2 * mu / 2;
+// 3x3 helpers (Kelvin-Mandel 2D tensors of order 4).
+__device__ inline void mm3(const double A[3][3], const double B[3][3], double C[3][3]) {  // C = A B
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+  for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        const double Iij = (i == j) ? 1.0 : 0.0;
-        cP[i][j] = bulk * (Rp * IxI[i][j]) + 2 * mu * (Iij - 1.0 / 2 * IxI[i][j]);
-        cM[i][j] = bulk * (Rm * IxI[i][j]);
-      }
-    return;
-  }
-  // Miehe: eigenvalues / projectors of the strain tensor
+    for (int j = 0; j < 3; ++j) C[i][j] = A[i][0] * B[0][j] + A[i][1] * B[1][j] + A[i][2] * B[2][j];
+}
+__device__ inline void mtm3(const double A[3][3], const double B[3][3], double C[3][3]) {  // C = A^T B
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) C[i][j] = A[0][i] * B[0][j] + A[1][i] * B[1][j] + A[2][i] * B[2][j];
+}
+
+// Projector of the positive part of a 2D Kelvin-Mandel tensor t = [txx, tyy,
+// sqrt(2) txy] (__Spectral_Decomposition, Models/_phasefield.py:1043-1128,
+// with the 2D eigen decomposition :751-802): P = betaP I + g0 m1 m1^T + g1
+// m2 m2^T; the negative projector is I - P.
+__device__ inline void proj2d(const double t[3], double P[3][3]) {
   const double r2 = sqrt(2.0);
-  const double m00 = e[0], m11 = e[1], m01 = e[2] / r2;
+  const double m00 = t[0], m11 = t[1], m01 = t[2] / r2;
   const double det = m00 * m11 - m01 * m01;
   const double tr = m00 + m11;
   const double dl = tr * tr - 4 * det;
@@ -81,10 +81,167 @@ __device__ inline void split2d(int split, const double e[3], double lam, double 
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       const double Iij = (i == j) ? 1.0 : 0.0;
-      const double P = ((betaP * Iij) + (g0 * (m1[i] * m1[j]))) + (g1 * (m2[i] * m2[j]));
-      const double M = Iij - P;
-      cP[i][j] = lam * (Rp * IxI[i][j]) + 2 * mu * P;
-      cM[i][j] = lam * (Rm * IxI[i][j]) + 2 * mu * M;
+      P[i][j] = ((betaP * Iij) + (g0 * (m1[i] * m1[j]))) + (g1 * (m2[i] * m2[j]));
+    }
+}
+
+// cP, cM of the 2D split (PhaseField.Calc_C, Models/_phasefield.py:396-431)
+// for the strain e = [exx, eyy, sqrt(2) exy] of an isotropic material (Lame
+// constants lam, mu of the 2D model; E, nu for the stress split), codes
+// EFEA_SPLIT_*:
+//   Amor (:451-483), Miehe (:503-537), Bourdin (:433-449), Stress (:573-634),
+//   He (:680-749), AnisotStrain[_NoCross] (:539-566),
+//   AnisotStress[_NoCross] (:636-676).
+// The reference's evaluation order of each product and sum is kept.
+__device__ inline void split2d(int split, const double e[3], double lam, double mu, double E, double nu,
+                               int planeStress, double cP[3][3], double cM[3][3]) {
+  const double IxI[3][3] = {{1, 1, 0}, {1, 1, 0}, {0, 0, 0}};
+  // the isotropic Kelvin-Mandel stiffness lam IxI + 2 mu I
+  double C[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) C[i][j] = lam * IxI[i][j] + (i == j ? 2 * mu : 0.0);
+  if (split == EFEA_SPLIT_AMOR) {
+    const double tr_e = e[0] + e[1];
+    const double Rp = (1 + sgn(tr_e)) / 2, Rm = (1 + sgn(-tr_e)) / 2;
+    const double bulk = lam + 2 * mu / 2;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double Iij = (i == j) ? 1.0 : 0.0;
+        cP[i][j] = bulk * (Rp * IxI[i][j]) + 2 * mu * (Iij - 1.0 / 2 * IxI[i][j]);
+        cM[i][j] = bulk * (Rm * IxI[i][j]);
+      }
+    return;
+  }
+  if (split == EFEA_SPLIT_BOURDIN) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        cP[i][j] = C[i][j];
+        cM[i][j] = 0;
+      }
+    return;
+  }
+  double P[3][3], M[3][3];
+  if (split == EFEA_SPLIT_MIEHE || split == EFEA_SPLIT_ANISOT_STRAIN || split == EFEA_SPLIT_ANISOT_STRAIN_NOCROSS) {
+    proj2d(e, P);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) M[i][j] = (i == j ? 1.0 : 0.0) - P[i][j];
+    if (split == EFEA_SPLIT_MIEHE) {
+      const double tr_e = e[0] + e[1];
+      const double Rp = (1 + sgn(tr_e)) / 2, Rm = (1 + sgn(-tr_e)) / 2;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          cP[i][j] = lam * (Rp * IxI[i][j]) + 2 * mu * P[i][j];
+          cM[i][j] = lam * (Rm * IxI[i][j]) + 2 * mu * M[i][j];
+        }
+      return;
+    }
+    double PTC[3][3], MTC[3][3], Cpp[3][3], Cpm[3][3], Cmm[3][3], Cmp[3][3];
+    mtm3(P, C, PTC);
+    mtm3(M, C, MTC);
+    mm3(PTC, P, Cpp);
+    mm3(PTC, M, Cpm);
+    mm3(MTC, M, Cmm);
+    mm3(MTC, P, Cmp);
+    const bool cross = split == EFEA_SPLIT_ANISOT_STRAIN;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        cP[i][j] = cross ? (Cpp[i][j] + Cpm[i][j]) + Cmp[i][j] : Cpp[i][j];
+        cM[i][j] = cross ? Cmm[i][j] : (Cmm[i][j] + Cpm[i][j]) + Cmp[i][j];
+      }
+    return;
+  }
+  if (split == EFEA_SPLIT_HE) {
+    // C^(1/2) and C^(-1/2) of the isotropic C: eigenvalue 2 mu + 2 lam along
+    // IxI / 2, 2 mu on its complement
+    const double a = sqrt(2 * mu), b = sqrt(2 * mu + 2 * lam);
+    double sqC[3][3], isqC[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double Iij = (i == j) ? 1.0 : 0.0;
+        sqC[i][j] = a * Iij + (b - a) * (IxI[i][j] / 2);
+        isqC[i][j] = Iij / a + (1 / b - 1 / a) * (IxI[i][j] / 2);
+      }
+    double et[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) et[i] = sqC[i][0] * e[0] + sqC[i][1] * e[1] + sqC[i][2] * e[2];
+    double Pt[3][3], Mt[3][3], T[3][3];
+    proj2d(et, Pt);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) Mt[i][j] = (i == j ? 1.0 : 0.0) - Pt[i][j];
+    mm3(isqC, Pt, T);
+    mm3(T, sqC, P);
+    mm3(isqC, Mt, T);
+    mm3(T, sqC, M);
+    mm3(C, P, cP);
+    mm3(C, M, cM);
+    return;
+  }
+  // stress-based splits: the projectors of sigma = C e
+  double sg[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) sg[i] = C[i][0] * e[0] + C[i][1] * e[1] + C[i][2] * e[2];
+  proj2d(sg, P);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) M[i][j] = (i == j ? 1.0 : 0.0) - P[i][j];
+  if (split == EFEA_SPLIT_STRESS) {
+    const double tr_s = sg[0] + sg[1];
+    const double Rp = (1 + sgn(tr_s)) / 2, Rm = (1 + sgn(-tr_s)) / 2;
+    const double cv = planeStress ? nu / E : nu * (1 + nu) / E;
+    double sP[3][3], sM[3][3], T[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        sP[i][j] = ((1 + nu) / E * P[i][j]) - (cv * Rp * IxI[i][j]);
+        sM[i][j] = ((1 + nu) / E * M[i][j]) - (cv * Rm * IxI[i][j]);
+      }
+    mtm3(C, sP, T);
+    mm3(T, C, cP);
+    mtm3(C, sM, T);
+    mm3(T, C, cM);
+    return;
+  }
+  // AnisotStress[_NoCross]: Cp = P C, Cm = M C and the compliance S = C^-1
+  // = I / (2 mu) - lam / (2 mu (2 mu + 2 lam)) IxI
+  double S[3][3], Cp[3][3], Cm[3][3], PS[3][3], MS[3][3], Cpp[3][3], Cpm[3][3], Cmm[3][3], Cmp[3][3];
+  const double s2 = lam / (2 * mu * (2 * mu + 2 * lam));
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) S[i][j] = (i == j ? 1 / (2 * mu) : 0.0) - s2 * IxI[i][j];
+  mm3(P, C, Cp);
+  mm3(M, C, Cm);
+  mtm3(Cp, S, PS);
+  mtm3(Cm, S, MS);
+  mm3(PS, Cp, Cpp);
+  mm3(PS, Cm, Cpm);
+  mm3(MS, Cm, Cmm);
+  mm3(MS, Cp, Cmp);
+  const bool cross = split == EFEA_SPLIT_ANISOT_STRESS;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      cP[i][j] = cross ? (Cpp[i][j] + Cpm[i][j]) + Cmp[i][j] : Cpp[i][j];
+      cM[i][j] = cross ? Cmm[i][j] : (Cmm[i][j] + Cpm[i][j]) + Cmp[i][j];
     }
 }
 
@@ -143,9 +300,10 @@ k_pf_damage(int64_t Ne, const int* __restrict__ connect, const double* __restric
         double eps[3];
         strain2d<NPE>(dNx, ue, eps);
         double lam, mu;
-        lame2d(E[E_stride ? e : 0], P.nu, P.planeStress, lam, mu);
+        const double Ee = E[E_stride ? e : 0];
+        lame2d(Ee, P.nu, P.planeStress, lam, mu);
         double cP[3][3], cM[3][3];
-        split2d(P.split, eps, lam, mu, cP, cM);
+        split2d(P.split, eps, lam, mu, Ee, P.nu, P.planeStress, cP, cM);
         double psi = 0;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
@@ -259,9 +417,10 @@ k_pf_elastic(int64_t Ne, const int* __restrict__ connect, const double* __restri
       double eps[3];
       strain2d<NPE>(dNx, ue, eps);
       double lam, mu;
-      lame2d(E[E_stride ? e : 0], P.nu, P.planeStress, lam, mu);
+      const double Ee = E[E_stride ? e : 0];
+      lame2d(Ee, P.nu, P.planeStress, lam, mu);
       double cP[3][3], cM[3][3];
-      split2d(P.split, eps, lam, mu, cP, cM);
+      split2d(P.split, eps, lam, mu, Ee, P.nu, P.planeStress, cP, cM);
       double dp = 0;
 #pragma unroll
       for (int n = 0; n < NPE; ++n) dp += tab[TB::N + p * NPE + n] * de[n];
@@ -340,7 +499,8 @@ k_pf_strain_stress(int64_t Ne, const int* __restrict__ connect, const double* __
     de[n] = d[node];
   }
   double lam, mu;
-  lame2d(E[E_stride ? e : 0], P.nu, P.planeStress, lam, mu);
+  const double Ee = E[E_stride ? e : 0];
+  lame2d(Ee, P.nu, P.planeStress, lam, mu);
   const double isq2 = 1 / sqrt(2.0);
   double se[3] = {0, 0, 0}, ss[3] = {0, 0, 0}, evm = 0, svm = 0, w = 0;
 #pragma unroll
@@ -350,7 +510,7 @@ k_pf_strain_stress(int64_t Ne, const int* __restrict__ connect, const double* __
     double eps[3];
     strain2d<NPE>(dNx, ue, eps);
     double cP[3][3], cM[3][3];
-    split2d(P.split, eps, lam, mu, cP, cM);
+    split2d(P.split, eps, lam, mu, Ee, P.nu, P.planeStress, cP, cM);
     double dp = 0;
 #pragma unroll
     for (int n = 0; n < NPE; ++n) dp += tab[TB::N + p * NPE + n] * de[n];
@@ -397,9 +557,10 @@ __global__ void k_pf_split_points(int64_t n, int split, const double* __restrict
   if (i >= n) return;
   double e[3] = {eps[3 * i], eps[3 * i + 1], eps[3 * i + 2]};
   double lam, mu;
-  lame2d(E[E_stride ? i : 0], nu, planeStress, lam, mu);
+  const double Ei = E[E_stride ? i : 0];
+  lame2d(Ei, nu, planeStress, lam, mu);
   double cP[3][3], cM[3][3];
-  split2d(split, e, lam, mu, cP, cM);
+  split2d(split, e, lam, mu, Ei, nu, planeStress, cP, cM);
   double pp = 0, pm = 0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {

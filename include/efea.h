@@ -251,9 +251,23 @@ int efea_dot(int64_t n, const double* a, const double* b, double* scratch, doubl
 /* out = sum_k y[idx[k]] in index order (reaction resultants). */
 int efea_gather_sum(int64_t n, const int32_t* idx, const double* y, double* out, void* stream);
 
+/* Energy splits of the phase-field kernels (PhaseField.SplitType,
+ * Models/_phasefield.py:55-78) whose stiffness g c+ + c- is symmetric, as the
+ * PCG requires: Zhang and the *_PM / *_MP variants give non-symmetric ones and
+ * are not provided. */
+#define EFEA_SPLIT_AMOR 0
+#define EFEA_SPLIT_MIEHE 1
+#define EFEA_SPLIT_BOURDIN 2
+#define EFEA_SPLIT_STRESS 3
+#define EFEA_SPLIT_HE 4
+#define EFEA_SPLIT_ANISOT_STRAIN 5
+#define EFEA_SPLIT_ANISOT_STRAIN_NOCROSS 6
+#define EFEA_SPLIT_ANISOT_STRESS 7
+#define EFEA_SPLIT_ANISOT_STRESS_NOCROSS 8
+
 /* Phase field, damage problem: psi+ at the mass points with the History
  * irreversibility (psi+ <- H_old where smaller; H_old may be NULL), then
- * Kd_e (Ne, nPe, nPe), Fd_e (Ne, nPe).  split 0 Amor / 1 Miehe, regu 0 AT1 / 1 AT2.
+ * Kd_e (Ne, nPe, nPe), Fd_e (Ne, nPe).  split EFEA_SPLIT_*, regu 0 AT1 / 1 AT2.
  * Replaces PhaseField.__Calc_psiPlus_e_pg and __Construct_Damage_Matrix
  * (Simulations/_phasefield.py:494-573) with Models/_phasefield.py:236-571, 751-1128. */
 int efea_pf_damage_ke(int elem_type, int64_t Ne, const int32_t* connect, const double* coord,

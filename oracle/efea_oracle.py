@@ -670,12 +670,123 @@ def split_miehe(eps, E, v, dim, planeStress):
     return cP, cM
 
 
+def _mT(A):
+    return np.swapaxes(A, -2, -1)
+
+
+def _C_iso(E, v, dim, planeStress):
+    """Isotropic Kelvin-Mandel stiffness ``lam IxI + 2 mu I`` broadcast over
+    the leading axes of ``E`` (``Isotropic._Behavior``, ``Models/Elastic/_laws.py:407-482``)."""
+    lmbda, mu, _ = lame(E, v, dim, planeStress)
+    n = 3 if dim == 2 else 6
+    lm = np.asarray(lmbda)[..., None, None]
+    mu = np.asarray(mu)[..., None, None]
+    return lm * IxI(dim) + 2 * mu * np.eye(n)
+
+
+def _matvec(A, x):
+    """``A x`` summed in index order with unfused products.  The reference
+    forms ``C eps`` with numpy's matmul, whose BLAS may fuse multiply-adds:
+    equal principal stresses (a hydrostatic strain) then come out an ulp
+    apart, the 2D discriminant ``tr^2 - 4 det`` rounds negative and its
+    unguarded square root (``Models/_phasefield.py:783-787``) returns NaN.
+    Summed this way the equal components are equal and the discriminant is 0,
+    the value the decomposition is written for."""
+    A = np.broadcast_to(A, x.shape + (x.shape[-1],))
+    out = A[..., :, 0] * x[..., 0, None]
+    for j in range(1, x.shape[-1]):
+        out = out + A[..., :, j] * x[..., j, None]
+    return out
+
+
+def _cross_variants(split, Cpp, Cpm, Cmm, Cmp):
+    """The four ways of assigning the cross terms (:548-566, :660-676)."""
+    if split.endswith("_PM"):
+        return Cpp + Cpm, Cmm + Cmp
+    if split.endswith("_MP"):
+        return Cpp + Cmp, Cmm + Cpm
+    if split.endswith("_NoCross"):
+        return Cpp, Cmm + Cpm + Cmp
+    return Cpp + Cpm + Cmp, Cmm
+
+
+def split_bourdin(eps, E, v, dim, planeStress):
+    """``__Split_Bourdin`` (:433-449): ``c+ = C``, ``c- = 0``."""
+    C = np.broadcast_to(_C_iso(E, v, dim, planeStress), eps.shape + (eps.shape[-1],))
+    return C.copy(), np.zeros_like(C)
+
+
+def split_anisot_strain(split, eps, E, v, dim, planeStress):
+    """"Strain" branch of ``__Split_Strain`` (:539-566)."""
+    projP, projM = spectral_projectors(eps, dim)
+    C = _C_iso(E, v, dim, planeStress)
+    projPTC = _mT(projP) @ C
+    projMTc = _mT(projM) @ C
+    return _cross_variants(split, projPTC @ projP, projPTC @ projM, projMTc @ projM, projMTc @ projP)
+
+
+def split_stress(split, eps, E, v, dim, planeStress):
+    """``__Split_Stress`` (:573-678): the Stress split, Zhang and the
+    AnisotStress variants, from the projectors of ``sigma = C eps``."""
+    C = _C_iso(E, v, dim, planeStress)
+    Sigma = _matvec(C, eps)
+    projP, projM = spectral_projectors(Sigma, dim)
+    if split == "Stress":
+        E_ = np.asarray(E, float)[..., None, None] if np.ndim(E) else E
+        mu = E_ / (2 * (1 + v))
+        Rp, Rm = Rp_Rm(Sigma, dim)
+        Rp, Rm = Rp[..., None, None], Rm[..., None, None]
+        II = IxI(dim)
+        if dim == 2:
+            cv = v / E_ if planeStress else v * (1 + v) / E_
+            sP = ((1 + v) / E_ * projP) - (cv * Rp * II)
+            sM = ((1 + v) / E_ * projM) - (cv * Rm * II)
+        else:
+            sP = (1 / (2 * mu) * projP) - (v / E_ * Rp * II)
+            sM = (1 / (2 * mu) * projM) - (v / E_ * Rm * II)
+        return _mT(C) @ sP @ C, _mT(C) @ sM @ C
+    Cp, Cm = projP @ C, projM @ C
+    if split == "Zhang":
+        return Cp, Cm
+    S = np.linalg.inv(C)
+    ps, ms = _mT(Cp) @ S, _mT(Cm) @ S
+    return _cross_variants(split, ps @ Cp, ps @ Cm, ms @ Cm, ms @ Cp)
+
+
+def split_he(eps, E, v, dim, planeStress):
+    """``__Split_He`` (:680-749) with ``Get_sqrt_C_S``
+    (``Models/Elastic/_laws.py:223-247``: eigh of C)."""
+    C = _C_iso(E, v, dim, planeStress)
+    lam, Q = np.linalg.eigh(C)
+    sqrt_lam = np.sqrt(lam)[..., None, :]
+    sqrtC = (Q * sqrt_lam) @ _mT(Q)
+    inv_sqrtC = (Q / sqrt_lam) @ _mT(Q)
+    epst = _matvec(sqrtC, eps)
+    projPt, projMt = spectral_projectors(epst, dim)
+    projP = inv_sqrtC @ projPt @ sqrtC
+    projM = inv_sqrtC @ projMt @ sqrtC
+    return C @ projP, C @ projM
+
+
+SPLITS = ("Bourdin", "Amor", "Miehe", "He", "Stress", "Zhang", "AnisotStrain", "AnisotStrain_PM",
+          "AnisotStrain_MP", "AnisotStrain_NoCross", "AnisotStress", "AnisotStress_PM", "AnisotStress_MP",
+          "AnisotStress_NoCross")
+
+
 def calc_C(split, eps, E, v, dim, planeStress):
-    """``PhaseField.Calc_C`` (:396-431) for the Amor and Miehe splits."""
+    """``PhaseField.Calc_C`` (:396-431) for an isotropic material."""
+    if split == "Bourdin":
+        return split_bourdin(eps, E, v, dim, planeStress)
     if split == "Amor":
         return split_amor(eps, E, v, dim, planeStress)
     if split == "Miehe":
         return split_miehe(eps, E, v, dim, planeStress)
+    if "Strain" in split:
+        return split_anisot_strain(split, eps, E, v, dim, planeStress)
+    if split == "Zhang" or "Stress" in split:
+        return split_stress(split, eps, E, v, dim, planeStress)
+    if split == "He":
+        return split_he(eps, E, v, dim, planeStress)
     raise NotImplementedError(split)
 
 

@@ -198,7 +198,10 @@ def _strain_samples(rng, n):
     return np.concatenate([special, e])
 
 
-@pytest.mark.parametrize("split", ["Amor", "Miehe"])
+DEVICE_SPLITS = list(Models.PhaseField.SPLIT_CODE)
+
+
+@pytest.mark.parametrize("split", DEVICE_SPLITS)
 @pytest.mark.parametrize("ps", [False, True])
 def test_split_pointwise(split, ps):
     rng = np.random.default_rng(3)
@@ -210,7 +213,8 @@ def test_split_pointwise(split, ps):
     cM = torch.empty_like(cP)
     pP = torch.empty(n, dtype=torch.float64, device="cuda")
     pM = torch.empty_like(pP)
-    _lib.call("efea_pf_split_points", n, 0 if split == "Amor" else 1, ptr(d_eps), ptr(E), 0, 0.3, int(ps),
+    print(f"split {split} (code {Models.PhaseField.SPLIT_CODE[split]}), plane stress {ps}", flush=True)
+    _lib.call("efea_pf_split_points", n, Models.PhaseField.SPLIT_CODE[split], ptr(d_eps), ptr(E), 0, 0.3, int(ps),
               ptr(cP), ptr(cM), ptr(pP), ptr(pM), stream_ptr())
     rcP, rcM = O.calc_C(split, eps[None], 210000.0, 0.3, 2, ps)
     rpP, rpM = O.calc_psi(split, eps[None], 210000.0, 0.3, 2, ps)
@@ -224,7 +228,7 @@ def test_split_pointwise(split, ps):
     assert np.max(np.abs(pM.cpu().numpy() - rpM[0])) / psi_scale < 1e-12
 
 
-@pytest.mark.parametrize("split", ["Amor", "Miehe"])
+@pytest.mark.parametrize("split", DEVICE_SPLITS)
 @pytest.mark.parametrize("et", ["TRI3", "QUAD4"])
 def test_phasefield_element_operators(split, et):
     base = ea.tri3_notched_square(12) if et == "TRI3" else ea.quad4_rectangle(1.0, 1.0, 12, 12)
@@ -260,16 +264,18 @@ def test_phasefield_element_operators(split, et):
     assert np.max(np.abs(Ku.data - pfo.Ku.data)) <= 1e-12 * np.max(np.abs(pfo.Ku.data))
 
 
-@pytest.mark.parametrize("solver", ["hip_fp64", "hip_mixed"])
-def test_staggered_steps_match_oracle(solver):
-    """Five load steps of the notched-square tension test (Miehe, AT2)."""
+@pytest.mark.parametrize("solver,split", [("hip_fp64", "Miehe"), ("hip_mixed", "Miehe"), ("hip_fp64", "He"),
+                                          ("hip_fp64", "Stress"), ("hip_fp64", "AnisotStrain"),
+                                          ("hip_fp64", "AnisotStress_NoCross"), ("hip_fp64", "Bourdin")])
+def test_staggered_steps_match_oracle(solver, split):
+    """Five load steps of the notched-square tension test (AT2)."""
     n = 40
     mesh = ea.tri3_notched_square(n)
     mat = Models.Elastic.Isotropic(2, 210000.0, 0.3, planeStress=False, thickness=1.0)
-    pfm = Models.PhaseField(mat, "Miehe", "AT2", 2.7, 0.05)
+    pfm = Models.PhaseField(mat, split, "AT2", 2.7, 0.05)
     simu = Simulations.PhaseField(mesh, pfm)
     simu.solver = solver
-    ref = O.PhaseFieldOracle(mesh.connect, mesh.coord, "TRI3", 210000.0, 0.3, False, 1.0, "Miehe", "AT2", 2.7, 0.05)
+    ref = O.PhaseFieldOracle(mesh.connect, mesh.coord, "TRI3", 210000.0, 0.3, False, 1.0, split, "AT2", 2.7, 0.05)
     top = mesh.Nodes_Conditions(lambda x, y, z: y == 1.0)
     bot = mesh.Nodes_Conditions(lambda x, y, z: y == 0.0)
     dofsY = simu.Bc_dofs_nodes(top, ["y"])
@@ -293,6 +299,34 @@ def test_staggered_steps_match_oracle(solver):
         assert eu < 1e-8 and ed < 1e-8, errs
     f_dev, f_ref = np.array(f_dev), np.array(f_ref)
     assert np.max(np.abs(f_dev - f_ref) / np.abs(f_ref)) < 1e-6
+
+
+@pytest.mark.parametrize("split", ["Zhang", "AnisotStrain_PM", "AnisotStress_MP"])
+def test_non_symmetric_splits_are_refused(split):
+    mesh = ea.tri3_notched_square(8)
+    mat = Models.Elastic.Isotropic(2, 210000.0, 0.3, planeStress=False, thickness=1.0)
+    simu = Simulations.PhaseField(mesh, Models.PhaseField(mat, split, "AT2", 2.7, 0.05))
+    top = mesh.Nodes_Conditions(lambda x, y, z: y == 1.0)
+    simu.add_dirichlet(top, [0, 1e-3], ["x", "y"])
+    with pytest.raises(NotImplementedError, match="not symmetric"):
+        simu.Solve()
+
+
+def test_history_damage_returns_the_larger_damage():
+    n = 30
+    mesh = ea.tri3_notched_square(n)
+    mat = Models.Elastic.Isotropic(2, 210000.0, 0.3, planeStress=False, thickness=1.0)
+    simu = Simulations.PhaseField(mesh, Models.PhaseField(mat, "Miehe", "AT2", 2.7, 0.05, solver="HistoryDamage"))
+    top = mesh.Nodes_Conditions(lambda x, y, z: y == 1.0)
+    bot = mesh.Nodes_Conditions(lambda x, y, z: y == 0.0)
+    d_prev = simu.damage
+    for dep in (4e-3, 8e-3, 2e-3):  # unloading: the solved damage falls, the returned one does not
+        simu.Bc_Init()
+        simu.add_dirichlet(top, [0, dep], ["x", "y"])
+        simu.add_dirichlet(bot, [0], ["y"])
+        _, d, _ = simu.Solve(1.0, 500, 1)
+        assert np.array_equal(d, np.maximum(d_prev, simu.damage))
+        d_prev = simu.damage
 
 
 def test_solver_deterministic():

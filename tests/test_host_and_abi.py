@@ -196,7 +196,7 @@ def test_models_parameters_and_observers():
     with pytest.raises(AssertionError):
         pfm.l0 = -1.0
     with pytest.raises(ValueError):
-        ea.Models.PhaseField(mat, "Bourdin", "AT2", 1.0, 1.0)
+        ea.Models.PhaseField(mat, "Unknown", "AT2", 1.0, 1.0)
 
 
 def test_solver_type_names():

@@ -409,6 +409,89 @@ def test_split_special_cases():
         assert abs(pP[0, 1]) < 1e-12 * psi[0, 1]
 
 
+def _pos_neg(t, dim):
+    """Positive / negative parts of Kelvin-Mandel tensors from numpy's eigh."""
+    w, Q = np.linalg.eigh(O.vec_to_mat(t))
+    pos = O.mat_to_vec((Q * np.maximum(w, 0)[..., None, :]) @ np.swapaxes(Q, -2, -1))
+    neg = O.mat_to_vec((Q * np.minimum(w, 0)[..., None, :]) @ np.swapaxes(Q, -2, -1))
+    return pos, neg
+
+
+def _q(a, A, b):
+    return np.einsum("...i,...ij,...j->...", a, np.broadcast_to(A, a.shape + (a.shape[-1],)), b)
+
+
+@pytest.mark.parametrize("split", O.SPLITS)
+@pytest.mark.parametrize("dim,ps", [(2, False), (2, True), (3, False)])
+def test_every_split_adds_up_to_C(split, dim, ps):
+    """c+ + c- = C for every split of ``PhaseField.SplitType`` (heterogeneous E)."""
+    eps = _strains(dim, 120)
+    E = 210000.0 * (1 + 0.2 * np.random.default_rng(1).random((1, 120)))
+    cP, cM = O.calc_C(split, eps, E, 0.3, dim, ps)
+    C = O._C_iso(E, 0.3, dim, ps)
+    assert np.max(np.abs(cP + cM - C)) < 1e-11 * np.abs(C).max()
+
+
+@pytest.mark.parametrize("dim,ps", [(2, False), (2, True), (3, False)])
+def test_split_energies_in_closed_form(dim, ps):
+    """psi+ of each split from eigh-based positive parts, an independent
+    route: Bourdin 1/2 e.Ce; AnisotStrain 1/2 (e.Ce - e-.Ce-), its NoCross
+    1/2 e+.Ce+ and PM/MP 1/2 (e+.Ce+ + e+.Ce-); the stress splits with
+    s = Ce and S = C^-1: AnisotStress 1/2 (s.Ss - s-.Ss-), NoCross 1/2
+    s+.Ss+, Zhang 1/2 e.s+, Stress 1/2 ((1+nu)/E s+.s+ - c <tr s>+^2) with
+    c = nu(1+nu)/E in plane strain, nu/E in plane stress and 3D (the
+    compliance of each model); He 1/2 |et+|^2 for et = C^(1/2) e."""
+    eps = _strains(dim, 200)
+    E, nu = 210000.0, 0.3
+    C = O._C_iso(E, nu, dim, ps)
+    S = np.linalg.inv(C)
+    ep, em = _pos_neg(eps, dim)
+    sig = (C @ eps[..., None])[..., 0]
+    sp, sm = _pos_neg(sig, dim)
+    w, Q = np.linalg.eigh(C)
+    et = ((Q * np.sqrt(w)) @ Q.T @ eps[..., None])[..., 0]
+    etp, _ = _pos_neg(et, dim)
+    trs = sig[..., :dim].sum(-1)
+    cst = nu * (1 + nu) / E if (dim == 2 and not ps) else nu / E
+    ref = {
+        "Bourdin": 0.5 * _q(eps, C, eps),
+        "AnisotStrain": 0.5 * (_q(eps, C, eps) - _q(em, C, em)),
+        "AnisotStrain_NoCross": 0.5 * _q(ep, C, ep),
+        "AnisotStrain_PM": 0.5 * (_q(ep, C, ep) + _q(ep, C, em)),
+        "AnisotStrain_MP": 0.5 * (_q(ep, C, ep) + _q(em, C, ep)),
+        "AnisotStress": 0.5 * (_q(sig, S, sig) - _q(sm, S, sm)),
+        "AnisotStress_NoCross": 0.5 * _q(sp, S, sp),
+        "AnisotStress_PM": 0.5 * (_q(sp, S, sp) + _q(sp, S, sm)),
+        "AnisotStress_MP": 0.5 * (_q(sp, S, sp) + _q(sm, S, sp)),
+        "Zhang": 0.5 * np.sum(eps * sp, -1),
+        "Stress": 0.5 * ((1 + nu) / E * np.sum(sp * sp, -1) - cst * np.maximum(trs, 0) ** 2),
+        "He": 0.5 * np.sum(etp * etp, -1),
+    }
+    scale = np.abs(ref["Bourdin"]).max()
+    for split, r in ref.items():
+        psiP, psiM = O.calc_psi(split, eps, E, nu, dim, ps)
+        assert np.max(np.abs(psiP - r)) < 1e-11 * scale, split
+        assert np.max(np.abs(psiP + psiM - ref["Bourdin"])) < 1e-11 * scale, split
+
+
+@pytest.mark.parametrize("dim,ps", [(2, False), (2, True), (3, False)])
+def test_spectral_splits_in_pure_tension_and_compression(dim, ps):
+    """All principal strains (and stresses) positive: c+ = C, c- = 0; all
+    negative: c+ = 0, c- = C -- for every split built on spectral projectors."""
+    n = 3 if dim == 2 else 6
+    e = np.zeros((1, 2, n))
+    e[0, 0, :dim] = [1e-3, 2e-3, 1.5e-3][:dim]
+    e[0, 1, :dim] = [-1e-3, -2e-3, -1.5e-3][:dim]
+    C = O._C_iso(210000.0, 0.3, dim, ps)
+    for split in ("Miehe", "He", "Stress", "Zhang", "AnisotStrain", "AnisotStrain_NoCross", "AnisotStrain_PM",
+                  "AnisotStress", "AnisotStress_NoCross", "AnisotStress_MP"):
+        cP, cM = O.calc_C(split, e, 210000.0, 0.3, dim, ps)
+        assert np.max(np.abs(cP[0, 0] - C)) < 1e-10 * np.abs(C).max(), split
+        assert np.max(np.abs(cM[0, 0])) < 1e-10 * np.abs(C).max(), split
+        assert np.max(np.abs(cP[0, 1])) < 1e-10 * np.abs(C).max(), split
+        assert np.max(np.abs(cM[0, 1] - C)) < 1e-10 * np.abs(C).max(), split
+
+
 def test_pf_coefficients():
     psi = np.array([[0.0, 1.0, 2.0]])
     assert np.allclose(O.pf_r(psi, 2.7, 0.01, "AT2"), 2 * psi + 270.0)
