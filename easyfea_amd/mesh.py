@@ -88,6 +88,7 @@ class Mesh:
         for elemType, connect in groups.items():
             self.dict_groupElem[ElemType(elemType)] = GroupElem(elemType, connect, coord)
         self.dim = max(g.dim for g in self.dict_groupElem.values())
+        self.tags: dict = {}  # node sets by name (gmsh physical groups)
         used = np.zeros(self.Nn, dtype=bool)
         for g in self.Get_list_groupElem(self.dim):
             used[g.connect.ravel()] = True
@@ -125,6 +126,22 @@ class Mesh:
         if dim is None:
             dim = self.dim
         return [g for g in self.dict_groupElem.values() if g.dim == dim]
+
+    def Nodes_Tags(self, tags: list) -> np.ndarray:
+        """Sorted nodes of the named node sets (``FEM/_mesh.py`` ``Nodes_Tags``)."""
+        tags = [tags] if isinstance(tags, str) else list(tags)
+        missing = [t for t in tags if t not in self.tags]
+        assert not missing, f"unknown tags {missing}; available: {sorted(self.tags)}"
+        if not tags:
+            return np.array([], dtype=np.int64)
+        return np.unique(np.concatenate([np.asarray(self.tags[t], dtype=np.int64) for t in tags]))
+
+    def Save(self, folder: str, filename: str = "mesh") -> str:
+        """Writes ``folder/filename.npz`` (coordinates, connectivities, tags);
+        ``mesh_io.load_mesh`` reads it back."""
+        from .mesh_io import save_mesh
+
+        return save_mesh(self, folder, filename)
 
     def Nodes_Conditions(self, func: Callable) -> np.ndarray:
         """Nodes of the main-dimension groups meeting ``func(x, y, z)``
