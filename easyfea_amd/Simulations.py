@@ -983,8 +983,9 @@ class PhaseField(_Simu):
 
     def __init__(self, mesh: Mesh, model: PhaseFieldModel, folder: str = "", verbosity=False):
         assert isinstance(model, PhaseFieldModel), "model must be a phase field model"
-        assert mesh.dim == 2 and mesh.elemType in (ElemType.TRI3, ElemType.QUAD4), \
-            "the device phase-field path supports 2D TRI3 / QUAD4 meshes"
+        assert mesh.elemType in (ElemType.TRI3, ElemType.QUAD4, ElemType.TETRA4, ElemType.TETRA10, ElemType.HEXA8), \
+            "the device phase-field path supports TRI3 / QUAD4 / TETRA4 / TETRA10 / HEXA8 meshes"
+        assert mesh.dim == model.dim, "mesh and material dimensions differ"
         assert model.solver != PhaseFieldModel.SolverType.BoundConstrain, \
             "BoundConstrain (scipy lsq_linear on the host) has no device solver; use History or HistoryDamage"
         self._updU = False
@@ -1072,6 +1073,8 @@ class PhaseField(_Simu):
             raise NotImplementedError(
                 f"split {split}: its displacement stiffness g c+ + c- is not symmetric; the device solver "
                 f"(PCG) needs a symmetric matrix. Device splits: {list(PhaseFieldModel.SPLIT_CODE)}")
+        if self.model.dim == 3 and split not in ("Amor", "Bourdin"):
+            raise NotImplementedError(f"split {split} in 3D: the device kernels provide Amor and Bourdin in 3D")
         return code
 
     def _Gc_dev(self):
@@ -1291,16 +1294,22 @@ class PhaseField(_Simu):
             return False
         return bool(torch.any(self._d[to_dev(nodes)] >= threshold))
 
+    def _comps(self) -> list:
+        return ["xx", "yy", "xy"] if self.model.dim == 2 else ["xx", "yy", "zz", "yz", "xz", "xy"]
+
     def Results_Available(self) -> list:
         """``PhaseField.Results_Available`` (``Simulations/_phasefield.py:659-680``)."""
-        return ["displacement", "displacement_norm", "displacement_matrix", "ux", "uy", "Sxx", "Syy", "Sxy",
-                "Exx", "Eyy", "Exy", "Svm", "Stress", "Evm", "Strain", "damage", "psiP", "Psi_Crack", "Wdef"]
+        axes = ["x", "y"] if self.model.dim == 2 else ["x", "y", "z"]
+        return (["displacement", "displacement_norm", "displacement_matrix"] + ["u" + a for a in axes]
+                + ["S" + c for c in self._comps()] + ["E" + c for c in self._comps()]
+                + ["Svm", "Stress", "Evm", "Strain", "damage", "psiP", "Psi_Crack", "Wdef"])
 
     def _strain_stress_e(self) -> torch.Tensor:
-        """(Ne, 9) per element on the device: strain components, Evm, damaged
-        stress components, Svm, W_e (``efea_pf_strain_stress``)."""
+        """(Ne, 2 ns + 3) per element on the device: strain components, Evm,
+        damaged stress components, Svm, W_e (``efea_pf_strain_stress``)."""
         dev, pfm = self.dev, self.model
-        out = torch.empty((dev.Ne, 9), dtype=torch.float64, device="cuda")
+        ns = len(self._comps())
+        out = torch.empty((dev.Ne, 2 * ns + 3), dtype=torch.float64, device="cuda")
         tr, nr = dev.tables(MatrixType.rigi)
         E, Es, nu, ps = self._material_args()
         _lib.call("efea_pf_strain_stress", dev.code, dev.Ne, ptr(dev.connect), ptr(dev.coord), ptr(tr), nr,
@@ -1340,29 +1349,30 @@ class PhaseField(_Simu):
             values = self._psiP_now().mean(1).contiguous()
         elif result == "damage":
             values = self.damage
-        elif result in ("ux", "uy"):
-            values = self.displacement.reshape(Nn, -1)[:, 0 if result == "ux" else 1]
+        elif result in ("ux", "uy", "uz"):
+            values = self.displacement.reshape(Nn, -1)[:, "xyz".index(result[1])]
         elif result == "displacement":
             values = self.displacement
         elif result == "displacement_norm":
             values = np.linalg.norm(self.displacement.reshape(Nn, -1), axis=1)
         elif result == "displacement_matrix":
             values = np.zeros((Nn, 3))
-            values[:, :2] = self.displacement.reshape(Nn, -1)
+            values[:, :self.model.dim] = self.displacement.reshape(Nn, -1)
         else:
             e = self._strain_stress_e()
-            comps = ["xx", "yy", "xy"]
+            comps = self._comps()
+            ns = len(comps)
             if result == "Strain":
-                values = e[:, :3]
+                values = e[:, :ns]
             elif result == "Stress":
-                values = e[:, 4:7]
+                values = e[:, ns + 1:2 * ns + 1]
             elif result == "Evm":
-                values = e[:, 3]
+                values = e[:, ns]
             elif result == "Svm":
-                values = e[:, 7]
+                values = e[:, 2 * ns + 1]
             elif result[0] == "E":
                 values = e[:, comps.index(result[1:])]
             else:
-                values = e[:, 4 + comps.index(result[1:])]
+                values = e[:, ns + 1 + comps.index(result[1:])]
             values = values.contiguous()
         return self._reshape_values(values, nodeValues)

@@ -1,4 +1,5 @@
-// Phase-field element kernels (2D: TRI3, QUAD4), FP64 throughout.
+// Phase-field element kernels (2D: TRI3, QUAD4; 3D: TETRA4, TETRA10, HEXA8),
+// FP64 throughout.
 //
 //  * k_pf_damage: at the mass points, strain from u, energy split psi+,
 //    history psi+ <- old where psi+ < old (Simulations/_phasefield.py:494-538),
@@ -10,7 +11,9 @@
 //    K_e = sum_p wJ B^T c B x thickness (_phasefield.py:444-482).
 // The split follows Models/_phasefield.py: Amor :451-483, Miehe :503-537 with
 // the 2D closed-form eigen decomposition :751-802 and the projector build
-// :1043-1128; Rp/Rm :485-501.
+// :1043-1128; Rp/Rm :485-501.  In 3D the splits without a spectral
+// decomposition (Amor, Bourdin) are provided.  Element matrices carry the
+// thickness in 2D only (the reference scales them when its dimension is 2).
 #include "efea_common.h"
 
 namespace efea {
@@ -257,6 +260,92 @@ __device__ inline void strain2d(const double (&dNx)[2][NPE], const double* ue, d
   }
 }
 
+template <int DIM>
+struct PfDim {
+  static constexpr int NS = DIM == 2 ? 3 : 6;  // Kelvin-Mandel components
+};
+
+// 3D Kelvin-Mandel strain [xx, yy, zz, sqrt2 yz, sqrt2 xz, sqrt2 xy].
+template <int NPE>
+__device__ inline void strain3d(const double (&dNx)[3][NPE], const double* ue, double e[6]) {
+  const double c = 1 / sqrt(2.0);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) e[k] = 0;
+#pragma unroll
+  for (int n = 0; n < NPE; ++n) {
+    const double ux = ue[3 * n], uy = ue[3 * n + 1], uz = ue[3 * n + 2];
+    e[0] += dNx[0][n] * ux;
+    e[1] += dNx[1][n] * uy;
+    e[2] += dNx[2][n] * uz;
+    e[3] += (dNx[2][n] * c) * uy + (dNx[1][n] * c) * uz;
+    e[4] += (dNx[2][n] * c) * ux + (dNx[0][n] * c) * uz;
+    e[5] += (dNx[1][n] * c) * ux + (dNx[0][n] * c) * uy;
+  }
+}
+
+// 3D splits without a spectral decomposition: Amor (bulk lam + 2 mu / 3 on
+// the volumetric part, Isym - IxI / 3 deviatoric) and Bourdin.
+__device__ inline void split3d(int split, const double e[6], double lam, double mu, double cP[6][6],
+                               double cM[6][6]) {
+  const double tr_e = e[0] + e[1] + e[2];
+  const double Rp = (1 + sgn(tr_e)) / 2, Rm = (1 + sgn(-tr_e)) / 2;
+  const double bulk = lam + 2 * mu / 3;
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const double IxI = (i < 3 && j < 3) ? 1.0 : 0.0;
+      const double Iij = (i == j) ? 1.0 : 0.0;
+      if (split == EFEA_SPLIT_AMOR) {
+        cP[i][j] = bulk * (Rp * IxI) + 2 * mu * (Iij - 1.0 / 3 * IxI);
+        cM[i][j] = bulk * (Rm * IxI);
+      } else {  // Bourdin: c+ = C
+        cP[i][j] = lam * IxI + (i == j ? 2 * mu : 0.0);
+        cM[i][j] = 0;
+      }
+    }
+}
+
+template <int DIM, int NPE>
+__device__ inline void strain_km(const double (&dNx)[DIM][NPE], const double* ue, double (&e)[PfDim<DIM>::NS]) {
+  if constexpr (DIM == 2)
+    strain2d<NPE>(dNx, ue, e);
+  else
+    strain3d<NPE>(dNx, ue, e);
+}
+
+// c+ / c- of the element's material at one point (E, nu, plane stress of the model)
+template <int DIM>
+__device__ inline void split_km(int split, const double (&e)[PfDim<DIM>::NS], double E, double nu, int planeStress,
+                                double (&cP)[PfDim<DIM>::NS][PfDim<DIM>::NS],
+                                double (&cM)[PfDim<DIM>::NS][PfDim<DIM>::NS]) {
+  double lam, mu;
+  if constexpr (DIM == 2) {
+    lame2d(E, nu, planeStress, lam, mu);
+    split2d(split, e, lam, mu, E, nu, planeStress, cP, cM);
+  } else {
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu));
+    mu = E / (2 * (1 + nu));
+    split3d(split, e, lam, mu, cP, cM);
+  }
+}
+
+// B columns of node a (derivatives dx, dy[, dz]) in Kelvin-Mandel rows.
+template <int DIM>
+__device__ inline void bcols(const double* dn, double (&B)[DIM][PfDim<DIM>::NS]) {
+  const double c = 1 / sqrt(2.0);
+  if constexpr (DIM == 2) {
+    const double ax = dn[0], ay = dn[1];
+    B[0][0] = ax; B[0][1] = 0; B[0][2] = ay * c;
+    B[1][0] = 0; B[1][1] = ay; B[1][2] = ax * c;
+  } else {
+    const double ax = dn[0], ay = dn[1], az = dn[2];
+    B[0][0] = ax; B[0][1] = 0; B[0][2] = 0; B[0][3] = 0; B[0][4] = az * c; B[0][5] = ay * c;
+    B[1][0] = 0; B[1][1] = ay; B[1][2] = 0; B[1][3] = az * c; B[1][4] = 0; B[1][5] = ax * c;
+    B[2][0] = 0; B[2][1] = 0; B[2][2] = az; B[2][3] = ay * c; B[2][4] = ax * c; B[2][5] = 0;
+  }
+}
+
 template <int ET, int NPGM, int NPGR>
 __global__ void __launch_bounds__(256)
 k_pf_damage(int64_t Ne, const int* __restrict__ connect, const double* __restrict__ coord,
@@ -264,7 +353,7 @@ k_pf_damage(int64_t Ne, const int* __restrict__ connect, const double* __restric
             const double* __restrict__ H_old, const double* __restrict__ E, int E_stride,
             const double* __restrict__ Gc, int Gc_stride, PfParams P, double* __restrict__ psiP,
             double* __restrict__ Kd, double* __restrict__ Fd) {
-  constexpr int NPE = Elem<ET>::NPE, DIM = 2;
+  constexpr int NPE = Elem<ET>::NPE, DIM = Elem<ET>::DIM, NS = PfDim<DIM>::NS;
   constexpr int NPAIR = NPE * NPE;
   constexpr int EPB = 256 / NPAIR;
   constexpr int NPGT = NPGM + NPGR;
@@ -284,32 +373,29 @@ k_pf_damage(int64_t Ne, const int* __restrict__ connect, const double* __restric
     const int64_t e = e0 + el;
     if (e < Ne) {
       double X[NPE][3];
-      double ue[2 * NPE];
+      double ue[DIM * NPE];
 #pragma unroll
       for (int n = 0; n < NPE; ++n) {
         const int node = connect[e * NPE + n];
 #pragma unroll
         for (int k = 0; k < 3; ++k) X[n][k] = coord[(int64_t)node * 3 + k];
-        ue[2 * n] = u[(int64_t)node * 2];
-        ue[2 * n + 1] = u[(int64_t)node * 2 + 1];
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) ue[DIM * n + k] = u[(int64_t)node * DIM + k];
       }
       double dNx[DIM][NPE];
       if (q < NPGM) {
         const double detJ = point_geometry<double, NPE, DIM>(tab_m + TM::DN + q * DIM * NPE, X, dNx);
         s_wJm[el][q] = detJ * tab_m[TM::W + q];
-        double eps[3];
-        strain2d<NPE>(dNx, ue, eps);
-        double lam, mu;
-        const double Ee = E[E_stride ? e : 0];
-        lame2d(Ee, P.nu, P.planeStress, lam, mu);
-        double cP[3][3], cM[3][3];
-        split2d(P.split, eps, lam, mu, Ee, P.nu, P.planeStress, cP, cM);
+        double eps[NS];
+        strain_km<DIM, NPE>(dNx, ue, eps);
+        double cP[NS][NS], cM[NS][NS];
+        split_km<DIM>(P.split, eps, E[E_stride ? e : 0], P.nu, P.planeStress, cP, cM);
         double psi = 0;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
+        for (int i = 0; i < NS; ++i) {
           double sig = 0;
 #pragma unroll
-          for (int j = 0; j < 3; ++j) sig += cP[i][j] * eps[j];
+          for (int j = 0; j < NS; ++j) sig += cP[i][j] * eps[j];
           psi += 1.0 / 2 * eps[i] * sig;
         }
         if (H_old) {
@@ -360,12 +446,13 @@ k_pf_damage(int64_t Ne, const int* __restrict__ connect, const double* __restric
       }
       const double gc = Gc[Gc_stride ? e : 0];
       const double kd = P.regu == 1 ? gc * P.l0 : 3.0 / 4 * gc * P.l0;
-      s_K[el * NPAIR + ab] = (R + kd * D) * P.thickness;
+      const double th = DIM == 2 ? P.thickness : 1.0;
+      s_K[el * NPAIR + ab] = (R + kd * D) * th;
       if (b == 0) {
         double F = 0;
 #pragma unroll
         for (int p = 0; p < NPGM; ++p) F += s_f[el][p] * s_wJm[el][p] * tab_m[TM::N + p * NPE + a];
-        Fd[e * NPE + a] = F * P.thickness;
+        Fd[e * NPE + a] = F * th;
       }
     }
   }
@@ -380,13 +467,13 @@ __global__ void __launch_bounds__(256)
 k_pf_elastic(int64_t Ne, const int* __restrict__ connect, const double* __restrict__ coord,
              const double* __restrict__ tab, const double* __restrict__ u, const double* __restrict__ d,
              const double* __restrict__ E, int E_stride, PfParams P, double* __restrict__ Ke) {
-  constexpr int NPE = Elem<ET>::NPE, DIM = 2, ND = 2 * NPE;
+  constexpr int NPE = Elem<ET>::NPE, DIM = Elem<ET>::DIM, ND = DIM * NPE, NS = PfDim<DIM>::NS;
   constexpr int NPAIR = NPE * NPE;
   constexpr int EPB = 256 / NPAIR;
   using TB = Tab<NPG, NPE, DIM>;
   __shared__ double s_dNx[EPB][NPG][DIM][NPE];
   __shared__ double s_wJ[EPB][NPG];
-  __shared__ double s_c[EPB][NPG][3][3];
+  __shared__ double s_c[EPB][NPG][NS][NS];
   __shared__ double s_K[EPB * ND * ND];
 
   const int64_t e0 = (int64_t)blockIdx.x * EPB;
@@ -396,15 +483,15 @@ k_pf_elastic(int64_t Ne, const int* __restrict__ connect, const double* __restri
     const int64_t e = e0 + el;
     if (e < Ne) {
       double X[NPE][3];
-      double ue[2 * NPE];
+      double ue[DIM * NPE];
       double de[NPE];
 #pragma unroll
       for (int n = 0; n < NPE; ++n) {
         const int node = connect[e * NPE + n];
 #pragma unroll
         for (int k = 0; k < 3; ++k) X[n][k] = coord[(int64_t)node * 3 + k];
-        ue[2 * n] = u[(int64_t)node * 2];
-        ue[2 * n + 1] = u[(int64_t)node * 2 + 1];
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) ue[DIM * n + k] = u[(int64_t)node * DIM + k];
         de[n] = d[node];
       }
       double dNx[DIM][NPE];
@@ -414,21 +501,18 @@ k_pf_elastic(int64_t Ne, const int* __restrict__ connect, const double* __restri
       for (int i = 0; i < DIM; ++i)
 #pragma unroll
         for (int n = 0; n < NPE; ++n) s_dNx[el][p][i][n] = dNx[i][n];
-      double eps[3];
-      strain2d<NPE>(dNx, ue, eps);
-      double lam, mu;
-      const double Ee = E[E_stride ? e : 0];
-      lame2d(Ee, P.nu, P.planeStress, lam, mu);
-      double cP[3][3], cM[3][3];
-      split2d(P.split, eps, lam, mu, Ee, P.nu, P.planeStress, cP, cM);
+      double eps[NS];
+      strain_km<DIM, NPE>(dNx, ue, eps);
+      double cP[NS][NS], cM[NS][NS];
+      split_km<DIM>(P.split, eps, E[E_stride ? e : 0], P.nu, P.planeStress, cP, cM);
       double dp = 0;
 #pragma unroll
       for (int n = 0; n < NPE; ++n) dp += tab[TB::N + p * NPE + n] * de[n];
       const double g = (1 - dp) * (1 - dp) + P.k_res;
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
+      for (int i = 0; i < NS; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) s_c[el][p][i][j] = g * cP[i][j] + cM[i][j];
+        for (int j = 0; j < NS; ++j) s_c[el][p][i][j] = g * cP[i][j] + cM[i][j];
     }
   }
   __syncthreads();
@@ -437,34 +521,42 @@ k_pf_elastic(int64_t Ne, const int* __restrict__ connect, const double* __restri
     const int el = t / NPAIR, ab = t % NPAIR, a = ab / NPE, b = ab % NPE;
     const int64_t e = e0 + el;
     if (e < Ne) {
-      const double c = 1 / sqrt(2.0);
-      double acc[2][2] = {{0, 0}, {0, 0}};
+      double acc[DIM][DIM];
+#pragma unroll
+      for (int i = 0; i < DIM; ++i)
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) acc[i][j] = 0;
 #pragma unroll
       for (int p = 0; p < NPG; ++p) {
-        const double ax = s_dNx[el][p][0][a], ay = s_dNx[el][p][1][a];
-        const double bx = s_dNx[el][p][0][b], by = s_dNx[el][p][1][b];
-        // B columns: node a, component x -> [ax, 0, c ay]; component y -> [0, ay, c ax]
-        const double Ba[2][3] = {{ax, 0, ay * c}, {0, ay, ax * c}};
-        const double Bb[2][3] = {{bx, 0, by * c}, {0, by, bx * c}};
+        double da[DIM], db[DIM];
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < DIM; ++i) {
+          da[i] = s_dNx[el][p][i][a];
+          db[i] = s_dNx[el][p][i][b];
+        }
+        double Ba[DIM][NS], Bb[DIM][NS];
+        bcols<DIM>(da, Ba);
+        bcols<DIM>(db, Bb);
 #pragma unroll
-          for (int j = 0; j < 2; ++j) {
+        for (int i = 0; i < DIM; ++i)
+#pragma unroll
+          for (int j = 0; j < DIM; ++j) {
             double s = 0;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
+            for (int k = 0; k < NS; ++k) {
               double cb = 0;
 #pragma unroll
-              for (int l = 0; l < 3; ++l) cb += s_c[el][p][k][l] * Bb[j][l];
+              for (int l = 0; l < NS; ++l) cb += s_c[el][p][k][l] * Bb[j][l];
               s += Ba[i][k] * cb;
             }
             acc[i][j] += s_wJ[el][p] * s;
           }
       }
+      const double th = DIM == 2 ? P.thickness : 1.0;
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < DIM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) s_K[el * ND * ND + (2 * a + i) * ND + 2 * b + j] = acc[i][j] * P.thickness;
+        for (int j = 0; j < DIM; ++j) s_K[el * ND * ND + (DIM * a + i) * ND + DIM * b + j] = acc[i][j] * th;
     }
   }
   __syncthreads();
@@ -484,67 +576,79 @@ __global__ void __launch_bounds__(256)
 k_pf_strain_stress(int64_t Ne, const int* __restrict__ connect, const double* __restrict__ coord,
                    const double* __restrict__ tab, const double* __restrict__ u, const double* __restrict__ d,
                    const double* __restrict__ E, int E_stride, PfParams P, double* __restrict__ out) {
-  constexpr int NPE = Elem<ET>::NPE, DIM = 2;
+  constexpr int NPE = Elem<ET>::NPE, DIM = Elem<ET>::DIM, NS = PfDim<DIM>::NS, NOUT = 2 * NS + 3;
   using TB = Tab<NPG, NPE, DIM>;
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= Ne) return;
-  double X[NPE][3], ue[2 * NPE], de[NPE];
+  double X[NPE][3], ue[DIM * NPE], de[NPE];
 #pragma unroll
   for (int n = 0; n < NPE; ++n) {
     const int node = connect[e * NPE + n];
 #pragma unroll
     for (int k = 0; k < 3; ++k) X[n][k] = coord[(int64_t)node * 3 + k];
-    ue[2 * n] = u[(int64_t)node * 2];
-    ue[2 * n + 1] = u[(int64_t)node * 2 + 1];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) ue[DIM * n + k] = u[(int64_t)node * DIM + k];
     de[n] = d[node];
   }
-  double lam, mu;
   const double Ee = E[E_stride ? e : 0];
-  lame2d(Ee, P.nu, P.planeStress, lam, mu);
   const double isq2 = 1 / sqrt(2.0);
-  double se[3] = {0, 0, 0}, ss[3] = {0, 0, 0}, evm = 0, svm = 0, w = 0;
+  double se[NS], ss[NS], evm = 0, svm = 0, w = 0;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) se[k] = ss[k] = 0;
 #pragma unroll
   for (int p = 0; p < NPG; ++p) {
     double dNx[DIM][NPE];
     const double detJ = point_geometry<double, NPE, DIM>(tab + TB::DN + p * DIM * NPE, X, dNx);
-    double eps[3];
-    strain2d<NPE>(dNx, ue, eps);
-    double cP[3][3], cM[3][3];
-    split2d(P.split, eps, lam, mu, Ee, P.nu, P.planeStress, cP, cM);
+    double eps[NS];
+    strain_km<DIM, NPE>(dNx, ue, eps);
+    double cP[NS][NS], cM[NS][NS];
+    split_km<DIM>(P.split, eps, Ee, P.nu, P.planeStress, cP, cM);
     double dp = 0;
 #pragma unroll
     for (int n = 0; n < NPE; ++n) dp += tab[TB::N + p * NPE + n] * de[n];
     const double g = (1 - dp) * (1 - dp) + P.k_res;
-    double sig[3];
+    double sig[NS];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
+    for (int a = 0; a < NS; ++a) {
       double sp = 0, sm = 0;
 #pragma unroll
-      for (int b = 0; b < 3; ++b) {
+      for (int b = 0; b < NS; ++b) {
         sp += cP[a][b] * eps[b];
         sm += cM[a][b] * eps[b];
       }
       sig[a] = g * sp + sm;
     }
-    w += detJ * tab[TB::W + p] * (sig[0] * eps[0] + sig[1] * eps[1] + sig[2] * eps[2]);
-    const double te[3] = {eps[0], eps[1], eps[2] * isq2}, ts[3] = {sig[0], sig[1], sig[2] * isq2};
+    double dot = 0;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < NS; ++k) dot += sig[k] * eps[k];
+    w += detJ * tab[TB::W + p] * dot;
+    double te[NS], ts[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      te[k] = k < DIM ? eps[k] : eps[k] * isq2;
+      ts[k] = k < DIM ? sig[k] : sig[k] * isq2;
       se[k] += te[k];
       ss[k] += ts[k];
     }
-    evm += sqrt(te[0] * te[0] + te[1] * te[1] - te[0] * te[1] + 3 * te[2] * te[2]);
-    svm += sqrt(ts[0] * ts[0] + ts[1] * ts[1] - ts[0] * ts[1] + 3 * ts[2] * ts[2]);
+    if constexpr (DIM == 2) {
+      evm += sqrt(te[0] * te[0] + te[1] * te[1] - te[0] * te[1] + 3 * te[2] * te[2]);
+      svm += sqrt(ts[0] * ts[0] + ts[1] * ts[1] - ts[0] * ts[1] + 3 * ts[2] * ts[2]);
+    } else {
+      evm += sqrt(0.5 * ((te[0] - te[1]) * (te[0] - te[1]) + (te[1] - te[2]) * (te[1] - te[2]) +
+                         (te[2] - te[0]) * (te[2] - te[0]) + 6 * (te[5] * te[5] + te[3] * te[3] + te[4] * te[4])));
+      svm += sqrt(0.5 * ((ts[0] - ts[1]) * (ts[0] - ts[1]) + (ts[1] - ts[2]) * (ts[1] - ts[2]) +
+                         (ts[2] - ts[0]) * (ts[2] - ts[0]) + 6 * (ts[5] * ts[5] + ts[3] * ts[3] + ts[4] * ts[4])));
+    }
   }
-  double* o = out + e * 9;
+  double* o = out + e * NOUT;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < NS; ++k) {
     o[k] = se[k] / NPG;
-    o[4 + k] = ss[k] / NPG;
+    o[NS + 1 + k] = ss[k] / NPG;
   }
-  o[3] = evm / NPG;
-  o[7] = svm / NPG;
-  o[8] = 0.5 * w * P.thickness;
+  o[NS] = evm / NPG;
+  o[2 * NS + 1] = svm / NPG;
+  o[2 * NS + 2] = 0.5 * w * (DIM == 2 ? P.thickness : 1.0);
 }
 
 // Gauss-point split evaluation for testing and post-processing:
@@ -591,24 +695,41 @@ static PfParams make_params(int split, int regu, int ps, double nu, double l0, d
   return P;
 }
 
+template <int ET>
+static unsigned pf_blocks(int64_t Ne) {
+  constexpr int EPB = 256 / (Elem<ET>::NPE * Elem<ET>::NPE);
+  return (unsigned)((Ne + EPB - 1) / EPB);
+}
+
+// In 3D only the splits without a spectral decomposition are provided.
+static bool pf_split_ok(int elem, int split, const char* what) {
+  const bool three_d = elem == ET_TETRA4 || elem == ET_TETRA10 || elem == ET_HEXA8;
+  if (three_d && split != EFEA_SPLIT_AMOR && split != EFEA_SPLIT_BOURDIN) {
+    set_error("%s: split %d is not provided in 3D (Amor and Bourdin are)", what, split);
+    return false;
+  }
+  return true;
+}
+
 int pf_damage(int elem, int64_t Ne, const int* connect, const double* coord, const double* tab_m, int nPgm,
               const double* tab_r, int nPgr, const double* u, const double* H_old, const double* E,
               int E_stride, double nu, int ps, const double* Gc, int Gc_stride, double l0, int split, int regu,
               double thickness, double* psiP, double* Kd, double* Fd, hipStream_t s) {
   const PfParams P = make_params(split, regu, ps, nu, l0, thickness);
+  if (!pf_split_ok(elem, split, "pf_damage")) return EFEA_ERR_ARG;
   if (Ne == 0) return EFEA_OK;
-  if (elem == ET_TRI3 && nPgm == 3 && nPgr == 1) {
-    const unsigned nb = (unsigned)((Ne + 27) / 28);
-    hipLaunchKernelGGL((k_pf_damage<ET_TRI3, 3, 1>), dim3(nb), dim3(256), 0, s, Ne, connect, coord, tab_m, tab_r,
-                       u, H_old, E, E_stride, Gc, Gc_stride, P, psiP, Kd, Fd);
-    return check_launch("pf_damage");
+#define PFD(ET, M, R)                                                                                           \
+  if (elem == ET && nPgm == M && nPgr == R) {                                                                   \
+    hipLaunchKernelGGL((k_pf_damage<ET, M, R>), dim3(pf_blocks<ET>(Ne)), dim3(256), 0, s, Ne, connect, coord,   \
+                       tab_m, tab_r, u, H_old, E, E_stride, Gc, Gc_stride, P, psiP, Kd, Fd);                     \
+    return check_launch("pf_damage");                                                                           \
   }
-  if (elem == ET_QUAD4 && nPgm == 4 && nPgr == 4) {
-    const unsigned nb = (unsigned)((Ne + 15) / 16);
-    hipLaunchKernelGGL((k_pf_damage<ET_QUAD4, 4, 4>), dim3(nb), dim3(256), 0, s, Ne, connect, coord, tab_m,
-                       tab_r, u, H_old, E, E_stride, Gc, Gc_stride, P, psiP, Kd, Fd);
-    return check_launch("pf_damage");
-  }
+  PFD(ET_TRI3, 3, 1)
+  PFD(ET_QUAD4, 4, 4)
+  PFD(ET_TETRA4, 4, 1)
+  PFD(ET_TETRA10, 4, 4)
+  PFD(ET_HEXA8, 8, 8)
+#undef PFD
   set_error("pf_damage: unsupported element %d / nPg %d,%d", elem, nPgm, nPgr);
   return EFEA_ERR_ARG;
 }
@@ -617,19 +738,20 @@ int pf_elastic(int elem, int64_t Ne, const int* connect, const double* coord, co
                const double* u, const double* d, const double* E, int E_stride, double nu, int ps, int split,
                double thickness, double* Ke, hipStream_t s) {
   const PfParams P = make_params(split, 1, ps, nu, 1.0, thickness);
+  if (!pf_split_ok(elem, split, "pf_elastic")) return EFEA_ERR_ARG;
   if (Ne == 0) return EFEA_OK;
-  if (elem == ET_TRI3 && nPg == 1) {
-    const unsigned nb = (unsigned)((Ne + 27) / 28);
-    hipLaunchKernelGGL((k_pf_elastic<ET_TRI3, 1>), dim3(nb), dim3(256), 0, s, Ne, connect, coord, tab, u, d, E,
-                       E_stride, P, Ke);
-    return check_launch("pf_elastic");
+#define PFE(ET, G)                                                                                              \
+  if (elem == ET && nPg == G) {                                                                                 \
+    hipLaunchKernelGGL((k_pf_elastic<ET, G>), dim3(pf_blocks<ET>(Ne)), dim3(256), 0, s, Ne, connect, coord, tab, \
+                       u, d, E, E_stride, P, Ke);                                                               \
+    return check_launch("pf_elastic");                                                                          \
   }
-  if (elem == ET_QUAD4 && nPg == 4) {
-    const unsigned nb = (unsigned)((Ne + 15) / 16);
-    hipLaunchKernelGGL((k_pf_elastic<ET_QUAD4, 4>), dim3(nb), dim3(256), 0, s, Ne, connect, coord, tab, u, d, E,
-                       E_stride, P, Ke);
-    return check_launch("pf_elastic");
-  }
+  PFE(ET_TRI3, 1)
+  PFE(ET_QUAD4, 4)
+  PFE(ET_TETRA4, 1)
+  PFE(ET_TETRA10, 4)
+  PFE(ET_HEXA8, 8)
+#undef PFE
   set_error("pf_elastic: unsupported element %d / nPg %d", elem, nPg);
   return EFEA_ERR_ARG;
 }
@@ -638,18 +760,21 @@ int pf_strain_stress(int elem, int64_t Ne, const int* connect, const double* coo
                      const double* u, const double* d, const double* E, int E_stride, double nu, int ps, int split,
                      double thickness, double* out, hipStream_t s) {
   const PfParams P = make_params(split, 1, ps, nu, 1.0, thickness);
+  if (!pf_split_ok(elem, split, "pf_strain_stress")) return EFEA_ERR_ARG;
   if (Ne == 0) return EFEA_OK;
   const unsigned nb = (unsigned)((Ne + 255) / 256);
-  if (elem == ET_TRI3 && nPg == 1) {
-    hipLaunchKernelGGL((k_pf_strain_stress<ET_TRI3, 1>), dim3(nb), dim3(256), 0, s, Ne, connect, coord, tab, u, d,
-                       E, E_stride, P, out);
-    return check_launch("pf_strain_stress");
+#define PFS(ET, G)                                                                                              \
+  if (elem == ET && nPg == G) {                                                                                 \
+    hipLaunchKernelGGL((k_pf_strain_stress<ET, G>), dim3(nb), dim3(256), 0, s, Ne, connect, coord, tab, u, d, E, \
+                       E_stride, P, out);                                                                       \
+    return check_launch("pf_strain_stress");                                                                    \
   }
-  if (elem == ET_QUAD4 && nPg == 4) {
-    hipLaunchKernelGGL((k_pf_strain_stress<ET_QUAD4, 4>), dim3(nb), dim3(256), 0, s, Ne, connect, coord, tab, u, d,
-                       E, E_stride, P, out);
-    return check_launch("pf_strain_stress");
-  }
+  PFS(ET_TRI3, 1)
+  PFS(ET_QUAD4, 4)
+  PFS(ET_TETRA4, 1)
+  PFS(ET_TETRA10, 4)
+  PFS(ET_HEXA8, 8)
+#undef PFS
   set_error("pf_strain_stress: unsupported element %d / nPg %d", elem, nPg);
   return EFEA_ERR_ARG;
 }

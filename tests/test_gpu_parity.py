@@ -335,3 +335,101 @@ def test_solver_deterministic():
     simu._u.zero_()
     u2 = simu.Solve()
     assert np.array_equal(u1, u2)
+
+
+MAKERS_3D = {
+    "TETRA4": lambda: ea.mesh.tetra4_box(1.0, 1.0, 1.0, 4, 3, 3),
+    "TETRA10": lambda: ea.mesh.tetra10_from_tetra4(ea.mesh.tetra4_box(1.0, 1.0, 1.0, 2, 2, 2)),
+    "HEXA8": lambda: ea.hexa8_box(1.0, 1.0, 1.0, 4, 3, 3),
+}
+
+
+def _distorted3d(et, amp=0.03, seed=1):
+    m = MAKERS_3D[et]()
+    if et == "TETRA10":
+        return m
+    rng = np.random.default_rng(seed)
+    c = m.coord.copy()
+    inner = np.ones(m.Nn, bool)
+    for k in range(3):
+        inner &= (c[:, k] > c[:, k].min()) & (c[:, k] < c[:, k].max())
+    c[inner] += amp * (rng.random((inner.sum(), 3)) - 0.5)
+    return ea.Mesh({t: g.connect for t, g in m.dict_groupElem.items()}, c)
+
+
+@pytest.mark.parametrize("split", ["Amor", "Bourdin"])
+@pytest.mark.parametrize("et", list(MAKERS_3D))
+def test_phasefield_element_operators_3d(split, et):
+    mesh = _distorted3d(et)
+    rng = np.random.default_rng(7)
+    Nn, Ne = mesh.Nn, mesh.Ne
+    u = 1e-3 * rng.standard_normal(Nn * 3)
+    d = np.clip(rng.random(Nn), 0, 1)
+    E = 210000.0 * np.exp(0.1 * rng.standard_normal(Ne))
+    Gc = 2.7 * np.exp(0.15 * rng.standard_normal(Ne))
+    l0 = 0.05
+    print(f"{et} {split}: Ne {Ne}", flush=True)
+    pfo = O.PhaseFieldOracle(mesh.connect, mesh.coord, et, E, 0.3, False, 1.0, split, "AT2", Gc, l0)
+    pfo.u, pfo.d = u.copy(), d.copy()
+    nPgm = pfo.geo_m.nPg
+    H = np.abs(rng.standard_normal((Ne, nPgm))) * 10.0
+    pfo.old_psiP = H.copy()
+    mat = Models.Elastic.Isotropic(3, E, 0.3)
+    simu = Simulations.PhaseField(mesh, Models.PhaseField(mat, split, "AT2", Gc, l0))
+    simu._u, simu._d = to_dev(u), to_dev(d)
+    simu._H = to_dev(H.ravel())
+    pfo.assemble_damage()
+    simu._assemble_damage()
+    Kd = simu._Kd.to_scipy()
+    assert np.max(np.abs(Kd.data - pfo.Kd.data)) <= 1e-12 * np.max(np.abs(pfo.Kd.data))
+    assert _rel(simu._Fd.cpu().numpy(), pfo.Fd) < 1e-12
+    assert _rel(simu._psiP.cpu().numpy().reshape(Ne, nPgm), pfo.psiP) < 1e-12
+    pfo.assemble_elastic()
+    simu._assemble_elastic()
+    Ku = simu._Ku.to_scipy()
+    assert np.array_equal(Ku.indptr, pfo.Ku.indptr) and np.array_equal(Ku.indices, pfo.Ku.indices)
+    assert np.max(np.abs(Ku.data - pfo.Ku.data)) <= 1e-12 * np.max(np.abs(pfo.Ku.data))
+    # results: damaged stress and strain per element, von Mises, node values
+    geo = O.Geometry(et, mesh.connect, mesh.coord, "rigi", 3)
+    eps = O.strain_e_pg(geo, u, mesh.connect)
+    sig = O.pf_stress_e_pg(geo, u, d, mesh.connect, split, E, 0.3, False)
+    for name in ("Sxx", "Syz", "Sxy", "Svm", "Ezz", "Exz", "Evm"):
+        field = sig if name[0] == "S" else eps
+        ref_e = O.strain_or_stress_result_e(field, name)
+        assert _rel(simu.Result(name, nodeValues=False), ref_e) < 1e-12, name
+    assert simu.Result("Stress").shape == (Nn, 6)
+
+
+def test_staggered_3d_matches_oracle():
+    """Three tension steps of a HEXA8 bar (Amor, AT2) against the oracle."""
+    mesh = ea.hexa8_box(2.0, 1.0, 1.0, 8, 4, 4)
+    mat = Models.Elastic.Isotropic(3, 210000.0, 0.3)
+    simu = Simulations.PhaseField(mesh, Models.PhaseField(mat, "Amor", "AT2", 2.7, 0.1))
+    simu.solver = "hip_fp64"
+    ref = O.PhaseFieldOracle(mesh.connect, mesh.coord, "HEXA8", 210000.0, 0.3, False, 1.0, "Amor", "AT2", 2.7, 0.1)
+    left = mesh.Nodes_Conditions(lambda x, y, z: x == 0.0)
+    right = mesh.Nodes_Conditions(lambda x, y, z: x == 2.0)
+    errs = []
+    for it in range(1, 4):
+        for s in (simu, ref):
+            s.Bc_Init()
+            s.add_dirichlet(left, [0, 0, 0], ["x", "y", "z"])
+            s.add_dirichlet(right, [it * 4e-3], ["x"])
+        u, d, conv = simu.Solve(1.0, 500, 1)
+        ur, dr, _ = ref.Solve(1.0, 500, 1)
+        simu.Save_Iter()
+        ref.Save_Iter()
+        errs.append((it, _rel(u, ur), _rel(d, dr) if np.linalg.norm(dr) else np.linalg.norm(d)))
+    print("3D step errors (it, u, d):", errs, flush=True)
+    assert np.linalg.norm(dr) > 0
+    for it, eu, ed in errs:
+        assert eu < 1e-8 and ed < 1e-8, errs
+
+
+def test_3d_spectral_splits_are_refused():
+    mesh = ea.hexa8_box(1.0, 1.0, 1.0, 2, 2, 2)
+    mat = Models.Elastic.Isotropic(3, 210000.0, 0.3)
+    simu = Simulations.PhaseField(mesh, Models.PhaseField(mat, "Miehe", "AT2", 2.7, 0.1))
+    simu.add_dirichlet(mesh.Nodes_Conditions(lambda x, y, z: x == 0.0), [0, 0, 0], ["x", "y", "z"])
+    with pytest.raises(NotImplementedError, match="3D"):
+        simu.Solve()
