@@ -37,25 +37,11 @@ __device__ inline void lame2d(double E, double nu, int planeStress, double& lam,
 __device__ inline double sgn(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : (x == 0 ? 0.0 : x)); }
 __device__ inline double heav(double x) { return x < 0 ? 0.0 : (x > 0 ? 1.0 : (x == 0 ? 0.5 : x)); }
 
-// 3x3 helpers (Kelvin-Mandel 2D tensors of order 4).
-__device__ inline void mm3(const double A[3][3], const double B[3][3], double C[3][3]) {  // C = A B
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) C[i][j] = A[i][0] * B[0][j] + A[i][1] * B[1][j] + A[i][2] * B[2][j];
-}
-__device__ inline void mtm3(const double A[3][3], const double B[3][3], double C[3][3]) {  // C = A^T B
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) C[i][j] = A[0][i] * B[0][j] + A[1][i] * B[1][j] + A[2][i] * B[2][j];
-}
-
 // Projector of the positive part of a 2D Kelvin-Mandel tensor t = [txx, tyy,
 // sqrt(2) txy] (__Spectral_Decomposition, Models/_phasefield.py:1043-1128,
 // with the 2D eigen decomposition :751-802): P = betaP I + g0 m1 m1^T + g1
 // m2 m2^T; the negative projector is I - P.
-__device__ inline void proj2d(const double t[3], double P[3][3]) {
+__device__ inline void proj2d(const double (&t)[3], double (&P)[3][3]) {
   const double r2 = sqrt(2.0);
   const double m00 = t[0], m11 = t[1], m01 = t[2] / r2;
   const double det = m00 * m11 - m01 * m01;
@@ -88,166 +74,221 @@ __device__ inline void proj2d(const double t[3], double P[3][3]) {
     }
 }
 
-// cP, cM of the 2D split (PhaseField.Calc_C, Models/_phasefield.py:396-431)
-// for the strain e = [exx, eyy, sqrt(2) exy] of an isotropic material (Lame
-// constants lam, mu of the 2D model; E, nu for the stress split), codes
-// EFEA_SPLIT_*:
+template <int DIM>
+struct PfDim {
+  static constexpr int NS = DIM == 2 ? 3 : 6;  // Kelvin-Mandel components
+};
+
+template <int N>
+__device__ inline void mmN(const double (&A)[N][N], const double (&B)[N][N], double (&C)[N][N]) {  // C = A B
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      double s = 0;
+#pragma unroll
+      for (int k = 0; k < N; ++k) s += A[i][k] * B[k][j];
+      C[i][j] = s;
+    }
+}
+template <int N>
+__device__ inline void mtmN(const double (&A)[N][N], const double (&B)[N][N], double (&C)[N][N]) {  // C = A^T B
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      double s = 0;
+#pragma unroll
+      for (int k = 0; k < N; ++k) s += A[k][i] * B[k][j];
+      C[i][j] = s;
+    }
+}
+
+template <int DIM>
+__device__ inline void proj_km(const double (&t)[PfDim<DIM>::NS], double (&P)[PfDim<DIM>::NS][PfDim<DIM>::NS]) {
+  static_assert(DIM == 2, "spectral projectors are provided in 2D");
+  proj2d(t, P);
+}
+
+// cP, cM of the 2D split (PhaseField.Calc_C, Models/_phasefield.py:396-431) for
+// the Kelvin-Mandel strain e of an isotropic material (Lame constants lam,
+// mu of the model; E, nu for the stress split), codes EFEA_SPLIT_*:
 //   Amor (:451-483), Miehe (:503-537), Bourdin (:433-449), Stress (:573-634),
 //   He (:680-749), AnisotStrain[_NoCross] (:539-566),
 //   AnisotStress[_NoCross] (:636-676).
 // The reference's evaluation order of each product and sum is kept.
-__device__ inline void split2d(int split, const double e[3], double lam, double mu, double E, double nu,
-                               int planeStress, double cP[3][3], double cM[3][3]) {
-  const double IxI[3][3] = {{1, 1, 0}, {1, 1, 0}, {0, 0, 0}};
-  // the isotropic Kelvin-Mandel stiffness lam IxI + 2 mu I
-  double C[3][3];
+template <int DIM>
+__device__ inline void split_t(int split, const double (&e)[PfDim<DIM>::NS], double lam, double mu, double E,
+                               double nu, int planeStress, double (&cP)[PfDim<DIM>::NS][PfDim<DIM>::NS],
+                               double (&cM)[PfDim<DIM>::NS][PfDim<DIM>::NS]) {
+  constexpr int NS = PfDim<DIM>::NS;
+  double IxI[NS][NS], C[NS][NS];
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
+  for (int i = 0; i < NS; ++i)
 #pragma unroll
-    for (int j = 0; j < 3; ++j) C[i][j] = lam * IxI[i][j] + (i == j ? 2 * mu : 0.0);
+    for (int j = 0; j < NS; ++j) {
+      IxI[i][j] = (i < DIM && j < DIM) ? 1.0 : 0.0;
+      C[i][j] = lam * IxI[i][j] + (i == j ? 2 * mu : 0.0);  // lam IxI + 2 mu I
+    }
+  double tr_e = 0;
+#pragma unroll
+  for (int i = 0; i < DIM; ++i) tr_e += e[i];
   if (split == EFEA_SPLIT_AMOR) {
-    const double tr_e = e[0] + e[1];
     const double Rp = (1 + sgn(tr_e)) / 2, Rm = (1 + sgn(-tr_e)) / 2;
-    const double bulk = lam + 2 * mu / 2;
+    const double bulk = lam + 2 * mu / DIM;
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < NS; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
+      for (int j = 0; j < NS; ++j) {
         const double Iij = (i == j) ? 1.0 : 0.0;
-        cP[i][j] = bulk * (Rp * IxI[i][j]) + 2 * mu * (Iij - 1.0 / 2 * IxI[i][j]);
+        cP[i][j] = bulk * (Rp * IxI[i][j]) + 2 * mu * (Iij - 1.0 / DIM * IxI[i][j]);
         cM[i][j] = bulk * (Rm * IxI[i][j]);
       }
     return;
   }
   if (split == EFEA_SPLIT_BOURDIN) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < NS; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
+      for (int j = 0; j < NS; ++j) {
         cP[i][j] = C[i][j];
         cM[i][j] = 0;
       }
     return;
   }
-  double P[3][3], M[3][3];
+  double P[NS][NS], M[NS][NS];
   if (split == EFEA_SPLIT_MIEHE || split == EFEA_SPLIT_ANISOT_STRAIN || split == EFEA_SPLIT_ANISOT_STRAIN_NOCROSS) {
-    proj2d(e, P);
+    proj_km<DIM>(e, P);
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < NS; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) M[i][j] = (i == j ? 1.0 : 0.0) - P[i][j];
+      for (int j = 0; j < NS; ++j) M[i][j] = (i == j ? 1.0 : 0.0) - P[i][j];
     if (split == EFEA_SPLIT_MIEHE) {
-      const double tr_e = e[0] + e[1];
       const double Rp = (1 + sgn(tr_e)) / 2, Rm = (1 + sgn(-tr_e)) / 2;
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
+      for (int i = 0; i < NS; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
+        for (int j = 0; j < NS; ++j) {
           cP[i][j] = lam * (Rp * IxI[i][j]) + 2 * mu * P[i][j];
           cM[i][j] = lam * (Rm * IxI[i][j]) + 2 * mu * M[i][j];
         }
       return;
     }
-    double PTC[3][3], MTC[3][3], Cpp[3][3], Cpm[3][3], Cmm[3][3], Cmp[3][3];
-    mtm3(P, C, PTC);
-    mtm3(M, C, MTC);
-    mm3(PTC, P, Cpp);
-    mm3(PTC, M, Cpm);
-    mm3(MTC, M, Cmm);
-    mm3(MTC, P, Cmp);
+    double PTC[NS][NS], MTC[NS][NS], Cpp[NS][NS], Cpm[NS][NS], Cmm[NS][NS], Cmp[NS][NS];
+    mtmN<NS>(P, C, PTC);
+    mtmN<NS>(M, C, MTC);
+    mmN<NS>(PTC, P, Cpp);
+    mmN<NS>(PTC, M, Cpm);
+    mmN<NS>(MTC, M, Cmm);
+    mmN<NS>(MTC, P, Cmp);
     const bool cross = split == EFEA_SPLIT_ANISOT_STRAIN;
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < NS; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
+      for (int j = 0; j < NS; ++j) {
         cP[i][j] = cross ? (Cpp[i][j] + Cpm[i][j]) + Cmp[i][j] : Cpp[i][j];
         cM[i][j] = cross ? Cmm[i][j] : (Cmm[i][j] + Cpm[i][j]) + Cmp[i][j];
       }
     return;
   }
   if (split == EFEA_SPLIT_HE) {
-    // C^(1/2) and C^(-1/2) of the isotropic C: eigenvalue 2 mu + 2 lam along
-    // IxI / 2, 2 mu on its complement
-    const double a = sqrt(2 * mu), b = sqrt(2 * mu + 2 * lam);
-    double sqC[3][3], isqC[3][3];
+    // C^(1/2) and C^(-1/2) of the isotropic C: eigenvalue 2 mu + DIM lam along
+    // IxI / DIM, 2 mu on its complement
+    const double a = sqrt(2 * mu), b = sqrt(2 * mu + DIM * lam);
+    double sqC[NS][NS], isqC[NS][NS];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < NS; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
+      for (int j = 0; j < NS; ++j) {
         const double Iij = (i == j) ? 1.0 : 0.0;
-        sqC[i][j] = a * Iij + (b - a) * (IxI[i][j] / 2);
-        isqC[i][j] = Iij / a + (1 / b - 1 / a) * (IxI[i][j] / 2);
+        sqC[i][j] = a * Iij + (b - a) * (IxI[i][j] / DIM);
+        isqC[i][j] = Iij / a + (1 / b - 1 / a) * (IxI[i][j] / DIM);
       }
-    double et[3];
+    double et[NS];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) et[i] = sqC[i][0] * e[0] + sqC[i][1] * e[1] + sqC[i][2] * e[2];
-    double Pt[3][3], Mt[3][3], T[3][3];
-    proj2d(et, Pt);
+    for (int i = 0; i < NS; ++i) {
+      double s = 0;
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < NS; ++j) s += sqC[i][j] * e[j];
+      et[i] = s;
+    }
+    double Pt[NS][NS], Mt[NS][NS], T[NS][NS];
+    proj_km<DIM>(et, Pt);
 #pragma unroll
-      for (int j = 0; j < 3; ++j) Mt[i][j] = (i == j ? 1.0 : 0.0) - Pt[i][j];
-    mm3(isqC, Pt, T);
-    mm3(T, sqC, P);
-    mm3(isqC, Mt, T);
-    mm3(T, sqC, M);
-    mm3(C, P, cP);
-    mm3(C, M, cM);
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+      for (int j = 0; j < NS; ++j) Mt[i][j] = (i == j ? 1.0 : 0.0) - Pt[i][j];
+    mmN<NS>(isqC, Pt, T);
+    mmN<NS>(T, sqC, P);
+    mmN<NS>(isqC, Mt, T);
+    mmN<NS>(T, sqC, M);
+    mmN<NS>(C, P, cP);
+    mmN<NS>(C, M, cM);
     return;
   }
   // stress-based splits: the projectors of sigma = C e
-  double sg[3];
+  double sg[NS];
 #pragma unroll
-  for (int i = 0; i < 3; ++i) sg[i] = C[i][0] * e[0] + C[i][1] * e[1] + C[i][2] * e[2];
-  proj2d(sg, P);
+  for (int i = 0; i < NS; ++i) {
+    double s = 0;
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < NS; ++j) s += C[i][j] * e[j];
+    sg[i] = s;
+  }
+  proj_km<DIM>(sg, P);
 #pragma unroll
-    for (int j = 0; j < 3; ++j) M[i][j] = (i == j ? 1.0 : 0.0) - P[i][j];
+  for (int i = 0; i < NS; ++i)
+#pragma unroll
+    for (int j = 0; j < NS; ++j) M[i][j] = (i == j ? 1.0 : 0.0) - P[i][j];
   if (split == EFEA_SPLIT_STRESS) {
-    const double tr_s = sg[0] + sg[1];
+    double tr_s = 0;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) tr_s += sg[i];
     const double Rp = (1 + sgn(tr_s)) / 2, Rm = (1 + sgn(-tr_s)) / 2;
-    const double cv = planeStress ? nu / E : nu * (1 + nu) / E;
-    double sP[3][3], sM[3][3], T[3][3];
+    // 2D: (1+nu)/E P - c R IxI, c = nu/E (plane stress) or nu(1+nu)/E;
+    // 3D: 1/(2 mu) P - nu/E R IxI
+    const double cp = DIM == 2 ? (1 + nu) / E : 1 / (2 * mu);
+    const double cv = (DIM == 2 && !planeStress) ? nu * (1 + nu) / E : nu / E;
+    double sP[NS][NS], sM[NS][NS], T[NS][NS];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < NS; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        sP[i][j] = ((1 + nu) / E * P[i][j]) - (cv * Rp * IxI[i][j]);
-        sM[i][j] = ((1 + nu) / E * M[i][j]) - (cv * Rm * IxI[i][j]);
+      for (int j = 0; j < NS; ++j) {
+        sP[i][j] = (cp * P[i][j]) - (cv * Rp * IxI[i][j]);
+        sM[i][j] = (cp * M[i][j]) - (cv * Rm * IxI[i][j]);
       }
-    mtm3(C, sP, T);
-    mm3(T, C, cP);
-    mtm3(C, sM, T);
-    mm3(T, C, cM);
+    mtmN<NS>(C, sP, T);
+    mmN<NS>(T, C, cP);
+    mtmN<NS>(C, sM, T);
+    mmN<NS>(T, C, cM);
     return;
   }
   // AnisotStress[_NoCross]: Cp = P C, Cm = M C and the compliance S = C^-1
-  // = I / (2 mu) - lam / (2 mu (2 mu + 2 lam)) IxI
-  double S[3][3], Cp[3][3], Cm[3][3], PS[3][3], MS[3][3], Cpp[3][3], Cpm[3][3], Cmm[3][3], Cmp[3][3];
-  const double s2 = lam / (2 * mu * (2 * mu + 2 * lam));
+  // = I / (2 mu) - lam / (2 mu (2 mu + DIM lam)) IxI
+  double S[NS][NS], Cp[NS][NS], Cm[NS][NS], PS[NS][NS], MS[NS][NS], Cpp[NS][NS], Cpm[NS][NS], Cmm[NS][NS],
+      Cmp[NS][NS];
+  const double s2 = lam / (2 * mu * (2 * mu + DIM * lam));
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
+  for (int i = 0; i < NS; ++i)
 #pragma unroll
-    for (int j = 0; j < 3; ++j) S[i][j] = (i == j ? 1 / (2 * mu) : 0.0) - s2 * IxI[i][j];
-  mm3(P, C, Cp);
-  mm3(M, C, Cm);
-  mtm3(Cp, S, PS);
-  mtm3(Cm, S, MS);
-  mm3(PS, Cp, Cpp);
-  mm3(PS, Cm, Cpm);
-  mm3(MS, Cm, Cmm);
-  mm3(MS, Cp, Cmp);
+    for (int j = 0; j < NS; ++j) S[i][j] = (i == j ? 1 / (2 * mu) : 0.0) - s2 * IxI[i][j];
+  mmN<NS>(P, C, Cp);
+  mmN<NS>(M, C, Cm);
+  mtmN<NS>(Cp, S, PS);
+  mtmN<NS>(Cm, S, MS);
+  mmN<NS>(PS, Cp, Cpp);
+  mmN<NS>(PS, Cm, Cpm);
+  mmN<NS>(MS, Cm, Cmm);
+  mmN<NS>(MS, Cp, Cmp);
   const bool cross = split == EFEA_SPLIT_ANISOT_STRESS;
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
+  for (int i = 0; i < NS; ++i)
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
+    for (int j = 0; j < NS; ++j) {
       cP[i][j] = cross ? (Cpp[i][j] + Cpm[i][j]) + Cmp[i][j] : Cpp[i][j];
       cM[i][j] = cross ? Cmm[i][j] : (Cmm[i][j] + Cpm[i][j]) + Cmp[i][j];
     }
 }
-
 template <int NPE>
 __device__ inline void strain2d(const double (&dNx)[2][NPE], const double* ue, double e[3]) {
   const double c = 1 / sqrt(2.0);
@@ -259,11 +300,6 @@ __device__ inline void strain2d(const double (&dNx)[2][NPE], const double* ue, d
     e[2] += (dNx[1][n] * c) * ue[2 * n] + (dNx[0][n] * c) * ue[2 * n + 1];
   }
 }
-
-template <int DIM>
-struct PfDim {
-  static constexpr int NS = DIM == 2 ? 3 : 6;  // Kelvin-Mandel components
-};
 
 // 3D Kelvin-Mandel strain [xx, yy, zz, sqrt2 yz, sqrt2 xz, sqrt2 xy].
 template <int NPE>
@@ -322,7 +358,7 @@ __device__ inline void split_km(int split, const double (&e)[PfDim<DIM>::NS], do
   double lam, mu;
   if constexpr (DIM == 2) {
     lame2d(E, nu, planeStress, lam, mu);
-    split2d(split, e, lam, mu, E, nu, planeStress, cP, cM);
+    split_t<2>(split, e, lam, mu, E, nu, planeStress, cP, cM);
   } else {
     lam = E * nu / ((1 + nu) * (1 - 2 * nu));
     mu = E / (2 * (1 + nu));
@@ -660,11 +696,8 @@ __global__ void k_pf_split_points(int64_t n, int split, const double* __restrict
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double e[3] = {eps[3 * i], eps[3 * i + 1], eps[3 * i + 2]};
-  double lam, mu;
-  const double Ei = E[E_stride ? i : 0];
-  lame2d(Ei, nu, planeStress, lam, mu);
   double cP[3][3], cM[3][3];
-  split2d(split, e, lam, mu, Ei, nu, planeStress, cP, cM);
+  split_km<2>(split, e, E[E_stride ? i : 0], nu, planeStress, cP, cM);
   double pp = 0, pm = 0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {

@@ -433,3 +433,32 @@ def test_3d_spectral_splits_are_refused():
     simu.add_dirichlet(mesh.Nodes_Conditions(lambda x, y, z: x == 0.0), [0, 0, 0], ["x", "y", "z"])
     with pytest.raises(NotImplementedError, match="3D"):
         simu.Solve()
+
+
+@pytest.mark.skip(reason="3D spectral-split kernels withdrawn: the GPU run of the 3D phase-field tests with them "
+                         "compiled in did not finish (silent for 180 s, no cause found); the split code itself "
+                         "terminates and satisfies c+ + c- = C on the host")
+def test_staggered_3d_miehe_matches_oracle():
+    """Two tension steps of a TETRA4 bar with the 3D spectral (Miehe) split."""
+    mesh = ea.mesh.tetra4_box(2.0, 1.0, 1.0, 6, 3, 3)
+    mat = Models.Elastic.Isotropic(3, 210000.0, 0.3)
+    simu = Simulations.PhaseField(mesh, Models.PhaseField(mat, "Miehe", "AT2", 2.7, 0.1))
+    simu.solver = "hip_fp64"
+    ref = O.PhaseFieldOracle(mesh.connect, mesh.coord, "TETRA4", 210000.0, 0.3, False, 1.0, "Miehe", "AT2", 2.7,
+                             0.1)
+    left = mesh.Nodes_Conditions(lambda x, y, z: x == 0.0)
+    right = mesh.Nodes_Conditions(lambda x, y, z: x == 2.0)
+    for it in range(1, 3):
+        for s in (simu, ref):
+            s.Bc_Init()
+            s.add_dirichlet(left, [0, 0, 0], ["x", "y", "z"])
+            s.add_dirichlet(right, [it * 4e-3, 1e-3], ["x", "y"])
+        u, d, _ = simu.Solve(1.0, 500, 1)
+        ur, dr, _ = ref.Solve(1.0, 500, 1)
+        simu.Save_Iter()
+        ref.Save_Iter()
+        print(f"3D Miehe step {it}: u {_rel(u, ur):.2e}, d {_rel(d, dr) if np.linalg.norm(dr) else 0.0:.2e}",
+              flush=True)
+        assert _rel(u, ur) < 1e-8
+        if np.linalg.norm(dr):
+            assert _rel(d, dr) < 1e-8
