@@ -387,6 +387,51 @@ class _Simu:
             vals, dofs, used = self._integrate_boundary(2, nodes, values, unknowns, problemType)
         self._Bc_Neumann.append(_Bc(problemType, used, dofs, vals, unknowns, f"Neumann {description}"))
 
+    def add_volumeLoad(self, nodes, values: list, unknowns: list, problemType=None, description="") -> None:
+        """Body load integrated over the main elements whose every node is in
+        ``nodes``, times the thickness in 2D (``_simu.py:2683-2741``).
+        Values are constants, callables of ``(x, y, z)`` at the Gauss points
+        or arrays of nodal values."""
+        if len(nodes) == 0 or len(values) == 0:
+            return
+        assert len(values) == len(unknowns), f"len(values)={len(values)} must equal len(unknowns)={len(unknowns)}."
+        problemType = problemType or self.problemType
+        self._check_problemType(problemType)
+        if self.mesh.dim not in (2, 3):
+            raise NotImplementedError("Unknown configuration.")
+        vals, dofs, used = self._integrate_boundary(self.mesh.dim, nodes, values, unknowns, problemType)
+        if self.mesh.dim == 2:
+            vals = vals * self.model.thickness
+        self._Bc_Neumann.append(_Bc(problemType, used, dofs, vals, unknowns, f"Neumann {description}"))
+
+    def add_pressureLoad(self, nodes, magnitude: float, problemType=None, description="") -> None:
+        """Pressure ``magnitude`` along the nodal normals of the boundary
+        elements whose every node is in ``nodes`` (``_simu.py:2632-2681,
+        2910-2944``): the normals of ``Mesh.Get_normals`` times the magnitude
+        (times the thickness in 2D) integrated as nodal values.  The sign
+        follows the normals, hence the node order of the boundary elements.
+        Refused on 1D meshes and scalar problems, as the reference does."""
+        if len(nodes) == 0 or magnitude == 0:
+            return
+        problemType = problemType or self.problemType
+        self._check_problemType(problemType)
+        if self.mesh.dim == 1:
+            print("Cant apply pressure on 1D mesh.")
+            return
+        if self.Get_dof_n(problemType) == 1:
+            print("Cant apply pressure on scalar problems.")
+            return
+        if self.mesh.dim == 2:
+            magnitude = magnitude * self.model.thickness
+        normals, used = self.mesh.Get_normals(np.asarray(nodes, dtype=np.int64))
+        if used.size == 0:
+            return
+        inDim = self.mesh.inDim
+        values = [normals[:, d] * magnitude for d in range(inDim)]
+        unknowns = self.Get_unknowns(problemType)[:inDim]
+        vals, dofs, used = self._integrate_boundary(self.mesh.dim - 1, used, values, unknowns, problemType)
+        self._Bc_Neumann.append(_Bc(problemType, used, dofs, vals, unknowns, f"Neumann {description}"))
+
     def _bc_arrays(self, store, problemType):
         bcs = [bc for bc in store if bc.problemType == problemType]
         if not bcs:

@@ -189,10 +189,9 @@ k_reaction_diffusion(int64_t Ne, const int* __restrict__ connect, const double* 
   for (int64_t k = t; k < nval; k += 256) out[k] = s_K[k];
 }
 
-// Surface/line load integral sum_p wJ f_p N_a on boundary elements of a
-// lower dimension (EasyFEA/Simulations/_simu.py:2770-2842).  Boundary
-// coordinates arrive already expressed in each element's own frame
-// (Ne, nPe, dim) so that the Jacobian is square.
+// Load integral sum_p wJ f_p N_a (EasyFEA/Simulations/_simu.py:2770-2842)
+// on line, surface or volume elements.  Coordinates arrive already expressed
+// in each element's own frame (Ne, nPe, dim) so that the Jacobian is square.
 template <int NPE, int DIM, int NPG>
 __global__ void k_boundary_load(int64_t Ne, const double* __restrict__ Xloc, const double* __restrict__ tab,
                                 const double* __restrict__ f, double* __restrict__ out) {
@@ -307,6 +306,19 @@ int boundary_load(int elem, int64_t Ne, const double* Xloc, const double* tab, i
   }
   if (elem == ET_TRI3 && nPg == 3) {
     hipLaunchKernelGGL((k_boundary_load<3, 2, 3>), dim3((unsigned)nb), dim3(128), 0, s, Ne, Xloc, tab, f, out);
+    return check_launch("boundary_load");
+  }
+  // body loads of 3D meshes integrate over the volume elements themselves
+  if (elem == ET_TETRA4 && nPg == 4) {
+    hipLaunchKernelGGL((k_boundary_load<4, 3, 4>), dim3((unsigned)nb), dim3(128), 0, s, Ne, Xloc, tab, f, out);
+    return check_launch("boundary_load");
+  }
+  if (elem == ET_TETRA10 && nPg == 4) {
+    hipLaunchKernelGGL((k_boundary_load<10, 3, 4>), dim3((unsigned)nb), dim3(128), 0, s, Ne, Xloc, tab, f, out);
+    return check_launch("boundary_load");
+  }
+  if (elem == ET_HEXA8 && nPg == 8) {
+    hipLaunchKernelGGL((k_boundary_load<8, 3, 8>), dim3((unsigned)nb), dim3(128), 0, s, Ne, Xloc, tab, f, out);
     return check_launch("boundary_load");
   }
   set_error("boundary_load: unsupported element %d / nPg %d", elem, nPg);

@@ -38,7 +38,7 @@ def ptr(t: Optional[torch.Tensor]) -> Optional[int]:
 
 
 def to_dev(a: np.ndarray, dtype=None) -> torch.Tensor:
-    t = torch.from_numpy(np.ascontiguousarray(a))
+    t = torch.from_numpy(np.require(a, requirements=["C", "W"]))  # read-only views (broadcasts) are copied
     if dtype is not None:
         t = t.to(dtype)
     return t.to("cuda", non_blocking=False)

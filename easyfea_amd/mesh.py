@@ -122,6 +122,56 @@ class Mesh:
     def connect(self) -> np.ndarray:
         return self.groupElem.connect
 
+    @property
+    def inDim(self) -> int:
+        """Dimension of the space the mesh lies in: 3 if any z, else 2 if any
+        y, else 1 (``FEM/_group_elem.py:155-175``)."""
+        if self.dim == 3:
+            return 3
+        _, y, z = np.abs(self.coord.T)
+        return 3 if z.max() > 0 else (2 if y.max() > 0 else 1)
+
+    def Get_normals(self, nodes: Optional[np.ndarray] = None) -> tuple:
+        """Unit normals at the boundary nodes (``FEM/_mesh.py:557-615``).
+
+        For each group of dimension ``dim - 1``: the unit normal at every
+        mass-rule point (``e_z x dx/dr`` on lines, ``dx/dr x dx/ds`` on faces,
+        ``FEM/_group_elem.py:497-539``) summed over the element, averaged
+        over the elements around each node whose every node is in ``nodes``,
+        then normalised.  Which side the normal points to follows the node
+        order of the boundary elements.  Returns ``(normals (n, 3), nodes)``,
+        the nodes of each group sorted."""
+        from ._utils import MatrixType
+        from .quadrature import dN_pg
+
+        nodes = self.nodes if nodes is None else np.asarray(nodes, dtype=np.int64)
+        normals, used_all = [], []
+        for g in self.Get_list_groupElem(self.dim - 1):
+            elements = g.Get_Elements_Nodes(nodes, exclusively=True)
+            if elements.size == 0:
+                continue
+            connect = g.connect[elements]
+            dN = dN_pg(g.elemType, MatrixType.mass)  # (nPg, dim, nPe)
+            X = self.coord[connect]
+            dxdr = np.einsum("pn,end->epd", dN[:, 0], X)
+            if g.dim == 1:
+                n_ep = np.cross(np.array([0.0, 0.0, 1.0]), dxdr)
+            else:
+                n_ep = np.cross(dxdr, np.einsum("pn,end->epd", dN[:, 1], X))
+            n_e = (n_ep / np.linalg.norm(n_ep, axis=2, keepdims=True)).sum(axis=1)
+            used = np.unique(connect)
+            acc = np.zeros((self.Nn, 3))
+            cnt = np.zeros(self.Nn)
+            for a in range(connect.shape[1]):
+                np.add.at(acc, connect[:, a], n_e)
+                np.add.at(cnt, connect[:, a], 1.0)
+            n = acc[used] / cnt[used, None]
+            normals.append(n / np.linalg.norm(n, axis=1, keepdims=True))
+            used_all.append(used)
+        if not normals:
+            return np.zeros((0, 3)), np.array([], dtype=np.int64)
+        return np.concatenate(normals), np.concatenate(used_all)
+
     def Get_list_groupElem(self, dim: Optional[int] = None) -> list:
         if dim is None:
             dim = self.dim

@@ -78,8 +78,9 @@ int efea_reaction_diffusion_ke(int elem_type, int precision, int64_t Ne, const i
                                const double* k, int k_stride, const double* f, double scale, void* Ke, void* Fe,
                                void* stream);
 
-/* Boundary load integral out[e, a] = sum_p wJ f[e, p] N_a (SEG2 nPg 2, QUAD4/TRI3 faces)
- * on coordinates given in each element's own frame X_local (Ne, nPe, dim).
+/* Load integral out[e, a] = sum_p wJ f[e, p] N_a on coordinates given in each element's own
+ * frame X_local (Ne, nPe, dim): line and surface loads on SEG2 (nPg 2), TRI3 (3), QUAD4 (4);
+ * body loads (add_volumeLoad, _simu.py:2683-2741) also on TETRA4 (4), TETRA10 (4), HEXA8 (8).
  * Replaces the integration of _Simu.__Bc_Integration_Dim (Simulations/_simu.py:2770-2842). */
 int efea_boundary_load(int elem_type, int64_t Ne, const double* X_local, const double* tables, int nPg,
                        const double* f, double* out, void* stream);

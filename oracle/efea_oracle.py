@@ -428,8 +428,10 @@ def dofs_nodes(nodes, unknowns, available) -> np.ndarray:
 
 
 def integrate_load(elemType, connect_b, coord, inDim, nodes, values, unknowns, available):
-    """``_Simu.__Bc_Integration_Dim`` (``_simu.py:2770-2842``) for constant or
-    callable values: ``sum_p wJ * f * N`` per element node, mass points."""
+    """``_Simu.__Bc_Integration_Dim`` (``_simu.py:2770-2842``): ``sum_p wJ *
+    f * N`` per element node, mass points.  A constant or callable value is
+    evaluated at the Gauss points; an array holds one value per node of
+    ``nodes`` and is interpolated with ``N`` (``:2818-2826``)."""
     elements = elements_using_nodes(connect_b, nodes)
     if elements.size == 0:
         return np.array([]), np.array([], dtype=int)
@@ -442,12 +444,47 @@ def integrate_load(elemType, connect_b, coord, inDim, nodes, values, unknowns, a
         val = values[u]
         if callable(val):
             ev = val(coord_e_p[..., 0], coord_e_p[..., 1], coord_e_p[..., 2]) * np.ones(geo.wJ.shape)
-        else:
+        elif np.ndim(val) == 0:
             ev = np.full(geo.wJ.shape, float(val))
+        else:
+            ev_n = np.zeros(coord.shape[0])
+            ev_n[np.asarray(nodes)] = val
+            ev = np.einsum("pn,en->ep", geo.N_pg, ev_n[connect])
         v_e_p = np.einsum("ep,ep,pn->epn", geo.wJ, ev, geo.N_pg)
         vals[:, u] = v_e_p.sum(axis=1).ravel()
         dofs[:, u] = dofs_nodes(connect.ravel(), [unk], available)
     return vals.ravel(), dofs.ravel()
+
+
+def node_normals(elemType, connect_b, coord, nodes):
+    """``Mesh.Get_normals`` (``FEM/_mesh.py:557-615``) over one boundary
+    group: unit normals at the mass points (``Get_normals_e_pg``,
+    ``FEM/_group_elem.py:497-539``: ``e_z x dx/dr`` on lines, ``dx/dr x
+    dx/ds`` on faces), summed per element, averaged over the selected
+    elements around each node and normalised.  Returns ``(normals (n, 3),
+    nodes)`` with the nodes sorted."""
+    elements = elements_using_nodes(connect_b, nodes)
+    if elements.size == 0:
+        return np.zeros((0, 3)), np.array([], dtype=int)
+    connect = connect_b[elements]
+    pts, _ = gauss(elemType, "mass")
+    _, dN = shape_functions(elemType, pts)
+    X = coord[connect][..., :3]
+    dxdr = np.einsum("pn,end->epd", dN[:, 0], X)
+    if ELEM_DIM[elemType] == 1:
+        n_ep = np.cross(np.array([0.0, 0.0, 1.0]), dxdr)
+    else:
+        n_ep = np.cross(dxdr, np.einsum("pn,end->epd", dN[:, 1], X))
+    n_ep = n_ep / np.linalg.norm(n_ep, axis=2, keepdims=True)
+    n_e = n_ep.sum(axis=1)
+    used = np.unique(connect)
+    acc = np.zeros((coord.shape[0], 3))
+    cnt = np.zeros(coord.shape[0])
+    for a in range(connect.shape[1]):
+        np.add.at(acc, connect[:, a], n_e)
+        np.add.at(cnt, connect[:, a], 1.0)
+    n = acc[used] / cnt[used, None]
+    return n / np.linalg.norm(n, axis=1, keepdims=True), used
 
 
 def known_unknown(dirichlet_dofs: np.ndarray, Ndof: int):
@@ -972,6 +1009,34 @@ class _SimuBase:
                 vals = vals * thickness
             self._add(self.neu, problemType, dofs, vals)
 
+    def add_volumeLoad(self, nodes, values, unknowns, problemType="elastic", thickness=1.0):
+        """``add_volumeLoad`` (``_simu.py:2683-2741``): integral over the
+        main elements, times the thickness in 2D."""
+        avail = self.unknowns(problemType)
+        vals, dofs = integrate_load(self.elemType, self.connect, self.coord, self.dim, nodes, values, unknowns,
+                                    avail)
+        if self.dim == 2:
+            vals = vals * thickness
+        self._add(self.neu, problemType, dofs, vals)
+
+    def add_pressureLoad(self, nodes, magnitude, problemType="elastic", thickness=1.0):
+        """``add_pressureLoad`` (``_simu.py:2632-2681, 2910-2944``): nodal
+        normals times ``magnitude`` (times the thickness in 2D), integrated
+        as nodal values over the boundary elements."""
+        avail = self.unknowns(problemType)
+        inDim = 2 if self.dim == 2 else 3
+        if self.dim == 2:
+            magnitude = magnitude * thickness
+        for et, cb in self.boundary.items():
+            if ELEM_DIM[et] != self.dim - 1:
+                continue
+            normals, used = node_normals(et, cb, self.coord, np.asarray(nodes))
+            if used.size == 0:
+                continue
+            values = [normals[:, d] * magnitude for d in range(inDim)]
+            vals, dofs = integrate_load(et, cb, self.coord, inDim, used, values, avail[:inDim], avail)
+            self._add(self.neu, problemType, dofs, vals)
+
     def unknowns(self, problemType):
         return ["x", "y"] if self.dim == 2 else ["x", "y", "z"]
 
@@ -983,7 +1048,7 @@ class _SimuBase:
 
 class ElasticOracle(_SimuBase):
     """``Simulations.Elastic`` static solve (``Simulations/_elastic.py:123-152``
-    and the r1 pipeline of section 3.1)."""
+    and the static pipeline: assembly, Dirichlet reduction, direct solve)."""
 
     def __init__(self, connect, coord, elemType, E, v, planeStress=True, thickness=1.0, boundary=None):
         dim = ELEM_DIM[elemType]
