@@ -597,6 +597,15 @@ class _Simu:
             return None
         return self._results[iter]
 
+    def Save(self, folder: str, filename: str = "simulation", additionalInfos: str = "") -> str:
+        """Writes the mesh, the simulation (model, solver settings, saved
+        iterations, current fields) and a summary to ``folder``
+        (``_simu.py:3157-3214``) as ``.npz`` data; ``Load_Simu`` reads it."""
+        from .sim_io import save_simu
+
+        self.folder = folder
+        return save_simu(self, folder, filename, additionalInfos)
+
     def Need_Update(self, value: bool = True) -> None:
         raise NotImplementedError
 
@@ -1421,3 +1430,10 @@ class PhaseField(_Simu):
                 values = e[:, ns + 1 + comps.index(result[1:])]
             values = values.contiguous()
         return self._reshape_values(values, nodeValues)
+
+
+def Load_Simu(folder: str, filename: str = "simulation") -> _Simu:
+    """The simulation written by ``_Simu.Save`` (``_simu.py:3333-3366``)."""
+    from .sim_io import load_simu
+
+    return load_simu(folder, filename)

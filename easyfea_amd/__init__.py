@@ -12,6 +12,7 @@ from ._utils import ElemType, MatrixType  # noqa: F401
 from .mesh import Mesh, hexa8_box, hexa8_cube, quad4_rectangle, tri3_notched_square, tri3_rectangle  # noqa: F401
 from .mesh_io import load_mesh as Load_Mesh  # noqa: F401
 from .mesh_io import read_msh  # noqa: F401
+from .Simulations import Load_Simu  # noqa: F401
 from .Solvers import AlgoType, SolverType  # noqa: F401
 
 __version__ = "0.1.0"

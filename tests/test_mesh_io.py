@@ -130,3 +130,26 @@ def test_binary_and_unknown_files_are_refused(tmp_path):
         read_msh(_write(tmp_path, "b.msh", "$MeshFormat\n4.1 1 8\n$EndMeshFormat\n"))
     with pytest.raises(AssertionError):
         read_msh(_write(tmp_path, "x.msh", "hello\n"))
+
+
+def test_model_headers_round_trip():
+    """Model parameters (scalars and per-element arrays) survive the
+    simulation file's header."""
+    import json
+
+    from easyfea_amd import Models
+    from easyfea_amd.sim_io import _model_from, _model_header
+
+    E = np.linspace(1.0, 2.0, 7)
+    mats = [Models.Elastic.Isotropic(2, E, 0.25, planeStress=False, thickness=3.0),
+            Models.PhaseField(Models.Elastic.Isotropic(3, 5.0, 0.2), "Bourdin", "AT1", np.arange(1.0, 8.0), 0.1,
+                              "HistoryDamage"),
+            Models.Thermal(np.full(7, 2.0), 0.5, thickness=2.0)]
+    for m in mats:
+        arrays = {}
+        h = json.loads(json.dumps(_model_header(m, arrays)))
+        back = _model_from(h, arrays)
+        assert json.dumps(_model_header(back, {}), sort_keys=True) == json.dumps(h, sort_keys=True)
+    pf = _model_from(json.loads(json.dumps(_model_header(mats[1], {}))), {"p/Gc": np.arange(1.0, 8.0)})
+    assert pf.split.value == "Bourdin" and pf.regularization.value == "AT1" and pf.solver.value == "HistoryDamage"
+    assert np.array_equal(pf.Gc, np.arange(1.0, 8.0)) and pf.material.dim == 3
