@@ -1040,8 +1040,6 @@ class PhaseField(_Simu):
         assert mesh.elemType in (ElemType.TRI3, ElemType.QUAD4, ElemType.TETRA4, ElemType.TETRA10, ElemType.HEXA8), \
             "the device phase-field path supports TRI3 / QUAD4 / TETRA4 / TETRA10 / HEXA8 meshes"
         assert mesh.dim == model.dim, "mesh and material dimensions differ"
-        assert model.solver != PhaseFieldModel.SolverType.BoundConstrain, \
-            "BoundConstrain (scipy lsq_linear on the host) has no device solver; use History or HistoryDamage"
         self._updU = False
         self._updD = False
         super().__init__(mesh, model, folder, verbosity)
@@ -1058,6 +1056,12 @@ class PhaseField(_Simu):
         self._Fd = torch.zeros(Nn, dtype=torch.float64, device="cuda")
         self.Niter_staggered = 0
         self.convIter = 0.0
+        # BoundConstrain solve: CGLS reduction per pass, first-order tolerance
+        # (damage units), CGLS iterations per pass, active-set updates
+        self.bounded_tol = 1e-10
+        self.bounded_kkt_tol = 1e-9
+        self.bounded_maxit = 100000
+        self.bounded_max_outer = 200
 
     def Get_problemTypes(self) -> list:
         return ["damage", "elastic"]
@@ -1231,8 +1235,34 @@ class PhaseField(_Simu):
         Kd = self._device_K("damage")
         b = self._Fd + self._neumann_device("damage")
         x = self._d.clone()
-        yield from self._solve_system_gen(Kd, b, x, "damage")
+        if self.model.solver == PhaseFieldModel.SolverType.BoundConstrain:
+            self._solve_bounded(Kd, b, x)
+        else:
+            yield from self._solve_system_gen(Kd, b, x, "damage")
         self._d = x
+
+    def _solve_bounded(self, Kd, b: torch.Tensor, x: torch.Tensor) -> None:
+        """BoundConstrain damage solve: ``min |Kd x - b|^2`` with the current
+        damage as lower bound (values >= 1 lowered by one ulp) and 1 as
+        upper bound (``_phasefield.py:148-163``; the reference calls
+        ``scipy.optimize.lsq_linear``, ``Solvers.py:376-383``), solved to
+        its minimiser on the device by ``efea_solve_bounded_lsq``."""
+        assert self.Bc_dofs_Dirichlet("damage").size == 0, \
+            "BoundConstrain bounds every damage DOF: Dirichlet conditions on the damage are not supported"
+        assert Kd.dof_n == 1 and Kd.node_ptr.dtype == torch.int32
+        lb = self._d.clone()
+        lb[lb >= 1] = 1 - np.finfo(float).eps
+        ub = torch.ones_like(lb)
+        N = Kd.Nn
+        nbytes = int(_lib.load().efea_bounded_lsq_workspace_bytes(N))
+        ws = self._ws.get(nbytes)
+        info = np.zeros(4)
+        rc = _lib.call("efea_solve_bounded_lsq", N, ptr(Kd.node_ptr), ptr(Kd.node_cols), ptr(Kd.values), ptr(b),
+                       ptr(lb), ptr(ub), ptr(x), ptr(ws), nbytes, self.bounded_tol, self.bounded_kkt_tol,
+                       self.bounded_maxit, self.bounded_max_outer, info.ctypes.data, stream_ptr())
+        self.last_bounded = {"updates": int(info[0]), "iterations": int(info[1]), "violations": int(info[2])}
+        if rc == _lib.ERR_NOCONV:
+            raise RuntimeError(f"BoundConstrain damage solve did not converge: {_lib.last_error()}")
 
     def _solve_elastic_gen(self):
         Ku = self._device_K("elastic")

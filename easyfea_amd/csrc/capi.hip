@@ -71,6 +71,9 @@ int lincomb(int64_t, double, const double*, double, const double*, double, const
             double*, hipStream_t);
 int time_update(int, int64_t, double, double, double, double, const double*, const double*, const double*,
                 const double*, double*, double*, double*, hipStream_t);
+size_t bounded_lsq_workspace_bytes(int);
+int solve_bounded_lsq(int, const int*, const int*, const double*, const double*, const double*, const double*,
+                      double*, void*, double, double, int, int, double*, hipStream_t);
 int batch_create(int, int, int, int, int, void**);
 int batch_destroy(void*);
 int batch_add(void*, int, const int*, const int*, const double*, const float*, const float*, const unsigned char*,
@@ -503,6 +506,29 @@ int efea_time_update(int scheme, int64_t n, double dt, double beta, double gamma
     return EFEA_ERR_ARG;
   }
   return time_update(scheme, n, dt, beta, gamma, alpha, u_n, v_n, a_n, x, u_out, v_out, a_out, STREAM(stream));
+}
+
+size_t efea_bounded_lsq_workspace_bytes(int N) { return bounded_lsq_workspace_bytes(N); }
+
+int efea_solve_bounded_lsq(int N, const int32_t* row_ptr, const int32_t* cols, const double* values,
+                           const double* b, const double* lb, const double* ub, double* x, void* workspace,
+                           size_t workspace_bytes, double tol, double kkt_tol, int maxit, int max_outer,
+                           double* info, void* stream) {
+  if (N < 0 || (N > 0 && (!row_ptr || !cols || !values || !b || !lb || !ub || !x || !workspace))) {
+    set_error("efea_solve_bounded_lsq: null pointer or negative size");
+    return EFEA_ERR_ARG;
+  }
+  if (workspace_bytes < bounded_lsq_workspace_bytes(N)) {
+    set_error("efea_solve_bounded_lsq: workspace of %zu bytes, %zu needed", workspace_bytes,
+              bounded_lsq_workspace_bytes(N));
+    return EFEA_ERR_SIZE;
+  }
+  if (!(tol > 0 && tol < 1) || !(kkt_tol > 0) || maxit < 1 || max_outer < 1) {
+    set_error("efea_solve_bounded_lsq: need 0 < tol < 1, kkt_tol > 0, maxit >= 1, max_outer >= 1");
+    return EFEA_ERR_ARG;
+  }
+  return solve_bounded_lsq(N, row_ptr, cols, values, b, lb, ub, x, workspace, tol, kkt_tol, maxit, max_outer, info,
+                           STREAM(stream));
 }
 
 }  // extern "C"

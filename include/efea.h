@@ -337,6 +337,24 @@ int efea_time_update(int scheme, int64_t n, double dt, double beta, double gamma
                      const double* v_n, const double* a_n, const double* x, double* u_out, double* v_out,
                      double* a_out, void* stream);
 
+/* Workspace of efea_solve_bounded_lsq for N unknowns (bytes). */
+size_t efea_bounded_lsq_workspace_bytes(int N);
+
+/* Bound-constrained least squares on the device: x = argmin |A x - b|^2 subject to
+ * lb <= x <= ub, for a symmetric scalar CSR matrix A (row_ptr N+1, cols, values, int32
+ * indices), started from x clamped to the bounds (x in/out).  Primal-dual active-set updates
+ * with CGLS on the free set: each pass reduces the scaled free-set gradient by tol (maxit
+ * iterations at most); done when no variable violates the first-order conditions by more
+ * than kkt_tol; EFEA_ERR_NOCONV after max_outer updates.  info (4 doubles, may be NULL):
+ * active-set updates, CGLS iterations, violations left, first scaled gradient norm^2.
+ * Synchronises the stream.  Replaces scipy.optimize.lsq_linear(A, b, bounds=(lb, ub),
+ * tol=1e-10, method="trf") of the BoundConstrain damage solve (Simulations/Solvers.py:376-383,
+ * bounds from Simulations/_phasefield.py:148-163). */
+int efea_solve_bounded_lsq(int N, const int32_t* row_ptr, const int32_t* cols, const double* values,
+                           const double* b, const double* lb, const double* ub, double* x, void* workspace,
+                           size_t workspace_bytes, double tol, double kkt_tol, int maxit, int max_outer,
+                           double* info, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

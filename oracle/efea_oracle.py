@@ -494,6 +494,32 @@ def known_unknown(dirichlet_dofs: np.ndarray, Ndof: int):
     return np.nonzero(~mask)[0], np.nonzero(mask)[0]
 
 
+def bounded_lsq(A: sp.csr_matrix, b: np.ndarray, lb: np.ndarray, ub: np.ndarray) -> np.ndarray:
+    """The BoundConstrain damage solve (``Simulations/Solvers.py:376-383``):
+    ``scipy.optimize.lsq_linear(A, b, bounds=(lb, ub), tol=1e-10,
+    method="trf")``, the reference's own call of its scipy dependency."""
+    from scipy import optimize
+
+    res = optimize.lsq_linear(sp.csr_matrix(A), np.asarray(b, float).ravel(), bounds=(lb, ub), tol=1e-10,
+                              method="trf", verbose=0)
+    return np.array(res["x"])
+
+
+def bounded_lsq_exact(A, b: np.ndarray, lb: np.ndarray, ub: np.ndarray) -> np.ndarray:
+    """The minimiser the reference's call approximates: ``lsq_linear`` with
+    the bounded-variable least-squares active-set method (``method="bvls"``,
+    Stark and Parker 1995) on the dense matrix, tolerance 1e-14.  The
+    reference's trust-region call stops on a relative cost change of 1e-10,
+    which on ill-conditioned damage systems leaves it measurably short of
+    this point (percent-level in x, 1e-3 in the cost on a 1D AT1-like system)."""
+    from scipy import optimize
+
+    Ad = A.toarray() if sp.issparse(A) else np.asarray(A, float)
+    res = optimize.lsq_linear(Ad, np.asarray(b, float).ravel(), bounds=(lb, ub), method="bvls", tol=1e-14,
+                              verbose=0)
+    return np.array(res["x"])
+
+
 def solve_r1(K: sp.csr_matrix, b: np.ndarray, dir_dofs, dir_vals, Ndof, orphan_dofs=()):
     """``__Solver_1`` (``Simulations/Solvers.py:502-553``) with
     ``__Solver_Get_Dirichlet_A_x`` (``_simu.py:1916-1984``): reduce to the
@@ -1092,9 +1118,11 @@ class PhaseFieldOracle(_SimuBase):
     the cached ``Ku``."""
 
     def __init__(self, connect, coord, elemType, E, v, planeStress, thickness, split, regu, Gc, l0,
-                 boundary=None):
+                 boundary=None, solver="History", lsq="exact"):
         dim = ELEM_DIM[elemType]
         super().__init__(connect, coord, elemType, dim, boundary or {})
+        assert solver in ("History", "BoundConstrain") and lsq in ("exact", "reference")
+        self.solver, self.lsq = solver, lsq
         self.E, self.v, self.planeStress, self.thickness = E, v, planeStress, thickness
         self.split, self.regu, self.Gc, self.l0 = split, regu, Gc, l0
         self.u = np.zeros(self.Nn * dim)
@@ -1128,8 +1156,9 @@ class PhaseFieldOracle(_SimuBase):
         """``__Calc_psiPlus_e_pg`` (:494-538)."""
         eps = self._eps(self.Bm)
         psiP, _ = calc_psi(self.split, eps, self._E_pg(), self.v, self.dim, self.planeStress)
-        inc = psiP - self.old_psiP
-        psiP = np.where(inc < 0, self.old_psiP, psiP)
+        if self.solver == "History":  # only History clamps psi+ (_phasefield.py:512-529)
+            inc = psiP - self.old_psiP
+            psiP = np.where(inc < 0, self.old_psiP, psiP)
         self.psiP = psiP
         return psiP
 
@@ -1183,7 +1212,14 @@ class PhaseFieldOracle(_SimuBase):
             K = self.Get_K("damage")
             b = self.Fd.copy()
             dofs, vals = self.dir.get("damage", (np.array([], int), np.array([])))
-            self.d = solve_r1(K, b, dofs, vals, self.Nn)
+            if self.solver == "BoundConstrain":
+                assert len(dofs) == 0, "BoundConstrain: the reference's bounds cover every damage DOF"
+                lb = self.d.copy()  # Get_lb_ub (_phasefield.py:148-163)
+                lb[lb >= 1] = 1 - np.finfo(float).eps
+                solve = bounded_lsq_exact if self.lsq == "exact" else bounded_lsq
+                self.d = solve(K, b, lb, np.ones_like(lb))
+            else:
+                self.d = solve_r1(K, b, dofs, vals, self.Nn)
             return self.d
         K = self.Get_K("elastic")
         b = self.neumann_vector("elastic", self.Nn * self.dim)

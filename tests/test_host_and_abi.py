@@ -444,3 +444,18 @@ def test_thermal_model_parameters_and_observers():
     assert flags == [True, True] and m.isHeterogeneous
     with pytest.raises(AssertionError):
         m.k = -1.0
+
+
+def test_bounded_lsq_abi_argument_errors_without_gpu():
+    lib = _lib.load()
+    one = (C.c_double * 1)()
+    p = C.cast(one, C.c_void_p)
+    need = lib.efea_bounded_lsq_workspace_bytes(100)
+    assert need >= 6 * 800 + 100 and lib.efea_bounded_lsq_workspace_bytes(0) > 0
+    assert lib.efea_solve_bounded_lsq(-1, p, p, p, p, p, p, p, p, need, 1e-10, 1e-9, 10, 10, None, None) == -1
+    assert lib.efea_solve_bounded_lsq(100, p, p, p, p, p, None, p, p, need, 1e-10, 1e-9, 10, 10, None, None) == -1
+    assert "null pointer" in _lib.last_error()
+    assert lib.efea_solve_bounded_lsq(100, p, p, p, p, p, p, p, p, need - 1, 1e-10, 1e-9, 10, 10, None, None) == -4
+    assert "workspace" in _lib.last_error()
+    assert lib.efea_solve_bounded_lsq(100, p, p, p, p, p, p, p, p, need, 1.5, 1e-9, 10, 10, None, None) == -1
+    assert lib.efea_solve_bounded_lsq(100, p, p, p, p, p, p, p, p, need, 1e-10, 1e-9, 10, 0, None, None) == -1
