@@ -1471,6 +1471,9 @@ class ThermalOracle(_SimuBase):
     def add_surfLoad(self, nodes, values, unknowns, problemType="thermal"):
         super().add_surfLoad(nodes, values, unknowns, problemType, self.thickness)
 
+    def add_volumeLoad(self, nodes, values, unknowns, problemType="thermal"):
+        super().add_volumeLoad(nodes, values, unknowns, problemType, self.thickness)
+
     def Get_K_C(self):
         geo_r = Geometry(self.elemType, self.connect, self.coord, "rigi", self.dim)
         geo_m = Geometry(self.elemType, self.connect, self.coord, "mass", self.dim)

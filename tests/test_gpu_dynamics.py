@@ -281,3 +281,17 @@ def test_a_solve_started_at_its_solution_converges():
     print(f"repeat: {info.iterations} it, rel residual {info.rel_residual:.2e}", flush=True)
     assert info.status == 1
     assert _rel(simu.thermal, t0) < 1e-12
+
+
+@pytest.mark.parametrize("et", ["QUAD4", "HEXA8", "TETRA10"])
+def test_thermal_heat_source_matches_the_oracle(et):
+    """A volumetric heat source (add_volumeLoad, times the thickness in 2D)
+    in a steady solve: T within 1e-9 of the oracle's direct solve."""
+    simu, ref = _thermal_pair(et, seed=3)
+    mesh = simu.mesh
+    for s in (simu, ref):
+        s.add_volumeLoad(mesh.nodes, [lambda x, y, z: 3.0 + x * y], ["t"], "thermal")
+    t = simu.Solve()
+    assert _rel(t, ref.Step()) < 1e-9
+    assert abs(simu._neumann_vector("thermal").sum() - ref.neumann_vector("thermal", mesh.Nn).sum()) \
+        <= 1e-12 * abs(ref.neumann_vector("thermal", mesh.Nn).sum())
