@@ -798,8 +798,26 @@ class Elastic(_Simu):
         axes = ["x", "y"] if dim == 2 else ["x", "y", "z"]
         out += ["u" + a for a in axes] + ["v" + a for a in axes] + ["a" + a for a in axes]
         out += ["S" + c for c in comps] + ["E" + c for c in comps]
-        out += ["Svm", "Stress", "Evm", "Strain", "Wdef", "Wdef_e"]
+        out += ["Svm", "Stress", "Evm", "Strain", "Wdef", "Wdef_e", "ZZ1", "ZZ1_e"]
         return out
+
+    def _Calc_ZZ1(self):
+        """ZZ1 error estimate (``Simulations/_elastic.py:398-418``): the
+        deformation energy with the stresses smoothed at the nodes against
+        the element energy, ``(|W - Ws| / W, |Ws_e - W_e| / W)``, on the
+        device (``efea_elastic_strain_stress``, ``efea_node_average``,
+        ``efea_zz1_energy``)."""
+        dev = self.dev
+        ns = 3 if self.model.dim == 2 else 6
+        e = self._strain_stress_e()
+        W_e = e[:, -1]
+        sig_n = self.dev.node_values(e[:, ns + 1:2 * ns + 1].contiguous()).contiguous()
+        Ws_e = torch.empty(dev.Ne, dtype=torch.float64, device="cuda")
+        tab, nPg = dev.tables(MatrixType.rigi)
+        _lib.call("efea_zz1_energy", dev.code, dev.Ne, ptr(dev.connect), ptr(dev.coord), ptr(tab), nPg, ptr(self._u),
+                  ptr(sig_n), self.model.thickness, ptr(Ws_e), stream_ptr())
+        W = W_e.sum()
+        return float(torch.abs(W - Ws_e.sum()) / W), torch.abs(Ws_e - W_e) / W
 
     def _strain_stress_e(self) -> torch.Tensor:
         """(Ne, 2 ns + 3) per element on the device: strain components, Evm,
@@ -842,6 +860,10 @@ class Elastic(_Simu):
             values = np.linalg.norm((self.speed if result == "speed_norm" else self.accel).reshape(Nn, -1), axis=1)
         elif result == "Wdef":
             return float(self._strain_stress_e()[:, -1].sum())
+        elif result == "ZZ1":
+            return self._Calc_ZZ1()[0]
+        elif result == "ZZ1_e":
+            values = self._Calc_ZZ1()[1]
         else:
             ns = 3 if dim == 2 else 6
             e = self._strain_stress_e()

@@ -73,6 +73,7 @@ SIGNATURES = {
     "efea_uv_ke": (i32, [i32, i32, i64, vp, vp, vp, i32, vp, i32, f64, i32, f64, vp, vp]),
     "efea_lincomb": (i32, [i64, f64, vp, f64, vp, f64, vp, f64, vp, vp, vp]),
     "efea_time_update": (i32, [i32, i64, f64, f64, f64, f64, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "efea_zz1_energy": (i32, [i32, i64, vp, vp, vp, i32, vp, vp, f64, vp, vp]),
     "efea_bounded_lsq_workspace_bytes": (sz, [i32]),
     "efea_solve_bounded_lsq": (i32, [i32, vp, vp, vp, vp, vp, vp, vp, vp, sz, f64, f64, i32, i32, vp, vp]),
 }

@@ -71,6 +71,8 @@ int lincomb(int64_t, double, const double*, double, const double*, double, const
             double*, hipStream_t);
 int time_update(int, int64_t, double, double, double, double, const double*, const double*, const double*,
                 const double*, double*, double*, double*, hipStream_t);
+int zz1_energy(int, int64_t, const int*, const double*, const double*, int, const double*, const double*, double,
+               double*, hipStream_t);
 size_t bounded_lsq_workspace_bytes(int);
 int solve_bounded_lsq(int, const int*, const int*, const double*, const double*, const double*, const double*,
                       double*, void*, double, double, int, int, double*, hipStream_t);
@@ -506,6 +508,15 @@ int efea_time_update(int scheme, int64_t n, double dt, double beta, double gamma
     return EFEA_ERR_ARG;
   }
   return time_update(scheme, n, dt, beta, gamma, alpha, u_n, v_n, a_n, x, u_out, v_out, a_out, STREAM(stream));
+}
+
+int efea_zz1_energy(int elem_type, int64_t Ne, const int32_t* connect, const double* coord, const double* tables,
+                    int nPg, const double* u, const double* sig_n, double thickness, double* out, void* stream) {
+  if (Ne < 0 || (Ne > 0 && (!connect || !coord || !tables || !u || !sig_n || !out))) {
+    set_error("efea_zz1_energy: null pointer or negative size");
+    return EFEA_ERR_ARG;
+  }
+  return zz1_energy(elem_type, Ne, connect, coord, tables, nPg, u, sig_n, thickness, out, STREAM(stream));
 }
 
 size_t efea_bounded_lsq_workspace_bytes(int N) { return bounded_lsq_workspace_bytes(N); }

@@ -122,6 +122,95 @@ k_strain_stress(int64_t Ne, const int* __restrict__ connect, const double* __res
   o[2 * NS + 2] = 0.5 * w * (DIM == 2 ? thickness : 1.0);
 }
 
+// ZZ1 smoothed energy (Simulations/_elastic.py:323-418): at each stiffness
+// point the stress interpolated from the nodal stresses sig_n (Nn, ns; tensor
+// components, shear not scaled) with N, dotted with the point's strain in
+// Kelvin-Mandel form: Ws_e = 1/2 sum_p wJ sig_s . eps (x thickness in 2D).
+template <int ET, int NPG>
+__global__ void __launch_bounds__(256)
+k_zz1_energy(int64_t Ne, const int* __restrict__ connect, const double* __restrict__ coord,
+             const double* __restrict__ tab, const double* __restrict__ u, const double* __restrict__ sig_n,
+             double thickness, double* __restrict__ out) {
+  constexpr int NPE = Elem<ET>::NPE, DIM = Elem<ET>::DIM, NS = KM<DIM>::NS;
+  using TB = Tab<NPG, NPE, DIM>;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= Ne) return;
+  const double isq2 = 0.70710678118654752440, sq2 = 1.41421356237309504880;
+  double X[NPE][3], ue[NPE][DIM], sn[NPE][NS];
+#pragma unroll
+  for (int n = 0; n < NPE; ++n) {
+    const int node = connect[e * NPE + n];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) X[n][k] = coord[(int64_t)node * 3 + k];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) ue[n][k] = u[(int64_t)node * DIM + k];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) sn[n][k] = sig_n[(int64_t)node * NS + k];
+  }
+  double w = 0;
+#pragma unroll
+  for (int p = 0; p < NPG; ++p) {
+    double dNx[DIM][NPE];
+    const double detJ = point_geometry<double, NPE, DIM>(tab + TB::DN + p * DIM * NPE, X, dNx);
+    double g[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) {
+        double s = 0;
+#pragma unroll
+        for (int n = 0; n < NPE; ++n) s += dNx[j][n] * ue[n][i];
+        g[i][j] = s;
+      }
+    double eps[NS];
+    if (DIM == 2) {
+      eps[0] = g[0][0];
+      eps[1] = g[1][1];
+      eps[2] = (g[0][1] + g[1][0]) * isq2;
+    } else {
+      eps[0] = g[0][0];
+      eps[1] = g[1][1];
+      eps[2] = g[2][2];
+      eps[3] = (g[1][2] + g[2][1]) * isq2;
+      eps[4] = (g[0][2] + g[2][0]) * isq2;
+      eps[5] = (g[0][1] + g[1][0]) * isq2;
+    }
+    double dot = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      double sk = 0;
+#pragma unroll
+      for (int n = 0; n < NPE; ++n) sk += tab[TB::N + p * NPE + n] * sn[n][k];
+      dot += (k < DIM ? sk : sk * sq2) * eps[k];
+    }
+    w += detJ * tab[TB::W + p] * dot;
+  }
+  out[e] = 0.5 * w * (DIM == 2 ? thickness : 1.0);
+}
+
+template <int ET, int NPG>
+static int launch_zz1(int64_t Ne, const int* connect, const double* coord, const double* tab, const double* u,
+                      const double* sig_n, double th, double* out, hipStream_t s) {
+  const int64_t nb = (Ne + 255) / 256;
+  if (nb > 0)
+    hipLaunchKernelGGL((k_zz1_energy<ET, NPG>), dim3((unsigned)nb), dim3(256), 0, s, Ne, connect, coord, tab, u,
+                       sig_n, th, out);
+  return check_launch("zz1_energy");
+}
+
+int zz1_energy(int elem, int64_t Ne, const int* connect, const double* coord, const double* tab, int nPg,
+               const double* u, const double* sig_n, double th, double* out, hipStream_t s) {
+#define ZZ(ET, NPG) return launch_zz1<ET, NPG>(Ne, connect, coord, tab, u, sig_n, th, out, s)
+  if (elem == ET_TRI3 && nPg == 1) ZZ(ET_TRI3, 1);
+  if (elem == ET_QUAD4 && nPg == 4) ZZ(ET_QUAD4, 4);
+  if (elem == ET_HEXA8 && nPg == 8) ZZ(ET_HEXA8, 8);
+  if (elem == ET_TETRA4 && nPg == 1) ZZ(ET_TETRA4, 1);
+  if (elem == ET_TETRA10 && nPg == 4) ZZ(ET_TETRA10, 4);
+#undef ZZ
+  set_error("zz1_energy: unsupported element %d with %d points", elem, nPg);
+  return EFEA_ERR_ARG;
+}
+
 // values_n[n] = (sum of values_e over the elements holding node n, in
 // increasing element order) / their count; 0 for nodes of no element
 // (FEM/_mesh.py:822-873 with the node -> element incidence of

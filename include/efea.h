@@ -337,6 +337,14 @@ int efea_time_update(int scheme, int64_t n, double dt, double beta, double gamma
                      const double* v_n, const double* a_n, const double* x, double* u_out, double* v_out,
                      double* a_out, void* stream);
 
+/* ZZ1 smoothed deformation energy per element (Simulations/_elastic.py:323-418, the
+ * smoothedStress branch of _Calc_Psi_Elas used by _Calc_ZZ1): out[e] = 1/2 sum_p wJ sig_s . eps
+ * (x thickness in 2D) at the stiffness points, eps = B u_e and sig_s = sum_a N_a sig_n[a], with
+ * sig_n (Nn, ns) the nodal average of the element-mean stresses (tensor components, shear not
+ * scaled by sqrt 2, as efea_elastic_strain_stress reports them, then efea_node_average). */
+int efea_zz1_energy(int elem_type, int64_t Ne, const int32_t* connect, const double* coord, const double* tables,
+                    int nPg, const double* u, const double* sig_n, double thickness, double* out, void* stream);
+
 /* Workspace of efea_solve_bounded_lsq for N unknowns (bytes). */
 size_t efea_bounded_lsq_workspace_bytes(int N);
 

@@ -966,6 +966,22 @@ def pf_stress_e_pg(geo: Geometry, u: np.ndarray, d: np.ndarray, connect: np.ndar
     return g[..., None] * sP + sM
 
 
+def zz1(geo: Geometry, u: np.ndarray, connect: np.ndarray, C: np.ndarray, Nn: int, thickness: float = 1.0):
+    """``Elastic._Calc_ZZ1`` (``Simulations/_elastic.py:398-418``) with the
+    smoothed energy of ``_Calc_Psi_Elas(smoothedStress=True)`` (:357-378):
+    the element-mean stresses (Kelvin-Mandel) averaged at the nodes,
+    interpolated back to the points with ``N`` and dotted with the strain.
+    Returns ``(error, error_e)``."""
+    eps = strain_e_pg(geo, u, connect)
+    sig = stress_e_pg(eps, C)
+    W_e = psi_elas_e(geo, eps, sig, thickness)
+    sig_n = node_values(connect, Nn, sig.mean(axis=1))
+    sig_s = np.einsum("eni,pn->epi", sig_n[connect], geo.N_pg)
+    Ws_e = psi_elas_e(geo, eps, sig_s, thickness)
+    W, Ws = W_e.sum(), Ws_e.sum()
+    return abs(W - Ws) / W, np.abs(Ws_e - W_e) / W
+
+
 def node_values(connect: np.ndarray, Nn: int, values_e: np.ndarray) -> np.ndarray:
     """``Mesh.Get_Node_Values`` (``FEM/_mesh.py:822-873``) with the incidence of
     ``_GroupElem.Get_connect_n_e`` (``FEM/_group_elem.py:322-344``): per node,

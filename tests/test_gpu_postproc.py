@@ -136,3 +136,22 @@ def test_phasefield_results_match_the_oracle(split, et):
     psiP, _ = O.calc_psi(split, np.einsum("epij,ej->epi", geo_m.B(), u[O.assembly_e(mesh.connect, 2)]),
                          E[:, None], 0.3, 2, False)
     assert _rel(simu.Result("psiP", nodeValues=False), psiP.mean(1)) < 1e-12
+
+
+@pytest.mark.parametrize("ps", [True, False])
+@pytest.mark.parametrize("et", list(MAKERS))
+def test_zz1_error_estimate(et, ps):
+    """``Result("ZZ1")`` and ``Result("ZZ1_e")`` against the oracle's ``zz1``
+    (heterogeneous E, random displacement, distorted mesh): within 1e-10
+    relative; a linear displacement on homogeneous E gives 0 to round-off."""
+    simu, mesh, geo, eps, sig, th = _setup(et, ps=ps)
+    dim = mesh.dim
+    E = simu.model.E
+    err, err_e = O.zz1(geo, simu.displacement, mesh.connect, O.C_isotropic(E, 0.3, dim, ps), mesh.Nn, th)
+    assert abs(simu.Result("ZZ1") - err) <= 1e-10 * err
+    assert _rel(simu.Result("ZZ1_e", nodeValues=False), err_e) < 1e-10
+    assert _rel(simu.Result("ZZ1_e"), O.node_values(mesh.connect, mesh.Nn, err_e)) < 1e-10
+    simu.model.E = 210000.0
+    A = np.random.default_rng(4).standard_normal((dim, dim)) * 1e-3
+    simu._u = torch.from_numpy((mesh.coord[:, :dim] @ A.T).ravel()).cuda()
+    assert simu.Result("ZZ1") < 1e-11 and np.abs(simu.Result("ZZ1_e", nodeValues=False)).max() < 1e-11
