@@ -21,7 +21,8 @@
 //     Numerical Methods for Least Squares Problems, 1996, section 7.4) with
 //     the same column scaling as preconditioner;
 //     each pass stops once the scaled free-set gradient norm has fallen by
-//     the factor tol;
+//     the factor tol, or once it bounds every free variable's first-order
+//     error below kkt_tol / 2;
 //   * stop when no variable violates the first-order conditions by more
 //     than kkt_tol (in units of x): |x_i - clamp(x_i - g_i/d_i, lb_i, ub_i)|;
 //     after max_outer updates without that, the solve reports
@@ -66,6 +67,29 @@ __global__ void __launch_bounds__(BS) k_colnorm2(int N, const int* __restrict__ 
     double s = 0;
     for (int k = rp[i]; k < rp[i + 1]; ++k) s += val[k] * val[k];
     d[i] = s;
+  }
+}
+
+// smallest positive d_i per block (+inf when none), then over the blocks
+__global__ void __launch_bounds__(BS) k_dmin_part(int N, const double* __restrict__ d, double* __restrict__ part) {
+  __shared__ double sh[BS];
+  double m = INFINITY;
+  for (int i = blockIdx.x * BS + threadIdx.x; i < N; i += gridDim.x * BS)
+    if (d[i] > 0) m = fmin(m, d[i]);
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = BS / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(BS) k_dmin_fin(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  if (threadIdx.x == 0) {
+    double m = INFINITY;
+    for (int k = 0; k < nb; ++k) m = fmin(m, part[k]);
+    *out = m;
   }
 }
 
@@ -245,7 +269,15 @@ int solve_bounded_lsq(int N, const int* rp, const int* ci, const double* val, co
   unsigned viol = 0;
   double gref = -1.0;
   hipLaunchKernelGGL(k_colnorm2, dim3(nb), dim3(BS), 0, s, N, rp, val, w.d);
+  hipLaunchKernelGGL(k_dmin_part, dim3(nb), dim3(BS), 0, s, N, (const double*)w.d, w.part);
+  hipLaunchKernelGGL(k_dmin_fin, dim3(1), dim3(BS), 0, s, (const double*)w.part, (int)nb, w.sc + SC_TMP);
+  hipMemcpyAsync(hp.gam + 1, w.sc + SC_TMP, sizeof(double), hipMemcpyDeviceToHost, s);
   hipLaunchKernelGGL(k_clamp, dim3(nb), dim3(BS), 0, s, N, lb, ub, x);
+  if (hipStreamSynchronize(s) != hipSuccess) return check_launch("solve_bounded_lsq");
+  // a pass may also stop once no free variable can be off by more than
+  // kkt_tol / 2: |g_i| / d_i <= sqrt(gamma / d_i) <= sqrt(gamma / d_min)
+  const double dmin = hp.gam[1];
+  const double enough = dmin < INFINITY ? 0.25 * kkt_tol * kkt_tol * dmin : 0.0;
   for (;; ++outer) {
     // gradient g = A (A x - b) = -A s with s = b - A x
     hipLaunchKernelGGL(k_spmv, dim3(nb), dim3(BS), 0, s, N, rp, ci, val, (const double*)x, -1.0, 1.0, b, w.s);
@@ -273,7 +305,7 @@ int solve_bounded_lsq(int N, const int* rp, const int* ci, const double* val, co
     // each pass reduces the scaled free-set gradient norm by tol from where it starts
     if (gref < 0) gref = hp.gam[0];
     double gam = hp.gam[0];
-    const double stop = tol * tol * gam;
+    const double stop = fmax(tol * tol * gam, enough);
     for (int it = 0; gam > stop && it < maxit;) {
       for (int k = 0; k < POLL; ++k, ++it) {
         hipLaunchKernelGGL(k_spmv, dim3(nb), dim3(BS), 0, s, N, rp, ci, val, (const double*)w.p, 1.0, 0.0,
