@@ -124,6 +124,44 @@ __global__ void __launch_bounds__(BS) k_normal_q(int N, const int* __restrict__ 
   }
 }
 
+// t = A p with the per-block partial sums of t.t (one pass instead of two)
+__global__ void __launch_bounds__(BS) k_spmv_tt(int N, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                const double* __restrict__ val, const double* __restrict__ p,
+                                                double* __restrict__ t, double* __restrict__ part) {
+  __shared__ double sh[BS / 64];
+  double acc = 0;
+  for (int i = blockIdx.x * BS + threadIdx.x; i < N; i += gridDim.x * BS) {
+    double v = 0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) v += val[k] * p[ci[k]];
+    t[i] = v;
+    acc += v * v;
+  }
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// k_normal_q with the per-block partial sums of q.z
+__global__ void __launch_bounds__(BS) k_normal_qz(int N, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                  const double* __restrict__ val, const double* __restrict__ s,
+                                                  const unsigned char* __restrict__ state,
+                                                  const double* __restrict__ d, double* __restrict__ q,
+                                                  double* __restrict__ z, double* __restrict__ part) {
+  __shared__ double sh[BS / 64];
+  double acc = 0;
+  for (int i = blockIdx.x * BS + threadIdx.x; i < N; i += gridDim.x * BS) {
+    double v = 0, zi = 0;
+    if (state[i] == 0 && d[i] > 0) {
+      for (int k = rp[i]; k < rp[i + 1]; ++k) v += val[k] * s[ci[k]];
+      zi = v / d[i];
+    }
+    q[i] = v;
+    z[i] = zi;
+    acc += v * zi;
+  }
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
 __global__ void __launch_bounds__(BS) k_dot_part(int64_t n, const double* __restrict__ a,
                                                  const double* __restrict__ b, double* __restrict__ part) {
   __shared__ double sh[BS / 64];
@@ -308,14 +346,13 @@ int solve_bounded_lsq(int N, const int* rp, const int* ci, const double* val, co
     const double stop = fmax(tol * tol * gam, enough);
     for (int it = 0; gam > stop && it < maxit;) {
       for (int k = 0; k < POLL; ++k, ++it) {
-        hipLaunchKernelGGL(k_spmv, dim3(nb), dim3(BS), 0, s, N, rp, ci, val, (const double*)w.p, 1.0, 0.0,
-                           (const double*)nullptr, w.t);
-        dot(N, w.t, w.t, 1, w, s);
+        hipLaunchKernelGGL(k_spmv_tt, dim3(nb), dim3(BS), 0, s, N, rp, ci, val, (const double*)w.p, w.t, w.part);
+        hipLaunchKernelGGL(k_dot_fin, dim3(1), dim3(BS), 0, s, (const double*)w.part, (int)nb, 1, w.sc);
         hipLaunchKernelGGL(k_step_x, dim3(nb), dim3(BS), 0, s, N, (const double*)w.sc, (const double*)w.p,
                            (const double*)w.t, x, w.s);
-        hipLaunchKernelGGL(k_normal_q, dim3(nb), dim3(BS), 0, s, N, rp, ci, val, (const double*)w.s,
-                           (const unsigned char*)w.state, (const double*)w.d, w.g, w.z);
-        dot(N, w.g, w.z, 2, w, s);
+        hipLaunchKernelGGL(k_normal_qz, dim3(nb), dim3(BS), 0, s, N, rp, ci, val, (const double*)w.s,
+                           (const unsigned char*)w.state, (const double*)w.d, w.g, w.z, w.part);
+        hipLaunchKernelGGL(k_dot_fin, dim3(1), dim3(BS), 0, s, (const double*)w.part, (int)nb, 2, w.sc);
         hipLaunchKernelGGL(k_step_p, dim3(nb), dim3(BS), 0, s, N, (const double*)w.sc, (const double*)w.z, w.p, 0);
       }
       hipMemcpyAsync(hp.gam, w.sc + SC_GAM, sizeof(double), hipMemcpyDeviceToHost, s);
