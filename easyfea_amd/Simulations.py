@@ -90,6 +90,9 @@ class _Simu:
         self.maxit = 200000
         self.check_every = 32
         self.preconditioner = "auto"  # "bj" | "two-level" | "auto"
+        # hip_mixed: keep FP64 Krylov products for matrices under
+        # device.MIXED_MIN_MATRIX_BYTES (latency-bound); False forces FP32 values
+        self.mixed_adaptive = True
         self.max_blocks = 0  # grid cap of the solver kernels (0: the library's default)
         # keep the two-level coarse inverse over successive solves of one matrix
         # until their iteration count grows (device.COARSE_REUSE_GROWTH)
@@ -533,8 +536,9 @@ class _Simu:
         prev = self.last_solve.get(problemType)
         expected = prev.iterations if self.predict_polls and prev is not None and prev.status == 1 else None
         return SolveJob(A, b_dev, x_dev, known, self._mode(), self.tol_residual, self.tol_anorm,
-                        self.replace_ratio, self.maxit, self.check_every, owner=self, precond=self.preconditioner,
-                        max_blocks=self.max_blocks, expected_iterations=expected,
+                        self.replace_ratio, self.maxit, self.check_every, adaptive=self.mixed_adaptive,
+                        owner=self, precond=self.preconditioner, max_blocks=self.max_blocks,
+                        expected_iterations=expected,
                         coarse_target=self.coarse_target)
 
     def _solve_system_gen(self, A, b_dev: torch.Tensor, x_dev: torch.Tensor, problemType,
@@ -1253,13 +1257,30 @@ class PhaseField(_Simu):
         return E
 
     # ------------------------------------------------------------ solves
+    def _start(self, field: str) -> torch.Tensor:
+        """Start of the iterative solve of ``field`` ("u" or "d"): on the
+        first pass of an increment that follows two saved ones (the field is
+        still the last saved one), the secant extrapolation
+        ``2 x_n - x_(n-1)`` of those saves; the current field otherwise.
+        Only the Krylov iteration's start changes: the solution meets the
+        same stopping test (Dirichlet values are written over it)."""
+        cur = self._u if field == "u" else self._d
+        saved = self.__dict__.get("_saved_states")
+        if not self.extrapolate_guess or saved is None:
+            return cur.clone()
+        prev, last = saved[field]
+        if prev is None or last is not cur or prev.shape != cur.shape:
+            return cur.clone()
+        return 2.0 * cur - prev
+
     def _solve_damage_gen(self):
         Kd = self._device_K("damage")
         b = self._Fd + self._neumann_device("damage")
-        x = self._d.clone()
         if self.model.solver == PhaseFieldModel.SolverType.BoundConstrain:
+            x = self._d.clone()
             self._solve_bounded(Kd, b, x)
         else:
+            x = self._start("d")
             yield from self._solve_system_gen(Kd, b, x, "damage")
         self._d = x
 
@@ -1289,7 +1310,7 @@ class PhaseField(_Simu):
     def _solve_elastic_gen(self):
         Ku = self._device_K("elastic")
         b = self._neumann_device("elastic")
-        x = self._u.clone()
+        x = self._start("u")
         yield from self._solve_system_gen(Ku, b, x, "elastic")
         self._u = x
 
@@ -1371,6 +1392,10 @@ class PhaseField(_Simu):
         self._H = self._psiP.clone()
         it["displacement"] = self._u.clone()
         it["damage"] = self._d.clone()
+        prev = self.__dict__.get("_saved_states")
+        # the last two saved fields (tensor identity marks an unchanged field)
+        self._saved_states = {"u": (prev["u"][1] if prev else None, self._u),
+                              "d": (prev["d"][1] if prev else None, self._d)}
         super().Save_Iter(it)
 
     def Set_Iter(self, iter: int = -1, resetAll=False) -> Optional[dict]:
@@ -1380,6 +1405,7 @@ class PhaseField(_Simu):
         self._u = r["displacement"].clone()
         self._d = r["damage"].clone()
         self._updD = self._updU = False
+        self._saved_states = None
         if resetAll:
             self._H = None
             self._assemble_damage()

@@ -8,11 +8,12 @@ Every function restates the reference's algorithm from its source text
 (citations are to files of the EasyFEA v3.5.1 tree); inputs are plain arrays
 (``connect``, ``coord``) so that both this oracle and the device path consume
 the same mesh.  Pinning (see ``tests/test_oracle_*.py``): closed forms,
-invariants and the reference's own known-answer tests.  Parity unpinned:
-the 3D Miehe spectral split at exactly repeated eigenvalues (no 3D
-phase-field workload exercises it), and the staggered-loop caching order,
-which no closed form fixes (pinned only by the cross-path elastic-limit
-invariant).
+invariants and the reference's own known-answer tests; the staggered loop
+(``PhaseFieldOracle.Solve``, every convergence option) by per-pass residual
+identities of independently assembled systems, with planted slips shown to
+fail them (``tests/test_oracle_staggered.py``).  Parity unpinned: the 3D
+Miehe spectral split at exactly repeated eigenvalues (no 3D phase-field
+workload exercises it; the device has no 3D spectral kernel).
 """
 
 from __future__ import annotations
@@ -1126,18 +1127,21 @@ class ElasticOracle(_SimuBase):
 
 
 class PhaseFieldOracle(_SimuBase):
-    """``Simulations.PhaseField`` staggered solve with the ``History``
-    irreversibility (``Simulations/_phasefield.py:248-657``), Amor or Miehe
-    split, AT1/AT2.  Reproduces the assembly caching order of the reference:
-    ``Ku`` is built from the pre-solve displacement and the new damage, the
-    history ``H`` advances only in ``Save_Iter``, and ``Calc_Reaction`` uses
-    the cached ``Ku``."""
+    """``Simulations.PhaseField`` staggered solve with the ``History``,
+    ``HistoryDamage`` or ``BoundConstrain`` irreversibility
+    (``Simulations/_phasefield.py:248-657``), every split, AT1/AT2.
+    Reproduces the assembly caching order of the reference (``Get_K_C_M_F``
+    :248-269): ``Ku`` is built from the pre-solve displacement and the new
+    damage, ``Kd`` from the displacement current when it is next needed (the
+    energies of options 1 and 2 assemble it from the post-solve one), the
+    history ``H`` advances only in ``Save_Iter`` (to the last assembled psi+),
+    and ``Calc_Reaction`` uses the cached ``Ku``."""
 
     def __init__(self, connect, coord, elemType, E, v, planeStress, thickness, split, regu, Gc, l0,
                  boundary=None, solver="History", lsq="exact"):
         dim = ELEM_DIM[elemType]
         super().__init__(connect, coord, elemType, dim, boundary or {})
-        assert solver in ("History", "BoundConstrain") and lsq in ("exact", "reference")
+        assert solver in ("History", "HistoryDamage", "BoundConstrain") and lsq in ("exact", "reference")
         self.solver, self.lsq = solver, lsq
         self.E, self.v, self.planeStress, self.thickness = E, v, planeStress, thickness
         self.split, self.regu, self.Gc, self.l0 = split, regu, Gc, l0
@@ -1217,11 +1221,35 @@ class PhaseFieldOracle(_SimuBase):
         return self.Kd
 
     def psi_crack(self):
-        """``_Calc_Psi_Crack`` (:798-816)."""
+        """``_Calc_Psi_Crack`` (:798-816): ``1/2 d.Kd d`` with the cached
+        (or freshly assembled) ``Kd``; 0 for a zero damage field."""
         Kd = self.Get_K("damage")
         if np.linalg.norm(self.d) == 0:
             return 0.0
         return 0.5 * self.d @ (Kd @ self.d)
+
+    def psi_elas(self):
+        """``_Calc_Psi_Elas`` (:775-796): ``1/2 u.Ku u`` with the cached (or
+        freshly assembled) ``Ku``; 0 for a zero displacement."""
+        Ku = self.Get_K("elastic")
+        if np.linalg.norm(self.u) == 0:
+            return 0.0
+        return 0.5 * self.u @ (Ku @ self.u)
+
+    def psi_ext(self, fu):
+        """``_Calc_Psi_Ext`` (:818-836): ``u.f`` (no 1/2); 0 when either is zero."""
+        if np.linalg.norm(self.u) == 0 or np.linalg.norm(fu) == 0:
+            return 0.0
+        return self.u @ fu
+
+    def _energy(self, convOption, fu):
+        """The convergence energy of ``Solve`` (:354-361, 376-381): crack
+        energy (option 1) or total energy crack + elastic - external (option 2,
+        Ambati 2015 eq. 39), each term in the reference's order of calls."""
+        E = self.psi_crack()
+        if convOption == 2:
+            E += self.psi_elas() - self.psi_ext(fu)
+        return E
 
     def _solve(self, problemType):
         if problemType == "damage":
@@ -1243,27 +1271,59 @@ class PhaseFieldOracle(_SimuBase):
         self.u = solve_r1(K, b, dofs, vals, self.Nn * self.dim)
         return self.u
 
-    def Solve(self, tolConv=1.0, maxIter=500, convOption=1):
-        """``PhaseField.Solve`` (:303-432) for convOption 0 and 1."""
-        assert convOption in (0, 1)
+    def Solve(self, tolConv=1.0, maxIter=500, convOption=2):
+        """``PhaseField.Solve`` (:303-432), every convergence option:
+        0 max |d_np1 - d_n|, 1 relative change of the crack energy, 2 of the
+        total energy, 3 the summed relative changes of u and d (converged when
+        convD <= tolConv and convU <= 0.999 tolConv).  A pass solves the
+        damage with the cached ``Kd``, invalidates ``Ku``, solves the
+        displacement, invalidates ``Kd``; ``tolConv == 1`` or an all-zero
+        damage stops after one pass.  ``self.trace`` records, per pass,
+        ``(u_n, d_n, u_np1, d_np1, convIter)``; ``HistoryDamage`` returns
+        the node-wise maximum of the damage before and after the call."""
+        assert 0 < tolConv <= 1, "tolConv must be between 0 and 1."
+        assert maxIter > 1, "Must be > 1."
+        assert convOption in (0, 1, 2, 3)
         Niter = 0
         converged = False
+        old_damage = self.d.copy()
+        fu = self.neumann_vector("elastic", self.Nn * self.dim) if convOption == 2 else None
+        self.trace = []
         while not converged and Niter < maxIter:
             Niter += 1
             d_n = self.d.copy()
-            if convOption == 1:
-                E_n = self.psi_crack()
-            d_np1 = self._solve("damage")
+            u_n = self.u.copy()
+            if convOption in (1, 2):
+                E_n = self._energy(convOption, fu)
+            d_np1 = self._solve("damage").copy()
             self.updU = False
-            u_np1 = self._solve("elastic")
+            u_np1 = self._solve("elastic").copy()
             self.updD = False
             if convOption == 0:
                 conv = np.max(np.abs(d_np1 - d_n))
+            elif convOption in (1, 2):
+                E_np1 = self._energy(convOption, fu)
+                conv = np.abs(E_n - E_np1) if E_np1 == 0 else np.abs((E_n - E_np1) / E_np1)
             else:
-                E_np1 = self.psi_crack()
-                conv = abs(E_n - E_np1) if E_np1 == 0 else abs((E_n - E_np1) / E_np1)
-            converged = True if (tolConv == 1 or d_np1.max() == 0) else conv <= tolConv
+                diffU = np.abs(u_np1 - u_n)
+                nz = u_np1 != 0
+                diffU[nz] *= 1 / np.abs(u_np1[nz])
+                diffD = np.abs(d_np1 - d_n)
+                nz = d_np1 != 0
+                diffD[nz] *= 1 / np.abs(d_np1[nz])
+                convU, convD = np.sum(diffU), np.sum(diffD)
+                conv = max(convD, convU)
+            self.trace.append((u_n, d_n, u_np1, d_np1, float(conv)))
+            if tolConv == 1 or d_np1.max() == 0:
+                converged = True
+            elif convOption == 3:
+                converged = (convD <= tolConv) and (convU <= tolConv * 0.999)
+            else:
+                converged = conv <= tolConv
         self.Niter = Niter
+        self.convIter = float(conv)
+        if self.solver == "HistoryDamage":
+            d_np1 = np.maximum(old_damage, d_np1)
         return u_np1, d_np1, converged
 
     def Save_Iter(self):
