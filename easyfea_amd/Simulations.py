@@ -1258,15 +1258,21 @@ class PhaseField(_Simu):
 
     # ------------------------------------------------------------ solves
     def _start(self, field: str) -> torch.Tensor:
-        """Start of the iterative solve of ``field`` ("u" or "d"): on the
-        first pass of an increment that follows two saved ones (the field is
-        still the last saved one), the secant extrapolation
-        ``2 x_n - x_(n-1)`` of those saves; the current field otherwise.
-        Only the Krylov iteration's start changes: the solution meets the
-        same stopping test (Dirichlet values are written over it)."""
+        """Start of the iterative solve of ``field`` ("u" or "d").  The
+        damage starts, on the first pass of an increment that follows two
+        saved ones (the field is still the last saved one), from the secant
+        extrapolation ``2 d_n - d_(n-1)`` of those saves (SENT 200 x 200
+        history: 72.2 instead of 76.6 iterations per damage solve); the
+        displacement, and the damage otherwise, from the current field.  (A
+        secant displacement start measured 197.6 instead of 182.7 iterations:
+        the stopping test is relative to max(sum alpha r.z, x0.A x0), and the
+        current field with the new Dirichlet values carries the energy of the
+        top-edge jump, which the secant start does not.)  Only the Krylov
+        iteration's start changes: the solution meets the same stopping test
+        (Dirichlet values are written over it)."""
         cur = self._u if field == "u" else self._d
         saved = self.__dict__.get("_saved_states")
-        if not self.extrapolate_guess or saved is None:
+        if field == "u" or not self.extrapolate_guess or saved is None:
             return cur.clone()
         prev, last = saved[field]
         if prev is None or last is not cur or prev.shape != cur.shape:

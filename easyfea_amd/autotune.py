@@ -43,13 +43,37 @@ def _cases(quick: bool):
         yield f"TETRA10 box {n}^3", (lambda n=n: tetra10_from_tetra4(tetra4_box(1.0, 1.0, 1.0, n, n, n))), (3,)
 
 
+def time_step1(A, precision: int, reps: int, group: int = 0) -> float:
+    """Average duration (s) of ``reps`` back-to-back step-1 kernels (p update
+    fused into the node-block SpMV, ``efea_spmv_repeat``) on ``A``'s matrix
+    with ``group`` lanes per node, timed with HIP events on the stream the
+    kernels run on; FP64 or FP32 values."""
+    import torch
+
+    from . import _lib
+    from .device import ptr, stream_ptr
+
+    x = torch.rand(A.Ndof, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    vals = A.values if precision == 64 else A.make_fp32()
+    head = (precision, A.Nn, A.dof_n, group or A.group, ptr(A.node_ptr), ptr(A.node_cols), ptr(vals), None, ptr(x),
+            ptr(y), ptr(A.spmv_scratch))
+    _lib.call("efea_spmv_repeat", *head, 10, stream_ptr())
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    _lib.call("efea_spmv_repeat", *head, reps, stream_ptr())
+    e1.record(s)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / reps
+
+
 def tune_pattern(pattern, reps: int) -> dict:
     """Times every candidate lanes-per-node value on ``pattern``."""
     import torch
 
     from . import _lib
     from .device import DeviceCsr
-    from .kernel_probe import time_step1
 
     A = DeviceCsr(pattern)
     g = torch.Generator(device="cuda")

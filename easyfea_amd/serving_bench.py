@@ -203,7 +203,7 @@ def run_phasefield(args) -> dict:
     return out
 
 
-def main(argv=None):
+def main(argv=None, quiet: bool = False):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--workload", choices=["elastic", "phasefield"], default="elastic")
     ap.add_argument("--mode", choices=["mixed", "fp64"], default="mixed")
@@ -235,7 +235,8 @@ def main(argv=None):
         # a batch streams many matrices per iteration: FP32 values pay at any size
         device.MIXED_MIN_MATRIX_BYTES = 0
     out = run_elastic(args) if args.workload == "elastic" else run_phasefield(args)
-    print(json.dumps({k: v for k, v in out.items() if not k.startswith("_")}), flush=True)
+    if not quiet:
+        print(json.dumps({k: v for k, v in out.items() if not k.startswith("_")}), flush=True)
     return out
 
 

@@ -41,7 +41,10 @@ def _timed_solves(simu, repeats):
     return np.array(lat)
 
 
-def run(workload: str, cells: int, repeats: int, coarse_target=None) -> dict:
+def run(workload: str, cells: int, repeats: int, coarse_target=None, on_run=None) -> dict:
+    """Times every (preconditioner, precision) pair; ``on_run(simu, precond,
+    solver)`` is called after each pair's timed solves (the simulation still
+    holds its matrix and last solve)."""
     import torch
 
     from . import workloads
@@ -64,6 +67,8 @@ def run(workload: str, cells: int, repeats: int, coarse_target=None) -> dict:
             setup_s = time.perf_counter() - t0
             simu.Solve()  # warm-up: code objects, workspaces
             lat = _timed_solves(simu, repeats)
+            if on_run is not None:
+                on_run(simu, precond, solver)
             info = simu.last_solve["elastic"]
             u = simu._u.clone()
             ndof = simu.mesh.Nn * simu.model.dim
