@@ -40,6 +40,10 @@ SIGNATURES = {
         i32,
         [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, f64, f64, f64, i32, i32, i32, i32, vp, vp],
     ),
+    "efea_solve_pcg_noalloc": (
+        i32,
+        [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, f64, f64, f64, i32, i32, i32, i32, vp, vp],
+    ),
     "efea_coarse_ld": (i32, [i32, i32]),
     "efea_coarse_scratch_bytes": (sz, [i32]),
     "efea_coarse_setup": (i32, [i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp]),

@@ -5,6 +5,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "efea_common.h"
 #include "pcg_device.h"
 
@@ -44,7 +46,7 @@ int bj_setup(int, int, int, const int*, const int*, const double*, const unsigne
 size_t pcg_workspace_bytes(int, int, int);
 int solve_pcg(int, int, int, int, const int*, const int*, const double*, const float*, const float*,
               const unsigned char*, const void*, const double*, double*, void*, double, double, double, int, int,
-              int, int, hipStream_t, double*);
+              int, int, hipStream_t, double*, void*);
 int split_f32(int, int, const int*, const double*, float*, float*, hipStream_t);
 int strain_stress(int, int64_t, const int*, const double*, const double*, int, const double*, const double*, int,
                   double, int, double, double*, hipStream_t);
@@ -100,6 +102,44 @@ int solve_pcg2(int, int, int, int, const int*, const int*, const double*, const 
 using namespace efea;
 
 #define STREAM(s) ((hipStream_t)(s))
+
+// Scratch of efea_solve_pcg_noalloc: the PCG workspace (vectors, partials,
+// state) in a file-scope device array, one copy per GPU, held until the
+// process ends.  Host code never passes the array's name to a kernel or a
+// copy: it is a host-side stand-in there.  Its device address, per device,
+// comes from hipGetSymbolAddress at the entry point's first call on that
+// device (with the device current) and is kept in noalloc_ptr.  Nothing in
+// it has to start at zero: the solve's start kernels write the state, the
+// partials and the vectors before any read.  Calls that overlap on one GPU
+// share the array and race.
+static __device__ __align__(256) unsigned char scratch[EFEA_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex noalloc_mu;
+void* noalloc_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on the current device.
+static int noalloc_scratch(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    set_error("efea_solve_pcg_noalloc: no current device (or device id %d above %d)", dev, kMaxDevices - 1);
+    return EFEA_ERR_HIP;
+  }
+  std::lock_guard<std::mutex> lock(noalloc_mu);
+  if (!noalloc_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("efea_solve_pcg_noalloc: hipGetSymbolAddress on device %d: %s", dev, hipGetErrorString(e));
+      return EFEA_ERR_HIP;
+    }
+    noalloc_ptr[dev] = p;
+  }
+  *out = noalloc_ptr[dev];
+  return EFEA_OK;
+}
 
 extern "C" {
 
@@ -262,7 +302,32 @@ int efea_solve_pcg(int mode, int Nnode, int dof_n, int group, const int32_t* nod
   }
   return solve_pcg(mode, Nnode, dof_n, group, node_ptr, node_cols, values64, values32, corr32, known, Minv, b, x,
                    workspace, tol_residual, tol_anorm, replace_ratio, maxit, check_every, first_check, max_blocks,
-                   STREAM(stream), info);
+                   STREAM(stream), info, nullptr);
+}
+
+int efea_solve_pcg_noalloc(int mode, int Nnode, int dof_n, int group, const int32_t* node_ptr,
+                           const int32_t* node_cols, const double* values64, const float* values32,
+                           const float* corr32, const uint8_t* known, const void* Minv, const double* b, double* x,
+                           double tol_residual, double tol_anorm, double replace_ratio, int maxit, int check_every,
+                           int first_check, int max_blocks, void* stream, double* info) {
+  if (Nnode <= 0 || !node_ptr || !node_cols || !values64 || !Minv || !b || !x || !info) {
+    set_error("efea_solve_pcg_noalloc: null pointer or empty system");
+    return EFEA_ERR_ARG;
+  }
+  const size_t need = pcg_workspace_bytes(Nnode, dof_n, group);
+  if (need > (size_t)EFEA_NOALLOC_SCRATCH_BYTES) {
+    set_error("efea_solve_pcg_noalloc: %d nodes x %d DOF need %zu bytes of scratch, more than the %zu of the "
+              "library's static scratch; use efea_solve_pcg with a workspace", Nnode, dof_n, need,
+              (size_t)EFEA_NOALLOC_SCRATCH_BYTES);
+    return EFEA_ERR_SIZE;
+  }
+  void* ws = nullptr;
+  if (int e = noalloc_scratch(&ws)) return e;
+  // the state read-backs land in pageable memory on this stack: no pinned allocation
+  PcgState hs;
+  return solve_pcg(mode, Nnode, dof_n, group, node_ptr, node_cols, values64, values32, corr32, known, Minv, b, x, ws,
+                   tol_residual, tol_anorm, replace_ratio, maxit, check_every, first_check, max_blocks,
+                   STREAM(stream), info, &hs);
 }
 
 int efea_batch_create(int mode, int dof_n, int group, int two_level, int capacity, void** handle) {

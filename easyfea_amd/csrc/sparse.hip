@@ -458,14 +458,15 @@ int pcg_finish_info(const void* state, double rr_true, double* info) {
 template <typename TM, typename TP, int DN, int G>
 static int pcg_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, const double* b, double* x,
                    void* ws, double tol_r, double tol_A, double delta, int maxit, int check_every,
-                   int first_check, hipStream_t s, double* info) {
+                   int first_check, hipStream_t s, double* info, PcgState* hs_buf) {
   const int Nnode = A64.Nnode;
   PcgWork w = carve(ws, Nnode, DN);
   const unsigned nbS = grid_for((int64_t)Nnode * G, A64.cap);  // node-group kernels
   const unsigned nbN = grid_for(Nnode, A64.cap);               // node kernels
   const bool mixed = (void*)AK.vals != (void*)A64.vals;
   if (int e = pcg_start<TP, DN, G>(A64, Minv, b, x, w, tol_r, tol_A, mixed ? delta * delta : 0.0, s)) return e;
-  PcgState* hs = host_state();
+  // the caller's host buffer for the state read-backs, else this thread's pinned one
+  PcgState* hs = hs_buf ? hs_buf : host_state();
   if (!hs) {
     set_error("pcg: pinned host allocation failed");
     return EFEA_ERR_HIP;
@@ -526,11 +527,15 @@ static int pcg_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, cons
 
 // mode 64: FP64 matrix and preconditioner.  mode 32: FP32 matrix values and
 // preconditioner for the Krylov products, FP64 vectors, reliable FP64
-// residual replacement.
+// residual replacement.  host_state: a host PcgState the state read-backs
+// land in (pageable memory is fine); null: this thread's pinned buffer,
+// allocated on its first use.
 int solve_pcg(int mode, int Nnode, int dof_n, int group, const int* node_ptr, const int* node_cols,
               const double* vals64, const float* vals32, const float* corr32, const unsigned char* known,
               const void* Minv, const double* b, double* x, void* ws, double tol_r, double tol_A, double delta,
-              int maxit, int check_every, int first_check, int max_blocks, hipStream_t s, double* info) {
+              int maxit, int check_every, int first_check, int max_blocks, hipStream_t s, double* info,
+              void* host_state_buf) {
+  PcgState* hsb = (PcgState*)host_state_buf;
   SpmvArgs A64{Nnode, node_ptr, node_cols, vals64, known, nullptr, max_blocks};
   SpmvArgs A32{Nnode, node_ptr, node_cols, vals32, known, corr32, max_blocks};
   if (check_every < 1) check_every = 1;
@@ -538,9 +543,9 @@ int solve_pcg(int mode, int Nnode, int dof_n, int group, const int* node_ptr, co
   if (dof_n == D && group == GV) {                                                                         \
     if (mode == 64)                                                                                        \
       return pcg_run<double, double, D, GV>(A64, A64, (const double*)Minv, b, x, ws, tol_r, tol_A, delta,  \
-                                            maxit, check_every, first_check, s, info);                     \
+                                            maxit, check_every, first_check, s, info, hsb);                \
     return pcg_run<float, float, D, GV>(A64, A32, (const float*)Minv, b, x, ws, tol_r, tol_A, delta, maxit, \
-                                        check_every, first_check, s, info);                                \
+                                        check_every, first_check, s, info, hsb);                           \
   }
   if (mode == 64 || (mode == 32 && vals32)) {
     EFEA_GROUPS(RUN)

@@ -171,6 +171,25 @@ int efea_solve_pcg(int mode, int Nnode, int dof_n, int group, const int32_t* nod
                    double tol_anorm, double replace_ratio, int maxit, int check_every, int first_check,
                    int max_blocks, void* stream, double* info);
 
+/* efea_solve_pcg without a workspace, for callers that cannot allocate
+ * device or pinned memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * solver's workspace lives in a static device array of
+ * EFEA_NOALLOC_SCRATCH_BYTES (16 MiB) per GPU, held until the process ends
+ * (systems up to about 418,000 DOF; larger ones return EFEA_ERR_SIZE before
+ * any device work), and the state read-backs go to pageable host memory on
+ * the stack; the call makes no hipMalloc, hipFree, hipHostMalloc or other
+ * allocation.  Arguments, iterates, results and info are those of
+ * efea_solve_pcg (the same kernels on the same grids: bitwise the same
+ * solution).  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (other streams or threads) race. */
+#define EFEA_NOALLOC_SCRATCH_BYTES (16u << 20)
+int efea_solve_pcg_noalloc(int mode, int Nnode, int dof_n, int group, const int32_t* node_ptr,
+                           const int32_t* node_cols, const double* values64, const float* values32,
+                           const float* corr32, const uint8_t* known, const void* Minv, const double* b, double* x,
+                           double tol_residual, double tol_anorm, double replace_ratio, int maxit, int check_every,
+                           int first_check, int max_blocks, void* stream, double* info);
+
 /* Two-level preconditioner: M^-1 = B^-1 + P Ac^-1 P^T with B the nodal block
  * Jacobi and P the rigid-body modes (2D: 3, 3D: 6, scalar: 1 per aggregate) of
  * node aggregates, zero at known DOFs; rel (Nnode, 3) holds node coordinates
