@@ -93,6 +93,8 @@ class _Simu:
         # hip_mixed: keep FP64 Krylov products for matrices under
         # device.MIXED_MIN_MATRIX_BYTES (latency-bound); False forces FP32 values
         self.mixed_adaptive = True
+        # hip_mixed two-level solves: "auto" | "values" | "refine" (device.py)
+        self.mixed_strategy = "auto"
         self.max_blocks = 0  # grid cap of the solver kernels (0: the library's default)
         # keep the two-level coarse inverse over successive solves of one matrix
         # until their iteration count grows (device.COARSE_REUSE_GROWTH)
@@ -538,7 +540,7 @@ class _Simu:
         return SolveJob(A, b_dev, x_dev, known, self._mode(), self.tol_residual, self.tol_anorm,
                         self.replace_ratio, self.maxit, self.check_every, adaptive=self.mixed_adaptive,
                         owner=self, precond=self.preconditioner, max_blocks=self.max_blocks,
-                        expected_iterations=expected,
+                        expected_iterations=expected, strategy=self.mixed_strategy,
                         coarse_target=self.coarse_target)
 
     def _solve_system_gen(self, A, b_dev: torch.Tensor, x_dev: torch.Tensor, problemType,
@@ -1087,7 +1089,10 @@ class PhaseField(_Simu):
         self.bounded_tol = 1e-10
         self.bounded_kkt_tol = 1e-9
         self.bounded_maxit = 100000
-        self.bounded_max_outer = 200
+        # active-set updates: None = max(200, number of damage DOFs) (a primal-dual
+        # update frees or binds a few variables, so the count grows with the width
+        # of the damage band in nodes)
+        self.bounded_max_outer = None
 
     def Get_problemTypes(self) -> list:
         return ["damage", "elastic"]
@@ -1308,7 +1313,9 @@ class PhaseField(_Simu):
         info = np.zeros(4)
         rc = _lib.call("efea_solve_bounded_lsq", N, ptr(Kd.node_ptr), ptr(Kd.node_cols), ptr(Kd.values), ptr(b),
                        ptr(lb), ptr(ub), ptr(x), ptr(ws), nbytes, self.bounded_tol, self.bounded_kkt_tol,
-                       self.bounded_maxit, self.bounded_max_outer, info.ctypes.data, stream_ptr())
+                       self.bounded_maxit,
+                       self.bounded_max_outer if self.bounded_max_outer is not None else max(200, N),
+                       info.ctypes.data, stream_ptr())
         self.last_bounded = {"updates": int(info[0]), "iterations": int(info[1]), "violations": int(info[2])}
         if rc == _lib.ERR_NOCONV:
             raise RuntimeError(f"BoundConstrain damage solve did not converge: {_lib.last_error()}")

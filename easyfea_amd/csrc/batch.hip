@@ -90,7 +90,7 @@ template <int DN, bool ONLY_REPLACED>
 __global__ void __launch_bounds__(BS) kb2_prolong(const BatchSys* __restrict__ sys, const int* __restrict__ map) {
   const BatchSys& S = sys[map[blockIdx.x]];
   if (ONLY_REPLACED && !S.replace) return;
-  pcg2_prolong_body<DN>(S.C, S.A64.known, S.z, S.part2, S.st, (int)blockIdx.x - S.blkA0);
+  pcg2_prolong_body<DN, double, double>(S.C, S.A64.known, S.z, S.part2, S.st, (int)blockIdx.x - S.blkA0);
 }
 
 template <typename TP, int DN>

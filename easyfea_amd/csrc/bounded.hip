@@ -70,29 +70,6 @@ __global__ void __launch_bounds__(BS) k_colnorm2(int N, const int* __restrict__ 
   }
 }
 
-// smallest positive d_i per block (+inf when none), then over the blocks
-__global__ void __launch_bounds__(BS) k_dmin_part(int N, const double* __restrict__ d, double* __restrict__ part) {
-  __shared__ double sh[BS];
-  double m = INFINITY;
-  for (int i = blockIdx.x * BS + threadIdx.x; i < N; i += gridDim.x * BS)
-    if (d[i] > 0) m = fmin(m, d[i]);
-  sh[threadIdx.x] = m;
-  __syncthreads();
-  for (int o = BS / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + o]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
-}
-
-__global__ void __launch_bounds__(BS) k_dmin_fin(const double* __restrict__ part, int nb, double* __restrict__ out) {
-  if (threadIdx.x == 0) {
-    double m = INFINITY;
-    for (int k = 0; k < nb; ++k) m = fmin(m, part[k]);
-    *out = m;
-  }
-}
-
 // y = alpha (A x) + beta c   (c may be null)
 __global__ void __launch_bounds__(BS) k_spmv(int N, const int* __restrict__ rp, const int* __restrict__ ci,
                                              const double* __restrict__ val, const double* __restrict__ x,
@@ -307,15 +284,8 @@ int solve_bounded_lsq(int N, const int* rp, const int* ci, const double* val, co
   unsigned viol = 0;
   double gref = -1.0;
   hipLaunchKernelGGL(k_colnorm2, dim3(nb), dim3(BS), 0, s, N, rp, val, w.d);
-  hipLaunchKernelGGL(k_dmin_part, dim3(nb), dim3(BS), 0, s, N, (const double*)w.d, w.part);
-  hipLaunchKernelGGL(k_dmin_fin, dim3(1), dim3(BS), 0, s, (const double*)w.part, (int)nb, w.sc + SC_TMP);
-  hipMemcpyAsync(hp.gam + 1, w.sc + SC_TMP, sizeof(double), hipMemcpyDeviceToHost, s);
   hipLaunchKernelGGL(k_clamp, dim3(nb), dim3(BS), 0, s, N, lb, ub, x);
   if (hipStreamSynchronize(s) != hipSuccess) return check_launch("solve_bounded_lsq");
-  // a pass may also stop once no free variable can be off by more than
-  // kkt_tol / 2: |g_i| / d_i <= sqrt(gamma / d_i) <= sqrt(gamma / d_min)
-  const double dmin = hp.gam[1];
-  const double enough = dmin < INFINITY ? 0.25 * kkt_tol * kkt_tol * dmin : 0.0;
   for (;; ++outer) {
     // gradient g = A (A x - b) = -A s with s = b - A x
     hipLaunchKernelGGL(k_spmv, dim3(nb), dim3(BS), 0, s, N, rp, ci, val, (const double*)x, -1.0, 1.0, b, w.s);
@@ -328,6 +298,10 @@ int solve_bounded_lsq(int N, const int* rp, const int* ci, const double* val, co
     if (hipStreamSynchronize(s) != hipSuccess) return check_launch("solve_bounded_lsq");
     viol = hp.cnt[0];
     if (viol == 0) {
+      // free variables may sit up to kkt_tol outside their bounds: clamp
+      // them (a move of at most kkt_tol; the first-order conditions then
+      // hold within 2 kkt_tol)
+      hipLaunchKernelGGL(k_clamp, dim3(nb), dim3(BS), 0, s, N, lb, ub, x);
       status = EFEA_OK;
       break;
     }
@@ -343,7 +317,9 @@ int solve_bounded_lsq(int N, const int* rp, const int* ci, const double* val, co
     // each pass reduces the scaled free-set gradient norm by tol from where it starts
     if (gref < 0) gref = hp.gam[0];
     double gam = hp.gam[0];
-    const double stop = fmax(tol * tol * gam, enough);
+    // (a bound on the scaled gradient alone does not bound the distance to
+    // the minimiser on ill-conditioned systems, so the pass runs to tol)
+    const double stop = tol * tol * gam;
     for (int it = 0; gam > stop && it < maxit;) {
       for (int k = 0; k < POLL; ++k, ++it) {
         hipLaunchKernelGGL(k_spmv_tt, dim3(nb), dim3(BS), 0, s, N, rp, ci, val, (const double*)w.p, w.t, w.part);

@@ -421,26 +421,28 @@ int coarse_ld(int nagg, int dof_n) {
 // ---------------------------------------------------------------------------
 // Per-iteration kernels on the aggregate grid (block = aggregate).
 
-template <typename TP, int DN>
+// TV: vector type (float in the refinement mode's inner solves); TA: type of
+// the coarse inverse's values.
+template <typename TP, int DN, typename TV = double>
 __global__ void __launch_bounds__(BS)
 k_pcg2_step2(CoarseArgs C, const unsigned char* __restrict__ known, const TP* __restrict__ Minv,
-             double* __restrict__ x, double* __restrict__ r, double* __restrict__ z, const double* __restrict__ P0,
-             const double* __restrict__ P1, const double* __restrict__ q, const double* __restrict__ part1, int nb1,
+             TV* __restrict__ x, TV* __restrict__ r, TV* __restrict__ z, const TV* __restrict__ P0,
+             const TV* __restrict__ P1, const TV* __restrict__ q, const double* __restrict__ part1, int nb1,
              double* __restrict__ part2, PcgState* st) {
   pcg2_step2_body<TP, DN>(C, known, Minv, x, r, z, P0, P1, q, part1, nb1, part2, st, blockIdx.x);
 }
 
-template <int DN>
+template <int DN, typename TV = double, typename TA = double>
 __global__ void __launch_bounds__(BS)
-k_pcg2_prolong(CoarseArgs C, const unsigned char* __restrict__ known, double* __restrict__ z,
+k_pcg2_prolong(CoarseArgs C, const unsigned char* __restrict__ known, TV* __restrict__ z,
                double* __restrict__ part2, PcgState* st) {
-  pcg2_prolong_body<DN>(C, known, z, part2, st, blockIdx.x);
+  pcg2_prolong_body<DN, TV, TA>(C, known, z, part2, st, blockIdx.x);
 }
 
-template <typename TP, int DN>
+template <typename TP, int DN, typename TV = double>
 __global__ void __launch_bounds__(BS)
 k_pcg2_init(CoarseArgs C, const unsigned char* __restrict__ known, const TP* __restrict__ Minv,
-            const double* __restrict__ r, double* __restrict__ z, double* __restrict__ P0,
+            const TV* __restrict__ r, TV* __restrict__ z, TV* __restrict__ P0,
             const double* __restrict__ partI, int nbI, double* __restrict__ part2, PcgState* st) {
   pcg2_init_body<TP, DN>(C, known, Minv, r, z, P0, partI, nbI, part2, st, blockIdx.x);
 }
@@ -454,10 +456,10 @@ k_pcg2_replace_z(CoarseArgs C, const unsigned char* __restrict__ known, const TP
 
 // Step 1 on a two-level solve: the one-level kernel (partials of step 2 now
 // come from nagg blocks).
-template <typename TM, int DN, int G>
+template <typename TM, int DN, int G, typename TV = double>
 __global__ void __launch_bounds__(BS)
-k_pcg2_step1(SpmvArgs A, const double* __restrict__ z, const double* __restrict__ P0, const double* __restrict__ P1,
-             double* __restrict__ q, const double* __restrict__ part2, int nb2, double* __restrict__ part1,
+k_pcg2_step1(SpmvArgs A, const TV* __restrict__ z, const TV* __restrict__ P0, const TV* __restrict__ P1,
+             TV* __restrict__ q, const double* __restrict__ part2, int nb2, double* __restrict__ part1,
              PcgState* st) {
   const bool one_pass = (int64_t)gridDim.x * BS >= (int64_t)A.Nnode * G;
   pcg_step1_body<TM, DN, G>(A, z, P0, P1, q, part2, nb2, part1, st,
@@ -621,6 +623,232 @@ static int pcg2_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, con
   return pcg_finish_info(&fin, rr_true, info);
 }
 
+// ---------------------------------------------------------------------------
+// Refinement mode (mode 33): FP64 iterative refinement around an FP32 inner
+// two-level PCG.  Each outer step forms r = b - A64 x in FP64 (masked rows)
+// and its FP32 copy, solves A32 c = r from c = 0 with FP32 vectors, FP32
+// matrix values (row-sum corrected), FP32 block Jacobi and FP32 Ac^-1, to a
+// relative accuracy tol_in, and updates x += c in FP64.  It stops when the
+// FP64 residual is below tol_r of the first one, or when the error estimate
+// after the update - the inner solve's own A-norm error estimate plus the
+// FP32 operator's share, taken as 1e-3 of the correction (measured: the
+// refinement contracts like the FP64 solve) - is below tol_A of x's energy
+// (x.A64 x from the residual pass).  A residual that grows between outer
+// steps reports divergence (status 3), for the caller's FP64 fallback.
+
+// r64 = b - A64 x on the masked rows -> r32; partials of r.r and x.A x.
+template <int DN, int G>
+__global__ void __launch_bounds__(BS)
+k_ref_residual(SpmvArgs A64, const double* __restrict__ x, const double* __restrict__ b, float* __restrict__ r32,
+               double* __restrict__ part) {
+  __shared__ double shn[2][BS / 64];
+  double red[2] = {0, 0};
+  const int64_t lanes = (int64_t)A64.Nnode * G;
+  for (int64_t gid = (int64_t)blockIdx.x * BS + threadIdx.x; gid < lanes; gid += (int64_t)gridDim.x * BS) {
+    const int n = (int)(gid / G), g = (int)(gid % G);
+    const int p0 = A64.node_ptr[n], cnt = A64.node_ptr[n + 1] - p0;
+    double sum[1][DN];
+#pragma unroll
+    for (int a = 0; a < DN; ++a) sum[0][a] = 0;
+    gather_rows<double, DN, G, 1>((const double*)A64.vals, A64.node_cols, p0, cnt, g,
+                                  [&](int c, double (&xv)[1][DN]) {
+#pragma unroll
+                                    for (int bb = 0; bb < DN; ++bb) xv[0][bb] = x[(int64_t)c * DN + bb];
+                                  },
+                                  sum);
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) sum[0][a] += __shfl_xor(sum[0][a], o, G);
+    }
+    if (g == 0) {
+#pragma unroll
+      for (int a = 0; a < DN; ++a) {
+        const int64_t i = (int64_t)n * DN + a;
+        const double xi = x[i];
+        red[1] += xi * sum[0][a];
+        double ri = b[i] - sum[0][a];
+        if (A64.known && A64.known[i]) ri = 0;
+        r32[i] = (float)ri;
+        red[0] += ri * ri;
+      }
+    }
+  }
+  block_sum_all<2, BS>(red, shn);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = red[0];
+    part[2 * blockIdx.x + 1] = red[1];
+  }
+}
+
+__global__ void __launch_bounds__(BS) k_ref_sums(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  __shared__ double sh[BS / 64];
+  const double rr = reduce_all(part, nb, 2, 0, sh);
+  const double xax = reduce_all(part, nb, 2, 1, sh);
+  if (threadIdx.x == 0) {
+    out[0] = rr;
+    out[1] = xax;
+  }
+}
+
+// x += c (c = 0 at known DOFs); c is cleared for the next inner solve.
+__global__ void __launch_bounds__(BS) k_ref_update(int64_t n, float* __restrict__ c, double* __restrict__ x) {
+  for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+    x[i] += (double)c[i];
+    c[i] = 0.0f;
+  }
+}
+
+__global__ void __launch_bounds__(BS) k_fill_f32(int64_t n, float* __restrict__ a) {
+  for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) a[i] = 0.0f;
+}
+
+// Inner state: relative tolerances, and x0.A x0 standing for the absolute
+// A-norm target, so the stopping test max(Sall, xAx0) * tol_in^2 stops the
+// inner solve at the outer target as well.
+__global__ void k_ref_state(PcgState* st, double tol_in2, double xax0) {
+  if (threadIdx.x == 0) {
+    st->rz = st->rz_prev = st->alpha = st->beta = st->pq = 0;
+    st->rr = st->rrmax = st->rrmax_prev = st->bb = 0;
+    st->tolr2 = tol_in2;
+    st->tolA2 = tol_in2;
+    st->delta2 = 0;
+    st->Sall = st->Sw = 0;
+    st->xAx0 = xax0;
+    for (int i = 0; i < RING; ++i) st->ring[i] = 0;
+    st->head = st->done = st->iters = st->nrepl = 0;
+    st->fresh = 1;
+    st->justrep = 0;
+    st->par = 0;
+    st->par_next = 1;
+  }
+}
+
+struct RefWork {
+  float *r, *c, *z, *q, *P0, *P1;
+  double *part1, *part2, *partR, *small, *rc;
+  PcgState* st;
+};
+
+// The two-level workspace with each FP64 vector slot holding two FP32 vectors.
+static RefWork carve_ref(void* ws, int Nnode, int dof_n, int nagg) {
+  const Pcg2Work v = carve2(ws, Nnode, dof_n, nagg);
+  const int64_t n = (int64_t)Nnode * dof_n;
+  RefWork w;
+  w.r = (float*)v.w.r;
+  w.c = w.r + n;
+  w.z = (float*)v.w.z;
+  w.q = w.z + n;
+  w.P0 = (float*)v.w.P0;
+  w.P1 = w.P0 + n;
+  w.part1 = v.w.part1;
+  w.part2 = v.w.part2;
+  w.small = v.w.small;
+  w.st = v.w.st;
+  w.rc = v.rc;
+  const int nm = dof_n == 1 ? 1 : (dof_n == 2 ? 3 : 6);
+  w.partR = (double*)((char*)v.rc + align256(8 * (size_t)nagg * nm));
+  return w;
+}
+
+template <int DN, int G>
+static int pcg2_refine(const SpmvArgs& A64, const SpmvArgs& A32, const float* Minv32, const CoarseArgs& C32,
+                       const double* b, double* x, void* ws, double tol_r, double tol_A, double tol_in, int maxit,
+                       int check_every, hipStream_t s, double* info, PcgState* hs) {
+  const int Nnode = A64.Nnode;
+  const int64_t n = (int64_t)Nnode * DN;
+  const RefWork w = carve_ref(ws, Nnode, DN, C32.nagg);
+  CoarseArgs C = C32;
+  C.rc = w.rc;
+  const unsigned nbS = grid_for((int64_t)Nnode * G, A64.cap);
+  const unsigned nbA = (unsigned)C.nagg;
+  const unsigned nbV = grid_for(n, A64.cap);
+  constexpr double MU2 = 1e-6;  // (FP32 operator's share of the error per step)^2
+  double hsum[2] = {0, 0}, rr0 = -1, rr_prev = -1;
+  int total = 0, outer = 0, status = 0;
+  double est = INFINITY;  // estimated squared A-norm error after the last update
+  hipLaunchKernelGGL(k_fill_f32, dim3(nbV), dim3(BS), 0, s, n, w.c);
+  while (true) {
+    hipLaunchKernelGGL((k_ref_residual<DN, G>), dim3(nbS), dim3(BS), 0, s, A64, (const double*)x, b, w.r, w.partR);
+    hipLaunchKernelGGL(k_ref_sums, dim3(1), dim3(BS), 0, s, (const double*)w.partR, (int)nbS, w.small);
+    if (hipMemcpyAsync(hsum, w.small, 2 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      set_error("pcg2 refinement: residual readback failed");
+      return EFEA_ERR_HIP;
+    }
+    const double rr = hsum[0], xax = hsum[1];
+    if (rr0 < 0) rr0 = rr;
+    if (!isfinite(rr) || !isfinite(xax)) {
+      status = 2;
+      break;
+    }
+    if (rr <= tol_r * tol_r * rr0 || rr == 0 || (outer > 0 && est <= tol_A * tol_A * xax)) {
+      status = 1;
+      break;
+    }
+    if (outer > 0 && rr > 4.0 * rr_prev) {  // the correction made the residual grow: diverged
+      status = 3;
+      break;
+    }
+    if (total >= maxit) break;
+    rr_prev = rr;
+    // inner solve A32 c = r, c from 0; its stopping test relative to
+    // max(its Sall, (tol_A / tol_in)^2 x.A x), i.e. also at the outer target
+    const double xax0 = xax > 0 ? xax * (tol_A / tol_in) * (tol_A / tol_in) : 0.0;
+    hipLaunchKernelGGL(k_ref_state, dim3(1), dim3(64), 0, s, w.st, tol_in * tol_in, xax0);
+    hipLaunchKernelGGL((k_pcg2_init<float, DN, float>), dim3(nbA), dim3(BS), 0, s, C, A64.known, Minv32,
+                       (const float*)w.r, w.z, w.P0, (const double*)nullptr, 0, w.part2, w.st);
+    hipLaunchKernelGGL((k_pcg2_prolong<DN, float, float>), dim3(nbA), dim3(BS), 0, s, C, A64.known, w.z, w.part2,
+                       w.st);
+    hipLaunchKernelGGL(k_pcg2_init_scalars, dim3(1), dim3(BS), 0, s, (const double*)w.part2, (int)nbA, w.st);
+    int launched = 0;
+    while (true) {
+      for (int k = 0; k < check_every; ++k) {
+        hipLaunchKernelGGL((k_pcg2_step1<float, DN, G, float>), dim3(nbS), dim3(BS), 0, s, A32, (const float*)w.z,
+                           (const float*)w.P0, (const float*)w.P1, w.q, (const double*)w.part2, (int)nbA, w.part1,
+                           w.st);
+        hipLaunchKernelGGL((k_pcg2_step2<float, DN, float>), dim3(nbA), dim3(BS), 0, s, C, A64.known, Minv32, w.c,
+                           w.r, w.z, (const float*)w.P0, (const float*)w.P1, (const float*)w.q,
+                           (const double*)w.part1, (int)nbS, w.part2, w.st);
+        hipLaunchKernelGGL((k_pcg2_prolong<DN, float, float>), dim3(nbA), dim3(BS), 0, s, C, A64.known, w.z,
+                           w.part2, w.st);
+      }
+      launched += check_every;
+      if (int e = check_launch("pcg2_refine_inner")) return e;
+      if (hipMemcpyAsync(hs, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess) {
+        set_error("pcg2 refinement: state readback failed");
+        return EFEA_ERR_HIP;
+      }
+      if (hs->done || total + launched >= maxit) break;
+    }
+    total += hs->iters;
+    if (hs->done == 2) {  // breakdown of the FP32 inner solve
+      status = 2;
+      break;
+    }
+    // error after the update: the inner's own estimate (its window sum once
+    // it has one, else everything it did) plus the FP32 operator's share
+    est = (hs->iters >= RING ? hs->Sw : hs->Sall) + MU2 * hs->Sall;
+    hipLaunchKernelGGL(k_ref_update, dim3(nbV), dim3(BS), 0, s, n, w.c, x);
+    ++outer;
+  }
+  info[0] = total;
+  info[1] = rr0 > 0 ? sqrt(hsum[0] / rr0) : 0.0;
+  info[2] = outer;
+  info[3] = status;
+  if (status == 2 || status == 3) {
+    set_error("pcg2 refinement: %s after %d outer steps (%d inner iterations)",
+              status == 2 ? "breakdown or non-finite values" : "divergence", outer, total);
+    return EFEA_ERR_NOCONV;
+  }
+  if (status != 1) {
+    set_error("pcg2 refinement: not converged after %d inner iterations (relative residual %.3e)", total, info[1]);
+    return EFEA_ERR_NOCONV;
+  }
+  return check_launch("pcg2_refine");
+}
+
 int step1_bench_init(const Step1Bench& w, int nb2, int64_t ndof, hipStream_t s);
 
 // `reps` launches of the two-level solve's step-1 kernel (k_pcg2_step1: r.z
@@ -664,6 +892,19 @@ int solve_pcg2(int mode, int Nnode, int dof_n, int group, const int* node_ptr, c
   SpmvArgs A32{Nnode, node_ptr, node_cols, vals32, known, corr32, max_blocks};
   CoarseArgs C{nagg, agg_ptr, agg_nodes, rel, Ainv, ld, nullptr};
   if (check_every < 1) check_every = 1;
+  if (mode == 33) {
+    // refinement: Minv and Ainv are the FP32 block-Jacobi and coarse inverses;
+    // delta (replace_ratio) is the inner solves' relative accuracy
+    CoarseArgs C32{nagg, agg_ptr, agg_nodes, rel, Ainv, ld, nullptr};
+    PcgState hs;
+    const double tol_in = delta > 0 ? delta : 1e-5;
+#define RUNR(D, GV)                                                                                          \
+    if (dof_n == D && group == GV && vals32)                                                                \
+      return pcg2_refine<D, GV>(A64, A32, (const float*)Minv, C32, b, x, ws, tol_r, tol_A, tol_in, maxit,    \
+                                check_every, s, info, &hs);
+    EFEA_GROUPS(RUNR)
+#undef RUNR
+  }
 #define RUN2(D, GV)                                                                                            \
   if (dof_n == D && group == GV) {                                                                            \
     if (mode == 64)                                                                                           \

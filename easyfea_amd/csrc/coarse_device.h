@@ -35,17 +35,19 @@ struct CoarseArgs {
   const int* agg_ptr;
   const int* agg_nodes;
   const double* rel;
-  const double* Ainv;  // ld x ld
+  const void* Ainv;  // ld x ld, FP64 (FP32 in the refinement mode's inner solves)
   int ld;
   double* rc;  // nagg * NM
 };
 
 // Step 2 with restriction: alpha = rz / pq ; x += alpha p ; r -= alpha q ;
 // z = B^-1 r ; partials of r.z (block Jacobi part) and r.r ; rc_I = P_I^T r.
-template <typename TP, int DN>
+// TV: the vector type (double; float in the refinement mode's inner solves,
+// with the arithmetic in double registers).
+template <typename TP, int DN, typename TV>
 __device__ inline void pcg2_step2_body(CoarseArgs C, const unsigned char* __restrict__ known, const TP* __restrict__ Minv,
-             double* __restrict__ x, double* __restrict__ r, double* __restrict__ z, const double* __restrict__ P0,
-             const double* __restrict__ P1, const double* __restrict__ q, const double* __restrict__ part1, int nb1,
+             TV* __restrict__ x, TV* __restrict__ r, TV* __restrict__ z, const TV* __restrict__ P0,
+             const TV* __restrict__ P1, const TV* __restrict__ q, const double* __restrict__ part1, int nb1,
              double* __restrict__ part2, PcgState* st, int blk) {
   constexpr int NM = Modes<DN>::NM;
   // the state test follows the loads and the block reduction (a launch after
@@ -57,7 +59,7 @@ __device__ inline void pcg2_step2_body(CoarseArgs C, const unsigned char* __rest
   double ppq = pp1[0];
   const int pn = st->par_next;
   const double rz = st->rz;
-  const double* __restrict__ p = pn ? P1 : P0;
+  const TV* __restrict__ p = pn ? P1 : P0;
   const int k0 = C.agg_ptr[blk], k1 = C.agg_ptr[blk + 1];
   // the first node's operands are loaded before the reduction that yields alpha
   const int kf = k0 + threadIdx.x;
@@ -68,10 +70,10 @@ __device__ inline void pcg2_step2_body(CoarseArgs C, const unsigned char* __rest
 #pragma unroll
     for (int a = 0; a < DN; ++a) {
       const int64_t i = (int64_t)nf * DN + a;
-      xf[a] = x[i];
-      pf[a] = p[i];
-      rf[a] = r[i];
-      qf[a] = q[i];
+      xf[a] = (double)x[i];
+      pf[a] = (double)p[i];
+      rf[a] = (double)r[i];
+      qf[a] = (double)q[i];
     }
 #pragma unroll
     for (int e = 0; e < DN * DN; ++e) Mf[e] = (double)Minv[(int64_t)nf * DN * DN + e];
@@ -105,9 +107,9 @@ __device__ inline void pcg2_step2_body(CoarseArgs C, const unsigned char* __rest
 #pragma unroll
     for (int a = 0; a < DN; ++a) {
       const int64_t i = (int64_t)nf * DN + a;
-      x[i] = xf[a] + al * pf[a];
+      x[i] = (TV)(xf[a] + al * pf[a]);
       r2[a] = rf[a] - al * qf[a];
-      r[i] = r2[a];
+      r[i] = (TV)r2[a];
     }
 #pragma unroll
     for (int a = 0; a < DN; ++a) {
@@ -115,7 +117,7 @@ __device__ inline void pcg2_step2_body(CoarseArgs C, const unsigned char* __rest
 #pragma unroll
       for (int b = 0; b < DN; ++b) zz += Mf[a * DN + b] * r2[b];
       if (known && known[(int64_t)nf * DN + a]) zz = 0;
-      z[(int64_t)nf * DN + a] = zz;
+      z[(int64_t)nf * DN + a] = (TV)zz;
       s_rz += r2[a] * zz;
       s_rr += r2[a] * r2[a];
 #pragma unroll
@@ -128,15 +130,15 @@ __device__ inline void pcg2_step2_body(CoarseArgs C, const unsigned char* __rest
 #pragma unroll
     for (int a = 0; a < DN; ++a) {
       const int64_t i = (int64_t)n * DN + a;
-      x[i] += al * p[i];
-      r2[a] = r[i] - al * q[i];
-      r[i] = r2[a];
+      x[i] = (TV)((double)x[i] + al * (double)p[i]);
+      r2[a] = (double)r[i] - al * (double)q[i];
+      r[i] = (TV)r2[a];
     }
     apply_bj<TP, DN>(Minv, known, n, r2, zv);
     const double* rn = C.rel + 3 * (int64_t)n;
 #pragma unroll
     for (int a = 0; a < DN; ++a) {
-      z[(int64_t)n * DN + a] = zv[a];
+      z[(int64_t)n * DN + a] = (TV)zv[a];
       s_rz += r2[a] * zv[a];
       s_rr += r2[a] * r2[a];
 #pragma unroll
@@ -159,8 +161,10 @@ __device__ inline void pcg2_step2_body(CoarseArgs C, const unsigned char* __rest
 }
 
 // z += P_I zc_I with zc_I = (Ac^-1)_I rc ; r.z partial += rc_I . zc_I.
-template <int DN>
-__device__ inline void pcg2_prolong_body(CoarseArgs C, const unsigned char* __restrict__ known, double* __restrict__ z,
+// TV: vector type; TA: type of the Ac^-1 values (float in the refinement
+// mode's inner solves: half the bytes of the dense rows).
+template <int DN, typename TV, typename TA>
+__device__ inline void pcg2_prolong_body(CoarseArgs C, const unsigned char* __restrict__ known, TV* __restrict__ z,
                double* __restrict__ part2, PcgState* st, int blk) {
   constexpr int NM = Modes<DN>::NM;
   // the state test follows the loads and the coarse rows
@@ -178,7 +182,7 @@ __device__ inline void pcg2_prolong_body(CoarseArgs C, const unsigned char* __re
     for (int d = 0; d < 3; ++d) relf[d] = C.rel[3 * (int64_t)nf + d];
 #pragma unroll
     for (int a = 0; a < DN; ++a) {
-      zf[a] = z[(int64_t)nf * DN + a];
+      zf[a] = (double)z[(int64_t)nf * DN + a];
       kfree[a] = !(known && known[(int64_t)nf * DN + a]);
     }
   }
@@ -187,7 +191,7 @@ __device__ inline void pcg2_prolong_body(CoarseArgs C, const unsigned char* __re
   // columns per pass with their loads issued together, then one block
   // reduction of the NM sums (a wavefront per row walking the row alone paid
   // one load latency per 64 columns).
-  const double* __restrict__ rows = C.Ainv + (int64_t)(blk * NM) * C.ld;
+  const TA* __restrict__ rows = (const TA*)C.Ainv + (int64_t)(blk * NM) * C.ld;
   double zc[NM];
 #pragma unroll
   for (int m = 0; m < NM; ++m) zc[m] = 0;
@@ -199,7 +203,7 @@ __device__ inline void pcg2_prolong_body(CoarseArgs C, const unsigned char* __re
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int m = 0; m < NM; ++m) av[u][m] = rows[(int64_t)m * C.ld + c + u * BS];
+      for (int m = 0; m < NM; ++m) av[u][m] = (double)rows[(int64_t)m * C.ld + c + u * BS];
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -208,7 +212,7 @@ __device__ inline void pcg2_prolong_body(CoarseArgs C, const unsigned char* __re
   for (; c < nc; c += BS) {
     const double rv = C.rc[c];
 #pragma unroll
-    for (int m = 0; m < NM; ++m) zc[m] += rows[(int64_t)m * C.ld + c] * rv;
+    for (int m = 0; m < NM; ++m) zc[m] += (double)rows[(int64_t)m * C.ld + c] * rv;
   }
   block_sum_all<NM, BS>(zc, shz);
   if (st->done) return;
@@ -219,7 +223,7 @@ __device__ inline void pcg2_prolong_body(CoarseArgs C, const unsigned char* __re
       double v = 0;
 #pragma unroll
       for (int m = 0; m < NM; ++m) v += pmode<DN>(relf, a, m) * zc[m];
-      z[(int64_t)nf * DN + a] = zf[a] + v;
+      z[(int64_t)nf * DN + a] = (TV)(zf[a] + v);
     }
   }
   for (int k = kf + BS; k < k1; k += BS) {
@@ -232,7 +236,7 @@ __device__ inline void pcg2_prolong_body(CoarseArgs C, const unsigned char* __re
       double v = 0;
 #pragma unroll
       for (int m = 0; m < NM; ++m) v += pmode<DN>(rn, a, m) * zc[m];
-      z[i] += v;
+      z[i] = (TV)((double)z[i] + v);
     }
   }
   if (threadIdx.x == 0) {
@@ -245,13 +249,13 @@ __device__ inline void pcg2_prolong_body(CoarseArgs C, const unsigned char* __re
 
 // Initial z = B^-1 r, P0 = 0, partials of r.z and r.r and the restriction;
 // block 0 also folds the x0.A x0 partials into the state.
-template <typename TP, int DN>
+template <typename TP, int DN, typename TV>
 __device__ inline void pcg2_init_body(CoarseArgs C, const unsigned char* __restrict__ known, const TP* __restrict__ Minv,
-            const double* __restrict__ r, double* __restrict__ z, double* __restrict__ P0,
+            const TV* __restrict__ r, TV* __restrict__ z, TV* __restrict__ P0,
             const double* __restrict__ partI, int nbI, double* __restrict__ part2, PcgState* st, int blk) {
   constexpr int NM = Modes<DN>::NM;
   __shared__ double sh[BS / 64];
-  if (blk == 0) {
+  if (blk == 0 && partI) {
     const double xax = reduce_all(partI, nbI, 1, 0, sh);
     if (threadIdx.x == 0) st->xAx0 = xax;
   }
@@ -263,13 +267,13 @@ __device__ inline void pcg2_init_body(CoarseArgs C, const unsigned char* __restr
     const int n = C.agg_nodes[k];
     double rv[DN], zv[DN];
 #pragma unroll
-    for (int a = 0; a < DN; ++a) rv[a] = r[(int64_t)n * DN + a];
+    for (int a = 0; a < DN; ++a) rv[a] = (double)r[(int64_t)n * DN + a];
     apply_bj<TP, DN>(Minv, known, n, rv, zv);
     const double* rn = C.rel + 3 * (int64_t)n;
 #pragma unroll
     for (int a = 0; a < DN; ++a) {
-      z[(int64_t)n * DN + a] = zv[a];
-      P0[(int64_t)n * DN + a] = 0.0;
+      z[(int64_t)n * DN + a] = (TV)zv[a];
+      P0[(int64_t)n * DN + a] = (TV)0;
       s_rz += rv[a] * zv[a];
       s_rr += rv[a] * rv[a];
 #pragma unroll
@@ -346,8 +350,9 @@ static inline size_t pcg2_workspace_size(int Nnode, int dof_n, int nagg) {
   const int nb = nagg > MAXB ? nagg : MAXB;
   const int nm = dof_n == 1 ? 1 : (dof_n == 2 ? 3 : 6);
   const size_t vb = align256(8 * (size_t)Nnode * dof_n);
+  // (+ the refinement mode's three partials per block of its FP64 residual pass)
   return 5 * vb + align256(8 * (size_t)MAXB) * 2 + align256(16 * (size_t)nb) + align256(sizeof(PcgState)) + 256 +
-         align256(8 * (size_t)nagg * nm);
+         align256(8 * (size_t)nagg * nm) + align256(24 * (size_t)MAXB);
 }
 
 // The one-level work arrays (rc in `small`'s place is separate) of a

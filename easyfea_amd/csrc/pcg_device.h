@@ -254,10 +254,12 @@ __device__ inline void pcg_close_iteration(PcgState* st, double rz_new, double r
 // Latency: the partial loads and the first node group's products (A z and
 // A p_old, kept apart because beta is not known yet) are issued before the
 // block reduction that yields beta.
-template <typename TM, int DN, int G>
-__device__ inline void pcg_step1_body(const SpmvArgs& A, const double* __restrict__ z,
-                                      const double* __restrict__ P0, const double* __restrict__ P1,
-                                      double* __restrict__ q, const double* __restrict__ part2, int nb2,
+// TV: the vector type (double; float for the FP32 inner solves of the
+// refinement mode, whose products are still formed in double registers).
+template <typename TM, int DN, int G, typename TV>
+__device__ inline void pcg_step1_body(const SpmvArgs& A, const TV* __restrict__ z,
+                                      const TV* __restrict__ P0, const TV* __restrict__ P1,
+                                      TV* __restrict__ q, const double* __restrict__ part2, int nb2,
                                       double* __restrict__ part1, PcgState* st, int blk, int nblk) {
   const int done = st->done, par = st->par;
   const bool fresh = st->fresh;
@@ -281,8 +283,8 @@ __device__ inline void pcg_step1_body(const SpmvArgs& A, const double* __restric
   __shared__ double sh[BS / 64];
   __shared__ double sh2[BS / 64];
   __shared__ double bc[2];
-  const double* __restrict__ pold = par ? P1 : P0;
-  double* __restrict__ pnew = (double*)(par ? P0 : P1);
+  const TV* __restrict__ pold = par ? P1 : P0;
+  TV* __restrict__ pnew = (TV*)(par ? P0 : P1);
   const TM* __restrict__ vals = (const TM*)A.vals;
   double sz[DN], sp[DN], zo[DN], po[DN];
 #pragma unroll
@@ -296,8 +298,8 @@ __device__ inline void pcg_step1_body(const SpmvArgs& A, const double* __restric
                               [&](int c, double (&xv)[2][DN]) {
 #pragma unroll
                                 for (int bb = 0; bb < DN; ++bb) {
-                                  xv[0][bb] = z[(int64_t)c * DN + bb];
-                                  xv[1][bb] = pold[(int64_t)c * DN + bb];
+                                  xv[0][bb] = (double)z[(int64_t)c * DN + bb];
+                                  xv[1][bb] = (double)pold[(int64_t)c * DN + bb];
                                 }
                               },
                               acc);
@@ -310,8 +312,8 @@ __device__ inline void pcg_step1_body(const SpmvArgs& A, const double* __restric
     if (g0 == 0) {
 #pragma unroll
       for (int a = 0; a < DN; ++a) {
-        zo[a] = z[(int64_t)n0 * DN + a];
-        po[a] = pold[(int64_t)n0 * DN + a];
+        zo[a] = (double)z[(int64_t)n0 * DN + a];
+        po[a] = (double)pold[(int64_t)n0 * DN + a];
       }
     }
   }
@@ -351,7 +353,7 @@ __device__ inline void pcg_step1_body(const SpmvArgs& A, const double* __restric
 #pragma unroll
       for (int a = 0; a < DN; ++a) {
         const int64_t r = (int64_t)n0 * DN + a;
-        pnew[r] = pn[a];
+        pnew[r] = (TV)pn[a];
         double qv = sz[a] + beta * sp[a];
         if (A.corr) {
           const TM* cr = (const TM*)A.corr + (int64_t)n0 * DN * DN + a * DN;
@@ -359,7 +361,7 @@ __device__ inline void pcg_step1_body(const SpmvArgs& A, const double* __restric
           for (int b = 0; b < DN; ++b) qv += (double)cr[b] * pn[b];
         }
         if (A.known && A.known[r]) qv = 0;
-        q[r] = qv;
+        q[r] = (TV)qv;
         d += pn[a] * qv;
       }
     }
@@ -387,7 +389,8 @@ __device__ inline void pcg_step1_body(const SpmvArgs& A, const double* __restric
                               [&](int c, double (&xv)[1][DN]) {
 #pragma unroll
                                 for (int bb = 0; bb < DN; ++bb)
-                                  xv[0][bb] = z[(int64_t)c * DN + bb] + beta * pold[(int64_t)c * DN + bb];
+                                  xv[0][bb] = (double)z[(int64_t)c * DN + bb] +
+                                              beta * (double)pold[(int64_t)c * DN + bb];
                               },
                               sum);
 #pragma unroll
@@ -398,11 +401,12 @@ __device__ inline void pcg_step1_body(const SpmvArgs& A, const double* __restric
     if (g == 0) {
       double pn[DN];
 #pragma unroll
-      for (int a = 0; a < DN; ++a) pn[a] = z[(int64_t)n * DN + a] + beta * pold[(int64_t)n * DN + a];
+      for (int a = 0; a < DN; ++a)
+        pn[a] = (double)z[(int64_t)n * DN + a] + beta * (double)pold[(int64_t)n * DN + a];
 #pragma unroll
       for (int a = 0; a < DN; ++a) {
         const int64_t r = (int64_t)n * DN + a;
-        pnew[r] = pn[a];
+        pnew[r] = (TV)pn[a];
         double qv = sum[0][a];
         if (A.corr) {
           const TM* cr = (const TM*)A.corr + (int64_t)n * DN * DN + a * DN;
@@ -410,7 +414,7 @@ __device__ inline void pcg_step1_body(const SpmvArgs& A, const double* __restric
           for (int b = 0; b < DN; ++b) qv += (double)cr[b] * pn[b];
         }
         if (A.known && A.known[r]) qv = 0;
-        q[r] = qv;
+        q[r] = (TV)qv;
         d += pn[a] * qv;
       }
     }

@@ -375,12 +375,28 @@ def choose_preconditioner(A: "DeviceCsr", precond: str = "auto") -> str:
 class SolveInfo:
     iterations: int
     rel_residual: float
-    replacements: int
+    replacements: int  # residual replacements; outer steps of a refinement solve
     status: int
     mode: str
     fallback: bool = False
     precond: str = "bj"
     coarse_reused: bool = False
+    refine: bool = False  # FP32 inner PCG inside FP64 iterative refinement
+
+
+# Mixed-precision strategies of a single two-level solve:
+#   "values": FP32 matrix values (and block Jacobi) in the Krylov products,
+#             FP64 vectors, FP64 residual replacement at every state read-back;
+#   "refine": FP64 iterative refinement (r = b - A64 x and x += c in FP64)
+#             around an FP32 inner PCG (FP32 vectors, values, block Jacobi and
+#             coarse inverse), each inner solve to REFINE_INNER_TOL;
+#   "auto":   "refine" for matrices of at least REFINE_MIN_MATRIX_BYTES (the
+#             products are bandwidth-bound), "values" below.
+# Block-Jacobi solves and the continuous-batching engine use "values" (the
+# serving requests ask for it, so that a request alone and in a batch run the
+# same algorithm).
+REFINE_INNER_TOL = 1e-4
+REFINE_MIN_MATRIX_BYTES = 128 * 2**20
 
 
 # Lagged coarse inverse.  Any symmetric positive definite Ac^-1 keeps
@@ -402,6 +418,7 @@ class _CoarseCache:
     ref_iterations: Optional[int] = None
     uses: int = 0
     stale: bool = False
+    Ainv32: Optional[torch.Tensor] = None
 
 
 class Workspace:
@@ -425,7 +442,7 @@ class SolveJob:
                  mode: str = "mixed", tol_residual: float = 1e-12, tol_anorm: float = 1e-9,
                  replace_ratio: float = 0.0, maxit: int = 100000, check_every: int = 32, adaptive: bool = True,
                  owner=None, precond: str = "auto", coarse_target: Optional[int] = None, max_blocks: int = 0,
-                 expected_iterations: Optional[int] = None):
+                 expected_iterations: Optional[int] = None, strategy: str = "auto"):
         self.A, self.b, self.x, self.known = A, b, x, known
         self.expected_iterations = expected_iterations
         self.precond = choose_preconditioner(A, precond)
@@ -444,6 +461,10 @@ class SolveJob:
         self.Minv: Optional[torch.Tensor] = None
         self.coarse_reused = False
         self._cache: Optional[_CoarseCache] = None
+        assert strategy in ("auto", "values", "refine"), strategy
+        self.strategy = strategy
+        self.refine = False  # decided by run(): single two-level solves only
+        self.Ainv32: Optional[torch.Tensor] = None
 
     def setup(self, precision: int) -> None:
         """Block-Jacobi inverse (and the FP32 value copy) for ``precision``."""
@@ -493,7 +514,23 @@ class SolveJob:
     def info(self, raw) -> SolveInfo:
         return SolveInfo(int(raw[0]), float(raw[1]), int(raw[2]), int(raw[3]),
                          "fp64" if self.precision == 64 else "mixed", self.fallback, self.precond,
-                         self.coarse_reused)
+                         self.coarse_reused, self.refine and self.precision == 32)
+
+    def _wants_refine(self) -> bool:
+        if self.precision != 32 or self.precond != "two-level" or self.strategy == "values":
+            return False
+        return self.strategy == "refine" or self.A.nnz * 8 >= REFINE_MIN_MATRIX_BYTES
+
+    def _coarse_inverse32(self) -> torch.Tensor:
+        """FP32 copy of the coarse inverse (kept with a lagged inverse)."""
+        c = self._cache
+        if c is not None and c.Ainv32 is not None and c.Ainv is self.Ainv:
+            return c.Ainv32
+        a32 = torch.empty(self.Ainv.numel(), dtype=torch.float32, device="cuda")
+        _lib.call("efea_cast_f64_f32", self.Ainv.numel(), ptr(self.Ainv), ptr(a32), stream_ptr())
+        if c is not None and c.Ainv is self.Ainv:
+            c.Ainv32 = a32
+        return a32
 
     def _polls(self) -> tuple:
         """(check_every, first_check) of the host's state read-backs.  With
@@ -516,11 +553,16 @@ class SolveJob:
             cs = self.cs
             nbytes = _lib.load().efea_pcg2_workspace_bytes(A.Nn, A.dof_n, cs.nagg)
             buf = ws.get(nbytes)
-            rc = _lib.call("efea_solve_pcg2", self.precision, A.Nn, A.dof_n, A.group, ptr(A.node_ptr),
-                           ptr(A.node_cols), ptr(A.values), ptr(v32), ptr(c32), ptr(self.known), ptr(self.Minv),
-                           cs.nagg,
-                           ptr(cs.agg_ptr), ptr(cs.agg_nodes), ptr(cs.rel), ptr(self.Ainv), cs.ld, ptr(self.b),
-                           ptr(self.x), ptr(buf), nbytes, self.tol_residual, self.tol_anorm, self.replace_ratio,
+            refine = self.refine and self.precision == 32
+            # mode 33 (refinement): FP32 block Jacobi and coarse inverse; the
+            # replace-ratio slot carries the inner solves' accuracy
+            rc = _lib.call("efea_solve_pcg2", 33 if refine else self.precision, A.Nn, A.dof_n, A.group,
+                           ptr(A.node_ptr), ptr(A.node_cols), ptr(A.values), ptr(v32), ptr(c32), ptr(self.known),
+                           ptr(self.Minv), cs.nagg,
+                           ptr(cs.agg_ptr), ptr(cs.agg_nodes), ptr(cs.rel),
+                           ptr(self._coarse_inverse32() if refine else self.Ainv), cs.ld, ptr(self.b),
+                           ptr(self.x), ptr(buf), nbytes, self.tol_residual, self.tol_anorm,
+                           REFINE_INNER_TOL if refine else self.replace_ratio,
                            self.maxit, check_every, first_check, self.max_blocks, stream_ptr(),
                            C.cast(raw, C.c_void_p))
             return rc, self.info(raw)
@@ -538,6 +580,7 @@ class SolveJob:
         """Solves now with ``efea_solve_pcg``; a mixed solve that breaks down,
         diverges or stalls is redone in FP64 from the same initial guess."""
         ws = ws or Workspace()
+        self.refine = self._wants_refine()
         self.setup(self.precision)
         rc, si = self._solve_once(ws)
         if rc != 0 and self.precision == 32:

@@ -205,6 +205,7 @@ def elastic_request(spec: ElasticRequestSpec, mesh: Mesh, mode: str, precond: st
     mat = Models.Elastic.Isotropic(2, E=spec.E, v=0.3, planeStress=True, thickness=1.0)
     simu = Simulations.Elastic(mesh, mat)
     simu.solver = "hip_fp64" if mode == "fp64" else "hip_mixed"
+    simu.mixed_strategy = "values"  # the batch engine's algorithm, alone or batched
     simu.preconditioner = precond
     simu.max_blocks = max_blocks
     left, right = _square_sides(mesh)
@@ -273,4 +274,5 @@ def _sent_on(mesh: Mesh, Gc_e: np.ndarray, mode: str) -> Simulations.PhaseField:
     mat = Models.Elastic.Isotropic(2, E=210000.0, v=0.3, planeStress=False, thickness=1.0)
     simu = Simulations.PhaseField(mesh, Models.PhaseField(mat, "Amor", "AT2", Gc_e, 0.01))
     simu.solver = "hip_fp64" if mode == "fp64" else "hip_mixed"
+    simu.mixed_strategy = "values"  # the batch engine's algorithm, alone or batched
     return simu

@@ -370,9 +370,11 @@ size_t efea_bounded_lsq_workspace_bytes(int N);
 /* Bound-constrained least squares on the device: x = argmin |A x - b|^2 subject to
  * lb <= x <= ub, for a symmetric scalar CSR matrix A (row_ptr N+1, cols, values, int32
  * indices), started from x clamped to the bounds (x in/out).  Primal-dual active-set updates
- * with CGLS on the free set: each pass reduces the scaled free-set gradient by tol (maxit
- * iterations at most); done when no variable violates the first-order conditions by more
- * than kkt_tol; EFEA_ERR_NOCONV after max_outer updates.  info (4 doubles, may be NULL):
+ * with CGLS on the free set: each pass reduces the column-scaled free-set gradient norm by
+ * tol, or stops after maxit iterations (rounded up to a multiple of 16: the host checks the
+ * norm every 16 iterations); done when no variable violates the first-order conditions by
+ * more than kkt_tol, after which x is clamped to [lb, ub] (a move of at most kkt_tol);
+ * EFEA_ERR_NOCONV after max_outer updates.  info (4 doubles, may be NULL):
  * active-set updates, CGLS iterations, violations left, first scaled gradient norm^2.
  * Synchronises the stream.  Replaces scipy.optimize.lsq_linear(A, b, bounds=(lb, ub),
  * tol=1e-10, method="trf") of the BoundConstrain damage solve (Simulations/Solvers.py:376-383,
