@@ -95,6 +95,8 @@ class _Simu:
         self.mixed_adaptive = True
         # hip_mixed two-level solves: "auto" | "values" | "refine" (device.py)
         self.mixed_strategy = "auto"
+        # FP64 two-level solves: pipelined PCG "auto" (latency-bound systems) | True | False
+        self.pipelined = "auto"
         self.max_blocks = 0  # grid cap of the solver kernels (0: the library's default)
         # keep the two-level coarse inverse over successive solves of one matrix
         # until their iteration count grows (device.COARSE_REUSE_GROWTH)
@@ -540,7 +542,7 @@ class _Simu:
         return SolveJob(A, b_dev, x_dev, known, self._mode(), self.tol_residual, self.tol_anorm,
                         self.replace_ratio, self.maxit, self.check_every, adaptive=self.mixed_adaptive,
                         owner=self, precond=self.preconditioner, max_blocks=self.max_blocks,
-                        expected_iterations=expected, strategy=self.mixed_strategy,
+                        expected_iterations=expected, strategy=self.mixed_strategy, pipelined=self.pipelined,
                         coarse_target=self.coarse_target)
 
     def _solve_system_gen(self, A, b_dev: torch.Tensor, x_dev: torch.Tensor, problemType,
@@ -551,6 +553,7 @@ class _Simu:
         t = time.perf_counter()
         info = yield job
         job.record(info)
+        self.last_job = job
         self.last_solve[problemType] = info
         if info.status != 1:
             raise _lib.EfeaError("solve", _lib.ERR_NOCONV,

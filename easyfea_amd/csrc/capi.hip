@@ -97,6 +97,12 @@ int solve_pcg2(int, int, int, int, const int*, const int*, const double*, const 
                const unsigned char*, const void*, int, const int*, const int*, const double*, const double*, int,
                const double*, double*, void*, double, double, double, int, int, int, int, hipStream_t, double*);
 
+size_t pcg2p_workspace_bytes(int, int, int);
+int solve_pcg2p(int, int, int, const int*, const int*, const double*, const unsigned char*, const double*, int,
+                const int*, const int*, const double*, const double*, int, int, const int*, const int*, const int*,
+                int, const int*, const double*, double*, void*, double, double, int, int, int, hipStream_t, double*,
+                void*);
+
 }  // namespace efea
 
 using namespace efea;
@@ -328,6 +334,48 @@ int efea_solve_pcg_noalloc(int mode, int Nnode, int dof_n, int group, const int3
   return solve_pcg(mode, Nnode, dof_n, group, node_ptr, node_cols, values64, values32, corr32, known, Minv, b, x, ws,
                    tol_residual, tol_anorm, replace_ratio, maxit, check_every, first_check, max_blocks,
                    STREAM(stream), info, &hs);
+}
+
+size_t efea_pcg2p_workspace_bytes(int Nnode, int dof_n, int nslot) {
+  return pcg2p_workspace_bytes(Nnode, dof_n, nslot);
+}
+
+int efea_solve_pcg2p(int Nnode, int dof_n, int group, const int32_t* node_ptr, const int32_t* node_cols,
+                     const double* values64, const uint8_t* known, const double* Minv, int nagg,
+                     const int32_t* agg_ptr, const int32_t* agg_nodes, const double* rel, const double* Ainv, int ld,
+                     int nchunk, const int32_t* chunk_k0, const int32_t* chunk_k1, const int32_t* chunk_slot,
+                     int chunks_per_agg, const int32_t* chunk_node_ptr, const double* b, double* x,
+                     void* workspace, size_t workspace_bytes, double tol_residual, double tol_anorm, int maxit,
+                     int check_every, int first_check, void* stream, double* info) {
+  if (Nnode <= 0 || nagg <= 0 || nchunk <= 0 || chunks_per_agg <= 0 || !node_ptr || !node_cols || !values64 ||
+      !Minv || !agg_ptr || !agg_nodes || !rel || !Ainv || !chunk_k0 || !chunk_k1 || !chunk_slot ||
+      !chunk_node_ptr || !b || !x || !workspace || !info) {
+    set_error("efea_solve_pcg2p: null pointer or empty system");
+    return EFEA_ERR_ARG;
+  }
+  if (nchunk > 1024) {
+    set_error("efea_solve_pcg2p: %d node chunks, at most 1024 (one block each)", nchunk);
+    return EFEA_ERR_ARG;
+  }
+  if (ld < coarse_ld(nagg, dof_n)) {
+    set_error("efea_solve_pcg2p: leading dimension %d < %d", ld, coarse_ld(nagg, dof_n));
+    return EFEA_ERR_ARG;
+  }
+  const int nm = dof_n == 1 ? 1 : (dof_n == 2 ? 3 : 6);
+  if ((size_t)nagg * nm * sizeof(double) > 32768) {
+    set_error("efea_solve_pcg2p: %d coarse unknowns, at most %d (the restriction is summed in LDS)", nagg * nm,
+              (int)(32768 / sizeof(double)));
+    return EFEA_ERR_ARG;
+  }
+  if (workspace_bytes < pcg2p_workspace_bytes(Nnode, dof_n, nagg * chunks_per_agg)) {
+    set_error("efea_solve_pcg2p: workspace of %zu bytes, %zu needed", workspace_bytes,
+              pcg2p_workspace_bytes(Nnode, dof_n, nagg * chunks_per_agg));
+    return EFEA_ERR_SIZE;
+  }
+  return solve_pcg2p(Nnode, dof_n, group, node_ptr, node_cols, values64, known, Minv, nagg, agg_ptr, agg_nodes, rel,
+                     Ainv, ld, nchunk, chunk_k0, chunk_k1, chunk_slot, chunks_per_agg, chunk_node_ptr, b, x,
+                     workspace, tol_residual, tol_anorm, maxit, check_every, first_check, STREAM(stream), info,
+                     nullptr);
 }
 
 int efea_batch_create(int mode, int dof_n, int group, int two_level, int capacity, void** handle) {

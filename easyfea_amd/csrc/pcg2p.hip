@@ -1,0 +1,694 @@
+// Pipelined two-level PCG (Ghysels and Vanroose 2014, preconditioned
+// pipelined CG) for latency-bound systems: two kernels per iteration
+// instead of the classic solve's three.
+//
+// The classic two-level iteration needs three global synchronisations (the
+// p.q reduction, the restriction P^T r feeding the dense coarse solve, the
+// r.z reduction), hence three launches.  The pipelined recurrences
+//   gamma = (r, u), delta = (w, u), m = M w, n = A m,
+//   z = n + beta z, q = m + beta q, s = w + beta s, p = u + beta p,
+//   x += alpha p, r -= alpha s, u -= alpha q, w -= alpha z
+// (u = M r and w = A u kept by recurrence) put both dot products and the
+// restriction of w in the same pass as the product n = A m:
+//   kernel A (aggregate grid): rc = P^T w summed from the previous pass's
+//     per-chunk partials, zc = Ac^-1 rc for the block's aggregate,
+//     m = B^-1 w + P zc on its nodes;
+//   kernel B (chunks of aggregate-ordered nodes, node-group lanes): n = A m,
+//     alpha and beta from the previous pass's partials, the eight vector
+//     updates, the partials of (r, u), (w, u), (r, r) and, per chunk (nodes
+//     of one aggregate), of P^T w.
+// The stopping test is the classic solve's (delayed A-norm estimate from
+// alpha_j (r_j, u_j), residual test), evaluated by kernel B's block 0 one
+// pass late: a solve that converged at iteration i also applies update i+1.
+// The recurrences let rounding errors grow (the gap between the recursive and
+// the true residual; on the 360 x 39 cantilever the plain pipelined solve
+// breaks down with a true relative residual of 1e-3), so every
+// PIPE_REPLACE iterations the vectors are recomputed from x and p (residual
+// replacement, Cools and Vanroose): r = b - A x, u = M r, w = A u, s = A p,
+// q = M s, z = A q, six launches.
+// EasyFEA solves directly (Simulations/Solvers.py:225-394); this changes the
+// iteration's synchronisation pattern, not the solution (same stopping test).
+#include "coarse_device.h"
+
+namespace efea {
+
+// Chunks: runs of at most BS / G nodes of one aggregate, in the aggregate
+// order of agg_nodes, one per block of kernel B (nchunk <= MAXB); chunk c
+// holds positions [ck0[c], ck1[c]) and writes its restriction partial to slot
+// cslot[c] = a * maxch + (its index among aggregate a's chunks), so that
+// aggregate a's restriction is the sum of slots a * maxch .. a * maxch +
+// maxch - 1 (slots without a chunk stay zero).  nptr[2k], nptr[2k+1]:
+// node_ptr of node agg_nodes[k] and of the next node.
+struct PipeArgs {
+  int nchunk;
+  const int* ck0;
+  const int* ck1;
+  const int* cslot;
+  int maxch;
+  const int* nptr;
+};
+
+constexpr int PIPE_REPLACE = 64;
+
+struct PipeWork {
+  double *r, *u, *w, *m, *z, *q, *s, *p;
+  double* partB;   // 4 per block of kernel B: (r,u), (w,u), (r,r), x.A x; two halves of 4 MAXB used in turn
+  double* partR;   // 4 MAXB: partials of the residual passes (read-backs, replacements)
+  double* rcpart;  // NM per slot: P^T (w or r) of the chunk's nodes
+  double* rcpart2; // NM per slot: P^T s (replacement)
+  double* small;
+  PcgState* st;
+};
+
+static inline size_t pcg2p_workspace_size(int Nnode, int dof_n, int nslot) {
+  const int nm = dof_n == 1 ? 1 : (dof_n == 2 ? 3 : 6);
+  const size_t vb = align256(8 * (size_t)Nnode * dof_n);
+  return 8 * vb + align256(64 * (size_t)MAXB) + align256(32 * (size_t)MAXB) +
+         2 * align256(8 * (size_t)nslot * nm) + 256 + align256(sizeof(PcgState));
+}
+
+static inline PipeWork carve_pipe(void* ws, int Nnode, int dof_n, int nslot) {
+  const int nm = dof_n == 1 ? 1 : (dof_n == 2 ? 3 : 6);
+  const size_t vb = align256(8 * (size_t)Nnode * dof_n);
+  char* c = (char*)ws;
+  PipeWork w;
+  double** v[8] = {&w.r, &w.u, &w.w, &w.m, &w.z, &w.q, &w.s, &w.p};
+  for (int k = 0; k < 8; ++k) {
+    *v[k] = (double*)c;
+    c += vb;
+  }
+  w.partB = (double*)c; c += align256(64 * (size_t)MAXB);
+  w.partR = (double*)c; c += align256(32 * (size_t)MAXB);
+  w.rcpart = (double*)c; c += align256(8 * (size_t)nslot * nm);
+  w.rcpart2 = (double*)c; c += align256(8 * (size_t)nslot * nm);
+  w.small = (double*)c; c += 256;
+  w.st = (PcgState*)c;
+  return w;
+}
+
+// Kernel B and its start-up / replacement variants, grid-stride over chunks.
+//   MODE 0: y = A x on the masked rows; r = b - y; partials x.y; rc = P^T r.
+//   MODE 1: w = A u (masked); partials (r,u), (w,u), (r,r); rc = P^T w.
+//   MODE 2: n = A m; the updates; partials; rc = P^T w (new w).
+//   MODE 3: s = A p (masked); rc = P^T s.
+//   MODE 4: z = A q (masked).
+// Every block reduces the previous kernel B's partials itself (fixed order).
+template <int DN, int G, int MODE>
+__global__ void __launch_bounds__(BS)
+k_pp_spmv(SpmvArgs A, PipeArgs PA, const int* __restrict__ agg_nodes, const double* __restrict__ rel,
+          const double* __restrict__ vin, double* __restrict__ x, const double* __restrict__ b, PipeWork W,
+          const double* __restrict__ partPrev, int nbPrev) {
+  constexpr int NM = Modes<DN>::NM;  // a chunk holds BS / G nodes at most
+  __shared__ double shn[NM][BS / 64];
+  __shared__ double shp[4][BS / 64];
+  PcgState* st = W.st;
+  // previous pass's partials (MODE 2): issued first, reduced below
+  double pr[3] = {0, 0, 0};
+  if (MODE == 2) {
+    for (int i = threadIdx.x; i < nbPrev; i += BS) {
+      pr[0] += partPrev[4 * i];
+      pr[1] += partPrev[4 * i + 1];
+      pr[2] += partPrev[4 * i + 2];
+    }
+  }
+  double alpha = 0, beta = 0;
+  bool first = false;
+  if (MODE == 2) {
+    // snapshots taken by kernel A: block 0 below rewrites done and fresh
+    const int done = st->par_next;
+    first = st->justrep != 0;  // no iteration to close yet
+    const double a_prev = st->alpha_prev, g_prev = st->rz_prev;
+    block_sum_all<3, BS>(pr, shp);
+    if (done) return;
+    const double gam = pr[0], del = pr[1], rr = pr[2];
+    if (first) {
+      beta = 0;
+      alpha = gam / del;
+    } else {
+      beta = gam / g_prev;
+      alpha = gam / (del - beta * gam / a_prev);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      if (!first) {  // close the previous iteration: term alpha_(i-1) (r_(i-1), u_(i-1))
+        const double term = a_prev * g_prev;
+        st->Sall += term;
+        st->ring[st->head] = term;
+        st->head = (st->head + 1) % RING;
+        double Sw = 0;
+        for (int i = 0; i < RING; ++i) Sw += st->ring[i];
+        st->Sw = Sw;
+        st->iters += 1;
+      }
+      st->fresh = 0;
+      st->rr = rr;
+      st->rz = gam;
+      st->alpha = alpha;
+      st->beta = beta;
+      st->pq = del;
+      const double scale = st->Sall > st->xAx0 ? st->Sall : st->xAx0;
+      const bool finite = isfinite(rr) && isfinite(gam) && isfinite(del) && isfinite(st->Sw);
+      if (finite && ((st->iters >= RING && st->Sw <= st->tolA2 * scale) || rr <= st->tolr2 * st->bb || gam == 0))
+        st->done = 1;  // converged (gam = 0: r = 0 exactly), whatever the next step's alpha
+      else if (!finite || !(alpha > 0) || !isfinite(alpha))
+        st->done = 2;  // breakdown: the caller redoes the solve with the classic iteration
+    }
+    if (!(alpha > 0) || !isfinite(alpha)) return;  // the same test in every block
+  }
+  double acc[4] = {0, 0, 0, 0};
+  for (int c = blockIdx.x; c < PA.nchunk; c += gridDim.x) {
+    const int k0 = PA.ck0[c], k1 = PA.ck1[c];
+    const int k = k0 + (int)threadIdx.x / G, g = (int)threadIdx.x % G;
+    const bool have = k < k1;
+    int n = 0, p0 = 0, p1 = 0;
+    if (have) {
+      n = agg_nodes[k];
+      p0 = PA.nptr[2 * k];
+      p1 = PA.nptr[2 * k + 1];
+    }
+    double sum[1][DN];
+#pragma unroll
+    for (int a = 0; a < DN; ++a) sum[0][a] = 0;
+    // own-row operands of the updates, loaded with the gathers
+    double own[9][DN];
+    if (have && g == 0) {
+#pragma unroll
+      for (int a = 0; a < DN; ++a) {
+        const int64_t i = (int64_t)n * DN + a;
+        if (MODE == 0) {
+          own[0][a] = x[i];
+          own[1][a] = b[i];
+        } else if (MODE >= 3) {
+        } else if (MODE == 1) {
+          own[0][a] = vin[i];  // u
+          own[1][a] = W.r[i];
+        } else {
+          own[0][a] = vin[i];  // m
+          own[1][a] = W.w[i];
+          own[2][a] = W.u[i];
+          own[3][a] = W.r[i];
+          own[4][a] = x[i];
+          own[5][a] = W.z[i];
+          own[6][a] = W.q[i];
+          own[7][a] = W.s[i];
+          own[8][a] = W.p[i];
+        }
+      }
+    }
+    if (have) {
+      gather_rows<double, DN, G, 1>((const double*)A.vals, A.node_cols, p0, p1 - p0, g,
+                                    [&](int col, double (&xv)[1][DN]) {
+#pragma unroll
+                                      for (int bb = 0; bb < DN; ++bb) xv[0][bb] = vin[(int64_t)col * DN + bb];
+                                    },
+                                    sum);
+    }
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) sum[0][a] += __shfl_xor(sum[0][a], o, G);
+    }
+    double rcv[NM];
+#pragma unroll
+    for (int mm = 0; mm < NM; ++mm) rcv[mm] = 0;
+    if (have && g == 0) {
+      const double* rn = rel + 3 * (int64_t)n;
+#pragma unroll
+      for (int a = 0; a < DN; ++a) {
+        const int64_t i = (int64_t)n * DN + a;
+        const bool kn = A.known && A.known[i];
+        const double y = kn ? 0.0 : sum[0][a];
+        double rest;  // the vector restricted by P^T in this pass
+        if (MODE == 3) {
+          W.s[i] = y;
+          rest = y;
+        } else if (MODE == 4) {
+          W.z[i] = y;
+          rest = 0;
+        } else if (MODE == 0) {
+          acc[3] += own[0][a] * y;  // x.A x over the unknown rows
+          const double r = kn ? 0.0 : own[1][a] - y;
+          W.r[i] = r;
+          rest = r;
+        } else if (MODE == 1) {
+          W.w[i] = y;
+          acc[0] += own[1][a] * own[0][a];  // (r, u)
+          acc[1] += y * own[0][a];          // (w, u)
+          acc[2] += own[1][a] * own[1][a];  // (r, r)
+          rest = y;
+        } else {
+          const double mv = own[0][a], wv = own[1][a], uv = own[2][a];
+          const double zv = y + (first ? 0.0 : beta * own[5][a]);
+          const double qv = mv + (first ? 0.0 : beta * own[6][a]);
+          const double sv = wv + (first ? 0.0 : beta * own[7][a]);
+          const double pv = uv + (first ? 0.0 : beta * own[8][a]);
+          const double xn = own[4][a] + alpha * pv;
+          const double rnw = own[3][a] - alpha * sv;
+          const double un = uv - alpha * qv;
+          const double wn = wv - alpha * zv;
+          W.z[i] = zv;
+          W.q[i] = qv;
+          W.s[i] = sv;
+          W.p[i] = pv;
+          x[i] = xn;
+          W.r[i] = rnw;
+          W.u[i] = un;
+          W.w[i] = wn;
+          acc[0] += rnw * un;
+          acc[1] += wn * un;
+          acc[2] += rnw * rnw;
+          rest = wn;
+        }
+#pragma unroll
+        for (int mm = 0; mm < NM; ++mm) rcv[mm] += pmode<DN>(rn, a, mm) * rest;  // rest = 0 at known DOFs
+      }
+    }
+    if (MODE != 4) {
+      block_sum_all<NM, BS>(rcv, shn);
+      if (threadIdx.x == 0) {
+#pragma unroll
+        for (int mm = 0; mm < NM; ++mm) W.rcpart[(int64_t)PA.cslot[c] * NM + mm] = rcv[mm];
+      }
+    }
+  }
+  if (MODE <= 2) {
+    block_sum_all<4, BS>(acc, shp);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) W.partB[4 * blockIdx.x + q] = acc[q];
+    }
+  }
+}
+
+// Kernel B of an iteration, one chunk per block: the loads that do not need
+// the scalars (chunk, nodes, offsets, the own rows' nine operands, and the
+// product n = A m) are issued and consumed before the reduction of the
+// previous pass's partials that yields alpha and beta; one combined block
+// reduction ends it.
+template <int DN, int G>
+__global__ void __launch_bounds__(BS)
+k_pp_iter(SpmvArgs A, PipeArgs PA, const int* __restrict__ agg_nodes, const double* __restrict__ rel,
+          const double* __restrict__ m, double* __restrict__ x, PipeWork W, const double* __restrict__ partPrev,
+          int nbPrev) {
+  constexpr int NM = Modes<DN>::NM;
+  __shared__ double shp[3][BS / 64];
+  __shared__ double shr[NM + 3][BS / 64];
+  PcgState* st = W.st;
+  const int c = blockIdx.x;
+  const int k0 = PA.ck0[c], k1 = PA.ck1[c];
+  const int k = k0 + (int)threadIdx.x / G, g = (int)threadIdx.x % G;
+  const bool have = k < k1;
+  int n = 0, p0 = 0, p1 = 0;
+  if (have) {
+    n = agg_nodes[k];
+    p0 = PA.nptr[2 * k];
+    p1 = PA.nptr[2 * k + 1];
+  }
+  double pr[3] = {0, 0, 0};
+  for (int i = threadIdx.x; i < nbPrev; i += BS) {
+    pr[0] += partPrev[4 * i];
+    pr[1] += partPrev[4 * i + 1];
+    pr[2] += partPrev[4 * i + 2];
+  }
+  // snapshots taken by kernel A (block 0 below rewrites the originals)
+  const int done = st->par_next;
+  const bool first = st->justrep != 0;  // no iteration to close yet
+  const double a_prev = st->alpha_prev, g_prev = st->rz_prev;
+  double own[9][DN], relf[3];
+  const bool lead = have && g == 0;
+  if (lead) {
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      const int64_t i = (int64_t)n * DN + a;
+      own[0][a] = m[i];
+      own[1][a] = W.w[i];
+      own[2][a] = W.u[i];
+      own[3][a] = W.r[i];
+      own[4][a] = x[i];
+      own[5][a] = W.z[i];
+      own[6][a] = W.q[i];
+      own[7][a] = W.s[i];
+      own[8][a] = W.p[i];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) relf[d] = rel[3 * (int64_t)n + d];
+  }
+  double sum[1][DN];
+#pragma unroll
+  for (int a = 0; a < DN; ++a) sum[0][a] = 0;
+  if (have)
+    gather_rows<double, DN, G, 1>((const double*)A.vals, A.node_cols, p0, p1 - p0, g,
+                                  [&](int col, double (&xv)[1][DN]) {
+#pragma unroll
+                                    for (int bb = 0; bb < DN; ++bb) xv[0][bb] = m[(int64_t)col * DN + bb];
+                                  },
+                                  sum);
+#pragma unroll
+  for (int a = 0; a < DN; ++a) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) sum[0][a] += __shfl_xor(sum[0][a], o, G);
+  }
+  block_sum_all<3, BS>(pr, shp);
+  if (done) return;
+  const double gam = pr[0], del = pr[1], rr = pr[2];
+  double alpha, beta;
+  if (first) {
+    beta = 0;
+    alpha = gam / del;
+  } else {
+    beta = gam / g_prev;
+    alpha = gam / (del - beta * gam / a_prev);
+  }
+  if (c == 0 && threadIdx.x == 0) {
+    if (!first) {  // close the previous iteration: term alpha_(i-1) (r_(i-1), u_(i-1))
+      const double term = a_prev * g_prev;
+      st->Sall += term;
+      st->ring[st->head] = term;
+      st->head = (st->head + 1) % RING;
+      double Sw = 0;
+      for (int i = 0; i < RING; ++i) Sw += st->ring[i];
+      st->Sw = Sw;
+      st->iters += 1;
+    }
+    st->fresh = 0;
+    st->rr = rr;
+    st->rz = gam;
+    st->alpha = alpha;
+    st->beta = beta;
+    st->pq = del;
+    const double scale = st->Sall > st->xAx0 ? st->Sall : st->xAx0;
+    const bool finite = isfinite(rr) && isfinite(gam) && isfinite(del) && isfinite(st->Sw);
+    if (finite && ((st->iters >= RING && st->Sw <= st->tolA2 * scale) || rr <= st->tolr2 * st->bb || gam == 0))
+      st->done = 1;  // converged (gam = 0: r = 0 exactly), whatever the next step's alpha
+    else if (!finite || !(alpha > 0) || !isfinite(alpha))
+      st->done = 2;  // breakdown: the caller redoes the solve with the classic iteration
+  }
+  if (!(alpha > 0) || !isfinite(alpha)) return;  // the same test in every block
+  double red[NM + 3];
+#pragma unroll
+  for (int q = 0; q < NM + 3; ++q) red[q] = 0;
+  if (lead) {
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      const int64_t i = (int64_t)n * DN + a;
+      const bool kn = A.known && A.known[i];
+      const double y = kn ? 0.0 : sum[0][a];
+      const double mv = own[0][a], wv = own[1][a], uv = own[2][a];
+      const double zv = y + (first ? 0.0 : beta * own[5][a]);
+      const double qv = mv + (first ? 0.0 : beta * own[6][a]);
+      const double sv = wv + (first ? 0.0 : beta * own[7][a]);
+      const double pv = uv + (first ? 0.0 : beta * own[8][a]);
+      const double rnw = own[3][a] - alpha * sv;
+      const double un = uv - alpha * qv;
+      const double wn = wv - alpha * zv;
+      W.z[i] = zv;
+      W.q[i] = qv;
+      W.s[i] = sv;
+      W.p[i] = pv;
+      x[i] = own[4][a] + alpha * pv;
+      W.r[i] = rnw;
+      W.u[i] = un;
+      W.w[i] = wn;
+      red[0] += rnw * un;
+      red[1] += wn * un;
+      red[2] += rnw * rnw;
+#pragma unroll
+      for (int mm = 0; mm < NM; ++mm) red[3 + mm] += pmode<DN>(relf, a, mm) * wn;  // w = 0 at known DOFs
+    }
+  }
+  block_sum_all<NM + 3, BS>(red, shr);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) W.partB[4 * c + q] = red[q];
+    W.partB[4 * c + 3] = 0;
+#pragma unroll
+    for (int mm = 0; mm < NM; ++mm) W.rcpart[(int64_t)PA.cslot[c] * NM + mm] = red[3 + mm];
+  }
+}
+
+// Kernel A: out = B^-1 vin + P Ac^-1 rc on the block's aggregate.  Every
+// thread takes columns c = t, t + BS, ... of the aggregate's NM rows of
+// Ac^-1 with rc_c summed from the aggregate slots of c (fixed order); the
+// node operands and the rows are loaded before the partials.  MODE 1
+// (iterations) also snapshots the scalars kernel B reads while its block 0
+// rewrites them.
+template <typename TP, int DN, int MODE>
+__global__ void __launch_bounds__(BS)
+k_pp_coarse(CoarseArgs C, PipeArgs PA, const unsigned char* __restrict__ known, const TP* __restrict__ Minv,
+            const double* __restrict__ vin, double* __restrict__ out, const double* __restrict__ rcpart,
+            PcgState* st) {
+  constexpr int NM = Modes<DN>::NM;
+  constexpr int U = 4;  // columns per pass, loads issued together
+  __shared__ double shz[NM][BS / 64];
+  const int nc = C.nagg * NM;
+  const int blk = blockIdx.x;
+  if (MODE == 1 && blk == 0 && threadIdx.x == 0) {
+    // taken also after convergence, so that kernel B sees done
+    st->alpha_prev = st->alpha;
+    st->rz_prev = st->rz;
+    st->justrep = st->fresh;
+    st->par_next = st->done;
+  }
+  const int k0 = C.agg_ptr[blk], k1 = C.agg_ptr[blk + 1];
+  const int kf = k0 + threadIdx.x;
+  const bool have = kf < k1;
+  const int nf = have ? C.agg_nodes[kf] : 0;
+  double vf[DN], Mf[DN * DN], relf[3];
+  if (have) {
+#pragma unroll
+    for (int a = 0; a < DN; ++a) vf[a] = vin[(int64_t)nf * DN + a];
+#pragma unroll
+    for (int e = 0; e < DN * DN; ++e) Mf[e] = (double)Minv[(int64_t)nf * DN * DN + e];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) relf[d] = C.rel[3 * (int64_t)nf + d];
+  }
+  const double* __restrict__ rows = (const double*)C.Ainv + (int64_t)(blk * NM) * C.ld;
+  const int mx = PA.maxch;
+  double zc[NM];
+#pragma unroll
+  for (int mm = 0; mm < NM; ++mm) zc[mm] = 0;
+  for (int c0 = threadIdx.x; c0 < nc; c0 += U * BS) {
+    double av[U][NM], rv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int c = c0 + u * BS;
+      rv[u] = 0;
+#pragma unroll
+      for (int mm = 0; mm < NM; ++mm) av[u][mm] = c < nc ? rows[(int64_t)mm * C.ld + c] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int c = c0 + u * BS;
+      if (c < nc) {
+        const int64_t base = (int64_t)(c / NM) * mx * NM + (c % NM);
+        for (int j = 0; j < mx; ++j) rv[u] += rcpart[base + (int64_t)j * NM];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int mm = 0; mm < NM; ++mm) zc[mm] += av[u][mm] * rv[u];
+  }
+  block_sum_all<NM, BS>(zc, shz);
+  if (MODE == 1 && st->done) return;
+  if (have) {
+    double zv[DN];
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      double s = 0;
+#pragma unroll
+      for (int b = 0; b < DN; ++b) s += Mf[a * DN + b] * vf[b];
+      zv[a] = s;
+    }
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      const int64_t i = (int64_t)nf * DN + a;
+      double cv = 0;
+#pragma unroll
+      for (int mm = 0; mm < NM; ++mm) cv += pmode<DN>(relf, a, mm) * zc[mm];
+      out[i] = (known && known[i]) ? 0.0 : zv[a] + cv;
+    }
+  }
+  for (int k = kf + BS; k < k1; k += BS) {  // aggregates of more than BS nodes
+    const int n = C.agg_nodes[k];
+    double v[DN], zv[DN];
+#pragma unroll
+    for (int a = 0; a < DN; ++a) v[a] = vin[(int64_t)n * DN + a];
+    apply_bj<TP, DN>(Minv, nullptr, n, v, zv);
+    const double* rn = C.rel + 3 * (int64_t)n;
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      const int64_t i = (int64_t)n * DN + a;
+      double cv = 0;
+#pragma unroll
+      for (int mm = 0; mm < NM; ++mm) cv += pmode<DN>(rn, a, mm) * zc[mm];
+      out[i] = (known && known[i]) ? 0.0 : zv[a] + cv;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(BS) k_pp_zero(int64_t n, double* __restrict__ a, double* __restrict__ b) {
+  for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+    a[i] = 0;
+    b[i] = 0;
+  }
+}
+
+__global__ void k_pp_state(PcgState* st, double tolr2, double tolA2) {
+  if (threadIdx.x == 0) {
+    st->rz = st->rz_prev = st->alpha = st->beta = st->pq = st->alpha_prev = 0;
+    st->rr = st->rrmax = st->rrmax_prev = st->bb = 0;
+    st->tolr2 = tolr2;
+    st->tolA2 = tolA2;
+    st->delta2 = 0;
+    st->Sall = st->Sw = st->xAx0 = 0;
+    for (int i = 0; i < RING; ++i) st->ring[i] = 0;
+    st->head = st->done = st->iters = st->nrepl = 0;
+    st->fresh = 1;
+    st->justrep = 0;
+    st->par = 0;
+    st->par_next = 1;
+  }
+}
+
+// After the start-up passes: x0.A x0, the initial residual norm.
+__global__ void __launch_bounds__(BS) k_pp_init_scalars(const double* __restrict__ partX, int nbX,
+                                                        const double* __restrict__ partW, int nbW, PcgState* st) {
+  __shared__ double sh[BS / 64];
+  const double xax = reduce_all(partX, nbX, 4, 3, sh);
+  const double rr = reduce_all(partW, nbW, 4, 2, sh);
+  if (threadIdx.x == 0) {
+    st->xAx0 = xax;
+    st->bb = rr;
+    st->rr = rr;
+    st->rrmax = rr;
+    st->done = (rr == 0) ? 1 : 0;
+  }
+}
+
+size_t pcg2p_workspace_bytes(int Nnode, int dof_n, int nslot) { return pcg2p_workspace_size(Nnode, dof_n, nslot); }
+
+template <int DN, int G>
+static int pcg2p_run(const SpmvArgs& A, const PipeArgs& PA, const double* Minv, const CoarseArgs& C,
+                     const double* b, double* x, void* ws, double tol_r, double tol_A, int maxit, int check_every,
+                     int first_check, hipStream_t s, double* info, PcgState* hs) {
+  const int Nnode = A.Nnode;
+  constexpr int NM = Modes<DN>::NM;
+  const int nslot = C.nagg * PA.maxch;
+  const PipeWork W = carve_pipe(ws, Nnode, DN, nslot);
+  const unsigned nbB = (unsigned)PA.nchunk;  // one chunk per block (nchunk <= MAXB)
+  // slots without a chunk are never written: zero both restriction arrays
+  hipLaunchKernelGGL(k_pp_zero, dim3(grid_for((int64_t)nslot * NM)), dim3(BS), 0, s, (int64_t)nslot * NM, W.rcpart,
+                     W.rcpart2);
+  const unsigned nbA = (unsigned)C.nagg;
+  const size_t lds = 0;
+  // partials: start-up pass 1 -> partB; the iteration kernels alternate with
+  // pass 2's block of four-value partials in the same array: each kernel B
+  // reads the previous kernel B's partials, so two halves are used in turn
+  double* partA = W.partB;              // kernel B of even passes (and start-up pass 0)
+  double* partBb = W.partB + 4 * MAXB;  // kernel B of odd passes (and start-up pass 1)
+  hipLaunchKernelGGL(k_pp_state, dim3(1), dim3(64), 0, s, W.st, tol_r * tol_r, tol_A * tol_A);
+  hipLaunchKernelGGL((k_pp_spmv<DN, G, 0>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
+                     (const double*)x, x, b, W, (const double*)nullptr, 0);
+  hipLaunchKernelGGL((k_pp_coarse<double, DN, 0>), dim3(nbA), dim3(BS), lds, s, C, PA, A.known, Minv,
+                     (const double*)W.r, W.u, (const double*)W.rcpart, W.st);
+  // start-up pass 1 writes the other half (pass 0 left x.A x in partA)
+  PipeWork W1 = W;
+  W1.partB = partBb;
+  hipLaunchKernelGGL((k_pp_spmv<DN, G, 1>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
+                     (const double*)W.u, x, b, W1, (const double*)nullptr, 0);
+  hipLaunchKernelGGL(k_pp_init_scalars, dim3(1), dim3(BS), 0, s, (const double*)partA, (int)nbB,
+                     (const double*)partBb, (int)nbB, W.st);
+  if (int e = check_launch("pcg2p_start")) return e;
+  int launched = 0, pass = 1;
+  bool spec = false;
+  double rr_spec = 0;
+  PcgWork fw{};  // residual passes of the read-backs: m is dead there (kernel A rewrites it)
+  fw.r = W.m;
+  fw.q = W.m;
+  fw.part1 = W.partR;
+  fw.small = W.small;
+  while (true) {
+    const int chunk = launched == 0 && first_check > 0 ? first_check : check_every;
+    for (int k = 0; k < chunk; ++k) {
+      if (pass > 1 && (pass - 1) % PIPE_REPLACE == 0) {
+        // residual replacement: r = b - A x, u = M r, w = A u (with the
+        // partials the next pass reads), s = A p, q = M s, z = A q
+        PipeWork Wr = W;
+        Wr.partB = W.partR;
+        hipLaunchKernelGGL((k_pp_spmv<DN, G, 0>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
+                           (const double*)x, x, b, Wr, (const double*)nullptr, 0);
+        hipLaunchKernelGGL((k_pp_coarse<double, DN, 0>), dim3(nbA), dim3(BS), lds, s, C, PA, A.known, Minv,
+                           (const double*)W.r, W.u, (const double*)W.rcpart, W.st);
+        Wr.partB = (pass & 1) ? partBb : partA;
+        hipLaunchKernelGGL((k_pp_spmv<DN, G, 1>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
+                           (const double*)W.u, x, b, Wr, (const double*)nullptr, 0);
+        Wr.rcpart = W.rcpart2;
+        hipLaunchKernelGGL((k_pp_spmv<DN, G, 3>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
+                           (const double*)W.p, x, b, Wr, (const double*)nullptr, 0);
+        hipLaunchKernelGGL((k_pp_coarse<double, DN, 0>), dim3(nbA), dim3(BS), lds, s, C, PA, A.known, Minv,
+                           (const double*)W.s, W.q, (const double*)W.rcpart2, W.st);
+        hipLaunchKernelGGL((k_pp_spmv<DN, G, 4>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
+                           (const double*)W.q, x, b, Wr, (const double*)nullptr, 0);
+      }
+      hipLaunchKernelGGL((k_pp_coarse<double, DN, 1>), dim3(nbA), dim3(BS), lds, s, C, PA, A.known, Minv,
+                         (const double*)W.w, W.m, (const double*)W.rcpart, W.st);
+      const double* prev = (pass & 1) ? partBb : partA;
+      PipeWork Wk = W;
+      Wk.partB = (pass & 1) ? partA : partBb;
+      hipLaunchKernelGGL((k_pp_iter<DN, G>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
+                         (const double*)W.m, x, Wk, prev, (int)nbB);
+      ++pass;
+    }
+    launched += chunk;
+    spec = first_check > 0;
+    if (spec) {
+      pcg_spec_residual<DN, G>(A, b, x, fw, s);
+      if (hipMemcpyAsync(&rr_spec, W.small, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) {
+        set_error("pcg2p: residual readback failed");
+        return EFEA_ERR_HIP;
+      }
+    }
+    if (int e = check_launch("pcg2p_iter")) return e;
+    if (hipMemcpyAsync(hs, W.st, sizeof(PcgState), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      set_error("pcg2p: state readback failed");
+      return EFEA_ERR_HIP;
+    }
+    if (hs->done || launched >= maxit) break;
+  }
+  const PcgState fin = *hs;
+  double rr_true = rr_spec;
+  if (!spec) {
+    pcg_true_residual<DN, G>(A, b, x, fw, s);
+    if (hipMemcpyAsync(&rr_true, W.small, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      set_error("pcg2p: residual readback failed");
+      return EFEA_ERR_HIP;
+    }
+  }
+  return pcg_finish_info(&fin, rr_true, info);
+}
+
+int solve_pcg2p(int Nnode, int dof_n, int group, const int* node_ptr, const int* node_cols, const double* vals64,
+                const unsigned char* known, const double* Minv, int nagg, const int* agg_ptr, const int* agg_nodes,
+                const double* rel, const double* Ainv, int ld, int nchunk, const int* ck0, const int* ck1,
+                const int* cslot, int maxch, const int* nptr, const double* b, double* x, void* ws,
+                double tol_r, double tol_A, int maxit, int check_every, int first_check, hipStream_t s,
+                double* info, void* host_state_buf) {
+  SpmvArgs A{Nnode, node_ptr, node_cols, vals64, known, nullptr, 0};
+  CoarseArgs C{nagg, agg_ptr, agg_nodes, rel, Ainv, ld, nullptr};
+  PipeArgs PA{nchunk, ck0, ck1, cslot, maxch, nptr};
+  if (check_every < 1) check_every = 1;
+  PcgState local;
+  PcgState* hs = host_state_buf ? (PcgState*)host_state_buf : &local;
+#define RUNP(D, GV)                                                                                          \
+  if (dof_n == D && group == GV)                                                                            \
+    return pcg2p_run<D, GV>(A, PA, Minv, C, b, x, ws, tol_r, tol_A, maxit, check_every, first_check, s, info, \
+                            hs);
+  EFEA_GROUPS(RUNP)
+#undef RUNP
+  set_error("solve_pcg2p: unsupported dof_n %d / group %d", dof_n, group);
+  return EFEA_ERR_ARG;
+}
+
+}  // namespace efea

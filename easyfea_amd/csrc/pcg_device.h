@@ -206,7 +206,8 @@ struct PcgState {
   int fresh;     // no iteration to close yet: the first step uses beta = 0
   int justrep;   // the r.z / r.r partials come from a residual replacement
   int par;       // which of the two direction buffers holds p
-  int par_next;  // the buffer the current step writes
+  int par_next;  // the buffer the current step writes (pipelined solve: snapshot of done)
+  double alpha_prev;  // pipelined solve: snapshot of alpha for the next pass
 };
 // done: 0 running, 1 converged, 2 breakdown, 3 diverged, 4 paused for a
 // residual replacement (mixed mode; the host runs the replacement kernel).
@@ -666,5 +667,10 @@ int pcg_start(const SpmvArgs& A64, const TP* Minv, const double* b, const double
               double tol_r, double tol_A, double delta2, hipStream_t s);
 template <int DN, int G>
 void pcg_true_residual(const SpmvArgs& A64, const double* b, const double* x, const PcgWork& w, hipStream_t s);
+// The same norm with the residual vector written to w.q (dead at a read-back).
+template <int DN, int G>
+void pcg_spec_residual(const SpmvArgs& A64, const double* b, const double* x, const PcgWork& w, hipStream_t s);
+// Fills info[0..3] from a read-back state and the true residual norm^2.
+int pcg_finish_info(const void* state, double rr_true, double* info);
 
 }  // namespace efea

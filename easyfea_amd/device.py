@@ -229,6 +229,14 @@ class CoarseSpace:
         self.scratch = torch.empty(_lib.load().efea_coarse_scratch_bytes(self.ld) // 8, dtype=torch.float64,
                                    device="cuda")
 
+    def pipeline(self, pattern: "CsrPattern", group: int) -> "PipeChunks":
+        """Node chunks of the pipelined solve's product pass (built once per
+        lanes-per-node value)."""
+        c = self.__dict__.setdefault("_pipe", {})
+        if group not in c:
+            c[group] = PipeChunks(self, pattern, group)
+        return c[group]
+
     def setup(self, A: "DeviceCsr", known: Optional[torch.Tensor]) -> torch.Tensor:
         """``Ac^-1`` (ld x ld, FP64) of ``A``'s current FP64 values."""
         Ainv = torch.empty(self.ld * self.ld, dtype=torch.float64, device="cuda")
@@ -236,6 +244,33 @@ class CoarseSpace:
                   ptr(self.pair_pos), ptr(self.blk_I), ptr(self.blk_J), ptr(A.node_ptr), ptr(A.node_cols),
                   ptr(A.values), ptr(known), ptr(self.rel), self.ld, ptr(Ainv), ptr(self.scratch), stream_ptr())
         return Ainv
+
+
+class PipeChunks:
+    """Chunks of the pipelined two-level solve (``efea_solve_pcg2p``): runs of
+    at most 256 / group nodes of one aggregate in aggregate order, so that
+    each chunk's partial of ``P^T w`` belongs to one aggregate."""
+
+    def __init__(self, cs: "CoarseSpace", pattern: "CsrPattern", group: int):
+        ag = cs.ag
+        npb = 256 // int(group)
+        sizes = np.diff(ag.agg_ptr).astype(np.int64)
+        self.maxch = int(max(1, ((sizes + npb - 1) // npb).max()))
+        k0, k1, slot = [], [], []
+        for a in range(ag.nagg):
+            lo, hi = int(ag.agg_ptr[a]), int(ag.agg_ptr[a + 1])
+            for j, s in enumerate(range(lo, hi, npb)):
+                k0.append(s)
+                k1.append(min(s + npb, hi))
+                slot.append(a * self.maxch + j)
+        self.nchunk = len(k0)
+        self.nslot = ag.nagg * self.maxch
+        nodes = ag.agg_nodes.astype(np.int64)
+        nptr = np.stack([pattern.host.node_ptr[nodes], pattern.host.node_ptr[nodes + 1]], axis=1)
+        self.k0 = to_dev(np.asarray(k0, dtype=np.int32))
+        self.k1 = to_dev(np.asarray(k1, dtype=np.int32))
+        self.slot = to_dev(np.asarray(slot, dtype=np.int32))
+        self.nptr = to_dev(np.ascontiguousarray(nptr, dtype=np.int32).ravel())
 
 
 class DeviceCsr:
@@ -382,6 +417,7 @@ class SolveInfo:
     precond: str = "bj"
     coarse_reused: bool = False
     refine: bool = False  # FP32 inner PCG inside FP64 iterative refinement
+    pipelined: bool = False  # pipelined two-level PCG (two launches per iteration)
 
 
 # Mixed-precision strategies of a single two-level solve:
@@ -397,6 +433,11 @@ class SolveInfo:
 # same algorithm).
 REFINE_INNER_TOL = 1e-4
 REFINE_MIN_MATRIX_BYTES = 128 * 2**20
+# FP64 two-level solves of latency-bound systems (matrix under
+# REFINE_MIN_MATRIX_BYTES, at most PIPELINE_MAX_COARSE coarse unknowns) run
+# the pipelined PCG (efea_solve_pcg2p): two launches per iteration instead of
+# three, at 8 instead of 5 work vectors.
+PIPELINE_MAX_COARSE = 4096
 
 
 # Lagged coarse inverse.  Any symmetric positive definite Ac^-1 keeps
@@ -442,7 +483,7 @@ class SolveJob:
                  mode: str = "mixed", tol_residual: float = 1e-12, tol_anorm: float = 1e-9,
                  replace_ratio: float = 0.0, maxit: int = 100000, check_every: int = 32, adaptive: bool = True,
                  owner=None, precond: str = "auto", coarse_target: Optional[int] = None, max_blocks: int = 0,
-                 expected_iterations: Optional[int] = None, strategy: str = "auto"):
+                 expected_iterations: Optional[int] = None, strategy: str = "auto", pipelined="auto"):
         self.A, self.b, self.x, self.known = A, b, x, known
         self.expected_iterations = expected_iterations
         self.precond = choose_preconditioner(A, precond)
@@ -457,7 +498,7 @@ class SolveJob:
         self.owner = owner  # the simulation (lagged coarse inverse cache)
         self.request = None  # the serving request waiting for this solve
         self.fallback = False
-        self.x0 = x.clone() if self.precision == 32 else None
+        self.x0 = x.clone() if self.precision == 32 else None  # mixed: start of an FP64 redo
         self.Minv: Optional[torch.Tensor] = None
         self.coarse_reused = False
         self._cache: Optional[_CoarseCache] = None
@@ -465,6 +506,9 @@ class SolveJob:
         self.strategy = strategy
         self.refine = False  # decided by run(): single two-level solves only
         self.Ainv32: Optional[torch.Tensor] = None
+        assert pipelined in ("auto", True, False), pipelined
+        self.pipelined_mode = pipelined
+        self.pipelined = False  # decided by run(), as refine
 
     def setup(self, precision: int) -> None:
         """Block-Jacobi inverse (and the FP32 value copy) for ``precision``."""
@@ -514,7 +558,16 @@ class SolveJob:
     def info(self, raw) -> SolveInfo:
         return SolveInfo(int(raw[0]), float(raw[1]), int(raw[2]), int(raw[3]),
                          "fp64" if self.precision == 64 else "mixed", self.fallback, self.precond,
-                         self.coarse_reused, self.refine and self.precision == 32)
+                         self.coarse_reused, self.refine and self.precision == 32,
+                         self.pipelined and self.precision == 64 and self.precond == "two-level")
+
+    def _wants_pipeline(self) -> bool:
+        if self.precision != 64 or self.precond != "two-level" or self.pipelined_mode is False:
+            return False
+        cs = self.A.coarse(self.coarse_target)
+        if cs.nagg * cs.nm > PIPELINE_MAX_COARSE or cs.pipeline(self.A.pattern, self.A.group).nchunk > 1024:
+            return False
+        return self.pipelined_mode is True or self.A.nnz * 8 < REFINE_MIN_MATRIX_BYTES
 
     def _wants_refine(self) -> bool:
         if self.precision != 32 or self.precond != "two-level" or self.strategy == "values":
@@ -549,6 +602,18 @@ class SolveJob:
         check_every, first_check = self._polls()
         v32 = A.values32 if self.precision == 32 else None
         c32 = A.corr32 if self.precision == 32 else None
+        if self.precond == "two-level" and self.pipelined and self.precision == 64:
+            cs = self.cs
+            pc = cs.pipeline(A.pattern, A.group)
+            nbytes = _lib.load().efea_pcg2p_workspace_bytes(A.Nn, A.dof_n, pc.nslot)
+            buf = ws.get(nbytes)
+            rc = _lib.call("efea_solve_pcg2p", A.Nn, A.dof_n, A.group, ptr(A.node_ptr), ptr(A.node_cols),
+                           ptr(A.values), ptr(self.known), ptr(self.Minv), cs.nagg, ptr(cs.agg_ptr),
+                           ptr(cs.agg_nodes), ptr(cs.rel), ptr(self.Ainv), cs.ld, pc.nchunk, ptr(pc.k0), ptr(pc.k1),
+                           ptr(pc.slot), pc.maxch, ptr(pc.nptr), ptr(self.b), ptr(self.x), ptr(buf), nbytes,
+                           self.tol_residual, self.tol_anorm, self.maxit, check_every, first_check, stream_ptr(),
+                           C.cast(raw, C.c_void_p))
+            return rc, self.info(raw)
         if self.precond == "two-level":
             cs = self.cs
             nbytes = _lib.load().efea_pcg2_workspace_bytes(A.Nn, A.dof_n, cs.nagg)
@@ -581,8 +646,19 @@ class SolveJob:
         diverges or stalls is redone in FP64 from the same initial guess."""
         ws = ws or Workspace()
         self.refine = self._wants_refine()
+        self.pipelined = self._wants_pipeline()
+        if self.pipelined and self.x0 is None:
+            self.x0 = self.x.clone()
         self.setup(self.precision)
         rc, si = self._solve_once(ws)
+        if rc != 0 and self.pipelined:
+            # a pipelined breakdown (rounding in its recurrences): the classic
+            # iteration from the same start
+            self.pipeline_failure = (rc, _lib.last_error(), si)
+            self.x.copy_(self.x0)
+            self.pipelined = False
+            self.fallback = True
+            rc, si = self._solve_once(ws)
         if rc != 0 and self.precision == 32:
             self.restart_fp64()
             rc, si = self._solve_once(ws)

@@ -219,6 +219,31 @@ int efea_solve_pcg2(int mode, int Nnode, int dof_n, int group, const int32_t* no
                     double tol_anorm, double replace_ratio, int maxit, int check_every, int first_check,
                     int max_blocks, void* stream, double* info);
 
+/* Pipelined two-level PCG (Ghysels and Vanroose's preconditioned pipelined
+ * CG) with the preconditioner of efea_solve_pcg2, FP64: two kernels per
+ * iteration instead of three (the restriction P^T w and both dot products
+ * ride on the product n = A m), for latency-bound systems.  Same arguments,
+ * stopping test and info as efea_solve_pcg2 (mode 64), plus the node chunks
+ * the product pass runs over, one block each (at most 1024): runs of at most
+ * 256 / group nodes of one aggregate in agg_nodes order, chunk c holding
+ * positions [chunk_k0[c], chunk_k1[c]) of agg_nodes and writing its share of
+ * P^T w to slot chunk_slot[c] = a * chunks_per_agg + (its index among its
+ * aggregate a's chunks); chunk_node_ptr (2 Nnode) holds node_ptr[agg_nodes[k]]
+ * and node_ptr[agg_nodes[k] + 1] for each position k.  At most 4096 coarse
+ * unknowns.  Workspace: efea_pcg2p_workspace_bytes(Nnode, dof_n,
+ * nagg * chunks_per_agg).  Every 64 iterations the recursive vectors are
+ * recomputed from x and the direction (residual replacement). */
+/* Replaces __Solver_1 (Simulations/Solvers.py:502-553) with _Solve_Axb's
+ * backends (Solvers.py:225-394), as efea_solve_pcg does. */
+size_t efea_pcg2p_workspace_bytes(int Nnode, int dof_n, int nslot);
+int efea_solve_pcg2p(int Nnode, int dof_n, int group, const int32_t* node_ptr, const int32_t* node_cols,
+                     const double* values64, const uint8_t* known, const double* Minv, int nagg,
+                     const int32_t* agg_ptr, const int32_t* agg_nodes, const double* rel, const double* Ainv, int ld,
+                     int nchunk, const int32_t* chunk_k0, const int32_t* chunk_k1, const int32_t* chunk_slot,
+                     int chunks_per_agg, const int32_t* chunk_node_ptr, const double* b, double* x,
+                     void* workspace, size_t workspace_bytes, double tol_residual, double tol_anorm, int maxit,
+                     int check_every, int first_check, void* stream, double* info);
+
 /* Continuous batching of Dirichlet-reduced solves (the serving loop).
  * A batch holds up to `capacity` systems of one (mode, dof_n, group) class and
  * advances all of them with one launch per PCG step; systems join
