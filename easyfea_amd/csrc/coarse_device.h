@@ -54,9 +54,8 @@ __device__ inline void pcg2_step2_body(CoarseArgs C, const unsigned char* __rest
   // convergence does the loads and returns there)
   __shared__ double sh[BS / 64];
   __shared__ double bc;
-  double pp1[1];
-  partial_share<1>(part1, nb1, pp1);
-  double ppq = pp1[0];
+  Partials<1> P1s;
+  P1s.issue(part1, nb1);
   const int pn = st->par_next;
   const double rz = st->rz;
   const TV* __restrict__ p = pn ? P1 : P0;
@@ -80,7 +79,9 @@ __device__ inline void pcg2_step2_body(CoarseArgs C, const unsigned char* __rest
 #pragma unroll
     for (int d = 0; d < 3; ++d) relf[d] = C.rel[3 * (int64_t)nf + d];
   }
-  ppq = block_sum<double, BS>(ppq, sh);
+  double pp1[1];
+  P1s.sum(part1, nb1, pp1);
+  const double ppq = block_sum<double, BS>(pp1[0], sh);
   if (threadIdx.x == 0) bc = ppq;
   __syncthreads();
   if (st->done) return;

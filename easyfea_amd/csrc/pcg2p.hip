@@ -10,16 +10,22 @@
 //   x += alpha p, r -= alpha s, u -= alpha q, w -= alpha z
 // (u = M r and w = A u kept by recurrence) put both dot products and the
 // restriction of w in the same pass as the product n = A m:
-//   kernel A (aggregate grid): rc = P^T w summed from the previous pass's
-//     per-chunk partials, zc = Ac^-1 rc for the block's aggregate,
+//   kernel A (aggregate grid plus one block): the extra block reduces the
+//     previous kernel B's partials, forms alpha and beta, closes the
+//     previous iteration's stopping-test term and decides whether to stop;
+//     every aggregate block sums rc = P^T w from the previous kernel B's
+//     per-chunk partials, forms zc = Ac^-1 rc for its aggregate and
 //     m = B^-1 w + P zc on its nodes;
-//   kernel B (chunks of aggregate-ordered nodes, node-group lanes): n = A m,
-//     alpha and beta from the previous pass's partials, the eight vector
-//     updates, the partials of (r, u), (w, u), (r, r) and, per chunk (nodes
-//     of one aggregate), of P^T w.
+//   kernel B (chunks of aggregate-ordered nodes, node-group lanes): reads
+//     alpha and beta, forms n = A m, the eight vector updates and the
+//     partials of (r, u), (w, u), (r, r) and, per chunk (nodes of one
+//     aggregate), of P^T w.
+// Kernel B thus starts its product at once: no block of it waits for a
+// reduction of the previous pass's partials.
 // The stopping test is the classic solve's (delayed A-norm estimate from
-// alpha_j (r_j, u_j), residual test), evaluated by kernel B's block 0 one
-// pass late: a solve that converged at iteration i also applies update i+1.
+// alpha_j (r_j, u_j), residual test), evaluated by kernel A's extra block
+// from the partials of the last update: the next kernel B sees the decision
+// before it updates anything.
 // The recurrences let rounding errors grow (the gap between the recursive and
 // the true residual; on the 360 x 39 cantilever the plain pipelined solve
 // breaks down with a true relative residual of 1e-3), so every
@@ -52,7 +58,7 @@ constexpr int PIPE_REPLACE = 64;
 
 struct PipeWork {
   double *r, *u, *w, *m, *z, *q, *s, *p;
-  double* partB;   // 4 per block of kernel B: (r,u), (w,u), (r,r), x.A x; two halves of 4 MAXB used in turn
+  double* partB;   // 4 per block of kernel B: (r,u), (w,u), (r,r), x.A x; two halves of 4 MAXB
   double* partR;   // 4 MAXB: partials of the residual passes (read-backs, replacements)
   double* rcpart;  // NM per slot: P^T (w or r) of the chunk's nodes
   double* rcpart2; // NM per slot: P^T s (replacement)
@@ -86,74 +92,18 @@ static inline PipeWork carve_pipe(void* ws, int Nnode, int dof_n, int nslot) {
   return w;
 }
 
-// Kernel B and its start-up / replacement variants, grid-stride over chunks.
+// The start-up and residual-replacement passes, grid-stride over chunks.
 //   MODE 0: y = A x on the masked rows; r = b - y; partials x.y; rc = P^T r.
 //   MODE 1: w = A u (masked); partials (r,u), (w,u), (r,r); rc = P^T w.
-//   MODE 2: n = A m; the updates; partials; rc = P^T w (new w).
 //   MODE 3: s = A p (masked); rc = P^T s.
 //   MODE 4: z = A q (masked).
-// Every block reduces the previous kernel B's partials itself (fixed order).
 template <int DN, int G, int MODE>
 __global__ void __launch_bounds__(BS)
 k_pp_spmv(SpmvArgs A, PipeArgs PA, const int* __restrict__ agg_nodes, const double* __restrict__ rel,
-          const double* __restrict__ vin, double* __restrict__ x, const double* __restrict__ b, PipeWork W,
-          const double* __restrict__ partPrev, int nbPrev) {
+          const double* __restrict__ vin, double* __restrict__ x, const double* __restrict__ b, PipeWork W) {
   constexpr int NM = Modes<DN>::NM;  // a chunk holds BS / G nodes at most
   __shared__ double shn[NM][BS / 64];
   __shared__ double shp[4][BS / 64];
-  PcgState* st = W.st;
-  // previous pass's partials (MODE 2): issued first, reduced below
-  double pr[3] = {0, 0, 0};
-  if (MODE == 2) {
-    for (int i = threadIdx.x; i < nbPrev; i += BS) {
-      pr[0] += partPrev[4 * i];
-      pr[1] += partPrev[4 * i + 1];
-      pr[2] += partPrev[4 * i + 2];
-    }
-  }
-  double alpha = 0, beta = 0;
-  bool first = false;
-  if (MODE == 2) {
-    // snapshots taken by kernel A: block 0 below rewrites done and fresh
-    const int done = st->par_next;
-    first = st->justrep != 0;  // no iteration to close yet
-    const double a_prev = st->alpha_prev, g_prev = st->rz_prev;
-    block_sum_all<3, BS>(pr, shp);
-    if (done) return;
-    const double gam = pr[0], del = pr[1], rr = pr[2];
-    if (first) {
-      beta = 0;
-      alpha = gam / del;
-    } else {
-      beta = gam / g_prev;
-      alpha = gam / (del - beta * gam / a_prev);
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      if (!first) {  // close the previous iteration: term alpha_(i-1) (r_(i-1), u_(i-1))
-        const double term = a_prev * g_prev;
-        st->Sall += term;
-        st->ring[st->head] = term;
-        st->head = (st->head + 1) % RING;
-        double Sw = 0;
-        for (int i = 0; i < RING; ++i) Sw += st->ring[i];
-        st->Sw = Sw;
-        st->iters += 1;
-      }
-      st->fresh = 0;
-      st->rr = rr;
-      st->rz = gam;
-      st->alpha = alpha;
-      st->beta = beta;
-      st->pq = del;
-      const double scale = st->Sall > st->xAx0 ? st->Sall : st->xAx0;
-      const bool finite = isfinite(rr) && isfinite(gam) && isfinite(del) && isfinite(st->Sw);
-      if (finite && ((st->iters >= RING && st->Sw <= st->tolA2 * scale) || rr <= st->tolr2 * st->bb || gam == 0))
-        st->done = 1;  // converged (gam = 0: r = 0 exactly), whatever the next step's alpha
-      else if (!finite || !(alpha > 0) || !isfinite(alpha))
-        st->done = 2;  // breakdown: the caller redoes the solve with the classic iteration
-    }
-    if (!(alpha > 0) || !isfinite(alpha)) return;  // the same test in every block
-  }
   double acc[4] = {0, 0, 0, 0};
   for (int c = blockIdx.x; c < PA.nchunk; c += gridDim.x) {
     const int k0 = PA.ck0[c], k1 = PA.ck1[c];
@@ -168,8 +118,8 @@ k_pp_spmv(SpmvArgs A, PipeArgs PA, const int* __restrict__ agg_nodes, const doub
     double sum[1][DN];
 #pragma unroll
     for (int a = 0; a < DN; ++a) sum[0][a] = 0;
-    // own-row operands of the updates, loaded with the gathers
-    double own[9][DN];
+    // own-row operands, loaded with the gathers
+    double own[2][DN];
     if (have && g == 0) {
 #pragma unroll
       for (int a = 0; a < DN; ++a) {
@@ -177,20 +127,9 @@ k_pp_spmv(SpmvArgs A, PipeArgs PA, const int* __restrict__ agg_nodes, const doub
         if (MODE == 0) {
           own[0][a] = x[i];
           own[1][a] = b[i];
-        } else if (MODE >= 3) {
         } else if (MODE == 1) {
           own[0][a] = vin[i];  // u
           own[1][a] = W.r[i];
-        } else {
-          own[0][a] = vin[i];  // m
-          own[1][a] = W.w[i];
-          own[2][a] = W.u[i];
-          own[3][a] = W.r[i];
-          own[4][a] = x[i];
-          own[5][a] = W.z[i];
-          own[6][a] = W.q[i];
-          own[7][a] = W.s[i];
-          own[8][a] = W.p[i];
         }
       }
     }
@@ -229,34 +168,12 @@ k_pp_spmv(SpmvArgs A, PipeArgs PA, const int* __restrict__ agg_nodes, const doub
           const double r = kn ? 0.0 : own[1][a] - y;
           W.r[i] = r;
           rest = r;
-        } else if (MODE == 1) {
+        } else {
           W.w[i] = y;
           acc[0] += own[1][a] * own[0][a];  // (r, u)
           acc[1] += y * own[0][a];          // (w, u)
           acc[2] += own[1][a] * own[1][a];  // (r, r)
           rest = y;
-        } else {
-          const double mv = own[0][a], wv = own[1][a], uv = own[2][a];
-          const double zv = y + (first ? 0.0 : beta * own[5][a]);
-          const double qv = mv + (first ? 0.0 : beta * own[6][a]);
-          const double sv = wv + (first ? 0.0 : beta * own[7][a]);
-          const double pv = uv + (first ? 0.0 : beta * own[8][a]);
-          const double xn = own[4][a] + alpha * pv;
-          const double rnw = own[3][a] - alpha * sv;
-          const double un = uv - alpha * qv;
-          const double wn = wv - alpha * zv;
-          W.z[i] = zv;
-          W.q[i] = qv;
-          W.s[i] = sv;
-          W.p[i] = pv;
-          x[i] = xn;
-          W.r[i] = rnw;
-          W.u[i] = un;
-          W.w[i] = wn;
-          acc[0] += rnw * un;
-          acc[1] += wn * un;
-          acc[2] += rnw * rnw;
-          rest = wn;
         }
 #pragma unroll
         for (int mm = 0; mm < NM; ++mm) rcv[mm] += pmode<DN>(rn, a, mm) * rest;  // rest = 0 at known DOFs
@@ -270,7 +187,7 @@ k_pp_spmv(SpmvArgs A, PipeArgs PA, const int* __restrict__ agg_nodes, const doub
       }
     }
   }
-  if (MODE <= 2) {
+  if (MODE <= 1) {
     block_sum_all<4, BS>(acc, shp);
     if (threadIdx.x == 0) {
 #pragma unroll
@@ -279,21 +196,24 @@ k_pp_spmv(SpmvArgs A, PipeArgs PA, const int* __restrict__ agg_nodes, const doub
   }
 }
 
-// Kernel B of an iteration, one chunk per block: the loads that do not need
-// the scalars (chunk, nodes, offsets, the own rows' nine operands, and the
-// product n = A m) are issued and consumed before the reduction of the
-// previous pass's partials that yields alpha and beta; one combined block
-// reduction ends it.
+// Kernel B of an iteration, one chunk per block: alpha, beta and the stop
+// decision come from kernel A's extra block (three scalar loads), so the
+// chain node -> offsets -> product starts at once; the own rows' nine
+// operands are loaded with the gather.  One block reduction ends it.
 template <int DN, int G>
 __global__ void __launch_bounds__(BS)
 k_pp_iter(SpmvArgs A, PipeArgs PA, const int* __restrict__ agg_nodes, const double* __restrict__ rel,
-          const double* __restrict__ m, double* __restrict__ x, PipeWork W, const double* __restrict__ partPrev,
-          int nbPrev) {
+          const double* __restrict__ m, double* __restrict__ x, PipeWork W) {
   constexpr int NM = Modes<DN>::NM;
-  __shared__ double shp[3][BS / 64];
   __shared__ double shr[NM + 3][BS / 64];
   PcgState* st = W.st;
   const int c = blockIdx.x;
+  if (st->done) {  // decided by kernel A; recorded for the next kernel A
+    if (c == 0 && threadIdx.x == 0) st->stop = 1;
+    return;
+  }
+  const bool first = st->justrep != 0;  // the first iteration: beta = 0, no old directions
+  const double alpha = st->alpha, beta = st->beta;
   const int k0 = PA.ck0[c], k1 = PA.ck1[c];
   const int k = k0 + (int)threadIdx.x / G, g = (int)threadIdx.x % G;
   const bool have = k < k1;
@@ -303,16 +223,6 @@ k_pp_iter(SpmvArgs A, PipeArgs PA, const int* __restrict__ agg_nodes, const doub
     p0 = PA.nptr[2 * k];
     p1 = PA.nptr[2 * k + 1];
   }
-  double pr[3] = {0, 0, 0};
-  for (int i = threadIdx.x; i < nbPrev; i += BS) {
-    pr[0] += partPrev[4 * i];
-    pr[1] += partPrev[4 * i + 1];
-    pr[2] += partPrev[4 * i + 2];
-  }
-  // snapshots taken by kernel A (block 0 below rewrites the originals)
-  const int done = st->par_next;
-  const bool first = st->justrep != 0;  // no iteration to close yet
-  const double a_prev = st->alpha_prev, g_prev = st->rz_prev;
   double own[9][DN], relf[3];
   const bool lead = have && g == 0;
   if (lead) {
@@ -347,42 +257,6 @@ k_pp_iter(SpmvArgs A, PipeArgs PA, const int* __restrict__ agg_nodes, const doub
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) sum[0][a] += __shfl_xor(sum[0][a], o, G);
   }
-  block_sum_all<3, BS>(pr, shp);
-  if (done) return;
-  const double gam = pr[0], del = pr[1], rr = pr[2];
-  double alpha, beta;
-  if (first) {
-    beta = 0;
-    alpha = gam / del;
-  } else {
-    beta = gam / g_prev;
-    alpha = gam / (del - beta * gam / a_prev);
-  }
-  if (c == 0 && threadIdx.x == 0) {
-    if (!first) {  // close the previous iteration: term alpha_(i-1) (r_(i-1), u_(i-1))
-      const double term = a_prev * g_prev;
-      st->Sall += term;
-      st->ring[st->head] = term;
-      st->head = (st->head + 1) % RING;
-      double Sw = 0;
-      for (int i = 0; i < RING; ++i) Sw += st->ring[i];
-      st->Sw = Sw;
-      st->iters += 1;
-    }
-    st->fresh = 0;
-    st->rr = rr;
-    st->rz = gam;
-    st->alpha = alpha;
-    st->beta = beta;
-    st->pq = del;
-    const double scale = st->Sall > st->xAx0 ? st->Sall : st->xAx0;
-    const bool finite = isfinite(rr) && isfinite(gam) && isfinite(del) && isfinite(st->Sw);
-    if (finite && ((st->iters >= RING && st->Sw <= st->tolA2 * scale) || rr <= st->tolr2 * st->bb || gam == 0))
-      st->done = 1;  // converged (gam = 0: r = 0 exactly), whatever the next step's alpha
-    else if (!finite || !(alpha > 0) || !isfinite(alpha))
-      st->done = 2;  // breakdown: the caller redoes the solve with the classic iteration
-  }
-  if (!(alpha > 0) || !isfinite(alpha)) return;  // the same test in every block
   double red[NM + 3];
 #pragma unroll
   for (int q = 0; q < NM + 3; ++q) red[q] = 0;
@@ -425,28 +299,76 @@ k_pp_iter(SpmvArgs A, PipeArgs PA, const int* __restrict__ agg_nodes, const doub
   }
 }
 
+// The scalar step of an iteration (kernel A's extra block): the previous
+// kernel B's partials summed in a fixed order give gamma = (r, u),
+// delta = (w, u) and (r, r); beta = gamma / gamma_old and
+// alpha = gamma / (delta - beta gamma / alpha_old) (alpha = gamma / delta on
+// the first iteration); the previous iteration's term alpha_old gamma_old
+// enters the delayed A-norm estimate; then the stopping test.
+__device__ void pp_scalars(PcgState* st, const double* __restrict__ partPrev, int nbPrev) {
+  __shared__ double shp[3][BS / 64];
+  if (st->done) return;
+  double pr4[4];
+  partial_share<4>(partPrev, nbPrev, pr4);
+  double pr[3] = {pr4[0], pr4[1], pr4[2]};
+  block_sum_all<3, BS>(pr, shp);
+  if (threadIdx.x != 0) return;
+  const bool first = st->fresh != 0;
+  const double gam = pr[0], del = pr[1], rr = pr[2];
+  double alpha, beta;
+  if (first) {
+    beta = 0;
+    alpha = gam / del;
+  } else {
+    const double a_prev = st->alpha, g_prev = st->rz;
+    beta = gam / g_prev;
+    alpha = gam / (del - beta * gam / a_prev);
+    const double term = a_prev * g_prev;  // closes the previous iteration
+    st->Sall += term;
+    st->ring[st->head] = term;
+    st->head = (st->head + 1) % RING;
+    double Sw = 0;
+    for (int i = 0; i < RING; ++i) Sw += st->ring[i];
+    st->Sw = Sw;
+    st->iters += 1;
+  }
+  st->justrep = first;
+  st->fresh = 0;
+  st->rr = rr;
+  st->rz = gam;
+  st->alpha = alpha;
+  st->beta = beta;
+  st->pq = del;
+  const double scale = st->Sall > st->xAx0 ? st->Sall : st->xAx0;
+  const bool finite = isfinite(rr) && isfinite(gam) && isfinite(del) && isfinite(st->Sw);
+  if (finite && ((st->iters >= RING && st->Sw <= st->tolA2 * scale) || rr <= st->tolr2 * st->bb || gam == 0))
+    st->done = 1;  // converged (gam = 0: r = 0 exactly), whatever the next step's alpha
+  else if (!finite || !(alpha > 0) || !isfinite(alpha))
+    st->done = 2;  // breakdown: the caller redoes the solve with the classic iteration
+}
+
 // Kernel A: out = B^-1 vin + P Ac^-1 rc on the block's aggregate.  Every
 // thread takes columns c = t, t + BS, ... of the aggregate's NM rows of
 // Ac^-1 with rc_c summed from the aggregate slots of c (fixed order); the
 // node operands and the rows are loaded before the partials.  MODE 1
-// (iterations) also snapshots the scalars kernel B reads while its block 0
-// rewrites them.
+// (iterations) runs one more block, the scalar step; its aggregate blocks
+// skip the pass once a kernel B has seen the solve stop.
 template <typename TP, int DN, int MODE>
 __global__ void __launch_bounds__(BS)
 k_pp_coarse(CoarseArgs C, PipeArgs PA, const unsigned char* __restrict__ known, const TP* __restrict__ Minv,
             const double* __restrict__ vin, double* __restrict__ out, const double* __restrict__ rcpart,
-            PcgState* st) {
+            PcgState* st, const double* __restrict__ partPrev, int nbPrev) {
   constexpr int NM = Modes<DN>::NM;
   constexpr int U = 4;  // columns per pass, loads issued together
   __shared__ double shz[NM][BS / 64];
   const int nc = C.nagg * NM;
   const int blk = blockIdx.x;
-  if (MODE == 1 && blk == 0 && threadIdx.x == 0) {
-    // taken also after convergence, so that kernel B sees done
-    st->alpha_prev = st->alpha;
-    st->rz_prev = st->rz;
-    st->justrep = st->fresh;
-    st->par_next = st->done;
+  if (MODE == 1) {
+    if (blk == C.nagg) {
+      pp_scalars(st, partPrev, nbPrev);
+      return;
+    }
+    if (st->stop) return;
   }
   const int k0 = C.agg_ptr[blk], k1 = C.agg_ptr[blk + 1];
   const int kf = k0 + threadIdx.x;
@@ -480,7 +402,14 @@ k_pp_coarse(CoarseArgs C, PipeArgs PA, const unsigned char* __restrict__ known, 
       const int c = c0 + u * BS;
       if (c < nc) {
         const int64_t base = (int64_t)(c / NM) * mx * NM + (c % NM);
-        for (int j = 0; j < mx; ++j) rv[u] += rcpart[base + (int64_t)j * NM];
+        // the first four slots unrolled (their loads in flight together)
+        double sl[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) sl[j] = j < mx ? rcpart[base + (int64_t)j * NM] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (j < mx) rv[u] += sl[j];
+        for (int j = 4; j < mx; ++j) rv[u] += rcpart[base + (int64_t)j * NM];
       }
     }
 #pragma unroll
@@ -489,7 +418,6 @@ k_pp_coarse(CoarseArgs C, PipeArgs PA, const unsigned char* __restrict__ known, 
       for (int mm = 0; mm < NM; ++mm) zc[mm] += av[u][mm] * rv[u];
   }
   block_sum_all<NM, BS>(zc, shz);
-  if (MODE == 1 && st->done) return;
   if (have) {
     double zv[DN];
 #pragma unroll
@@ -535,7 +463,7 @@ __global__ void __launch_bounds__(BS) k_pp_zero(int64_t n, double* __restrict__ 
 
 __global__ void k_pp_state(PcgState* st, double tolr2, double tolA2) {
   if (threadIdx.x == 0) {
-    st->rz = st->rz_prev = st->alpha = st->beta = st->pq = st->alpha_prev = 0;
+    st->rz = st->rz_prev = st->alpha = st->beta = st->pq = 0;
     st->rr = st->rrmax = st->rrmax_prev = st->bb = 0;
     st->tolr2 = tolr2;
     st->tolA2 = tolA2;
@@ -547,6 +475,7 @@ __global__ void k_pp_state(PcgState* st, double tolr2, double tolA2) {
     st->justrep = 0;
     st->par = 0;
     st->par_next = 1;
+    st->stop = 0;
   }
 }
 
@@ -580,24 +509,23 @@ static int pcg2p_run(const SpmvArgs& A, const PipeArgs& PA, const double* Minv, 
   hipLaunchKernelGGL(k_pp_zero, dim3(grid_for((int64_t)nslot * NM)), dim3(BS), 0, s, (int64_t)nslot * NM, W.rcpart,
                      W.rcpart2);
   const unsigned nbA = (unsigned)C.nagg;
-  const size_t lds = 0;
-  // partials: start-up pass 1 -> partB; the iteration kernels alternate with
-  // pass 2's block of four-value partials in the same array: each kernel B
-  // reads the previous kernel B's partials, so two halves are used in turn
-  double* partA = W.partB;              // kernel B of even passes (and start-up pass 0)
-  double* partBb = W.partB + 4 * MAXB;  // kernel B of odd passes (and start-up pass 1)
+  // partials: start-up pass 0 leaves x.A x in partX; the (r,u), (w,u), (r,r)
+  // partials of start-up pass 1, of every kernel B and of the replacements'
+  // w = A u pass go to partW, which the next kernel A's extra block reads
+  double* partX = W.partB;
+  double* partW = W.partB + 4 * MAXB;
+  PipeWork WX = W, WW = W;
+  WX.partB = partX;
+  WW.partB = partW;
   hipLaunchKernelGGL(k_pp_state, dim3(1), dim3(64), 0, s, W.st, tol_r * tol_r, tol_A * tol_A);
   hipLaunchKernelGGL((k_pp_spmv<DN, G, 0>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
-                     (const double*)x, x, b, W, (const double*)nullptr, 0);
-  hipLaunchKernelGGL((k_pp_coarse<double, DN, 0>), dim3(nbA), dim3(BS), lds, s, C, PA, A.known, Minv,
-                     (const double*)W.r, W.u, (const double*)W.rcpart, W.st);
-  // start-up pass 1 writes the other half (pass 0 left x.A x in partA)
-  PipeWork W1 = W;
-  W1.partB = partBb;
+                     (const double*)x, x, b, WX);
+  hipLaunchKernelGGL((k_pp_coarse<double, DN, 0>), dim3(nbA), dim3(BS), 0, s, C, PA, A.known, Minv,
+                     (const double*)W.r, W.u, (const double*)W.rcpart, W.st, (const double*)nullptr, 0);
   hipLaunchKernelGGL((k_pp_spmv<DN, G, 1>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
-                     (const double*)W.u, x, b, W1, (const double*)nullptr, 0);
-  hipLaunchKernelGGL(k_pp_init_scalars, dim3(1), dim3(BS), 0, s, (const double*)partA, (int)nbB,
-                     (const double*)partBb, (int)nbB, W.st);
+                     (const double*)W.u, x, b, WW);
+  hipLaunchKernelGGL(k_pp_init_scalars, dim3(1), dim3(BS), 0, s, (const double*)partX, (int)nbB,
+                     (const double*)partW, (int)nbB, W.st);
   if (int e = check_launch("pcg2p_start")) return e;
   int launched = 0, pass = 1;
   bool spec = false;
@@ -616,27 +544,24 @@ static int pcg2p_run(const SpmvArgs& A, const PipeArgs& PA, const double* Minv, 
         PipeWork Wr = W;
         Wr.partB = W.partR;
         hipLaunchKernelGGL((k_pp_spmv<DN, G, 0>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
-                           (const double*)x, x, b, Wr, (const double*)nullptr, 0);
-        hipLaunchKernelGGL((k_pp_coarse<double, DN, 0>), dim3(nbA), dim3(BS), lds, s, C, PA, A.known, Minv,
-                           (const double*)W.r, W.u, (const double*)W.rcpart, W.st);
-        Wr.partB = (pass & 1) ? partBb : partA;
+                           (const double*)x, x, b, Wr);
+        hipLaunchKernelGGL((k_pp_coarse<double, DN, 0>), dim3(nbA), dim3(BS), 0, s, C, PA, A.known, Minv,
+                           (const double*)W.r, W.u, (const double*)W.rcpart, W.st, (const double*)nullptr, 0);
         hipLaunchKernelGGL((k_pp_spmv<DN, G, 1>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
-                           (const double*)W.u, x, b, Wr, (const double*)nullptr, 0);
+                           (const double*)W.u, x, b, WW);
         Wr.rcpart = W.rcpart2;
         hipLaunchKernelGGL((k_pp_spmv<DN, G, 3>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
-                           (const double*)W.p, x, b, Wr, (const double*)nullptr, 0);
-        hipLaunchKernelGGL((k_pp_coarse<double, DN, 0>), dim3(nbA), dim3(BS), lds, s, C, PA, A.known, Minv,
-                           (const double*)W.s, W.q, (const double*)W.rcpart2, W.st);
+                           (const double*)W.p, x, b, Wr);
+        hipLaunchKernelGGL((k_pp_coarse<double, DN, 0>), dim3(nbA), dim3(BS), 0, s, C, PA, A.known, Minv,
+                           (const double*)W.s, W.q, (const double*)W.rcpart2, W.st, (const double*)nullptr, 0);
         hipLaunchKernelGGL((k_pp_spmv<DN, G, 4>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
-                           (const double*)W.q, x, b, Wr, (const double*)nullptr, 0);
+                           (const double*)W.q, x, b, Wr);
       }
-      hipLaunchKernelGGL((k_pp_coarse<double, DN, 1>), dim3(nbA), dim3(BS), lds, s, C, PA, A.known, Minv,
-                         (const double*)W.w, W.m, (const double*)W.rcpart, W.st);
-      const double* prev = (pass & 1) ? partBb : partA;
-      PipeWork Wk = W;
-      Wk.partB = (pass & 1) ? partA : partBb;
+      // kernel A: aggregate blocks plus the scalar step's block
+      hipLaunchKernelGGL((k_pp_coarse<double, DN, 1>), dim3(nbA + 1), dim3(BS), 0, s, C, PA, A.known, Minv,
+                         (const double*)W.w, W.m, (const double*)W.rcpart, W.st, (const double*)partW, (int)nbB);
       hipLaunchKernelGGL((k_pp_iter<DN, G>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
-                         (const double*)W.m, x, Wk, prev, (int)nbB);
+                         (const double*)W.m, x, WW);
       ++pass;
     }
     launched += chunk;

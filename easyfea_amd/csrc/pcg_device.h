@@ -179,21 +179,45 @@ __device__ inline double reduce_all(const double* __restrict__ part, int n, int 
 // of that kernel: rz_prev, rrmax_prev and par_next are per-iteration
 // snapshots that break those races.
 constexpr int RING = 32;
-// This thread's share (stride BS) of n partials of S values each.  The first
-// load sits outside the loop, so that it is in flight together with the
-// loads that follow it in the kernel (n <= BS leaves the loop empty).
+// This thread's share (stride BS) of n partials of S values each, added in
+// the order i = t, t + BS, t + 2 BS, ...  issue() loads the first MAXB / BS
+// slots with predicated loads (zero past n), unrolled, so that they are all
+// in flight together and with the loads that follow in the kernel; sum()
+// adds them where the value is needed, so that the wait for them comes late
+// (vector loads complete in order: an early add would also wait for every
+// load issued before it).  Grids of more than MAXB blocks (one block per
+// aggregate) add the rest in a loop.
+template <int S>
+struct Partials {
+  static constexpr int R = MAXB / BS;
+  double v[R][S];
+  __device__ __forceinline__ void issue(const double* __restrict__ part, int n) {
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int i = (int)threadIdx.x + j * BS;
+#pragma unroll
+      for (int k = 0; k < S; ++k) v[j][k] = i < n ? part[S * i + k] : 0.0;
+    }
+  }
+  __device__ __forceinline__ void sum(const double* __restrict__ part, int n, double (&out)[S]) const {
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      out[k] = v[0][k];
+#pragma unroll
+      for (int j = 1; j < R; ++j) out[k] += v[j][k];
+    }
+    for (int i = (int)threadIdx.x + MAXB; i < n; i += BS) {
+#pragma unroll
+      for (int k = 0; k < S; ++k) out[k] += part[S * i + k];
+    }
+  }
+};
+
 template <int S>
 __device__ inline void partial_share(const double* __restrict__ part, int n, double (&out)[S]) {
-#pragma unroll
-  for (int k = 0; k < S; ++k) out[k] = 0;
-  if ((int)threadIdx.x < n) {
-#pragma unroll
-    for (int k = 0; k < S; ++k) out[k] = part[S * threadIdx.x + k];
-  }
-  for (int i = threadIdx.x + BS; i < n; i += BS) {
-#pragma unroll
-    for (int k = 0; k < S; ++k) out[k] += part[S * i + k];
-  }
+  Partials<S> P;
+  P.issue(part, n);
+  P.sum(part, n, out);
 }
 
 struct PcgState {
@@ -206,8 +230,8 @@ struct PcgState {
   int fresh;     // no iteration to close yet: the first step uses beta = 0
   int justrep;   // the r.z / r.r partials come from a residual replacement
   int par;       // which of the two direction buffers holds p
-  int par_next;  // the buffer the current step writes (pipelined solve: snapshot of done)
-  double alpha_prev;  // pipelined solve: snapshot of alpha for the next pass
+  int par_next;  // the buffer the current step writes
+  int stop;      // pipelined solve: done as seen by the last product pass (read by the next coarse pass)
 };
 // done: 0 running, 1 converged, 2 breakdown, 3 diverged, 4 paused for a
 // residual replacement (mixed mode; the host runs the replacement kernel).
@@ -276,9 +300,8 @@ __device__ inline void pcg_step1_body(const SpmvArgs& A, const TV* __restrict__ 
     p0 = A.node_ptr[n0];
     p1 = A.node_ptr[n0 + 1];
   }
-  double pr2[2];
-  partial_share<2>(part2, nb2, pr2);
-  double prz = pr2[0], prr = pr2[1];
+  Partials<2> P2;
+  P2.issue(part2, nb2);
   // p1 < p0 never holds; testing it keeps the offset loads ahead of the branch
   if (done || p1 < p0) return;
   __shared__ double sh[BS / 64];
@@ -320,8 +343,10 @@ __device__ inline void pcg_step1_body(const SpmvArgs& A, const TV* __restrict__ 
   }
   double beta = 0;
   if (!fresh) {
-    prz = block_sum<double, BS>(prz, sh);
-    prr = block_sum<double, BS>(prr, sh2);
+    double pr2[2];
+    P2.sum(part2, nb2, pr2);
+    const double prz = block_sum<double, BS>(pr2[0], sh);
+    const double prr = block_sum<double, BS>(pr2[1], sh2);
     if (threadIdx.x == 0) {
       bc[0] = prz;
       bc[1] = prr;
@@ -490,9 +515,8 @@ __device__ inline void pcg_step2_body(int Nnode, const unsigned char* __restrict
   __shared__ double sh[BS / 64];
   __shared__ double sh2[BS / 64];
   __shared__ double bc;
-  double pp1[1];
-  partial_share<1>(part1, nb1, pp1);
-  double ppq = pp1[0];
+  Partials<1> P1s;
+  P1s.issue(part1, nb1);
   const int pn = st->par_next;
   const double* __restrict__ p = pn ? P1 : P0;
   int n = blk * BS + threadIdx.x;
@@ -509,7 +533,9 @@ __device__ inline void pcg_step2_body(int Nnode, const unsigned char* __restrict
 #pragma unroll
     for (int k = 0; k < DN * DN; ++k) Mv[k] = (double)Minv[(int64_t)n * DN * DN + k];
   }
-  ppq = block_sum<double, BS>(ppq, sh);
+  double pp1[1];
+  P1s.sum(part1, nb1, pp1);
+  const double ppq = block_sum<double, BS>(pp1[0], sh);
   if (threadIdx.x == 0) bc = ppq;
   __syncthreads();
   if (st->done) return;

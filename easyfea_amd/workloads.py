@@ -206,6 +206,7 @@ def elastic_request(spec: ElasticRequestSpec, mesh: Mesh, mode: str, precond: st
     simu = Simulations.Elastic(mesh, mat)
     simu.solver = "hip_fp64" if mode == "fp64" else "hip_mixed"
     simu.mixed_strategy = "values"  # the batch engine's algorithm, alone or batched
+    simu.pipelined = False
     simu.preconditioner = precond
     simu.max_blocks = max_blocks
     left, right = _square_sides(mesh)
@@ -275,4 +276,5 @@ def _sent_on(mesh: Mesh, Gc_e: np.ndarray, mode: str) -> Simulations.PhaseField:
     simu = Simulations.PhaseField(mesh, Models.PhaseField(mat, "Amor", "AT2", Gc_e, 0.01))
     simu.solver = "hip_fp64" if mode == "fp64" else "hip_mixed"
     simu.mixed_strategy = "values"  # the batch engine's algorithm, alone or batched
+    simu.pipelined = False
     return simu
