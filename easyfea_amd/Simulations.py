@@ -539,11 +539,18 @@ class _Simu:
             x_dev.index_copy_(0, idx_dev, pin.to("cuda", non_blocking=True))
         prev = self.last_solve.get(problemType)
         expected = prev.iterations if self.predict_polls and prev is not None and prev.status == 1 else None
-        return SolveJob(A, b_dev, x_dev, known, self._mode(), self.tol_residual, self.tol_anorm,
-                        self.replace_ratio, self.maxit, self.check_every, adaptive=self.mixed_adaptive,
-                        owner=self, precond=self.preconditioner, max_blocks=self.max_blocks,
-                        expected_iterations=expected, strategy=self.mixed_strategy, pipelined=self.pipelined,
-                        coarse_target=self.coarse_target)
+        job = SolveJob(A, b_dev, x_dev, known, self._mode(), self.tol_residual, self.tol_anorm,
+                       self.replace_ratio, self.maxit, self.check_every, adaptive=self.mixed_adaptive,
+                       owner=self, precond=self.preconditioner, max_blocks=self.max_blocks,
+                       expected_iterations=expected, strategy=self.mixed_strategy, pipelined=self.pipelined,
+                       coarse_target=self.coarse_target)
+        job.isolate = self._isolation(problemType)
+        return job
+
+    def _isolation(self, problemType):
+        """Nodes a stalled solve of ``problemType`` may give aggregates of
+        their own (``SolveJob.isolate``); none outside the phase field."""
+        return None
 
     def _solve_system_gen(self, A, b_dev: torch.Tensor, x_dev: torch.Tensor, problemType,
                           zero_dirichlet: bool = False):
@@ -1059,6 +1066,10 @@ class Thermal(_Simu):
         return self._reshape_values(values, nodeValues)
 
 
+# damage from which a node's elements count as broken for a stalled displacement solve
+ISOLATE_DAMAGE = 0.99
+
+
 class PhaseField(_Simu):
     """Staggered phase-field fracture simulation (``Simulations/_phasefield.py``)."""
 
@@ -1435,6 +1446,20 @@ class PhaseField(_Simu):
         if dofs is None:
             return y.cpu().numpy()
         return y[to_dev(np.asarray(dofs, dtype=np.int64))].cpu().numpy()
+
+    def _isolation(self, problemType):
+        """Fully damaged nodes (d >= ISOLATE_DAMAGE), most damaged first: a
+        displacement solve that stalls on a broken band is redone with them as
+        aggregates of their own (``SolveJob._isolated_retry``)."""
+        if problemType != "elastic":
+            return None
+
+        def nodes():
+            d = self._d
+            sel = torch.nonzero(d >= ISOLATE_DAMAGE).flatten()
+            return sel[torch.argsort(d[sel], descending=True, stable=True)].cpu().numpy()
+
+        return nodes
 
     def Detect_Damage(self, nodes, threshold: float = 1.0) -> bool:
         nodes = np.asarray(nodes, dtype=np.int64)

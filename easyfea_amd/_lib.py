@@ -46,7 +46,7 @@ SIGNATURES = {
     ),
     "efea_coarse_ld": (i32, [i32, i32]),
     "efea_coarse_scratch_bytes": (sz, [i32]),
-    "efea_coarse_setup": (i32, [i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp]),
+    "efea_coarse_setup": (i32, [i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, f64, vp, vp, vp]),
     "efea_pcg2_workspace_bytes": (sz, [i32, i32, i32]),
     "efea_solve_pcg2": (
         i32,

@@ -21,6 +21,7 @@ components smaller than ``target // 4`` merged into a neighbouring aggregate.
 from __future__ import annotations
 
 from dataclasses import dataclass
+from typing import Optional
 
 import numpy as np
 import scipy.sparse as sp
@@ -59,7 +60,11 @@ def _rcb(idx: np.ndarray, coord: np.ndarray, target: int, out: list) -> None:
         stack.append(ids[order[:h]])
 
 
-def aggregate(coord: np.ndarray, node_ptr: np.ndarray, node_cols: np.ndarray, target: int) -> Aggregation:
+def aggregate(coord: np.ndarray, node_ptr: np.ndarray, node_cols: np.ndarray, target: int,
+              singletons: Optional[np.ndarray] = None) -> Aggregation:
+    """``singletons``: nodes that become aggregates of their own (after the
+    bisection, the component split and the merge), so that the coarse space
+    holds their translations exactly (fully damaged nodes)."""
     Nn = coord.shape[0]
     parts: list = []
     _rcb(np.arange(Nn), coord, max(int(target), 1), parts)
@@ -84,6 +89,16 @@ def aggregate(coord: np.ndarray, node_ptr: np.ndarray, node_cols: np.ndarray, ta
             if keep.any():
                 target_of[c] = nb[keep][np.argmax(w[keep])]
         comp = target_of[comp]
+    if singletons is not None and len(singletons):
+        # the other nodes of an aggregate that loses nodes may fall apart
+        # (two lips of a broken band): its pieces become aggregates
+        single = np.zeros(Nn, dtype=bool)
+        single[np.asarray(singletons, dtype=np.int64)] = True
+        keep = (comp[rows] == comp[cols]) & ~single[rows] & ~single[cols]
+        G2 = sp.csr_matrix((np.ones(int(keep.sum())), (rows[keep], cols[keep])), shape=(Nn, Nn))
+        _, comp = connected_components(G2, directed=False)
+        comp = comp.astype(np.int64)
+        comp[single] = comp.max() + 1 + np.arange(int(single.sum()))
     uniq, agg = np.unique(comp, return_inverse=True)
     nagg = uniq.size
     agg = agg.astype(np.int32)

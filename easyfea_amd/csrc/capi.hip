@@ -91,7 +91,7 @@ int batch_active(void*);
 int coarse_ld(int, int);
 size_t coarse_scratch_doubles(int);
 int coarse_setup(int, int, const int*, const int*, const int*, const int*, const int*, const int*, const int*,
-                 const double*, const unsigned char*, const double*, int, double*, double*, hipStream_t);
+                 const double*, const unsigned char*, const double*, int, double, double*, double*, hipStream_t);
 size_t pcg2_workspace_bytes(int, int, int);
 int solve_pcg2(int, int, int, int, const int*, const int*, const double*, const float*, const float*,
                const unsigned char*, const void*, int, const int*, const int*, const double*, const double*, int,
@@ -361,12 +361,6 @@ int efea_solve_pcg2p(int Nnode, int dof_n, int group, const int32_t* node_ptr, c
     set_error("efea_solve_pcg2p: leading dimension %d < %d", ld, coarse_ld(nagg, dof_n));
     return EFEA_ERR_ARG;
   }
-  const int nm = dof_n == 1 ? 1 : (dof_n == 2 ? 3 : 6);
-  if ((size_t)nagg * nm * sizeof(double) > 32768) {
-    set_error("efea_solve_pcg2p: %d coarse unknowns, at most %d (the restriction is summed in LDS)", nagg * nm,
-              (int)(32768 / sizeof(double)));
-    return EFEA_ERR_ARG;
-  }
   if (workspace_bytes < pcg2p_workspace_bytes(Nnode, dof_n, nagg * chunks_per_agg)) {
     set_error("efea_solve_pcg2p: workspace of %zu bytes, %zu needed", workspace_bytes,
               pcg2p_workspace_bytes(Nnode, dof_n, nagg * chunks_per_agg));
@@ -465,15 +459,15 @@ size_t efea_coarse_scratch_bytes(int ld) { return ld > 0 ? 8 * coarse_scratch_do
 int efea_coarse_setup(int dof_n, int nblocks, const int32_t* pair_ptr, const int32_t* pair_node,
                       const int32_t* pair_pos, const int32_t* blk_I, const int32_t* blk_J, const int32_t* node_ptr,
                       const int32_t* node_cols, const double* values64, const uint8_t* known, const double* rel,
-                      int ld, double* Ainv, double* scratch, void* stream) {
-  if (nblocks < 0 || ld <= 0 || !Ainv || !scratch ||
+                      int ld, double diag_shift, double* Ainv, double* scratch, void* stream) {
+  if (nblocks < 0 || ld <= 0 || !Ainv || !scratch || !(diag_shift >= 0.0) ||
       (nblocks > 0 && (!pair_ptr || !pair_node || !pair_pos || !blk_I || !blk_J || !node_ptr || !node_cols ||
                        !values64 || !rel))) {
     set_error("efea_coarse_setup: null pointer or negative size");
     return EFEA_ERR_ARG;
   }
   return coarse_setup(dof_n, nblocks, pair_ptr, pair_node, pair_pos, blk_I, blk_J, node_ptr, node_cols, values64,
-                      known, rel, ld, Ainv, scratch, STREAM(stream));
+                      known, rel, ld, diag_shift, Ainv, scratch, STREAM(stream));
 }
 
 size_t efea_pcg2_workspace_bytes(int Nnode, int dof_n, int nagg) { return pcg2_workspace_bytes(Nnode, dof_n, nagg); }

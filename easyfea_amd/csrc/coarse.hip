@@ -140,10 +140,14 @@ k_coarse_assemble(int ncb, const int* __restrict__ pair_ptr, const int* __restri
   }
 }
 
-// Unit diagonal where a mode vanishes on the unknown DOFs (and in the padding).
-__global__ void k_coarse_guard(int ld, double* __restrict__ Ac) {
+// Unit diagonal where a mode vanishes on the unknown DOFs (and in the padding);
+// the other diagonal entries scaled by 1 + shift (shift = 0: unchanged).
+__global__ void k_coarse_guard(int ld, double* __restrict__ Ac, double shift) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < ld && Ac[(int64_t)k * ld + k] == 0.0) Ac[(int64_t)k * ld + k] = 1.0;
+  if (k < ld) {
+    const double v = Ac[(int64_t)k * ld + k];
+    Ac[(int64_t)k * ld + k] = v == 0.0 ? 1.0 : v * (1.0 + shift);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -376,14 +380,14 @@ static int gj_invert_blocked(double* B, int ld, double* scratch, hipStream_t s) 
 template <int DN>
 static int coarse_setup_t(int ncb, const int* pair_ptr, const int* pair_node, const int* pair_pos,
                           const int* blk_I, const int* blk_J, const int* node_ptr, const int* node_cols,
-                          const double* vals, const unsigned char* known, const double* rel, int ld, double* Ac,
-                          double* Pb, hipStream_t s) {
+                          const double* vals, const unsigned char* known, const double* rel, int ld,
+                          double shift, double* Ac, double* Pb, hipStream_t s) {
   const int64_t n2 = (int64_t)ld * ld;
   hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, n2, Ac, 0.0);
   if (ncb > 0)
     hipLaunchKernelGGL((k_coarse_assemble<DN>), dim3((unsigned)((ncb + 3) / 4)), dim3(256), 0, s, ncb, pair_ptr,
                        pair_node, pair_pos, blk_I, blk_J, node_ptr, node_cols, vals, known, rel, Ac, ld);
-  hipLaunchKernelGGL(k_coarse_guard, dim3((unsigned)((ld + 255) / 256)), dim3(256), 0, s, ld, Ac);
+  hipLaunchKernelGGL(k_coarse_guard, dim3((unsigned)((ld + 255) / 256)), dim3(256), 0, s, ld, Ac, shift);
   if (ld > GJ_BLAS_MIN) {
     if (int e = gj_invert_blocked(Ac, ld, Pb, s)) return e;
   } else {
@@ -395,20 +399,21 @@ static int coarse_setup_t(int ncb, const int* pair_ptr, const int* pair_node, co
 
 int coarse_setup(int dof_n, int ncb, const int* pair_ptr, const int* pair_node, const int* pair_pos,
                  const int* blk_I, const int* blk_J, const int* node_ptr, const int* node_cols, const double* vals,
-                 const unsigned char* known, const double* rel, int ld, double* Ac, double* Pb, hipStream_t s) {
+                 const unsigned char* known, const double* rel, int ld, double shift, double* Ac, double* Pb,
+                 hipStream_t s) {
   if (ld % GB != 0) {
     set_error("coarse_setup: leading dimension %d not a multiple of %d", ld, GB);
     return EFEA_ERR_ARG;
   }
   if (dof_n == 1)
     return coarse_setup_t<1>(ncb, pair_ptr, pair_node, pair_pos, blk_I, blk_J, node_ptr, node_cols, vals, known,
-                             rel, ld, Ac, Pb, s);
+                             rel, ld, shift, Ac, Pb, s);
   if (dof_n == 2)
     return coarse_setup_t<2>(ncb, pair_ptr, pair_node, pair_pos, blk_I, blk_J, node_ptr, node_cols, vals, known,
-                             rel, ld, Ac, Pb, s);
+                             rel, ld, shift, Ac, Pb, s);
   if (dof_n == 3)
     return coarse_setup_t<3>(ncb, pair_ptr, pair_node, pair_pos, blk_I, blk_J, node_ptr, node_cols, vals, known,
-                             rel, ld, Ac, Pb, s);
+                             rel, ld, shift, Ac, Pb, s);
   set_error("coarse_setup: dof_n %d", dof_n);
   return EFEA_ERR_ARG;
 }

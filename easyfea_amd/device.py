@@ -9,7 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 from dataclasses import dataclass
-from typing import Optional
+from typing import Callable, Optional
 
 import numpy as np
 import scipy.sparse as sp
@@ -208,11 +208,12 @@ class CoarseSpace:
     """Aggregates, rigid-body modes and the pair lists of ``Ac = P^T A P`` on
     the device (mesh data: built once per pattern and aggregate size)."""
 
-    def __init__(self, pattern: "CsrPattern", target: int):
+    def __init__(self, pattern: "CsrPattern", target: int, singletons: Optional[np.ndarray] = None):
         h = pattern.host
         self.target = target
         self.dof_n = pattern.dof_n
-        self.ag = aggregate(pattern.coord, h.node_ptr, h.node_cols, target)
+        self.n_isolated = 0 if singletons is None else int(len(singletons))
+        self.ag = aggregate(pattern.coord, h.node_ptr, h.node_cols, target, singletons)
         cp = coarse_pairs(self.ag, h.node_ptr, h.node_cols)
         self.nagg = self.ag.nagg
         self.nm = n_modes(pattern.dim, pattern.dof_n)
@@ -237,12 +238,14 @@ class CoarseSpace:
             c[group] = PipeChunks(self, pattern, group)
         return c[group]
 
-    def setup(self, A: "DeviceCsr", known: Optional[torch.Tensor]) -> torch.Tensor:
-        """``Ac^-1`` (ld x ld, FP64) of ``A``'s current FP64 values."""
+    def setup(self, A: "DeviceCsr", known: Optional[torch.Tensor], shift: float = 0.0) -> torch.Tensor:
+        """``Ac^-1`` (ld x ld, FP64) of ``A``'s current FP64 values, the
+        diagonal of ``Ac`` scaled by ``1 + shift`` first."""
         Ainv = torch.empty(self.ld * self.ld, dtype=torch.float64, device="cuda")
         _lib.call("efea_coarse_setup", self.dof_n, self.ncb, ptr(self.pair_ptr), ptr(self.pair_node),
                   ptr(self.pair_pos), ptr(self.blk_I), ptr(self.blk_J), ptr(A.node_ptr), ptr(A.node_cols),
-                  ptr(A.values), ptr(known), ptr(self.rel), self.ld, ptr(Ainv), ptr(self.scratch), stream_ptr())
+                  ptr(A.values), ptr(known), ptr(self.rel), self.ld, shift, ptr(Ainv), ptr(self.scratch),
+                  stream_ptr())
         return Ainv
 
 
@@ -418,6 +421,7 @@ class SolveInfo:
     coarse_reused: bool = False
     refine: bool = False  # FP32 inner PCG inside FP64 iterative refinement
     pipelined: bool = False  # pipelined two-level PCG (two launches per iteration)
+    isolated: int = 0  # nodes given aggregates of their own after a stalled solve
 
 
 # Mixed-precision strategies of a single two-level solve:
@@ -438,6 +442,13 @@ REFINE_MIN_MATRIX_BYTES = 128 * 2**20
 # the pipelined PCG (efea_solve_pcg2p): two launches per iteration instead of
 # three, at 8 instead of 5 work vectors.
 PIPELINE_MAX_COARSE = 4096
+# A phase-field displacement solve that stalls with fully damaged nodes is
+# redone with those nodes as singleton aggregates; the attempts before stop
+# after this many iterations (or 4x the previous solve's), the coarse space
+# of the retry holds at most ISOLATE_MAX_COARSE unknowns.
+ISOLATE_FIRST_MAXIT = 5000
+ISOLATE_MAX_COARSE = 8192
+ISOLATE_SHIFTS = (1e-8, 1e-6, 1e-4)
 
 
 # Lagged coarse inverse.  Any symmetric positive definite Ac^-1 keeps
@@ -509,6 +520,11 @@ class SolveJob:
         assert pipelined in ("auto", True, False), pipelined
         self.pipelined_mode = pipelined
         self.pipelined = False  # decided by run(), as refine
+        # isolate(): the nodes whose elements have lost their stiffness (a
+        # phase-field displacement solve); set, a two-level solve that stalls
+        # is redone with those nodes as aggregates of their own
+        self.isolate: Optional[Callable[[], np.ndarray]] = None
+        self.n_isolated = 0
 
     def setup(self, precision: int) -> None:
         """Block-Jacobi inverse (and the FP32 value copy) for ``precision``."""
@@ -559,7 +575,7 @@ class SolveJob:
         return SolveInfo(int(raw[0]), float(raw[1]), int(raw[2]), int(raw[3]),
                          "fp64" if self.precision == 64 else "mixed", self.fallback, self.precond,
                          self.coarse_reused, self.refine and self.precision == 32,
-                         self.pipelined and self.precision == 64 and self.precond == "two-level")
+                         self.pipelined and self.precision == 64 and self.precond == "two-level", self.n_isolated)
 
     def _wants_pipeline(self) -> bool:
         if self.precision != 64 or self.precond != "two-level" or self.pipelined_mode is False:
@@ -643,12 +659,20 @@ class SolveJob:
 
     def run(self, ws: Optional[Workspace] = None, raise_on_fail: bool = True) -> SolveInfo:
         """Solves now with ``efea_solve_pcg``; a mixed solve that breaks down,
-        diverges or stalls is redone in FP64 from the same initial guess."""
+        diverges or stalls is redone in FP64 from the same initial guess.
+        With ``isolate`` set, the attempts before the last are capped at
+        ``ISOLATE_FIRST_MAXIT`` (or 4x the expected count) iterations and a
+        solve that still fails is redone, two-level, with the nodes
+        ``isolate()`` names as aggregates of their own."""
         ws = ws or Workspace()
         self.refine = self._wants_refine()
         self.pipelined = self._wants_pipeline()
-        if self.pipelined and self.x0 is None:
+        isolating = self.isolate is not None and self.A.pattern.coord is not None
+        if (self.pipelined or isolating) and self.x0 is None:
             self.x0 = self.x.clone()
+        maxit = self.maxit
+        if isolating:
+            self.maxit = min(maxit, max(ISOLATE_FIRST_MAXIT, 4 * (self.expected_iterations or 0)))
         self.setup(self.precision)
         rc, si = self._solve_once(ws)
         if rc != 0 and self.pipelined:
@@ -662,9 +686,57 @@ class SolveJob:
         if rc != 0 and self.precision == 32:
             self.restart_fp64()
             rc, si = self._solve_once(ws)
+        self.maxit = maxit
+        if rc != 0 and isolating:
+            rc, si = self._isolated_retry(ws, rc, si)
         if rc != 0 and raise_on_fail:
             raise _lib.EfeaError("efea_solve_pcg", rc, _lib.last_error())
         return si
+
+    def _isolated_retry(self, ws: Workspace, rc: int, si: SolveInfo):
+        """A stalled two-level solve of a body with fully damaged nodes: their
+        elements keep only the residual stiffness, so local mechanisms of the
+        broken band (and the rigid motions of separated pieces) have near-zero
+        energy that block Jacobi and the aggregate modes cannot see.  Given
+        aggregates of their own, those nodes' translations enter the coarse
+        space and the dense coarse solve resolves them.  The retry starts
+        again from the initial guess, in FP64, without the lagged inverse."""
+        nodes = np.asarray(self.isolate(), dtype=np.int64)
+        cs0 = self.A.coarse(self.coarse_target)
+        budget = (ISOLATE_MAX_COARSE - cs0.nagg * cs0.nm) // max(cs0.nm, 1)
+        if nodes.size == 0 or budget <= 0:
+            return rc, si
+        nodes = nodes[:budget]
+        key = (self.coarse_target, nodes.tobytes())
+        cache = self.A.__dict__.setdefault("_isolated_coarse", {})
+        if key not in cache:
+            cache.clear()
+            cache[key] = CoarseSpace(self.A.pattern, cs0.target, nodes)
+        self.x.copy_(self.x0)
+        self.fallback = True
+        self.pipelined = False
+        self.precision = 64
+        self.refine = False
+        self.precond = "two-level"
+        self.cs = cache[key]
+        self.n_isolated = int(nodes.size)
+        self.coarse_reused, self._cache = False, None
+        self.Minv = torch.empty(self.A.Nn * self.A.dof_n * self.A.dof_n, dtype=torch.float64, device="cuda")
+        _lib.call("efea_bj_setup", 64, self.A.Nn, self.A.dof_n, ptr(self.A.node_ptr), ptr(self.A.node_cols),
+                  ptr(self.A.values), ptr(self.known), ptr(self.Minv), stream_ptr())
+        expected, self.expected_iterations = self.expected_iterations, None
+        # Ac is nearly singular here (the isolated modes keep the residual
+        # stiffness): the blocked unpivoted Gauss-Jordan inverse has a
+        # backward error of a few 1e-7 of max |Ac| and can lose definiteness
+        # (a breakdown); a larger diagonal shift is tried then
+        for shift in ISOLATE_SHIFTS:
+            self.Ainv = self.cs.setup(self.A, self.known, shift)
+            rc, si = self._solve_once(ws)
+            if rc == 0 or si.status != 2:
+                break
+            self.x.copy_(self.x0)
+        self.expected_iterations = expected
+        return rc, si
 
 
 def solve(A: DeviceCsr, b: torch.Tensor, x: torch.Tensor, known: Optional[torch.Tensor], mode: str,

@@ -199,7 +199,10 @@ int efea_solve_pcg_noalloc(int mode, int Nnode, int dof_n, int group, const int3
  * pair_ptr[k]..pair_ptr[k+1]-1, and inverts it in place (blocked Gauss-Jordan,
  * FP64; above order 1024 with 256-column steps whose products are rocBLAS
  * DGEMMs) into Ainv (ld x ld, ld = efea_coarse_ld; scratch:
- * efea_coarse_scratch_bytes(ld) bytes).
+ * efea_coarse_scratch_bytes(ld) bytes).  Diagonal entries of modes that
+ * vanish on the unknown DOFs are set to 1, the others scaled by
+ * 1 + diag_shift (0 in general; a small shift keeps the inverse of a nearly
+ * singular Ac positive definite, which no-pivot elimination cannot promise).
  * efea_solve_pcg2 is efea_solve_pcg with that preconditioner; agg_ptr /
  * agg_nodes list the nodes of each aggregate.  EasyFEA solves the reduced
  * system directly (Solvers.py:225-394); the preconditioner changes the
@@ -209,7 +212,7 @@ size_t efea_coarse_scratch_bytes(int ld);
 int efea_coarse_setup(int dof_n, int nblocks, const int32_t* pair_ptr, const int32_t* pair_node,
                       const int32_t* pair_pos, const int32_t* blk_I, const int32_t* blk_J, const int32_t* node_ptr,
                       const int32_t* node_cols, const double* values64, const uint8_t* known, const double* rel,
-                      int ld, double* Ainv, double* scratch, void* stream);
+                      int ld, double diag_shift, double* Ainv, double* scratch, void* stream);
 size_t efea_pcg2_workspace_bytes(int Nnode, int dof_n, int nagg);
 int efea_solve_pcg2(int mode, int Nnode, int dof_n, int group, const int32_t* node_ptr, const int32_t* node_cols,
                     const double* values64, const float* values32, const float* corr32, const uint8_t* known,
@@ -229,8 +232,8 @@ int efea_solve_pcg2(int mode, int Nnode, int dof_n, int group, const int32_t* no
  * positions [chunk_k0[c], chunk_k1[c]) of agg_nodes and writing its share of
  * P^T w to slot chunk_slot[c] = a * chunks_per_agg + (its index among its
  * aggregate a's chunks); chunk_node_ptr (2 Nnode) holds node_ptr[agg_nodes[k]]
- * and node_ptr[agg_nodes[k] + 1] for each position k.  At most 4096 coarse
- * unknowns.  Workspace: efea_pcg2p_workspace_bytes(Nnode, dof_n,
+ * and node_ptr[agg_nodes[k] + 1] for each position k.  Workspace:
+ * efea_pcg2p_workspace_bytes(Nnode, dof_n,
  * nagg * chunks_per_agg).  Every 64 iterations the recursive vectors are
  * recomputed from x and the direction (residual replacement). */
 /* Replaces __Solver_1 (Simulations/Solvers.py:502-553) with _Solve_Axb's

@@ -506,19 +506,59 @@ def bounded_lsq(A: sp.csr_matrix, b: np.ndarray, lb: np.ndarray, ub: np.ndarray)
     return np.array(res["x"])
 
 
+def _lsq_kkt(H: np.ndarray, g: np.ndarray, lb: np.ndarray, ub: np.ndarray, x: np.ndarray) -> float:
+    """Largest violation of the first-order conditions of
+    ``min 1/2 |A x - b|^2`` over the box (``H = A^T A``, ``g = A^T b``):
+    gradient zero on free variables, >= 0 at the lower bound, <= 0 at the
+    upper bound; relative to ``max |g|``."""
+    grad = H @ x - g
+    at_lb, at_ub = x <= lb, x >= ub
+    free = ~(at_lb | at_ub)
+    v = max(np.abs(grad[free]).max(initial=0.0), (-grad[at_lb]).max(initial=0.0), grad[at_ub].max(initial=0.0))
+    return v / max(np.abs(g).max(), 1e-300)
+
+
 def bounded_lsq_exact(A, b: np.ndarray, lb: np.ndarray, ub: np.ndarray) -> np.ndarray:
     """The minimiser the reference's call approximates: ``lsq_linear`` with
     the bounded-variable least-squares active-set method (``method="bvls"``,
-    Stark and Parker 1995) on the dense matrix, tolerance 1e-14.  The
+    Stark and Parker 1995) on the dense matrix, tolerance 1e-14, then
+    primal-dual active-set steps (Hintermueller, Ito and Kunisch 2002) on the
+    normal equations, each solving the free variables exactly, until the
+    active sets repeat; the first-order conditions are checked to 1e-11.
+    BVLS alone can stop on a small cost change short of the minimum (an AT1
+    damage system of 637 unknowns: cost 8 % above it, d off by 0.55).  The
     reference's trust-region call stops on a relative cost change of 1e-10,
     which on ill-conditioned damage systems leaves it measurably short of
     this point (percent-level in x, 1e-3 in the cost on a 1D AT1-like system)."""
     from scipy import optimize
 
     Ad = A.toarray() if sp.issparse(A) else np.asarray(A, float)
-    res = optimize.lsq_linear(Ad, np.asarray(b, float).ravel(), bounds=(lb, ub), method="bvls", tol=1e-14,
-                              verbose=0)
-    return np.array(res["x"])
+    b = np.asarray(b, float).ravel()
+    lb, ub = np.asarray(lb, float), np.asarray(ub, float)
+    res = optimize.lsq_linear(Ad, b, bounds=(lb, ub), method="bvls", tol=1e-14, verbose=0)
+    x = np.clip(np.array(res["x"]), lb, ub)
+    H, g = Ad.T @ Ad, Ad.T @ b
+    hd = np.diag(H)
+    lam = H @ x - g
+    sets = None
+    for _ in range(500):
+        t = x - lam / hd
+        AL, AU = t <= lb, t >= ub
+        F = ~(AL | AU)
+        new = (AL.tobytes(), AU.tobytes())
+        if new == sets:
+            break
+        sets = new
+        x = np.where(AL, lb, np.where(AU, ub, 0.0))
+        if F.any():
+            x[F] = np.linalg.solve(H[np.ix_(F, F)], g[F] - H[np.ix_(F, ~F)] @ x[~F])
+        lam = H @ x - g
+        lam[F] = 0.0
+    x = np.clip(x, lb, ub)
+    kkt = _lsq_kkt(H, g, lb, ub, x)
+    if kkt > 1e-11:
+        raise RuntimeError(f"bounded_lsq_exact: first-order conditions violated by {kkt:.2e}")
+    return x
 
 
 def solve_r1(K: sp.csr_matrix, b: np.ndarray, dir_dofs, dir_vals, Ndof, orphan_dofs=()):
