@@ -56,8 +56,8 @@ SIGNATURES = {
     "efea_pcg2p_workspace_bytes": (sz, [i32, i32, i32]),
     "efea_solve_pcg2p": (
         i32,
-        [i32, i32, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, i32, i32, vp, vp, vp, i32, vp, vp, vp, vp, sz, f64,
-         f64, i32, i32, i32, vp, vp],
+        [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, i32, i32, vp, vp, vp, i32, vp, vp, vp,
+         vp, sz, f64, f64, i32, i32, i32, vp, vp],
     ),
     "efea_batch_create": (i32, [i32, i32, i32, i32, i32, C.POINTER(vp)]),
     "efea_batch_destroy": (i32, [vp]),

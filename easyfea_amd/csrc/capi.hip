@@ -98,10 +98,10 @@ int solve_pcg2(int, int, int, int, const int*, const int*, const double*, const 
                const double*, double*, void*, double, double, double, int, int, int, int, hipStream_t, double*);
 
 size_t pcg2p_workspace_bytes(int, int, int);
-int solve_pcg2p(int, int, int, const int*, const int*, const double*, const unsigned char*, const double*, int,
-                const int*, const int*, const double*, const double*, int, int, const int*, const int*, const int*,
-                int, const int*, const double*, double*, void*, double, double, int, int, int, hipStream_t, double*,
-                void*);
+int solve_pcg2p(int, int, int, int, const int*, const int*, const double*, const float*, const float*,
+                const unsigned char*, const void*, int, const int*, const int*, const double*, const double*, int, int,
+                const int*, const int*, const int*, int, const int*, const double*, double*, void*, double, double,
+                int, int, int, hipStream_t, double*, void*);
 
 }  // namespace efea
 
@@ -340,8 +340,9 @@ size_t efea_pcg2p_workspace_bytes(int Nnode, int dof_n, int nslot) {
   return pcg2p_workspace_bytes(Nnode, dof_n, nslot);
 }
 
-int efea_solve_pcg2p(int Nnode, int dof_n, int group, const int32_t* node_ptr, const int32_t* node_cols,
-                     const double* values64, const uint8_t* known, const double* Minv, int nagg,
+int efea_solve_pcg2p(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr,
+                     const int32_t* node_cols, const double* values64, const float* values32, const float* corr32,
+                     const uint8_t* known, const void* Minv, int nagg,
                      const int32_t* agg_ptr, const int32_t* agg_nodes, const double* rel, const double* Ainv, int ld,
                      int nchunk, const int32_t* chunk_k0, const int32_t* chunk_k1, const int32_t* chunk_slot,
                      int chunks_per_agg, const int32_t* chunk_node_ptr, const double* b, double* x,
@@ -351,6 +352,10 @@ int efea_solve_pcg2p(int Nnode, int dof_n, int group, const int32_t* node_ptr, c
       !Minv || !agg_ptr || !agg_nodes || !rel || !Ainv || !chunk_k0 || !chunk_k1 || !chunk_slot ||
       !chunk_node_ptr || !b || !x || !workspace || !info) {
     set_error("efea_solve_pcg2p: null pointer or empty system");
+    return EFEA_ERR_ARG;
+  }
+  if ((precision != 64 && precision != 32) || (precision == 32 && !values32)) {
+    set_error("efea_solve_pcg2p: precision %d (64, or 32 with FP32 values)", precision);
     return EFEA_ERR_ARG;
   }
   if (nchunk > 1024) {
@@ -366,7 +371,8 @@ int efea_solve_pcg2p(int Nnode, int dof_n, int group, const int32_t* node_ptr, c
               pcg2p_workspace_bytes(Nnode, dof_n, nagg * chunks_per_agg));
     return EFEA_ERR_SIZE;
   }
-  return solve_pcg2p(Nnode, dof_n, group, node_ptr, node_cols, values64, known, Minv, nagg, agg_ptr, agg_nodes, rel,
+  return solve_pcg2p(precision, Nnode, dof_n, group, node_ptr, node_cols, values64, values32, corr32, known, Minv,
+                     nagg, agg_ptr, agg_nodes, rel,
                      Ainv, ld, nchunk, chunk_k0, chunk_k1, chunk_slot, chunks_per_agg, chunk_node_ptr, b, x,
                      workspace, tol_residual, tol_anorm, maxit, check_every, first_check, STREAM(stream), info,
                      nullptr);

@@ -32,6 +32,11 @@
 // PIPE_REPLACE iterations the vectors are recomputed from x and p (residual
 // replacement, Cools and Vanroose): r = b - A x, u = M r, w = A u, s = A p,
 // q = M s, z = A q, six launches.
+// Mixed precision (TM = float): the iterations' products and the block-Jacobi
+// inverse use the FP32 copies (with the FP32 values' per-node row-sum
+// correction); the start-up and replacement passes, every 32 iterations, use
+// the FP64 matrix, so that r = b - A x is the FP64 residual at each
+// replacement and the solve converges to the FP64 solution.
 // EasyFEA solves directly (Simulations/Solvers.py:225-394); this changes the
 // iteration's synchronisation pattern, not the solution (same stopping test).
 #include "coarse_device.h"
@@ -54,7 +59,8 @@ struct PipeArgs {
   const int* nptr;
 };
 
-constexpr int PIPE_REPLACE = 64;
+constexpr int PIPE_REPLACE = 64;       // FP64 iterations
+constexpr int PIPE_REPLACE_MIXED = 32;  // FP32 products (64 let the recurrences drift into breakdowns)
 
 struct PipeWork {
   double *r, *u, *w, *m, *z, *q, *s, *p;
@@ -200,7 +206,7 @@ k_pp_spmv(SpmvArgs A, PipeArgs PA, const int* __restrict__ agg_nodes, const doub
 // decision come from kernel A's extra block (three scalar loads), so the
 // chain node -> offsets -> product starts at once; the own rows' nine
 // operands are loaded with the gather.  One block reduction ends it.
-template <int DN, int G>
+template <int DN, int G, typename TM>
 __global__ void __launch_bounds__(BS)
 k_pp_iter(SpmvArgs A, PipeArgs PA, const int* __restrict__ agg_nodes, const double* __restrict__ rel,
           const double* __restrict__ m, double* __restrict__ x, PipeWork W) {
@@ -246,16 +252,24 @@ k_pp_iter(SpmvArgs A, PipeArgs PA, const int* __restrict__ agg_nodes, const doub
 #pragma unroll
   for (int a = 0; a < DN; ++a) sum[0][a] = 0;
   if (have)
-    gather_rows<double, DN, G, 1>((const double*)A.vals, A.node_cols, p0, p1 - p0, g,
-                                  [&](int col, double (&xv)[1][DN]) {
+    gather_rows<TM, DN, G, 1>((const TM*)A.vals, A.node_cols, p0, p1 - p0, g,
+                              [&](int col, double (&xv)[1][DN]) {
 #pragma unroll
-                                    for (int bb = 0; bb < DN; ++bb) xv[0][bb] = m[(int64_t)col * DN + bb];
-                                  },
-                                  sum);
+                                for (int bb = 0; bb < DN; ++bb) xv[0][bb] = m[(int64_t)col * DN + bb];
+                              },
+                              sum);
 #pragma unroll
   for (int a = 0; a < DN; ++a) {
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) sum[0][a] += __shfl_xor(sum[0][a], o, G);
+  }
+  if (lead && A.corr) {  // FP32 values: the rounding's row sums on the own rows
+#pragma unroll
+    for (int a = 0; a < DN; ++a) {
+      const TM* cr = (const TM*)A.corr + (int64_t)n * DN * DN + a * DN;
+#pragma unroll
+      for (int bb = 0; bb < DN; ++bb) sum[0][a] += (double)cr[bb] * own[0][bb];
+    }
   }
   double red[NM + 3];
 #pragma unroll
@@ -496,11 +510,15 @@ __global__ void __launch_bounds__(BS) k_pp_init_scalars(const double* __restrict
 
 size_t pcg2p_workspace_bytes(int Nnode, int dof_n, int nslot) { return pcg2p_workspace_size(Nnode, dof_n, nslot); }
 
-template <int DN, int G>
-static int pcg2p_run(const SpmvArgs& A, const PipeArgs& PA, const double* Minv, const CoarseArgs& C,
-                     const double* b, double* x, void* ws, double tol_r, double tol_A, int maxit, int check_every,
-                     int first_check, hipStream_t s, double* info, PcgState* hs) {
+// A: the FP64 matrix (start-up, replacements, residuals); Ait: the
+// iterations' matrix (A itself, or the FP32 copy with its correction);
+// Minv: TP (the job's block-Jacobi inverse, FP32 with FP32 values).
+template <int DN, int G, typename TM, typename TP>
+static int pcg2p_run(const SpmvArgs& A, const SpmvArgs& Ait, const PipeArgs& PA, const TP* Minv,
+                     const CoarseArgs& C, const double* b, double* x, void* ws, double tol_r, double tol_A,
+                     int maxit, int check_every, int first_check, hipStream_t s, double* info, PcgState* hs) {
   const int Nnode = A.Nnode;
+  const int replace_every = sizeof(TM) == 4 ? PIPE_REPLACE_MIXED : PIPE_REPLACE;
   constexpr int NM = Modes<DN>::NM;
   const int nslot = C.nagg * PA.maxch;
   const PipeWork W = carve_pipe(ws, Nnode, DN, nslot);
@@ -520,7 +538,7 @@ static int pcg2p_run(const SpmvArgs& A, const PipeArgs& PA, const double* Minv, 
   hipLaunchKernelGGL(k_pp_state, dim3(1), dim3(64), 0, s, W.st, tol_r * tol_r, tol_A * tol_A);
   hipLaunchKernelGGL((k_pp_spmv<DN, G, 0>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
                      (const double*)x, x, b, WX);
-  hipLaunchKernelGGL((k_pp_coarse<double, DN, 0>), dim3(nbA), dim3(BS), 0, s, C, PA, A.known, Minv,
+  hipLaunchKernelGGL((k_pp_coarse<TP, DN, 0>), dim3(nbA), dim3(BS), 0, s, C, PA, A.known, Minv,
                      (const double*)W.r, W.u, (const double*)W.rcpart, W.st, (const double*)nullptr, 0);
   hipLaunchKernelGGL((k_pp_spmv<DN, G, 1>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
                      (const double*)W.u, x, b, WW);
@@ -538,29 +556,29 @@ static int pcg2p_run(const SpmvArgs& A, const PipeArgs& PA, const double* Minv, 
   while (true) {
     const int chunk = launched == 0 && first_check > 0 ? first_check : check_every;
     for (int k = 0; k < chunk; ++k) {
-      if (pass > 1 && (pass - 1) % PIPE_REPLACE == 0) {
+      if (pass > 1 && (pass - 1) % replace_every == 0) {
         // residual replacement: r = b - A x, u = M r, w = A u (with the
         // partials the next pass reads), s = A p, q = M s, z = A q
         PipeWork Wr = W;
         Wr.partB = W.partR;
         hipLaunchKernelGGL((k_pp_spmv<DN, G, 0>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
                            (const double*)x, x, b, Wr);
-        hipLaunchKernelGGL((k_pp_coarse<double, DN, 0>), dim3(nbA), dim3(BS), 0, s, C, PA, A.known, Minv,
+        hipLaunchKernelGGL((k_pp_coarse<TP, DN, 0>), dim3(nbA), dim3(BS), 0, s, C, PA, A.known, Minv,
                            (const double*)W.r, W.u, (const double*)W.rcpart, W.st, (const double*)nullptr, 0);
         hipLaunchKernelGGL((k_pp_spmv<DN, G, 1>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
                            (const double*)W.u, x, b, WW);
         Wr.rcpart = W.rcpart2;
         hipLaunchKernelGGL((k_pp_spmv<DN, G, 3>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
                            (const double*)W.p, x, b, Wr);
-        hipLaunchKernelGGL((k_pp_coarse<double, DN, 0>), dim3(nbA), dim3(BS), 0, s, C, PA, A.known, Minv,
+        hipLaunchKernelGGL((k_pp_coarse<TP, DN, 0>), dim3(nbA), dim3(BS), 0, s, C, PA, A.known, Minv,
                            (const double*)W.s, W.q, (const double*)W.rcpart2, W.st, (const double*)nullptr, 0);
         hipLaunchKernelGGL((k_pp_spmv<DN, G, 4>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
                            (const double*)W.q, x, b, Wr);
       }
       // kernel A: aggregate blocks plus the scalar step's block
-      hipLaunchKernelGGL((k_pp_coarse<double, DN, 1>), dim3(nbA + 1), dim3(BS), 0, s, C, PA, A.known, Minv,
+      hipLaunchKernelGGL((k_pp_coarse<TP, DN, 1>), dim3(nbA + 1), dim3(BS), 0, s, C, PA, A.known, Minv,
                          (const double*)W.w, W.m, (const double*)W.rcpart, W.st, (const double*)partW, (int)nbB);
-      hipLaunchKernelGGL((k_pp_iter<DN, G>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
+      hipLaunchKernelGGL((k_pp_iter<DN, G, TM>), dim3(nbB), dim3(BS), 0, s, Ait, PA, C.agg_nodes, C.rel,
                          (const double*)W.m, x, WW);
       ++pass;
     }
@@ -594,22 +612,27 @@ static int pcg2p_run(const SpmvArgs& A, const PipeArgs& PA, const double* Minv, 
   return pcg_finish_info(&fin, rr_true, info);
 }
 
-int solve_pcg2p(int Nnode, int dof_n, int group, const int* node_ptr, const int* node_cols, const double* vals64,
-                const unsigned char* known, const double* Minv, int nagg, const int* agg_ptr, const int* agg_nodes,
-                const double* rel, const double* Ainv, int ld, int nchunk, const int* ck0, const int* ck1,
-                const int* cslot, int maxch, const int* nptr, const double* b, double* x, void* ws,
-                double tol_r, double tol_A, int maxit, int check_every, int first_check, hipStream_t s,
-                double* info, void* host_state_buf) {
+int solve_pcg2p(int precision, int Nnode, int dof_n, int group, const int* node_ptr, const int* node_cols,
+                const double* vals64, const float* vals32, const float* corr32, const unsigned char* known,
+                const void* Minv, int nagg, const int* agg_ptr, const int* agg_nodes, const double* rel,
+                const double* Ainv, int ld, int nchunk, const int* ck0, const int* ck1, const int* cslot, int maxch,
+                const int* nptr, const double* b, double* x, void* ws, double tol_r, double tol_A, int maxit,
+                int check_every, int first_check, hipStream_t s, double* info, void* host_state_buf) {
   SpmvArgs A{Nnode, node_ptr, node_cols, vals64, known, nullptr, 0};
+  SpmvArgs A32{Nnode, node_ptr, node_cols, vals32, known, corr32, 0};
   CoarseArgs C{nagg, agg_ptr, agg_nodes, rel, Ainv, ld, nullptr};
   PipeArgs PA{nchunk, ck0, ck1, cslot, maxch, nptr};
   if (check_every < 1) check_every = 1;
   PcgState local;
   PcgState* hs = host_state_buf ? (PcgState*)host_state_buf : &local;
-#define RUNP(D, GV)                                                                                          \
-  if (dof_n == D && group == GV)                                                                            \
-    return pcg2p_run<D, GV>(A, PA, Minv, C, b, x, ws, tol_r, tol_A, maxit, check_every, first_check, s, info, \
-                            hs);
+#define RUNP(D, GV)                                                                                              \
+  if (dof_n == D && group == GV) {                                                                              \
+    if (precision == 32)                                                                                        \
+      return pcg2p_run<D, GV, float, float>(A, A32, PA, (const float*)Minv, C, b, x, ws, tol_r, tol_A, maxit,   \
+                                            check_every, first_check, s, info, hs);                             \
+    return pcg2p_run<D, GV, double, double>(A, A, PA, (const double*)Minv, C, b, x, ws, tol_r, tol_A, maxit,    \
+                                            check_every, first_check, s, info, hs);                             \
+  }
   EFEA_GROUPS(RUNP)
 #undef RUNP
   set_error("solve_pcg2p: unsupported dof_n %d / group %d", dof_n, group);

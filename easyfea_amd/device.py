@@ -575,10 +575,10 @@ class SolveJob:
         return SolveInfo(int(raw[0]), float(raw[1]), int(raw[2]), int(raw[3]),
                          "fp64" if self.precision == 64 else "mixed", self.fallback, self.precond,
                          self.coarse_reused, self.refine and self.precision == 32,
-                         self.pipelined and self.precision == 64 and self.precond == "two-level", self.n_isolated)
+                         self.pipelined and self.precond == "two-level", self.n_isolated)
 
     def _wants_pipeline(self) -> bool:
-        if self.precision != 64 or self.precond != "two-level" or self.pipelined_mode is False:
+        if self.precond != "two-level" or self.pipelined_mode is False or self._wants_refine():
             return False
         cs = self.A.coarse(self.coarse_target)
         if cs.nagg * cs.nm > PIPELINE_MAX_COARSE or cs.pipeline(self.A.pattern, self.A.group).nchunk > 1024:
@@ -605,10 +605,11 @@ class SolveJob:
         """(check_every, first_check) of the host's state read-backs.  With
         an expected iteration count (the previous solve of a sequence), the
         first read-back comes just before it and later ones every 8
-        iterations; FP32 solves keep the regular polls, where their residual
-        replacements happen."""
+        iterations; classic FP32 solves keep the regular polls, where their
+        residual replacements happen (the pipelined solve replaces on the
+        device)."""
         n = self.expected_iterations
-        if self.precision == 64 and n:
+        if (self.precision == 64 or self.pipelined) and n:
             return min(self.check_every, 8), max(1, int(0.95 * n))
         return self.check_every, 0
 
@@ -618,13 +619,14 @@ class SolveJob:
         check_every, first_check = self._polls()
         v32 = A.values32 if self.precision == 32 else None
         c32 = A.corr32 if self.precision == 32 else None
-        if self.precond == "two-level" and self.pipelined and self.precision == 64:
+        if self.precond == "two-level" and self.pipelined:
             cs = self.cs
             pc = cs.pipeline(A.pattern, A.group)
             nbytes = _lib.load().efea_pcg2p_workspace_bytes(A.Nn, A.dof_n, pc.nslot)
             buf = ws.get(nbytes)
-            rc = _lib.call("efea_solve_pcg2p", A.Nn, A.dof_n, A.group, ptr(A.node_ptr), ptr(A.node_cols),
-                           ptr(A.values), ptr(self.known), ptr(self.Minv), cs.nagg, ptr(cs.agg_ptr),
+            rc = _lib.call("efea_solve_pcg2p", self.precision, A.Nn, A.dof_n, A.group, ptr(A.node_ptr),
+                           ptr(A.node_cols), ptr(A.values), ptr(v32), ptr(c32), ptr(self.known), ptr(self.Minv),
+                           cs.nagg, ptr(cs.agg_ptr),
                            ptr(cs.agg_nodes), ptr(cs.rel), ptr(self.Ainv), cs.ld, pc.nchunk, ptr(pc.k0), ptr(pc.k1),
                            ptr(pc.slot), pc.maxch, ptr(pc.nptr), ptr(self.b), ptr(self.x), ptr(buf), nbytes,
                            self.tol_residual, self.tol_anorm, self.maxit, check_every, first_check, stream_ptr(),

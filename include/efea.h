@@ -223,10 +223,13 @@ int efea_solve_pcg2(int mode, int Nnode, int dof_n, int group, const int32_t* no
                     int max_blocks, void* stream, double* info);
 
 /* Pipelined two-level PCG (Ghysels and Vanroose's preconditioned pipelined
- * CG) with the preconditioner of efea_solve_pcg2, FP64: two kernels per
- * iteration instead of three (the restriction P^T w and both dot products
- * ride on the product n = A m), for latency-bound systems.  Same arguments,
- * stopping test and info as efea_solve_pcg2 (mode 64), plus the node chunks
+ * CG) with the preconditioner of efea_solve_pcg2: two kernels per iteration
+ * instead of three (the restriction P^T w and both dot products ride on the
+ * product n = A m), for latency-bound systems.  precision 64: FP64
+ * throughout; 32: the iterations' products use values32 (+ corr32, as
+ * efea_solve_pcg) and Minv is FP32, the start-up and replacement passes use
+ * values64.  Same arguments, stopping test and info as efea_solve_pcg2 (mode
+ * 64 or 32 with FP64 vectors), plus the node chunks
  * the product pass runs over, one block each (at most 1024): runs of at most
  * 256 / group nodes of one aggregate in agg_nodes order, chunk c holding
  * positions [chunk_k0[c], chunk_k1[c]) of agg_nodes and writing its share of
@@ -234,13 +237,15 @@ int efea_solve_pcg2(int mode, int Nnode, int dof_n, int group, const int32_t* no
  * aggregate a's chunks); chunk_node_ptr (2 Nnode) holds node_ptr[agg_nodes[k]]
  * and node_ptr[agg_nodes[k] + 1] for each position k.  Workspace:
  * efea_pcg2p_workspace_bytes(Nnode, dof_n,
- * nagg * chunks_per_agg).  Every 64 iterations the recursive vectors are
- * recomputed from x and the direction (residual replacement). */
+ * nagg * chunks_per_agg).  Every 64 iterations (32 with FP32 values) the
+ * recursive vectors are recomputed from x and the direction with the FP64
+ * matrix (residual replacement). */
 /* Replaces __Solver_1 (Simulations/Solvers.py:502-553) with _Solve_Axb's
  * backends (Solvers.py:225-394), as efea_solve_pcg does. */
 size_t efea_pcg2p_workspace_bytes(int Nnode, int dof_n, int nslot);
-int efea_solve_pcg2p(int Nnode, int dof_n, int group, const int32_t* node_ptr, const int32_t* node_cols,
-                     const double* values64, const uint8_t* known, const double* Minv, int nagg,
+int efea_solve_pcg2p(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr,
+                     const int32_t* node_cols, const double* values64, const float* values32, const float* corr32,
+                     const uint8_t* known, const void* Minv, int nagg,
                      const int32_t* agg_ptr, const int32_t* agg_nodes, const double* rel, const double* Ainv, int ld,
                      int nchunk, const int32_t* chunk_k0, const int32_t* chunk_k1, const int32_t* chunk_slot,
                      int chunks_per_agg, const int32_t* chunk_node_ptr, const double* b, double* x,

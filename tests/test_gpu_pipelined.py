@@ -2,7 +2,9 @@
 with two launches per iteration gives the direct solution within 1e-8 on a
 2D cantilever, a 3D cube and a cracked phase-field body over staggered
 increments; it is deterministic and takes about the classic solve's
-iteration count."""
+iteration count.  With FP32 matrix values in the iterations (the mixed
+"values" strategy, forced at these sizes) it reaches the same 1e-8 on the
+cantilever and the cracked body."""
 
 import numpy as np
 import pytest
@@ -46,6 +48,32 @@ def test_pipelined_cantilever_matches_direct_and_classic():
     assert np.array_equal(s1.Solve(), u1)
 
 
+@pytest.fixture
+def _fp32_values(monkeypatch):
+    """FP32 matrix values in the Krylov products at every size."""
+    import easyfea_amd.device as dv
+
+    monkeypatch.setattr(dv, "MIXED_MIN_MATRIX_BYTES", 0)
+
+
+def test_pipelined_mixed_cantilever_matches_direct(_fp32_values):
+    s1 = workloads.cantilever("hip_mixed")
+    s1.preconditioner = "two-level"
+    s1.pipelined = True
+    u1 = s1.Solve()
+    i1 = s1.last_solve["elastic"]
+    mesh = s1.mesh
+    ref = O.ElasticOracle(mesh.connect, mesh.coord, "QUAD4", 210000.0, 0.3, True, 13.0,
+                          {"SEG2": mesh.dict_groupElem[ElemType.SEG2].connect})
+    ref.add_dirichlet(mesh.Nodes_Conditions(lambda x, y, z: x == 0), [0, 0], ["x", "y"])
+    ref.add_surfLoad(mesh.Nodes_Conditions(lambda x, y, z: x == 120.0), [-800.0 / 13.0 ** 2], ["y"])
+    err = _rel(u1, ref.Solve())
+    print(f"cantilever, FP32 values: pipelined {i1.iterations} it, mode {i1.mode}, fallback {i1.fallback}, "
+          f"vs direct {err:.2e}")
+    assert i1.pipelined and i1.mode == "mixed" and not i1.fallback and i1.status == 1
+    assert err < 1e-8
+
+
 def test_pipelined_cube_matches_direct():
     from easyfea_amd.mesh import hexa8_cube
 
@@ -65,11 +93,12 @@ def test_pipelined_cube_matches_direct():
     assert info.pipelined and err < 1e-8
 
 
-def test_pipelined_cracked_body_staggered_matches_oracle():
+@pytest.mark.parametrize("solver", ["hip_fp64", "hip_mixed"])
+def test_pipelined_cracked_body_staggered_matches_oracle(solver, _fp32_values):
     mesh = ea.tri3_notched_square(60)
     mat = Models.Elastic.Isotropic(2, 210000.0, 0.3, planeStress=False, thickness=1.0)
     simu = Simulations.PhaseField(mesh, Models.PhaseField(mat, "Miehe", "AT2", 2.7, 0.02))
-    simu.solver = "hip_fp64"
+    simu.solver = solver
     simu.pipelined = True
     ref = O.PhaseFieldOracle(mesh.connect, mesh.coord, "TRI3", 210000.0, 0.3, False, 1.0, "Miehe", "AT2", 2.7, 0.02)
     top = mesh.Nodes_Conditions(lambda x, y, z: y == 1.0)
@@ -85,7 +114,8 @@ def test_pipelined_cracked_body_staggered_matches_oracle():
         simu.Save_Iter()
         ref.Save_Iter()
         i = simu.last_solve["elastic"]
-        rows.append((dep, i.pipelined, i.iterations, _rel(u, ur), _rel(d, dr), float(dr.max())))
-    print("dep, pipelined, iterations, u err, d err, max d:", rows)
+        rows.append((dep, i.pipelined, i.iterations, _rel(u, ur), _rel(d, dr), float(dr.max()), i.mode, i.fallback))
+    print(f"{solver}: dep, pipelined, iterations, u err, d err, max d, mode, fallback:", rows)
     for r in rows:
-        assert r[1] and r[3] < 1e-8 and r[4] < 1e-8, rows
+        assert r[1] and r[3] < 1e-8 and r[4] < 1e-8 and not r[7], rows
+        assert r[6] == ("mixed" if solver == "hip_mixed" else "fp64"), rows
