@@ -138,6 +138,95 @@ def cpu_baseline(mesh, steps):
     return time.perf_counter() - t0, states, np.array(forces)
 
 
+def _profiled_traffic(kname, ndof, nnz, precision):
+    """HBM bytes per launch of ``kname`` on this system from the committed
+    PMC summaries (``profiles/*_traffic.json``), or None."""
+    pdir = os.path.join(ROOT, "profiles")
+    for fn in sorted(os.listdir(pdir)) if os.path.isdir(pdir) else []:
+        if not fn.endswith("_traffic.json"):
+            continue
+        with open(os.path.join(pdir, fn)) as f:
+            t = json.load(f)
+        k = t.get("kernel", "").replace("void ", "").replace("efea::", "")
+        if t.get("ndof") == ndof and t.get("nnz", nnz) == nnz and t.get("precision") == precision and \
+                k.startswith(kname + "<"):
+            return t["hbm_bytes_per_launch"]
+    return None
+
+
+def dominant_roofline(simu, problemType, reps):
+    """Roofline of the last ``problemType`` solve's dominant kernel: the
+    product pass of a pipelined solve, step 1 otherwise."""
+    if simu.last_solve[problemType].pipelined:
+        return pp_iter_roofline(simu, problemType, reps)
+    return step1_roofline(simu, problemType, reps)
+
+
+def pp_iter_roofline(simu, problemType, reps):
+    """Average duration of the pipelined solve's product pass (k_pp_iter:
+    n = A m, the eight vector updates, the partials) over ``reps`` launches
+    through ``efea_pp_iter_bench``: the solve's own kernel, chunks, matrix
+    (FP32 copy with its correction in a mixed solve), mask and aggregate
+    coordinates; HIP events on the stream the kernels run on."""
+    import torch
+
+    from easyfea_amd import _lib
+    from easyfea_amd.device import ptr, stream_ptr
+
+    A = simu._device_K(problemType) if hasattr(simu, "Detect_Damage") else simu._device_K()
+    job = simu.last_job
+    info = simu.last_solve[problemType]
+    assert job.pipelined and job.cs is not None, "the last solve was not the pipelined one"
+    precision = 64 if info.mode == "fp64" else 32
+    known = simu._mask_cache[problemType][1]
+    cs = job.cs
+    pc = cs.pipeline(A.pattern, A.group)
+    x = torch.empty(A.Ndof, dtype=torch.float64, device="cuda")
+    nbytes = _lib.load().efea_pcg2p_workspace_bytes(A.Nn, A.dof_n, pc.nslot)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    if precision == 32:
+        A.make_fp32()
+    head = (precision, A.Nn, A.dof_n, A.group, ptr(A.node_ptr), ptr(A.node_cols), ptr(A.values),
+            ptr(A.values32 if precision == 32 else None), ptr(A.corr32 if precision == 32 else None), ptr(known),
+            cs.nagg, ptr(cs.agg_nodes), ptr(cs.rel), pc.nchunk, ptr(pc.k0), ptr(pc.k1), ptr(pc.slot), pc.maxch,
+            ptr(pc.nptr), ptr(x), ptr(ws), nbytes)
+    _lib.call("efea_pp_iter_bench", *head, 20, stream_ptr())
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    _lib.call("efea_pp_iter_bench", *head, reps, stream_ptr())
+    e1.record(s)
+    e1.synchronize()
+    dt = e0.elapsed_time(e1) / 1e3 / reps  # six fill / state kernels, then reps product passes
+    hA = A.host
+    nnz, nnz_node, Nn, Ndof, DN = A.nnz, hA.node_cols.size, A.Nn, A.Ndof, A.dof_n
+    vb = 4 if precision == 32 else 8
+    nm = cs.nm
+    # values (+ FP32 row-sum correction) + node columns + per-position node and
+    # offsets + mask + aggregate coordinates + nine vectors read (m gathered
+    # once) + eight written + chunk bounds and the partials
+    algo_bytes = (nnz * vb + nnz_node * 4 + Nn * (4 + 8) + Ndof + Nn * 24 + Ndof * 8 * 9 + Ndof * 8 * 8
+                  + pc.nchunk * 12 + pc.nchunk * 8 * (4 + nm))
+    if precision == 32:
+        algo_bytes += Nn * DN * DN * 4
+    kname = "k_pp_iter"
+    traffic = _profiled_traffic(kname, Ndof, nnz, precision)
+    peak = 8.0e12
+    return {
+        "kernel": f"{kname}<{DN},{A.group},{'float' if precision == 32 else 'double'}>",
+        "bound": "hbm",
+        "achieved": algo_bytes / dt / 1e9,
+        "peak": peak / 1e9,
+        "unit": "GB/s",
+        "frac": algo_bytes / dt / peak,
+        "traffic": traffic,
+        "bytes_per_launch": algo_bytes,
+        "avg_launch_us": dt * 1e6,
+        "launches_timed": reps,
+        "nnz": nnz,
+    }
+
+
 def step1_roofline(simu, problemType, reps):
     """Average duration of the solve's dominant kernel - step 1: p update
     fused into the node-block SpMV (k_pcg2_step1 in a two-level solve,
@@ -180,17 +269,8 @@ def step1_roofline(simu, problemType, reps):
     if precision == 32:
         algo_bytes += Nn * A.dof_n * A.dof_n * 4
     csr_bytes = nnz * (vb + 4) + (Ndof + 1) * 4 + 2 * Ndof * vb
-    traffic = None
     kname = "k_pcg2_step1" if two_level else "k_pcg_step1"
-    pdir = os.path.join(ROOT, "profiles")
-    for fn in sorted(os.listdir(pdir)) if os.path.isdir(pdir) else []:
-        if not fn.endswith("_traffic.json"):
-            continue
-        with open(os.path.join(pdir, fn)) as f:
-            t = json.load(f)
-        if t.get("ndof") == Ndof and t.get("nnz", nnz) == nnz and t.get("precision") == precision and \
-                t.get("kernel", "").startswith(kname + "<"):
-            traffic = t["hbm_bytes_per_launch"]
+    traffic = _profiled_traffic(kname, Ndof, nnz, precision)
     peak = 8.0e12
     return {
         "kernel": f"{kname}<{'float' if precision == 32 else 'double'},{A.dof_n},{A.group}>",
@@ -252,7 +332,7 @@ def bench_single(args, workload) -> dict:
 
     def on_run(simu, precond, solver):
         if precond == "two-level" and solver == "hip_mixed":
-            roofs["head"] = step1_roofline(simu, "elastic", args.spmv_reps)
+            roofs["head"] = dominant_roofline(simu, "elastic", args.spmv_reps)
 
     res = solve_bench.run(workload, 64, 5, on_run=on_run)
     cores = len(os.sched_getaffinity(0))
@@ -521,8 +601,8 @@ def bench_sent(args) -> dict:
         accuracy["reference"] = ("oracle FP64 history with sparse direct (SuperLU) solves, all 100 increments "
                                  "(tests/golden/sent200_history.npz)")
 
-    roof = step1_roofline(mixed["simu"], "elastic", args.spmv_reps)
-    roof_fp64 = step1_roofline(res["fp64"]["simu"], "elastic", args.spmv_reps) if "fp64" in res else None
+    roof = dominant_roofline(mixed["simu"], "elastic", args.spmv_reps)
+    roof_fp64 = dominant_roofline(res["fp64"]["simu"], "elastic", args.spmv_reps) if "fp64" in res else None
 
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:

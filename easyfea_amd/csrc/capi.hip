@@ -102,6 +102,9 @@ int solve_pcg2p(int, int, int, int, const int*, const int*, const double*, const
                 const unsigned char*, const void*, int, const int*, const int*, const double*, const double*, int, int,
                 const int*, const int*, const int*, int, const int*, const double*, double*, void*, double, double,
                 int, int, int, hipStream_t, double*, void*);
+int pp_iter_bench(int, int, int, int, const int*, const int*, const double*, const float*, const float*,
+                  const unsigned char*, int, const int*, const double*, int, const int*, const int*, const int*, int,
+                  const int*, double*, void*, int, hipStream_t);
 
 }  // namespace efea
 
@@ -338,6 +341,29 @@ int efea_solve_pcg_noalloc(int mode, int Nnode, int dof_n, int group, const int3
 
 size_t efea_pcg2p_workspace_bytes(int Nnode, int dof_n, int nslot) {
   return pcg2p_workspace_bytes(Nnode, dof_n, nslot);
+}
+
+int efea_pp_iter_bench(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr,
+                       const int32_t* node_cols, const double* values64, const float* values32, const float* corr32,
+                       const uint8_t* known, int nagg, const int32_t* agg_nodes, const double* rel, int nchunk,
+                       const int32_t* chunk_k0, const int32_t* chunk_k1, const int32_t* chunk_slot,
+                       int chunks_per_agg, const int32_t* chunk_node_ptr, double* x, void* workspace,
+                       size_t workspace_bytes, int reps, void* stream) {
+  if (Nnode <= 0 || nagg <= 0 || nchunk <= 0 || nchunk > 1024 || chunks_per_agg <= 0 || reps < 1 || !node_ptr ||
+      !node_cols || !agg_nodes || !rel || !chunk_k0 || !chunk_k1 || !chunk_slot || !chunk_node_ptr || !x ||
+      !workspace || (precision == 64 && !values64) || (precision == 32 && !values32) ||
+      (precision != 64 && precision != 32)) {
+    set_error("efea_pp_iter_bench: null pointer, empty system, bad precision, reps < 1 or over 1024 chunks");
+    return EFEA_ERR_ARG;
+  }
+  if (workspace_bytes < pcg2p_workspace_bytes(Nnode, dof_n, nagg * chunks_per_agg)) {
+    set_error("efea_pp_iter_bench: workspace of %zu bytes, %zu needed", workspace_bytes,
+              pcg2p_workspace_bytes(Nnode, dof_n, nagg * chunks_per_agg));
+    return EFEA_ERR_SIZE;
+  }
+  return pp_iter_bench(precision, Nnode, dof_n, group, node_ptr, node_cols, values64, values32, corr32, known, nagg,
+                       agg_nodes, rel, nchunk, chunk_k0, chunk_k1, chunk_slot, chunks_per_agg, chunk_node_ptr, x,
+                       workspace, reps, STREAM(stream));
 }
 
 int efea_solve_pcg2p(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr,

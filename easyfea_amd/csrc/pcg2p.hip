@@ -639,4 +639,57 @@ int solve_pcg2p(int precision, int Nnode, int dof_n, int group, const int* node_
   return EFEA_ERR_ARG;
 }
 
+// Timing harness of the product pass k_pp_iter (the pipelined solve's
+// dominant kernel): its grid, chunks, matrix (FP64, or FP32 with the row-sum
+// correction), mask and aggregate coordinates, in the state of a running
+// solve (an iteration past the first: the beta terms are formed), on zeroed
+// vectors (the timing does not depend on the values).  Launches: four zero
+// fills, two state kernels, then reps product passes back to back.
+__global__ void k_pp_bench_state(PcgState* st) {
+  if (threadIdx.x == 0) {
+    st->done = 0;
+    st->stop = 0;
+    st->justrep = 0;
+    st->alpha = 0.5;
+    st->beta = 0.25;
+  }
+}
+
+template <int DN, int G, typename TM>
+static int pp_iter_bench_run(const SpmvArgs& Ait, const PipeArgs& PA, const int* agg_nodes, const double* rel,
+                             int nagg, double* x, void* ws, int reps, hipStream_t s) {
+  const PipeWork W = carve_pipe(ws, Ait.Nnode, DN, nagg * PA.maxch);
+  const int64_t nd = (int64_t)Ait.Nnode * DN;
+  const unsigned gz = grid_for(nd);
+  hipLaunchKernelGGL(k_pp_zero, dim3(gz), dim3(BS), 0, s, nd, W.r, W.u);
+  hipLaunchKernelGGL(k_pp_zero, dim3(gz), dim3(BS), 0, s, nd, W.w, W.m);
+  hipLaunchKernelGGL(k_pp_zero, dim3(gz), dim3(BS), 0, s, nd, W.z, W.q);
+  hipLaunchKernelGGL(k_pp_zero, dim3(gz), dim3(BS), 0, s, nd, W.s, W.p);
+  hipLaunchKernelGGL(k_pp_zero, dim3(gz), dim3(BS), 0, s, nd, x, W.r);
+  hipLaunchKernelGGL(k_pp_state, dim3(1), dim3(64), 0, s, W.st, 0.0, 0.0);
+  hipLaunchKernelGGL(k_pp_bench_state, dim3(1), dim3(64), 0, s, W.st);
+  for (int r = 0; r < reps; ++r)
+    hipLaunchKernelGGL((k_pp_iter<DN, G, TM>), dim3((unsigned)PA.nchunk), dim3(BS), 0, s, Ait, PA, agg_nodes, rel,
+                       (const double*)W.m, x, W);
+  return check_launch("pp_iter_bench");
+}
+
+int pp_iter_bench(int precision, int Nnode, int dof_n, int group, const int* node_ptr, const int* node_cols,
+                  const double* vals64, const float* vals32, const float* corr32, const unsigned char* known,
+                  int nagg, const int* agg_nodes, const double* rel, int nchunk, const int* ck0, const int* ck1,
+                  const int* cslot, int maxch, const int* nptr, double* x, void* ws, int reps, hipStream_t s) {
+  SpmvArgs A{Nnode, node_ptr, node_cols, vals64, known, nullptr, 0};
+  SpmvArgs A32{Nnode, node_ptr, node_cols, vals32, known, corr32, 0};
+  PipeArgs PA{nchunk, ck0, ck1, cslot, maxch, nptr};
+#define RUNB(D, GV)                                                                                         \
+  if (dof_n == D && group == GV) {                                                                         \
+    if (precision == 32) return pp_iter_bench_run<D, GV, float>(A32, PA, agg_nodes, rel, nagg, x, ws, reps, s); \
+    return pp_iter_bench_run<D, GV, double>(A, PA, agg_nodes, rel, nagg, x, ws, reps, s);                  \
+  }
+  EFEA_GROUPS(RUNB)
+#undef RUNB
+  set_error("pp_iter_bench: unsupported dof_n %d / group %d", dof_n, group);
+  return EFEA_ERR_ARG;
+}
+
 }  // namespace efea

@@ -252,6 +252,21 @@ int efea_solve_pcg2p(int precision, int Nnode, int dof_n, int group, const int32
                      void* workspace, size_t workspace_bytes, double tol_residual, double tol_anorm, int maxit,
                      int check_every, int first_check, void* stream, double* info);
 
+/* Timing harness of efea_solve_pcg2p's dominant kernel (the product pass:
+ * n = A m, the eight vector updates, the dot-product and restriction
+ * partials): reps launches back to back on `stream`, on the solve's chunks,
+ * matrix (precision 64: values64; 32: values32 + corr32), mask and aggregate
+ * coordinates, in the state of an iteration past the first, after four fill
+ * and two state kernels; x (Ndof doubles) and the workspace
+ * (efea_pcg2p_workspace_bytes) are overwritten.  Benchmark support, no
+ * counterpart in the reference. */
+int efea_pp_iter_bench(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr,
+                       const int32_t* node_cols, const double* values64, const float* values32, const float* corr32,
+                       const uint8_t* known, int nagg, const int32_t* agg_nodes, const double* rel, int nchunk,
+                       const int32_t* chunk_k0, const int32_t* chunk_k1, const int32_t* chunk_slot,
+                       int chunks_per_agg, const int32_t* chunk_node_ptr, double* x, void* workspace,
+                       size_t workspace_bytes, int reps, void* stream);
+
 /* Continuous batching of Dirichlet-reduced solves (the serving loop).
  * A batch holds up to `capacity` systems of one (mode, dof_n, group) class and
  * advances all of them with one launch per PCG step; systems join
