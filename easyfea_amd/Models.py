@@ -99,10 +99,76 @@ class Isotropic(_Observable):
         return np.atleast_1d(np.asarray(self.E, dtype=np.float64))
 
 
+def _mandel_rotation(axis_1: np.ndarray, axis_2: np.ndarray) -> np.ndarray:
+    """The 6 x 6 map P of Kelvin-Mandel strain vectors from the material
+    basis (axis_1, axis_2, axis_1 x axis_2) to the global one: column k is
+    the rotated Kelvin-Mandel basis tensor k, rotated as p T p^T with the
+    axes as the columns of p.  ``C_global = P C_material P^T``."""
+    p = np.stack([axis_1, axis_2, np.cross(axis_1, axis_2)], axis=1)
+    r = np.sqrt(0.5)
+    pairs = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+    P = np.empty((6, 6))
+    for k, (i, j) in enumerate(pairs):
+        T = np.zeros((3, 3))
+        if i == j:
+            T[i, i] = 1.0
+        else:
+            T[i, j] = T[j, i] = r
+        R = p @ T @ p.T
+        P[:, k] = [R[0, 0], R[1, 1], R[2, 2], R[1, 2] / r, R[0, 2] / r, R[0, 1] / r]
+    return P
+
+
+class Anisotropic(_Observable):
+    """Anisotropic linear elastic material (``Models/Elastic/_laws.py:
+    1111-1259``): ``C`` (6 x 6, or one per element) given in the anisotropy
+    basis (axis1, axis2), kept in Kelvin-Mandel notation in the global basis,
+    ``C = P C_material P^T``.  Like the reference, a 6 x 6 ``C`` is taken as
+    given whatever ``useVoigtNotation`` says (its Kelvin-Mandel scaling only
+    applies to 3 x 3 inputs, ``_laws.py:1230-1246``).  3D; the device builds
+    its element matrices on the matrix cores (HEXA8, TETRA10)."""
+
+    def __init__(self, dim: int, C, useVoigtNotation: bool, axis1=(1, 0, 0), axis2=(0, 1, 0), thickness=1.0):
+        super().__init__()
+        assert dim == 3, "anisotropic laws run in 3D on the device"
+        a1 = np.asarray(axis1, dtype=float).reshape(-1)
+        a2 = np.asarray(axis2, dtype=float).reshape(-1)
+        assert a1.size == 3 and a2.size == 3, "axis1 and axis2 must be 3D vectors"
+        assert a1 @ a2 <= 1e-12, "axis1 and axis2 must be perpendicular"
+        self.dim = dim
+        self.planeStress = False
+        self.thickness = float(thickness)
+        self._axis1 = a1 / np.linalg.norm(a1)
+        self._axis2 = a2 / np.linalg.norm(a2)
+        self.Set_C(C, useVoigtNotation)
+
+    def Set_C(self, C, useVoigtNotation: bool = True, update_S: bool = True) -> None:
+        """``Anisotropic.Set_C`` (``_laws.py:1172-1188``)."""
+        C = np.asarray(C, dtype=float)
+        assert C.shape[-2:] == (6, 6) and C.ndim in (2, 3), "C must be (6, 6) or (Ne, 6, 6)"
+        sym = np.linalg.norm(np.swapaxes(C, -1, -2) - C, axis=(-2, -1)) / np.linalg.norm(C, axis=(-2, -1))
+        assert np.max(sym) <= 1e-12, "The matrix is not symmetrical."
+        P = _mandel_rotation(self._axis1, self._axis2)
+        self.C = np.ascontiguousarray(P @ C @ P.T)
+        if update_S:
+            self.S = np.linalg.inv(self.C)
+        if "_observers" in self.__dict__:
+            self.Need_Update()
+
+    @property
+    def isHeterogeneous(self) -> bool:
+        return self.C.ndim == 3
+
+    @property
+    def coef(self) -> float:
+        return np.sqrt(2)
+
+
 class Elastic:
     """Namespace mirroring ``EasyFEA.Models.Elastic``."""
 
     Isotropic = Isotropic
+    Anisotropic = Anisotropic
 
 
 class PhaseField(_Observable):

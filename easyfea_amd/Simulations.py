@@ -32,7 +32,7 @@ from . import _lib
 from ._utils import ElemType, MatrixType
 from .device import DeviceCsr, DeviceMesh, SolveJob, Workspace, drive, lincomb, ptr, stream_ptr, to_dev
 from .mesh import Mesh
-from .Models import Isotropic
+from .Models import Anisotropic, Isotropic
 from .Models import PhaseField as PhaseFieldModel
 from .Models import Thermal as ThermalModel
 from .quadrature import N_pg, dN_pg, gauss_points
@@ -635,7 +635,7 @@ class Elastic(_Simu):
     """Linear elastic simulation (``Simulations/_elastic.py``)."""
 
     def __init__(self, mesh: Mesh, model: Isotropic, folder: str = "", verbosity=False):
-        assert isinstance(model, Isotropic), "model must be a elastic model"
+        assert isinstance(model, (Isotropic, Anisotropic)), "model must be a elastic model"
         self._needUpdate = True
         super().__init__(mesh, model, folder, verbosity)
         assert mesh.dim == model.dim, "mesh and material dimensions differ"
@@ -690,10 +690,18 @@ class Elastic(_Simu):
         A = dev.csr_map(self.model.dim)
         Ke = A.new_entries()
         tab, nPg = dev.tables(MatrixType.rigi)
-        E = to_dev(self.model._E_array())
-        _lib.call("efea_elastic_ke", dev.code, 64, dev.Ne, ptr(dev.connect), ptr(dev.coord), ptr(tab), nPg, ptr(E),
-                  1 if self.model.isHeterogeneous else 0, self.model.v, int(self.model.planeStress),
-                  self.model.thickness, ptr(Ke), stream_ptr())
+        if isinstance(self.model, Anisotropic):
+            # a general constitutive matrix: B^T C B on the matrix cores
+            if dev.code not in (_lib.ELEM_CODE["HEXA8"], _lib.ELEM_CODE["TETRA10"]):
+                raise NotImplementedError("anisotropic laws: HEXA8 and TETRA10 meshes on the device")
+            C = to_dev(np.ascontiguousarray(self.model.C).reshape(-1))
+            _lib.call("efea_elastic_ke_dense", dev.code, dev.Ne, ptr(dev.connect), ptr(dev.coord), ptr(tab), nPg,
+                      ptr(C), 1 if self.model.isHeterogeneous else 0, None, 0, 0.0, ptr(Ke), stream_ptr())
+        else:
+            E = to_dev(self.model._E_array())
+            _lib.call("efea_elastic_ke", dev.code, 64, dev.Ne, ptr(dev.connect), ptr(dev.coord), ptr(tab), nPg,
+                      ptr(E), 1 if self.model.isHeterogeneous else 0, self.model.v, int(self.model.planeStress),
+                      self.model.thickness, ptr(Ke), stream_ptr())
         A.scatter(Ke)
         self._K = A
         self._needUpdate = False
@@ -839,6 +847,8 @@ class Elastic(_Simu):
         """(Ne, 2 ns + 3) per element on the device: strain components, Evm,
         stress components, Svm, W_e (``efea_elastic_strain_stress``)."""
         dev = self.dev
+        if isinstance(self.model, Anisotropic):
+            raise NotImplementedError("strain / stress results of anisotropic laws are not on the device")
         ns = 3 if self.model.dim == 2 else 6
         out = torch.empty((dev.Ne, 2 * ns + 3), dtype=torch.float64, device="cuda")
         tab, nPg = dev.tables(MatrixType.rigi)

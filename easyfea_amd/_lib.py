@@ -23,6 +23,7 @@ SIGNATURES = {
         [i32, i32, i64, vp, vp, vp, i32, vp, i32, vp, f64, vp, i32, vp, f64, vp, vp, vp],
     ),
     "efea_boundary_load": (i32, [i32, i64, vp, vp, i32, vp, vp, vp]),
+    "efea_elastic_ke_dense": (i32, [i32, i64, vp, vp, vp, i32, vp, i32, vp, i32, f64, vp, vp]),
     "efea_elastic_strain_stress": (i32, [i32, i64, vp, vp, vp, i32, vp, vp, i32, f64, i32, f64, vp, vp]),
     "efea_node_average": (i32, [i32, i32, vp, vp, vp, vp, vp]),
     "efea_csr_scatter": (i32, [i32, i64, vp, vp, vp, vp, vp]),

@@ -106,6 +106,8 @@ int pp_iter_bench(int, int, int, int, const int*, const int*, const double*, con
                   const unsigned char*, int, const int*, const double*, int, const int*, const int*, const int*, int,
                   const int*, double*, void*, int, hipStream_t);
 
+int elastic_ke_dense(int, int64_t, const int*, const double*, const double*, int, const double*, int, const double*,
+                     int, double, double*, hipStream_t);
 }  // namespace efea
 
 using namespace efea;
@@ -165,6 +167,17 @@ int efea_elastic_ke(int elem_type, int precision, int64_t Ne, const int32_t* con
   }
   return elastic_ke(elem_type, precision, Ne, connect, coord, tables, nPg, E, E_stride, nu, plane_stress,
                     thickness, Ke, STREAM(stream));
+}
+
+int efea_elastic_ke_dense(int elem_type, int64_t Ne, const int32_t* connect, const double* coord,
+                          const double* tables, int nPg, const double* D, int D_stride, const double* E, int E_stride,
+                          double nu, double* Ke, void* stream) {
+  if (Ne < 0 || (Ne > 0 && (!connect || !coord || !tables || !Ke || (!D && !E)))) {
+    set_error("efea_elastic_ke_dense: null pointer or negative size");
+    return EFEA_ERR_ARG;
+  }
+  return elastic_ke_dense(elem_type, Ne, connect, coord, tables, nPg, D, D_stride, E, E_stride, nu, Ke,
+                          STREAM(stream));
 }
 
 int efea_elastic_strain_stress(int elem_type, int64_t Ne, const int32_t* connect, const double* coord,

@@ -48,6 +48,19 @@ int efea_elastic_ke(int elem_type, int precision, int64_t Ne, const int32_t* con
                     const double* tables, int nPg, const double* E, int E_stride, double nu, int plane_stress,
                     double thickness, void* Ke, void* stream);
 
+/* Dense 3D element stiffness on the FP64 matrix cores,
+ * K_e = sum_p wJ_p B_p^T D B_p, for HEXA8 (nPg 8) and TETRA10 (nPg 4): D is a
+ * Kelvin-Mandel 6 x 6 (row-major; rows xx yy zz yz xz xy) per element
+ * (D_stride 1) or shared (0); with D null, the isotropic matrix of E
+ * (E_stride 0 / 1) and nu.  Output (Ne, nd, nd) FP64 as efea_elastic_ke.
+ * Replaces Operators.Bilinear.LinearizedElasticity (FEM/Operators/
+ * Bilinear.py:62-79, the einsum of :79) for constitutive matrices of any
+ * form (Models/Elastic/_laws.py: Orthotropic, TransverselyIsotropic,
+ * Anisotropic). */
+int efea_elastic_ke_dense(int elem_type, int64_t Ne, const int32_t* connect, const double* coord,
+                          const double* tables, int nPg, const double* D, int D_stride, const double* E, int E_stride,
+                          double nu, double* Ke, void* stream);
+
 /* Linear-elastic post-processing per element (Simulations/_elastic.py:229-380,
  * Models/Elastic/_laws.py:127-185, Models/_utils.py:302-434): at each stiffness
  * integration point eps = B u_e (Kelvin-Mandel) and sig = C eps (isotropic,

@@ -308,6 +308,62 @@ def _bcast(coef, Ne, nPg):
     raise ValueError(coef.shape)
 
 
+def kelvin_mandel(dim: int, M: np.ndarray) -> np.ndarray:
+    """``KelvinMandel_Matrix`` (``Models/_utils.py:111-152``): entries of the
+    normal-shear blocks times sqrt(2), of the shear-shear block times 2."""
+    r2 = np.sqrt(2)
+    n = 3 if dim == 2 else 6
+    k = 2 if dim == 2 else 3
+    T = np.ones((n, n))
+    T[:k, k:] = T[k:, :k] = r2
+    T[k:, k:] = 2
+    return M * T
+
+
+def rotation_mandel(axis_1, axis_2) -> np.ndarray:
+    """``Get_Pmat(axis_1, axis_2, useMandel=True)`` (``Models/_utils.py:
+    433-589``) for one pair of 3D axes: ``C_global = P C_material P^T``.
+    p holds the axes as columns (axis_3 = axis_1 x axis_2); P = [[p**2,
+    sqrt2 A], [sqrt2 B, D2]] with A, B, D2 the reference's products of the
+    direction cosines (Chevalier 1988).  The axes are scaled by their norm as
+    the reference's einsum does (the caller passes unit vectors)."""
+    a1 = np.asarray(axis_1, float)
+    a2 = np.asarray(axis_2, float)
+    a1 = a1 * np.linalg.norm(a1)
+    a2 = a2 * np.linalg.norm(a2)
+    assert abs(a1 @ a2) <= 1e-12, "Must give perpendicular axes"
+    a3 = np.cross(a1, a2)
+    p = np.stack([a1, a2, a3], axis=1)
+    (p11, p12, p13), (p21, p22, p23), (p31, p32, p33) = a1, a2, a3
+    A = np.array([[p21 * p31, p11 * p31, p11 * p21],
+                  [p22 * p32, p12 * p32, p12 * p22],
+                  [p23 * p33, p13 * p33, p13 * p23]])
+    B = np.array([[p12 * p13, p22 * p23, p32 * p33],
+                  [p11 * p13, p21 * p23, p31 * p33],
+                  [p11 * p12, p21 * p22, p31 * p32]])
+    D2 = np.array([[p22 * p33 + p32 * p23, p12 * p33 + p32 * p13, p12 * p23 + p22 * p13],
+                   [p21 * p33 + p31 * p23, p11 * p33 + p31 * p13, p11 * p23 + p21 * p13],
+                   [p21 * p32 + p31 * p22, p11 * p32 + p31 * p12, p11 * p22 + p21 * p12]])
+    r2 = np.sqrt(2)
+    return np.block([[p ** 2, r2 * A], [r2 * B, D2]])
+
+
+def C_anisotropic(C, useVoigtNotation: bool, axis_1=(1, 0, 0), axis_2=(0, 1, 0)) -> np.ndarray:
+    """``Anisotropic._Behavior`` (``Models/Elastic/_laws.py:1190-1259``) for
+    a 3D law (``C`` of shape (6, 6) or (Ne, 6, 6)): the symmetry check, then
+    ``P C P^T`` with P of the normalised axes.  As the reference does, a 6 x 6
+    ``C`` enters the rotation as given: its Kelvin-Mandel scaling
+    (``useVoigtNotation``) only reaches 3 x 3 inputs (``:1230-1246``)."""
+    C = np.asarray(C, float)
+    assert C.shape[-2:] == (6, 6), "a 3D law takes a (6, 6) matrix"
+    Ct = np.swapaxes(C, -1, -2)
+    assert np.max(np.linalg.norm(Ct - C, axis=(-2, -1)) / np.linalg.norm(C, axis=(-2, -1))) <= 1e-12
+    a1 = np.asarray(axis_1, float) / np.linalg.norm(axis_1)
+    a2 = np.asarray(axis_2, float) / np.linalg.norm(axis_2)
+    P = rotation_mandel(a1, a2)
+    return np.einsum("ij,...jk,lk->...il", P, C, P)
+
+
 def K_elastic(geo: Geometry, C: np.ndarray) -> np.ndarray:
     """``Bilinear.LinearizedElasticity`` (``FEM/Operators/Bilinear.py:62-79``):
     ``einsum('epij,epjk->eik', (wJ*B^T) @ C, B)``.  ``C`` may be
@@ -1133,17 +1189,18 @@ class ElasticOracle(_SimuBase):
     """``Simulations.Elastic`` static solve (``Simulations/_elastic.py:123-152``
     and the static pipeline: assembly, Dirichlet reduction, direct solve)."""
 
-    def __init__(self, connect, coord, elemType, E, v, planeStress=True, thickness=1.0, boundary=None):
+    def __init__(self, connect, coord, elemType, E, v, planeStress=True, thickness=1.0, boundary=None, C=None):
         dim = ELEM_DIM[elemType]
         super().__init__(connect, coord, elemType, dim, boundary or {})
         self.E, self.v, self.planeStress, self.thickness = E, v, planeStress, thickness
+        self.C = C  # a Kelvin-Mandel stiffness replacing the isotropic one (anisotropic laws)
         self.Ndof = self.Nn * dim
         self._cmap = None
         self.u = np.zeros(self.Ndof)
 
     def element_K(self):
         geo = Geometry(self.elemType, self.connect, self.coord, "rigi", self.dim)
-        C = C_isotropic(self.E, self.v, self.dim, self.planeStress)
+        C = C_isotropic(self.E, self.v, self.dim, self.planeStress) if self.C is None else self.C
         Ke = K_elastic(geo, C)
         if self.dim == 2:
             Ke = Ke * self.thickness
