@@ -214,13 +214,16 @@ k_pp_iter(SpmvArgs A, PipeArgs PA, const int* __restrict__ agg_nodes, const doub
   __shared__ double shr[NM + 3][BS / 64];
   PcgState* st = W.st;
   const int c = blockIdx.x;
-  if (st->done) {  // decided by kernel A; recorded for the next kernel A
-    if (c == 0 && threadIdx.x == 0) st->stop = 1;
+  // the chunk bounds are read together with the solver state: the exit test
+  // uses both (k1 < k0 never holds), so the state read adds no round trip
+  const int k0 = PA.ck0[c], k1 = PA.ck1[c];
+  const int done = st->done;
+  if (done != 0 || k1 < k0) {  // done: decided by kernel A; recorded for the next kernel A
+    if (done != 0 && c == 0 && threadIdx.x == 0) st->stop = 1;
     return;
   }
   const bool first = st->justrep != 0;  // the first iteration: beta = 0, no old directions
   const double alpha = st->alpha, beta = st->beta;
-  const int k0 = PA.ck0[c], k1 = PA.ck1[c];
   const int k = k0 + (int)threadIdx.x / G, g = (int)threadIdx.x % G;
   const bool have = k < k1;
   int n = 0, p0 = 0, p1 = 0;
@@ -321,9 +324,11 @@ k_pp_iter(SpmvArgs A, PipeArgs PA, const int* __restrict__ agg_nodes, const doub
 // enters the delayed A-norm estimate; then the stopping test.
 __device__ void pp_scalars(PcgState* st, const double* __restrict__ partPrev, int nbPrev) {
   __shared__ double shp[3][BS / 64];
+  Partials<4> P;
+  P.issue(partPrev, nbPrev);  // in flight with the state read
   if (st->done) return;
   double pr4[4];
-  partial_share<4>(partPrev, nbPrev, pr4);
+  P.sum(partPrev, nbPrev, pr4);
   double pr[3] = {pr4[0], pr4[1], pr4[2]};
   block_sum_all<3, BS>(pr, shp);
   if (threadIdx.x != 0) return;
@@ -377,14 +382,14 @@ k_pp_coarse(CoarseArgs C, PipeArgs PA, const unsigned char* __restrict__ known, 
   __shared__ double shz[NM][BS / 64];
   const int nc = C.nagg * NM;
   const int blk = blockIdx.x;
-  if (MODE == 1) {
-    if (blk == C.nagg) {
-      pp_scalars(st, partPrev, nbPrev);
-      return;
-    }
-    if (st->stop) return;
+  if (MODE == 1 && blk == C.nagg) {
+    pp_scalars(st, partPrev, nbPrev);
+    return;
   }
+  // the aggregate's bounds are read together with the stop flag (the test
+  // uses both; k1 < k0 never holds)
   const int k0 = C.agg_ptr[blk], k1 = C.agg_ptr[blk + 1];
+  if (MODE == 1 && (st->stop != 0 || k1 < k0)) return;
   const int kf = k0 + threadIdx.x;
   const bool have = kf < k1;
   const int nf = have ? C.agg_nodes[kf] : 0;
