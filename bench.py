@@ -188,8 +188,7 @@ def pp_iter_roofline(simu, problemType, reps):
         A.make_fp32()
     head = (precision, A.Nn, A.dof_n, A.group, ptr(A.node_ptr), ptr(A.node_cols), ptr(A.values),
             ptr(A.values32 if precision == 32 else None), ptr(A.corr32 if precision == 32 else None), ptr(known),
-            cs.nagg, ptr(cs.agg_nodes), ptr(cs.rel), pc.nchunk, ptr(pc.k0), ptr(pc.k1), ptr(pc.slot), pc.maxch,
-            ptr(pc.nptr), ptr(x), ptr(ws), nbytes)
+            cs.nagg, ptr(cs.rel), pc.nchunk, ptr(pc.rows), ptr(pc.slot), pc.maxch, ptr(x), ptr(ws), nbytes)
     _lib.call("efea_pp_iter_bench", *head, 20, stream_ptr())
     s = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -203,10 +202,10 @@ def pp_iter_roofline(simu, problemType, reps):
     vb = 4 if precision == 32 else 8
     nm = cs.nm
     # values (+ FP32 row-sum correction) + node columns + per-position node and
-    # offsets + mask + aggregate coordinates + nine vectors read (m gathered
-    # once) + eight written + chunk bounds and the partials
+    # row range + mask + aggregate coordinates + nine vectors read (m gathered
+    # once) + eight written + chunk slots and the partials
     algo_bytes = (nnz * vb + nnz_node * 4 + Nn * (4 + 8) + Ndof + Nn * 24 + Ndof * 8 * 9 + Ndof * 8 * 8
-                  + pc.nchunk * 12 + pc.nchunk * 8 * (4 + nm))
+                  + pc.nchunk * 4 + pc.nchunk * 8 * (4 + nm))
     if precision == 32:
         algo_bytes += Nn * DN * DN * 4
     kname = "k_pp_iter"

@@ -57,11 +57,11 @@ SIGNATURES = {
     "efea_pcg2p_workspace_bytes": (sz, [i32, i32, i32]),
     "efea_pp_iter_bench": (
         i32,
-        [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, i32, vp, vp, i32, vp, vp, vp, i32, vp, vp, vp, sz, i32, vp],
+        [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, i32, vp, i32, vp, vp, i32, vp, vp, sz, i32, vp],
     ),
     "efea_solve_pcg2p": (
         i32,
-        [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, i32, i32, vp, vp, vp, i32, vp, vp, vp,
+        [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, i32, i32, vp, vp, i32, vp, vp, vp,
          vp, sz, f64, f64, i32, i32, i32, vp, vp],
     ),
     "efea_batch_create": (i32, [i32, i32, i32, i32, i32, C.POINTER(vp)]),

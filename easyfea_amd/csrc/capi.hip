@@ -100,11 +100,11 @@ int solve_pcg2(int, int, int, int, const int*, const int*, const double*, const 
 size_t pcg2p_workspace_bytes(int, int, int);
 int solve_pcg2p(int, int, int, int, const int*, const int*, const double*, const float*, const float*,
                 const unsigned char*, const void*, int, const int*, const int*, const double*, const double*, int, int,
-                const int*, const int*, const int*, int, const int*, const double*, double*, void*, double, double,
-                int, int, int, hipStream_t, double*, void*);
+                const int*, const int*, int, const int*, const double*, double*, void*, double, double, int, int, int,
+                hipStream_t, double*, void*);
 int pp_iter_bench(int, int, int, int, const int*, const int*, const double*, const float*, const float*,
-                  const unsigned char*, int, const int*, const double*, int, const int*, const int*, const int*, int,
-                  const int*, double*, void*, int, hipStream_t);
+                  const unsigned char*, int, const double*, int, const int*, const int*, int, double*, void*, int,
+                  hipStream_t);
 
 int elastic_ke_dense(int, int64_t, const int*, const double*, const double*, int, const double*, int, const double*,
                      int, double, double*, hipStream_t);
@@ -358,12 +358,11 @@ size_t efea_pcg2p_workspace_bytes(int Nnode, int dof_n, int nslot) {
 
 int efea_pp_iter_bench(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr,
                        const int32_t* node_cols, const double* values64, const float* values32, const float* corr32,
-                       const uint8_t* known, int nagg, const int32_t* agg_nodes, const double* rel, int nchunk,
-                       const int32_t* chunk_k0, const int32_t* chunk_k1, const int32_t* chunk_slot,
-                       int chunks_per_agg, const int32_t* chunk_node_ptr, double* x, void* workspace,
+                       const uint8_t* known, int nagg, const double* rel, int nchunk, const int32_t* chunk_rows,
+                       const int32_t* chunk_slot, int chunks_per_agg, double* x, void* workspace,
                        size_t workspace_bytes, int reps, void* stream) {
   if (Nnode <= 0 || nagg <= 0 || nchunk <= 0 || nchunk > 1024 || chunks_per_agg <= 0 || reps < 1 || !node_ptr ||
-      !node_cols || !agg_nodes || !rel || !chunk_k0 || !chunk_k1 || !chunk_slot || !chunk_node_ptr || !x ||
+      !node_cols || !rel || !chunk_rows || !chunk_slot || !x ||
       !workspace || (precision == 64 && !values64) || (precision == 32 && !values32) ||
       (precision != 64 && precision != 32)) {
     set_error("efea_pp_iter_bench: null pointer, empty system, bad precision, reps < 1 or over 1024 chunks");
@@ -375,21 +374,20 @@ int efea_pp_iter_bench(int precision, int Nnode, int dof_n, int group, const int
     return EFEA_ERR_SIZE;
   }
   return pp_iter_bench(precision, Nnode, dof_n, group, node_ptr, node_cols, values64, values32, corr32, known, nagg,
-                       agg_nodes, rel, nchunk, chunk_k0, chunk_k1, chunk_slot, chunks_per_agg, chunk_node_ptr, x,
-                       workspace, reps, STREAM(stream));
+                       rel, nchunk, chunk_rows, chunk_slot, chunks_per_agg, x, workspace, reps, STREAM(stream));
 }
 
 int efea_solve_pcg2p(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr,
                      const int32_t* node_cols, const double* values64, const float* values32, const float* corr32,
                      const uint8_t* known, const void* Minv, int nagg,
                      const int32_t* agg_ptr, const int32_t* agg_nodes, const double* rel, const double* Ainv, int ld,
-                     int nchunk, const int32_t* chunk_k0, const int32_t* chunk_k1, const int32_t* chunk_slot,
-                     int chunks_per_agg, const int32_t* chunk_node_ptr, const double* b, double* x,
-                     void* workspace, size_t workspace_bytes, double tol_residual, double tol_anorm, int maxit,
-                     int check_every, int first_check, void* stream, double* info) {
+                     int nchunk, const int32_t* chunk_rows, const int32_t* chunk_slot, int chunks_per_agg,
+                     const int32_t* agg_first, const double* b, double* x, void* workspace, size_t workspace_bytes,
+                     double tol_residual, double tol_anorm, int maxit, int check_every, int first_check, void* stream,
+                     double* info) {
   if (Nnode <= 0 || nagg <= 0 || nchunk <= 0 || chunks_per_agg <= 0 || !node_ptr || !node_cols || !values64 ||
-      !Minv || !agg_ptr || !agg_nodes || !rel || !Ainv || !chunk_k0 || !chunk_k1 || !chunk_slot ||
-      !chunk_node_ptr || !b || !x || !workspace || !info) {
+      !Minv || !agg_ptr || !agg_nodes || !rel || !Ainv || !chunk_rows || !chunk_slot || !agg_first || !b || !x ||
+      !workspace || !info) {
     set_error("efea_solve_pcg2p: null pointer or empty system");
     return EFEA_ERR_ARG;
   }
@@ -412,8 +410,7 @@ int efea_solve_pcg2p(int precision, int Nnode, int dof_n, int group, const int32
   }
   return solve_pcg2p(precision, Nnode, dof_n, group, node_ptr, node_cols, values64, values32, corr32, known, Minv,
                      nagg, agg_ptr, agg_nodes, rel,
-                     Ainv, ld, nchunk, chunk_k0, chunk_k1, chunk_slot, chunks_per_agg, chunk_node_ptr, b, x,
-                     workspace, tol_residual, tol_anorm, maxit, check_every, first_check, STREAM(stream), info,
+                     Ainv, ld, nchunk, chunk_rows, chunk_slot, chunks_per_agg, agg_first, b, x, workspace, tol_residual, tol_anorm, maxit, check_every, first_check, STREAM(stream), info,
                      nullptr);
 }
 

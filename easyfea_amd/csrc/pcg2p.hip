@@ -44,19 +44,22 @@
 namespace efea {
 
 // Chunks: runs of at most BS / G nodes of one aggregate, in the aggregate
-// order of agg_nodes, one per block of kernel B (nchunk <= MAXB); chunk c
-// holds positions [ck0[c], ck1[c]) and writes its restriction partial to slot
-// cslot[c] = a * maxch + (its index among aggregate a's chunks), so that
-// aggregate a's restriction is the sum of slots a * maxch .. a * maxch +
-// maxch - 1 (slots without a chunk stay zero).  nptr[2k], nptr[2k+1]:
-// node_ptr of node agg_nodes[k] and of the next node.
+// order of agg_nodes, one per block of kernel B (nchunk <= MAXB).  Chunk c
+// owns the BS / G positions c * BS / G + t; position j holds rows[3j] = its
+// node (-1: none), rows[3j + 1], rows[3j + 2] = that node's row range in
+// node_cols, so a lane finds its node and row with one load that depends on
+// nothing but its block and thread index.  Chunk c writes its restriction
+// partial to slot cslot[c] = a * maxch + (its index among aggregate a's
+// chunks), so that aggregate a's restriction is the sum of slots a * maxch ..
+// a * maxch + maxch - 1 (slots without a chunk stay zero).  first[a * BS + t]:
+// node t of aggregate a in agg_nodes order for t < BS (-1 past its end), the
+// coarse kernel's node lookup without the agg_ptr load.
 struct PipeArgs {
   int nchunk;
-  const int* ck0;
-  const int* ck1;
+  const int* rows;
   const int* cslot;
   int maxch;
-  const int* nptr;
+  const int* first;
 };
 
 constexpr int PIPE_REPLACE = 64;       // FP64 iterations
@@ -105,22 +108,17 @@ static inline PipeWork carve_pipe(void* ws, int Nnode, int dof_n, int nslot) {
 //   MODE 4: z = A q (masked).
 template <int DN, int G, int MODE>
 __global__ void __launch_bounds__(BS)
-k_pp_spmv(SpmvArgs A, PipeArgs PA, const int* __restrict__ agg_nodes, const double* __restrict__ rel,
-          const double* __restrict__ vin, double* __restrict__ x, const double* __restrict__ b, PipeWork W) {
+k_pp_spmv(SpmvArgs A, PipeArgs PA, const double* __restrict__ rel, const double* __restrict__ vin, double* __restrict__ x, const double* __restrict__ b, PipeWork W) {
   constexpr int NM = Modes<DN>::NM;  // a chunk holds BS / G nodes at most
   __shared__ double shn[NM][BS / 64];
   __shared__ double shp[4][BS / 64];
   double acc[4] = {0, 0, 0, 0};
   for (int c = blockIdx.x; c < PA.nchunk; c += gridDim.x) {
-    const int k0 = PA.ck0[c], k1 = PA.ck1[c];
-    const int k = k0 + (int)threadIdx.x / G, g = (int)threadIdx.x % G;
-    const bool have = k < k1;
-    int n = 0, p0 = 0, p1 = 0;
-    if (have) {
-      n = agg_nodes[k];
-      p0 = PA.nptr[2 * k];
-      p1 = PA.nptr[2 * k + 1];
-    }
+    const int g = (int)threadIdx.x % G;
+    const int* pos = PA.rows + 3 * ((int64_t)c * (BS / G) + (int)threadIdx.x / G);
+    int n = pos[0], p0 = pos[1], p1 = pos[2];
+    const bool have = n >= 0;
+    if (!have) n = p0 = p1 = 0;
     double sum[1][DN];
 #pragma unroll
     for (int a = 0; a < DN; ++a) sum[0][a] = 0;
@@ -204,34 +202,31 @@ k_pp_spmv(SpmvArgs A, PipeArgs PA, const int* __restrict__ agg_nodes, const doub
 
 // Kernel B of an iteration, one chunk per block: alpha, beta and the stop
 // decision come from kernel A's extra block (three scalar loads), so the
-// chain node -> offsets -> product starts at once; the own rows' nine
+// chain position -> row -> product starts at once; the own rows' nine
 // operands are loaded with the gather.  One block reduction ends it.
 template <int DN, int G, typename TM>
 __global__ void __launch_bounds__(BS)
-k_pp_iter(SpmvArgs A, PipeArgs PA, const int* __restrict__ agg_nodes, const double* __restrict__ rel,
-          const double* __restrict__ m, double* __restrict__ x, PipeWork W) {
+k_pp_iter(SpmvArgs A, PipeArgs PA, const double* __restrict__ rel, const double* __restrict__ m,
+          double* __restrict__ x, PipeWork W) {
   constexpr int NM = Modes<DN>::NM;
   __shared__ double shr[NM + 3][BS / 64];
   PcgState* st = W.st;
   const int c = blockIdx.x;
-  // the chunk bounds are read together with the solver state: the exit test
-  // uses both (k1 < k0 never holds), so the state read adds no round trip
-  const int k0 = PA.ck0[c], k1 = PA.ck1[c];
+  // the lane's position (node and row range) is read together with the
+  // solver state: the exit test uses both (n < -1 never holds), so the state
+  // read adds no round trip
+  const int g = (int)threadIdx.x % G;
+  const int* pos = PA.rows + 3 * ((int64_t)c * (BS / G) + (int)threadIdx.x / G);
+  int n = pos[0], p0 = pos[1], p1 = pos[2];
   const int done = st->done;
-  if (done != 0 || k1 < k0) {  // done: decided by kernel A; recorded for the next kernel A
+  if (done != 0 || n < -1) {  // done: decided by kernel A; recorded for the next kernel A
     if (done != 0 && c == 0 && threadIdx.x == 0) st->stop = 1;
     return;
   }
   const bool first = st->justrep != 0;  // the first iteration: beta = 0, no old directions
   const double alpha = st->alpha, beta = st->beta;
-  const int k = k0 + (int)threadIdx.x / G, g = (int)threadIdx.x % G;
-  const bool have = k < k1;
-  int n = 0, p0 = 0, p1 = 0;
-  if (have) {
-    n = agg_nodes[k];
-    p0 = PA.nptr[2 * k];
-    p1 = PA.nptr[2 * k + 1];
-  }
+  const bool have = n >= 0;
+  if (!have) n = p0 = p1 = 0;
   double own[9][DN], relf[3];
   const bool lead = have && g == 0;
   if (lead) {
@@ -386,13 +381,12 @@ k_pp_coarse(CoarseArgs C, PipeArgs PA, const unsigned char* __restrict__ known, 
     pp_scalars(st, partPrev, nbPrev);
     return;
   }
-  // the aggregate's bounds are read together with the stop flag (the test
-  // uses both; k1 < k0 never holds)
-  const int k0 = C.agg_ptr[blk], k1 = C.agg_ptr[blk + 1];
-  if (MODE == 1 && (st->stop != 0 || k1 < k0)) return;
-  const int kf = k0 + threadIdx.x;
-  const bool have = kf < k1;
-  const int nf = have ? C.agg_nodes[kf] : 0;
+  // the thread's node is read together with the stop flag (the test uses
+  // both; nf < -1 never holds)
+  int nf = PA.first[(int64_t)blk * BS + threadIdx.x];
+  if (MODE == 1 && (st->stop != 0 || nf < -1)) return;
+  const bool have = nf >= 0;
+  if (!have) nf = 0;
   double vf[DN], Mf[DN * DN], relf[3];
   if (have) {
 #pragma unroll
@@ -455,7 +449,8 @@ k_pp_coarse(CoarseArgs C, PipeArgs PA, const unsigned char* __restrict__ known, 
       out[i] = (known && known[i]) ? 0.0 : zv[a] + cv;
     }
   }
-  for (int k = kf + BS; k < k1; k += BS) {  // aggregates of more than BS nodes
+  const int k1 = C.agg_ptr[blk + 1];
+  for (int k = C.agg_ptr[blk] + BS + (int)threadIdx.x; k < k1; k += BS) {  // aggregates of more than BS nodes
     const int n = C.agg_nodes[k];
     double v[DN], zv[DN];
 #pragma unroll
@@ -541,11 +536,11 @@ static int pcg2p_run(const SpmvArgs& A, const SpmvArgs& Ait, const PipeArgs& PA,
   WX.partB = partX;
   WW.partB = partW;
   hipLaunchKernelGGL(k_pp_state, dim3(1), dim3(64), 0, s, W.st, tol_r * tol_r, tol_A * tol_A);
-  hipLaunchKernelGGL((k_pp_spmv<DN, G, 0>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
+  hipLaunchKernelGGL((k_pp_spmv<DN, G, 0>), dim3(nbB), dim3(BS), 0, s, A, PA, C.rel,
                      (const double*)x, x, b, WX);
   hipLaunchKernelGGL((k_pp_coarse<TP, DN, 0>), dim3(nbA), dim3(BS), 0, s, C, PA, A.known, Minv,
                      (const double*)W.r, W.u, (const double*)W.rcpart, W.st, (const double*)nullptr, 0);
-  hipLaunchKernelGGL((k_pp_spmv<DN, G, 1>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
+  hipLaunchKernelGGL((k_pp_spmv<DN, G, 1>), dim3(nbB), dim3(BS), 0, s, A, PA, C.rel,
                      (const double*)W.u, x, b, WW);
   hipLaunchKernelGGL(k_pp_init_scalars, dim3(1), dim3(BS), 0, s, (const double*)partX, (int)nbB,
                      (const double*)partW, (int)nbB, W.st);
@@ -566,24 +561,24 @@ static int pcg2p_run(const SpmvArgs& A, const SpmvArgs& Ait, const PipeArgs& PA,
         // partials the next pass reads), s = A p, q = M s, z = A q
         PipeWork Wr = W;
         Wr.partB = W.partR;
-        hipLaunchKernelGGL((k_pp_spmv<DN, G, 0>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
+        hipLaunchKernelGGL((k_pp_spmv<DN, G, 0>), dim3(nbB), dim3(BS), 0, s, A, PA, C.rel,
                            (const double*)x, x, b, Wr);
         hipLaunchKernelGGL((k_pp_coarse<TP, DN, 0>), dim3(nbA), dim3(BS), 0, s, C, PA, A.known, Minv,
                            (const double*)W.r, W.u, (const double*)W.rcpart, W.st, (const double*)nullptr, 0);
-        hipLaunchKernelGGL((k_pp_spmv<DN, G, 1>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
+        hipLaunchKernelGGL((k_pp_spmv<DN, G, 1>), dim3(nbB), dim3(BS), 0, s, A, PA, C.rel,
                            (const double*)W.u, x, b, WW);
         Wr.rcpart = W.rcpart2;
-        hipLaunchKernelGGL((k_pp_spmv<DN, G, 3>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
+        hipLaunchKernelGGL((k_pp_spmv<DN, G, 3>), dim3(nbB), dim3(BS), 0, s, A, PA, C.rel,
                            (const double*)W.p, x, b, Wr);
         hipLaunchKernelGGL((k_pp_coarse<TP, DN, 0>), dim3(nbA), dim3(BS), 0, s, C, PA, A.known, Minv,
                            (const double*)W.s, W.q, (const double*)W.rcpart2, W.st, (const double*)nullptr, 0);
-        hipLaunchKernelGGL((k_pp_spmv<DN, G, 4>), dim3(nbB), dim3(BS), 0, s, A, PA, C.agg_nodes, C.rel,
+        hipLaunchKernelGGL((k_pp_spmv<DN, G, 4>), dim3(nbB), dim3(BS), 0, s, A, PA, C.rel,
                            (const double*)W.q, x, b, Wr);
       }
       // kernel A: aggregate blocks plus the scalar step's block
       hipLaunchKernelGGL((k_pp_coarse<TP, DN, 1>), dim3(nbA + 1), dim3(BS), 0, s, C, PA, A.known, Minv,
                          (const double*)W.w, W.m, (const double*)W.rcpart, W.st, (const double*)partW, (int)nbB);
-      hipLaunchKernelGGL((k_pp_iter<DN, G, TM>), dim3(nbB), dim3(BS), 0, s, Ait, PA, C.agg_nodes, C.rel,
+      hipLaunchKernelGGL((k_pp_iter<DN, G, TM>), dim3(nbB), dim3(BS), 0, s, Ait, PA, C.rel,
                          (const double*)W.m, x, WW);
       ++pass;
     }
@@ -620,13 +615,13 @@ static int pcg2p_run(const SpmvArgs& A, const SpmvArgs& Ait, const PipeArgs& PA,
 int solve_pcg2p(int precision, int Nnode, int dof_n, int group, const int* node_ptr, const int* node_cols,
                 const double* vals64, const float* vals32, const float* corr32, const unsigned char* known,
                 const void* Minv, int nagg, const int* agg_ptr, const int* agg_nodes, const double* rel,
-                const double* Ainv, int ld, int nchunk, const int* ck0, const int* ck1, const int* cslot, int maxch,
-                const int* nptr, const double* b, double* x, void* ws, double tol_r, double tol_A, int maxit,
+                const double* Ainv, int ld, int nchunk, const int* rows, const int* cslot, int maxch,
+                const int* first, const double* b, double* x, void* ws, double tol_r, double tol_A, int maxit,
                 int check_every, int first_check, hipStream_t s, double* info, void* host_state_buf) {
   SpmvArgs A{Nnode, node_ptr, node_cols, vals64, known, nullptr, 0};
   SpmvArgs A32{Nnode, node_ptr, node_cols, vals32, known, corr32, 0};
   CoarseArgs C{nagg, agg_ptr, agg_nodes, rel, Ainv, ld, nullptr};
-  PipeArgs PA{nchunk, ck0, ck1, cslot, maxch, nptr};
+  PipeArgs PA{nchunk, rows, cslot, maxch, first};
   if (check_every < 1) check_every = 1;
   PcgState local;
   PcgState* hs = host_state_buf ? (PcgState*)host_state_buf : &local;
@@ -661,7 +656,7 @@ __global__ void k_pp_bench_state(PcgState* st) {
 }
 
 template <int DN, int G, typename TM>
-static int pp_iter_bench_run(const SpmvArgs& Ait, const PipeArgs& PA, const int* agg_nodes, const double* rel,
+static int pp_iter_bench_run(const SpmvArgs& Ait, const PipeArgs& PA, const double* rel,
                              int nagg, double* x, void* ws, int reps, hipStream_t s) {
   const PipeWork W = carve_pipe(ws, Ait.Nnode, DN, nagg * PA.maxch);
   const int64_t nd = (int64_t)Ait.Nnode * DN;
@@ -674,22 +669,22 @@ static int pp_iter_bench_run(const SpmvArgs& Ait, const PipeArgs& PA, const int*
   hipLaunchKernelGGL(k_pp_state, dim3(1), dim3(64), 0, s, W.st, 0.0, 0.0);
   hipLaunchKernelGGL(k_pp_bench_state, dim3(1), dim3(64), 0, s, W.st);
   for (int r = 0; r < reps; ++r)
-    hipLaunchKernelGGL((k_pp_iter<DN, G, TM>), dim3((unsigned)PA.nchunk), dim3(BS), 0, s, Ait, PA, agg_nodes, rel,
+    hipLaunchKernelGGL((k_pp_iter<DN, G, TM>), dim3((unsigned)PA.nchunk), dim3(BS), 0, s, Ait, PA, rel,
                        (const double*)W.m, x, W);
   return check_launch("pp_iter_bench");
 }
 
 int pp_iter_bench(int precision, int Nnode, int dof_n, int group, const int* node_ptr, const int* node_cols,
                   const double* vals64, const float* vals32, const float* corr32, const unsigned char* known,
-                  int nagg, const int* agg_nodes, const double* rel, int nchunk, const int* ck0, const int* ck1,
-                  const int* cslot, int maxch, const int* nptr, double* x, void* ws, int reps, hipStream_t s) {
+                  int nagg, const double* rel, int nchunk, const int* rows, const int* cslot, int maxch, double* x,
+                  void* ws, int reps, hipStream_t s) {
   SpmvArgs A{Nnode, node_ptr, node_cols, vals64, known, nullptr, 0};
   SpmvArgs A32{Nnode, node_ptr, node_cols, vals32, known, corr32, 0};
-  PipeArgs PA{nchunk, ck0, ck1, cslot, maxch, nptr};
+  PipeArgs PA{nchunk, rows, cslot, maxch, nullptr};
 #define RUNB(D, GV)                                                                                         \
   if (dof_n == D && group == GV) {                                                                         \
-    if (precision == 32) return pp_iter_bench_run<D, GV, float>(A32, PA, agg_nodes, rel, nagg, x, ws, reps, s); \
-    return pp_iter_bench_run<D, GV, double>(A, PA, agg_nodes, rel, nagg, x, ws, reps, s);                  \
+    if (precision == 32) return pp_iter_bench_run<D, GV, float>(A32, PA, rel, nagg, x, ws, reps, s); \
+    return pp_iter_bench_run<D, GV, double>(A, PA, rel, nagg, x, ws, reps, s);                  \
   }
   EFEA_GROUPS(RUNB)
 #undef RUNB

@@ -251,8 +251,12 @@ class CoarseSpace:
 
 class PipeChunks:
     """Chunks of the pipelined two-level solve (``efea_solve_pcg2p``): runs of
-    at most 256 / group nodes of one aggregate in aggregate order, so that
-    each chunk's partial of ``P^T w`` belongs to one aggregate."""
+    at most W = 256 / group nodes of one aggregate in aggregate order, so that
+    each chunk's partial of ``P^T w`` belongs to one aggregate.  Chunk c owns
+    the W positions ``c W + t`` of ``rows`` (node, then its row range; node
+    -1 pads a short chunk), so a lane reaches its row without first reading
+    the chunk's bounds; ``first`` lists each aggregate's first 256 nodes
+    (-1 past its end) for the coarse kernel's one-thread-per-node pass."""
 
     def __init__(self, cs: "CoarseSpace", pattern: "CsrPattern", group: int):
         ag = cs.ag
@@ -268,12 +272,24 @@ class PipeChunks:
                 slot.append(a * self.maxch + j)
         self.nchunk = len(k0)
         self.nslot = ag.nagg * self.maxch
+        self.width = npb
         nodes = ag.agg_nodes.astype(np.int64)
-        nptr = np.stack([pattern.host.node_ptr[nodes], pattern.host.node_ptr[nodes + 1]], axis=1)
-        self.k0 = to_dev(np.asarray(k0, dtype=np.int32))
-        self.k1 = to_dev(np.asarray(k1, dtype=np.int32))
+        node_ptr = pattern.host.node_ptr.astype(np.int64)
+        rows = np.zeros((self.nchunk, npb, 3), dtype=np.int32)
+        rows[:, :, 0] = -1
+        for c, (lo, hi) in enumerate(zip(k0, k1)):
+            nd = nodes[lo:hi]
+            rows[c, :hi - lo, 0] = nd
+            rows[c, :hi - lo, 1] = node_ptr[nd]
+            rows[c, :hi - lo, 2] = node_ptr[nd + 1]
+        first = np.full((ag.nagg, 256), -1, dtype=np.int32)
+        for a in range(ag.nagg):
+            lo, hi = int(ag.agg_ptr[a]), int(ag.agg_ptr[a + 1])
+            m = min(hi - lo, 256)
+            first[a, :m] = nodes[lo:lo + m]
+        self.rows = to_dev(rows.ravel())
         self.slot = to_dev(np.asarray(slot, dtype=np.int32))
-        self.nptr = to_dev(np.ascontiguousarray(nptr, dtype=np.int32).ravel())
+        self.first = to_dev(first.ravel())
 
 
 class DeviceCsr:
@@ -627,8 +643,8 @@ class SolveJob:
             rc = _lib.call("efea_solve_pcg2p", self.precision, A.Nn, A.dof_n, A.group, ptr(A.node_ptr),
                            ptr(A.node_cols), ptr(A.values), ptr(v32), ptr(c32), ptr(self.known), ptr(self.Minv),
                            cs.nagg, ptr(cs.agg_ptr),
-                           ptr(cs.agg_nodes), ptr(cs.rel), ptr(self.Ainv), cs.ld, pc.nchunk, ptr(pc.k0), ptr(pc.k1),
-                           ptr(pc.slot), pc.maxch, ptr(pc.nptr), ptr(self.b), ptr(self.x), ptr(buf), nbytes,
+                           ptr(cs.agg_nodes), ptr(cs.rel), ptr(self.Ainv), cs.ld, pc.nchunk, ptr(pc.rows),
+                           ptr(pc.slot), pc.maxch, ptr(pc.first), ptr(self.b), ptr(self.x), ptr(buf), nbytes,
                            self.tol_residual, self.tol_anorm, self.maxit, check_every, first_check, stream_ptr(),
                            C.cast(raw, C.c_void_p))
             return rc, self.info(raw)

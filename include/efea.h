@@ -244,12 +244,13 @@ int efea_solve_pcg2(int mode, int Nnode, int dof_n, int group, const int32_t* no
  * values64.  Same arguments, stopping test and info as efea_solve_pcg2 (mode
  * 64 or 32 with FP64 vectors), plus the node chunks
  * the product pass runs over, one block each (at most 1024): runs of at most
- * 256 / group nodes of one aggregate in agg_nodes order, chunk c holding
- * positions [chunk_k0[c], chunk_k1[c]) of agg_nodes and writing its share of
- * P^T w to slot chunk_slot[c] = a * chunks_per_agg + (its index among its
- * aggregate a's chunks); chunk_node_ptr (2 Nnode) holds node_ptr[agg_nodes[k]]
- * and node_ptr[agg_nodes[k] + 1] for each position k.  Workspace:
- * efea_pcg2p_workspace_bytes(Nnode, dof_n,
+ * W = 256 / group nodes of one aggregate in agg_nodes order.  Chunk c owns the
+ * W positions c * W + t of chunk_rows (3 * nchunk * W ints): position j holds
+ * its node (-1: none), then node_ptr[node] and node_ptr[node + 1]; chunk c
+ * writes its share of P^T w to slot chunk_slot[c] = a * chunks_per_agg + (its
+ * index among its aggregate a's chunks).  agg_first (256 * nagg ints): the
+ * first 256 nodes of each aggregate in agg_nodes order, -1 past its end.
+ * Workspace: efea_pcg2p_workspace_bytes(Nnode, dof_n,
  * nagg * chunks_per_agg).  Every 64 iterations (32 with FP32 values) the
  * recursive vectors are recomputed from x and the direction with the FP64
  * matrix (residual replacement). */
@@ -260,10 +261,10 @@ int efea_solve_pcg2p(int precision, int Nnode, int dof_n, int group, const int32
                      const int32_t* node_cols, const double* values64, const float* values32, const float* corr32,
                      const uint8_t* known, const void* Minv, int nagg,
                      const int32_t* agg_ptr, const int32_t* agg_nodes, const double* rel, const double* Ainv, int ld,
-                     int nchunk, const int32_t* chunk_k0, const int32_t* chunk_k1, const int32_t* chunk_slot,
-                     int chunks_per_agg, const int32_t* chunk_node_ptr, const double* b, double* x,
-                     void* workspace, size_t workspace_bytes, double tol_residual, double tol_anorm, int maxit,
-                     int check_every, int first_check, void* stream, double* info);
+                     int nchunk, const int32_t* chunk_rows, const int32_t* chunk_slot, int chunks_per_agg,
+                     const int32_t* agg_first, const double* b, double* x, void* workspace, size_t workspace_bytes,
+                     double tol_residual, double tol_anorm, int maxit, int check_every, int first_check, void* stream,
+                     double* info);
 
 /* Timing harness of efea_solve_pcg2p's dominant kernel (the product pass:
  * n = A m, the eight vector updates, the dot-product and restriction
@@ -275,9 +276,8 @@ int efea_solve_pcg2p(int precision, int Nnode, int dof_n, int group, const int32
  * counterpart in the reference. */
 int efea_pp_iter_bench(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr,
                        const int32_t* node_cols, const double* values64, const float* values32, const float* corr32,
-                       const uint8_t* known, int nagg, const int32_t* agg_nodes, const double* rel, int nchunk,
-                       const int32_t* chunk_k0, const int32_t* chunk_k1, const int32_t* chunk_slot,
-                       int chunks_per_agg, const int32_t* chunk_node_ptr, double* x, void* workspace,
+                       const uint8_t* known, int nagg, const double* rel, int nchunk, const int32_t* chunk_rows,
+                       const int32_t* chunk_slot, int chunks_per_agg, double* x, void* workspace,
                        size_t workspace_bytes, int reps, void* stream);
 
 /* Continuous batching of Dirichlet-reduced solves (the serving loop).
