@@ -156,8 +156,9 @@ def _profiled_traffic(kname, ndof, nnz, precision):
 
 def dominant_roofline(simu, problemType, reps):
     """Roofline of the last ``problemType`` solve's dominant kernel: the
-    product pass of a pipelined solve, step 1 otherwise."""
-    if simu.last_solve[problemType].pipelined:
+    product pass of a pipelined two-level solve, step 1 otherwise."""
+    last = simu.last_solve[problemType]
+    if last.pipelined and last.precond == "two-level":
         return pp_iter_roofline(simu, problemType, reps)
     return step1_roofline(simu, problemType, reps)
 

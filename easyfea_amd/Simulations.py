@@ -95,7 +95,8 @@ class _Simu:
         self.mixed_adaptive = True
         # hip_mixed two-level solves: "auto" | "values" | "refine" (device.py)
         self.mixed_strategy = "auto"
-        # FP64 two-level solves: pipelined PCG "auto" (latency-bound systems) | True | False
+        # pipelined PCG "auto" (latency-bound systems; block Jacobi: one launch per iteration,
+        # two-level: two) | True | False
         self.pipelined = "auto"
         self.max_blocks = 0  # grid cap of the solver kernels (0: the library's default)
         # keep the two-level coarse inverse over successive solves of one matrix

@@ -55,6 +55,12 @@ SIGNATURES = {
          i32, i32, i32, i32, vp, vp],
     ),
     "efea_pcg2p_workspace_bytes": (sz, [i32, i32, i32]),
+    "efea_pcgp_workspace_bytes": (sz, [i32, i32]),
+    "efea_pcgp_max_lanes": (i32, []),
+    "efea_solve_pcgp": (
+        i32,
+        [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, f64, f64, i32, i32, i32, i32, vp, vp],
+    ),
     "efea_pp_iter_bench": (
         i32,
         [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, i32, vp, i32, vp, vp, i32, vp, vp, sz, i32, vp],

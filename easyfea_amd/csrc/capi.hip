@@ -98,6 +98,10 @@ int solve_pcg2(int, int, int, int, const int*, const int*, const double*, const 
                const double*, double*, void*, double, double, double, int, int, int, int, hipStream_t, double*);
 
 size_t pcg2p_workspace_bytes(int, int, int);
+size_t pcgp_workspace_bytes(int, int);
+int solve_pcgp(int, int, int, int, const int*, const int*, const double*, const float*, const float*,
+               const unsigned char*, const void*, const double*, double*, void*, double, double, int, int, int, int,
+               hipStream_t, double*, void*);
 int solve_pcg2p(int, int, int, int, const int*, const int*, const double*, const float*, const float*,
                 const unsigned char*, const void*, int, const int*, const int*, const double*, const double*, int, int,
                 const int*, const int*, int, const int*, const double*, double*, void*, double, double, int, int, int,
@@ -351,6 +355,38 @@ int efea_solve_pcg_noalloc(int mode, int Nnode, int dof_n, int group, const int3
                    tol_residual, tol_anorm, replace_ratio, maxit, check_every, first_check, max_blocks,
                    STREAM(stream), info, &hs);
 }
+
+size_t efea_pcgp_workspace_bytes(int Nnode, int dof_n) { return pcgp_workspace_bytes(Nnode, dof_n); }
+
+int efea_solve_pcgp(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr,
+                    const int32_t* node_cols, const double* values64, const float* values32, const float* corr32,
+                    const uint8_t* known, const void* Minv, const double* b, double* x, void* workspace,
+                    size_t workspace_bytes, double tol_residual, double tol_anorm, int replace_every, int maxit,
+                    int check_every, int first_check, void* stream, double* info) {
+  if (Nnode <= 0 || !node_ptr || !node_cols || !values64 || !Minv || !b || !x || !workspace || !info) {
+    set_error("efea_solve_pcgp: null pointer or empty system");
+    return EFEA_ERR_ARG;
+  }
+  if ((precision != 64 && precision != 32) || (precision == 32 && !values32)) {
+    set_error("efea_solve_pcgp: precision %d (64, or 32 with FP32 values)", precision);
+    return EFEA_ERR_ARG;
+  }
+  if (group <= 0 || (int64_t)Nnode * group > (int64_t)efea_pcgp_max_lanes()) {
+    set_error("efea_solve_pcgp: %d nodes x %d lanes exceed %d lanes (one pass of the grid)", Nnode, group,
+              efea_pcgp_max_lanes());
+    return EFEA_ERR_ARG;
+  }
+  if (workspace_bytes < pcgp_workspace_bytes(Nnode, dof_n)) {
+    set_error("efea_solve_pcgp: workspace of %zu bytes, %zu needed", workspace_bytes,
+              pcgp_workspace_bytes(Nnode, dof_n));
+    return EFEA_ERR_SIZE;
+  }
+  return solve_pcgp(precision, Nnode, dof_n, group, node_ptr, node_cols, values64, values32, corr32, known, Minv, b,
+                    x, workspace, tol_residual, tol_anorm, replace_every, maxit, check_every, first_check,
+                    STREAM(stream), info, nullptr);
+}
+
+int efea_pcgp_max_lanes(void) { return 1024 * 256; }
 
 size_t efea_pcg2p_workspace_bytes(int Nnode, int dof_n, int nslot) {
   return pcg2p_workspace_bytes(Nnode, dof_n, nslot);

@@ -436,7 +436,7 @@ class SolveInfo:
     precond: str = "bj"
     coarse_reused: bool = False
     refine: bool = False  # FP32 inner PCG inside FP64 iterative refinement
-    pipelined: bool = False  # pipelined two-level PCG (two launches per iteration)
+    pipelined: bool = False  # pipelined PCG (block Jacobi: one launch per iteration; two-level: two)
     isolated: int = 0  # nodes given aggregates of their own after a stalled solve
 
 
@@ -458,6 +458,14 @@ REFINE_MIN_MATRIX_BYTES = 128 * 2**20
 # the pipelined PCG (efea_solve_pcg2p): two launches per iteration instead of
 # three, at 8 instead of 5 work vectors.
 PIPELINE_MAX_COARSE = 4096
+# Block-Jacobi solves take the pipelined PCG (efea_solve_pcgp, one launch per
+# iteration) under "auto" when the previous solve of their sequence took at
+# most this many iterations (the staggered damage solves: about 40-50).  Its
+# recurrences drift on long ill-conditioned solves: on the 360 x 39
+# cantilever (about 1,200 block-Jacobi iterations) the pipelined solve
+# stopped 5e-9 to 6e-8 from the direct solution depending on the
+# replacement period, against 3e-10 for the classic iteration.
+PIPELINE_BJ_MAX_EXPECTED = 128
 # A phase-field displacement solve that stalls with fully damaged nodes is
 # redone with those nodes as singleton aggregates; the attempts before stop
 # after this many iterations (or 4x the previous solve's), the coarse space
@@ -535,6 +543,7 @@ class SolveJob:
         self.Ainv32: Optional[torch.Tensor] = None
         assert pipelined in ("auto", True, False), pipelined
         self.pipelined_mode = pipelined
+        self.pipe_replace_every = 0  # pipelined block Jacobi: residual replacement period (0: the library's)
         self.pipelined = False  # decided by run(), as refine
         # isolate(): the nodes whose elements have lost their stiffness (a
         # phase-field displacement solve); set, a two-level solve that stalls
@@ -591,10 +600,20 @@ class SolveJob:
         return SolveInfo(int(raw[0]), float(raw[1]), int(raw[2]), int(raw[3]),
                          "fp64" if self.precision == 64 else "mixed", self.fallback, self.precond,
                          self.coarse_reused, self.refine and self.precision == 32,
-                         self.pipelined and self.precond == "two-level", self.n_isolated)
+                         self.pipelined, self.n_isolated)
 
     def _wants_pipeline(self) -> bool:
-        if self.precond != "two-level" or self.pipelined_mode is False or self._wants_refine():
+        if self.pipelined_mode is False or self._wants_refine():
+            return False
+        if self.precond == "bj":
+            # one launch per iteration; its grid covers the node groups in one pass
+            if self.A.Nn * self.A.group > _lib.load().efea_pcgp_max_lanes():
+                return False
+            if self.pipelined_mode is True:
+                return True
+            n = self.expected_iterations
+            return n is not None and n <= PIPELINE_BJ_MAX_EXPECTED and self.A.nnz * 8 < REFINE_MIN_MATRIX_BYTES
+        if self.precond != "two-level":
             return False
         cs = self.A.coarse(self.coarse_target)
         if cs.nagg * cs.nm > PIPELINE_MAX_COARSE or cs.pipeline(self.A.pattern, self.A.group).nchunk > 1024:
@@ -635,6 +654,15 @@ class SolveJob:
         check_every, first_check = self._polls()
         v32 = A.values32 if self.precision == 32 else None
         c32 = A.corr32 if self.precision == 32 else None
+        if self.precond == "bj" and self.pipelined:
+            nbytes = _lib.load().efea_pcgp_workspace_bytes(A.Nn, A.dof_n)
+            buf = ws.get(nbytes)
+            rc = _lib.call("efea_solve_pcgp", self.precision, A.Nn, A.dof_n, A.group, ptr(A.node_ptr),
+                           ptr(A.node_cols), ptr(A.values), ptr(v32), ptr(c32), ptr(self.known), ptr(self.Minv),
+                           ptr(self.b), ptr(self.x), ptr(buf), nbytes, self.tol_residual, self.tol_anorm,
+                           self.pipe_replace_every, self.maxit, check_every, first_check, stream_ptr(),
+                           C.cast(raw, C.c_void_p))
+            return rc, self.info(raw)
         if self.precond == "two-level" and self.pipelined:
             cs = self.cs
             pc = cs.pipeline(A.pattern, A.group)

@@ -235,6 +235,32 @@ int efea_solve_pcg2(int mode, int Nnode, int dof_n, int group, const int32_t* no
                     double tol_anorm, double replace_ratio, int maxit, int check_every, int first_check,
                     int max_blocks, void* stream, double* info);
 
+/* Pipelined block-Jacobi PCG (Ghysels and Vanroose's preconditioned pipelined
+ * CG with the nodal block-Jacobi inverse of efea_bj_setup): one kernel per
+ * iteration instead of efea_solve_pcg's two, for latency-bound systems.  The
+ * preconditioner is node-local, so the launch that updates w = A u also forms
+ * the m = M w the next launch multiplies; every block reduces the previous
+ * launch's dot-product partials itself.  Same arguments, stopping test and
+ * info as efea_solve_pcg (precision 64: FP64 throughout; 32: the iterations'
+ * products use values32 + corr32 and Minv is FP32, the start-up and the
+ * residual replacements use values64), without replace_ratio and max_blocks:
+ * every replace_every iterations (0: 64, or 32 with FP32 values) the
+ * recursive vectors are recomputed from x and the direction with the FP64
+ * matrix.  The grid covers
+ * Nnode * group lanes in one pass: at most efea_pcgp_max_lanes() lanes
+ * (EFEA_ERR_ARG beyond).  A solve whose recurrences break down returns
+ * EFEA_ERR_NOCONV with info status 2; the caller redoes it with
+ * efea_solve_pcg from the same start.  Workspace: efea_pcgp_workspace_bytes.
+ * Replaces __Solver_1 (Simulations/Solvers.py:502-553) with _Solve_Axb's
+ * backends (Solvers.py:225-394), as efea_solve_pcg does. */
+size_t efea_pcgp_workspace_bytes(int Nnode, int dof_n);
+int efea_pcgp_max_lanes(void);
+int efea_solve_pcgp(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr,
+                    const int32_t* node_cols, const double* values64, const float* values32, const float* corr32,
+                    const uint8_t* known, const void* Minv, const double* b, double* x, void* workspace,
+                    size_t workspace_bytes, double tol_residual, double tol_anorm, int replace_every, int maxit,
+                    int check_every, int first_check, void* stream, double* info);
+
 /* Pipelined two-level PCG (Ghysels and Vanroose's preconditioned pipelined
  * CG) with the preconditioner of efea_solve_pcg2: two kernels per iteration
  * instead of three (the restriction P^T w and both dot products ride on the

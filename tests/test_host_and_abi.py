@@ -63,6 +63,18 @@ def test_abi_argument_errors_are_reported_without_gpu():
     jid = C.c_int(5)
     assert lib.efea_batch_pop(None, C.byref(jid), None) == -1
     assert lib.efea_batch_destroy(None) == 0
+    # pipelined block Jacobi: one grid pass of at most efea_pcgp_max_lanes() lanes
+    assert lib.efea_pcgp_max_lanes() == 1024 * 256
+    fake = C.c_void_p(256)  # never dereferenced: the size checks come first
+    assert lib.efea_solve_pcgp(64, 0, 1, 4, *[fake] * 10, 1 << 30, 1e-12, 1e-9, 0, 10, 8, 0, None, fake) == -1
+    assert "null pointer or empty" in _lib.last_error()
+    assert lib.efea_solve_pcgp(16, 100, 1, 4, *[fake] * 10, 1 << 30, 1e-12, 1e-9, 0, 10, 8, 0, None, fake) == -1
+    assert "precision" in _lib.last_error()
+    assert lib.efea_solve_pcgp(64, 70000, 1, 4, *[fake] * 10, 1 << 30, 1e-12, 1e-9, 0, 10, 8, 0, None, fake) == -1
+    assert "lanes" in _lib.last_error()
+    assert lib.efea_solve_pcgp(64, 100, 1, 4, *[fake] * 10, 16, 1e-12, 1e-9, 0, 10, 8, 0, None, fake) == -4
+    assert "workspace" in _lib.last_error()
+    assert lib.efea_pcgp_workspace_bytes(100, 2) >= 10 * 100 * 2 * 8
 
 
 def test_workspace_sizes_and_groups():
