@@ -522,14 +522,6 @@ k_xax_residual(SpmvArgs A, const double* __restrict__ x, const double* __restric
 
 size_t pcg2_workspace_bytes(int Nnode, int dof_n, int nagg) { return pcg2_workspace_size(Nnode, dof_n, nagg); }
 
-static PcgState* host_state2() {
-  static thread_local PcgState* h = nullptr;
-  if (!h) {
-    if (hipHostMalloc((void**)&h, sizeof(PcgState), hipHostMallocDefault) != hipSuccess) h = nullptr;
-  }
-  return h;
-}
-
 template <typename TP, int DN, int G>
 int pcg2_start(const SpmvArgs& A64, const TP* Minv, const CoarseArgs& C, const double* b, const double* x,
                const Pcg2Work& v, double tol_r, double tol_A, double delta2, hipStream_t s) {
@@ -568,14 +560,14 @@ static int pcg2_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, con
   const unsigned nbA = (unsigned)C.nagg;
   const bool mixed = (void*)AK.vals != (void*)A64.vals;
   if (int e = pcg2_start<TP, DN, G>(A64, Minv, C, b, x, v, tol_r, tol_A, mixed ? delta * delta : 0.0, s)) return e;
-  PcgState* hs = host_state2();
-  if (!hs) {
+  PollBuf* pb = poll_buffer();
+  if (!pb) {
     set_error("pcg2: pinned host allocation failed");
     return EFEA_ERR_HIP;
   }
+  PcgState* hs = &pb->st;
   int launched = 0;
   bool spec = false;
-  double rr_spec = 0;
   while (true) {
     const int chunk = launched == 0 && first_check > 0 ? first_check : check_every;
     for (int k = 0; k < chunk; ++k) {
@@ -593,7 +585,7 @@ static int pcg2_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, con
     spec = !mixed && first_check > 0;
     if (spec) {
       pcg_spec_residual<DN, G>(A64, b, x, w, s);
-      if (hipMemcpyAsync(&rr_spec, w.small, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) {
+      if (hipMemcpyAsync(&pb->rr, w.small, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) {
         set_error("pcg2: residual readback failed");
         return EFEA_ERR_HIP;
       }
@@ -616,16 +608,15 @@ static int pcg2_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, con
     if (hs->done || launched >= maxit) break;
   }
   const PcgState fin = *hs;
-  double rr_true = rr_spec;  // the last read-back's residual when it carried one
-  if (!spec) {
+  if (!spec) {  // else the last read-back carried the residual
     pcg_true_residual<DN, G>(A64, b, x, w, s);
-    if (hipMemcpyAsync(&rr_true, w.small, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+    if (hipMemcpyAsync(&pb->rr, w.small, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
       set_error("pcg2: residual readback failed");
       return EFEA_ERR_HIP;
     }
   }
-  return pcg_finish_info(&fin, rr_true, info);
+  return pcg_finish_info(&fin, pb->rr, info);
 }
 
 // ---------------------------------------------------------------------------

@@ -547,7 +547,12 @@ static int pcg2p_run(const SpmvArgs& A, const SpmvArgs& Ait, const PipeArgs& PA,
   if (int e = check_launch("pcg2p_start")) return e;
   int launched = 0, pass = 1;
   bool spec = false;
-  double rr_spec = 0;
+  PollBuf* pb = poll_buffer();
+  if (!pb) {
+    set_error("pcg2p: pinned host allocation failed");
+    return EFEA_ERR_HIP;
+  }
+  if (!hs) hs = &pb->st;
   PcgWork fw{};  // residual passes of the read-backs: m is dead there (kernel A rewrites it)
   fw.r = W.m;
   fw.q = W.m;
@@ -586,7 +591,7 @@ static int pcg2p_run(const SpmvArgs& A, const SpmvArgs& Ait, const PipeArgs& PA,
     spec = first_check > 0;
     if (spec) {
       pcg_spec_residual<DN, G>(A, b, x, fw, s);
-      if (hipMemcpyAsync(&rr_spec, W.small, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) {
+      if (hipMemcpyAsync(&pb->rr, W.small, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) {
         set_error("pcg2p: residual readback failed");
         return EFEA_ERR_HIP;
       }
@@ -600,16 +605,15 @@ static int pcg2p_run(const SpmvArgs& A, const SpmvArgs& Ait, const PipeArgs& PA,
     if (hs->done || launched >= maxit) break;
   }
   const PcgState fin = *hs;
-  double rr_true = rr_spec;
-  if (!spec) {
+  if (!spec) {  // else the last read-back carried the residual
     pcg_true_residual<DN, G>(A, b, x, fw, s);
-    if (hipMemcpyAsync(&rr_true, W.small, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+    if (hipMemcpyAsync(&pb->rr, W.small, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
       set_error("pcg2p: residual readback failed");
       return EFEA_ERR_HIP;
     }
   }
-  return pcg_finish_info(&fin, rr_true, info);
+  return pcg_finish_info(&fin, pb->rr, info);
 }
 
 int solve_pcg2p(int precision, int Nnode, int dof_n, int group, const int* node_ptr, const int* node_cols,
@@ -623,8 +627,7 @@ int solve_pcg2p(int precision, int Nnode, int dof_n, int group, const int* node_
   CoarseArgs C{nagg, agg_ptr, agg_nodes, rel, Ainv, ld, nullptr};
   PipeArgs PA{nchunk, rows, cslot, maxch, first};
   if (check_every < 1) check_every = 1;
-  PcgState local;
-  PcgState* hs = host_state_buf ? (PcgState*)host_state_buf : &local;
+  PcgState* hs = (PcgState*)host_state_buf;  // null: this thread's pinned poll buffer
 #define RUNP(D, GV)                                                                                              \
   if (dof_n == D && group == GV) {                                                                              \
     if (precision == 32)                                                                                        \

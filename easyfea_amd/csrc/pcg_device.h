@@ -704,5 +704,15 @@ template <int DN, int G>
 void pcg_spec_residual(const SpmvArgs& A64, const double* b, const double* x, const PcgWork& w, hipStream_t s);
 // Fills info[0..3] from a read-back state and the true residual norm^2.
 int pcg_finish_info(const void* state, double rr_true, double* info);
+// This thread's pinned buffer of the solvers' polls (null if the pinned
+// allocation failed): the state and the speculative residual norm are two
+// asynchronous copies followed by one synchronisation, where a copy into
+// pageable memory would be a blocking staged copy of its own.  Defined in
+// sparse.hip.
+struct PollBuf {
+  PcgState st;
+  double rr;
+};
+PollBuf* poll_buffer();
 
 }  // namespace efea

@@ -444,7 +444,12 @@ static int pcgp_run(const SpmvArgs& A, const SpmvArgs& Ait, const TP* Minv, cons
   fw.small = W.small;
   int launched = 0;
   bool spec = false;
-  double rr_spec = 0;
+  PollBuf* pb = poll_buffer();
+  if (!pb) {
+    set_error("pcgp: pinned host allocation failed");
+    return EFEA_ERR_HIP;
+  }
+  if (!hs) hs = &pb->st;
   while (true) {
     const int chunk = launched == 0 && first_check > 0 ? first_check : check_every;
     for (int k = 0; k < chunk; ++k) {
@@ -463,7 +468,7 @@ static int pcgp_run(const SpmvArgs& A, const SpmvArgs& Ait, const TP* Minv, cons
     spec = first_check > 0;
     if (spec) {
       pcg_spec_residual<DN, G>(A, b, x, fw, s);
-      if (hipMemcpyAsync(&rr_spec, W.small, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) {
+      if (hipMemcpyAsync(&pb->rr, W.small, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) {
         set_error("pcgp: residual readback failed");
         return EFEA_ERR_HIP;
       }
@@ -477,16 +482,15 @@ static int pcgp_run(const SpmvArgs& A, const SpmvArgs& Ait, const TP* Minv, cons
     if (hs->done || launched >= maxit) break;
   }
   const PcgState fin = *hs;
-  double rr_true = rr_spec;
-  if (!spec) {
+  if (!spec) {  // else the last read-back carried the residual
     pcg_true_residual<DN, G>(A, b, x, fw, s);
-    if (hipMemcpyAsync(&rr_true, W.small, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+    if (hipMemcpyAsync(&pb->rr, W.small, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
       set_error("pcgp: residual readback failed");
       return EFEA_ERR_HIP;
     }
   }
-  return pcg_finish_info(&fin, rr_true, info);
+  return pcg_finish_info(&fin, pb->rr, info);
 }
 
 int solve_pcgp(int precision, int Nnode, int dof_n, int group, const int* node_ptr, const int* node_cols,
@@ -496,8 +500,7 @@ int solve_pcgp(int precision, int Nnode, int dof_n, int group, const int* node_p
   SpmvArgs A{Nnode, node_ptr, node_cols, vals64, known, nullptr, 0};
   SpmvArgs A32{Nnode, node_ptr, node_cols, vals32, known, corr32, 0};
   if (check_every < 1) check_every = 1;
-  PcgState local;
-  PcgState* hs = host_state_buf ? (PcgState*)host_state_buf : &local;
+  PcgState* hs = (PcgState*)host_state_buf;  // null: this thread's pinned poll buffer
 #define RUNB(D, GV)                                                                                             \
   if (dof_n == D && group == GV) {                                                                             \
     if (precision == 32)                                                                                       \

@@ -401,10 +401,10 @@ size_t pcg_workspace_bytes(int Nnode, int dof_n, int group) {
   return pcg_workspace_size(Nnode, dof_n);
 }
 
-static PcgState* host_state() {
-  static thread_local PcgState* h = nullptr;
+PollBuf* poll_buffer() {
+  static thread_local PollBuf* h = nullptr;
   if (!h) {
-    if (hipHostMalloc((void**)&h, sizeof(PcgState), hipHostMallocDefault) != hipSuccess) h = nullptr;
+    if (hipHostMalloc((void**)&h, sizeof(PollBuf), hipHostMallocDefault) != hipSuccess) h = nullptr;
   }
   return h;
 }
@@ -512,14 +512,14 @@ static int pcg_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, cons
   const bool mixed = (void*)AK.vals != (void*)A64.vals;
   if (int e = pcg_start<TP, DN, G>(A64, Minv, b, x, w, tol_r, tol_A, mixed ? delta * delta : 0.0, s)) return e;
   // the caller's host buffer for the state read-backs, else this thread's pinned one
-  PcgState* hs = hs_buf ? hs_buf : host_state();
-  if (!hs) {
+  PollBuf* pb = poll_buffer();
+  if (!pb) {
     set_error("pcg: pinned host allocation failed");
     return EFEA_ERR_HIP;
   }
+  PcgState* hs = hs_buf ? hs_buf : &pb->st;
   int launched = 0;
   bool spec = false;
-  double rr_spec = 0;
   while (true) {
     const int chunk = launched == 0 && first_check > 0 ? first_check : check_every;
     for (int k = 0; k < chunk; ++k) {
@@ -535,7 +535,7 @@ static int pcg_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, cons
     spec = !mixed && first_check > 0;
     if (spec) {
       pcg_spec_residual<DN, G>(A64, b, x, w, s);
-      if (hipMemcpyAsync(&rr_spec, w.small, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) {
+      if (hipMemcpyAsync(&pb->rr, w.small, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) {
         set_error("pcg: residual readback failed");
         return EFEA_ERR_HIP;
       }
@@ -559,16 +559,15 @@ static int pcg_run(const SpmvArgs& A64, const SpmvArgs& AK, const TP* Minv, cons
     if (hs->done || launched >= maxit) break;
   }
   const PcgState fin = *hs;
-  double rr_true = rr_spec;  // the last read-back's residual when it carried one
-  if (!spec) {
+  if (!spec) {  // else the last read-back carried the residual
     pcg_true_residual<DN, G>(A64, b, x, w, s);
-    if (hipMemcpyAsync(&rr_true, w.small, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+    if (hipMemcpyAsync(&pb->rr, w.small, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
       set_error("pcg: residual readback failed");
       return EFEA_ERR_HIP;
     }
   }
-  return pcg_finish(fin, rr_true, info);
+  return pcg_finish(fin, pb->rr, info);
 }
 
 // mode 64: FP64 matrix and preconditioner.  mode 32: FP32 matrix values and
