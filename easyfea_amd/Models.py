@@ -205,7 +205,10 @@ class PhaseField(_Observable):
     # the splits of the device kernels (EFEA_SPLIT_* of include/efea.h); the
     # others make g c+ + c- non-symmetric, which the PCG cannot solve
     SPLIT_CODE = {"Amor": 0, "Miehe": 1, "Bourdin": 2, "Stress": 3, "He": 4, "AnisotStrain": 5,
-                  "AnisotStrain_NoCross": 6, "AnisotStress": 7, "AnisotStress_NoCross": 8}
+                  "AnisotStrain_NoCross": 6, "AnisotStress": 7, "AnisotStress_NoCross": 8, "Zhang": 9,
+                  "AnisotStrain_PM": 10, "AnisotStrain_MP": 11, "AnisotStress_PM": 12, "AnisotStress_MP": 13}
+    # splits whose c+ / c- (hence the displacement stiffness) are not symmetric
+    NONSYMMETRIC = ("Zhang", "AnisotStrain_PM", "AnisotStrain_MP", "AnisotStress_PM", "AnisotStress_MP")
 
     @staticmethod
     def Get_splits() -> list:

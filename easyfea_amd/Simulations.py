@@ -546,7 +546,12 @@ class _Simu:
                        expected_iterations=expected, strategy=self.mixed_strategy, pipelined=self.pipelined,
                        coarse_target=self.coarse_target)
         job.isolate = self._isolation(problemType)
+        job.method = self._solve_method(problemType)
         return job
+
+    def _solve_method(self, problemType) -> str:
+        """The Krylov method of ``problemType``'s system: "pcg" (symmetric)."""
+        return "pcg"
 
     def _isolation(self, problemType):
         """Nodes a stalled solve of ``problemType`` may give aggregates of
@@ -1178,15 +1183,11 @@ class PhaseField(_Simu):
         return self._H0
 
     def _split_code(self) -> int:
-        """``EFEA_SPLIT_*`` of the model's split; Zhang and the ``_PM`` /
-        ``_MP`` variants make ``g c+ + c-`` non-symmetric, which the device's
-        conjugate-gradient solve cannot take."""
+        """``EFEA_SPLIT_*`` of the model's split (all fourteen in 2D)."""
         split = str(self.model.split)
         code = PhaseFieldModel.SPLIT_CODE.get(split)
         if code is None:
-            raise NotImplementedError(
-                f"split {split}: its displacement stiffness g c+ + c- is not symmetric; the device solver "
-                f"(PCG) needs a symmetric matrix. Device splits: {list(PhaseFieldModel.SPLIT_CODE)}")
+            raise NotImplementedError(f"split {split}. Device splits: {list(PhaseFieldModel.SPLIT_CODE)}")
         if self.model.dim == 3 and split not in ("Amor", "Bourdin"):
             raise NotImplementedError(f"split {split} in 3D: the device kernels provide Amor and Bourdin in 3D")
         return code
@@ -1457,6 +1458,14 @@ class PhaseField(_Simu):
         if dofs is None:
             return y.cpu().numpy()
         return y[to_dev(np.asarray(dofs, dtype=np.int64))].cpu().numpy()
+
+    def _solve_method(self, problemType) -> str:
+        """Zhang and the ``_PM`` / ``_MP`` splits make the displacement
+        stiffness ``g c+ + c-`` non-symmetric (``_phasefield.py:554, 634-676``;
+        EasyFEA solves it directly): BiCGStab with block Jacobi."""
+        if problemType == "elastic" and str(self.model.split) in PhaseFieldModel.NONSYMMETRIC:
+            return "bicgstab"
+        return "pcg"
 
     def _isolation(self, problemType):
         """Fully damaged nodes (d >= ISOLATE_DAMAGE), most damaged first: a

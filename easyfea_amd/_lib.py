@@ -57,6 +57,8 @@ SIGNATURES = {
     "efea_pcg2p_workspace_bytes": (sz, [i32, i32, i32]),
     "efea_pcgp_workspace_bytes": (sz, [i32, i32]),
     "efea_pcgp_max_lanes": (i32, []),
+    "efea_bicgstab_workspace_bytes": (sz, [i32, i32]),
+    "efea_solve_bicgstab": (i32, [i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, sz, f64, i32, i32, vp, vp]),
     "efea_solve_pcgp": (
         i32,
         [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, f64, f64, i32, i32, i32, i32, vp, vp],

@@ -8,7 +8,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 ROOT = os.path.dirname(PKG)
 SOURCES = ["capi.hip", "elements.hip", "sparse.hip", "batch.hip", "coarse.hip", "phasefield.hip", "postproc.hip",
-           "dynamics.hip", "bounded.hip", "pcg2p.hip", "elements_mfma.hip", "pcgp.hip"]
+           "dynamics.hip", "bounded.hip", "pcg2p.hip", "elements_mfma.hip", "pcgp.hip",
+           "bicgstab.hip"]
 # The phase-field split branches on exact equality of eigenvalues (as the
 # reference does); contracting a*b - c*d into an FMA changes which states
 # compare equal, so that file is compiled operation by operation.

@@ -99,6 +99,9 @@ int solve_pcg2(int, int, int, int, const int*, const int*, const double*, const 
 
 size_t pcg2p_workspace_bytes(int, int, int);
 size_t pcgp_workspace_bytes(int, int);
+size_t bicgstab_workspace_bytes(int, int);
+int solve_bicgstab(int, int, int, const int*, const int*, const double*, const unsigned char*, const double*,
+                   const double*, double*, void*, double, int, int, hipStream_t, double*);
 int solve_pcgp(int, int, int, int, const int*, const int*, const double*, const float*, const float*,
                const unsigned char*, const void*, const double*, double*, void*, double, double, int, int, int, int,
                hipStream_t, double*, void*);
@@ -388,6 +391,25 @@ int efea_solve_pcgp(int precision, int Nnode, int dof_n, int group, const int32_
 
 int efea_pcgp_max_lanes(void) { return 1024 * 256; }
 
+size_t efea_bicgstab_workspace_bytes(int Nnode, int dof_n) { return bicgstab_workspace_bytes(Nnode, dof_n); }
+
+int efea_solve_bicgstab(int Nnode, int dof_n, int group, const int32_t* node_ptr, const int32_t* node_cols,
+                        const double* values, const uint8_t* known, const double* Minv, const double* b, double* x,
+                        void* workspace, size_t workspace_bytes, double tol_residual, int maxit, int check_every,
+                        void* stream, double* info) {
+  if (Nnode <= 0 || !node_ptr || !node_cols || !values || !Minv || !b || !x || !workspace || !info) {
+    set_error("efea_solve_bicgstab: null pointer or empty system");
+    return EFEA_ERR_ARG;
+  }
+  if (workspace_bytes < bicgstab_workspace_bytes(Nnode, dof_n)) {
+    set_error("efea_solve_bicgstab: workspace of %zu bytes, %zu needed", workspace_bytes,
+              bicgstab_workspace_bytes(Nnode, dof_n));
+    return EFEA_ERR_SIZE;
+  }
+  return solve_bicgstab(Nnode, dof_n, group, node_ptr, node_cols, values, known, Minv, b, x, workspace,
+                        tol_residual, maxit, check_every, STREAM(stream), info);
+}
+
 size_t efea_pcg2p_workspace_bytes(int Nnode, int dof_n, int nslot) {
   return pcg2p_workspace_bytes(Nnode, dof_n, nslot);
 }
@@ -603,7 +625,7 @@ int efea_pf_damage_ke(int elem_type, int64_t Ne, const int32_t* connect, const d
     set_error("efea_pf_damage_ke: null pointer or negative size");
     return EFEA_ERR_ARG;
   }
-  if (split < EFEA_SPLIT_AMOR || split > EFEA_SPLIT_ANISOT_STRESS_NOCROSS || regu < 0 || regu > 1) {
+  if (split < EFEA_SPLIT_AMOR || split > EFEA_SPLIT_ANISOT_STRESS_MP || regu < 0 || regu > 1) {
     set_error("efea_pf_damage_ke: split %d / regularisation %d not supported", split, regu);
     return EFEA_ERR_ARG;
   }
@@ -620,7 +642,7 @@ int efea_pf_strain_stress(int elem_type, int64_t Ne, const int32_t* connect, con
     set_error("efea_pf_strain_stress: null pointer or negative size");
     return EFEA_ERR_ARG;
   }
-  if (split < EFEA_SPLIT_AMOR || split > EFEA_SPLIT_ANISOT_STRESS_NOCROSS) {
+  if (split < EFEA_SPLIT_AMOR || split > EFEA_SPLIT_ANISOT_STRESS_MP) {
     set_error("efea_pf_strain_stress: split %d not supported", split);
     return EFEA_ERR_ARG;
   }
@@ -636,7 +658,7 @@ int efea_pf_elastic_ke(int elem_type, int64_t Ne, const int32_t* connect, const 
     set_error("efea_pf_elastic_ke: null pointer or negative size");
     return EFEA_ERR_ARG;
   }
-  if (split < EFEA_SPLIT_AMOR || split > EFEA_SPLIT_ANISOT_STRESS_NOCROSS) {
+  if (split < EFEA_SPLIT_AMOR || split > EFEA_SPLIT_ANISOT_STRESS_MP) {
     set_error("efea_pf_elastic_ke: split %d not supported", split);
     return EFEA_ERR_ARG;
   }
@@ -650,7 +672,7 @@ int efea_pf_split_points(int64_t n, int split, const double* eps, const double* 
     set_error("efea_pf_split_points: null pointer or negative size");
     return EFEA_ERR_ARG;
   }
-  if (split < EFEA_SPLIT_AMOR || split > EFEA_SPLIT_ANISOT_STRESS_NOCROSS) {
+  if (split < EFEA_SPLIT_AMOR || split > EFEA_SPLIT_ANISOT_STRESS_MP) {
     set_error("efea_pf_split_points: split %d not supported", split);
     return EFEA_ERR_ARG;
   }

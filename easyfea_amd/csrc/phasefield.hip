@@ -110,12 +110,40 @@ __device__ inline void proj_km(const double (&t)[PfDim<DIM>::NS], double (&P)[Pf
   proj2d(t, P);
 }
 
+// The cross terms of the anisotropic splits (Models/_phasefield.py:548-566,
+// 660-676): 0 both to c+, 1 (_NoCross) both to c-, 2 (_PM) Cpm to c+ and Cmp
+// to c-, 3 (_MP) Cmp to c+ and Cpm to c-.
+template <int NS>
+__device__ inline void cross_assign(int how, const double (&Cpp)[NS][NS], const double (&Cpm)[NS][NS],
+                                    const double (&Cmm)[NS][NS], const double (&Cmp)[NS][NS],
+                                    double (&cP)[NS][NS], double (&cM)[NS][NS]) {
+#pragma unroll
+  for (int i = 0; i < NS; ++i)
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      if (how == 0) {
+        cP[i][j] = (Cpp[i][j] + Cpm[i][j]) + Cmp[i][j];
+        cM[i][j] = Cmm[i][j];
+      } else if (how == 1) {
+        cP[i][j] = Cpp[i][j];
+        cM[i][j] = (Cmm[i][j] + Cpm[i][j]) + Cmp[i][j];
+      } else if (how == 2) {
+        cP[i][j] = Cpp[i][j] + Cpm[i][j];
+        cM[i][j] = Cmm[i][j] + Cmp[i][j];
+      } else {
+        cP[i][j] = Cpp[i][j] + Cmp[i][j];
+        cM[i][j] = Cmm[i][j] + Cpm[i][j];
+      }
+    }
+}
+
 // cP, cM of the 2D split (PhaseField.Calc_C, Models/_phasefield.py:396-431) for
 // the Kelvin-Mandel strain e of an isotropic material (Lame constants lam,
 // mu of the model; E, nu for the stress split), codes EFEA_SPLIT_*:
 //   Amor (:451-483), Miehe (:503-537), Bourdin (:433-449), Stress (:573-634),
-//   He (:680-749), AnisotStrain[_NoCross] (:539-566),
-//   AnisotStress[_NoCross] (:636-676).
+//   He (:680-749), AnisotStrain[_NoCross, _PM, _MP] (:539-566),
+//   Zhang (:636-658), AnisotStress[_NoCross, _PM, _MP] (:636-676).
+// Zhang and the _PM / _MP cross assignments give a non-symmetric c+ / c-.
 // The reference's evaluation order of each product and sum is kept.
 template <int DIM>
 __device__ inline void split_t(int split, const double (&e)[PfDim<DIM>::NS], double lam, double mu, double E,
@@ -157,7 +185,8 @@ __device__ inline void split_t(int split, const double (&e)[PfDim<DIM>::NS], dou
     return;
   }
   double P[NS][NS], M[NS][NS];
-  if (split == EFEA_SPLIT_MIEHE || split == EFEA_SPLIT_ANISOT_STRAIN || split == EFEA_SPLIT_ANISOT_STRAIN_NOCROSS) {
+  if (split == EFEA_SPLIT_MIEHE || split == EFEA_SPLIT_ANISOT_STRAIN || split == EFEA_SPLIT_ANISOT_STRAIN_NOCROSS ||
+      split == EFEA_SPLIT_ANISOT_STRAIN_PM || split == EFEA_SPLIT_ANISOT_STRAIN_MP) {
     proj_km<DIM>(e, P);
 #pragma unroll
     for (int i = 0; i < NS; ++i)
@@ -181,14 +210,8 @@ __device__ inline void split_t(int split, const double (&e)[PfDim<DIM>::NS], dou
     mmN<NS>(PTC, M, Cpm);
     mmN<NS>(MTC, M, Cmm);
     mmN<NS>(MTC, P, Cmp);
-    const bool cross = split == EFEA_SPLIT_ANISOT_STRAIN;
-#pragma unroll
-    for (int i = 0; i < NS; ++i)
-#pragma unroll
-      for (int j = 0; j < NS; ++j) {
-        cP[i][j] = cross ? (Cpp[i][j] + Cpm[i][j]) + Cmp[i][j] : Cpp[i][j];
-        cM[i][j] = cross ? Cmm[i][j] : (Cmm[i][j] + Cpm[i][j]) + Cmp[i][j];
-      }
+    cross_assign<NS>(split == EFEA_SPLIT_ANISOT_STRAIN ? 0 : split == EFEA_SPLIT_ANISOT_STRAIN_PM ? 2
+                     : split == EFEA_SPLIT_ANISOT_STRAIN_MP ? 3 : 1, Cpp, Cpm, Cmm, Cmp, cP, cM);
     return;
   }
   if (split == EFEA_SPLIT_HE) {
@@ -263,10 +286,16 @@ __device__ inline void split_t(int split, const double (&e)[PfDim<DIM>::NS], dou
     mmN<NS>(T, C, cM);
     return;
   }
-  // AnisotStress[_NoCross]: Cp = P C, Cm = M C and the compliance S = C^-1
-  // = I / (2 mu) - lam / (2 mu (2 mu + DIM lam)) IxI
+  // Zhang: c+ = P C, c- = M C.  AnisotStress[_NoCross, _PM, _MP]: Cp = P C,
+  // Cm = M C and the compliance S = C^-1 = I / (2 mu) - lam / (2 mu (2 mu +
+  // DIM lam)) IxI
   double S[NS][NS], Cp[NS][NS], Cm[NS][NS], PS[NS][NS], MS[NS][NS], Cpp[NS][NS], Cpm[NS][NS], Cmm[NS][NS],
       Cmp[NS][NS];
+  if (split == EFEA_SPLIT_ZHANG) {
+    mmN<NS>(P, C, cP);
+    mmN<NS>(M, C, cM);
+    return;
+  }
   const double s2 = lam / (2 * mu * (2 * mu + DIM * lam));
 #pragma unroll
   for (int i = 0; i < NS; ++i)
@@ -280,14 +309,8 @@ __device__ inline void split_t(int split, const double (&e)[PfDim<DIM>::NS], dou
   mmN<NS>(PS, Cm, Cpm);
   mmN<NS>(MS, Cm, Cmm);
   mmN<NS>(MS, Cp, Cmp);
-  const bool cross = split == EFEA_SPLIT_ANISOT_STRESS;
-#pragma unroll
-  for (int i = 0; i < NS; ++i)
-#pragma unroll
-    for (int j = 0; j < NS; ++j) {
-      cP[i][j] = cross ? (Cpp[i][j] + Cpm[i][j]) + Cmp[i][j] : Cpp[i][j];
-      cM[i][j] = cross ? Cmm[i][j] : (Cmm[i][j] + Cpm[i][j]) + Cmp[i][j];
-    }
+  cross_assign<NS>(split == EFEA_SPLIT_ANISOT_STRESS ? 0 : split == EFEA_SPLIT_ANISOT_STRESS_PM ? 2
+                   : split == EFEA_SPLIT_ANISOT_STRESS_MP ? 3 : 1, Cpp, Cpm, Cmm, Cmp, cP, cM);
 }
 template <int NPE>
 __device__ inline void strain2d(const double (&dNx)[2][NPE], const double* ue, double e[3]) {
@@ -736,6 +759,10 @@ static unsigned pf_blocks(int64_t Ne) {
 
 // In 3D only the splits without a spectral decomposition are provided.
 static bool pf_split_ok(int elem, int split, const char* what) {
+  if (split < EFEA_SPLIT_AMOR || split > EFEA_SPLIT_ANISOT_STRESS_MP) {
+    set_error("%s: unknown split %d", what, split);
+    return false;
+  }
   const bool three_d = elem == ET_TETRA4 || elem == ET_TETRA10 || elem == ET_HEXA8;
   if (three_d && split != EFEA_SPLIT_AMOR && split != EFEA_SPLIT_BOURDIN) {
     set_error("%s: split %d is not provided in 3D (Amor and Bourdin are)", what, split);

@@ -543,6 +543,8 @@ class SolveJob:
         self.Ainv32: Optional[torch.Tensor] = None
         assert pipelined in ("auto", True, False), pipelined
         self.pipelined_mode = pipelined
+        # "pcg", or "bicgstab" for a non-symmetric matrix (FP64, block Jacobi)
+        self.method = "pcg"
         self.pipe_replace_every = 0  # pipelined block Jacobi: residual replacement period (0: the library's)
         self.pipelined = False  # decided by run(), as refine
         # isolate(): the nodes whose elements have lost their stiffness (a
@@ -711,6 +713,8 @@ class SolveJob:
         solve that still fails is redone, two-level, with the nodes
         ``isolate()`` names as aggregates of their own."""
         ws = ws or Workspace()
+        if self.method == "bicgstab":
+            return self._run_bicgstab(ws, raise_on_fail)
         self.refine = self._wants_refine()
         self.pipelined = self._wants_pipeline()
         isolating = self.isolate is not None and self.A.pattern.coord is not None
@@ -737,6 +741,24 @@ class SolveJob:
             rc, si = self._isolated_retry(ws, rc, si)
         if rc != 0 and raise_on_fail:
             raise _lib.EfeaError("efea_solve_pcg", rc, _lib.last_error())
+        return si
+
+    def _run_bicgstab(self, ws: Workspace, raise_on_fail: bool) -> SolveInfo:
+        """A non-symmetric system: right-preconditioned BiCGStab with the
+        FP64 block-Jacobi inverse (``efea_solve_bicgstab``), to a relative
+        residual of ``tol_residual``."""
+        self.precond, self.pipelined, self.refine = "bj", False, False
+        self.setup(64)
+        A = self.A
+        raw = (C.c_double * 4)()
+        nbytes = _lib.load().efea_bicgstab_workspace_bytes(A.Nn, A.dof_n)
+        buf = ws.get(nbytes)
+        rc = _lib.call("efea_solve_bicgstab", A.Nn, A.dof_n, A.group, ptr(A.node_ptr), ptr(A.node_cols),
+                       ptr(A.values), ptr(self.known), ptr(self.Minv), ptr(self.b), ptr(self.x), ptr(buf), nbytes,
+                       self.tol_residual, self.maxit, self.check_every, stream_ptr(), C.cast(raw, C.c_void_p))
+        si = self.info(raw)
+        if rc != 0 and raise_on_fail:
+            raise _lib.EfeaError("efea_solve_bicgstab", rc, _lib.last_error())
         return si
 
     def _isolated_retry(self, ws: Workspace, rc: int, si: SolveInfo):
@@ -915,6 +937,8 @@ class BatchExecutor:
         self.waiting = {}  # jobs of a full batch, in submission order
 
     def submit(self, job: SolveJob) -> None:
+        if job.method != "pcg":
+            raise NotImplementedError(f"batched solves are PCG solves; a {job.method} job runs alone (job.run())")
         key = (job.precision, job.A.dof_n, job.A.group, job.precond)
         b = self.batches.get(key)
         if b is None:

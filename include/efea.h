@@ -261,6 +261,22 @@ int efea_solve_pcgp(int precision, int Nnode, int dof_n, int group, const int32_
                     size_t workspace_bytes, double tol_residual, double tol_anorm, int replace_every, int maxit,
                     int check_every, int first_check, void* stream, double* info);
 
+/* Right-preconditioned BiCGStab with the nodal block-Jacobi inverse of
+ * efea_bj_setup (mode 64, FP64 throughout) for non-symmetric systems: the
+ * displacement stiffness of the phase-field splits Zhang and *_PM / *_MP.
+ * Same matrix, mask, b / x conventions and info as efea_solve_pcg; stops when
+ * the recursive residual norm is below tol_residual times the initial one;
+ * info[1] is the true relative residual of the returned x.  Breakdown or no
+ * convergence within maxit: EFEA_ERR_NOCONV.  Synchronises the stream every
+ * check_every iterations.  Workspace: efea_bicgstab_workspace_bytes.
+ * Replaces the non-symmetric branch of _Solve_Axb (Simulations/Solvers.py:
+ * 225-394), which EasyFEA solves with a sparse direct solver. */
+size_t efea_bicgstab_workspace_bytes(int Nnode, int dof_n);
+int efea_solve_bicgstab(int Nnode, int dof_n, int group, const int32_t* node_ptr, const int32_t* node_cols,
+                        const double* values, const uint8_t* known, const double* Minv, const double* b, double* x,
+                        void* workspace, size_t workspace_bytes, double tol_residual, int maxit, int check_every,
+                        void* stream, double* info);
+
 /* Pipelined two-level PCG (Ghysels and Vanroose's preconditioned pipelined
  * CG) with the preconditioner of efea_solve_pcg2: two kernels per iteration
  * instead of three (the restriction P^T w and both dot products ride on the
@@ -359,9 +375,9 @@ int efea_dot(int64_t n, const double* a, const double* b, double* scratch, doubl
 int efea_gather_sum(int64_t n, const int32_t* idx, const double* y, double* out, void* stream);
 
 /* Energy splits of the phase-field kernels (PhaseField.SplitType,
- * Models/_phasefield.py:55-78) whose stiffness g c+ + c- is symmetric, as the
- * PCG requires: Zhang and the *_PM / *_MP variants give non-symmetric ones and
- * are not provided. */
+ * Models/_phasefield.py:55-78).  Codes 0-8 give a symmetric stiffness
+ * g c+ + c- (PCG); Zhang and the *_PM / *_MP variants (9-13) give a
+ * non-symmetric one, solved by efea_solve_bicgstab. */
 #define EFEA_SPLIT_AMOR 0
 #define EFEA_SPLIT_MIEHE 1
 #define EFEA_SPLIT_BOURDIN 2
@@ -371,6 +387,12 @@ int efea_gather_sum(int64_t n, const int32_t* idx, const double* y, double* out,
 #define EFEA_SPLIT_ANISOT_STRAIN_NOCROSS 6
 #define EFEA_SPLIT_ANISOT_STRESS 7
 #define EFEA_SPLIT_ANISOT_STRESS_NOCROSS 8
+/* non-symmetric c+ / c- (the displacement system needs a non-symmetric solver) */
+#define EFEA_SPLIT_ZHANG 9
+#define EFEA_SPLIT_ANISOT_STRAIN_PM 10
+#define EFEA_SPLIT_ANISOT_STRAIN_MP 11
+#define EFEA_SPLIT_ANISOT_STRESS_PM 12
+#define EFEA_SPLIT_ANISOT_STRESS_MP 13
 
 /* Phase field, damage problem: psi+ at the mass points with the History
  * irreversibility (psi+ <- H_old where smaller; H_old may be NULL), then

@@ -301,15 +301,46 @@ def test_staggered_steps_match_oracle(solver, split):
     assert np.max(np.abs(f_dev - f_ref) / np.abs(f_ref)) < 1e-6
 
 
-@pytest.mark.parametrize("split", ["Zhang", "AnisotStrain_PM", "AnisotStress_MP"])
-def test_non_symmetric_splits_are_refused(split):
-    mesh = ea.tri3_notched_square(8)
+@pytest.mark.parametrize("solver", ["hip_fp64", "hip_mixed"])
+@pytest.mark.parametrize("split", list(Models.PhaseField.NONSYMMETRIC))
+def test_non_symmetric_splits_match_oracle(split, solver):
+    """Zhang and the _PM / _MP splits: the displacement stiffness is not
+    symmetric (its asymmetry is checked), the device solves it with
+    BiCGStab + block Jacobi in FP64; five load steps of the notched square
+    against the oracle's direct solves (u, d within 1e-8, force within 1e-6)."""
+    n = 40
+    mesh = ea.tri3_notched_square(n)
     mat = Models.Elastic.Isotropic(2, 210000.0, 0.3, planeStress=False, thickness=1.0)
     simu = Simulations.PhaseField(mesh, Models.PhaseField(mat, split, "AT2", 2.7, 0.05))
+    simu.solver = solver
+    ref = O.PhaseFieldOracle(mesh.connect, mesh.coord, "TRI3", 210000.0, 0.3, False, 1.0, split, "AT2", 2.7, 0.05)
     top = mesh.Nodes_Conditions(lambda x, y, z: y == 1.0)
-    simu.add_dirichlet(top, [0, 1e-3], ["x", "y"])
-    with pytest.raises(NotImplementedError, match="not symmetric"):
-        simu.Solve()
+    bot = mesh.Nodes_Conditions(lambda x, y, z: y == 0.0)
+    dofsY = simu.Bc_dofs_nodes(top, ["y"])
+    rows, f_dev, f_ref = [], [], []
+    for it in range(1, 6):
+        dep = it * 2e-3
+        for s in (simu, ref):
+            s.Bc_Init()
+            s.add_dirichlet(top, [0, dep], ["x", "y"])
+            s.add_dirichlet(bot, [0], ["y"])
+        u, d, conv = simu.Solve(1.0, 500, 1)
+        ur, dr, _ = ref.Solve(1.0, 500, 1)
+        simu.Save_Iter()
+        ref.Save_Iter()
+        f_dev.append(np.sum(simu.Calc_Reaction(dofsY, "elastic")))
+        f_ref.append(np.sum(ref.Calc_Reaction(dofsY)))
+        i = simu.last_solve["elastic"]
+        Ku = simu._Ku.to_scipy()
+        asym = abs(Ku - Ku.T).max() / abs(Ku).max()
+        rows.append((it, i.iterations, i.rel_residual, i.mode, _rel(u, ur), _rel(d, dr), float(dr.max()), asym))
+        assert conv and i.status == 1
+    print(f"{split} {solver}: it, BiCGStab iterations, rel residual, mode, u err, d err, max d, K asymmetry:", rows)
+    assert max(r[7] for r in rows) > 1e-6, "the split's stiffness should be non-symmetric"
+    for r in rows:
+        assert r[4] < 1e-8 and r[5] < 1e-8 and r[3] == "fp64", rows
+    f_dev, f_ref = np.array(f_dev), np.array(f_ref)
+    assert np.max(np.abs(f_dev - f_ref) / np.abs(f_ref)) < 1e-6
 
 
 def test_history_damage_returns_the_larger_damage():

@@ -75,6 +75,12 @@ def test_abi_argument_errors_are_reported_without_gpu():
     assert lib.efea_solve_pcgp(64, 100, 1, 4, *[fake] * 10, 16, 1e-12, 1e-9, 0, 10, 8, 0, None, fake) == -4
     assert "workspace" in _lib.last_error()
     assert lib.efea_pcgp_workspace_bytes(100, 2) >= 10 * 100 * 2 * 8
+    # BiCGStab (non-symmetric systems)
+    assert lib.efea_solve_bicgstab(0, 2, 4, *[fake] * 8, 1 << 30, 1e-12, 10, 8, None, fake) == -1
+    assert "null pointer or empty" in _lib.last_error()
+    assert lib.efea_solve_bicgstab(100, 2, 4, *[fake] * 8, 16, 1e-12, 10, 8, None, fake) == -4
+    assert "workspace" in _lib.last_error()
+    assert lib.efea_bicgstab_workspace_bytes(100, 2) >= 8 * 100 * 2 * 8
 
 
 def test_workspace_sizes_and_groups():
