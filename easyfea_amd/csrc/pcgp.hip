@@ -180,8 +180,18 @@ k_pb_iter(SpmvArgs A, const TP* __restrict__ Minv, double* __restrict__ x, PbWor
     return;
   }
   const bool lead = have && g == 0;
-  double own[9][DN];
+  // the own rows' operands, block-Jacobi inverse, mask and (FP32 values)
+  // row-sum correction, all loaded together ahead of the gather
+  double own[9][DN], Mown[DN * DN], Cown[DN * DN];
+  bool kn[DN];
   if (lead) {
+#pragma unroll
+    for (int e = 0; e < DN * DN; ++e) {
+      Mown[e] = (double)Minv[(int64_t)n * DN * DN + e];
+      Cown[e] = A.corr ? (double)((const TM*)A.corr)[(int64_t)n * DN * DN + e] : 0.0;
+    }
+#pragma unroll
+    for (int a = 0; a < DN; ++a) kn[a] = A.known && A.known[(int64_t)n * DN + a];
 #pragma unroll
     for (int a = 0; a < DN; ++a) {
       const int64_t i = (int64_t)n * DN + a;
@@ -226,18 +236,15 @@ k_pb_iter(SpmvArgs A, const TP* __restrict__ Minv, double* __restrict__ x, PbWor
   if (lead) {
     if (A.corr) {  // FP32 values: the rounding's row sums on the own rows
 #pragma unroll
-      for (int a = 0; a < DN; ++a) {
-        const TM* cr = (const TM*)A.corr + (int64_t)n * DN * DN + a * DN;
+      for (int a = 0; a < DN; ++a)
 #pragma unroll
-        for (int bb = 0; bb < DN; ++bb) sum[0][a] += (double)cr[bb] * own[0][bb];
-      }
+        for (int bb = 0; bb < DN; ++bb) sum[0][a] += Cown[a * DN + bb] * own[0][bb];
     }
     double wn[DN];
 #pragma unroll
     for (int a = 0; a < DN; ++a) {
       const int64_t i = (int64_t)n * DN + a;
-      const bool kn = A.known && A.known[i];
-      const double y = kn ? 0.0 : sum[0][a];
+      const double y = kn[a] ? 0.0 : sum[0][a];
       const double mv = own[0][a], wv = own[1][a], uv = own[2][a];
       const double zv = y + (first ? 0.0 : beta * own[5][a]);
       const double qv = mv + (first ? 0.0 : beta * own[6][a]);
@@ -258,10 +265,13 @@ k_pb_iter(SpmvArgs A, const TP* __restrict__ Minv, double* __restrict__ x, PbWor
       red[1] += wn[a] * un;
       red[2] += rnw * rnw;
     }
-    double mn[DN];
-    apply_bj<TP, DN>(Minv, A.known, n, wn, mn);  // the next launch's m (0 at known DOFs)
 #pragma unroll
-    for (int a = 0; a < DN; ++a) mo[(int64_t)n * DN + a] = mn[a];
+    for (int a = 0; a < DN; ++a) {  // the next launch's m = M w (0 at known DOFs)
+      double mv = 0;
+#pragma unroll
+      for (int bb = 0; bb < DN; ++bb) mv += Mown[a * DN + bb] * wn[bb];
+      mo[(int64_t)n * DN + a] = kn[a] ? 0.0 : mv;
+    }
   }
   block_sum_all<3, BS>(red, shr);
   if (threadIdx.x == 0) {
