@@ -113,7 +113,7 @@ k_pp_spmv(SpmvArgs A, PipeArgs PA, const double* __restrict__ rel, const double*
   __shared__ double shn[NM][BS / 64];
   __shared__ double shp[4][BS / 64];
   double acc[4] = {0, 0, 0, 0};
-  for (int c = blockIdx.x; c < PA.nchunk; c += gridDim.x) {
+  for (int c = xcd_block(blockIdx.x, gridDim.x); c < PA.nchunk; c += gridDim.x) {  // XCD-contiguous chunks
     const int g = (int)threadIdx.x % G;
     const int* pos = PA.rows + 3 * ((int64_t)c * (BS / G) + (int)threadIdx.x / G);
     int n = pos[0], p0 = pos[1], p1 = pos[2];
@@ -211,7 +211,9 @@ k_pp_iter(SpmvArgs A, PipeArgs PA, const double* __restrict__ rel, const double*
   constexpr int NM = Modes<DN>::NM;
   __shared__ double shr[NM + 3][BS / 64];
   PcgState* st = W.st;
-  const int c = blockIdx.x;
+  // logical chunk: each XCD takes one contiguous range of chunks, so the m
+  // entries a chunk gathers from its neighbours sit mostly in that XCD's L2
+  const int c = xcd_block(blockIdx.x, gridDim.x);
   // the lane's position (node and row range) is read together with the
   // solver state: the exit test uses both (n < -1 never holds), so the state
   // read adds no round trip

@@ -161,7 +161,9 @@ k_pb_iter(SpmvArgs A, const TP* __restrict__ Minv, double* __restrict__ x, PbWor
   PcgState* so = W.st[par ^ 1];
   const double* __restrict__ mi = W.m[par];
   double* __restrict__ mo = W.m[par ^ 1];
-  const int nb = gridDim.x, blk = blockIdx.x;
+  // logical block: each XCD takes one contiguous range of node groups, so the m
+  // entries a block gathers from its neighbours sit mostly in that XCD's L2
+  const int nb = gridDim.x, blk = xcd_block(blockIdx.x, nb);
   const int64_t gid = (int64_t)blk * BS + threadIdx.x;
   const bool have = gid < (int64_t)A.Nnode * G;
   const int n = have ? (int)(gid / G) : 0, g = (int)(gid % G);
