@@ -29,8 +29,18 @@ def require_gpu() -> torch.device:
     return torch.device("cuda", torch.cuda.current_device())
 
 
-def stream_ptr() -> int:
+def _stream_ptr_public() -> int:
     return torch.cuda.current_stream().cuda_stream
+
+
+def _stream_ptr_raw() -> int:
+    # torch's current raw stream of the current device, without the Python
+    # stream object (several microseconds per call, paid before every
+    # launch the host issues while the GPU waits between solves)
+    return torch._C._cuda_getCurrentRawStream(torch._C._cuda_getDevice())
+
+
+stream_ptr = _stream_ptr_raw if hasattr(torch._C, "_cuda_getCurrentRawStream") else _stream_ptr_public
 
 
 def ptr(t: Optional[torch.Tensor]) -> Optional[int]:
