@@ -30,7 +30,7 @@ import torch
 
 from . import _lib
 from ._utils import ElemType, MatrixType
-from .device import DeviceCsr, DeviceMesh, SolveJob, Workspace, drive, lincomb, ptr, stream_ptr, to_dev
+from .device import DeviceCsr, DeviceMesh, PendingSolve, SolveJob, Workspace, drive, lincomb, ptr, stream_ptr, to_dev
 from .mesh import Mesh
 from .Models import Anisotropic, Isotropic
 from .Models import PhaseField as PhaseFieldModel
@@ -547,7 +547,20 @@ class _Simu:
                        coarse_target=self.coarse_target)
         job.isolate = self._isolation(problemType)
         job.method = self._solve_method(problemType)
+        job.defer = self._defer(problemType)
+        job.before_solve = self._before_solve(problemType)
         return job
+
+    def _before_solve(self, problemType):
+        """``SolveJob.before_solve`` of ``problemType``'s solve (none outside
+        the phase field)."""
+        return None
+
+    def _defer(self, problemType) -> bool:
+        """Whether the single-system driver may queue ``problemType``'s solve
+        and hand it back unfinished (``SolveJob.defer``); never outside the
+        phase field."""
+        return False
 
     def _solve_method(self, problemType) -> str:
         """The Krylov method of ``problemType``'s system: "pcg" (symmetric)."""
@@ -565,6 +578,25 @@ class _Simu:
         job = self._make_job(A, b_dev, x_dev, problemType, zero_dirichlet)
         t = time.perf_counter()
         info = yield job
+        if isinstance(info, PendingSolve):
+            # queued without a wait: _finish_pending ends it
+            self._pending = (info, problemType, t)
+            return
+        self._after_solve(job, info, problemType, t)
+
+    def _finish_pending(self) -> bool:
+        """Ends the solve the single-system driver left queued, if any; True
+        when its solution changed after the queued iterations, so that work
+        queued from it since must be redone."""
+        p = self.__dict__.pop("_pending", None)
+        if p is None:
+            return False
+        pending, problemType, t = p
+        info = pending.finish()
+        self._after_solve(pending.job, info, problemType, t)
+        return pending.resumed
+
+    def _after_solve(self, job: SolveJob, info, problemType, t: float) -> None:
         job.record(info)
         self.last_job = job
         self.last_solve[problemType] = info
@@ -603,6 +635,7 @@ class _Simu:
 
     def _solve_system(self, A, b_dev: torch.Tensor, x_dev: torch.Tensor, problemType) -> None:
         drive(self._solve_system_gen(A, b_dev, x_dev, problemType))
+        self._finish_pending()
 
     # ------------------------------------------------------------ results
     def Save_Iter(self, iter: Optional[dict] = None) -> None:
@@ -1114,6 +1147,14 @@ class PhaseField(_Simu):
         self._Fd = torch.zeros(Nn, dtype=torch.float64, device="cuda")
         self.Niter_staggered = 0
         self.convIter = 0.0
+        # The single-system path queues a damage solve with an expected
+        # iteration count (the pipelined block-Jacobi PCG) without waiting for
+        # it, queues the displacement assembly and its solve's set-up behind
+        # it, and reads the damage solve's end just before the displacement
+        # solve; a damage solve that needed more iterations than it queued is
+        # continued, and the displacement system assembled and set up again.
+        # Same iterates either way.
+        self.defer_damage = True
         # BoundConstrain solve: CGLS reduction per pass, first-order tolerance
         # (damage units), CGLS iterations per pass, active-set updates
         self.bounded_tol = 1e-10
@@ -1353,6 +1394,16 @@ class PhaseField(_Simu):
         yield from self._solve_system_gen(Ku, b, x, "elastic")
         self._u = x
 
+    def _settle_damage(self) -> bool:
+        """``before_solve`` of the displacement solve: ends a deferred damage
+        solve, queued before the displacement assembly and set-up.  If it went
+        on past its queued iterations, the assembly is redone from the final
+        damage (True: the solve's set-up is redone too)."""
+        if self._finish_pending():
+            self._assemble_elastic()
+            return True
+        return False
+
     def _staggered(self, tolConv=1.0, maxIter=500, convOption=2):
         """The staggered loop as a generator of solve jobs; returns
         ``converged``.  ``Solve`` runs it on the single-system path, the
@@ -1372,6 +1423,7 @@ class PhaseField(_Simu):
             yield from self._solve_damage_gen()
             self._updU = False
             yield from self._solve_elastic_gen()
+            self._finish_pending()  # no-op: the displacement solve's before_solve ended it
             self._updD = False
             d_np1, u_np1 = self._d, self._u
             if convOption == 0:
@@ -1466,6 +1518,14 @@ class PhaseField(_Simu):
         if problemType == "elastic" and str(self.model.split) in PhaseFieldModel.NONSYMMETRIC:
             return "bicgstab"
         return "pcg"
+
+    def _defer(self, problemType) -> bool:
+        """The damage solve, with ``defer_damage``: the displacement
+        assembly and its solve's set-up are queued while it runs."""
+        return problemType == "damage" and self.defer_damage
+
+    def _before_solve(self, problemType):
+        return self._settle_damage if problemType == "elastic" else None
 
     def _isolation(self, problemType):
         """Fully damaged nodes (d >= ISOLATE_DAMAGE), most damaged first: a

@@ -63,6 +63,14 @@ SIGNATURES = {
         i32,
         [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, f64, f64, i32, i32, i32, i32, vp, vp],
     ),
+    "efea_pcgp_start": (
+        i32,
+        [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, f64, f64, i32, i32, i32, i32, vp],
+    ),
+    "efea_pcgp_finish": (
+        i32,
+        [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, f64, f64, i32, i32, i32, i32, vp, vp, vp],
+    ),
     "efea_pp_iter_bench": (
         i32,
         [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, i32, vp, i32, vp, vp, i32, vp, vp, sz, i32, vp],

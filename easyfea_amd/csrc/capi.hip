@@ -104,7 +104,7 @@ int solve_bicgstab(int, int, int, const int*, const int*, const double*, const u
                    const double*, double*, void*, double, int, int, hipStream_t, double*);
 int solve_pcgp(int, int, int, int, const int*, const int*, const double*, const float*, const float*,
                const unsigned char*, const void*, const double*, double*, void*, double, double, int, int, int, int,
-               hipStream_t, double*, void*);
+               hipStream_t, double*, void*, int, int*);
 int solve_pcg2p(int, int, int, int, const int*, const int*, const double*, const float*, const float*,
                 const unsigned char*, const void*, int, const int*, const int*, const double*, const double*, int, int,
                 const int*, const int*, int, const int*, const double*, double*, void*, double, double, int, int, int,
@@ -361,32 +361,84 @@ int efea_solve_pcg_noalloc(int mode, int Nnode, int dof_n, int group, const int3
 
 size_t efea_pcgp_workspace_bytes(int Nnode, int dof_n) { return pcgp_workspace_bytes(Nnode, dof_n); }
 
+// Argument checks shared by efea_solve_pcgp and its deferred form.
+static int pcgp_check(const char* fn, int precision, int Nnode, int group, const int32_t* node_ptr,
+                      const int32_t* node_cols, const double* values64, const float* values32, const void* Minv,
+                      const double* b, double* x, void* workspace, size_t workspace_bytes, int dof_n) {
+  if (Nnode <= 0 || !node_ptr || !node_cols || !values64 || !Minv || !b || !x || !workspace) {
+    set_error("%s: null pointer or empty system", fn);
+    return EFEA_ERR_ARG;
+  }
+  if ((precision != 64 && precision != 32) || (precision == 32 && !values32)) {
+    set_error("%s: precision %d (64, or 32 with FP32 values)", fn, precision);
+    return EFEA_ERR_ARG;
+  }
+  if (group <= 0 || (int64_t)Nnode * group > (int64_t)efea_pcgp_max_lanes()) {
+    set_error("%s: %d nodes x %d lanes exceed %d lanes (one pass of the grid)", fn, Nnode, group,
+              efea_pcgp_max_lanes());
+    return EFEA_ERR_ARG;
+  }
+  if (workspace_bytes < pcgp_workspace_bytes(Nnode, dof_n)) {
+    set_error("%s: workspace of %zu bytes, %zu needed", fn, workspace_bytes, pcgp_workspace_bytes(Nnode, dof_n));
+    return EFEA_ERR_SIZE;
+  }
+  return EFEA_OK;
+}
+
 int efea_solve_pcgp(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr,
                     const int32_t* node_cols, const double* values64, const float* values32, const float* corr32,
                     const uint8_t* known, const void* Minv, const double* b, double* x, void* workspace,
                     size_t workspace_bytes, double tol_residual, double tol_anorm, int replace_every, int maxit,
                     int check_every, int first_check, void* stream, double* info) {
-  if (Nnode <= 0 || !node_ptr || !node_cols || !values64 || !Minv || !b || !x || !workspace || !info) {
-    set_error("efea_solve_pcgp: null pointer or empty system");
+  if (!info) {
+    set_error("efea_solve_pcgp: null info");
     return EFEA_ERR_ARG;
   }
-  if ((precision != 64 && precision != 32) || (precision == 32 && !values32)) {
-    set_error("efea_solve_pcgp: precision %d (64, or 32 with FP32 values)", precision);
+  if (int e = pcgp_check("efea_solve_pcgp", precision, Nnode, group, node_ptr, node_cols, values64, values32, Minv,
+                         b, x, workspace, workspace_bytes, dof_n))
+    return e;
+  return solve_pcgp(precision, Nnode, dof_n, group, node_ptr, node_cols, values64, values32, corr32, known, Minv, b,
+                    x, workspace, tol_residual, tol_anorm, replace_every, maxit, check_every, first_check,
+                    STREAM(stream), info, nullptr, PCGP_WHOLE, nullptr);
+}
+
+int efea_pcgp_start(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr,
+                    const int32_t* node_cols, const double* values64, const float* values32, const float* corr32,
+                    const uint8_t* known, const void* Minv, const double* b, double* x, void* workspace,
+                    size_t workspace_bytes, double tol_residual, double tol_anorm, int replace_every, int maxit,
+                    int check_every, int first_check, void* stream) {
+  if (int e = pcgp_check("efea_pcgp_start", precision, Nnode, group, node_ptr, node_cols, values64, values32, Minv,
+                         b, x, workspace, workspace_bytes, dof_n))
+    return e;
+  if (first_check < 1) {
+    set_error("efea_pcgp_start: first_check %d (the iterations to queue, >= 1)", first_check);
     return EFEA_ERR_ARG;
-  }
-  if (group <= 0 || (int64_t)Nnode * group > (int64_t)efea_pcgp_max_lanes()) {
-    set_error("efea_solve_pcgp: %d nodes x %d lanes exceed %d lanes (one pass of the grid)", Nnode, group,
-              efea_pcgp_max_lanes());
-    return EFEA_ERR_ARG;
-  }
-  if (workspace_bytes < pcgp_workspace_bytes(Nnode, dof_n)) {
-    set_error("efea_solve_pcgp: workspace of %zu bytes, %zu needed", workspace_bytes,
-              pcgp_workspace_bytes(Nnode, dof_n));
-    return EFEA_ERR_SIZE;
   }
   return solve_pcgp(precision, Nnode, dof_n, group, node_ptr, node_cols, values64, values32, corr32, known, Minv, b,
                     x, workspace, tol_residual, tol_anorm, replace_every, maxit, check_every, first_check,
-                    STREAM(stream), info, nullptr);
+                    STREAM(stream), nullptr, nullptr, PCGP_START, nullptr);
+}
+
+int efea_pcgp_finish(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr,
+                     const int32_t* node_cols, const double* values64, const float* values32, const float* corr32,
+                     const uint8_t* known, const void* Minv, const double* b, double* x, void* workspace,
+                     size_t workspace_bytes, double tol_residual, double tol_anorm, int replace_every, int maxit,
+                     int check_every, int first_check, void* stream, double* info, int* resumed) {
+  if (!info) {
+    set_error("efea_pcgp_finish: null info");
+    return EFEA_ERR_ARG;
+  }
+  if (int e = pcgp_check("efea_pcgp_finish", precision, Nnode, group, node_ptr, node_cols, values64, values32, Minv,
+                         b, x, workspace, workspace_bytes, dof_n))
+    return e;
+  if (first_check < 1) {
+    set_error("efea_pcgp_finish: first_check %d (the start's, >= 1)", first_check);
+    return EFEA_ERR_ARG;
+  }
+  if (check_every < 1) check_every = 1;
+  return solve_pcgp(precision, Nnode, dof_n, group, node_ptr, node_cols, values64, values32, corr32, known, Minv, b,
+                    x, workspace, tol_residual, tol_anorm, replace_every, maxit, check_every, first_check,
+                    STREAM(stream), info, nullptr, PCGP_FINISH, resumed);
 }
 
 int efea_pcgp_max_lanes(void) { return 1024 * 256; }

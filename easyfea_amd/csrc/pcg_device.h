@@ -707,12 +707,18 @@ int pcg_finish_info(const void* state, double rr_true, double* info);
 // This thread's pinned buffer of the solvers' polls (null if the pinned
 // allocation failed): the state and the speculative residual norm are two
 // asynchronous copies followed by one synchronisation, where a copy into
-// pageable memory would be a blocking staged copy of its own.  Defined in
+// pageable memory would be a blocking staged copy of its own.  Slot 0 serves
+// the solves that wait for their polls; slot 1 holds the read-back of a
+// deferred pipelined solve between its start and its finish.  Defined in
 // sparse.hip.
 struct PollBuf {
   PcgState st;
   double rr;
 };
-PollBuf* poll_buffer();
+PollBuf* poll_buffer(int slot = 0);
+// Phases of the pipelined block-Jacobi solve (solve_pcgp): the whole solve,
+// or its deferred form split into a start that only queues work and a
+// finish that waits for it.
+enum { PCGP_WHOLE = 0, PCGP_START = 1, PCGP_FINISH = 2 };
 
 }  // namespace efea

@@ -426,10 +426,17 @@ size_t pcgp_workspace_bytes(int Nnode, int dof_n) { return pcgp_workspace_size(N
 
 // A: the FP64 matrix (start-up, replacements, residuals); Ait: the
 // iterations' matrix (A itself, or the FP32 copy with its correction).
+// phase PCGP_WHOLE runs the solve to its end; PCGP_START queues the start-up
+// passes, the first first_check iterations, the residual estimate and the
+// state read-back into poll slot 1 (or hs) and returns without waiting;
+// PCGP_FINISH waits for that read-back and continues the same loop if the
+// stopping test had not been met (*resumed = 1 then).  The iterates do not
+// depend on where the polls fall (the stopping test and the replacements are
+// decided on the device), so the split solve returns what the whole one does.
 template <typename TM, typename TP, int DN, int G>
 static int pcgp_run(const SpmvArgs& A, const SpmvArgs& Ait, const TP* Minv, const double* b, double* x, void* ws,
                     double tol_r, double tol_A, int replace_every, int maxit, int check_every, int first_check,
-                    hipStream_t s, double* info, PcgState* hs) {
+                    hipStream_t s, double* info, PcgState* hs, int phase, int* resumed) {
   const int Nnode = A.Nnode;
   if (replace_every <= 0) replace_every = sizeof(TM) == 4 ? PB_REPLACE_MIXED : PB_REPLACE;
   const int restart = sizeof(TM) == 4 ? 1 : 0;  // FP32 values: replacements restart the recurrence
@@ -441,14 +448,16 @@ static int pcgp_run(const SpmvArgs& A, const SpmvArgs& Ait, const TP* Minv, cons
     return EFEA_ERR_ARG;
   }
   const unsigned nb = grid_for(lanes);
-  hipLaunchKernelGGL(k_pb_state, dim3(1), dim3(64), 0, s, W.st[0], tol_r * tol_r, tol_A * tol_A);
-  hipLaunchKernelGGL((k_pb_pass<TP, DN, G, 0>), dim3(nb), dim3(BS), 0, s, A, Minv, b, (const double*)x, W, W.m[0],
-                     W.part[1], W.st[0], 0);
-  hipLaunchKernelGGL((k_pb_pass<TP, DN, G, 1>), dim3(nb), dim3(BS), 0, s, A, Minv, b, (const double*)x, W, W.m[0],
-                     W.part[0], W.st[0], 0);
-  hipLaunchKernelGGL(k_pb_init_scalars, dim3(1), dim3(BS), 0, s, (const double*)W.part[1],
-                     (const double*)W.part[0], (int)nb, W.st[0]);
-  if (int e = check_launch("pcgp_start")) return e;
+  if (phase != PCGP_WHOLE && first_check < 1) {
+    set_error("pcgp: a deferred solve needs first_check >= 1 (got %d)", first_check);
+    return EFEA_ERR_ARG;
+  }
+  PollBuf* pb = poll_buffer(phase == PCGP_WHOLE ? 0 : 1);
+  if (!pb) {
+    set_error("pcgp: pinned host allocation failed");
+    return EFEA_ERR_HIP;
+  }
+  if (!hs) hs = &pb->st;
   PcgWork fw{};  // residual passes of the read-backs
   fw.q = W.t;
   fw.r = W.t;
@@ -456,13 +465,31 @@ static int pcgp_run(const SpmvArgs& A, const SpmvArgs& Ait, const TP* Minv, cons
   fw.small = W.small;
   int launched = 0;
   bool spec = false;
-  PollBuf* pb = poll_buffer();
-  if (!pb) {
-    set_error("pcgp: pinned host allocation failed");
-    return EFEA_ERR_HIP;
+  bool wait_only = false;  // finish: the start's read-back is in flight
+  if (phase == PCGP_FINISH) {
+    launched = first_check;
+    spec = true;
+    wait_only = true;
+  } else {
+    hipLaunchKernelGGL(k_pb_state, dim3(1), dim3(64), 0, s, W.st[0], tol_r * tol_r, tol_A * tol_A);
+    hipLaunchKernelGGL((k_pb_pass<TP, DN, G, 0>), dim3(nb), dim3(BS), 0, s, A, Minv, b, (const double*)x, W,
+                       W.m[0], W.part[1], W.st[0], 0);
+    hipLaunchKernelGGL((k_pb_pass<TP, DN, G, 1>), dim3(nb), dim3(BS), 0, s, A, Minv, b, (const double*)x, W,
+                       W.m[0], W.part[0], W.st[0], 0);
+    hipLaunchKernelGGL(k_pb_init_scalars, dim3(1), dim3(BS), 0, s, (const double*)W.part[1],
+                       (const double*)W.part[0], (int)nb, W.st[0]);
+    if (int e = check_launch("pcgp_start")) return e;
   }
-  if (!hs) hs = &pb->st;
   while (true) {
+    if (wait_only) {
+      wait_only = false;
+      if (hipStreamSynchronize(s) != hipSuccess) {
+        set_error("pcgp: state readback failed");
+        return EFEA_ERR_HIP;
+      }
+      if (hs->done || launched >= maxit) break;
+      if (resumed) *resumed = 1;
+    }
     const int chunk = launched == 0 && first_check > 0 ? first_check : check_every;
     for (int k = 0; k < chunk; ++k) {
       const int par = launched & 1;
@@ -486,8 +513,12 @@ static int pcgp_run(const SpmvArgs& A, const SpmvArgs& Ait, const TP* Minv, cons
       }
     }
     if (int e = check_launch("pcgp_iter")) return e;
-    if (hipMemcpyAsync(hs, W.st[launched & 1], sizeof(PcgState), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess) {
+    if (hipMemcpyAsync(hs, W.st[launched & 1], sizeof(PcgState), hipMemcpyDeviceToHost, s) != hipSuccess) {
+      set_error("pcgp: state readback failed");
+      return EFEA_ERR_HIP;
+    }
+    if (phase == PCGP_START) return 0;
+    if (hipStreamSynchronize(s) != hipSuccess) {
       set_error("pcgp: state readback failed");
       return EFEA_ERR_HIP;
     }
@@ -508,8 +539,10 @@ static int pcgp_run(const SpmvArgs& A, const SpmvArgs& Ait, const TP* Minv, cons
 int solve_pcgp(int precision, int Nnode, int dof_n, int group, const int* node_ptr, const int* node_cols,
                const double* vals64, const float* vals32, const float* corr32, const unsigned char* known,
                const void* Minv, const double* b, double* x, void* ws, double tol_r, double tol_A, int replace_every,
-               int maxit, int check_every, int first_check, hipStream_t s, double* info, void* host_state_buf) {
+               int maxit, int check_every, int first_check, hipStream_t s, double* info, void* host_state_buf,
+               int phase, int* resumed) {
   SpmvArgs A{Nnode, node_ptr, node_cols, vals64, known, nullptr, 0};
+  if (resumed) *resumed = 0;
   SpmvArgs A32{Nnode, node_ptr, node_cols, vals32, known, corr32, 0};
   if (check_every < 1) check_every = 1;
   PcgState* hs = (PcgState*)host_state_buf;  // null: this thread's pinned poll buffer
@@ -517,9 +550,9 @@ int solve_pcgp(int precision, int Nnode, int dof_n, int group, const int* node_p
   if (dof_n == D && group == GV) {                                                                             \
     if (precision == 32)                                                                                       \
       return pcgp_run<float, float, D, GV>(A, A32, (const float*)Minv, b, x, ws, tol_r, tol_A, replace_every,   \
-                                           maxit, check_every, first_check, s, info, hs);                      \
+                                           maxit, check_every, first_check, s, info, hs, phase, resumed);      \
     return pcgp_run<double, double, D, GV>(A, A, (const double*)Minv, b, x, ws, tol_r, tol_A, replace_every,    \
-                                           maxit, check_every, first_check, s, info, hs);                      \
+                                           maxit, check_every, first_check, s, info, hs, phase, resumed);      \
   }
   EFEA_GROUPS(RUNB)
 #undef RUNB

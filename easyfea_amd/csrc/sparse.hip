@@ -401,12 +401,13 @@ size_t pcg_workspace_bytes(int Nnode, int dof_n, int group) {
   return pcg_workspace_size(Nnode, dof_n);
 }
 
-PollBuf* poll_buffer() {
-  static thread_local PollBuf* h = nullptr;
-  if (!h) {
-    if (hipHostMalloc((void**)&h, sizeof(PollBuf), hipHostMallocDefault) != hipSuccess) h = nullptr;
+PollBuf* poll_buffer(int slot) {
+  static thread_local PollBuf* h[2] = {nullptr, nullptr};
+  PollBuf*& p = h[slot != 0 ? 1 : 0];
+  if (!p) {
+    if (hipHostMalloc((void**)&p, sizeof(PollBuf), hipHostMallocDefault) != hipSuccess) p = nullptr;
   }
-  return h;
+  return p;
 }
 
 // Initialisation of one solve: state, x0.A x0, r0 = b - A x0, z0 = p0 =

@@ -557,6 +557,13 @@ class SolveJob:
         self.method = "pcg"
         self.pipe_replace_every = 0  # pipelined block Jacobi: residual replacement period (0: the library's)
         self.pipelined = False  # decided by run(), as refine
+        # defer: the single-system driver may start the solve without waiting
+        # for it (start(): a PendingSolve the owner finishes later)
+        self.defer = False
+        # before_solve(): called once the set-up is queued, just before the
+        # solve; True when the matrix values were assembled again meanwhile,
+        # and the set-up that read them is then redone
+        self.before_solve: Optional[Callable[[], bool]] = None
         # isolate(): the nodes whose elements have lost their stiffness (a
         # phase-field displacement solve); set, a two-level solve that stalls
         # is redone with those nodes as aggregates of their own
@@ -660,6 +667,18 @@ class SolveJob:
             return min(self.check_every, 8), max(1, int(0.95 * n))
         return self.check_every, 0
 
+    def _pcgp_args(self, ws: Workspace, check_every: int, first_check: int) -> tuple:
+        """The arguments of ``efea_solve_pcgp`` (and of its deferred form)
+        before the stream."""
+        A = self.A
+        v32 = A.values32 if self.precision == 32 else None
+        c32 = A.corr32 if self.precision == 32 else None
+        nbytes = _lib.load().efea_pcgp_workspace_bytes(A.Nn, A.dof_n)
+        buf = ws.get(nbytes)
+        return (self.precision, A.Nn, A.dof_n, A.group, ptr(A.node_ptr), ptr(A.node_cols), ptr(A.values), ptr(v32),
+                ptr(c32), ptr(self.known), ptr(self.Minv), ptr(self.b), ptr(self.x), ptr(buf), nbytes,
+                self.tol_residual, self.tol_anorm, self.pipe_replace_every, self.maxit, check_every, first_check)
+
     def _solve_once(self, ws: Workspace):
         A = self.A
         raw = (C.c_double * 4)()
@@ -667,13 +686,8 @@ class SolveJob:
         v32 = A.values32 if self.precision == 32 else None
         c32 = A.corr32 if self.precision == 32 else None
         if self.precond == "bj" and self.pipelined:
-            nbytes = _lib.load().efea_pcgp_workspace_bytes(A.Nn, A.dof_n)
-            buf = ws.get(nbytes)
-            rc = _lib.call("efea_solve_pcgp", self.precision, A.Nn, A.dof_n, A.group, ptr(A.node_ptr),
-                           ptr(A.node_cols), ptr(A.values), ptr(v32), ptr(c32), ptr(self.known), ptr(self.Minv),
-                           ptr(self.b), ptr(self.x), ptr(buf), nbytes, self.tol_residual, self.tol_anorm,
-                           self.pipe_replace_every, self.maxit, check_every, first_check, stream_ptr(),
-                           C.cast(raw, C.c_void_p))
+            args = self._pcgp_args(ws, check_every, first_check)
+            rc = _lib.call("efea_solve_pcgp", *args, stream_ptr(), C.cast(raw, C.c_void_p))
             return rc, self.info(raw)
         if self.precond == "two-level" and self.pipelined:
             cs = self.cs
@@ -725,16 +739,36 @@ class SolveJob:
         ws = ws or Workspace()
         if self.method == "bicgstab":
             return self._run_bicgstab(ws, raise_on_fail)
+        self._prepare()
+        if self.before_solve is not None and self.before_solve():
+            self._redo_setup()
+        rc, si = self._solve_once(ws)
+        return self._conclude(ws, rc, si, raise_on_fail)
+
+    def _redo_setup(self) -> None:
+        """The set-up again after the matrix values changed: block-Jacobi
+        inverse, FP32 copy and, if this job built it, the coarse inverse (a
+        lagged one it reused stays, as it would have)."""
+        if self.precond == "two-level" and not self.coarse_reused:
+            if self._cache is not None:
+                self._cache.stale = True
+            self.Ainv = None
+        self.setup(self.precision)
+
+    def _prepare(self) -> None:
         self.refine = self._wants_refine()
         self.pipelined = self._wants_pipeline()
         isolating = self.isolate is not None and self.A.pattern.coord is not None
         if (self.pipelined or isolating) and self.x0 is None:
             self.x0 = self.x.clone()
-        maxit = self.maxit
+        self._maxit = self.maxit
         if isolating:
-            self.maxit = min(maxit, max(ISOLATE_FIRST_MAXIT, 4 * (self.expected_iterations or 0)))
+            self.maxit = min(self.maxit, max(ISOLATE_FIRST_MAXIT, 4 * (self.expected_iterations or 0)))
         self.setup(self.precision)
-        rc, si = self._solve_once(ws)
+
+    def _conclude(self, ws: Workspace, rc: int, si: SolveInfo, raise_on_fail: bool) -> SolveInfo:
+        """The fallbacks of a solve that did not converge (see ``run``)."""
+        isolating = self.isolate is not None and self.A.pattern.coord is not None
         if rc != 0 and self.pipelined:
             # a pipelined breakdown (rounding in its recurrences): the classic
             # iteration from the same start
@@ -746,12 +780,31 @@ class SolveJob:
         if rc != 0 and self.precision == 32:
             self.restart_fp64()
             rc, si = self._solve_once(ws)
-        self.maxit = maxit
+        self.maxit = self._maxit
         if rc != 0 and isolating:
             rc, si = self._isolated_retry(ws, rc, si)
         if rc != 0 and raise_on_fail:
             raise _lib.EfeaError("efea_solve_pcg", rc, _lib.last_error())
         return si
+
+    def start(self, ws: Optional[Workspace] = None) -> Optional["PendingSolve"]:
+        """Starts a pipelined block-Jacobi solve with an expected iteration
+        count without waiting for it (``efea_pcgp_start``): the start-up and
+        ``deferred_budget(expected)`` iterations are queued on the current
+        stream and the caller's host work overlaps them.  None (nothing
+        queued) for any other solve, which ``run`` then does at once."""
+        if self.method != "pcg" or self.isolate is not None or not self.expected_iterations:
+            return None
+        self.refine = self._wants_refine()
+        if self.precond != "bj" or not self._wants_pipeline():
+            return None
+        ws = ws or Workspace()
+        self._prepare()
+        check_every, _ = self._polls()
+        args = self._pcgp_args(ws, check_every, deferred_budget(self.expected_iterations))
+        s = stream_ptr()
+        _lib.call("efea_pcgp_start", *args, s)
+        return PendingSolve(self, ws, args, s)
 
     def _run_bicgstab(self, ws: Workspace, raise_on_fail: bool) -> SolveInfo:
         """A non-symmetric system: right-preconditioned BiCGStab with the
@@ -759,6 +812,8 @@ class SolveJob:
         residual of ``tol_residual``."""
         self.precond, self.pipelined, self.refine = "bj", False, False
         self.setup(64)
+        if self.before_solve is not None and self.before_solve():
+            self.setup(64)
         A = self.A
         raw = (C.c_double * 4)()
         nbytes = _lib.load().efea_bicgstab_workspace_bytes(A.Nn, A.dof_n)
@@ -817,6 +872,34 @@ class SolveJob:
         return rc, si
 
 
+def deferred_budget(expected: int) -> int:
+    """Iterations a deferred solve queues before its first read-back: the
+    expected count and a margin, so that the read-back rarely finds the
+    solve unfinished (a launch after convergence returns at once)."""
+    return int(expected) + max(4, int(expected) // 8)
+
+
+class PendingSolve:
+    """A solve queued by ``SolveJob.start``; ``finish`` waits for it and
+    continues it if the queued iterations were not enough.  ``resumed`` is
+    then True: the solution changed after the queued iterations (further
+    iterations, or a fallback solve), and work queued from it in between must
+    be redone."""
+
+    def __init__(self, job: SolveJob, ws: Workspace, args: tuple, stream: int):
+        self.job, self.ws, self.args, self.stream = job, ws, args, stream
+        self.resumed = False
+
+    def finish(self, raise_on_fail: bool = True) -> SolveInfo:
+        job = self.job
+        raw = (C.c_double * 4)()
+        resumed = C.c_int(0)
+        rc = _lib.call("efea_pcgp_finish", *self.args, self.stream, C.cast(raw, C.c_void_p), C.byref(resumed))
+        si = job._conclude(self.ws, rc, job.info(raw), raise_on_fail)
+        self.resumed = bool(resumed.value) or rc != 0
+        return si
+
+
 def solve(A: DeviceCsr, b: torch.Tensor, x: torch.Tensor, known: Optional[torch.Tensor], mode: str,
           ws: Workspace, tol_residual: float = 1e-12, tol_anorm: float = 1e-9, replace_ratio: float = 0.0,
           maxit: int = 100000, check_every: int = 32, raise_on_fail: bool = True) -> SolveInfo:
@@ -832,11 +915,14 @@ def solve(A: DeviceCsr, b: torch.Tensor, x: torch.Tensor, known: Optional[torch.
 
 def drive(gen):
     """Runs a generator of ``SolveJob`` s to completion, solving each job
-    at once (the single-system path); returns the generator's value."""
+    at once (the single-system path), except a job marked ``defer`` that
+    ``SolveJob.start`` can queue: the generator then receives the
+    ``PendingSolve`` and finishes it itself.  Returns the generator's value."""
     try:
         job = next(gen)
         while True:
-            job = gen.send(job.run())
+            pending = job.start() if job.defer else None
+            job = gen.send(job.run() if pending is None else pending)
     except StopIteration as stop:
         return stop.value
 

@@ -261,6 +261,31 @@ int efea_solve_pcgp(int precision, int Nnode, int dof_n, int group, const int32_
                     size_t workspace_bytes, double tol_residual, double tol_anorm, int replace_every, int maxit,
                     int check_every, int first_check, void* stream, double* info);
 
+/* efea_solve_pcgp split in two, for a caller with host work to overlap with
+ * the solve (the staggered loop prepares the displacement system while the
+ * damage solve runs).  efea_pcgp_start queues the start-up passes, the first
+ * first_check (>= 1) iterations, the residual estimate and the read-back of
+ * the solver state on `stream` and returns without waiting (no info).
+ * efea_pcgp_finish, called with the same arguments on the same stream before
+ * the next efea_pcgp_start of the calling thread, waits for that read-back
+ * and, if the stopping test had not been met, continues the solve with a
+ * read-back every check_every iterations, as efea_solve_pcgp does.  The
+ * iterates, x and info are those of efea_solve_pcgp with the same
+ * first_check.  *resumed (may be null) is set to 1 when iterations ran after
+ * the start's: x then changed after the work the start queued, and what the
+ * caller queued in between from x is stale.  Replaces the same reference
+ * call as efea_solve_pcgp (__Solver_1, Simulations/Solvers.py:502-553). */
+int efea_pcgp_start(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr,
+                    const int32_t* node_cols, const double* values64, const float* values32, const float* corr32,
+                    const uint8_t* known, const void* Minv, const double* b, double* x, void* workspace,
+                    size_t workspace_bytes, double tol_residual, double tol_anorm, int replace_every, int maxit,
+                    int check_every, int first_check, void* stream);
+int efea_pcgp_finish(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr,
+                     const int32_t* node_cols, const double* values64, const float* values32, const float* corr32,
+                     const uint8_t* known, const void* Minv, const double* b, double* x, void* workspace,
+                     size_t workspace_bytes, double tol_residual, double tol_anorm, int replace_every, int maxit,
+                     int check_every, int first_check, void* stream, double* info, int* resumed);
+
 /* Right-preconditioned BiCGStab with the nodal block-Jacobi inverse of
  * efea_bj_setup (mode 64, FP64 throughout) for non-symmetric systems: the
  * displacement stiffness of the phase-field splits Zhang and *_PM / *_MP.

@@ -75,6 +75,22 @@ def test_abi_argument_errors_are_reported_without_gpu():
     assert lib.efea_solve_pcgp(64, 100, 1, 4, *[fake] * 10, 16, 1e-12, 1e-9, 0, 10, 8, 0, None, fake) == -4
     assert "workspace" in _lib.last_error()
     assert lib.efea_pcgp_workspace_bytes(100, 2) >= 10 * 100 * 2 * 8
+    # its deferred form: the same checks, and at least one queued iteration
+    assert lib.efea_pcgp_start(64, 0, 1, 4, *[fake] * 10, 1 << 30, 1e-12, 1e-9, 0, 10, 8, 40, None) == -1
+    assert "efea_pcgp_start: null pointer or empty" in _lib.last_error()
+    assert lib.efea_pcgp_start(64, 100, 1, 4, *[fake] * 10, 1 << 30, 1e-12, 1e-9, 0, 10, 8, 0, None) == -1
+    assert "first_check" in _lib.last_error()
+    assert lib.efea_pcgp_start(64, 100, 1, 4, *[fake] * 10, 16, 1e-12, 1e-9, 0, 10, 8, 40, None) == -4
+    resumed = C.c_int(7)
+    assert lib.efea_pcgp_finish(64, 100, 1, 4, *[fake] * 10, 1 << 30, 1e-12, 1e-9, 0, 10, 8, 40, None, None,
+                                C.byref(resumed)) == -1
+    assert "null info" in _lib.last_error()
+    assert lib.efea_pcgp_finish(64, 100, 1, 4, *[fake] * 10, 1 << 30, 1e-12, 1e-9, 0, 10, 8, 0, None, fake,
+                                C.byref(resumed)) == -1
+    assert "first_check" in _lib.last_error()
+    assert lib.efea_pcgp_finish(32, 100, 1, 4, fake, fake, fake, None, *[fake] * 6, 1 << 30, 1e-12, 1e-9, 0, 10,
+                                8, 40, None, fake, None) == -1
+    assert "precision" in _lib.last_error()
     # BiCGStab (non-symmetric systems)
     assert lib.efea_solve_bicgstab(0, 2, 4, *[fake] * 8, 1 << 30, 1e-12, 10, 8, None, fake) == -1
     assert "null pointer or empty" in _lib.last_error()
