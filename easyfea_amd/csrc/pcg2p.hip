@@ -215,13 +215,13 @@ k_pp_iter(SpmvArgs A, PipeArgs PA, const double* __restrict__ rel, const double*
   // entries a chunk gathers from its neighbours sit mostly in that XCD's L2
   const int c = xcd_block(blockIdx.x, gridDim.x);
   // the lane's position (node and row range) is read together with the
-  // solver state: the exit test uses both (n < -1 never holds), so the state
-  // read adds no round trip
+  // solver state: the exit test uses all three (n < -1 and p1 < p0 never
+  // hold), so neither the state read nor the row range adds a round trip
   const int g = (int)threadIdx.x % G;
   const int* pos = PA.rows + 3 * ((int64_t)c * (BS / G) + (int)threadIdx.x / G);
   int n = pos[0], p0 = pos[1], p1 = pos[2];
   const int done = st->done;
-  if (done != 0 || n < -1) {  // done: decided by kernel A; recorded for the next kernel A
+  if (done != 0 || n < -1 || p1 < p0) {  // done: decided by kernel A; recorded for the next kernel A
     if (done != 0 && c == 0 && threadIdx.x == 0) st->stop = 1;
     return;
   }
@@ -229,9 +229,20 @@ k_pp_iter(SpmvArgs A, PipeArgs PA, const double* __restrict__ rel, const double*
   const double alpha = st->alpha, beta = st->beta;
   const bool have = n >= 0;
   if (!have) n = p0 = p1 = 0;
-  double own[9][DN], relf[3];
+  // The own rows' operands, mask and FP32 row-sum correction are loaded here,
+  // by every lane (a wave's load costs one instruction whichever lanes are
+  // active; the G lanes of a node read one address), with no branch around
+  // them: the gather then starts while they are in flight.  Loaded under the
+  // leading lane's branch, the gather waited for them where the branch ends;
+  // loaded where they are used (the mask after the gather's reduction, the
+  // second DOF's after the first DOF's stores), they were dependent memory
+  // round trips at the end of every wave.  Without a mask the flag is read
+  // from m and ignored; the correction exists only with FP32 values.
+  double own[9][DN], relf[3], cor[DN][DN];
+  int kb[DN];
+  const bool hk = A.known != nullptr;
   const bool lead = have && g == 0;
-  if (lead) {
+  {
 #pragma unroll
     for (int a = 0; a < DN; ++a) {
       const int64_t i = (int64_t)n * DN + a;
@@ -244,9 +255,17 @@ k_pp_iter(SpmvArgs A, PipeArgs PA, const double* __restrict__ rel, const double*
       own[6][a] = W.q[i];
       own[7][a] = W.s[i];
       own[8][a] = W.p[i];
+      kb[a] = *(hk ? A.known + i : (const unsigned char*)m);
     }
 #pragma unroll
     for (int d = 0; d < 3; ++d) relf[d] = rel[3 * (int64_t)n + d];
+    if constexpr (sizeof(TM) == 4) {
+      const TM* cr = (const TM*)A.corr + (int64_t)n * DN * DN;
+#pragma unroll
+      for (int a = 0; a < DN; ++a)
+#pragma unroll
+        for (int bb = 0; bb < DN; ++bb) cor[a][bb] = (double)cr[a * DN + bb];
+    }
   }
   double sum[1][DN];
 #pragma unroll
@@ -263,12 +282,13 @@ k_pp_iter(SpmvArgs A, PipeArgs PA, const double* __restrict__ rel, const double*
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) sum[0][a] += __shfl_xor(sum[0][a], o, G);
   }
-  if (lead && A.corr) {  // FP32 values: the rounding's row sums on the own rows
+  if constexpr (sizeof(TM) == 4) {  // FP32 values: the rounding's row sums on the own rows
+    if (lead) {
 #pragma unroll
-    for (int a = 0; a < DN; ++a) {
-      const TM* cr = (const TM*)A.corr + (int64_t)n * DN * DN + a * DN;
+      for (int a = 0; a < DN; ++a) {
 #pragma unroll
-      for (int bb = 0; bb < DN; ++bb) sum[0][a] += (double)cr[bb] * own[0][bb];
+        for (int bb = 0; bb < DN; ++bb) sum[0][a] += cor[a][bb] * own[0][bb];
+      }
     }
   }
   double red[NM + 3];
@@ -278,8 +298,7 @@ k_pp_iter(SpmvArgs A, PipeArgs PA, const double* __restrict__ rel, const double*
 #pragma unroll
     for (int a = 0; a < DN; ++a) {
       const int64_t i = (int64_t)n * DN + a;
-      const bool kn = A.known && A.known[i];
-      const double y = kn ? 0.0 : sum[0][a];
+      const double y = (hk && kb[a] != 0) ? 0.0 : sum[0][a];
       const double mv = own[0][a], wv = own[1][a], uv = own[2][a];
       const double zv = y + (first ? 0.0 : beta * own[5][a]);
       const double qv = mv + (first ? 0.0 : beta * own[6][a]);
