@@ -512,7 +512,6 @@ class Workspace:
 
     def __init__(self):
         self.buf: Optional[torch.Tensor] = None
-        self.small = torch.zeros(1100, dtype=torch.float64, device="cuda")
 
     def get(self, nbytes: int) -> torch.Tensor:
         if self.buf is None or self.buf.numel() < nbytes:
