@@ -1153,8 +1153,11 @@ class PhaseField(_Simu):
         # it, and reads the damage solve's end just before the displacement
         # solve; a damage solve that needed more iterations than it queued is
         # continued, and the displacement system assembled and set up again.
-        # Same iterates either way.
+        # The displacement solve likewise, with the damage assembly and the
+        # energy of the convergence measure (convOption 1 / 2) queued behind
+        # it.  Same iterates either way.
         self.defer_damage = True
+        self.defer_displacement = True
         # BoundConstrain solve: CGLS reduction per pass, first-order tolerance
         # (damage units), CGLS iterations per pass, active-set updates
         self.bounded_tol = 1e-10
@@ -1423,13 +1426,19 @@ class PhaseField(_Simu):
             yield from self._solve_damage_gen()
             self._updU = False
             yield from self._solve_elastic_gen()
-            self._finish_pending()  # no-op: the displacement solve's before_solve ended it
             self._updD = False
+            if convOption == 3:
+                self._finish_pending()  # its measure is read on the host
             d_np1, u_np1 = self._d, self._u
             if convOption == 0:
                 convIter = torch.max(torch.abs(d_np1 - d_n))
             elif convOption in (1, 2):
                 E_np1 = self._energy_dev(convOption, fu)
+                if self._finish_pending():
+                    # a deferred displacement solve went on past its queued
+                    # iterations: the damage system and the energy again
+                    self._updD = False
+                    E_np1 = self._energy_dev(convOption, fu)
                 dE = torch.abs(E_n - E_np1)
                 convIter = torch.where(E_np1 == 0, dE, dE / torch.abs(E_np1))
             elif convOption == 3:
@@ -1441,6 +1450,7 @@ class PhaseField(_Simu):
                 diffD[nz] = diffD[nz] / torch.abs(d_np1[nz])
                 convU, convD = float(diffU.sum()), float(diffD.sum())
                 convIter = max(convD, convU)
+            self._finish_pending()  # (convOption 0: its measure reads no displacement)
             if tolConv == 1 or float(d_np1.max()) == 0:
                 converged = True
             elif convOption == 3:
@@ -1521,8 +1531,12 @@ class PhaseField(_Simu):
 
     def _defer(self, problemType) -> bool:
         """The damage solve, with ``defer_damage``: the displacement
-        assembly and its solve's set-up are queued while it runs."""
-        return problemType == "damage" and self.defer_damage
+        assembly and its solve's set-up are queued while it runs; the
+        displacement solve, with ``defer_displacement``: the damage assembly
+        and energy of the convergence measure are queued while it runs."""
+        if problemType == "damage":
+            return self.defer_damage
+        return problemType == "elastic" and self.defer_displacement
 
     def _before_solve(self, problemType):
         return self._settle_damage if problemType == "elastic" else None

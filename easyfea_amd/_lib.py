@@ -80,6 +80,16 @@ SIGNATURES = {
         [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, i32, i32, vp, vp, i32, vp, vp, vp,
          vp, sz, f64, f64, i32, i32, i32, vp, vp],
     ),
+    "efea_pcg2p_start": (
+        i32,
+        [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, i32, i32, vp, vp, i32, vp, vp, vp,
+         vp, sz, f64, f64, i32, i32, i32, vp],
+    ),
+    "efea_pcg2p_finish": (
+        i32,
+        [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, i32, i32, vp, vp, i32, vp, vp, vp,
+         vp, sz, f64, f64, i32, i32, i32, vp, vp, vp],
+    ),
     "efea_batch_create": (i32, [i32, i32, i32, i32, i32, C.POINTER(vp)]),
     "efea_batch_destroy": (i32, [vp]),
     "efea_batch_add": (

@@ -108,7 +108,7 @@ int solve_pcgp(int, int, int, int, const int*, const int*, const double*, const 
 int solve_pcg2p(int, int, int, int, const int*, const int*, const double*, const float*, const float*,
                 const unsigned char*, const void*, int, const int*, const int*, const double*, const double*, int, int,
                 const int*, const int*, int, const int*, const double*, double*, void*, double, double, int, int, int,
-                hipStream_t, double*, void*);
+                hipStream_t, double*, void*, int, int*);
 int pp_iter_bench(int, int, int, int, const int*, const int*, const double*, const float*, const float*,
                   const unsigned char*, int, const double*, int, const int*, const int*, int, double*, void*, int,
                   hipStream_t);
@@ -487,42 +487,92 @@ int efea_pp_iter_bench(int precision, int Nnode, int dof_n, int group, const int
                        rel, nchunk, chunk_rows, chunk_slot, chunks_per_agg, x, workspace, reps, STREAM(stream));
 }
 
-int efea_solve_pcg2p(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr,
-                     const int32_t* node_cols, const double* values64, const float* values32, const float* corr32,
-                     const uint8_t* known, const void* Minv, int nagg,
-                     const int32_t* agg_ptr, const int32_t* agg_nodes, const double* rel, const double* Ainv, int ld,
-                     int nchunk, const int32_t* chunk_rows, const int32_t* chunk_slot, int chunks_per_agg,
-                     const int32_t* agg_first, const double* b, double* x, void* workspace, size_t workspace_bytes,
-                     double tol_residual, double tol_anorm, int maxit, int check_every, int first_check, void* stream,
-                     double* info) {
+// Argument checks shared by efea_solve_pcg2p and its deferred form.
+static int pcg2p_check(const char* fn, int precision, int Nnode, int dof_n, const int32_t* node_ptr,
+                       const int32_t* node_cols, const double* values64, const float* values32, const void* Minv,
+                       int nagg, const int32_t* agg_ptr, const int32_t* agg_nodes, const double* rel,
+                       const double* Ainv, int ld, int nchunk, const int32_t* chunk_rows, const int32_t* chunk_slot,
+                       int chunks_per_agg, const int32_t* agg_first, const double* b, double* x, void* workspace,
+                       size_t workspace_bytes) {
   if (Nnode <= 0 || nagg <= 0 || nchunk <= 0 || chunks_per_agg <= 0 || !node_ptr || !node_cols || !values64 ||
       !Minv || !agg_ptr || !agg_nodes || !rel || !Ainv || !chunk_rows || !chunk_slot || !agg_first || !b || !x ||
-      !workspace || !info) {
-    set_error("efea_solve_pcg2p: null pointer or empty system");
+      !workspace) {
+    set_error("%s: null pointer or empty system", fn);
     return EFEA_ERR_ARG;
   }
   if ((precision != 64 && precision != 32) || (precision == 32 && !values32)) {
-    set_error("efea_solve_pcg2p: precision %d (64, or 32 with FP32 values)", precision);
+    set_error("%s: precision %d (64, or 32 with FP32 values)", fn, precision);
     return EFEA_ERR_ARG;
   }
   if (nchunk > 1024) {
-    set_error("efea_solve_pcg2p: %d node chunks, at most 1024 (one block each)", nchunk);
+    set_error("%s: %d node chunks, at most 1024 (one block each)", fn, nchunk);
     return EFEA_ERR_ARG;
   }
   if (ld < coarse_ld(nagg, dof_n)) {
-    set_error("efea_solve_pcg2p: leading dimension %d < %d", ld, coarse_ld(nagg, dof_n));
+    set_error("%s: leading dimension %d < %d", fn, ld, coarse_ld(nagg, dof_n));
     return EFEA_ERR_ARG;
   }
   if (workspace_bytes < pcg2p_workspace_bytes(Nnode, dof_n, nagg * chunks_per_agg)) {
-    set_error("efea_solve_pcg2p: workspace of %zu bytes, %zu needed", workspace_bytes,
+    set_error("%s: workspace of %zu bytes, %zu needed", fn, workspace_bytes,
               pcg2p_workspace_bytes(Nnode, dof_n, nagg * chunks_per_agg));
     return EFEA_ERR_SIZE;
   }
-  return solve_pcg2p(precision, Nnode, dof_n, group, node_ptr, node_cols, values64, values32, corr32, known, Minv,
-                     nagg, agg_ptr, agg_nodes, rel,
-                     Ainv, ld, nchunk, chunk_rows, chunk_slot, chunks_per_agg, agg_first, b, x, workspace, tol_residual, tol_anorm, maxit, check_every, first_check, STREAM(stream), info,
-                     nullptr);
+  return EFEA_OK;
 }
+
+// The three pipelined two-level entry points share their argument list up
+// to the stream (PCG2P_ARGS) and differ in the phase they run.
+#define PCG2P_PARAMS                                                                                               \
+  int precision, int Nnode, int dof_n, int group, const int32_t *node_ptr, const int32_t *node_cols,              \
+      const double *values64, const float *values32, const float *corr32, const uint8_t *known, const void *Minv, \
+      int nagg, const int32_t *agg_ptr, const int32_t *agg_nodes, const double *rel, const double *Ainv, int ld,  \
+      int nchunk, const int32_t *chunk_rows, const int32_t *chunk_slot, int chunks_per_agg,                      \
+      const int32_t *agg_first, const double *b, double *x, void *workspace, size_t workspace_bytes,             \
+      double tol_residual, double tol_anorm, int maxit, int check_every, int first_check, void *stream
+#define PCG2P_CHECK(fn)                                                                                           \
+  pcg2p_check(fn, precision, Nnode, dof_n, node_ptr, node_cols, values64, values32, Minv, nagg, agg_ptr,          \
+              agg_nodes, rel, Ainv, ld, nchunk, chunk_rows, chunk_slot, chunks_per_agg, agg_first, b, x,          \
+              workspace, workspace_bytes)
+#define PCG2P_RUN(info, phase, resumed)                                                                           \
+  solve_pcg2p(precision, Nnode, dof_n, group, node_ptr, node_cols, values64, values32, corr32, known, Minv, nagg, \
+              agg_ptr, agg_nodes, rel, Ainv, ld, nchunk, chunk_rows, chunk_slot, chunks_per_agg, agg_first, b, x, \
+              workspace, tol_residual, tol_anorm, maxit, check_every, first_check, STREAM(stream), info, nullptr, \
+              phase, resumed)
+
+int efea_solve_pcg2p(PCG2P_PARAMS, double* info) {
+  if (!info) {
+    set_error("efea_solve_pcg2p: null info");
+    return EFEA_ERR_ARG;
+  }
+  if (int e = PCG2P_CHECK("efea_solve_pcg2p")) return e;
+  return PCG2P_RUN(info, PCGP_WHOLE, nullptr);
+}
+
+int efea_pcg2p_start(PCG2P_PARAMS) {
+  if (int e = PCG2P_CHECK("efea_pcg2p_start")) return e;
+  if (first_check < 1) {
+    set_error("efea_pcg2p_start: first_check %d (the iterations to queue, >= 1)", first_check);
+    return EFEA_ERR_ARG;
+  }
+  return PCG2P_RUN(nullptr, PCGP_START, nullptr);
+}
+
+int efea_pcg2p_finish(PCG2P_PARAMS, double* info, int* resumed) {
+  if (!info) {
+    set_error("efea_pcg2p_finish: null info");
+    return EFEA_ERR_ARG;
+  }
+  if (int e = PCG2P_CHECK("efea_pcg2p_finish")) return e;
+  if (first_check < 1) {
+    set_error("efea_pcg2p_finish: first_check %d (the start's, >= 1)", first_check);
+    return EFEA_ERR_ARG;
+  }
+  return PCG2P_RUN(info, PCGP_FINISH, resumed);
+}
+
+#undef PCG2P_RUN
+#undef PCG2P_CHECK
+#undef PCG2P_PARAMS
 
 int efea_batch_create(int mode, int dof_n, int group, int two_level, int capacity, void** handle) {
   if (!handle) {

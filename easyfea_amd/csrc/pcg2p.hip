@@ -534,19 +534,24 @@ size_t pcg2p_workspace_bytes(int Nnode, int dof_n, int nslot) { return pcg2p_wor
 // A: the FP64 matrix (start-up, replacements, residuals); Ait: the
 // iterations' matrix (A itself, or the FP32 copy with its correction);
 // Minv: TP (the job's block-Jacobi inverse, FP32 with FP32 values).
+// phase PCGP_WHOLE runs the solve to its end; PCGP_START queues the start-up,
+// the first first_check iterations, the residual estimate and the state
+// read-back into poll slot 1 (or hs) and returns without waiting;
+// PCGP_FINISH waits for that read-back and continues the same loop if the
+// stopping test had not been met (*resumed = 1 then).  The stopping test is
+// decided on the device and the replacements by the iteration count, so the
+// split solve returns what the whole one does.
 template <int DN, int G, typename TM, typename TP>
 static int pcg2p_run(const SpmvArgs& A, const SpmvArgs& Ait, const PipeArgs& PA, const TP* Minv,
                      const CoarseArgs& C, const double* b, double* x, void* ws, double tol_r, double tol_A,
-                     int maxit, int check_every, int first_check, hipStream_t s, double* info, PcgState* hs) {
+                     int maxit, int check_every, int first_check, hipStream_t s, double* info, PcgState* hs,
+                     int phase, int* resumed) {
   const int Nnode = A.Nnode;
   const int replace_every = sizeof(TM) == 4 ? PIPE_REPLACE_MIXED : PIPE_REPLACE;
   constexpr int NM = Modes<DN>::NM;
   const int nslot = C.nagg * PA.maxch;
   const PipeWork W = carve_pipe(ws, Nnode, DN, nslot);
   const unsigned nbB = (unsigned)PA.nchunk;  // one chunk per block (nchunk <= MAXB)
-  // slots without a chunk are never written: zero both restriction arrays
-  hipLaunchKernelGGL(k_pp_zero, dim3(grid_for((int64_t)nslot * NM)), dim3(BS), 0, s, (int64_t)nslot * NM, W.rcpart,
-                     W.rcpart2);
   const unsigned nbA = (unsigned)C.nagg;
   // partials: start-up pass 0 leaves x.A x in partX; the (r,u), (w,u), (r,r)
   // partials of start-up pass 1, of every kernel B and of the replacements'
@@ -556,30 +561,55 @@ static int pcg2p_run(const SpmvArgs& A, const SpmvArgs& Ait, const PipeArgs& PA,
   PipeWork WX = W, WW = W;
   WX.partB = partX;
   WW.partB = partW;
-  hipLaunchKernelGGL(k_pp_state, dim3(1), dim3(64), 0, s, W.st, tol_r * tol_r, tol_A * tol_A);
-  hipLaunchKernelGGL((k_pp_spmv<DN, G, 0>), dim3(nbB), dim3(BS), 0, s, A, PA, C.rel,
-                     (const double*)x, x, b, WX);
-  hipLaunchKernelGGL((k_pp_coarse<TP, DN, 0>), dim3(nbA), dim3(BS), 0, s, C, PA, A.known, Minv,
-                     (const double*)W.r, W.u, (const double*)W.rcpart, W.st, (const double*)nullptr, 0);
-  hipLaunchKernelGGL((k_pp_spmv<DN, G, 1>), dim3(nbB), dim3(BS), 0, s, A, PA, C.rel,
-                     (const double*)W.u, x, b, WW);
-  hipLaunchKernelGGL(k_pp_init_scalars, dim3(1), dim3(BS), 0, s, (const double*)partX, (int)nbB,
-                     (const double*)partW, (int)nbB, W.st);
-  if (int e = check_launch("pcg2p_start")) return e;
-  int launched = 0, pass = 1;
-  bool spec = false;
-  PollBuf* pb = poll_buffer();
+  if (phase != PCGP_WHOLE && first_check < 1) {
+    set_error("pcg2p: a deferred solve needs first_check >= 1 (got %d)", first_check);
+    return EFEA_ERR_ARG;
+  }
+  PollBuf* pb = poll_buffer(phase == PCGP_WHOLE ? 0 : 1);
   if (!pb) {
     set_error("pcg2p: pinned host allocation failed");
     return EFEA_ERR_HIP;
   }
   if (!hs) hs = &pb->st;
+  int launched = 0, pass = 1;
+  bool spec = false;
+  bool wait_only = false;  // finish: the start's read-back is in flight
+  if (phase == PCGP_FINISH) {
+    launched = first_check;
+    pass = 1 + first_check;
+    spec = true;
+    wait_only = true;
+  } else {
+    // slots without a chunk are never written: zero both restriction arrays
+    // (at the start only: a finish continues from the partials in them)
+    hipLaunchKernelGGL(k_pp_zero, dim3(grid_for((int64_t)nslot * NM)), dim3(BS), 0, s, (int64_t)nslot * NM,
+                       W.rcpart, W.rcpart2);
+    hipLaunchKernelGGL(k_pp_state, dim3(1), dim3(64), 0, s, W.st, tol_r * tol_r, tol_A * tol_A);
+    hipLaunchKernelGGL((k_pp_spmv<DN, G, 0>), dim3(nbB), dim3(BS), 0, s, A, PA, C.rel,
+                       (const double*)x, x, b, WX);
+    hipLaunchKernelGGL((k_pp_coarse<TP, DN, 0>), dim3(nbA), dim3(BS), 0, s, C, PA, A.known, Minv,
+                       (const double*)W.r, W.u, (const double*)W.rcpart, W.st, (const double*)nullptr, 0);
+    hipLaunchKernelGGL((k_pp_spmv<DN, G, 1>), dim3(nbB), dim3(BS), 0, s, A, PA, C.rel,
+                       (const double*)W.u, x, b, WW);
+    hipLaunchKernelGGL(k_pp_init_scalars, dim3(1), dim3(BS), 0, s, (const double*)partX, (int)nbB,
+                       (const double*)partW, (int)nbB, W.st);
+    if (int e = check_launch("pcg2p_start")) return e;
+  }
   PcgWork fw{};  // residual passes of the read-backs: m is dead there (kernel A rewrites it)
   fw.r = W.m;
   fw.q = W.m;
   fw.part1 = W.partR;
   fw.small = W.small;
   while (true) {
+    if (wait_only) {
+      wait_only = false;
+      if (hipStreamSynchronize(s) != hipSuccess) {
+        set_error("pcg2p: state readback failed");
+        return EFEA_ERR_HIP;
+      }
+      if (hs->done || launched >= maxit) break;
+      if (resumed) *resumed = 1;
+    }
     const int chunk = launched == 0 && first_check > 0 ? first_check : check_every;
     for (int k = 0; k < chunk; ++k) {
       if (pass > 1 && (pass - 1) % replace_every == 0) {
@@ -618,8 +648,12 @@ static int pcg2p_run(const SpmvArgs& A, const SpmvArgs& Ait, const PipeArgs& PA,
       }
     }
     if (int e = check_launch("pcg2p_iter")) return e;
-    if (hipMemcpyAsync(hs, W.st, sizeof(PcgState), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess) {
+    if (hipMemcpyAsync(hs, W.st, sizeof(PcgState), hipMemcpyDeviceToHost, s) != hipSuccess) {
+      set_error("pcg2p: state readback failed");
+      return EFEA_ERR_HIP;
+    }
+    if (phase == PCGP_START) return 0;
+    if (hipStreamSynchronize(s) != hipSuccess) {
       set_error("pcg2p: state readback failed");
       return EFEA_ERR_HIP;
     }
@@ -642,10 +676,12 @@ int solve_pcg2p(int precision, int Nnode, int dof_n, int group, const int* node_
                 const void* Minv, int nagg, const int* agg_ptr, const int* agg_nodes, const double* rel,
                 const double* Ainv, int ld, int nchunk, const int* rows, const int* cslot, int maxch,
                 const int* first, const double* b, double* x, void* ws, double tol_r, double tol_A, int maxit,
-                int check_every, int first_check, hipStream_t s, double* info, void* host_state_buf) {
+                int check_every, int first_check, hipStream_t s, double* info, void* host_state_buf, int phase,
+                int* resumed) {
   SpmvArgs A{Nnode, node_ptr, node_cols, vals64, known, nullptr, 0};
   SpmvArgs A32{Nnode, node_ptr, node_cols, vals32, known, corr32, 0};
   CoarseArgs C{nagg, agg_ptr, agg_nodes, rel, Ainv, ld, nullptr};
+  if (resumed) *resumed = 0;
   PipeArgs PA{nchunk, rows, cslot, maxch, first};
   if (check_every < 1) check_every = 1;
   PcgState* hs = (PcgState*)host_state_buf;  // null: this thread's pinned poll buffer
@@ -653,9 +689,9 @@ int solve_pcg2p(int precision, int Nnode, int dof_n, int group, const int* node_
   if (dof_n == D && group == GV) {                                                                              \
     if (precision == 32)                                                                                        \
       return pcg2p_run<D, GV, float, float>(A, A32, PA, (const float*)Minv, C, b, x, ws, tol_r, tol_A, maxit,   \
-                                            check_every, first_check, s, info, hs);                             \
+                                            check_every, first_check, s, info, hs, phase, resumed);             \
     return pcg2p_run<D, GV, double, double>(A, A, PA, (const double*)Minv, C, b, x, ws, tol_r, tol_A, maxit,    \
-                                            check_every, first_check, s, info, hs);                             \
+                                            check_every, first_check, s, info, hs, phase, resumed);             \
   }
   EFEA_GROUPS(RUNP)
 #undef RUNP

@@ -678,6 +678,21 @@ class SolveJob:
                 ptr(c32), ptr(self.known), ptr(self.Minv), ptr(self.b), ptr(self.x), ptr(buf), nbytes,
                 self.tol_residual, self.tol_anorm, self.pipe_replace_every, self.maxit, check_every, first_check)
 
+    def _pcg2p_args(self, ws: Workspace, check_every: int, first_check: int) -> tuple:
+        """The arguments of ``efea_solve_pcg2p`` (and of its deferred form)
+        before the stream."""
+        A, cs = self.A, self.cs
+        v32 = A.values32 if self.precision == 32 else None
+        c32 = A.corr32 if self.precision == 32 else None
+        pc = cs.pipeline(A.pattern, A.group)
+        nbytes = _lib.load().efea_pcg2p_workspace_bytes(A.Nn, A.dof_n, pc.nslot)
+        buf = ws.get(nbytes)
+        return (self.precision, A.Nn, A.dof_n, A.group, ptr(A.node_ptr), ptr(A.node_cols), ptr(A.values), ptr(v32),
+                ptr(c32), ptr(self.known), ptr(self.Minv), cs.nagg, ptr(cs.agg_ptr), ptr(cs.agg_nodes), ptr(cs.rel),
+                ptr(self.Ainv), cs.ld, pc.nchunk, ptr(pc.rows), ptr(pc.slot), pc.maxch, ptr(pc.first), ptr(self.b),
+                ptr(self.x), ptr(buf), nbytes, self.tol_residual, self.tol_anorm, self.maxit, check_every,
+                first_check)
+
     def _solve_once(self, ws: Workspace):
         A = self.A
         raw = (C.c_double * 4)()
@@ -689,17 +704,8 @@ class SolveJob:
             rc = _lib.call("efea_solve_pcgp", *args, stream_ptr(), C.cast(raw, C.c_void_p))
             return rc, self.info(raw)
         if self.precond == "two-level" and self.pipelined:
-            cs = self.cs
-            pc = cs.pipeline(A.pattern, A.group)
-            nbytes = _lib.load().efea_pcg2p_workspace_bytes(A.Nn, A.dof_n, pc.nslot)
-            buf = ws.get(nbytes)
-            rc = _lib.call("efea_solve_pcg2p", self.precision, A.Nn, A.dof_n, A.group, ptr(A.node_ptr),
-                           ptr(A.node_cols), ptr(A.values), ptr(v32), ptr(c32), ptr(self.known), ptr(self.Minv),
-                           cs.nagg, ptr(cs.agg_ptr),
-                           ptr(cs.agg_nodes), ptr(cs.rel), ptr(self.Ainv), cs.ld, pc.nchunk, ptr(pc.rows),
-                           ptr(pc.slot), pc.maxch, ptr(pc.first), ptr(self.b), ptr(self.x), ptr(buf), nbytes,
-                           self.tol_residual, self.tol_anorm, self.maxit, check_every, first_check, stream_ptr(),
-                           C.cast(raw, C.c_void_p))
+            args = self._pcg2p_args(ws, check_every, first_check)
+            rc = _lib.call("efea_solve_pcg2p", *args, stream_ptr(), C.cast(raw, C.c_void_p))
             return rc, self.info(raw)
         if self.precond == "two-level":
             cs = self.cs
@@ -787,23 +793,35 @@ class SolveJob:
         return si
 
     def start(self, ws: Optional[Workspace] = None) -> Optional["PendingSolve"]:
-        """Starts a pipelined block-Jacobi solve with an expected iteration
-        count without waiting for it (``efea_pcgp_start``): the start-up and
-        ``deferred_budget(expected)`` iterations are queued on the current
-        stream and the caller's host work overlaps them.  None (nothing
-        queued) for any other solve, which ``run`` then does at once."""
-        if self.method != "pcg" or self.isolate is not None or not self.expected_iterations:
+        """Starts a pipelined solve (block Jacobi or two-level) with an
+        expected iteration count without waiting for it (``efea_pcgp_start`` /
+        ``efea_pcg2p_start``): the start-up and ``deferred_budget`` iterations
+        are queued on the current stream and the caller's host work overlaps
+        them.  None (nothing queued) for any other solve, which ``run`` then
+        does at once."""
+        if self.method != "pcg" or not self.expected_iterations:
             return None
         self.refine = self._wants_refine()
-        if self.precond != "bj" or not self._wants_pipeline():
+        if not self._wants_pipeline():
+            return None
+        if self.precond == "bj":
+            if self.isolate is not None:
+                return None
+            kind = "pcgp"
+        elif self.precond == "two-level":
+            kind = "pcg2p"
+        else:
             return None
         ws = ws or Workspace()
         self._prepare()
+        if self.before_solve is not None and self.before_solve():
+            self._redo_setup()
         check_every, _ = self._polls()
-        args = self._pcgp_args(ws, check_every, deferred_budget(self.expected_iterations))
+        budget = deferred_budget(self.expected_iterations, kind)
+        args = (self._pcgp_args if kind == "pcgp" else self._pcg2p_args)(ws, check_every, budget)
         s = stream_ptr()
-        _lib.call("efea_pcgp_start", *args, s)
-        return PendingSolve(self, ws, args, s)
+        _lib.call(f"efea_{kind}_start", *args, s)
+        return PendingSolve(self, ws, kind, args, s)
 
     def _run_bicgstab(self, ws: Workspace, raise_on_fail: bool) -> SolveInfo:
         """A non-symmetric system: right-preconditioned BiCGStab with the
@@ -871,11 +889,15 @@ class SolveJob:
         return rc, si
 
 
-def deferred_budget(expected: int) -> int:
+def deferred_budget(expected: int, kind: str = "pcgp") -> int:
     """Iterations a deferred solve queues before its first read-back: the
     expected count and a margin, so that the read-back rarely finds the
-    solve unfinished (a launch after convergence returns at once)."""
-    return int(expected) + max(4, int(expected) // 8)
+    solve unfinished.  An iteration past convergence returns at once (about
+    2 us per launch): block Jacobi (one launch) takes an eighth more, the
+    two-level solve (two launches, iteration counts that change by a few from
+    one increment to the next) a thirty-second."""
+    n = int(expected)
+    return n + max(4, n // 8 if kind == "pcgp" else n // 32)
 
 
 class PendingSolve:
@@ -885,15 +907,16 @@ class PendingSolve:
     iterations, or a fallback solve), and work queued from it in between must
     be redone."""
 
-    def __init__(self, job: SolveJob, ws: Workspace, args: tuple, stream: int):
-        self.job, self.ws, self.args, self.stream = job, ws, args, stream
+    def __init__(self, job: SolveJob, ws: Workspace, kind: str, args: tuple, stream: int):
+        self.job, self.ws, self.kind, self.args, self.stream = job, ws, kind, args, stream
         self.resumed = False
 
     def finish(self, raise_on_fail: bool = True) -> SolveInfo:
         job = self.job
         raw = (C.c_double * 4)()
         resumed = C.c_int(0)
-        rc = _lib.call("efea_pcgp_finish", *self.args, self.stream, C.cast(raw, C.c_void_p), C.byref(resumed))
+        rc = _lib.call(f"efea_{self.kind}_finish", *self.args, self.stream, C.cast(raw, C.c_void_p),
+                       C.byref(resumed))
         si = job._conclude(self.ws, rc, job.info(raw), raise_on_fail)
         self.resumed = bool(resumed.value) or rc != 0
         return si

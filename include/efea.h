@@ -333,6 +333,32 @@ int efea_solve_pcg2p(int precision, int Nnode, int dof_n, int group, const int32
                      double tol_residual, double tol_anorm, int maxit, int check_every, int first_check, void* stream,
                      double* info);
 
+/* efea_solve_pcg2p split in two, as efea_pcgp_start / efea_pcgp_finish split
+ * efea_solve_pcgp: efea_pcg2p_start queues the start-up, the first first_check
+ * (>= 1) iterations, the residual estimate and the state read-back on
+ * `stream` and returns without waiting; efea_pcg2p_finish, called with the
+ * same arguments on the same stream before the next deferred start of the
+ * calling thread, waits and, if the stopping test had not been met, continues
+ * with a read-back every check_every iterations.  Same iterates, x and info
+ * as efea_solve_pcg2p with the same first_check (the residual replacements
+ * follow the iteration count); *resumed (may be null) = 1 when iterations ran
+ * after the start's.  Replaces the same reference call as efea_solve_pcg2p. */
+int efea_pcg2p_start(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr,
+                     const int32_t* node_cols, const double* values64, const float* values32, const float* corr32,
+                     const uint8_t* known, const void* Minv, int nagg,
+                     const int32_t* agg_ptr, const int32_t* agg_nodes, const double* rel, const double* Ainv, int ld,
+                     int nchunk, const int32_t* chunk_rows, const int32_t* chunk_slot, int chunks_per_agg,
+                     const int32_t* agg_first, const double* b, double* x, void* workspace, size_t workspace_bytes,
+                     double tol_residual, double tol_anorm, int maxit, int check_every, int first_check, void* stream);
+int efea_pcg2p_finish(int precision, int Nnode, int dof_n, int group, const int32_t* node_ptr,
+                      const int32_t* node_cols, const double* values64, const float* values32, const float* corr32,
+                      const uint8_t* known, const void* Minv, int nagg,
+                      const int32_t* agg_ptr, const int32_t* agg_nodes, const double* rel, const double* Ainv, int ld,
+                      int nchunk, const int32_t* chunk_rows, const int32_t* chunk_slot, int chunks_per_agg,
+                      const int32_t* agg_first, const double* b, double* x, void* workspace, size_t workspace_bytes,
+                      double tol_residual, double tol_anorm, int maxit, int check_every, int first_check,
+                      void* stream, double* info, int* resumed);
+
 /* Timing harness of efea_solve_pcg2p's dominant kernel (the product pass:
  * n = A m, the eight vector updates, the dot-product and restriction
  * partials): reps launches back to back on `stream`, on the solve's chunks,

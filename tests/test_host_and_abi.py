@@ -91,6 +91,17 @@ def test_abi_argument_errors_are_reported_without_gpu():
     assert lib.efea_pcgp_finish(32, 100, 1, 4, fake, fake, fake, None, *[fake] * 6, 1 << 30, 1e-12, 1e-9, 0, 10,
                                 8, 40, None, fake, None) == -1
     assert "precision" in _lib.last_error()
+    # the two-level pipelined solve's deferred form: its checks, then first_check
+    args = [64, 100, 2, 4, *[fake] * 7, 10, fake, fake, fake, fake, 64, 5, fake, fake, 2, fake, fake, fake, fake,
+            1 << 30, 1e-12, 1e-9, 10, 8]
+    assert lib.efea_pcg2p_start(*args, 0, None) == -1
+    assert "efea_pcg2p_start: first_check" in _lib.last_error()
+    assert lib.efea_pcg2p_start(*args[:11], 0, *args[12:], 40, None) == -1
+    assert "null pointer or empty" in _lib.last_error()
+    assert lib.efea_pcg2p_finish(*args, 40, None, None, None) == -1
+    assert "null info" in _lib.last_error()
+    assert lib.efea_pcg2p_finish(*args, 0, None, fake, None) == -1
+    assert "efea_pcg2p_finish: first_check" in _lib.last_error()
     # BiCGStab (non-symmetric systems)
     assert lib.efea_solve_bicgstab(0, 2, 4, *[fake] * 8, 1 << 30, 1e-12, 10, 8, None, fake) == -1
     assert "null pointer or empty" in _lib.last_error()
