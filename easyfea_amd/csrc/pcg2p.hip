@@ -408,48 +408,58 @@ k_pp_coarse(CoarseArgs C, PipeArgs PA, const unsigned char* __restrict__ known, 
   if (MODE == 1 && (st->stop != 0 || nf < -1)) return;
   const bool have = nf >= 0;
   if (!have) nf = 0;
+  // Every load below is issued by every thread, with clamped indices and no
+  // branch around it, so that they are all in flight together: under a
+  // branch each group's result was waited for where its branch ended (a
+  // dependent round trip per column group and per slot group), and the
+  // mask, read where it is used, cost two more after the block reduction.
   double vf[DN], Mf[DN * DN], relf[3];
-  if (have) {
+  int kb[DN];
 #pragma unroll
-    for (int a = 0; a < DN; ++a) vf[a] = vin[(int64_t)nf * DN + a];
-#pragma unroll
-    for (int e = 0; e < DN * DN; ++e) Mf[e] = (double)Minv[(int64_t)nf * DN * DN + e];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) relf[d] = C.rel[3 * (int64_t)nf + d];
+  for (int a = 0; a < DN; ++a) {
+    vf[a] = vin[(int64_t)nf * DN + a];
+    kb[a] = *(known ? known + (int64_t)nf * DN + a : (const unsigned char*)PA.first);  // no mask: ignored
   }
+#pragma unroll
+  for (int e = 0; e < DN * DN; ++e) Mf[e] = (double)Minv[(int64_t)nf * DN * DN + e];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) relf[d] = C.rel[3 * (int64_t)nf + d];
+  const int k0 = C.agg_ptr[blk], k1 = C.agg_ptr[blk + 1];
   const double* __restrict__ rows = (const double*)C.Ainv + (int64_t)(blk * NM) * C.ld;
   const int mx = PA.maxch;
   double zc[NM];
 #pragma unroll
   for (int mm = 0; mm < NM; ++mm) zc[mm] = 0;
   for (int c0 = threadIdx.x; c0 < nc; c0 += U * BS) {
-    double av[U][NM], rv[U];
+    double av[U][NM], sl[U][4];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int c = c0 + u * BS;
-      rv[u] = 0;
+      const int cc = c < nc ? c : 0;
 #pragma unroll
-      for (int mm = 0; mm < NM; ++mm) av[u][mm] = c < nc ? rows[(int64_t)mm * C.ld + c] : 0.0;
+      for (int mm = 0; mm < NM; ++mm) av[u][mm] = rows[(int64_t)mm * C.ld + cc];
+      const int64_t base = (int64_t)(cc / NM) * mx * NM + (cc % NM);
+      // the first four slots (a slot past the aggregate's reads its first)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) sl[u][j] = rcpart[base + (int64_t)(j < mx ? j : 0) * NM];
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int c = c0 + u * BS;
+      double rv = 0;
       if (c < nc) {
-        const int64_t base = (int64_t)(c / NM) * mx * NM + (c % NM);
-        // the first four slots unrolled (their loads in flight together)
-        double sl[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) sl[j] = j < mx ? rcpart[base + (int64_t)j * NM] : 0.0;
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          if (j < mx) rv[u] += sl[j];
-        for (int j = 4; j < mx; ++j) rv[u] += rcpart[base + (int64_t)j * NM];
+          if (j < mx) rv += sl[u][j];
+        const int64_t base = (int64_t)(c / NM) * mx * NM + (c % NM);
+        for (int j = 4; j < mx; ++j) rv += rcpart[base + (int64_t)j * NM];
+      } else {
+#pragma unroll
+        for (int mm = 0; mm < NM; ++mm) av[u][mm] = 0.0;
       }
+#pragma unroll
+      for (int mm = 0; mm < NM; ++mm) zc[mm] += av[u][mm] * rv;
     }
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-#pragma unroll
-      for (int mm = 0; mm < NM; ++mm) zc[mm] += av[u][mm] * rv[u];
   }
   block_sum_all<NM, BS>(zc, shz);
   if (have) {
@@ -467,11 +477,10 @@ k_pp_coarse(CoarseArgs C, PipeArgs PA, const unsigned char* __restrict__ known, 
       double cv = 0;
 #pragma unroll
       for (int mm = 0; mm < NM; ++mm) cv += pmode<DN>(relf, a, mm) * zc[mm];
-      out[i] = (known && known[i]) ? 0.0 : zv[a] + cv;
+      out[i] = (known && kb[a] != 0) ? 0.0 : zv[a] + cv;
     }
   }
-  const int k1 = C.agg_ptr[blk + 1];
-  for (int k = C.agg_ptr[blk] + BS + (int)threadIdx.x; k < k1; k += BS) {  // aggregates of more than BS nodes
+  for (int k = k0 + BS + (int)threadIdx.x; k < k1; k += BS) {  // aggregates of more than BS nodes
     const int n = C.agg_nodes[k];
     double v[DN], zv[DN];
 #pragma unroll
