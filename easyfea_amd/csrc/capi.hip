@@ -363,14 +363,15 @@ size_t efea_pcgp_workspace_bytes(int Nnode, int dof_n) { return pcgp_workspace_b
 
 // Argument checks shared by efea_solve_pcgp and its deferred form.
 static int pcgp_check(const char* fn, int precision, int Nnode, int group, const int32_t* node_ptr,
-                      const int32_t* node_cols, const double* values64, const float* values32, const void* Minv,
-                      const double* b, double* x, void* workspace, size_t workspace_bytes, int dof_n) {
+                      const int32_t* node_cols, const double* values64, const float* values32, const float* corr32,
+                      const void* Minv, const double* b, double* x, void* workspace, size_t workspace_bytes,
+                      int dof_n) {
   if (Nnode <= 0 || !node_ptr || !node_cols || !values64 || !Minv || !b || !x || !workspace) {
     set_error("%s: null pointer or empty system", fn);
     return EFEA_ERR_ARG;
   }
-  if ((precision != 64 && precision != 32) || (precision == 32 && !values32)) {
-    set_error("%s: precision %d (64, or 32 with FP32 values)", fn, precision);
+  if ((precision != 64 && precision != 32) || (precision == 32 && (!values32 || !corr32))) {
+    set_error("%s: precision %d (64, or 32 with FP32 values and their row-sum correction)", fn, precision);
     return EFEA_ERR_ARG;
   }
   if (group <= 0 || (int64_t)Nnode * group > (int64_t)efea_pcgp_max_lanes()) {
@@ -394,7 +395,7 @@ int efea_solve_pcgp(int precision, int Nnode, int dof_n, int group, const int32_
     set_error("efea_solve_pcgp: null info");
     return EFEA_ERR_ARG;
   }
-  if (int e = pcgp_check("efea_solve_pcgp", precision, Nnode, group, node_ptr, node_cols, values64, values32, Minv,
+  if (int e = pcgp_check("efea_solve_pcgp", precision, Nnode, group, node_ptr, node_cols, values64, values32, corr32, Minv,
                          b, x, workspace, workspace_bytes, dof_n))
     return e;
   return solve_pcgp(precision, Nnode, dof_n, group, node_ptr, node_cols, values64, values32, corr32, known, Minv, b,
@@ -407,7 +408,7 @@ int efea_pcgp_start(int precision, int Nnode, int dof_n, int group, const int32_
                     const uint8_t* known, const void* Minv, const double* b, double* x, void* workspace,
                     size_t workspace_bytes, double tol_residual, double tol_anorm, int replace_every, int maxit,
                     int check_every, int first_check, void* stream) {
-  if (int e = pcgp_check("efea_pcgp_start", precision, Nnode, group, node_ptr, node_cols, values64, values32, Minv,
+  if (int e = pcgp_check("efea_pcgp_start", precision, Nnode, group, node_ptr, node_cols, values64, values32, corr32, Minv,
                          b, x, workspace, workspace_bytes, dof_n))
     return e;
   if (first_check < 1) {
@@ -428,7 +429,7 @@ int efea_pcgp_finish(int precision, int Nnode, int dof_n, int group, const int32
     set_error("efea_pcgp_finish: null info");
     return EFEA_ERR_ARG;
   }
-  if (int e = pcgp_check("efea_pcgp_finish", precision, Nnode, group, node_ptr, node_cols, values64, values32, Minv,
+  if (int e = pcgp_check("efea_pcgp_finish", precision, Nnode, group, node_ptr, node_cols, values64, values32, corr32, Minv,
                          b, x, workspace, workspace_bytes, dof_n))
     return e;
   if (first_check < 1) {
@@ -473,7 +474,7 @@ int efea_pp_iter_bench(int precision, int Nnode, int dof_n, int group, const int
                        size_t workspace_bytes, int reps, void* stream) {
   if (Nnode <= 0 || nagg <= 0 || nchunk <= 0 || nchunk > 1024 || chunks_per_agg <= 0 || reps < 1 || !node_ptr ||
       !node_cols || !rel || !chunk_rows || !chunk_slot || !x ||
-      !workspace || (precision == 64 && !values64) || (precision == 32 && !values32) ||
+      !workspace || (precision == 64 && !values64) || (precision == 32 && (!values32 || !corr32)) ||
       (precision != 64 && precision != 32)) {
     set_error("efea_pp_iter_bench: null pointer, empty system, bad precision, reps < 1 or over 1024 chunks");
     return EFEA_ERR_ARG;
@@ -489,7 +490,8 @@ int efea_pp_iter_bench(int precision, int Nnode, int dof_n, int group, const int
 
 // Argument checks shared by efea_solve_pcg2p and its deferred form.
 static int pcg2p_check(const char* fn, int precision, int Nnode, int dof_n, const int32_t* node_ptr,
-                       const int32_t* node_cols, const double* values64, const float* values32, const void* Minv,
+                       const int32_t* node_cols, const double* values64, const float* values32,
+                       const float* corr32, const void* Minv,
                        int nagg, const int32_t* agg_ptr, const int32_t* agg_nodes, const double* rel,
                        const double* Ainv, int ld, int nchunk, const int32_t* chunk_rows, const int32_t* chunk_slot,
                        int chunks_per_agg, const int32_t* agg_first, const double* b, double* x, void* workspace,
@@ -500,8 +502,8 @@ static int pcg2p_check(const char* fn, int precision, int Nnode, int dof_n, cons
     set_error("%s: null pointer or empty system", fn);
     return EFEA_ERR_ARG;
   }
-  if ((precision != 64 && precision != 32) || (precision == 32 && !values32)) {
-    set_error("%s: precision %d (64, or 32 with FP32 values)", fn, precision);
+  if ((precision != 64 && precision != 32) || (precision == 32 && (!values32 || !corr32))) {
+    set_error("%s: precision %d (64, or 32 with FP32 values and their row-sum correction)", fn, precision);
     return EFEA_ERR_ARG;
   }
   if (nchunk > 1024) {
@@ -530,7 +532,7 @@ static int pcg2p_check(const char* fn, int precision, int Nnode, int dof_n, cons
       const int32_t *agg_first, const double *b, double *x, void *workspace, size_t workspace_bytes,             \
       double tol_residual, double tol_anorm, int maxit, int check_every, int first_check, void *stream
 #define PCG2P_CHECK(fn)                                                                                           \
-  pcg2p_check(fn, precision, Nnode, dof_n, node_ptr, node_cols, values64, values32, Minv, nagg, agg_ptr,          \
+  pcg2p_check(fn, precision, Nnode, dof_n, node_ptr, node_cols, values64, values32, corr32, Minv, nagg, agg_ptr,  \
               agg_nodes, rel, Ainv, ld, nchunk, chunk_rows, chunk_slot, chunks_per_agg, agg_first, b, x,          \
               workspace, workspace_bytes)
 #define PCG2P_RUN(info, phase, resumed)                                                                           \

@@ -183,17 +183,25 @@ k_pb_iter(SpmvArgs A, const TP* __restrict__ Minv, double* __restrict__ x, PbWor
   }
   const bool lead = have && g == 0;
   // the own rows' operands, block-Jacobi inverse, mask and (FP32 values)
-  // row-sum correction, all loaded together ahead of the gather
+  // row-sum correction, all loaded together ahead of the gather, by every
+  // lane and with no branch around them (a conditional load's result is
+  // waited for where its branch ends: the mask and correction were two
+  // dependent round trips before the gather); without a mask the flag is
+  // read from m and ignored
   double own[9][DN], Mown[DN * DN], Cown[DN * DN];
-  bool kn[DN];
-  if (lead) {
+  int kb[DN];
+  const bool hk = A.known != nullptr;
+  {
 #pragma unroll
     for (int e = 0; e < DN * DN; ++e) {
       Mown[e] = (double)Minv[(int64_t)n * DN * DN + e];
-      Cown[e] = A.corr ? (double)((const TM*)A.corr)[(int64_t)n * DN * DN + e] : 0.0;
+      if constexpr (sizeof(TM) == 4)
+        Cown[e] = (double)((const TM*)A.corr)[(int64_t)n * DN * DN + e];
+      else
+        Cown[e] = 0.0;
     }
 #pragma unroll
-    for (int a = 0; a < DN; ++a) kn[a] = A.known && A.known[(int64_t)n * DN + a];
+    for (int a = 0; a < DN; ++a) kb[a] = *(hk ? A.known + (int64_t)n * DN + a : (const unsigned char*)mi);
 #pragma unroll
     for (int a = 0; a < DN; ++a) {
       const int64_t i = (int64_t)n * DN + a;
@@ -236,12 +244,15 @@ k_pb_iter(SpmvArgs A, const TP* __restrict__ Minv, double* __restrict__ x, PbWor
   const bool first = o.first;
   double red[3] = {0, 0, 0};
   if (lead) {
-    if (A.corr) {  // FP32 values: the rounding's row sums on the own rows
+    if constexpr (sizeof(TM) == 4) {  // FP32 values: the rounding's row sums on the own rows
 #pragma unroll
       for (int a = 0; a < DN; ++a)
 #pragma unroll
         for (int bb = 0; bb < DN; ++bb) sum[0][a] += Cown[a * DN + bb] * own[0][bb];
     }
+    bool kn[DN];
+#pragma unroll
+    for (int a = 0; a < DN; ++a) kn[a] = hk && kb[a] != 0;
     double wn[DN];
 #pragma unroll
     for (int a = 0; a < DN; ++a) {

@@ -242,8 +242,9 @@ int efea_solve_pcg2(int mode, int Nnode, int dof_n, int group, const int32_t* no
  * the m = M w the next launch multiplies; every block reduces the previous
  * launch's dot-product partials itself.  Same arguments, stopping test and
  * info as efea_solve_pcg (precision 64: FP64 throughout; 32: the iterations'
- * products use values32 + corr32 and Minv is FP32, the start-up and the
- * residual replacements use values64), without replace_ratio and max_blocks:
+ * products use values32 + corr32, both required, and Minv is FP32, the
+ * start-up and the residual replacements use values64), without
+ * replace_ratio and max_blocks:
  * every replace_every iterations (0: 64, or 32 with FP32 values) the
  * recursive vectors are recomputed from x and the direction with the FP64
  * matrix.  The grid covers
@@ -306,9 +307,9 @@ int efea_solve_bicgstab(int Nnode, int dof_n, int group, const int32_t* node_ptr
  * CG) with the preconditioner of efea_solve_pcg2: two kernels per iteration
  * instead of three (the restriction P^T w and both dot products ride on the
  * product n = A m), for latency-bound systems.  precision 64: FP64
- * throughout; 32: the iterations' products use values32 (+ corr32, as
- * efea_solve_pcg) and Minv is FP32, the start-up and replacement passes use
- * values64.  Same arguments, stopping test and info as efea_solve_pcg2 (mode
+ * throughout; 32: the iterations' products use values32 + corr32 (both
+ * required; efea_split_f32 makes them) and Minv is FP32, the start-up and
+ * replacement passes use values64.  Same arguments, stopping test and info as efea_solve_pcg2 (mode
  * 64 or 32 with FP64 vectors), plus the node chunks
  * the product pass runs over, one block each (at most 1024): runs of at most
  * W = 256 / group nodes of one aggregate in agg_nodes order.  Chunk c owns the

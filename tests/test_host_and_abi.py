@@ -91,6 +91,10 @@ def test_abi_argument_errors_are_reported_without_gpu():
     assert lib.efea_pcgp_finish(32, 100, 1, 4, fake, fake, fake, None, *[fake] * 6, 1 << 30, 1e-12, 1e-9, 0, 10,
                                 8, 40, None, fake, None) == -1
     assert "precision" in _lib.last_error()
+    # FP32 values need their row-sum correction (the iterations read it unconditionally)
+    assert lib.efea_solve_pcgp(32, 100, 1, 4, fake, fake, fake, fake, None, *[fake] * 5, 1 << 30, 1e-12, 1e-9, 0,
+                               10, 8, 0, None, fake) == -1
+    assert "row-sum correction" in _lib.last_error()
     # the two-level pipelined solve's deferred form: its checks, then first_check
     args = [64, 100, 2, 4, *[fake] * 7, 10, fake, fake, fake, fake, 64, 5, fake, fake, 2, fake, fake, fake, fake,
             1 << 30, 1e-12, 1e-9, 10, 8]
@@ -102,6 +106,8 @@ def test_abi_argument_errors_are_reported_without_gpu():
     assert "null info" in _lib.last_error()
     assert lib.efea_pcg2p_finish(*args, 0, None, fake, None) == -1
     assert "efea_pcg2p_finish: first_check" in _lib.last_error()
+    assert lib.efea_solve_pcg2p(32, *args[1:8], None, *args[9:], 40, None, fake) == -1
+    assert "row-sum correction" in _lib.last_error()
     # BiCGStab (non-symmetric systems)
     assert lib.efea_solve_bicgstab(0, 2, 4, *[fake] * 8, 1 << 30, 1e-12, 10, 8, None, fake) == -1
     assert "null pointer or empty" in _lib.last_error()
