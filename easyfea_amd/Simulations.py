@@ -1317,18 +1317,24 @@ class PhaseField(_Simu):
 
     # device scalars of the same energies (no host wait): with zero d or u the
     # products are exactly 0, the value the early returns above give
-    def _psi_crack_dev(self) -> torch.Tensor:
+    def _psi_crack_dev(self, d: Optional[torch.Tensor] = None) -> torch.Tensor:
         Kd = self._device_K("damage")
-        return 0.5 * torch.dot(self._d, Kd.spmv(self._d))
+        d = self._d if d is None else d
+        return 0.5 * torch.dot(d, Kd.spmv(d))
 
-    def _psi_elas_dev(self) -> torch.Tensor:
+    def _psi_elas_dev(self, u: Optional[torch.Tensor] = None) -> torch.Tensor:
         Ku = self._device_K("elastic")
-        return 0.5 * torch.dot(self._u, Ku.spmv(self._u))
+        u = self._u if u is None else u
+        return 0.5 * torch.dot(u, Ku.spmv(u))
 
-    def _energy_dev(self, convOption: int, fu: Optional[torch.Tensor]) -> torch.Tensor:
-        E = self._psi_crack_dev()
+    def _energy_dev(self, convOption: int, fu: Optional[torch.Tensor], d: Optional[torch.Tensor] = None,
+                    u: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """The convergence energy of the fields ``d`` and ``u`` (the current
+        ones by default) with the current matrices."""
+        u = self._u if u is None else u
+        E = self._psi_crack_dev(d)
         if convOption == 2:
-            E = E + self._psi_elas_dev() - torch.dot(self._u, fu)
+            E = E + self._psi_elas_dev(u) - torch.dot(u, fu)
         return E
 
     # ------------------------------------------------------------ solves
@@ -1421,9 +1427,13 @@ class PhaseField(_Simu):
         while not converged and Niter < maxIter:
             Niter += 1
             d_n, u_n = self._d, self._u
-            if convOption in (1, 2):
-                E_n = self._energy_dev(convOption, fu)
             yield from self._solve_damage_gen()
+            if convOption in (1, 2):
+                # the energy of the pass's start, queued behind a deferred damage
+                # solve: from d_n and u_n with the matrices before the pass (the
+                # damage solve wrote a new field; the displacement matrix is
+                # assembled again only below, the damage one already was)
+                E_n = self._energy_dev(convOption, fu, d_n, u_n)
             self._updU = False
             yield from self._solve_elastic_gen()
             self._updD = False
@@ -1433,14 +1443,17 @@ class PhaseField(_Simu):
             if convOption == 0:
                 convIter = torch.max(torch.abs(d_np1 - d_n))
             elif convOption in (1, 2):
-                E_np1 = self._energy_dev(convOption, fu)
+                def measure():
+                    E_np1 = self._energy_dev(convOption, fu)
+                    dE = torch.abs(E_n - E_np1)
+                    return torch.where(E_np1 == 0, dE, dE / torch.abs(E_np1))
+
+                convIter = measure()
                 if self._finish_pending():
                     # a deferred displacement solve went on past its queued
-                    # iterations: the damage system and the energy again
+                    # iterations: the damage system and the measure again
                     self._updD = False
-                    E_np1 = self._energy_dev(convOption, fu)
-                dE = torch.abs(E_n - E_np1)
-                convIter = torch.where(E_np1 == 0, dE, dE / torch.abs(E_np1))
+                    convIter = measure()
             elif convOption == 3:
                 diffU = torch.abs(u_np1 - u_n)
                 nz = u_np1 != 0
