@@ -1421,6 +1421,7 @@ class PhaseField(_Simu):
         assert maxIter > 1, "Must be > 1."
         Niter, converged, convIter = 0, False, 0.0
         t0 = time.perf_counter()
+        self._finish_pending()  # one a failed earlier call left queued
         fu = self._neumann_device("elastic") if convOption == 2 else None
         # the convergence measure stays a device scalar unless the loop's
         # control needs it (tolConv < 1): no host wait between the solves
