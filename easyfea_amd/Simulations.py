@@ -30,7 +30,8 @@ import torch
 
 from . import _lib
 from ._utils import ElemType, MatrixType
-from .device import DeviceCsr, DeviceMesh, PendingSolve, SolveJob, Workspace, drive, lincomb, ptr, stream_ptr, to_dev
+from .device import (DeviceCsr, DeviceMesh, PendingSolve, SolveJob, Workspace, drive, lincomb, ptr, stream_ptr,
+                     to_dev, to_host)
 from .mesh import Mesh
 from .Models import Anisotropic, Isotropic
 from .Models import PhaseField as PhaseFieldModel
@@ -1493,10 +1494,16 @@ class PhaseField(_Simu):
         history_damage = self.model.solver == PhaseFieldModel.SolverType.HistoryDamage
         d_old = self.damage if history_damage else None
         converged = drive(self._staggered(tolConv, maxIter, convOption))
-        d = self.damage
+        # u, d and a device convergence measure in one wait
+        c = self.__dict__.get("_convIter", 0.0)
+        if isinstance(c, torch.Tensor):
+            u, d, c = to_host(self._u, self._d, c)
+            self._convIter = float(c)
+        else:
+            u, d = to_host(self._u, self._d)
         if history_damage:
             d = np.maximum(d_old, d)
-        return self.displacement, d, converged
+        return u, d, converged
 
     def Save_Iter(self, iter: Optional[dict] = None) -> None:
         """Advances the history field (``_phasefield.py:613-630``)."""
@@ -1533,7 +1540,18 @@ class PhaseField(_Simu):
         y = A.spmv(x)
         if dofs is None:
             return y.cpu().numpy()
-        return y[to_dev(np.asarray(dofs, dtype=np.int64))].cpu().numpy()
+        return y[self._dofs_dev(dofs)].cpu().numpy()
+
+    def _dofs_dev(self, dofs) -> torch.Tensor:
+        """``dofs`` as a device index, kept while the same DOFs are asked for
+        (a load history reads the reaction at the same DOFs every increment):
+        no blocking host-to-device copy per call."""
+        dofs = np.asarray(dofs, dtype=np.int64)
+        key = dofs.tobytes()
+        c = self.__dict__.get("_dofs_index")
+        if c is None or c[0] != key:
+            c = self._dofs_index = (key, to_dev(dofs))
+        return c[1]
 
     def _solve_method(self, problemType) -> str:
         """Zhang and the ``_PM`` / ``_MP`` splits make the displacement

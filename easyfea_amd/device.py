@@ -54,6 +54,19 @@ def to_dev(a: np.ndarray, dtype=None) -> torch.Tensor:
     return t.to("cuda", non_blocking=False)
 
 
+def to_host(*tensors: torch.Tensor) -> list:
+    """Device tensors as host arrays of their own: asynchronous copies into
+    fresh pinned buffers (torch's pinned-block cache) and one wait, instead
+    of one synchronised pageable copy per tensor."""
+    outs = []
+    for t in tensors:
+        h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+        h.copy_(t, non_blocking=True)
+        outs.append(h)
+    torch.cuda.current_stream().synchronize()
+    return [h.numpy() for h in outs]
+
+
 def table_block(elemType: ElemType, matrixType: MatrixType) -> np.ndarray:
     """[w (nPg)] [N (nPg*nPe)] [dN (nPg*dim*nPe)] as one FP64 block."""
     _, w = gauss_points(elemType, matrixType)
