@@ -11,9 +11,14 @@ the same mesh.  Pinning (see ``tests/test_oracle_*.py``): closed forms,
 invariants and the reference's own known-answer tests; the staggered loop
 (``PhaseFieldOracle.Solve``, every convergence option) by per-pass residual
 identities of independently assembled systems, with planted slips shown to
-fail them (``tests/test_oracle_staggered.py``).  Parity unpinned: the 3D
-Miehe spectral split at exactly repeated eigenvalues (no 3D phase-field
-workload exercises it; the device has no 3D spectral kernel).
+fail them (``tests/test_oracle_staggered.py``).  The 3D spectral
+decomposition at exactly repeated principal strains is pinned against numpy's
+eigh where the reference's case masks give the spectral split (all equal; the
+two smallest equal and not positive); where they do not (the two smallest
+equal and positive, the two largest equal: half-split or misshifted
+projectors, ``Models/_phasefield.py:854-893, 942-948``) the oracle keeps the
+reference's formulas and only the eigenvalues are pinned: parity unpinned
+there (the device has no 3D spectral kernel).
 """
 
 from __future__ import annotations

@@ -675,3 +675,110 @@ def test_damaged_stress_limits(split):
     s1 = O.pf_stress_e_pg(geo, dil, np.ones(m.Nn), m.connect, split, E, 0.3, False)
     s_el = O.stress_e_pg(O.strain_e_pg(geo, dil, m.connect), O.C_isotropic(E, 0.3, 2, False))
     assert np.allclose(s1, 1e-12 * s_el, rtol=1e-9, atol=0)
+
+
+
+def _repeated_3d(diags):
+    """3D strains diag(d) * 2^-10 with exactly repeated eigenvalues: binary
+    fractions, so that the invariants and the Lode angle come out exact."""
+    e = np.zeros((len(diags), 1, 6))
+    for k, d in enumerate(diags):
+        e[k, 0, :3] = np.array(d, dtype=float) * 2.0**-10
+    return e
+
+
+
+# Exactly repeated principal strains where the reference's 3D decomposition
+# is the spectral one: all three equal (g = 0), or the two smallest equal and
+# not positive (theta = 0; their half-split projectors then carry no energy).
+_REPEATED_SPECTRAL = [(2, 2, 2), (-3, -3, -3), (-1, -1, 2), (0, 0, 4), (-4, -4, 0)]
+# ... and where it is not: the two smallest equal and positive (theta = 0),
+# or the two largest equal (theta = pi/3).
+_REPEATED_REFERENCE_ONLY = [(1, 1, 4), (4, 1, 1), (4, 4, 1), (1, 1, -2), (1, 4, 4)]
+
+
+def _miehe_closed(eps, dim=3):
+    lam, mu, _ = O.lame(210000.0, 0.3, dim, False)
+    ev = np.linalg.eigvalsh(O.vec_to_mat(eps))
+    return lam / 2 * np.maximum(ev.sum(-1), 0) ** 2 + mu * np.sum(np.maximum(ev, 0) ** 2, -1)
+
+
+def test_eigen_3d_exactly_repeated_eigenvalues():
+    """The case masks of Models/_phasefield.py:832-906 where the general
+    formula would divide by zero (g = 0; theta = 0 with the repeated pair
+    not positive): eigenvalues, reconstruction, partition of unity and the
+    spectral projectors against numpy's eigh, and the Miehe energy against
+    its closed form."""
+    eps = _repeated_3d(_REPEATED_SPECTRAL)
+    M = O.vec_to_mat(eps)
+    val, _, Ms = O.eigen_3d(eps)
+    ref = np.linalg.eigvalsh(M)
+    assert np.array_equal(val, ref)
+    rec = sum(val[..., a, None, None] * Ms[a] for a in range(3))
+    assert np.max(np.abs(rec - M)) <= 1e-15 * np.max(np.abs(M))
+    assert np.max(np.abs(sum(Ms) - np.eye(3))) < 1e-15
+    P, Mm = O.spectral_projectors(eps, 3)
+    eP = (P @ eps[..., None])[..., 0]
+    eM = (Mm @ eps[..., None])[..., 0]
+    pos, neg = _pos_neg(eps, 3)
+    scale = np.max(np.abs(eps))
+    assert np.max(np.abs(eP - pos)) < 1e-15 * scale and np.max(np.abs(eM - neg)) < 1e-15 * scale
+    closed = _miehe_closed(eps)
+    psiP, psiM = O.calc_psi("Miehe", eps, 210000.0, 0.3, 3, False)
+    assert np.max(np.abs(psiP - closed)) <= 1e-13 * np.max(closed)
+    psi = 0.5 * np.einsum("...i,ij,...j->...", eps, O.C_isotropic(210000.0, 0.3, 3, False), eps)
+    assert np.max(np.abs(psiP + psiM - psi)) <= 1e-13 * np.max(psi)
+
+
+def test_eigen_3d_repeated_positive_or_two_largest_follow_the_reference():
+    """Where the reference's closed form is not the spectral split.  Its
+    eigenvalues are exact, but it splits a double eigenspace projector into
+    normalised halves (theta = 0: M1 = (I - M3) / sqrt 2, then
+    M2 = I - M1 - M3, :892, 942-948), which the 6x6 projector treats as two
+    eigenprojectors, and for theta = pi/3 it builds the lone projector from
+    I_rg = (I1 - sqrt g) / 3 I (:854, 870) where the eigenvalue shift is
+    (I1 + sqrt g) / 3.  With the pair in tension (or the two largest equal)
+    the Miehe energy then departs from the closed form.  The oracle keeps the
+    reference's formulas (parity is with the reference; its package does not
+    import here, gmsh being absent, so these values are pinned by reading
+    only): this test records the exact eigenvalues, the partition of unity
+    and that the departure is the reference's, not the oracle's rounding."""
+    eps = _repeated_3d(_REPEATED_REFERENCE_ONLY)
+    M = O.vec_to_mat(eps)
+    val, _, Ms = O.eigen_3d(eps)
+    assert np.array_equal(val, np.linalg.eigvalsh(M))
+    assert np.max(np.abs(sum(Ms) - np.eye(3))) < 1e-15
+    psiP, psiM = O.calc_psi("Miehe", eps, 210000.0, 0.3, 3, False)
+    closed = _miehe_closed(eps)
+    dev = np.abs(psiP - closed)[:, 0] / closed[:, 0]
+    assert np.all(dev > 1e-3), dev  # a formula departure, not rounding
+    psi = 0.5 * np.einsum("...i,ij,...j->...", eps, O.C_isotropic(210000.0, 0.3, 3, False), eps)
+    assert np.max(np.abs(psiP + psiM - psi)) <= 1e-13 * np.max(psi)  # c+ + c- = C still
+
+
+def test_eigen_3d_repeated_branches_meet_the_general_one():
+    """A strain a relative 1e-6 away from one of the spectral repeated cases
+    takes the general branch (distinct eigenvalues); its Miehe energy and
+    stress differ from the repeated branch's by the perturbation's order:
+    those masks join the general formula continuously."""
+    eps = _repeated_3d(_REPEATED_SPECTRAL)
+    pert = eps.copy()
+    d = np.diagonal(O.vec_to_mat(eps)[:, 0], axis1=-2, axis2=-1)
+    for k in range(eps.shape[0]):
+        # the repeated pair (or the first two of a triple) drift apart
+        i, j = [(a, b) for a in range(3) for b in range(a + 1, 3) if d[k, a] == d[k, b]][0]
+        h = 1e-6 * np.max(np.abs(eps[k]))
+        pert[k, 0, i] += h
+        pert[k, 0, j] -= h
+    val, _, _ = O.eigen_3d(pert)
+    assert np.all(np.diff(val, axis=-1) > 0)  # distinct: the general branch
+    C = O.C_isotropic(210000.0, 0.3, 3, False)
+    p0, _ = O.calc_psi("Miehe", eps, 210000.0, 0.3, 3, False)
+    p1, _ = O.calc_psi("Miehe", pert, 210000.0, 0.3, 3, False)
+    psi = 0.5 * np.einsum("...i,ij,...j->...", eps, C, eps)
+    assert np.max(np.abs(p1 - p0) / psi) < 1e-5
+    c0, _ = O.calc_C("Miehe", eps, 210000.0, 0.3, 3, False)
+    c1, _ = O.calc_C("Miehe", pert, 210000.0, 0.3, 3, False)
+    s0 = (c0 @ eps[..., None])[..., 0]
+    s1 = (c1 @ pert[..., None])[..., 0]
+    assert np.max(np.abs(s1 - s0)) < 1e-4 * np.max(np.abs(s0))
