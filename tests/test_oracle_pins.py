@@ -782,3 +782,25 @@ def test_eigen_3d_repeated_branches_meet_the_general_one():
     s0 = (c0 @ eps[..., None])[..., 0]
     s1 = (c1 @ pert[..., None])[..., 0]
     assert np.max(np.abs(s1 - s0)) < 1e-4 * np.max(np.abs(s0))
+
+
+def test_eigen_2d_repeated_eigenvalues():
+    """2D at equal principal strains (the `v1 != v2` mask of
+    Models/_phasefield.py:795-802): the projectors fall back to the axes,
+    which are eigenprojectors of a multiple of I, and the split is spectral:
+    eigh's positive part, Miehe's closed form, in tension and compression."""
+    for ps in (False, True):
+        e = np.zeros((4, 1, 3))
+        e[:, 0, :2] = np.array([[1, 1], [-1, -1], [0, 0], [3, 3]], dtype=float) * 2.0**-10
+        val, _, Ms = O.eigen_2d(e)
+        assert np.array_equal(val, np.linalg.eigvalsh(O.vec_to_mat(e)))
+        assert np.max(np.abs(sum(Ms) - np.eye(2))) == 0
+        P, Mm = O.spectral_projectors(e, 2)
+        eP = (P @ e[..., None])[..., 0]
+        pos, _ = _pos_neg(e, 2)
+        assert np.max(np.abs(eP - pos)) <= 1e-15 * np.max(np.abs(e))
+        lam, mu, _ = O.lame(210000.0, 0.3, 2, ps)
+        ev = np.linalg.eigvalsh(O.vec_to_mat(e))
+        closed = lam / 2 * np.maximum(ev.sum(-1), 0) ** 2 + mu * np.sum(np.maximum(ev, 0) ** 2, -1)
+        psiP, _ = O.calc_psi("Miehe", e, 210000.0, 0.3, 2, ps)
+        assert np.max(np.abs(psiP - closed)) <= 1e-13 * np.max(closed)
